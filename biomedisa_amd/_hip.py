@@ -1,0 +1,216 @@
+"""ctypes binding of libbmx_hip.so (C ABI declared in include/bmx.h).
+
+The library is built in place by ``biomedisa_amd.build`` (``make -C
+biomedisa_amd/csrc``).  Loading fails loudly when it is missing: there is no
+CPU fallback in the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbmx_hip.so")
+
+BMX_OK = 0
+IMG_U8, IMG_F32 = 0, 1
+RNG_PHILOX, RNG_REF = 0, 1
+LABEL_NONE = 0xFF
+
+SEED_DTYPE = np.dtype([("pos", "<u4"), ("label", "<u4"), ("var", "<f4"), ("reserved", "<u4")])
+
+# Every symbol include/bmx.h declares, with (restype, argtypes).
+_p, _i64, _i = C.c_void_p, C.c_int64, C.c_int
+SIGNATURES = {
+    "bmx_last_error": (C.c_char_p, []),
+    "bmx_version": (_i, []),
+    "bmx_open": (_i, [_i, C.POINTER(_p)]),
+    "bmx_close": (_i, [_p]),
+    "bmx_sync": (_i, [_p]),
+    "bmx_malloc": (_i, [_p, C.c_size_t, C.POINTER(_p)]),
+    "bmx_free": (_i, [_p, _p]),
+    "bmx_upload": (_i, [_p, _p, _p, C.c_size_t]),
+    "bmx_download": (_i, [_p, _p, _p, C.c_size_t]),
+    "bmx_zero": (_i, [_p, _p, C.c_size_t]),
+    "bmx_plane_flags": (_i, [_p, _p, _i64, _i64, _i64, _i, _p]),
+    "bmx_seeds": (_i, [_p, _p, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
+    "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
+                      C.POINTER(C.c_uint64)]),
+    "bmx_last_walk_ms": (C.c_float, [_p]),
+    "bmx_argmax": (_i, [_p, _p, _i, _i64, _p, _p]),
+    "bmx_smooth": (_i, [_p, _p, _p, _i64, _i64, _i64, _i]),
+    "bmx_smooth_argmax": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p]),
+    "bmx_uncertainty": (_i, [_p, _p, _i, _i64, _i64, _i64, _p]),
+}
+
+_lib = None
+
+
+class BmxError(RuntimeError):
+    """A libbmx_hip call returned an error code."""
+
+    def __init__(self, code, msg):
+        super().__init__("libbmx_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libbmx_hip.so and bind every declared entry point."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libbmx_hip.so not found at %s: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C %s/csrc`"
+                % (LIB_PATH, _HERE))
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != BMX_OK:
+        msg = load().bmx_last_error().decode(errors="replace")
+        raise BmxError(rc, msg)
+
+
+class DeviceBuffer:
+    """Device allocation owned by an Engine; freed on close()/garbage collection."""
+
+    def __init__(self, engine, nbytes):
+        self.engine = engine
+        self.nbytes = int(nbytes)
+        ptr = C.c_void_p()
+        _check(load().bmx_malloc(engine.ctx, self.nbytes, C.byref(ptr)))
+        self.ptr = ptr.value
+
+    def free(self):
+        if self.ptr is not None and self.engine.ctx is not None:
+            _check(load().bmx_free(self.engine.ctx, self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def zero(self):
+        _check(load().bmx_zero(self.engine.ctx, self.ptr, self.nbytes))
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(load().bmx_download(self.engine.ctx, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+
+class Engine:
+    """One HIP device context (bmx_ctx) and its stream."""
+
+    def __init__(self, device=0):
+        lib = load()
+        ctx = C.c_void_p()
+        _check(lib.bmx_open(int(device), C.byref(ctx)))
+        self.ctx = ctx.value
+        self.device = device
+
+    def close(self):
+        if self.ctx is not None:
+            _check(load().bmx_close(self.ctx))
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        _check(load().bmx_sync(self.ctx))
+
+    # memory ---------------------------------------------------------------
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        buf = DeviceBuffer(self, arr.nbytes)
+        _check(load().bmx_upload(self.ctx, buf.ptr, arr.ctypes.data, arr.nbytes))
+        return buf
+
+    def upload_into(self, buf, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= buf.nbytes
+        _check(load().bmx_upload(self.ctx, buf.ptr, arr.ctypes.data, arr.nbytes))
+
+    # path -----------------------------------------------------------------
+    def plane_flags(self, lab, dims, allaxis, out=None):
+        Z, Y, X = dims
+        flags = out if out is not None else DeviceBuffer(self, Z + Y + X)
+        _check(load().bmx_plane_flags(self.ctx, lab.ptr, Z, Y, X, int(bool(allaxis)), flags.ptr))
+        return flags
+
+    def count_seeds(self, lab, dims, flags):
+        Z, Y, X = dims
+        n = C.c_int64()
+        _check(load().bmx_seeds(self.ctx, lab.ptr, None, IMG_U8, Z, Y, X, flags.ptr, None, 0,
+                                C.byref(n)))
+        return n.value
+
+    def seeds(self, lab, img, img_dtype, dims, flags, capacity=None, out=None):
+        """Seed list on the device -> (buffer, count)."""
+        Z, Y, X = dims
+        if capacity is None:
+            capacity = self.count_seeds(lab, dims, flags)
+        buf = out if out is not None else DeviceBuffer(self, max(1, capacity) * SEED_DTYPE.itemsize)
+        n = C.c_int64()
+        _check(load().bmx_seeds(self.ctx, lab.ptr, img.ptr, img_dtype, Z, Y, X, flags.ptr, buf.ptr,
+                                capacity, C.byref(n)))
+        return buf, n.value
+
+    def walk(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits):
+        Z, Y, X = dims
+        acc = C.c_uint64()
+        _check(load().bmx_walk(self.ctx, img.ptr, img_dtype, Z, Y, X,
+                               seeds.ptr if seeds is not None else None, int(nseeds), int(nlabels),
+                               int(nbrw), int(sorw), int(rng), C.c_uint64(int(key) & (2**64 - 1)),
+                               hits.ptr, C.byref(acc)))
+        return acc.value
+
+    def last_walk_ms(self):
+        return float(load().bmx_last_walk_ms(self.ctx))
+
+    def argmax(self, hits, nlabels, V, values, out):
+        values = np.ascontiguousarray(values, dtype=np.uint8)
+        _check(load().bmx_argmax(self.ctx, hits.ptr, int(nlabels), int(V), values.ctypes.data, out.ptr))
+
+    def smooth(self, phi, tmp, dims, iters):
+        Z, Y, X = dims
+        _check(load().bmx_smooth(self.ctx, phi.ptr, tmp.ptr, Z, Y, X, int(iters)))
+
+    def smooth_argmax(self, hits, nlabels, dims, iters, values, out):
+        Z, Y, X = dims
+        values = np.ascontiguousarray(values, dtype=np.uint8)
+        _check(load().bmx_smooth_argmax(self.ctx, hits.ptr, int(nlabels), Z, Y, X, int(iters),
+                                        values.ctypes.data, out.ptr))
+
+    def uncertainty(self, hits, nlabels, dims, out):
+        Z, Y, X = dims
+        _check(load().bmx_uncertainty(self.ctx, hits.ptr, int(nlabels), Z, Y, X, out.ptr))
+
+
+_engines = {}
+
+
+def engine(device=0):
+    """Process-wide Engine per device (created on first use)."""
+    eng = _engines.get(device)
+    if eng is None or eng.ctx is None:
+        eng = Engine(device)
+        _engines[device] = eng
+    return eng

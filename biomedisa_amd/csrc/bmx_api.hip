@@ -1,0 +1,137 @@
+// bmx_api.hip -- context, device memory and error reporting of libbmx_hip.
+#include "bmx_internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+static thread_local char g_err[512] = "";
+
+int bmx_set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int bmx_set_device(bmx_ctx *ctx)
+{
+    BMX_CHECK(ctx != nullptr, "null context");
+    BMX_HIP(hipSetDevice(ctx->device));
+    return BMX_OK;
+}
+
+extern "C" {
+
+const char *bmx_last_error(void) { return g_err; }
+
+int bmx_version(void) { return 1; }
+
+int bmx_open(int device, bmx_ctx **out)
+{
+    BMX_CHECK(out != nullptr, "bmx_open: out is null");
+    *out = nullptr;
+    int n = 0;
+    BMX_HIP(hipGetDeviceCount(&n));
+    BMX_CHECK(device >= 0 && device < n, "bmx_open: device %d not in [0,%d)", device, n);
+    BMX_HIP(hipSetDevice(device));
+    bmx_ctx *c = (bmx_ctx *)calloc(1, sizeof(bmx_ctx));
+    if (!c) return bmx_set_error(BMX_E_OOM, "bmx_open: host allocation failed");
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_counter, 8 * sizeof(uint64_t));
+    if (e != hipSuccess) {
+        free(c);
+        return bmx_set_error(BMX_E_DEVICE, "bmx_open: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return BMX_OK;
+}
+
+int bmx_close(bmx_ctx *ctx)
+{
+    if (!ctx) return BMX_OK;
+    BMX_HIP(hipSetDevice(ctx->device));
+    BMX_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_block_counts) BMX_HIP(hipFree(ctx->d_block_counts));
+    BMX_HIP(hipFree(ctx->d_counter));
+    BMX_HIP(hipEventDestroy(ctx->ev0));
+    BMX_HIP(hipEventDestroy(ctx->ev1));
+    BMX_HIP(hipStreamDestroy(ctx->stream));
+    free(ctx);
+    return BMX_OK;
+}
+
+int bmx_sync(bmx_ctx *ctx)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_HIP(hipStreamSynchronize(ctx->stream));
+    return BMX_OK;
+}
+
+int bmx_malloc(bmx_ctx *ctx, size_t bytes, void **dptr)
+{
+    BMX_CHECK(dptr != nullptr, "bmx_malloc: dptr is null");
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return bmx_set_error(e == hipErrorOutOfMemory ? BMX_E_OOM : BMX_E_DEVICE,
+                             "bmx_malloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    return BMX_OK;
+}
+
+int bmx_free(bmx_ctx *ctx, void *dptr)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    if (!dptr) return BMX_OK;
+    BMX_HIP(hipStreamSynchronize(ctx->stream));
+    BMX_HIP(hipFree(dptr));
+    return BMX_OK;
+}
+
+int bmx_upload(bmx_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return BMX_OK;
+    BMX_CHECK(dst_dev && src_host, "bmx_upload: null pointer");
+    BMX_HIP(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    BMX_HIP(hipStreamSynchronize(ctx->stream));
+    return BMX_OK;
+}
+
+int bmx_download(bmx_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return BMX_OK;
+    BMX_CHECK(dst_host && src_dev, "bmx_download: null pointer");
+    BMX_HIP(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    BMX_HIP(hipStreamSynchronize(ctx->stream));
+    return BMX_OK;
+}
+
+int bmx_zero(bmx_ctx *ctx, void *dst_dev, size_t bytes)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return BMX_OK;
+    BMX_CHECK(dst_dev, "bmx_zero: null pointer");
+    BMX_HIP(hipMemsetAsync(dst_dev, 0, bytes, ctx->stream));
+    return BMX_OK;
+}
+
+float bmx_last_walk_ms(bmx_ctx *ctx) { return ctx ? ctx->last_walk_ms : -1.0f; }
+
+}  // extern "C"

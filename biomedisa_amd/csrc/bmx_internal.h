@@ -1,0 +1,120 @@
+// bmx_internal.h -- shared pieces of libbmx_hip: context, error reporting and
+// the device arithmetic every walker uses (Philox4x32-10, exp, weights).
+//
+// Arithmetic contract: all HIP sources are built with -ffp-contract=off, so
+// every float expression is evaluated as the IEEE binary32 operations written,
+// in the order written; fused multiply-adds appear only as __builtin_fmaf.
+// The CPU restatement in oracle/bmx_oracle.c follows the same contract, which
+// is what makes walker hit counts bit-exact between the two.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/bmx.h"
+
+struct bmx_ctx {
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;     // bracket the last walker launch (bmx_last_walk_ms)
+    float last_walk_ms;
+    uint64_t *d_counter;     // 8 x uint64 scratch counters (accepted moves, seed counts)
+    int64_t *d_block_counts; // seed compaction scratch
+    int64_t block_counts_cap;
+};
+
+// Error reporting ----------------------------------------------------------
+int bmx_set_error(int code, const char *fmt, ...);
+
+#define BMX_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return bmx_set_error(e_ == hipErrorOutOfMemory ? BMX_E_OOM : BMX_E_DEVICE, \
+                                 "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                 __FILE__, __LINE__);                              \
+    } while (0)
+
+#define BMX_CHECK(cond, ...)                                                       \
+    do {                                                                           \
+        if (!(cond)) return bmx_set_error(BMX_E_INVALID, __VA_ARGS__);             \
+    } while (0)
+
+#define BMX_LAUNCH_CHECK() BMX_HIP(hipGetLastError())
+
+int bmx_set_device(bmx_ctx *ctx);
+
+// Device arithmetic -----------------------------------------------------------
+
+// Philox4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64 multiplies
+// with the published multipliers and Weyl key increments.
+struct bmx_u4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ bmx_u4 bmx_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                             uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return bmx_u4{c0, c1, c2, c3};
+}
+
+// exp for binary32: Cody-Waite reduction x = n ln2 + r (ln2 split in a hi part
+// with 16 significant bits and a lo part, applied with fused multiply-adds),
+// e^r by its degree-7 Taylor polynomial (Horner, fused), 2^n added to the
+// exponent field.  Below 2^-125 the result flushes to +0.  Max error on
+// [-86.5, 0] is under 1 ulp against a float64 exp.
+__device__ __forceinline__ float bmx_expf(float x)
+{
+    const float LOG2E = 1.44269502925872802734375f;
+    const float LN2_HI = 0.693145751953125f;
+    const float LN2_LO = 1.428606765330187045e-06f;
+    if (x > 88.0f) return __builtin_inff();
+    const float t = x * LOG2E;
+    const float n = __builtin_rintf(t);
+    float r = __builtin_fmaf(-n, LN2_HI, x);
+    r = __builtin_fmaf(-n, LN2_LO, r);
+    float p = 1.98412701e-4f;
+    p = __builtin_fmaf(p, r, 1.38888892e-3f);
+    p = __builtin_fmaf(p, r, 8.33333377e-3f);
+    p = __builtin_fmaf(p, r, 4.16666679e-2f);
+    p = __builtin_fmaf(p, r, 1.66666672e-1f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const int32_t ni = (int32_t)n;
+    const float res = __uint_as_float(__float_as_uint(p) + ((uint32_t)ni << 23));
+    return (n < -125.0f) ? 0.0f : res;
+}
+
+// Transition weight of the reference (pyopencl_small.py:187-190):
+// exp(-(B-A)^2 * div1), evaluated as ((-t)*t)*div1.
+__device__ __forceinline__ float bmx_weight(float B, float A, float div1)
+{
+    const float t = B - A;
+    return bmx_expf(((-t) * t) * div1);
+}
+
+template <typename T> __device__ __forceinline__ float bmx_ld(const T *p, int64_t i);
+template <> __device__ __forceinline__ float bmx_ld<uint8_t>(const uint8_t *p, int64_t i)
+{
+    return (float)p[i];
+}
+template <> __device__ __forceinline__ float bmx_ld<float>(const float *p, int64_t i)
+{
+    return p[i];
+}
+
+// Grid helpers
+static inline unsigned bmx_blocks(int64_t n, int threads)
+{
+    return (unsigned)((n + threads - 1) / threads);
+}

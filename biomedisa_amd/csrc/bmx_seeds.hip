@@ -1,0 +1,315 @@
+// bmx_seeds.hip -- labelled-plane detection and seed extraction.
+//
+// Reference semantics:
+//   plane labelled (XY):      biomedisa_helper.py:1015-1022 (np.any(slice))
+//   plane labelled (allaxis): biomedisa_helper.py:1024-1066 (1-voxel-thin
+//                             lines removed along the first, then the second
+//                             in-plane axis; labelled iff anything survives)
+//   seed mask:                pyopencl_small.py:89-107 (labelled planes keep
+//                             their label, everything else is ignore_value)
+//   seed test:                pyopencl_small.py:205-207 (interior voxel whose
+//                             mask value is a label)
+//   variance:                 _calc_var, pyopencl_small.py:140-185
+#include "bmx_internal.h"
+
+namespace {
+
+constexpr int kSeedThreads = 256;
+constexpr int kSeedPerThread = 16;
+constexpr int kSeedChunk = kSeedThreads * kSeedPerThread;
+
+__device__ __forceinline__ bool nz(const uint8_t *lab, int64_t i) { return lab[i] != 0; }
+
+// Voxel survives the first thin-line pass along an axis of extent na, stride sa.
+__device__ __forceinline__ bool pass1(const uint8_t *lab, int64_t i, int a, int na, int64_t sa)
+{
+    if (!nz(lab, i)) return false;
+    if (a > 0 && a < na - 1 && !nz(lab, i - sa) && !nz(lab, i + sa)) return false;
+    return true;
+}
+
+__device__ __forceinline__ bool pass2(const uint8_t *lab, int64_t i, int a, int na, int64_t sa,
+                                      int b, int nb, int64_t sb)
+{
+    if (!pass1(lab, i, a, na, sa)) return false;
+    if (b > 0 && b < nb - 1 && !pass1(lab, i - sb, a, na, sa) && !pass1(lab, i + sb, a, na, sa))
+        return false;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_plane_flags(const uint8_t *__restrict__ lab, int Z, int Y,
+                                                     int X, int allaxis, uint8_t *fz, uint8_t *fy,
+                                                     uint8_t *fx)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
+        if (!nz(lab, i)) continue;
+        const int z = (int)(i / sz);
+        const int64_t r = i - z * sz;
+        const int y = (int)(r / sy);
+        const int x = (int)(r - y * sy);
+        if (!allaxis) {
+            if (!fz[z]) fz[z] = 1;
+            continue;
+        }
+        if (!fz[z] && pass2(lab, i, y, Y, sy, x, X, 1)) fz[z] = 1;
+        if (!fy[y] && pass2(lab, i, z, Z, sz, x, X, 1)) fy[y] = 1;
+        if (!fx[x] && pass2(lab, i, z, Z, sz, y, Y, sy)) fx[x] = 1;
+    }
+}
+
+struct Geom {
+    int Z, Y, X;
+    int64_t sy, sz, V;
+};
+
+__device__ __forceinline__ uint8_t mask_at(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
+                                           const uint8_t *fx, int z, int y, int x, int64_t i)
+{
+    return (fz[z] | fy[y] | fx[x]) ? lab[i] : (uint8_t)BMX_LABEL_NONE;
+}
+
+__device__ __forceinline__ bool is_seed(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
+                                        const uint8_t *fx, const Geom &g, int z, int y, int x,
+                                        int64_t i)
+{
+    if (z <= 0 || z >= g.Z - 1 || y <= 0 || y >= g.Y - 1 || x <= 0 || x >= g.X - 1) return false;
+    return mask_at(lab, fz, fy, fx, z, y, x, i) != BMX_LABEL_NONE;
+}
+
+__device__ __forceinline__ void coords(const Geom &g, int64_t i, int &z, int &y, int &x)
+{
+    z = (int)(i / g.sz);
+    const int64_t r = i - (int64_t)z * g.sz;
+    y = (int)(r / g.sy);
+    x = (int)(r - (int64_t)y * g.sy);
+}
+
+__device__ __forceinline__ void advance(const Geom &g, int &z, int &y, int &x)
+{
+    if (++x == g.X) {
+        x = 0;
+        if (++y == g.Y) { y = 0; ++z; }
+    }
+}
+
+// Seeds among this thread's kSeedPerThread consecutive voxels.
+__device__ int thread_seed_count(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
+                                 const uint8_t *fx, const Geom &g, int64_t first)
+{
+    int cnt = 0;
+    if (first >= g.V) return 0;
+    int z, y, x;
+    coords(g, first, z, y, x);
+    for (int k = 0; k < kSeedPerThread; ++k) {
+        const int64_t i = first + k;
+        if (i >= g.V) break;
+        cnt += is_seed(lab, fz, fy, fx, g, z, y, x, i) ? 1 : 0;
+        advance(g, z, y, x);
+    }
+    return cnt;
+}
+
+// Exclusive block scan of one int per thread (256 threads = 4 waves of 64).
+__device__ int block_exclusive_scan(int v, int *total)
+{
+    __shared__ int wave_sums[kSeedThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kSeedThreads / 64; ++w) {
+        if (w < wave) base += wave_sums[w];
+        tot += wave_sums[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + incl - v;
+}
+
+__global__ __launch_bounds__(kSeedThreads) void k_seed_count(const uint8_t *__restrict__ lab,
+                                                             const uint8_t *__restrict__ flags,
+                                                             Geom g, int64_t *block_counts)
+{
+    const uint8_t *fz = flags, *fy = flags + g.Z, *fx = flags + g.Z + g.Y;
+    const int64_t first = (int64_t)blockIdx.x * kSeedChunk + (int64_t)threadIdx.x * kSeedPerThread;
+    const int c = thread_seed_count(lab, fz, fy, fx, g, first);
+    int tot;
+    (void)block_exclusive_scan(c, &tot);
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = tot;
+}
+
+// Exclusive scan of n int64 counts in place by one block; total -> out_total.
+__global__ __launch_bounds__(1024) void k_scan_counts(int64_t *counts, int64_t n, uint64_t *out_total)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t lo = t * per, hi = (lo + per < n) ? lo + per : n;
+    int64_t s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - s;
+    for (int64_t i = lo; i < hi; ++i) {
+        const int64_t c = counts[i];
+        counts[i] = run;
+        run += c;
+    }
+    if (t == 1023) *out_total = (uint64_t)part[1023];
+}
+
+template <typename T>
+__device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy, const uint8_t *fx,
+                          const T *img, const Geom &g, int z, int y, int x, int64_t i0)
+{
+    const uint8_t seg = mask_at(lab, fz, fy, fx, z, y, x, i0);
+    const float B = bmx_ld(img, i0);
+    float dev = 0.0f, summe = 0.0f;
+    for (int n = -1; n < 2; ++n)
+        for (int o = -1; o < 2; ++o) {
+            const int64_t i = i0 + n * g.sy + o;
+            if (mask_at(lab, fz, fy, fx, z, y + n, x + o, i) == seg) {
+                const float t = B - bmx_ld(img, i);
+                dev += t * t;
+                summe += 1.0f;
+            }
+        }
+    for (int n = -1; n <= 1; n += 2)
+        for (int o = -1; o < 2; ++o) {
+            const int64_t i = i0 + n * g.sz + o;
+            if (summe < 9.0f && mask_at(lab, fz, fy, fx, z + n, y, x + o, i) == seg) {
+                const float t = B - bmx_ld(img, i);
+                dev += t * t;
+                summe += 1.0f;
+            }
+        }
+    for (int n = -1; n <= 1; n += 2)
+        for (int o = -1; o <= 1; o += 2) {
+            const int64_t i = i0 + n * g.sz + o * g.sy;
+            if (summe < 9.0f && mask_at(lab, fz, fy, fx, z + n, y + o, x, i) == seg) {
+                const float t = B - bmx_ld(img, i);
+                dev += t * t;
+                summe += 1.0f;
+            }
+        }
+    float var = dev / summe;
+    if (var < 1.0f) var = 1.0f;
+    return var;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSeedThreads) void k_seed_write(const uint8_t *__restrict__ lab,
+                                                             const T *__restrict__ img,
+                                                             const uint8_t *__restrict__ flags,
+                                                             Geom g, const int64_t *block_offsets,
+                                                             bmx_seed *seeds, int64_t capacity)
+{
+    const uint8_t *fz = flags, *fy = flags + g.Z, *fx = flags + g.Z + g.Y;
+    const int64_t first = (int64_t)blockIdx.x * kSeedChunk + (int64_t)threadIdx.x * kSeedPerThread;
+    const int c = thread_seed_count(lab, fz, fy, fx, g, first);
+    int tot;
+    const int off = block_exclusive_scan(c, &tot);
+    if (c == 0) return;
+    int64_t o = block_offsets[blockIdx.x] + off;
+    int z, y, x;
+    coords(g, first, z, y, x);
+    for (int k = 0; k < kSeedPerThread; ++k) {
+        const int64_t i = first + k;
+        if (i >= g.V) break;
+        if (is_seed(lab, fz, fy, fx, g, z, y, x, i)) {
+            if (o < capacity) {
+                bmx_seed s;
+                s.pos = (uint32_t)i;
+                s.label = lab[i];
+                s.var = calc_var(lab, fz, fy, fx, img, g, z, y, x, i);
+                s.reserved = 0;
+                seeds[o] = s;
+            }
+            ++o;
+        }
+        advance(g, z, y, x);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmx_plane_flags(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, int allaxis,
+                    uint8_t *flags)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(lab && flags, "bmx_plane_flags: null pointer");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32,
+              "bmx_plane_flags: bad dims %lld x %lld x %lld", (long long)Z, (long long)Y,
+              (long long)X);
+    BMX_HIP(hipMemsetAsync(flags, 0, (size_t)(Z + Y + X), ctx->stream));
+    const int64_t V = Z * Y * X;
+    int64_t blocks = (V + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_plane_flags<<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, (int)Z, (int)Y, (int)X, allaxis,
+                                                            flags, flags + Z, flags + Z + Y);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype, int64_t Z, int64_t Y,
+              int64_t X, const uint8_t *flags, bmx_seed *seeds, int64_t capacity, int64_t *nseeds)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(lab && flags && nseeds, "bmx_seeds: null pointer");
+    BMX_CHECK(img_dtype == BMX_IMG_U8 || img_dtype == BMX_IMG_F32, "bmx_seeds: bad image dtype %d",
+              img_dtype);
+    BMX_CHECK(seeds == nullptr || img != nullptr, "bmx_seeds: image needed to write seeds");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32,
+              "bmx_seeds: bad dims %lld x %lld x %lld", (long long)Z, (long long)Y, (long long)X);
+    Geom g{(int)Z, (int)Y, (int)X, X, X * Y, X * Y * Z};
+    const int64_t nblk = (g.V + kSeedChunk - 1) / kSeedChunk;
+    if (ctx->block_counts_cap < nblk) {
+        if (ctx->d_block_counts) BMX_HIP(hipFree(ctx->d_block_counts));
+        ctx->d_block_counts = nullptr;
+        ctx->block_counts_cap = 0;
+        BMX_HIP(hipMalloc((void **)&ctx->d_block_counts, (size_t)nblk * sizeof(int64_t)));
+        ctx->block_counts_cap = nblk;
+    }
+    k_seed_count<<<(unsigned)nblk, kSeedThreads, 0, ctx->stream>>>(lab, flags, g, ctx->d_block_counts);
+    BMX_LAUNCH_CHECK();
+    k_scan_counts<<<1, 1024, 0, ctx->stream>>>(ctx->d_block_counts, nblk, ctx->d_counter);
+    BMX_LAUNCH_CHECK();
+    if (seeds) {
+        if (img_dtype == BMX_IMG_U8)
+            k_seed_write<uint8_t><<<(unsigned)nblk, kSeedThreads, 0, ctx->stream>>>(
+                lab, (const uint8_t *)img, flags, g, ctx->d_block_counts, seeds, capacity);
+        else
+            k_seed_write<float><<<(unsigned)nblk, kSeedThreads, 0, ctx->stream>>>(
+                lab, (const float *)img, flags, g, ctx->d_block_counts, seeds, capacity);
+        BMX_LAUNCH_CHECK();
+    }
+    uint64_t total = 0;
+    BMX_HIP(hipMemcpyAsync(&total, ctx->d_counter, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+    BMX_HIP(hipStreamSynchronize(ctx->stream));
+    *nseeds = (int64_t)total;
+    if (seeds && (int64_t)total > capacity)
+        return bmx_set_error(BMX_E_INVALID, "bmx_seeds: %lld seeds exceed capacity %lld",
+                             (long long)total, (long long)capacity);
+    return BMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// bmx_smooth.hip -- curvature smoothing of the per-label hit volumes.
+//
+// One round of the reference is two launches (opencl_kernels.py:136-224):
+// CURVATURE_SRC writes kappa(phi) for interior voxels, then UPDATE_SRC adds
+// 0.01*kappa to phi.  kappa at round t only reads phi_t, so one fused stencil
+// pass phi_{t+1} = phi_t + 0.01*kappa(phi_t) into a second buffer gives the
+// same floats with half the traffic.  Interior expressions are written in
+// the reference's operation order (the build disables FP contraction).
+//
+// Tiling: a 64x4 (x,y) tile per block marches along a chunk of z planes; the
+// planes z-1, z, z+1 of the tile plus a 1-voxel halo rotate through three LDS
+// slots, so each input voxel is read from memory about once per pass.
+#include "bmx_internal.h"
+
+namespace {
+
+constexpr int TX = 64;
+constexpr int TY = 4;
+constexpr int kSmoothThreads = TX * TY;
+constexpr int kZChunk = 64;
+
+template <typename T> __device__ __forceinline__ float ldf(const T *p, int64_t i);
+template <> __device__ __forceinline__ float ldf<float>(const float *p, int64_t i) { return p[i]; }
+template <> __device__ __forceinline__ float ldf<uint32_t>(const uint32_t *p, int64_t i)
+{
+    return (float)p[i];
+}
+
+template <typename Tin>
+__global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__restrict__ in,
+                                                               float *__restrict__ out, int Z, int Y,
+                                                               int X)
+{
+    __shared__ float s[3][TY + 2][TX + 2];
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int x = blockIdx.x * TX + tx, y = blockIdx.y * TY + ty;
+    const int z_begin = blockIdx.z * kZChunk;
+    const int z_end = min(Z, z_begin + kZChunk);
+    const int64_t sy = X, sz = (int64_t)X * Y;
+
+    auto load_plane = [&](int slot, int zz) {
+        zz = min(max(zz, 0), Z - 1);
+        for (int k = threadIdx.x; k < (TY + 2) * (TX + 2); k += kSmoothThreads) {
+            const int ly = k / (TX + 2), lx = k - ly * (TX + 2);
+            const int gy = min(max((int)(blockIdx.y * TY) + ly - 1, 0), Y - 1);
+            const int gx = min(max((int)(blockIdx.x * TX) + lx - 1, 0), X - 1);
+            s[slot][ly][lx] = ldf(in, (int64_t)zz * sz + (int64_t)gy * sy + gx);
+        }
+    };
+
+    int sm = 0, sc = 1, sp = 2;
+    load_plane(sm, z_begin - 1);
+    load_plane(sc, z_begin);
+    const bool inside_xy = x > 0 && x < X - 1 && y > 0 && y < Y - 1;
+    const int ly = ty + 1, lx = tx + 1;
+    for (int z = z_begin; z < z_end; ++z) {
+        load_plane(sp, z + 1);
+        __syncthreads();
+        if (x < X && y < Y) {
+            const float pc = s[sc][ly][lx];
+            float res = pc;
+            if (inside_xy && z > 0 && z < Z - 1) {
+                const float eps = 1e-10f;
+                const float xm = s[sc][ly][lx - 1], xp = s[sc][ly][lx + 1];
+                const float ym = s[sc][ly - 1][lx], yp = s[sc][ly + 1][lx];
+                const float zm = s[sm][ly][lx], zp = s[sp][ly][lx];
+                const float dx = (xm - xp) * 0.5f;
+                const float dxx = (xm - 2.0f * pc + xp);
+                const float dx2 = dx * dx;
+                const float dy = (ym - yp) * 0.5f;
+                const float dyy = (ym - 2.0f * pc + yp);
+                const float dy2 = dy * dy;
+                const float dz = (zm - zp) * 0.5f;
+                const float dzz = (zm - 2.0f * pc + zp);
+                const float dz2 = dz * dz;
+                const float dxy = (s[sc][ly - 1][lx - 1] + s[sc][ly + 1][lx + 1]
+                                   - s[sc][ly - 1][lx + 1] - s[sc][ly + 1][lx - 1]) * 0.25f;
+                const float dxz = (s[sp][ly][lx - 1] + s[sm][ly][lx + 1]
+                                   - s[sp][ly][lx + 1] - s[sm][ly][lx - 1]) * 0.25f;
+                const float dyz = (s[sp][ly - 1][lx] + s[sm][ly + 1][lx]
+                                   - s[sp][ly + 1][lx] - s[sm][ly - 1][lx]) * 0.25f;
+                const float curv = (dxx * (dy2 + dz2)
+                                    + dyy * (dx2 + dz2)
+                                    + dzz * (dx2 + dy2)
+                                    - 2.0f * dx * dy * dxy
+                                    - 2.0f * dx * dz * dxz
+                                    - 2.0f * dy * dz * dyz)
+                                   / (dx2 + dy2 + dz2 + eps);
+                res = pc + 0.01f * curv;
+            }
+            out[(int64_t)z * sz + (int64_t)y * sy + x] = res;
+        }
+        __syncthreads();
+        const int t = sm; sm = sc; sc = sp; sp = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_argmax_update(const float *__restrict__ phi,
+                                                       float *__restrict__ best,
+                                                       uint8_t *__restrict__ bidx, int label, int64_t V)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const float p = phi[v];
+    if (label == 0 || p > best[v]) {
+        best[v] = p;
+        bidx[v] = (uint8_t)label;
+    }
+}
+
+struct LabelValues {
+    uint8_t v[256];
+};
+
+__global__ __launch_bounds__(256) void k_map_values(const uint8_t *__restrict__ bidx, LabelValues lv,
+                                                    uint8_t *__restrict__ out, int64_t V)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    out[v] = lv.v[bidx[v]];
+}
+
+template <typename Tin>
+int launch_step(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X)
+{
+    dim3 grid((X + TX - 1) / TX, (Y + TY - 1) / TY, (Z + kZChunk - 1) / kZChunk);
+    k_smooth_step<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+// iters >= 1 rounds starting from `in`; the result lands in *result
+// (one of a / b).
+template <typename Tin>
+int smooth_rounds(bmx_ctx *ctx, const Tin *in, float *a, float *b, int Z, int Y, int X, int iters,
+                  float **result)
+{
+    int rc = launch_step<Tin>(ctx, in, a, Z, Y, X);
+    if (rc) return rc;
+    float *cur = a, *nxt = b;
+    for (int it = 1; it < iters; ++it) {
+        rc = launch_step<float>(ctx, cur, nxt, Z, Y, X);
+        if (rc) return rc;
+        float *t = cur; cur = nxt; nxt = t;
+    }
+    *result = cur;
+    return BMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmx_smooth(bmx_ctx *ctx, float *phi, float *tmp, int64_t Z, int64_t Y, int64_t X, int iters)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(phi && tmp, "bmx_smooth: null pointer");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_smooth: bad dims");
+    BMX_CHECK(iters >= 0, "bmx_smooth: negative iters");
+    float *cur = phi, *nxt = tmp;
+    for (int it = 0; it < iters; ++it) {
+        rc = launch_step<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X);
+        if (rc) return rc;
+        float *t = cur; cur = nxt; nxt = t;
+    }
+    if (cur != phi)
+        BMX_HIP(hipMemcpyAsync(phi, cur, (size_t)(Z * Y * X) * sizeof(float), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    return BMX_OK;
+}
+
+int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y, int64_t X,
+                      int iters, const uint8_t *values, uint8_t *out)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(hits && values && out, "bmx_smooth_argmax: null pointer");
+    BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_smooth_argmax: nlabels %d not in [1,255]", nlabels);
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_smooth_argmax: bad dims");
+    BMX_CHECK(iters >= 1, "bmx_smooth_argmax: iters must be >= 1");
+    const int64_t V = Z * Y * X;
+    float *a = nullptr, *b = nullptr, *best = nullptr;
+    uint8_t *bidx = nullptr;
+    hipError_t e = hipMalloc((void **)&a, (size_t)V * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&b, (size_t)V * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&best, (size_t)V * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&bidx, (size_t)V);
+    if (e == hipSuccess) {
+        for (int l = 0; l < nlabels && rc == BMX_OK; ++l) {
+            float *res = nullptr;
+            rc = smooth_rounds<uint32_t>(ctx, hits + (int64_t)l * V, a, b, (int)Z, (int)Y, (int)X,
+                                         iters, &res);
+            if (rc == BMX_OK) {
+                k_argmax_update<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(res, best, bidx, l, V);
+                if (hipGetLastError() != hipSuccess)
+                    rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
+            }
+        }
+        if (rc == BMX_OK) {
+            LabelValues lv;
+            for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
+            k_map_values<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(bidx, lv, out, V);
+            if (hipGetLastError() != hipSuccess)
+                rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
+        }
+        hipError_t s = hipStreamSynchronize(ctx->stream);
+        if (s != hipSuccess && rc == BMX_OK)
+            rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: %s", hipGetErrorString(s));
+    } else {
+        (void)hipGetLastError();
+        rc = bmx_set_error(e == hipErrorOutOfMemory ? BMX_E_OOM : BMX_E_DEVICE,
+                           "bmx_smooth_argmax: allocation failed: %s", hipGetErrorString(e));
+    }
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (best) (void)hipFree(best);
+    if (bidx) (void)hipFree(bidx);
+    return rc;
+}
+
+}  // extern "C"

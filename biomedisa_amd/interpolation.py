@@ -1,0 +1,194 @@
+"""Drop-in ``smart_interpolation`` running on one MI355X through libbmx_hip.
+
+Mirrors ``biomedisa.interpolation.smart_interpolation``
+(biomedisa/interpolation.py:43-284) and the small-volume driver
+``rw_small._diffusion_child`` (biomedisa/features/random_walk/rw_small.py:
+39-281): same arguments, same pre-processing, same result dictionary.  The
+MPI split, the per-label OpenCL launches and the host argmax are replaced by
+one device pipeline per call:
+
+    plane flags -> seeds -> walk (all labels, one launch) -> argmax
+                 [-> smoothing + argmax] [-> uncertainty]
+
+Extra keyword arguments (after the reference's):
+    rng     'philox' (default; counter-based stream, walks of a seed run in
+            parallel) or 'reference' (the reference's float MRG recurrence,
+            one thread per seed, all walks in sequence).
+    key     64-bit Philox key.
+    device  HIP device ordinal.
+"""
+import time
+
+import numpy as np
+
+from . import _hip
+from .helpers import (Biomedisa, Dice_score, _error_, adjacent_planes, label_index_volume,
+                      normalize_image, pre_processing, predict_blocksize)
+
+DEFAULT_KEY = 0x5EED0B10D1CE0001
+_RNG = {"philox": _hip.RNG_PHILOX, "reference": _hip.RNG_REF}
+
+
+class DeviceVolume:
+    """Image + label-index volume resident on the device, ready for the pipeline."""
+
+    def __init__(self, eng, img, labidx):
+        assert img.shape == labidx.shape and img.ndim == 3
+        if img.dtype == np.uint8:
+            self.img_dtype = _hip.IMG_U8
+        elif img.dtype == np.float32:
+            self.img_dtype = _hip.IMG_F32
+        else:
+            raise TypeError("image must be uint8 or float32 after normalisation, got %s" % img.dtype)
+        self.eng = eng
+        self.dims = tuple(int(s) for s in img.shape)
+        self.img = eng.upload(img)
+        self.lab = eng.upload(np.ascontiguousarray(labidx, dtype=np.uint8))
+
+    @property
+    def V(self):
+        Z, Y, X = self.dims
+        return Z * Y * X
+
+
+class PipelineError(ValueError):
+    """Input rejected by the pipeline (the reference's _error_ cases)."""
+
+
+def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth=0,
+                 uncertainty=False, rng="philox", key=DEFAULT_KEY, keep_hits=False, timings=None):
+    """Seeds, walk, merge (and optional smoothing / uncertainty) on device-resident inputs.
+
+    Returns a dict of device buffers: 'regular' (uint8 V), optional 'smooth',
+    'uncertainty', 'hits' (uint32 [nlabels][V]), plus host ints 'nseeds',
+    'accepted', 'flags' (uint8 Z+Y+X).
+    """
+    eng = vol.eng
+    Z, Y, X = vol.dims
+    V = vol.V
+    if nlabels > 255:
+        raise PipelineError("at most 255 labels (including background) are supported")
+    t = timings if timings is not None else {}
+    t0 = time.perf_counter()
+    flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis)
+    flags = flags_buf.download((Z + Y + X,), np.uint8)
+    if allaxis and adjacent_planes((flags[:Z], flags[Z:Z + Y], flags[Z + Y:])):
+        raise PipelineError("At least one empty slice between labels required.")
+    seeds, nseeds = eng.seeds(vol.lab, vol.img, vol.img_dtype, vol.dims, flags_buf)
+    t1 = time.perf_counter()
+    hits = eng.alloc(nlabels * V * 4)
+    hits.zero()
+    accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, seeds, nseeds, nlabels, nbrw, sorw,
+                        _RNG[rng], key, hits)
+    t2 = time.perf_counter()
+    out = {"nseeds": nseeds, "accepted": accepted, "flags": flags}
+    regular = eng.alloc(V)
+    eng.argmax(hits, nlabels, V, values, regular)
+    out["regular"] = regular
+    if smooth:
+        sm = eng.alloc(V)
+        eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm)
+        out["smooth"] = sm
+    if uncertainty:
+        uq = eng.alloc(V)
+        eng.uncertainty(hits, nlabels, vol.dims, uq)
+        out["uncertainty"] = uq
+    eng.sync()
+    t3 = time.perf_counter()
+    if keep_hits:
+        out["hits"] = hits
+    else:
+        hits.free()
+    seeds.free()
+    t.update(seeds_s=t1 - t0, walk_s=t2 - t1, merge_s=t3 - t2, walk_kernel_ms=eng.last_walk_ms())
+    return out
+
+
+def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_alpha=1.0,
+                        acwe_smooth=1, acwe_steps=3, path_to_data=None, path_to_labels=None,
+                        denoise=False, uncertainty=False, platform=None, allaxis=False,
+                        ignore='none', only='all', smooth=0, compression=True, return_hits=False,
+                        img_id=None, label_id=None, remote=False, queue=0, clean=None, fill=None,
+                        slicer=False, rng="philox", key=DEFAULT_KEY, device=0):
+    """Smart interpolation of sparsely labelled slices (biomedisa/interpolation.py:43).
+
+    Returns the reference's result dict ({'regular': uint8 (Z,Y,X)} plus
+    'smooth', 'uncertainty', 'hits' when requested), or None when the input is
+    rejected (after printing "Error: ..." as the reference does).
+    """
+    if data is None or labelData is None:
+        raise NotImplementedError("file loading (path_to_data/path_to_labels) is not part of this "
+                                  "engine: pass NumPy arrays")
+    if acwe or clean or fill or denoise:
+        raise NotImplementedError("acwe/clean/fill/denoise post-processing is not implemented")
+    if rng not in _RNG:
+        raise ValueError("rng must be 'philox' or 'reference'")
+
+    bm = Biomedisa()
+    bm.TIC = time.time()
+    bm.success = True
+    bm.data, bm.labelData = data, labelData
+    bm.nbrw, bm.sorw, bm.allaxis, bm.smooth = int(nbrw), int(sorw), bool(allaxis), int(smooth)
+    bm.uncertainty, bm.return_hits = bool(uncertainty), bool(return_hits)
+    bm.ignore, bm.only, bm.platform, bm.slicer = ignore, only, platform, slicer
+
+    bm = pre_processing(bm)
+    if not bm.success:
+        return None
+
+    # data types (interpolation.py:135-147)
+    bm.data = normalize_image(bm.data)
+    if bm.labelData.dtype in ["uint32", "int64", "uint64"]:
+        print(f"Warning: Potential label loss during conversion from {bm.labelData.dtype} to int32.")
+        bm.labelData = bm.labelData.astype(np.int32)
+
+    # 1-voxel pad (:157-159) and label bounding box +-100 (:192, :203-204)
+    img = np.pad(bm.data, 1, mode="constant", constant_values=0)
+    lab = np.pad(bm.labelData, 1, mode="constant", constant_values=0)
+    bm.zsh, bm.ysh, bm.xsh = img.shape
+    z0, z1, y0, y1, x0, x1 = predict_blocksize(lab)
+    img_c = np.ascontiguousarray(img[z0:z1, y0:y1, x0:x1])
+    lab_c = np.ascontiguousarray(lab[z0:z1, y0:y1, x0:x1])
+    labidx = label_index_volume(lab_c, bm.allLabels)
+    values = np.asarray(bm.allLabels, dtype=np.uint8)
+    if bm.nol > 255:
+        _error_(bm, "At most 255 labels (including background) are supported.")
+        return None
+
+    eng = _hip.engine(device)
+    vol = DeviceVolume(eng, img_c, labidx)
+    try:
+        dev = run_pipeline(vol, bm.nol, values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
+                           bm.uncertainty, rng, key, keep_hits=bm.return_hits)
+    except PipelineError as e:
+        _error_(bm, str(e))
+        return None
+
+    shape_c = vol.dims
+    results = {}
+
+    def place(buf):
+        crop = buf.download(shape_c, np.uint8)
+        full = np.zeros((bm.zsh, bm.ysh, bm.xsh), dtype=np.uint8)
+        full[z0:z1, y0:y1, x0:x1] = crop
+        return full[1:-1, 1:-1, 1:-1], crop
+
+    final_result, final_crop = place(dev["regular"])
+    if "smooth" in dev:
+        results["smooth"] = place(dev["smooth"])[0]
+    if "uncertainty" in dev:
+        results["uncertainty"] = place(dev["uncertainty"])[0]
+    if bm.return_hits:
+        h = dev["hits"].download((bm.nol,) + shape_c, np.uint32)
+        hits = np.zeros((bm.nol, bm.zsh, bm.ysh, bm.xsh), dtype=np.float32)
+        hits[:, z0:z1, y0:y1, x0:x1] = h
+        results["hits"] = hits[:, 1:-1, 1:-1, 1:-1]
+
+    # validate result (rw_small.py:203-211)
+    mask = lab_c > 0
+    dice = Dice_score(lab_c, final_crop * mask)
+    if dice < 0.3:
+        print('Warning: Bad result! Use "--allaxis" if you labeled axes other than the xy-plane.')
+
+    results["regular"] = final_result
+    return results

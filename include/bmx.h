@@ -1,0 +1,131 @@
+/*
+ * bmx.h -- C ABI of libbmx_hip.so, the MI355X smart-interpolation engine.
+ *
+ * This library replaces the platform backends and device kernels of
+ * biomedisa's smart interpolation (biomedisa/features/random_walk/
+ * pyopencl_small.py, pycuda_small*.py, opencl_kernels.py, gpu_kernels.py)
+ * and the host merge of rw_small.py.  Each entry point names the reference
+ * interface it stands in for.
+ *
+ * Conventions
+ *   - Volumes are C-contiguous (Z, Y, X) arrays in device memory (allocate
+ *     with bmx_malloc, fill with bmx_upload).  V = Z*Y*X must be < 2^32.
+ *   - Label volumes hold a label INDEX per voxel (uint8): 0 = background,
+ *     1..nlabels-1 = the sorted non-zero label values, BMX_LABEL_NONE (0xFF)
+ *     = negative (ignored) label.  At most 255 label indices (0..254).
+ *   - Images are uint8 (BMX_IMG_U8) or float32 in [0,255] (BMX_IMG_F32);
+ *     the reference normalises every other dtype to the latter
+ *     (biomedisa/interpolation.py:135-144).
+ *   - Return value 0 = success, negative = error; bmx_last_error() gives the
+ *     message of the last error on the calling thread.
+ *   - A context owns one HIP stream; calls on one context are ordered on it.
+ *     Contexts are not shared between threads.
+ */
+#ifndef BMX_H
+#define BMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMX_OK 0
+#define BMX_E_INVALID (-1)
+#define BMX_E_OOM (-2)
+#define BMX_E_DEVICE (-3)
+
+#define BMX_IMG_U8 0
+#define BMX_IMG_F32 1
+
+/* Random streams of the walker. */
+#define BMX_RNG_PHILOX 0 /* counter-based Philox4x32-10 per (seed voxel, walk) */
+#define BMX_RNG_REF 1    /* the reference's float MRG recurrence per seed     */
+
+#define BMX_LABEL_NONE 0xFF
+
+typedef struct bmx_ctx bmx_ctx;
+
+/* One walker seed: voxel index, label index, intensity variance of the
+ * seed's neighbourhood (_calc_var, pyopencl_small.py:140-185). */
+typedef struct bmx_seed {
+    uint32_t pos;
+    uint32_t label;
+    float var;
+    uint32_t reserved;
+} bmx_seed;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *bmx_last_error(void);
+int bmx_version(void);
+
+/* Context: replaces the per-rank OpenCL context/queue selection of
+ * _get_device (biomedisa/features/biomedisa_helper.py:1003-1013) and the
+ * pycuda context of rw_small.py:63-72. */
+int bmx_open(int device, bmx_ctx **out);
+int bmx_close(bmx_ctx *ctx);
+int bmx_sync(bmx_ctx *ctx);
+
+/* Device memory: replaces cl.Buffer / gpuarray allocation and the blocking
+ * enqueue_copy transfers of pyopencl_small.py:118-121,133. */
+int bmx_malloc(bmx_ctx *ctx, size_t bytes, void **dptr);
+int bmx_free(bmx_ctx *ctx, void *dptr);
+int bmx_upload(bmx_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int bmx_download(bmx_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+int bmx_zero(bmx_ctx *ctx, void *dst_dev, size_t bytes);
+
+/* Labelled planes: replaces read_labeled_slices / read_labeled_slices_allx /
+ * read_indices_allx (biomedisa_helper.py:1015-1066).  flags (device, Z+Y+X
+ * bytes) receives 1 for every labelled z-plane, then y-plane, then x-plane. */
+int bmx_plane_flags(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
+                    int allaxis, uint8_t *flags);
+
+/* Seed extraction: replaces the label-volume rebuild of pyopencl_small.py:
+ * 89-107 and the per-work-item seed test and _calc_var of :140-185, :205-228.
+ * Writes the seeds in ascending voxel order.  seeds == NULL counts only.
+ * *nseeds receives the count; capacity bounds the writes. */
+int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype,
+              int64_t Z, int64_t Y, int64_t X, const uint8_t *flags,
+              bmx_seed *seeds, int64_t capacity, int64_t *nseeds);
+
+/* Random walks of every seed, all labels in one launch: replaces the
+ * per-label loop fill -> randomWalk -> D2H of pyopencl_small.py:128-134 and
+ * the randomWalk kernel :192-314.  hits (device, uint32 [nlabels][V]) is
+ * ACCUMULATED into (zero it first with bmx_zero).  rng selects the stream;
+ * key is the Philox key (ignored for BMX_RNG_REF).  *accepted (may be NULL)
+ * receives the number of accepted moves = the hits added. */
+int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,
+             const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw,
+             int rng, uint64_t key, uint32_t *hits, uint64_t *accepted);
+
+/* Milliseconds of the last bmx_walk kernel, measured with HIP events on the
+ * context stream. */
+float bmx_last_walk_ms(bmx_ctx *ctx);
+
+/* Per-voxel argmax over labels, first maximum wins, mapped through
+ * values[nlabels] (label index -> label value): replaces np.argmax +
+ * get_labels (rw_small.py:200-201, biomedisa_helper.py:1209-1213). */
+int bmx_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,
+               const uint8_t *values, uint8_t *out);
+
+/* Curvature smoothing of one float32 volume in place: `iters` rounds of
+ * CURVATURE_SRC then UPDATE_SRC (opencl_kernels.py:136-224), fused into one
+ * stencil pass per round.  tmp: device scratch of V floats. */
+int bmx_smooth(bmx_ctx *ctx, float *phi, float *tmp, int64_t Z, int64_t Y, int64_t X, int iters);
+
+/* The 'smooth' result of rw_small.py:114-140: every label's hits smoothed
+ * `iters` times, then first-maximum argmax mapped through values. */
+int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y,
+                      int64_t X, int iters, const uint8_t *values, uint8_t *out);
+
+/* The 'uncertainty' result of rw_small.py:152-176 (MAX_SRC + UNCERTAINTY_SRC,
+ * opencl_kernels.py:54-133): uint8(255 * (1 - (1-m0/m2)(1-m1/m2))) from the
+ * three largest hit counts, interior voxels; 0 elsewhere. */
+int bmx_uncertainty(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y,
+                    int64_t X, uint8_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BMX_H */

@@ -1,0 +1,520 @@
+/*
+ * bmx_oracle.c -- CPU restatement of biomedisa's smart-interpolation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or, for ref_walk, as the timed CPU baseline).  The product path
+ * (biomedisa_amd) never calls it.
+ *
+ * Two walkers live here:
+ *   orc_ref_walk    -- literal restatement of the reference OpenCL kernel
+ *                      `randomWalk` (biomedisa/features/random_walk/
+ *                      pyopencl_small.py:192-314), including its float-valued
+ *                      MRG32k3a-like recurrence seeded with the voxel index
+ *                      (:213-223, :256-282) and its direction chain (:285-290).
+ *                      Hits per label as in the per-label loop (:128-134).
+ *   orc_philox_walk -- the same transition law driven by a counter-based
+ *                      Philox4x32-10 stream per (seed voxel, walk).  This is
+ *                      the bit-exact target of the HIP walker.
+ * Both use orc_expf (Cody-Waite reduction + degree-7 Taylor polynomial with
+ * fused multiply-adds) in place of the vendor exp the reference JIT links
+ * (pyopencl_small.py:187-190), so that CPU and GPU agree bit for bit.
+ *
+ * Arithmetic contract (build with -ffp-contract=off): every float operation
+ * below is a single IEEE binary32 operation in the order written; fmaf is
+ * used only where written explicitly.
+ *
+ * Parity pins (tests/test_oracle.py): Philox known-answer vectors (Random123),
+ * orc_expf vs float64 exp, seed extraction vs brute force, transition law vs
+ * its closed form (chi^2), hit conservation, argmax ties, curvature closed
+ * forms, uncertainty closed forms.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_IGN 0xFFu
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10: Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as   */
+/* easy as 1, 2, 3", SC'11.  Multipliers and Weyl constants from the paper. */
+/* ------------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
+                       uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* ------------------------------------------------------------------------ */
+/* exp for binary32.  Cody & Waite argument reduction x = n*ln2 + r with a   */
+/* two-part ln2 (hi part exact in 16 bits), fused with fmaf; e^r by its      */
+/* Taylor series to degree 7 (|r| <= ln2/2, truncation < 6e-9 relative);     */
+/* 2^n applied to the exponent field.  Results below 2^-125 flush to +0.     */
+/* ------------------------------------------------------------------------ */
+static inline float orc_bits_to_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline uint32_t orc_float_to_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+float orc_expf(float x)
+{
+    const float LOG2E = 1.44269502925872802734375f;     /* 0x3fb8aa3b */
+    const float LN2_HI = 0.693145751953125f;            /* 0x3f317200 */
+    const float LN2_LO = 1.428606765330187045e-06f;     /* 0x35bfbe8e */
+    if (x > 88.0f) return INFINITY;
+    float t = x * LOG2E;
+    float n = rintf(t);
+    if (n < -125.0f) return 0.0f;
+    float r = fmaf(-n, LN2_HI, x);
+    r = fmaf(-n, LN2_LO, r);
+    float p = 1.98412701e-4f;                   /* 1/5040 */
+    p = fmaf(p, r, 1.38888892e-3f);             /* 1/720  */
+    p = fmaf(p, r, 8.33333377e-3f);             /* 1/120  */
+    p = fmaf(p, r, 4.16666679e-2f);             /* 1/24   */
+    p = fmaf(p, r, 1.66666672e-1f);             /* 1/6    */
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    int32_t ni = (int32_t)n;
+    uint32_t bits = orc_float_to_bits(p) + ((uint32_t)ni << 23);
+    return orc_bits_to_float(bits);
+}
+
+/* weight(B, A, div1) of pyopencl_small.py:187-190: exp(-tmp*tmp*div1). */
+static inline float orc_weight(float B, float A, float div1)
+{
+    float tmp = B - A;
+    float e = ((-tmp) * tmp) * div1;
+    return orc_expf(e);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Image access: dtype 0 = uint8, 1 = float32 (reference normalises every    */
+/* other dtype to float32 in [0,255], interpolation.py:135-144).             */
+/* ------------------------------------------------------------------------ */
+static inline float orc_img(const void *img, int dtype, int64_t i)
+{
+    if (dtype == 0) return (float)((const uint8_t *)img)[i];
+    return ((const float *)img)[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Labelled-plane detection.                                                  */
+/*   XY mode (read_labeled_slices, biomedisa_helper.py:1015-1022): plane z   */
+/*   is labelled iff it holds any non-zero label value.                      */
+/*   allaxis (read_labeled_slices_allx, :1024-1045): per axis, voxels of     */
+/*   1-voxel-thin lines are removed first along the first in-plane axis, then */
+/*   along the second; a plane is labelled iff anything survives.  In-plane  */
+/*   axis order: plane z -> (y, x); plane y -> (z, x); plane x -> (z, y).    */
+/* labidx: label index per voxel, 0 = background, ORC_IGN = negative label;  */
+/* "non-zero label value" <=> labidx != 0.                                   */
+/* ------------------------------------------------------------------------ */
+static inline int orc_nz(const uint8_t *lab, int64_t i) { return lab[i] != 0; }
+
+/* ones after the first pass, at voxel (a,b) of a plane whose first in-plane  */
+/* axis has extent na and stride sa. */
+static inline int orc_pass1(const uint8_t *lab, int64_t base, int a, int na, int64_t sa)
+{
+    if (!orc_nz(lab, base)) return 0;
+    if (a > 0 && a < na - 1 && !orc_nz(lab, base - sa) && !orc_nz(lab, base + sa)) return 0;
+    return 1;
+}
+
+static inline int orc_pass2(const uint8_t *lab, int64_t base, int a, int na, int64_t sa,
+                            int b, int nb, int64_t sb)
+{
+    if (!orc_pass1(lab, base, a, na, sa)) return 0;
+    if (b > 0 && b < nb - 1 && !orc_pass1(lab, base - sb, a, na, sa)
+        && !orc_pass1(lab, base + sb, a, na, sa)) return 0;
+    return 1;
+}
+
+void orc_plane_flags(const uint8_t *lab, int Z, int Y, int X, int allaxis,
+                     uint8_t *fz, uint8_t *fy, uint8_t *fx)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y;
+    memset(fz, 0, Z); memset(fy, 0, Y); memset(fx, 0, X);
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                int64_t i = z * sz + y * sy + x;
+                if (!orc_nz(lab, i)) continue;
+                if (!allaxis) { fz[z] = 1; continue; }
+                if (!fz[z] && orc_pass2(lab, i, y, Y, sy, x, X, 1)) fz[z] = 1;
+                if (!fy[y] && orc_pass2(lab, i, z, Z, sz, x, X, 1)) fy[y] = 1;
+                if (!fx[x] && orc_pass2(lab, i, z, Z, sz, y, Y, sy)) fx[x] = 1;
+            }
+}
+
+/* Mask value of the volume the reference rebuilds from the labelled slices  */
+/* (pyopencl_small.py:97-107): label index inside a labelled plane, ORC_IGN  */
+/* elsewhere (negative labels are ORC_IGN already). */
+static inline uint8_t orc_mask(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
+                               const uint8_t *fx, int z, int y, int x, int64_t i)
+{
+    return (fz[z] || fy[y] || fx[x]) ? lab[i] : (uint8_t)ORC_IGN;
+}
+
+/* _calc_var of pyopencl_small.py:140-185 for the seed at (z,y,x). */
+static float orc_calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
+                          const uint8_t *fx, const void *img, int dtype,
+                          int Y, int X, int z, int y, int x)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y;
+    const int64_t i0 = z * sz + y * sy + x;
+    const uint8_t seg = orc_mask(lab, fz, fy, fx, z, y, x, i0);
+    const float B = orc_img(img, dtype, i0);
+    float dev = 0.0f, summe = 0.0f;
+    for (int n = -1; n < 2; ++n)            /* XY plane */
+        for (int o = -1; o < 2; ++o) {
+            int64_t i = i0 + n * sy + o;
+            if (orc_mask(lab, fz, fy, fx, z, y + n, x + o, i) == seg) {
+                float t = B - orc_img(img, dtype, i);
+                dev += t * t; summe += 1.0f;
+            }
+        }
+    for (int n = -1; n <= 1; n += 2)        /* "XZ": (z+n, y, x+o) */
+        for (int o = -1; o < 2; ++o) {
+            int64_t i = i0 + n * sz + o;
+            if (summe < 9.0f && orc_mask(lab, fz, fy, fx, z + n, y, x + o, i) == seg) {
+                float t = B - orc_img(img, dtype, i);
+                dev += t * t; summe += 1.0f;
+            }
+        }
+    for (int n = -1; n <= 1; n += 2)        /* "YZ": (z+n, y+o, x) */
+        for (int o = -1; o <= 1; o += 2) {
+            int64_t i = i0 + n * sz + o * sy;
+            if (summe < 9.0f && orc_mask(lab, fz, fy, fx, z + n, y + o, x, i) == seg) {
+                float t = B - orc_img(img, dtype, i);
+                dev += t * t; summe += 1.0f;
+            }
+        }
+    float var = dev / summe;
+    if (var < 1.0f) var = 1.0f;
+    return var;
+}
+
+/* Seeds: every interior voxel of a labelled plane whose label is not        */
+/* negative (pyopencl_small.py:205-207), in ascending voxel order.           */
+/* Returns the seed count; with pos == NULL only counts.                     */
+int64_t orc_seeds(const uint8_t *lab, const void *img, int dtype, int Z, int Y, int X,
+                  const uint8_t *fz, const uint8_t *fy, const uint8_t *fx,
+                  int64_t *pos, uint8_t *label, float *var)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y;
+    int64_t n = 0;
+    for (int z = 1; z < Z - 1; ++z)
+        for (int y = 1; y < Y - 1; ++y)
+            for (int x = 1; x < X - 1; ++x) {
+                int64_t i = z * sz + y * sy + x;
+                uint8_t m = orc_mask(lab, fz, fy, fx, z, y, x, i);
+                if (m == ORC_IGN) continue;
+                if (pos) {
+                    pos[n] = i;
+                    label[n] = m;
+                    var[n] = orc_calc_var(lab, fz, fy, fx, img, dtype, Y, X, z, y, x);
+                }
+                ++n;
+            }
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Direction chain of pyopencl_small.py:285-290.  Returns 0..5 for           */
+/* +z,-z,+y,-y,+x,-x, or -1 when no branch matches (rand > W5).              */
+/* ------------------------------------------------------------------------ */
+static inline int orc_direction(float rand, const float W[6])
+{
+    if (rand < W[0] || rand == 0.0f) return 0;
+    if (rand >= W[0] && rand < W[1]) return 1;
+    if (rand >= W[1] && rand < W[2]) return 2;
+    if (rand >= W[2] && rand < W[3]) return 3;
+    if (rand >= W[3] && rand < W[4]) return 4;
+    if (rand >= W[4] && rand <= W[5]) return 5;
+    return -1;
+}
+
+/* Six weights at voxel i, prefix-summed in the order W0..W5 (:241-252). */
+static inline void orc_weights(const void *img, int dtype, int64_t i, int64_t sy, int64_t sz,
+                               float B, float div1, float W[6])
+{
+    W[0] = orc_weight(B, orc_img(img, dtype, i + sz), div1);
+    W[1] = orc_weight(B, orc_img(img, dtype, i - sz), div1);
+    W[2] = orc_weight(B, orc_img(img, dtype, i + sy), div1);
+    W[3] = orc_weight(B, orc_img(img, dtype, i - sy), div1);
+    W[4] = orc_weight(B, orc_img(img, dtype, i + 1), div1);
+    W[5] = orc_weight(B, orc_img(img, dtype, i - 1), div1);
+    W[1] += W[0];
+    W[2] += W[1];
+    W[3] += W[2];
+    W[4] += W[3];
+    W[5] += W[4];
+}
+
+static const int ORC_DZ[6] = {1, -1, 0, 0, 0, 0};
+static const int ORC_DY[6] = {0, 0, 1, -1, 0, 0};
+static const int ORC_DX[6] = {0, 0, 0, 0, 1, -1};
+
+/* ------------------------------------------------------------------------ */
+/* Walker, Philox stream.  One independent walk per (seed, w): counter       */
+/* (pos_lo, pos_hi, w, step/4), key (key_lo, key_hi); step s uses word s%4.  */
+/* u = (word >> 8) * 2^-24 in [0,1); rand = W5 * u.                          */
+/* hits: uint32 [nlabels][V].  Returns the number of accepted moves.          */
+/* ------------------------------------------------------------------------ */
+int64_t orc_philox_walk(const void *img, int dtype, int Z, int Y, int X,
+                        const int64_t *pos, const uint8_t *label, const float *var,
+                        int64_t nseeds, int nbrw, int sorw, uint64_t key,
+                        uint32_t *hits)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
+    const uint32_t kk[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    int64_t accepted = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+:accepted)
+    for (int64_t s = 0; s < nseeds; ++s) {
+        const int64_t p0 = pos[s];
+        const int z0 = (int)(p0 / sz), y0 = (int)((p0 / sy) % Y), x0 = (int)(p0 % X);
+        const float B = orc_img(img, dtype, p0);
+        const float div1 = 1.0f / (2.0f * var[s]);
+        uint32_t *h = hits + (int64_t)label[s] * V;
+        for (int w = 0; w < nbrw; ++w) {
+            int64_t i = p0;
+            int k = z0, l = y0, m = x0;
+            uint32_t r[4] = {0, 0, 0, 0};
+            for (int step = 0; step < sorw; ++step) {
+                if ((step & 3) == 0) {
+                    const uint32_t c[4] = {(uint32_t)p0, (uint32_t)((uint64_t)p0 >> 32),
+                                           (uint32_t)w, (uint32_t)(step >> 2)};
+                    orc_philox4x32_10(c, kk, r);
+                }
+                float W[6];
+                orc_weights(img, dtype, i, sy, sz, B, div1, W);
+                float u = (float)(r[step & 3] >> 8) * 5.9604644775390625e-08f;
+                float rand = W[5] * u;
+                int d = orc_direction(rand, W);
+                if (d < 0) d = 5;   /* unreachable: u < 1 implies rand <= W5 */
+                int kn = k + ORC_DZ[d], ln = l + ORC_DY[d], mn = m + ORC_DX[d];
+                if (kn > 0 && kn < Z - 1 && ln > 0 && ln < Y - 1 && mn > 0 && mn < X - 1) {
+                    k = kn; l = ln; m = mn;
+                    i = k * sz + l * sy + m;
+#pragma omp atomic
+                    h[i] += 1u;
+                    ++accepted;
+                }
+            }
+        }
+    }
+    return accepted;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Walker, reference stream (pyopencl_small.py:209-311), all labels in one    */
+/* pass over the seeds: each seed only ever writes its own label's hits, so  */
+/* this equals the per-label launches of :128-134.  The recurrence state is  */
+/* seeded with the voxel index (as float) and runs across all nbrw walks.    */
+/* The direction registers start at (0,0,0) (uninitialised in the kernel)    */
+/* and keep their value when no branch of the chain matches.                 */
+/* ------------------------------------------------------------------------ */
+int64_t orc_ref_walk(const void *img, int dtype, int Z, int Y, int X,
+                     const int64_t *pos, const uint8_t *label, const float *var,
+                     int64_t nseeds, int nbrw, int sorw, uint32_t *hits)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
+    int64_t accepted = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+:accepted)
+    for (int64_t s = 0; s < nseeds; ++s) {
+        const int64_t p0 = pos[s];
+        const int z0 = (int)(p0 / sz), y0 = (int)((p0 / sy) % Y), x0 = (int)(p0 % X);
+        uint32_t *h = hits + (int64_t)label[s] * V;
+        const float norm = 2.328306549295728e-10f;
+        const float m1 = 4294967087.0f, m2 = 4294944443.0f;
+        const float a12 = 1403580.0f, a13n = 810728.0f, a21 = 527612.0f, a23n = 1370589.0f;
+        const float fi = (float)(int32_t)p0;
+        float s10 = fi, s11 = fi, s12 = fi, s20 = fi, s21 = fi, s22 = fi;
+        const float B = orc_img(img, dtype, p0);
+        const float div1 = 1.0f / (2.0f * var[s]);
+        int k = z0, l = y0, m = x0;
+        int64_t i = p0;
+        int n = 0, o = 0, p = 0;
+        int step = 0, n_rw = 0;
+        while (n_rw < nbrw) {
+            float W[6];
+            orc_weights(img, dtype, i, sy, sz, B, div1, W);
+            float p1 = a12 * s11 - a13n * s10;
+            int64_t k1 = (int64_t)(p1 / m1);
+            p1 -= (float)k1 * m1;
+            if (p1 < 0.0f) p1 += m1;
+            s10 = s11; s11 = s12; s12 = p1;
+            float p2 = a21 * s22 - a23n * s20;
+            k1 = (int64_t)(p2 / m2);
+            p2 -= (float)k1 * m2;
+            if (p2 < 0.0f) p2 += m2;
+            s20 = s21; s21 = s22; s22 = p2;
+            float rand;
+            if (p1 <= p2) rand = W[5] * ((p1 - p2 + m1) * norm);
+            else          rand = W[5] * ((p1 - p2) * norm);
+            int d = orc_direction(rand, W);
+            if (d >= 0) { n = ORC_DZ[d]; o = ORC_DY[d]; p = ORC_DX[d]; }
+            if (k + n > 0 && k + n < Z - 1 && l + o > 0 && l + o < Y - 1 && m + p > 0 && m + p < X - 1) {
+                k += n; l += o; m += p;
+                i = k * sz + l * sy + m;
+#pragma omp atomic
+                h[i] += 1u;
+                ++accepted;
+            }
+            step += 1;
+            if (step == sorw) {
+                k = z0; l = y0; m = x0;
+                i = p0;
+                n_rw += 1;
+                step = 0;
+            }
+        }
+    }
+    return accepted;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Argmax merge (rw_small.py:200-201 + get_labels, biomedisa_helper.py:      */
+/* 1209-1213): first maximum wins; index -> label value.                     */
+/* ------------------------------------------------------------------------ */
+void orc_argmax_u32(const uint32_t *hits, int nlabels, int64_t V,
+                    const uint8_t *values, uint8_t *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < V; ++v) {
+        int best = 0;
+        uint32_t bv = hits[v];
+        for (int l = 1; l < nlabels; ++l) {
+            uint32_t h = hits[(int64_t)l * V + v];
+            if (h > bv) { bv = h; best = l; }
+        }
+        out[v] = values[best];
+    }
+}
+
+void orc_argmax_f32(const float *phi, int nlabels, int64_t V,
+                    const uint8_t *values, uint8_t *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < V; ++v) {
+        int best = 0;
+        float bv = phi[v];
+        for (int l = 1; l < nlabels; ++l) {
+            float h = phi[(int64_t)l * V + v];
+            if (h > bv) { bv = h; best = l; }
+        }
+        out[v] = values[best];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Curvature smoothing: CURVATURE_SRC then UPDATE_SRC                        */
+/* (opencl_kernels.py:136-224), `iters` times, interior voxels only.         */
+/* tmp: scratch of V floats.                                                 */
+/* ------------------------------------------------------------------------ */
+void orc_smooth(float *phi, float *curv, int Z, int Y, int X, int iters)
+{
+    const int64_t xsh = X, flat = (int64_t)X * Y;
+    for (int it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(static)
+        for (int z = 1; z < Z - 1; ++z)
+            for (int y = 1; y < Y - 1; ++y)
+                for (int x = 1; x < X - 1; ++x) {
+                    const int64_t index = z * flat + y * xsh + x;
+                    const float eps = 1e-10f;
+                    float dx = (phi[index - 1] - phi[index + 1]) * 0.5f;
+                    float dxx = (phi[index - 1] - 2.0f * phi[index] + phi[index + 1]);
+                    float dx2 = dx * dx;
+                    float dy = (phi[index - xsh] - phi[index + xsh]) * 0.5f;
+                    float dyy = (phi[index - xsh] - 2.0f * phi[index] + phi[index + xsh]);
+                    float dy2 = dy * dy;
+                    float dz = (phi[index - flat] - phi[index + flat]) * 0.5f;
+                    float dzz = (phi[index - flat] - 2.0f * phi[index] + phi[index + flat]);
+                    float dz2 = dz * dz;
+                    float dxy = (phi[index - xsh - 1] + phi[index + xsh + 1]
+                                 - phi[index - xsh + 1] - phi[index + xsh - 1]) * 0.25f;
+                    float dxz = (phi[index + flat - 1] + phi[index - flat + 1]
+                                 - phi[index + flat + 1] - phi[index - flat - 1]) * 0.25f;
+                    float dyz = (phi[index + flat - xsh] + phi[index - flat + xsh]
+                                 - phi[index + flat + xsh] - phi[index - flat - xsh]) * 0.25f;
+                    curv[index] = (dxx * (dy2 + dz2)
+                                   + dyy * (dx2 + dz2)
+                                   + dzz * (dx2 + dy2)
+                                   - 2.0f * dx * dy * dxy
+                                   - 2.0f * dx * dz * dxz
+                                   - 2.0f * dy * dz * dyz)
+                                  / (dx2 + dy2 + dz2 + eps);
+                }
+#pragma omp parallel for schedule(static)
+        for (int z = 1; z < Z - 1; ++z)
+            for (int y = 1; y < Y - 1; ++y)
+                for (int x = 1; x < X - 1; ++x) {
+                    const int64_t index = z * flat + y * xsh + x;
+                    phi[index] += 0.01f * curv[index];
+                }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Uncertainty: MAX_SRC per label in label order, then UNCERTAINTY_SRC       */
+/* (opencl_kernels.py:54-133), interior voxels; out = u (float).             */
+/* Voxels outside the interior keep the last label's hits                    */
+/* (the kernel writes into that label's buffer, rw_small.py:170-175).        */
+/* ------------------------------------------------------------------------ */
+void orc_uncertainty(const float *hits, int nlabels, int Z, int Y, int X, float *out)
+{
+    const int64_t flat = (int64_t)X * Y, V = flat * Z;
+    memcpy(out, hits + (int64_t)(nlabels - 1) * V, (size_t)V * sizeof(float));
+#pragma omp parallel for schedule(static)
+    for (int z = 1; z < Z - 1; ++z)
+        for (int y = 1; y < Y - 1; ++y)
+            for (int x = 1; x < X - 1; ++x) {
+                const int64_t index = z * flat + (int64_t)y * X + x;
+                float m0 = 0.0f, mid = 0.0f, mx = 0.0f;
+                for (int l = 0; l < nlabels; ++l) {
+                    float tmp = hits[(int64_t)l * V + index];
+                    if (tmp > mx) { m0 = mid; mid = mx; mx = tmp; }
+                    else if (tmp > mid) { m0 = mid; mid = tmp; }
+                    else if (tmp > m0) { m0 = tmp; }
+                }
+                float max2 = mx;
+                if (max2 == 0.0f) max2 = 1.0f;
+                float t = m0 / max2;
+                float t1 = mid / max2;
+                out[index] = 1.0f - (1.0f - t) * (1.0f - t1);
+            }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_expf_array(const float *x, float *y, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) y[i] = orc_expf(x[i]);
+}
+
+void orc_philox_array(const uint32_t *ctr, const uint32_t *key, uint32_t *out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) orc_philox4x32_10(ctr + 4 * i, key + 2 * i, out + 4 * i);
+}

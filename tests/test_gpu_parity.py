@@ -1,0 +1,223 @@
+"""Parity of the HIP path (libbmx_hip through its C ABI) against the CPU oracle.
+
+Integer and byte results (plane flags, seeds, hit counts, labels) must be
+bit-exact; so must the float32 smoothing (both sides evaluate the same IEEE
+operations in the same order, no contraction).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import orc
+import oracle.pipeline as opipe
+from biomedisa_amd import _hip
+from biomedisa_amd.synth import ellipsoids_volume, structures_volume
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _random_labels(shape, rng, every, allaxis, nlab=3, neg_frac=0.01):
+    gt = rng.integers(0, nlab + 1, size=shape).astype(np.uint8)
+    lab = np.zeros_like(gt)
+    lab[1::every] = gt[1::every]
+    if allaxis:
+        lab[:, 2::every] = gt[:, 2::every]
+        lab[:, :, 3::every] = gt[:, :, 3::every]
+    lab[(rng.random(shape) < neg_frac) & (lab > 0)] = oracle.IGN
+    return lab
+
+
+def _gpu_seeds(eng, lab, img, allaxis):
+    dims = lab.shape
+    lb = eng.upload(lab)
+    ib = eng.upload(img)
+    dt = _hip.IMG_U8 if img.dtype == np.uint8 else _hip.IMG_F32
+    fb = eng.plane_flags(lb, dims, allaxis)
+    flags = fb.download((sum(dims),), np.uint8)
+    sb, n = eng.seeds(lb, ib, dt, dims, fb)
+    seeds = sb.download((n,), _hip.SEED_DTYPE) if n else np.zeros(0, _hip.SEED_DTYPE)
+    return flags, seeds, (lb, ib, dt, sb, n)
+
+
+@pytest.mark.parametrize("allaxis", [False, True])
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_plane_flags_and_seeds_bitexact(eng, allaxis, dtype):
+    rng = np.random.default_rng(21 + allaxis)
+    shape = (17, 23, 29)
+    _say("flags/seeds shape=%s allaxis=%s dtype=%s" % (shape, allaxis, np.dtype(dtype).name))
+    lab = _random_labels(shape, rng, 4, allaxis)
+    img = (rng.integers(0, 256, shape).astype(np.uint8) if dtype == np.uint8
+           else (rng.random(shape) * 255).astype(np.float32))
+    flags, seeds, _ = _gpu_seeds(eng, lab, img, allaxis)
+    ofl = orc.plane_flags(lab, allaxis)
+    np.testing.assert_array_equal(flags, np.concatenate(ofl))
+    pos, l, var = orc.seeds(lab, img, ofl)
+    np.testing.assert_array_equal(seeds["pos"], pos)
+    np.testing.assert_array_equal(seeds["label"], l)
+    np.testing.assert_array_equal(seeds["var"].view(np.uint32), var.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", [
+    dict(shape=(20, 22, 24), dtype=np.uint8, nbrw=3, sorw=203, rng="philox", allaxis=False),
+    dict(shape=(19, 21, 18), dtype=np.float32, nbrw=2, sorw=150, rng="philox", allaxis=True),
+    dict(shape=(16, 18, 20), dtype=np.uint8, nbrw=2, sorw=97, rng="reference", allaxis=False),
+    dict(shape=(15, 16, 17), dtype=np.float32, nbrw=2, sorw=64, rng="reference", allaxis=True),
+])
+def test_walk_hits_bitexact(eng, case):
+    rng = np.random.default_rng(31)
+    shape = case["shape"]
+    _say("walk %s" % case)
+    lab = _random_labels(shape, rng, 5, case["allaxis"], nlab=4)
+    base = rng.integers(40, 200, shape).astype(np.float32)
+    img = base.astype(np.uint8) if case["dtype"] == np.uint8 else (base + rng.random(shape)).astype(np.float32)
+    _, seeds, (lb, ib, dt, sb, n) = _gpu_seeds(eng, lab, img, case["allaxis"])
+    nl = 5
+    hits = eng.alloc(nl * lab.size * 4)
+    hits.zero()
+    key = 0x1234_5678_9ABC_DEF0
+    rngc = _hip.RNG_PHILOX if case["rng"] == "philox" else _hip.RNG_REF
+    acc = eng.walk(ib, dt, shape, sb, n, nl, case["nbrw"], case["sorw"], rngc, key, hits)
+    h = hits.download((nl,) + shape, np.uint32)
+    pos, l, var = seeds["pos"].astype(np.int64), seeds["label"].astype(np.uint8), seeds["var"]
+    if case["rng"] == "philox":
+        oh, oacc = orc.philox_walk(img, pos, l, var, nl, case["nbrw"], case["sorw"], key)
+    else:
+        oh, oacc = orc.ref_walk(img, pos, l, var, nl, case["nbrw"], case["sorw"])
+    assert acc == oacc
+    assert int(h.sum(dtype=np.int64)) == acc
+    np.testing.assert_array_equal(h, oh)
+
+
+def test_walk_accumulates_and_handles_empty(eng):
+    shape = (8, 9, 10)
+    img = eng.upload(np.full(shape, 9, np.uint8))
+    hits = eng.alloc(2 * 720 * 4)
+    hits.zero()
+    assert eng.walk(img, _hip.IMG_U8, shape, None, 0, 2, 10, 100, _hip.RNG_PHILOX, 1, hits) == 0
+    assert hits.download((2 * 720,), np.uint32).sum() == 0
+
+
+def test_abi_rejects_bad_arguments(eng):
+    shape = (8, 9, 10)
+    img = eng.upload(np.zeros(shape, np.uint8))
+    hits = eng.alloc(720 * 4)
+    with pytest.raises(_hip.BmxError, match="nlabels"):
+        eng.walk(img, _hip.IMG_U8, shape, None, 0, 0, 1, 1, _hip.RNG_PHILOX, 1, hits)
+    with pytest.raises(_hip.BmxError, match="dims"):
+        eng.walk(img, _hip.IMG_U8, (2, 9, 10), None, 0, 1, 1, 1, _hip.RNG_PHILOX, 1, hits)
+    with pytest.raises(_hip.BmxError, match="rng"):
+        eng.walk(img, _hip.IMG_U8, shape, None, 0, 1, 1, 1, 7, 1, hits)
+
+
+@pytest.mark.parametrize("V", [4 * 1000, 4 * 1000 + 3])
+def test_argmax_bitexact(eng, V):
+    rng = np.random.default_rng(V)
+    nl = 6
+    h = rng.integers(0, 5, (nl, V)).astype(np.uint32)
+    h[:, :7] = 0
+    values = np.array([0, 2, 4, 8, 16, 250], np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(V)
+    eng.argmax(hb, nl, V, values, out)
+    got = out.download((V,), np.uint8)
+    np.testing.assert_array_equal(got, orc.argmax(h.reshape(nl, 1, 1, V), values).ravel())
+
+
+@pytest.mark.parametrize("iters", [1, 2, 5])
+def test_smooth_bitexact(eng, iters):
+    rng = np.random.default_rng(iters)
+    shape = (37, 11, 70)          # x spans two 64-wide tiles, z spans one chunk boundary? no: < 64
+    phi = (rng.random(shape) * 50).astype(np.float32)
+    a = eng.upload(phi)
+    b = eng.alloc(phi.nbytes)
+    eng.smooth(a, b, shape, iters)
+    got = a.download(shape, np.float32)
+    ref = orc.smooth(phi, iters)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_smooth_bitexact_tall(eng):
+    shape = (150, 9, 7)           # z spans three 64-plane chunks
+    phi = (np.random.default_rng(2).random(shape) * 9).astype(np.float32)
+    a = eng.upload(phi)
+    b = eng.alloc(phi.nbytes)
+    eng.smooth(a, b, shape, 3)
+    np.testing.assert_array_equal(a.download(shape, np.float32).view(np.uint32),
+                                  orc.smooth(phi, 3).view(np.uint32))
+
+
+def test_smooth_argmax_and_uncertainty_bitexact(eng):
+    rng = np.random.default_rng(4)
+    shape = (21, 30, 66)
+    nl = 4
+    h = rng.integers(0, 40, (nl,) + shape).astype(np.uint32)
+    values = np.array([0, 3, 5, 7], np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(h[0].size)
+    eng.smooth_argmax(hb, nl, shape, 4, values, out)
+    phi = np.stack([orc.smooth(x.astype(np.float32), 4) for x in h])
+    np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
+    uq = eng.alloc(h[0].size)
+    eng.uncertainty(hb, nl, shape, uq)
+    u = orc.uncertainty(h.astype(np.float32))
+    ref = (u * np.float32(255.0)).astype(np.uint8)
+    ref[0], ref[-1], ref[:, 0], ref[:, -1], ref[:, :, 0], ref[:, :, -1] = 0, 0, 0, 0, 0, 0
+    np.testing.assert_array_equal(uq.download(shape, np.uint8), ref)
+
+
+E2E = [
+    dict(name="xy_u8", kw=dict(nbrw=4, sorw=300, smooth=2, uncertainty=True, return_hits=True)),
+    dict(name="allaxis_u8", allaxis=True, kw=dict(nbrw=3, sorw=250, allaxis=True, smooth=1)),
+    dict(name="xy_u16_float_path", u16=True, kw=dict(nbrw=3, sorw=200)),
+    dict(name="xy_reference_stream", kw=dict(nbrw=2, sorw=150, rng="reference")),
+]
+
+
+@pytest.mark.parametrize("case", E2E, ids=[c["name"] for c in E2E])
+def test_smart_interpolation_matches_oracle(case):
+    from biomedisa_amd import smart_interpolation
+    _say("e2e %s" % case["name"])
+    allaxis = case.get("allaxis", False)
+    if case.get("u16"):
+        vol = structures_volume(shape=(36, 40, 44), seed=5, n_struct=4, radius=(4, 9), every=5,
+                                allaxis=False, dtype=np.uint16, max_value=4095,
+                                mean_range=(400, 3600), sigma=150, bias=200)
+    else:
+        vol = structures_volume(shape=(40, 44, 48), seed=3, n_struct=4, radius=(4, 10),
+                                every=6, allaxis=allaxis)
+    labels = vol.labels.astype(np.int32)
+    labels[0, 0, 0] = -1                     # an ignored voxel
+    res = smart_interpolation(vol.img, labels, **case["kw"])
+    ref = opipe.smart_interpolation(vol.img, labels, **case["kw"])
+    for k in ("regular", "smooth", "uncertainty", "hits"):
+        if k in ref:
+            assert k in res
+            np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+    assert set(res) == {k for k in ref if not k.startswith("_")}
+
+
+def test_adjacent_allaxis_planes_rejected(capsys):
+    from biomedisa_amd import smart_interpolation
+    img = np.full((12, 12, 12), 50, np.uint8)
+    lab = np.zeros((12, 12, 12), np.uint8)
+    lab[4, 2:9, 2:9] = 1
+    lab[5, 2:9, 2:9] = 1
+    assert smart_interpolation(img, lab, nbrw=1, sorw=10, allaxis=True) is None
+    assert "At least one empty slice between labels required." in capsys.readouterr().out
+
+
+def test_config1_shape_dice_vs_reference_stream():
+    """128^3 phantom of the CPU-reference workload, shortened walks: the Philox walker
+    and the reference stream agree to per-label Dice >= 0.98."""
+    from biomedisa_amd import smart_interpolation
+    from biomedisa_amd.helpers import per_label_dice
+    vol = ellipsoids_volume(seed=1)
+    _say("config1 shape, nbrw=10 sorw=1000")
+    a = smart_interpolation(vol.img, vol.labels, nbrw=10, sorw=1000)["regular"]
+    b = smart_interpolation(vol.img, vol.labels, nbrw=10, sorw=1000, rng="reference")["regular"]
+    d = per_label_dice(a, b)
+    assert min(d.values()) >= 0.98, d

@@ -1,0 +1,82 @@
+"""Host-side helpers of the drop-in against the reference's documented behaviour."""
+import numpy as np
+import pytest
+
+from biomedisa_amd import helpers as H
+import oracle.pipeline as opipe
+
+
+def _bm(data, labels, only="all", ignore="none"):
+    bm = H.Biomedisa()
+    bm.data, bm.labelData, bm.only, bm.ignore = data, labels, only, ignore
+    bm.success = True
+    return bm
+
+
+def test_pre_processing_label_set():
+    img = np.zeros((4, 5, 6), np.uint8)
+    lab = np.zeros((4, 5, 6), np.int32)
+    lab[1, 1, 1], lab[1, 2, 2], lab[2, 3, 3], lab[2, 0, 0] = 7, 3, -2, 300
+    bm = H.pre_processing(_bm(img, lab))
+    assert bm.success and list(bm.allLabels) == [0, 3, 7] and bm.nol == 3
+    assert bm.labelData[2, 0, 0] == 0 and bm.labelData[2, 3, 3] == -2
+
+
+def test_pre_processing_only_and_ignore():
+    img = np.zeros((3, 3, 3), np.uint8)
+    lab = np.zeros((3, 3, 3), np.uint8)
+    lab[0, 0, :] = [1, 2, 3]
+    bm = H.pre_processing(_bm(img, lab.copy(), only="1,3"))
+    assert list(bm.allLabels) == [0, 1, 3] and bm.labelData[0, 0, 1] == 0
+    bm = H.pre_processing(_bm(img, lab.copy(), ignore="2"))
+    assert list(bm.allLabels) == [0, 1, 3]
+
+
+def test_pre_processing_errors(capsys):
+    img = np.zeros((3, 3, 3), np.uint8)
+    assert not H.pre_processing(_bm(img, np.zeros((3, 3, 3), np.uint8))).success
+    assert "No labeled slices found." in capsys.readouterr().out
+    assert not H.pre_processing(_bm(img, np.zeros((3, 3), np.uint8))).success
+    assert not H.pre_processing(_bm(img, np.zeros((3, 3, 4), np.uint8))).success
+
+
+@pytest.mark.parametrize("trial", range(5))
+def test_predict_blocksize_matches_slice_loop(trial):
+    rng = np.random.default_rng(trial)
+    shape = tuple(int(v) for v in rng.integers(20, 260, 3))
+    lab = np.zeros(shape, np.int16)
+    for _ in range(3):
+        p = tuple(int(rng.integers(1, s - 1)) for s in shape)
+        lab[p] = rng.integers(1, 5)
+    assert H.predict_blocksize(lab) == opipe._bbox(lab)
+
+
+def test_label_index_volume():
+    lab = np.array([[[0, 5, 9, -1, 5]]], np.int32)
+    idx = H.label_index_volume(lab, np.array([0, 5, 9]))
+    assert idx.tolist() == [[[0, 1, 2, 255, 1]]]
+    lab8 = np.array([[[0, 5, 9]]], np.uint8)
+    assert H.label_index_volume(lab8, np.array([0, 5, 9])).tolist() == [[[0, 1, 2]]]
+
+
+def test_get_labels_and_dice():
+    pre = np.array([0, 1, 2, 2], np.uint8)
+    assert H.get_labels(pre, np.array([0, 4, 9])).tolist() == [0, 4, 9, 9]
+    a = np.array([0, 1, 1, 2, 2])
+    b = np.array([0, 1, 2, 2, 0])
+    assert H.Dice_score(a, b) == pytest.approx(2 * 2 / (4 + 3))
+    assert H.Dice_score(a, b, average_dice=True) == pytest.approx((2 / 3 + 1 / 2) / 2)
+    assert H.per_label_dice(a, b) == {1: pytest.approx(2 / 3), 2: pytest.approx(1 / 2)}
+
+
+def test_adjacent_planes():
+    assert H.adjacent_planes([np.array([0, 1, 1, 0]), np.zeros(3), np.zeros(3)])
+    assert not H.adjacent_planes([np.array([1, 0, 1, 0]), np.array([0, 1]), np.array([1])])
+
+
+def test_normalize_image():
+    i8 = np.array([-128, 0, 127], np.int8)
+    assert H.normalize_image(i8).tolist() == [0, 128, 255]
+    u16 = np.array([10, 20, 30], np.uint16)
+    f = H.normalize_image(u16)
+    assert f.dtype == np.float32 and f.tolist() == [0.0, 127.5, 255.0]
