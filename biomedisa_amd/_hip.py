@@ -15,7 +15,12 @@ LIB_PATH = os.path.join(_HERE, "libbmx_hip.so")
 BMX_OK = 0
 IMG_U8, IMG_F32 = 0, 1
 RNG_PHILOX, RNG_REF = 0, 1
+WALK_AUTO, WALK_DIRECT, WALK_SORTED, WALK_TILED = 0, 1, 2, 3
 LABEL_NONE = 0xFF
+
+
+class WalkOpts(C.Structure):
+    _fields_ = [("algorithm", C.c_int), ("segment_steps", C.c_int), ("brick_log2", C.c_int)]
 
 SEED_DTYPE = np.dtype([("pos", "<u4"), ("label", "<u4"), ("var", "<f4"), ("reserved", "<u4")])
 
@@ -36,7 +41,10 @@ SIGNATURES = {
     "bmx_seeds": (_i, [_p, _p, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),
+    "bmx_walk_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
+                         C.POINTER(C.c_uint64), C.POINTER(WalkOpts)]),
     "bmx_last_walk_ms": (C.c_float, [_p]),
+    "bmx_last_walk_stats": (_i, [_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(_i)]),
     "bmx_argmax": (_i, [_p, _p, _i, _i64, _p, _p]),
     "bmx_smooth": (_i, [_p, _p, _p, _i64, _i64, _i64, _i]),
     "bmx_smooth_argmax": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p]),
@@ -173,17 +181,27 @@ class Engine:
                                 capacity, C.byref(n)))
         return buf, n.value
 
-    def walk(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits):
+    def walk(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits,
+             algorithm=WALK_AUTO, segment_steps=0, brick_log2=0):
+        """Accumulate the walks of `nseeds` seeds into `hits`; returns the accepted moves."""
         Z, Y, X = dims
         acc = C.c_uint64()
-        _check(load().bmx_walk(self.ctx, img.ptr, img_dtype, Z, Y, X,
-                               seeds.ptr if seeds is not None else None, int(nseeds), int(nlabels),
-                               int(nbrw), int(sorw), int(rng), C.c_uint64(int(key) & (2**64 - 1)),
-                               hits.ptr, C.byref(acc)))
+        opts = WalkOpts(int(algorithm), int(segment_steps), int(brick_log2))
+        _check(load().bmx_walk_ex(self.ctx, img.ptr, img_dtype, Z, Y, X,
+                                  seeds.ptr if seeds is not None else None, int(nseeds),
+                                  int(nlabels), int(nbrw), int(sorw), int(rng),
+                                  C.c_uint64(int(key) & (2**64 - 1)), hits.ptr, C.byref(acc),
+                                  C.byref(opts)))
         return acc.value
 
     def last_walk_ms(self):
         return float(load().bmx_last_walk_ms(self.ctx))
+
+    def last_walk_stats(self):
+        """(whole walk phase ms, walker-kernel ms, walker-kernel launches) of the last walk."""
+        a, b, n = C.c_float(), C.c_float(), C.c_int()
+        _check(load().bmx_last_walk_stats(self.ctx, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
 
     def argmax(self, hits, nlabels, V, values, out):
         values = np.ascontiguousarray(values, dtype=np.uint8)

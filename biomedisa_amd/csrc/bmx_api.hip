@@ -58,6 +58,14 @@ int bmx_close(bmx_ctx *ctx)
     BMX_HIP(hipSetDevice(ctx->device));
     BMX_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->d_block_counts) BMX_HIP(hipFree(ctx->d_block_counts));
+    if (ctx->d_rec) BMX_HIP(hipFree(ctx->d_rec));
+    if (ctx->d_state) BMX_HIP(hipFree(ctx->d_state));
+    if (ctx->d_keys) BMX_HIP(hipFree(ctx->d_keys));
+    if (ctx->d_buckets) BMX_HIP(hipFree(ctx->d_buckets));
+    if (ctx->seg_ev) {
+        for (int k = 0; k < 2 * 1024; ++k) BMX_HIP(hipEventDestroy(ctx->seg_ev[k]));
+        free(ctx->seg_ev);
+    }
     BMX_HIP(hipFree(ctx->d_counter));
     BMX_HIP(hipEventDestroy(ctx->ev0));
     BMX_HIP(hipEventDestroy(ctx->ev1));

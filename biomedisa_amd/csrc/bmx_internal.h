@@ -22,6 +22,17 @@ struct bmx_ctx {
     uint64_t *d_counter;     // 8 x uint64 scratch counters (accepted moves, seed counts)
     int64_t *d_block_counts; // seed compaction scratch
     int64_t block_counts_cap;
+    void *d_rec;             // packed neighbour records of the walker
+    size_t rec_cap;
+    void *d_state;           // sorted walker: two walker-state arrays
+    size_t state_cap;
+    void *d_keys;            // sorted / tiled walker: per-key counts / cursors
+    size_t keys_cap;
+    void *d_buckets;         // tiled walker: non-empty (key, offset, count) lists
+    size_t buckets_cap;
+    hipEvent_t *seg_ev;      // sorted walker: event pairs around each walk launch
+    float last_walk_kernel_ms;
+    int last_walk_launches;
 };
 
 // Error reporting ----------------------------------------------------------
@@ -44,6 +55,12 @@ int bmx_set_error(int code, const char *fmt, ...);
 #define BMX_LAUNCH_CHECK() BMX_HIP(hipGetLastError())
 
 int bmx_set_device(bmx_ctx *ctx);
+
+// Tile-owner walker (bmx_tile.hip), called by bmx_walk_ex.
+int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
+                   const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
+                   uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc, hipEvent_t *ev,
+                   int max_timed, int *launches);
 
 // Device arithmetic -----------------------------------------------------------
 
@@ -77,7 +94,6 @@ __device__ __forceinline__ float bmx_expf(float x)
     const float LOG2E = 1.44269502925872802734375f;
     const float LN2_HI = 0.693145751953125f;
     const float LN2_LO = 1.428606765330187045e-06f;
-    if (x > 88.0f) return __builtin_inff();
     const float t = x * LOG2E;
     const float n = __builtin_rintf(t);
     float r = __builtin_fmaf(-n, LN2_HI, x);
@@ -92,7 +108,8 @@ __device__ __forceinline__ float bmx_expf(float x)
     p = __builtin_fmaf(p, r, 1.0f);
     const int32_t ni = (int32_t)n;
     const float res = __uint_as_float(__float_as_uint(p) + ((uint32_t)ni << 23));
-    return (n < -125.0f) ? 0.0f : res;
+    const float under = (n < -125.0f) ? 0.0f : res;
+    return (x > 88.0f) ? __builtin_inff() : under;
 }
 
 // Transition weight of the reference (pyopencl_small.py:187-190):

@@ -1,0 +1,330 @@
+// bmx_tile.hip -- tile-owner walker: hits privatised in LDS.
+//
+// Same transition law and Philox stream as the other walkers (bmx_walk.hip);
+// only the order of work and where the increments land differ, so the hit
+// counts are identical.
+//
+// Why: a device-scope atomic on this GPU is performed past the XCD's L2 (the
+// L2s of the 8 XCDs are not coherent with one another), so one global atomic
+// per step caps the walker at the fabric's atomic rate whatever the locality.
+// Here a workgroup owns one (core tile, label) bucket: a C^3 core tile plus a
+// halo of H voxels, i.e. an R^3 region (R = C + 2H).  It stages the region's
+// image (with one more voxel of halo for the neighbour reads) and R^3 hit
+// counters in LDS and runs every walker of the bucket - walkers whose current
+// voxel lies in the core - until the walker leaves the region, finishes its
+// walk, or has made `cap` steps in this round.  Steps taken outside the region
+// (inside the Philox block during which the walker left) read the image from
+// global memory and add their hit with a global atomic.  At the end the
+// workgroup adds each non-zero LDS counter to the hit volume with one global
+// atomic.  Between rounds the walkers are counting-sorted by (core tile,
+// label) again.  A walker pauses only at a multiple of 4 steps, so every
+// Philox block is used whole within one round.
+#include "bmx_walk_common.h"
+
+namespace {
+
+constexpr int kTileThreads = 1024;
+
+template <typename T> struct TileCfg;
+template <> struct TileCfg<uint8_t> {
+    static constexpr int C = 16, H = 6;
+};
+template <> struct TileCfg<float> {
+    static constexpr int C = 16, H = 4;
+};
+
+template <typename T> constexpr int tile_R() { return TileCfg<T>::C + 2 * TileCfg<T>::H; }
+template <typename T> constexpr size_t tile_img_bytes()
+{
+    return ((size_t)(tile_R<T>() + 2) * (tile_R<T>() + 2) * (tile_R<T>() + 2) * sizeof(T) + 15) & ~(size_t)15;
+}
+template <typename T> constexpr size_t tile_lds_bytes()
+{
+    return tile_img_bytes<T>() + (size_t)tile_R<T>() * tile_R<T>() * tile_R<T>() * sizeof(uint32_t);
+}
+
+struct TileGeom {
+    uint32_t ntz, nty, ntx, nlabels;
+};
+
+template <int C>
+__device__ __forceinline__ uint32_t tile_key(uint32_t pos, uint32_t label, const WalkGeom &g,
+                                             const TileGeom &tg)
+{
+    const uint32_t z = pos / g.sz;
+    const uint32_t r = pos - z * g.sz;
+    const uint32_t y = r / g.sy;
+    const uint32_t x = r - y * g.sy;
+    return (((z / C) * tg.nty + y / C) * tg.ntx + x / C) * tg.nlabels + label;
+}
+
+// Walker state: {pos, seed index, walk | label << 16, step}.
+__global__ __launch_bounds__(256) void k_tile_init(const bmx_seed *__restrict__ seeds, int64_t nseeds,
+                                                   int nbrw, uint4 *__restrict__ st)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseeds * nbrw) return;
+    const int64_t s = i / nbrw;
+    const bmx_seed sd = seeds[s];
+    st[i] = make_uint4(sd.pos, (uint32_t)s, (uint32_t)(i - s * nbrw) | (sd.label << 16), 0u);
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_tile_hist(const uint4 *__restrict__ st, int64_t n, WalkGeom g,
+                                                   TileGeom tg, int sorw, uint32_t *counts)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool live = false;
+    uint32_t key = 0;
+    if (i < n) {
+        const uint4 w4 = st[i];
+        live = (int)w4.w < sorw;
+        key = tile_key<C>(w4.x, w4.z >> 16, g, tg);
+    }
+    (void)agg_atomic_inc(counts, key, live);
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_tile_scatter(const uint4 *__restrict__ in, int64_t n,
+                                                      uint4 *__restrict__ out, uint32_t *cursor,
+                                                      WalkGeom g, TileGeom tg, int sorw)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    uint4 w4 = make_uint4(0, 0, 0, 0);
+    bool live = false;
+    uint32_t key = 0;
+    if (i < n) {
+        w4 = in[i];
+        live = (int)w4.w < sorw;
+        key = tile_key<C>(w4.x, w4.z >> 16, g, tg);
+    }
+    const uint32_t dst = agg_atomic_inc(cursor, key, live);
+    if (live) out[dst] = w4;
+}
+
+// Non-empty buckets of the exclusive offsets off[0..nkeys] -> (key, offset, count).
+__global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict__ off, int64_t nkeys,
+                                                     uint32_t *bkey, uint32_t *boff, uint32_t *bcnt,
+                                                     uint32_t *nb)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t c = 0, o = 0;
+    if (k < nkeys) {
+        o = off[k];
+        c = off[k + 1] - o;
+    }
+    const uint32_t idx = agg_atomic_inc(nb, 0u, c > 0);
+    if (c > 0) {
+        bkey[idx] = (uint32_t)k;
+        boff[idx] = o;
+        bcnt[idx] = c;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTileThreads) void k_tile_walk(
+    const T *__restrict__ img, WalkGeom g, TileGeom tg, const bmx_seed *__restrict__ seeds,
+    uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
+    const uint32_t *__restrict__ bcnt, int sorw, int cap, uint32_t k0, uint32_t k1,
+    uint32_t *__restrict__ hits, unsigned long long *accepted)
+{
+    constexpr int C = TileCfg<T>::C, H = TileCfg<T>::H, R = C + 2 * H, RI = R + 2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *simg = (T *)smem;
+    uint32_t *shits = (uint32_t *)(smem + tile_img_bytes<T>());
+    __shared__ uint32_t next;
+
+    const uint32_t key = bkey[blockIdx.x];
+    const uint32_t off = boff[blockIdx.x], cnt = bcnt[blockIdx.x];
+    const uint32_t label = key % tg.nlabels;
+    const uint32_t tile = key / tg.nlabels;
+    const int tx = (int)(tile % tg.ntx), ty = (int)((tile / tg.ntx) % tg.nty),
+              tz = (int)(tile / (tg.ntx * tg.nty));
+    const int oz = tz * C - H, oy = ty * C - H, ox = tx * C - H;
+
+    for (int k = threadIdx.x; k < RI * RI * RI; k += kTileThreads) {
+        const int lz = k / (RI * RI), ly = (k / RI) % RI, lx = k % RI;
+        const int gz = min(max(oz - 1 + lz, 0), g.Z - 1);
+        const int gy = min(max(oy - 1 + ly, 0), g.Y - 1);
+        const int gx = min(max(ox - 1 + lx, 0), g.X - 1);
+        simg[k] = img[(int64_t)gz * g.sz + (int64_t)gy * g.sy + gx];
+    }
+    for (int k = threadIdx.x; k < R * R * R; k += kTileThreads) shits[k] = 0u;
+    if (threadIdx.x == 0) next = 0u;
+    __syncthreads();
+
+    uint32_t *__restrict__ hg = hits + (int64_t)label * g.V;
+    uint64_t acc = 0;
+    bool have = false;
+    uint32_t j = 0, pos = 0, p0 = 0, seed = 0, wl = 0;
+    int step = 0, z = 0, y = 0, x = 0, taken = 0;
+    float B = 0.0f, div1 = 0.0f;
+    while (true) {
+        if (!have) {
+            j = atomicAdd(&next, 1u);
+            if (j >= cnt) break;
+            const uint4 w4 = st[off + j];
+            pos = w4.x;
+            seed = w4.y;
+            wl = w4.z;
+            step = (int)w4.w;
+            const bmx_seed sd = seeds[seed];
+            p0 = sd.pos;
+            B = bmx_ld(img, (int64_t)p0);
+            div1 = 1.0f / (2.0f * sd.var);
+            z = (int)(pos / g.sz);
+            const uint32_t r = pos - (uint32_t)z * g.sz;
+            y = (int)(r / g.sy);
+            x = (int)(r - (uint32_t)y * g.sy);
+            taken = 0;
+            have = true;
+        }
+        const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
+        const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (step < sorw) {
+                const int lz = z - oz, ly = y - oy, lx = x - ox;
+                const bool in = (unsigned)lz < (unsigned)R && (unsigned)ly < (unsigned)R &&
+                                (unsigned)lx < (unsigned)R;
+                float a[6];
+                if (in) {
+                    const int li = ((lz + 1) * RI + (ly + 1)) * RI + (lx + 1);
+                    a[0] = (float)simg[li + RI * RI];
+                    a[1] = (float)simg[li - RI * RI];
+                    a[2] = (float)simg[li + RI];
+                    a[3] = (float)simg[li - RI];
+                    a[4] = (float)simg[li + 1];
+                    a[5] = (float)simg[li - 1];
+                } else {
+                    a[0] = bmx_ld(img, (int64_t)pos + g.sz);
+                    a[1] = bmx_ld(img, (int64_t)pos - g.sz);
+                    a[2] = bmx_ld(img, (int64_t)pos + g.sy);
+                    a[3] = bmx_ld(img, (int64_t)pos - g.sy);
+                    a[4] = bmx_ld(img, (int64_t)pos + 1);
+                    a[5] = bmx_ld(img, (int64_t)pos - 1);
+                }
+                const int d = choose(bmx_weight(B, a[0], div1), bmx_weight(B, a[1], div1),
+                                     bmx_weight(B, a[2], div1), bmx_weight(B, a[3], div1),
+                                     bmx_weight(B, a[4], div1), bmx_weight(B, a[5], div1),
+                                     u01(rr[q]));
+                const int dz = (d == 0) - (d == 1), dy = (d == 2) - (d == 3), dx = (d == 4) - (d == 5);
+                const int zn = z + dz, yn = y + dy, xn = x + dx;
+                if (zn > 0 && zn < g.Z - 1 && yn > 0 && yn < g.Y - 1 && xn > 0 && xn < g.X - 1) {
+                    z = zn; y = yn; x = xn;
+                    pos += step_delta(d, g.sz, g.sy);
+                    ++acc;
+                    const int mz = z - oz, my = y - oy, mx = x - ox;
+                    if ((unsigned)mz < (unsigned)R && (unsigned)my < (unsigned)R && (unsigned)mx < (unsigned)R)
+                        atomicAdd(&shits[(mz * R + my) * R + mx], 1u);
+                    else
+                        atomicAdd(&hg[pos], 1u);
+                }
+                ++step;
+            }
+        }
+        taken += 4;
+        const bool inside = (unsigned)(z - oz) < (unsigned)R && (unsigned)(y - oy) < (unsigned)R &&
+                            (unsigned)(x - ox) < (unsigned)R;
+        if (step >= sorw || !inside || taken >= cap) {
+            st[off + j] = make_uint4(pos, seed, wl, (uint32_t)step);
+            have = false;
+        }
+    }
+    wave_add_u64(accepted, acc);
+    __syncthreads();
+    for (int k = threadIdx.x; k < R * R * R; k += kTileThreads) {
+        const uint32_t c = shits[k];
+        if (c) {
+            const int lz = k / (R * R), ly = (k / R) % R, lx = k % R;
+            const int64_t gi = (int64_t)(oz + lz) * g.sz + (int64_t)(oy + ly) * g.sy + (ox + lx);
+            atomicAdd(&hg[gi], c);
+        }
+    }
+}
+
+template <typename T>
+int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *seeds, int64_t nseeds,
+                 int nlabels, int nbrw, int sorw, int cap, uint32_t k0, uint32_t k1, uint32_t *hits,
+                 unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches)
+{
+    constexpr int C = TileCfg<T>::C;
+    const int64_t n0 = nseeds * (int64_t)nbrw;
+    BMX_CHECK(n0 < ((int64_t)1 << 32), "bmx_walk: %lld walkers exceed 2^32", (long long)n0);
+    BMX_CHECK(nbrw <= 65535, "bmx_walk: tiled walker needs nbrw <= 65535");
+    TileGeom tg;
+    tg.ntz = (uint32_t)((g.Z + C - 1) / C);
+    tg.nty = (uint32_t)((g.Y + C - 1) / C);
+    tg.ntx = (uint32_t)((g.X + C - 1) / C);
+    tg.nlabels = (uint32_t)nlabels;
+    const int64_t nkeys = (int64_t)tg.ntz * tg.nty * tg.ntx * nlabels;
+    BMX_CHECK(nkeys < ((int64_t)1 << 31), "bmx_walk: too many tiles");
+    int rc = grow(ctx, &ctx->d_state, &ctx->state_cap, (size_t)n0 * 2 * sizeof(uint4));
+    if (rc) return rc;
+    rc = grow(ctx, &ctx->d_keys, &ctx->keys_cap, (size_t)(nkeys + 1) * sizeof(uint32_t));
+    if (rc) return rc;
+    rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap, (size_t)nkeys * 3 * sizeof(uint32_t));
+    if (rc) return rc;
+    const size_t lds = tile_lds_bytes<T>();
+    // Raise the dynamic-LDS limit where the runtime wants it; an error here is
+    // left to the launch itself to report.
+    if (hipFuncSetAttribute((const void *)k_tile_walk<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+        (void)hipGetLastError();
+    uint4 *A = (uint4 *)ctx->d_state, *Bst = A + n0;
+    uint32_t *counts = (uint32_t *)ctx->d_keys;
+    uint32_t *bkey = (uint32_t *)ctx->d_buckets, *boff = bkey + nkeys, *bcnt = boff + nkeys;
+    uint32_t *d_nb = (uint32_t *)(ctx->d_counter + 1);
+    k_tile_init<<<bmx_blocks(n0, 256), 256, 0, ctx->stream>>>(seeds, nseeds, nbrw, A);
+    BMX_LAUNCH_CHECK();
+    int64_t n = n0;
+    int round = 0;
+    while (n > 0) {
+        BMX_HIP(hipMemsetAsync(counts, 0, (size_t)(nkeys + 1) * sizeof(uint32_t), ctx->stream));
+        BMX_HIP(hipMemsetAsync(d_nb, 0, sizeof(uint32_t), ctx->stream));
+        k_tile_hist<C><<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(A, n, g, tg, sorw, counts);
+        BMX_LAUNCH_CHECK();
+        k_scan_u32<<<1, 1024, 0, ctx->stream>>>(counts, nkeys + 1);
+        BMX_LAUNCH_CHECK();
+        k_bucket_list<<<bmx_blocks(nkeys, 256), 256, 0, ctx->stream>>>(counts, nkeys, bkey, boff, bcnt,
+                                                                       d_nb);
+        BMX_LAUNCH_CHECK();
+        k_tile_scatter<C><<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(A, n, Bst, counts, g, tg, sorw);
+        BMX_LAUNCH_CHECK();
+        uint32_t host[2] = {0, 0};
+        BMX_HIP(hipMemcpyAsync(&host[0], counts + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        BMX_HIP(hipMemcpyAsync(&host[1], d_nb, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        BMX_HIP(hipStreamSynchronize(ctx->stream));
+        const int64_t live = host[0];
+        const uint32_t nb = host[1];
+        if (live == 0) break;
+        if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round], ctx->stream));
+        k_tile_walk<T><<<nb, kTileThreads, lds, ctx->stream>>>(img, g, tg, seeds, Bst, bkey, boff, bcnt,
+                                                              sorw, cap, k0, k1, hits, acc);
+        BMX_LAUNCH_CHECK();
+        if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round + 1], ctx->stream));
+        uint4 *t = A; A = Bst; Bst = t;
+        n = live;
+        ++round;
+        BMX_CHECK(round <= sorw / 4 + 2, "bmx_walk: tiled walker did not converge");
+    }
+    *launches = round < max_timed ? round : max_timed;
+    return BMX_OK;
+}
+
+}  // namespace
+
+// Entry from bmx_walk_ex (bmx_walk.hip).
+int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
+                   const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
+                   uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc, hipEvent_t *ev,
+                   int max_timed, int *launches)
+{
+    WalkGeom g{Z, Y, X, (uint32_t)X, (uint32_t)X * (uint32_t)Y, (int64_t)Z * Y * X};
+    if (img_dtype == BMX_IMG_U8)
+        return walk_tiled_t<uint8_t>(ctx, (const uint8_t *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap,
+                                     k0, k1, hits, acc, ev, max_timed, launches);
+    return walk_tiled_t<float>(ctx, (const float *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap, k0,
+                               k1, hits, acc, ev, max_timed, launches);
+}

@@ -99,9 +99,33 @@ int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y,
              const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw,
              int rng, uint64_t key, uint32_t *hits, uint64_t *accepted);
 
-/* Milliseconds of the last bmx_walk kernel, measured with HIP events on the
- * context stream. */
+/* Walker algorithms (identical hit counts; they differ only in speed). */
+#define BMX_WALK_AUTO 0   /* tiled for >= 2^16 walkers (Philox), direct otherwise */
+#define BMX_WALK_DIRECT 1 /* one launch, each thread walks its whole walk */
+#define BMX_WALK_SORTED 2 /* lockstep segments, walkers re-sorted by brick between them */
+#define BMX_WALK_TILED 3  /* tile-owner rounds, hit counters privatised in LDS */
+
+typedef struct bmx_walk_opts {
+    int algorithm;     /* BMX_WALK_* */
+    int segment_steps; /* sorted: steps per launch; tiled: max steps per walker per round;
+                          multiple of 4 (0 = 128 sorted, 256 tiled) */
+    int brick_log2;    /* sorted: log2 of the sort-brick side in voxels (0 = 3) */
+} bmx_walk_opts;
+
+/* bmx_walk with a choice of algorithm (opts may be NULL = automatic).  No
+ * counterpart in the reference, which has one kernel per backend. */
+int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,
+                const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int rng,
+                uint64_t key, uint32_t *hits, uint64_t *accepted, const bmx_walk_opts *opts);
+
+/* Milliseconds of the last bmx_walk (whole walk phase: record packing
+ * excluded, sorting included), measured with HIP events on the context
+ * stream. */
 float bmx_last_walk_ms(bmx_ctx *ctx);
+
+/* Timing of the last bmx_walk: whole phase, sum of the walker-kernel
+ * launches alone, number of walker-kernel launches. */
+int bmx_last_walk_stats(bmx_ctx *ctx, float *phase_ms, float *kernel_ms, int *launches);
 
 /* Per-voxel argmax over labels, first maximum wins, mapped through
  * values[nlabels] (label index -> label value): replaces np.argmax +

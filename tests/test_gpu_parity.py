@@ -64,6 +64,9 @@ def test_plane_flags_and_seeds_bitexact(eng, allaxis, dtype):
 @pytest.mark.parametrize("case", [
     dict(shape=(20, 22, 24), dtype=np.uint8, nbrw=3, sorw=203, rng="philox", allaxis=False),
     dict(shape=(19, 21, 18), dtype=np.float32, nbrw=2, sorw=150, rng="philox", allaxis=True),
+    dict(shape=(22, 19, 26), dtype=np.uint8, nbrw=10, sorw=101, rng="philox", allaxis=True),
+    dict(shape=(18, 20, 17), dtype=np.uint8, nbrw=8, sorw=64, rng="philox", allaxis=False),
+    dict(shape=(17, 18, 19), dtype=np.float32, nbrw=10, sorw=55, rng="philox", allaxis=False),
     dict(shape=(16, 18, 20), dtype=np.uint8, nbrw=2, sorw=97, rng="reference", allaxis=False),
     dict(shape=(15, 16, 17), dtype=np.float32, nbrw=2, sorw=64, rng="reference", allaxis=True),
 ])
@@ -90,6 +93,59 @@ def test_walk_hits_bitexact(eng, case):
     assert acc == oacc
     assert int(h.sum(dtype=np.int64)) == acc
     np.testing.assert_array_equal(h, oh)
+
+
+@pytest.mark.parametrize("seg,blog", [(8, 1), (12, 2), (128, 3)])
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_sorted_walker_bitexact(eng, seg, blog, dtype):
+    """Lockstep segments + brick sort give the oracle's hits exactly (sorw not a
+    multiple of the segment, so the last segment is partial)."""
+    rng = np.random.default_rng(41)
+    shape = (23, 27, 25)
+    _say("sorted walker seg=%d brick_log2=%d dtype=%s" % (seg, blog, np.dtype(dtype).name))
+    lab = _random_labels(shape, rng, 4, True, nlab=3)
+    base = rng.integers(30, 220, shape).astype(np.float32)
+    img = base.astype(np.uint8) if dtype == np.uint8 else (base + rng.random(shape)).astype(np.float32)
+    _, seeds, (lb, ib, dt, sb, n) = _gpu_seeds(eng, lab, img, True)
+    nl, nbrw, sorw, key = 4, 3, 101, 77
+    hits = eng.alloc(nl * lab.size * 4)
+    hits.zero()
+    acc = eng.walk(ib, dt, shape, sb, n, nl, nbrw, sorw, _hip.RNG_PHILOX, key, hits,
+                   algorithm=_hip.WALK_SORTED, segment_steps=seg, brick_log2=blog)
+    h = hits.download((nl,) + shape, np.uint32)
+    oh, oacc = orc.philox_walk(img, seeds["pos"].astype(np.int64), seeds["label"].astype(np.uint8),
+                               seeds["var"], nl, nbrw, sorw, key)
+    assert acc == oacc
+    np.testing.assert_array_equal(h, oh)
+    phase, kern, launches = eng.last_walk_stats()
+    assert launches == (sorw + seg - 1) // seg and 0 < kern <= phase
+
+
+@pytest.mark.parametrize("cap", [8, 1024])
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+@pytest.mark.parametrize("shape", [(23, 27, 25), (40, 37, 45)])
+def test_tiled_walker_bitexact(eng, cap, dtype, shape):
+    """Tile-owner walker (LDS-privatised hits, rounds of bucket sorting) gives the
+    oracle's hits exactly: several tiles, partial edge tiles, walkers leaving and
+    re-entering regions, a round cap that forces many rounds, sorw % 4 != 0."""
+    rng = np.random.default_rng(43)
+    _say("tiled walker cap=%d dtype=%s shape=%s" % (cap, np.dtype(dtype).name, shape))
+    lab = _random_labels(shape, rng, 4, True, nlab=3)
+    base = rng.integers(30, 220, shape).astype(np.float32)
+    img = base.astype(np.uint8) if dtype == np.uint8 else (base + rng.random(shape)).astype(np.float32)
+    _, seeds, (lb, ib, dt, sb, n) = _gpu_seeds(eng, lab, img, True)
+    nl, nbrw, sorw, key = 4, 3, 203, 78
+    hits = eng.alloc(nl * lab.size * 4)
+    hits.zero()
+    acc = eng.walk(ib, dt, shape, sb, n, nl, nbrw, sorw, _hip.RNG_PHILOX, key, hits,
+                   algorithm=_hip.WALK_TILED, segment_steps=cap)
+    h = hits.download((nl,) + shape, np.uint32)
+    oh, oacc = orc.philox_walk(img, seeds["pos"].astype(np.int64), seeds["label"].astype(np.uint8),
+                               seeds["var"], nl, nbrw, sorw, key)
+    assert acc == oacc
+    np.testing.assert_array_equal(h, oh)
+    phase, kern, launches = eng.last_walk_stats()
+    assert launches >= 1 and 0 < kern <= phase
 
 
 def test_walk_accumulates_and_handles_empty(eng):
@@ -171,6 +227,7 @@ def test_smooth_argmax_and_uncertainty_bitexact(eng):
 
 E2E = [
     dict(name="xy_u8", kw=dict(nbrw=4, sorw=300, smooth=2, uncertainty=True, return_hits=True)),
+    dict(name="xy_u8_table_walker", kw=dict(nbrw=10, sorw=200, smooth=1)),
     dict(name="allaxis_u8", allaxis=True, kw=dict(nbrw=3, sorw=250, allaxis=True, smooth=1)),
     dict(name="xy_u16_float_path", u16=True, kw=dict(nbrw=3, sorw=200)),
     dict(name="xy_reference_stream", kw=dict(nbrw=2, sorw=150, rng="reference")),
