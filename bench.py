@@ -107,7 +107,7 @@ def main():
         out = step()
         log("warmup %d: %.2fs (seeds %d)" % (w, time.perf_counter() - t, out["nseeds"]))
     eng.sync()
-    walk_ms, accepted, nseeds, per = [], 0, 0, []
+    walk_ms, walk_launches, accepted, nseeds, per = [], [], 0, 0, []
     t0 = time.perf_counter()
     for k in range(args.steps):
         tm = {}
@@ -115,9 +115,11 @@ def main():
         out = step(tm)
         per.append(time.perf_counter() - ts)
         walk_ms.append(tm["walk_kernel_ms"])
+        walk_launches.append(tm["walk_launches"])
         accepted, nseeds = out["accepted"], out["nseeds"]
-        log("step %d: %.2fs  (seeds %.2fs, walk kernel %.0f ms, merge+smooth %.2fs)"
-            % (k, per[-1], tm["seeds_s"], tm["walk_kernel_ms"], tm["merge_s"]))
+        log("step %d: %.2fs  (seeds %.2fs, walk phase %.0f ms = %d walker launches %.0f ms, "
+            "merge+smooth %.2fs)" % (k, per[-1], tm["seeds_s"], tm["walk_phase_ms"],
+                                     tm["walk_launches"], tm["walk_kernel_ms"], tm["merge_s"]))
     eng.sync()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -132,7 +134,8 @@ def main():
 
     steps_total = nseeds * args.nbrw * args.sorw
     walk_bytes = steps_total * 6 * 1 + accepted * 8
-    kernel_s = float(np.mean(walk_ms)) / 1000.0
+    kernel_s = float(np.mean(walk_ms)) / 1000.0          # summed walker launches of one step
+    launches = int(np.mean(walk_launches))
     achieved = walk_bytes / kernel_s / 1e9
     line = {
         "metric": "segmented voxels/s at 512^3, 8 labels (plain HIP walker)",
@@ -152,11 +155,13 @@ def main():
                                                                             args.sorw, args.smooth),
                    "method": "plain", "seeds": nseeds, "walker_steps": steps_total,
                    "accepted_moves": accepted, "walk_kernel_ms": float(np.mean(walk_ms)),
+                   "walk_launches": launches,
+                   "walk_launch_avg_ms": float(np.mean(walk_ms)) / max(launches, 1),
                    "min_dice_vs_ground_truth": min(d_gt.values())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_walk_philox<uint8_t>",
-                     "algorithmic_bytes_per_launch": walk_bytes},
+                     "kernel": "k_tile_walk<uint8_t>",
+                     "algorithmic_bytes_per_launch": walk_bytes / max(launches, 1)},
     }
     if not args.no_cpu_baseline:
         seeds_buf, n = eng.seeds(dv.lab, dv.img, dv.img_dtype, dv.dims,

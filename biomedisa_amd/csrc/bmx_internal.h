@@ -86,9 +86,11 @@ __device__ __forceinline__ bmx_u4 bmx_philox(uint32_t c0, uint32_t c1, uint32_t 
 
 // exp for binary32: Cody-Waite reduction x = n ln2 + r (ln2 split in a hi part
 // with 16 significant bits and a lo part, applied with fused multiply-adds),
-// e^r by its degree-7 Taylor polynomial (Horner, fused), 2^n added to the
-// exponent field.  Below 2^-125 the result flushes to +0.  Max error on
-// [-86.5, 0] is under 1 ulp against a float64 exp.
+// e^r by the degree-6 polynomial 1 + r + c2 r^2 + ... + c6 r^6 (near-minimax
+// relative-error fit on |r| <= ln2/2, Horner with fused multiply-adds), 2^n
+// applied with ldexp (IEEE gradual underflow below FLT_MIN).  Max error on
+// [-86.5, 0] is under 1.1 ulp against a float64 exp.  Valid for x <= 88 and
+// x >= -1000 (weights: x >= -65025/2); the CPU oracle clamps outside.
 __device__ __forceinline__ float bmx_expf(float x)
 {
     const float LOG2E = 1.44269502925872802734375f;
@@ -98,18 +100,14 @@ __device__ __forceinline__ float bmx_expf(float x)
     const float n = __builtin_rintf(t);
     float r = __builtin_fmaf(-n, LN2_HI, x);
     r = __builtin_fmaf(-n, LN2_LO, r);
-    float p = 1.98412701e-4f;
-    p = __builtin_fmaf(p, r, 1.38888892e-3f);
-    p = __builtin_fmaf(p, r, 8.33333377e-3f);
-    p = __builtin_fmaf(p, r, 4.16666679e-2f);
-    p = __builtin_fmaf(p, r, 1.66666672e-1f);
-    p = __builtin_fmaf(p, r, 0.5f);
+    float p = 0.00138436537f;
+    p = __builtin_fmaf(p, r, 0.0083741555f);
+    p = __builtin_fmaf(p, r, 0.0416680016f);
+    p = __builtin_fmaf(p, r, 0.166664317f);
+    p = __builtin_fmaf(p, r, 0.49999994f);
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
-    const int32_t ni = (int32_t)n;
-    const float res = __uint_as_float(__float_as_uint(p) + ((uint32_t)ni << 23));
-    const float under = (n < -125.0f) ? 0.0f : res;
-    return (x > 88.0f) ? __builtin_inff() : under;
+    return __builtin_amdgcn_ldexpf(p, (int)n);
 }
 
 // Transition weight of the reference (pyopencl_small.py:187-190):

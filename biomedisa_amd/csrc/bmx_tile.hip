@@ -40,7 +40,8 @@ template <typename T> constexpr size_t tile_img_bytes()
 }
 template <typename T> constexpr size_t tile_lds_bytes()
 {
-    return tile_img_bytes<T>() + (size_t)tile_R<T>() * tile_R<T>() * tile_R<T>() * sizeof(uint32_t);
+    return tile_img_bytes<T>() +
+           (size_t)(tile_R<T>() + 2) * (tile_R<T>() + 2) * (tile_R<T>() + 2) * sizeof(uint32_t);
 }
 
 struct TileGeom {
@@ -121,6 +122,78 @@ __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict_
     }
 }
 
+// Weight exp(-(B-A)^2 * div1) of the reference (pyopencl_small.py:187-190).
+__device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
+{
+    return bmx_weight(B, A, div1);
+}
+
+// One Philox block (4 steps) of one walker.  Local coordinates (lz,ly,lx)
+// address the LDS tile, whose voxel (1,1,1) is the region origin; the region
+// proper is [1,R]^3 and the tile adds one voxel on each side for neighbour
+// reads.  li is the walker's index in the tile.  EDGE: the tile may reach the
+// volume boundary, so moves are checked against [lmin, lmax] per axis (the
+// local range of interior voxels); otherwise every move is interior.
+template <typename T, bool EDGE>
+__device__ __forceinline__ void tile_block(const T *__restrict__ img, const WalkGeom &g, const T *simg,
+                                           uint32_t *shits, uint32_t *__restrict__ hg, int sorw,
+                                           const uint32_t rr[4], float B, float div1, int &step,
+                                           uint32_t &pos, int &li, int &lz, int &ly, int &lx,
+                                           int lmin0, int lmin1, int lmin2, int lmax0, int lmax1,
+                                           int lmax2, uint64_t &acc)
+{
+    constexpr int R = TileCfg<T>::C + 2 * TileCfg<T>::H, RI = R + 2;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (step < sorw) {
+            const bool in = max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) <
+                            (unsigned)R;
+            float a0, a1, a2, a3, a4, a5;
+            if (in) {
+                a0 = (float)simg[li + RI * RI];
+                a1 = (float)simg[li - RI * RI];
+                a2 = (float)simg[li + RI];
+                a3 = (float)simg[li - RI];
+                a4 = (float)simg[li + 1];
+                a5 = (float)simg[li - 1];
+            } else {
+                a0 = bmx_ld(img, (int64_t)pos + g.sz);
+                a1 = bmx_ld(img, (int64_t)pos - g.sz);
+                a2 = bmx_ld(img, (int64_t)pos + g.sy);
+                a3 = bmx_ld(img, (int64_t)pos - g.sy);
+                a4 = bmx_ld(img, (int64_t)pos + 1);
+                a5 = bmx_ld(img, (int64_t)pos - 1);
+            }
+            const int d = choose(weight_nonpos(B, a0, div1), weight_nonpos(B, a1, div1),
+                                 weight_nonpos(B, a2, div1), weight_nonpos(B, a3, div1),
+                                 weight_nonpos(B, a4, div1), weight_nonpos(B, a5, div1), u01(rr[q]));
+            const int axis = d >> 1;
+            const int sgn = 1 - ((d & 1) << 1);
+            bool ok = true;
+            if (EDGE) {
+                const int c = (axis == 0 ? lz : (axis == 1 ? ly : lx)) + sgn;
+                const int lo = axis == 0 ? lmin0 : (axis == 1 ? lmin1 : lmin2);
+                const int hi = axis == 0 ? lmax0 : (axis == 1 ? lmax1 : lmax2);
+                ok = (c >= lo) & (c <= hi);
+            }
+            if (ok) {
+                lz += axis == 0 ? sgn : 0;
+                ly += axis == 1 ? sgn : 0;
+                lx += axis == 2 ? sgn : 0;
+                const int lmag = axis == 0 ? RI * RI : (axis == 1 ? RI : 1);
+                li += sgn * lmag;
+                pos += step_delta(d, g.sz, g.sy);
+                ++acc;
+                if (max(max((unsigned)lz, (unsigned)ly), (unsigned)lx) < (unsigned)RI)
+                    atomicAdd(&shits[li], 1u);
+                else
+                    atomicAdd(&hg[pos], 1u);
+            }
+            ++step;
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     const T *__restrict__ img, WalkGeom g, TileGeom tg, const bmx_seed *__restrict__ seeds,
@@ -129,6 +202,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     uint32_t *__restrict__ hits, unsigned long long *accepted)
 {
     constexpr int C = TileCfg<T>::C, H = TileCfg<T>::H, R = C + 2 * H, RI = R + 2;
+    constexpr int NT = RI * RI * RI;
     extern __shared__ __align__(16) unsigned char smem[];
     T *simg = (T *)smem;
     uint32_t *shits = (uint32_t *)(smem + tile_img_bytes<T>());
@@ -140,24 +214,33 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     const uint32_t tile = key / tg.nlabels;
     const int tx = (int)(tile % tg.ntx), ty = (int)((tile / tg.ntx) % tg.nty),
               tz = (int)(tile / (tg.ntx * tg.nty));
-    const int oz = tz * C - H, oy = ty * C - H, ox = tx * C - H;
+    // global coordinates of tile voxel (0,0,0)
+    const int bz = tz * C - H - 1, by = ty * C - H - 1, bx = tx * C - H - 1;
 
-    for (int k = threadIdx.x; k < RI * RI * RI; k += kTileThreads) {
+    for (int k = threadIdx.x; k < NT; k += kTileThreads) {
         const int lz = k / (RI * RI), ly = (k / RI) % RI, lx = k % RI;
-        const int gz = min(max(oz - 1 + lz, 0), g.Z - 1);
-        const int gy = min(max(oy - 1 + ly, 0), g.Y - 1);
-        const int gx = min(max(ox - 1 + lx, 0), g.X - 1);
+        const int gz = min(max(bz + lz, 0), g.Z - 1);
+        const int gy = min(max(by + ly, 0), g.Y - 1);
+        const int gx = min(max(bx + lx, 0), g.X - 1);
         simg[k] = img[(int64_t)gz * g.sz + (int64_t)gy * g.sy + gx];
+        shits[k] = 0u;
     }
-    for (int k = threadIdx.x; k < R * R * R; k += kTileThreads) shits[k] = 0u;
     if (threadIdx.x == 0) next = 0u;
     __syncthreads();
+
+    // Interior voxels are 1..dim-2; in tile coordinates 1-b..dim-2-b.  A walker
+    // is never more than 3 steps outside the region, so a tile whose region
+    // grown by 4 stays inside the interior needs no move checks.
+    const int lmin0 = 1 - bz, lmin1 = 1 - by, lmin2 = 1 - bx;
+    const int lmax0 = g.Z - 2 - bz, lmax1 = g.Y - 2 - by, lmax2 = g.X - 2 - bx;
+    const bool edge = (lmin0 > -3) | (lmin1 > -3) | (lmin2 > -3) | (lmax0 < R + 4) |
+                      (lmax1 < R + 4) | (lmax2 < R + 4);
 
     uint32_t *__restrict__ hg = hits + (int64_t)label * g.V;
     uint64_t acc = 0;
     bool have = false;
     uint32_t j = 0, pos = 0, p0 = 0, seed = 0, wl = 0;
-    int step = 0, z = 0, y = 0, x = 0, taken = 0;
+    int step = 0, lz = 0, ly = 0, lx = 0, li = 0, taken = 0;
     float B = 0.0f, div1 = 0.0f;
     while (true) {
         if (!have) {
@@ -172,60 +255,28 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
             p0 = sd.pos;
             B = bmx_ld(img, (int64_t)p0);
             div1 = 1.0f / (2.0f * sd.var);
-            z = (int)(pos / g.sz);
+            const int z = (int)(pos / g.sz);
             const uint32_t r = pos - (uint32_t)z * g.sz;
-            y = (int)(r / g.sy);
-            x = (int)(r - (uint32_t)y * g.sy);
+            const int y = (int)(r / g.sy);
+            const int x = (int)(r - (uint32_t)y * g.sy);
+            lz = z - bz;
+            ly = y - by;
+            lx = x - bx;
+            li = (lz * RI + ly) * RI + lx;
             taken = 0;
             have = true;
         }
         const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
         const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (step < sorw) {
-                const int lz = z - oz, ly = y - oy, lx = x - ox;
-                const bool in = (unsigned)lz < (unsigned)R && (unsigned)ly < (unsigned)R &&
-                                (unsigned)lx < (unsigned)R;
-                float a[6];
-                if (in) {
-                    const int li = ((lz + 1) * RI + (ly + 1)) * RI + (lx + 1);
-                    a[0] = (float)simg[li + RI * RI];
-                    a[1] = (float)simg[li - RI * RI];
-                    a[2] = (float)simg[li + RI];
-                    a[3] = (float)simg[li - RI];
-                    a[4] = (float)simg[li + 1];
-                    a[5] = (float)simg[li - 1];
-                } else {
-                    a[0] = bmx_ld(img, (int64_t)pos + g.sz);
-                    a[1] = bmx_ld(img, (int64_t)pos - g.sz);
-                    a[2] = bmx_ld(img, (int64_t)pos + g.sy);
-                    a[3] = bmx_ld(img, (int64_t)pos - g.sy);
-                    a[4] = bmx_ld(img, (int64_t)pos + 1);
-                    a[5] = bmx_ld(img, (int64_t)pos - 1);
-                }
-                const int d = choose(bmx_weight(B, a[0], div1), bmx_weight(B, a[1], div1),
-                                     bmx_weight(B, a[2], div1), bmx_weight(B, a[3], div1),
-                                     bmx_weight(B, a[4], div1), bmx_weight(B, a[5], div1),
-                                     u01(rr[q]));
-                const int dz = (d == 0) - (d == 1), dy = (d == 2) - (d == 3), dx = (d == 4) - (d == 5);
-                const int zn = z + dz, yn = y + dy, xn = x + dx;
-                if (zn > 0 && zn < g.Z - 1 && yn > 0 && yn < g.Y - 1 && xn > 0 && xn < g.X - 1) {
-                    z = zn; y = yn; x = xn;
-                    pos += step_delta(d, g.sz, g.sy);
-                    ++acc;
-                    const int mz = z - oz, my = y - oy, mx = x - ox;
-                    if ((unsigned)mz < (unsigned)R && (unsigned)my < (unsigned)R && (unsigned)mx < (unsigned)R)
-                        atomicAdd(&shits[(mz * R + my) * R + mx], 1u);
-                    else
-                        atomicAdd(&hg[pos], 1u);
-                }
-                ++step;
-            }
-        }
+        if (edge)
+            tile_block<T, true>(img, g, simg, shits, hg, sorw, rr, B, div1, step, pos, li, lz, ly, lx,
+                                lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, acc);
+        else
+            tile_block<T, false>(img, g, simg, shits, hg, sorw, rr, B, div1, step, pos, li, lz, ly, lx,
+                                 lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, acc);
         taken += 4;
-        const bool inside = (unsigned)(z - oz) < (unsigned)R && (unsigned)(y - oy) < (unsigned)R &&
-                            (unsigned)(x - ox) < (unsigned)R;
+        const bool inside =
+            max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
         if (step >= sorw || !inside || taken >= cap) {
             st[off + j] = make_uint4(pos, seed, wl, (uint32_t)step);
             have = false;
@@ -233,11 +284,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     }
     wave_add_u64(accepted, acc);
     __syncthreads();
-    for (int k = threadIdx.x; k < R * R * R; k += kTileThreads) {
+    for (int k = threadIdx.x; k < NT; k += kTileThreads) {
         const uint32_t c = shits[k];
         if (c) {
-            const int lz = k / (R * R), ly = (k / R) % R, lx = k % R;
-            const int64_t gi = (int64_t)(oz + lz) * g.sz + (int64_t)(oy + ly) * g.sy + (ox + lx);
+            const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
+            const int64_t gi = (int64_t)(bz + lz2) * g.sz + (int64_t)(by + ly2) * g.sy + (bx + lx2);
             atomicAdd(&hg[gi], c);
         }
     }

@@ -504,7 +504,6 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
     BMX_CHECK(algo == BMX_WALK_AUTO || algo == BMX_WALK_DIRECT || algo == BMX_WALK_SORTED ||
                   algo == BMX_WALK_TILED,
               "bmx_walk: bad algorithm %d", algo);
-    if (algo == BMX_WALK_TILED && !(opts && opts->segment_steps > 0)) seg = 256;
     BMX_CHECK(seg % 4 == 0, "bmx_walk: segment_steps %d must be a multiple of 4", seg);
     BMX_CHECK(blog >= 1 && blog <= 6, "bmx_walk: brick_log2 %d not in [1,6]", blog);
     WalkGeom g{(int)Z, (int)Y, (int)X, (uint32_t)X, (uint32_t)(X * Y), Z * Y * X};
@@ -514,6 +513,7 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
     const int64_t nwalkers = nseeds * (int64_t)nbrw;
     if (algo == BMX_WALK_AUTO) algo = nwalkers >= ((int64_t)1 << 16) ? BMX_WALK_TILED : BMX_WALK_DIRECT;
     if (rng != BMX_RNG_PHILOX) algo = BMX_WALK_DIRECT;
+    if (algo == BMX_WALK_TILED && !(opts && opts->segment_steps > 0)) seg = 256;
     if (run && rng == BMX_RNG_PHILOX && algo != BMX_WALK_TILED) {
         const size_t rec_bytes = (size_t)g.V * (img_dtype == BMX_IMG_U8 ? 8 : sizeof(RecF32));
         rc = ensure_records(ctx, rec_bytes);

@@ -100,7 +100,9 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
     else:
         hits.free()
     seeds.free()
-    t.update(seeds_s=t1 - t0, walk_s=t2 - t1, merge_s=t3 - t2, walk_kernel_ms=eng.last_walk_ms())
+    phase_ms, kernel_ms, launches = eng.last_walk_stats()
+    t.update(seeds_s=t1 - t0, walk_s=t2 - t1, merge_s=t3 - t2, walk_phase_ms=phase_ms,
+             walk_kernel_ms=kernel_ms, walk_launches=launches)
     return out
 
 

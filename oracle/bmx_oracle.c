@@ -16,8 +16,8 @@
  *   orc_philox_walk -- the same transition law driven by a counter-based
  *                      Philox4x32-10 stream per (seed voxel, walk).  This is
  *                      the bit-exact target of the HIP walker.
- * Both use orc_expf (Cody-Waite reduction + degree-7 Taylor polynomial with
- * fused multiply-adds) in place of the vendor exp the reference JIT links
+ * Both use orc_expf (Cody-Waite reduction + degree-6 near-minimax polynomial
+ * with fused multiply-adds, ldexp scaling) in place of the vendor exp the reference JIT links
  * (pyopencl_small.py:187-190), so that CPU and GPU agree bit for bit.
  *
  * Arithmetic contract (build with -ffp-contract=off): every float operation
@@ -64,35 +64,31 @@ void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
 
 /* ------------------------------------------------------------------------ */
 /* exp for binary32.  Cody & Waite argument reduction x = n*ln2 + r with a   */
-/* two-part ln2 (hi part exact in 16 bits), fused with fmaf; e^r by its      */
-/* Taylor series to degree 7 (|r| <= ln2/2, truncation < 6e-9 relative);     */
-/* 2^n applied to the exponent field.  Results below 2^-125 flush to +0.     */
+/* two-part ln2 (hi part exact in 16 bits), fused with fmaf; e^r by a degree-6 */
+/* polynomial 1 + r + c2 r^2 + ... + c6 r^6 whose coefficients are a near-     */
+/* minimax (relative error) fit on |r| <= ln2/2 (fit error 1.7e-8 with the     */
+/* float32 coefficients); 2^n applied with ldexpf, so results below FLT_MIN    */
+/* underflow gradually as IEEE scaling does.                                   */
 /* ------------------------------------------------------------------------ */
-static inline float orc_bits_to_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
-static inline uint32_t orc_float_to_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
-
 float orc_expf(float x)
 {
     const float LOG2E = 1.44269502925872802734375f;     /* 0x3fb8aa3b */
     const float LN2_HI = 0.693145751953125f;            /* 0x3f317200 */
     const float LN2_LO = 1.428606765330187045e-06f;     /* 0x35bfbe8e */
     if (x > 88.0f) return INFINITY;
+    if (x < -1000.0f) return 0.0f;
     float t = x * LOG2E;
     float n = rintf(t);
-    if (n < -125.0f) return 0.0f;
     float r = fmaf(-n, LN2_HI, x);
     r = fmaf(-n, LN2_LO, r);
-    float p = 1.98412701e-4f;                   /* 1/5040 */
-    p = fmaf(p, r, 1.38888892e-3f);             /* 1/720  */
-    p = fmaf(p, r, 8.33333377e-3f);             /* 1/120  */
-    p = fmaf(p, r, 4.16666679e-2f);             /* 1/24   */
-    p = fmaf(p, r, 1.66666672e-1f);             /* 1/6    */
-    p = fmaf(p, r, 0.5f);
+    float p = 0.00138436537f;                   /* 0x3ab57398 */
+    p = fmaf(p, r, 0.0083741555f);              /* 0x3c0933c1 */
+    p = fmaf(p, r, 0.0416680016f);              /* 0x3d2aac11 */
+    p = fmaf(p, r, 0.166664317f);               /* 0x3e2aaa0d */
+    p = fmaf(p, r, 0.49999994f);                /* 0x3efffffe */
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
-    int32_t ni = (int32_t)n;
-    uint32_t bits = orc_float_to_bits(p) + ((uint32_t)ni << 23);
-    return orc_bits_to_float(bits);
+    return ldexpf(p, (int)n);
 }
 
 /* weight(B, A, div1) of pyopencl_small.py:187-190: exp(-tmp*tmp*div1). */

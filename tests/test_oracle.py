@@ -41,7 +41,7 @@ def test_philox_planted_mistake_fails():
 
 
 # --- exp -----------------------------------------------------------------------
-def test_expf_within_one_ulp():
+def test_expf_within_1_1_ulp():
     rng = np.random.default_rng(0)
     x = np.concatenate([rng.uniform(-86.5, 0, 2_000_000), np.linspace(-86.5, 0, 100_001),
                         -np.arange(0, 65026, dtype=np.float64) / 2.0]).astype(np.float32)
@@ -49,14 +49,15 @@ def test_expf_within_one_ulp():
     y = orc.expf(x).astype(np.float64)
     r = np.exp(x.astype(np.float64))
     ulp = np.spacing(r.astype(np.float32)).astype(np.float64)
-    assert np.max(np.abs(y - r) / ulp) <= 1.0
+    assert np.max(np.abs(y - r) / ulp) <= 1.1
 
 
 def test_expf_special_values():
-    y = orc.expf(np.array([0.0, -0.0, -88.0, -1000.0, -87.5], np.float32))
+    y = orc.expf(np.array([0.0, -0.0, -1000.0, -104.0, -88.0], np.float32))
     assert y[0] == 1.0 and y[1] == 1.0
-    assert np.all(y[2:] == 0.0)          # flushes below 2^-125.5
-    assert orc.expf(np.array([-86.0], np.float32))[0] > 0.0
+    assert y[2] == 0.0 and y[3] == 0.0               # below the smallest denormal
+    assert 0.0 < y[4] < np.finfo(np.float32).tiny     # gradual underflow (ldexp)
+    np.testing.assert_allclose(y[4], np.exp(-88.0), rtol=1e-3)
 
 
 # --- labelled planes -------------------------------------------------------------
