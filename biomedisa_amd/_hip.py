@@ -39,6 +39,7 @@ SIGNATURES = {
     "bmx_zero": (_i, [_p, _p, C.c_size_t]),
     "bmx_plane_flags": (_i, [_p, _p, _i64, _i64, _i64, _i, _p]),
     "bmx_seeds": (_i, [_p, _p, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
+    "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),
     "bmx_walk_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
@@ -49,6 +50,20 @@ SIGNATURES = {
     "bmx_smooth": (_i, [_p, _p, _p, _i64, _i64, _i64, _i]),
     "bmx_smooth_argmax": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p]),
     "bmx_uncertainty": (_i, [_p, _p, _i, _i64, _i64, _i64, _p]),
+    "bmx_conv3d_fwd": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _i, _i]),
+    "bmx_conv3d_dgrad": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _i, _p]),
+    "bmx_conv3d_wgrad": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _i, _p, _p]),
+    "bmx_pool2": (_i, [_p, _p, _i, _i, _i, _i, _i, _p]),
+    "bmx_pool2_bwd": (_i, [_p, _p, _i, _i, _i, _i, _i, _p]),
+    "bmx_up2": (_i, [_p, _p, _i, _i, _i, _i, _i, _p]),
+    "bmx_up2_bwd": (_i, [_p, _p, _i, _i, _i, _i, _i, _p]),
+    "bmx_concat": (_i, [_p, _p, _i, _p, _i, _i64, _p]),
+    "bmx_split": (_i, [_p, _p, _i64, _p, _i, _p, _i, _i]),
+    "bmx_relu_mask": (_i, [_p, _p, _p, _i64]),
+    "bmx_head": (_i, [_p, _p, _i64, _i, _p, _p, _p, _p, _p, C.c_float, _p, _p, _p, _p]),
+    "bmx_adam": (_i, [_p, _p, _p, _p, _p, _p, _i64, C.c_float, C.c_float, C.c_float, C.c_float, _i,
+                      C.c_float]),
+    "bmx_f32_to_bf16": (_i, [_p, _p, _p, _i64]),
 }
 
 _lib = None
@@ -181,6 +196,14 @@ class Engine:
                                 capacity, C.byref(n)))
         return buf, n.value
 
+    def select_seeds(self, seeds, nseeds, mask, out=None):
+        """Seeds whose voxel has mask != 0 -> (buffer, count)."""
+        buf = out if out is not None else DeviceBuffer(self, max(1, nseeds) * SEED_DTYPE.itemsize)
+        n = C.c_int64()
+        _check(load().bmx_select_seeds(self.ctx, seeds.ptr, int(nseeds), mask.ptr, buf.ptr,
+                                       C.byref(n)))
+        return buf, n.value
+
     def walk(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits,
              algorithm=WALK_AUTO, segment_steps=0, brick_log2=0):
         """Accumulate the walks of `nseeds` seeds into `hits`; returns the accepted moves."""
@@ -220,6 +243,16 @@ class Engine:
     def uncertainty(self, hits, nlabels, dims, out):
         Z, Y, X = dims
         _check(load().bmx_uncertainty(self.ctx, hits.ptr, int(nlabels), Z, Y, X, out.ptr))
+
+
+def ptr(buf):
+    """Device pointer of a DeviceBuffer, or None."""
+    return None if buf is None else buf.ptr
+
+
+def call(name, *args):
+    """Call a bmx_* entry point with a context as first argument; raise on error."""
+    _check(getattr(load(), name)(*args))
 
 
 _engines = {}

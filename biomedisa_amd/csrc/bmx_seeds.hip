@@ -313,3 +313,53 @@ int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Seed selection by a voxel mask (surrogate-guided walking): keeps the seeds
+// whose voxel has mask != 0.  Output order is unspecified (the walk does not
+// depend on seed order).
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ __launch_bounds__(256) void k_select_seeds(const bmx_seed *__restrict__ in, int64_t n,
+                                                      const uint8_t *__restrict__ mask,
+                                                      bmx_seed *__restrict__ out,
+                                                      unsigned long long *count)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bmx_seed s{};
+    bool keep = false;
+    if (i < n) {
+        s = in[i];
+        keep = mask[s.pos] != 0;
+    }
+    const uint64_t ballot = __ballot(keep);
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == 0 && ballot) base = atomicAdd(count, (unsigned long long)__popcll(ballot));
+    base = __shfl(base, 0, 64);
+    if (keep) out[base + __popcll(ballot & ((1ull << lane) - 1ull))] = s;
+}
+
+}  // namespace
+
+extern "C" int bmx_select_seeds(bmx_ctx *ctx, const bmx_seed *seeds, int64_t nseeds, const uint8_t *mask,
+                                bmx_seed *out, int64_t *nout)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(nout != nullptr, "bmx_select_seeds: nout is null");
+    BMX_CHECK(nseeds == 0 || (seeds && mask && out), "bmx_select_seeds: null pointer");
+    BMX_CHECK(nseeds >= 0, "bmx_select_seeds: negative nseeds");
+    BMX_HIP(hipMemsetAsync(ctx->d_counter + 2, 0, sizeof(uint64_t), ctx->stream));
+    if (nseeds > 0) {
+        k_select_seeds<<<bmx_blocks(nseeds, 256), 256, 0, ctx->stream>>>(
+            seeds, nseeds, mask, out, (unsigned long long *)(ctx->d_counter + 2));
+        BMX_LAUNCH_CHECK();
+    }
+    uint64_t c = 0;
+    BMX_HIP(hipMemcpyAsync(&c, ctx->d_counter + 2, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    BMX_HIP(hipStreamSynchronize(ctx->stream));
+    *nout = (int64_t)c;
+    return BMX_OK;
+}

@@ -89,6 +89,13 @@ int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype,
               int64_t Z, int64_t Y, int64_t X, const uint8_t *flags,
               bmx_seed *seeds, int64_t capacity, int64_t *nseeds);
 
+/* Keep the seeds whose voxel has mask[pos] != 0 (uint8 mask over the
+ * volume), in unspecified order; *nout receives the count.  out must hold
+ * nseeds entries.  No counterpart in the reference: used by the
+ * surrogate-guided mode to walk only seeds that can reach uncertain voxels. */
+int bmx_select_seeds(bmx_ctx *ctx, const bmx_seed *seeds, int64_t nseeds, const uint8_t *mask,
+                     bmx_seed *out, int64_t *nout);
+
 /* Random walks of every seed, all labels in one launch: replaces the
  * per-label loop fill -> randomWalk -> D2H of pyopencl_small.py:128-134 and
  * the randomWalk kernel :192-314.  hits (device, uint32 [nlabels][V]) is
@@ -148,6 +155,41 @@ int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z
  * three largest hit counts, interior voxels; 0 elsewhere. */
 int bmx_uncertainty(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y,
                     int64_t X, uint8_t *out);
+
+/* ------------------------------------------------------------------------ */
+/* Learned proposal (surrogate) kernels.  No counterpart in the reference:   */
+/* a 3-D conv encoder-decoder that labels confident voxels so the walker can */
+/* skip seeds far from uncertain ones.  Activations: bf16 (uint16 bits),     */
+/* channels-last [N][D][H][W][C]; conv weights bf16 [Co][27][Ci]; FP32       */
+/* accumulation on MFMA; channel counts multiples of 16.                     */
+/* ------------------------------------------------------------------------ */
+int bmx_conv3d_fwd(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int Ci,
+                   const uint16_t *w, const float *b, uint16_t *y, int Co, int relu);
+/* dx = conv^T(dy, w) [* (xact > 0) if xact]; wt_scratch: Co*27*Ci bf16. */
+int bmx_conv3d_dgrad(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, int Co,
+                     const uint16_t *w, const uint16_t *xact, uint16_t *dx, int Ci,
+                     uint16_t *wt_scratch);
+/* dw += sum_v dy x(shifted), db += sum_v dy (FP32, zero them first). */
+int bmx_conv3d_wgrad(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int Ci,
+                     const uint16_t *dy, int Co, float *dw, float *db);
+int bmx_pool2(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int C, uint16_t *y);
+int bmx_pool2_bwd(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, int C, uint16_t *dx);
+int bmx_up2(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int C, uint16_t *y);
+int bmx_up2_bwd(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, int C, uint16_t *dx);
+int bmx_concat(bmx_ctx *ctx, const uint16_t *a, int Ca, const uint16_t *b, int Cb, int64_t nvox,
+               uint16_t *y);
+int bmx_split(bmx_ctx *ctx, const uint16_t *y, int64_t nvox, uint16_t *a, int Ca, uint16_t *b,
+              int Cb, int accumulate_a);
+int bmx_relu_mask(bmx_ctx *ctx, uint16_t *g, const uint16_t *a, int64_t n);
+/* 1x1x1 softmax head over 16 channels, L <= 16 outputs.  With target (soft
+ * labels [v][L]) also dx, dW, db (accumulated) and the summed cross-entropy
+ * (accumulated into *loss); vmask (may be NULL) excludes voxels. */
+int bmx_head(bmx_ctx *ctx, const uint16_t *x, int64_t nvox, int L, const float *W, const float *b,
+             float *prob, const float *target, const uint8_t *vmask, float scale, uint16_t *dx,
+             float *dW, float *db, float *loss);
+int bmx_adam(bmx_ctx *ctx, float *p, const float *g, float *m, float *v, uint16_t *pbf, int64_t n,
+             float lr, float beta1, float beta2, float eps, int step, float weight_decay);
+int bmx_f32_to_bf16(bmx_ctx *ctx, const float *a, uint16_t *b, int64_t n);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,161 @@
+"""Learned-proposal kernels (bmx_net.hip) against the float64 oracle (oracle/net.py).
+
+Inputs are bf16 values; the GPU accumulates in FP32 and rounds outputs to
+bf16, so outputs agree to bf16 resolution (relative 2^-8) and the FP32 weight
+gradients to 1e-3 of their scale.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from biomedisa_amd import _hip
+from oracle import net as O
+
+pytestmark = pytest.mark.gpu
+
+
+def up_bf16(eng, a):
+    return eng.upload(O.to_bf16_bits(a))
+
+
+def down_bf16(buf, shape):
+    return O.from_bf16_bits(buf.download(shape, np.uint16)).astype(np.float64)
+
+
+def _close_bf16(got, ref):
+    scale = np.abs(ref).max() + 1e-12
+    err = np.abs(got - ref)
+    assert np.all(err <= np.abs(ref) * 2.0 ** -7 + scale * 2.0 ** -12), (err.max(), scale)
+
+
+SHAPES = [(2, 4, 6, 8), (1, 3, 5, 7)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("ci,co", [(16, 16), (32, 32), (16, 64), (48, 16)])
+@pytest.mark.parametrize("relu", [0, 1])
+def test_conv_fwd(eng, shape, ci, co, relu):
+    rng = np.random.default_rng(ci * 100 + co + relu)
+    x = O.bf16_round(rng.standard_normal(shape + (ci,)))
+    w = O.bf16_round(rng.standard_normal((co, 27, ci)) * 0.1)
+    b = rng.standard_normal(co).astype(np.float32)
+    xb, wb, bb = up_bf16(eng, x), up_bf16(eng, w), eng.upload(b)
+    yb = eng.alloc(int(np.prod(shape)) * co * 2)
+    _hip.call("bmx_conv3d_fwd", eng.ctx, xb.ptr, *shape, ci, wb.ptr, bb.ptr, yb.ptr, co, relu)
+    ref = O.conv3d(x, w, b.astype(np.float64), relu=bool(relu))
+    _close_bf16(down_bf16(yb, shape + (co,)), ref)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("ci,co", [(16, 16), (32, 64), (64, 32), (96, 32)])
+@pytest.mark.parametrize("masked", [False, True])
+def test_conv_dgrad(eng, shape, ci, co, masked):
+    rng = np.random.default_rng(ci + co * 7 + masked)
+    dy = O.bf16_round(rng.standard_normal(shape + (co,)))
+    w = O.bf16_round(rng.standard_normal((co, 27, ci)) * 0.1)
+    xact = O.bf16_round(np.maximum(rng.standard_normal(shape + (ci,)), 0))
+    dyb, wb, xb = up_bf16(eng, dy), up_bf16(eng, w), up_bf16(eng, xact)
+    dxb = eng.alloc(int(np.prod(shape)) * ci * 2)
+    scratch = eng.alloc(co * 27 * ci * 2)
+    _hip.call("bmx_conv3d_dgrad", eng.ctx, dyb.ptr, *shape, co, wb.ptr, xb.ptr if masked else None,
+              dxb.ptr, ci, scratch.ptr)
+    ref = O.conv3d_dgrad(dy, w)
+    if masked:
+        ref = ref * (xact > 0)
+    _close_bf16(down_bf16(dxb, shape + (ci,)), ref)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("ci,co", [(16, 16), (32, 64), (48, 16)])
+def test_conv_wgrad(eng, shape, ci, co):
+    rng = np.random.default_rng(ci * 3 + co)
+    x = O.bf16_round(rng.standard_normal(shape + (ci,)))
+    dy = O.bf16_round(rng.standard_normal(shape + (co,)))
+    xb, dyb = up_bf16(eng, x), up_bf16(eng, dy)
+    dwb = eng.alloc(co * 27 * ci * 4)
+    dbb = eng.alloc(co * 4)
+    dwb.zero()
+    dbb.zero()
+    _hip.call("bmx_conv3d_wgrad", eng.ctx, xb.ptr, *shape, ci, dyb.ptr, co, dwb.ptr, dbb.ptr)
+    dw, db = O.conv3d_wgrad(x, dy)
+    got = dwb.download((co, 27, ci), np.float32)
+    np.testing.assert_allclose(got, dw, rtol=1e-4, atol=1e-4 * np.abs(dw).max())
+    np.testing.assert_allclose(dbb.download((co,), np.float32), db, rtol=1e-4, atol=1e-3)
+
+
+def test_pool_up_concat_split(eng):
+    rng = np.random.default_rng(9)
+    shape = (2, 4, 6, 8)
+    x = O.bf16_round(rng.standard_normal(shape + (16,)))
+    xb = up_bf16(eng, x)
+    pb = eng.alloc(x.size // 8 * 2)
+    _hip.call("bmx_pool2", eng.ctx, xb.ptr, *shape, 16, pb.ptr)
+    _close_bf16(down_bf16(pb, (2, 2, 3, 4, 16)), O.pool2(x))
+    g = O.bf16_round(rng.standard_normal((2, 2, 3, 4, 16)))
+    gb = up_bf16(eng, g)
+    gx = eng.alloc(x.size * 2)
+    _hip.call("bmx_pool2_bwd", eng.ctx, gb.ptr, *shape, 16, gx.ptr)
+    _close_bf16(down_bf16(gx, shape + (16,)), O.pool2_bwd(g))
+    ub = eng.alloc(x.size * 2)
+    _hip.call("bmx_up2", eng.ctx, gb.ptr, 2, 2, 3, 4, 16, ub.ptr)
+    _close_bf16(down_bf16(ub, shape + (16,)), O.up2(g))
+    db = eng.alloc(g.size * 2)
+    _hip.call("bmx_up2_bwd", eng.ctx, xb.ptr, 2, 2, 3, 4, 16, db.ptr)
+    _close_bf16(down_bf16(db, g.shape), O.up2_bwd(x))
+    nv = int(np.prod(shape))
+    y = O.bf16_round(rng.standard_normal(shape + (32,)))
+    yb = up_bf16(eng, y)
+    cb = eng.alloc(nv * 48 * 2)
+    _hip.call("bmx_concat", eng.ctx, xb.ptr, 16, yb.ptr, 32, nv, cb.ptr)
+    cat = down_bf16(cb, shape + (48,))
+    np.testing.assert_array_equal(cat, np.concatenate([x, y], -1))
+    a2, b2 = eng.alloc(nv * 16 * 2), eng.alloc(nv * 32 * 2)
+    _hip.call("bmx_split", eng.ctx, cb.ptr, nv, a2.ptr, 16, b2.ptr, 32, 0)
+    np.testing.assert_array_equal(down_bf16(a2, shape + (16,)), x)
+    np.testing.assert_array_equal(down_bf16(b2, shape + (32,)), y)
+
+
+def test_head_softmax_ce(eng):
+    rng = np.random.default_rng(10)
+    nv, L = 300, 9
+    x = O.bf16_round(rng.standard_normal((nv, 16)))
+    W = (rng.standard_normal((L, 16)) * 0.3).astype(np.float32)
+    b = rng.standard_normal(L).astype(np.float32)
+    t = rng.random((nv, L)); t /= t.sum(-1, keepdims=True)
+    vm = (rng.random(nv) > 0.2).astype(np.uint8)
+    xb, Wb, bb = up_bf16(eng, x), eng.upload(W), eng.upload(b)
+    tb, mb = eng.upload(t.astype(np.float32)), eng.upload(vm)
+    pb, dxb = eng.alloc(nv * L * 4), eng.alloc(nv * 16 * 2)
+    dWb, dbb, lb = eng.alloc(L * 16 * 4), eng.alloc(L * 4), eng.alloc(4)
+    for bf in (dWb, dbb, lb):
+        bf.zero()
+    scale = 1.0 / nv
+    _hip.call("bmx_head", eng.ctx, xb.ptr, nv, L, Wb.ptr, bb.ptr, pb.ptr, tb.ptr, mb.ptr, C.c_float(scale),
+              dxb.ptr, dWb.ptr, dbb.ptr, lb.ptr)
+    p, loss, dx, dW, db = O.head(x, W.astype(np.float64), b.astype(np.float64),
+                                 t.astype(np.float32).astype(np.float64), vm, scale)
+    np.testing.assert_allclose(pb.download((nv, L), np.float32), p, rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(lb.download((1,), np.float32)[0], loss, rtol=1e-4)
+    _close_bf16(down_bf16(dxb, (nv, 16)), dx)
+    np.testing.assert_allclose(dWb.download((L, 16), np.float32), dW, rtol=1e-3, atol=1e-6)
+    np.testing.assert_allclose(dbb.download((L,), np.float32), db, rtol=1e-3, atol=1e-6)
+
+
+def test_adam_step(eng):
+    rng = np.random.default_rng(11)
+    n = 1000
+    p = rng.standard_normal(n).astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    pb, gb = eng.upload(p), eng.upload(g)
+    mb, vb = eng.alloc(n * 4), eng.alloc(n * 4)
+    mb.zero(); vb.zero()
+    bf = eng.alloc(n * 2)
+    _hip.call("bmx_adam", eng.ctx, pb.ptr, gb.ptr, mb.ptr, vb.ptr, bf.ptr, n, C.c_float(1e-2),
+              C.c_float(0.9), C.c_float(0.999), C.c_float(1e-8), 1, C.c_float(0.0))
+    m = 0.1 * g
+    v = 0.001 * g * g
+    ref = p - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    got = pb.download((n,), np.float32)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(bf.download((n,), np.uint16), O.to_bf16_bits(got))
