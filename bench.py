@@ -7,11 +7,15 @@ smoothing with smooth=100.  A step is one full device pipeline over that
 volume: plane detection, seed extraction, the walk, the argmax merge and the
 smoothing + merge, with image and label volume already resident in HBM.
 
-Prints one JSON line.  `roofline` is the walker kernel's algorithmic bytes per
-launch (6 neighbour reads per step + one 8-byte hit RMW per accepted move)
-over its HIP-event time; `cpu_baseline` times the C restatement of the
-reference kernel (oracle ref_walk, float MRG stream) on a bounded sample of the
-same seeds and extrapolates.
+Both pipelines run on the same resident volume: the plain walker over every
+seed, and the surrogate-guided pipeline (network proposal, uncertain set,
+walks of the seeds near it, merge).  `value` is the guided throughput; the
+config carries the plain throughput, the speedup and the per-label Dice of
+guided against plain.  `roofline` is the walker kernel of the plain pipeline:
+algorithmic bytes (6 neighbour reads per step + one 8-byte hit RMW per
+accepted move) over the summed HIP-event time of its launches; `cpu_baseline`
+times the C restatement of the reference kernel (oracle ref_walk, float MRG
+stream) on a bounded sample of the same seeds and extrapolates.
 """
 import argparse
 import json
@@ -72,6 +76,25 @@ def cpu_baseline(img_c, seeds, nbrw, sorw, nlabels, budget_s):
     return rate, idx.size, dt, orc.num_threads()
 
 
+def time_pipeline(step, eng, warmup, steps, label):
+    for w in range(warmup):
+        t = time.perf_counter()
+        out = step({})
+        log("%s warmup %d: %.2fs" % (label, w, time.perf_counter() - t))
+    eng.sync()
+    tms = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        tm = {}
+        ts = time.perf_counter()
+        out = step(tm)
+        tms.append(tm)
+        log("%s step %d: %.2fs  %s" % (label, k, time.perf_counter() - ts,
+                                       " ".join("%s=%.3g" % kv for kv in sorted(tm.items()))))
+    eng.sync()
+    return out, tms, (time.perf_counter() - t0) / steps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +104,11 @@ def main():
     ap.add_argument("--nbrw", type=int, default=10)
     ap.add_argument("--sorw", type=int, default=4000)
     ap.add_argument("--smooth", type=int, default=100)
+    ap.add_argument("--method", choices=["both", "plain", "guided"], default="both")
+    ap.add_argument("--thr", type=float, default=0.9)
+    ap.add_argument("--brad", type=int, default=3)
+    ap.add_argument("--srad", type=int, default=16)
+    ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -90,91 +118,103 @@ def main():
         sys.exit(2)
 
     from biomedisa_amd import _hip
-    from biomedisa_amd.interpolation import DeviceVolume, run_pipeline
+    from biomedisa_amd.helpers import per_label_dice
+    from biomedisa_amd.interpolation import DeviceVolume, run_pipeline, run_pipeline_guided
 
     vol, img_c, labidx, values, nol = prepare(args)
     eng = _hip.engine(0)
     dv = DeviceVolume(eng, img_c, labidx)
     log("cropped volume %s, %d labels, uploaded" % (dv.dims, nol))
-
-    def step(timings=None):
-        out = run_pipeline(dv, nol, values, nbrw=args.nbrw, sorw=args.sorw, allaxis=True,
-                           smooth=args.smooth, timings=timings)
-        return out
-
-    for w in range(args.warmup):
-        t = time.perf_counter()
-        out = step()
-        log("warmup %d: %.2fs (seeds %d)" % (w, time.perf_counter() - t, out["nseeds"]))
-    eng.sync()
-    walk_ms, walk_launches, accepted, nseeds, per = [], [], 0, 0, []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        tm = {}
-        ts = time.perf_counter()
-        out = step(tm)
-        per.append(time.perf_counter() - ts)
-        walk_ms.append(tm["walk_kernel_ms"])
-        walk_launches.append(tm["walk_launches"])
-        accepted, nseeds = out["accepted"], out["nseeds"]
-        log("step %d: %.2fs  (seeds %.2fs, walk phase %.0f ms = %d walker launches %.0f ms, "
-            "merge+smooth %.2fs)" % (k, per[-1], tm["seeds_s"], tm["walk_phase_ms"],
-                                     tm["walk_launches"], tm["walk_kernel_ms"], tm["merge_s"]))
-    eng.sync()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1000.0 * elapsed / args.steps
     n_vox = args.size ** 3
-    value = n_vox / (elapsed / args.steps)
+    res = {}
 
-    # quality of the result against the generating ground truth (reported, not gated)
-    from biomedisa_amd.helpers import per_label_dice
-    Z, Y, X = dv.dims
-    regular = out["regular"].download(dv.dims, np.uint8)[1:-1, 1:-1, 1:-1]
-    d_gt = per_label_dice(regular, vol.gt)
+    if args.method in ("both", "plain"):
+        def plain_step(tm):
+            return run_pipeline(dv, nol, values, nbrw=args.nbrw, sorw=args.sorw, allaxis=True,
+                                smooth=args.smooth, timings=tm)
+        out, tms, sec = time_pipeline(plain_step, eng, args.warmup, args.steps, "plain")
+        res["plain"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], accepted=out["accepted"],
+                            regular=out["regular"].download(dv.dims, np.uint8))
+    if args.method in ("both", "guided"):
+        from biomedisa_amd import surrogate
+        net = surrogate.Net(eng)
+        net.load(args.weights or surrogate.WEIGHTS_PATH)
 
-    steps_total = nseeds * args.nbrw * args.sorw
-    walk_bytes = steps_total * 6 * 1 + accepted * 8
-    kernel_s = float(np.mean(walk_ms)) / 1000.0          # summed walker launches of one step
-    launches = int(np.mean(walk_launches))
-    achieved = walk_bytes / kernel_s / 1e9
+        def guided_step(tm):
+            return run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw,
+                                       allaxis=True, smooth=args.smooth, thr=args.thr,
+                                       brad=args.brad, srad=args.srad, timings=tm)
+        out, tms, sec = time_pipeline(guided_step, eng, args.warmup, args.steps, "guided")
+        res["guided"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], walked=out["walked_seeds"],
+                             accepted=out["accepted"],
+                             regular=out["regular"].download(dv.dims, np.uint8))
+
+    inner = (slice(1, -1),) * 3
+    main_key = "guided" if "guided" in res else "plain"
+    r = res[main_key]
+    value = n_vox / r["sec"]
+    config = {"workload": "%d^3 uint8, 8 ellipsoids/tubes, labels every 10th plane allaxis, "
+                          "nbrw=%d sorw=%d smooth=%d" % (args.size, args.nbrw, args.sorw, args.smooth),
+              "method": main_key}
+    if "guided" in res:
+        g = res["guided"]
+        config.update(guided_seconds=g["sec"], walked_seed_fraction=g["walked"] / g["nseeds"],
+                      thr=args.thr, brad=args.brad, srad=args.srad,
+                      guided_timings=g["tms"][-1])
+        config["min_dice_vs_ground_truth_guided"] = min(per_label_dice(g["regular"][inner], vol.gt).values())
+    if "plain" in res:
+        p = res["plain"]
+        config.update(plain_seconds=p["sec"], plain_voxels_per_s=n_vox / p["sec"],
+                      plain_timings=p["tms"][-1])
+        config["min_dice_vs_ground_truth_plain"] = min(per_label_dice(p["regular"][inner], vol.gt).values())
+    if "plain" in res and "guided" in res:
+        d = per_label_dice(res["guided"]["regular"], res["plain"]["regular"])
+        config.update(speedup_vs_plain=res["plain"]["sec"] / res["guided"]["sec"],
+                      min_dice_guided_vs_plain=min(d.values()),
+                      dice_guided_vs_plain={str(k): round(v, 5) for k, v in d.items()})
+
     line = {
-        "metric": "segmented voxels/s at 512^3, 8 labels (plain HIP walker)",
+        "metric": "segmented voxels/s at 512^3, 8 labels; surrogate-guided vs plain at Dice>=0.99",
         "value": value,
         "unit": "voxels/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1000.0 * r["sec"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "%d^3 uint8, 8 ellipsoids/tubes, labels every 10th plane allaxis, "
-                               "nbrw=%d sorw=%d smooth=%d, plain walker" % (args.size, args.nbrw,
-                                                                            args.sorw, args.smooth),
-                   "method": "plain", "seeds": nseeds, "walker_steps": steps_total,
-                   "accepted_moves": accepted, "walk_kernel_ms": float(np.mean(walk_ms)),
-                   "walk_launches": launches,
-                   "walk_launch_avg_ms": float(np.mean(walk_ms)) / max(launches, 1),
-                   "min_dice_vs_ground_truth": min(d_gt.values())},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile_walk<uint8_t>",
-                     "algorithmic_bytes_per_launch": walk_bytes / max(launches, 1)},
+        "config": config,
     }
+    # roofline of the dominant kernel: the walker of the plain pipeline (full seed set)
+    rk = res["plain"] if "plain" in res else res["guided"]
+    tm = rk["tms"][-1]
+    walked = rk.get("walked", rk["nseeds"])
+    steps_total = walked * args.nbrw * args.sorw
+    walk_bytes = steps_total * 6 * 1 + rk["accepted"] * 8
+    launches = max(int(tm["walk_launches"]), 1)
+    kernel_s = tm["walk_kernel_ms"] / 1000.0
+    achieved = walk_bytes / kernel_s / 1e9
+    line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "k_tile_walk<uint8_t>", "walker_steps": steps_total,
+                        "accepted_moves": rk["accepted"], "launches": launches,
+                        "launch_avg_ms": tm["walk_kernel_ms"] / launches,
+                        "algorithmic_bytes_per_launch": walk_bytes / launches}
     if not args.no_cpu_baseline:
         seeds_buf, n = eng.seeds(dv.lab, dv.img, dv.img_dtype, dv.dims,
                                  eng.plane_flags(dv.lab, dv.dims, True))
         seeds = seeds_buf.download((n,), _hip.SEED_DTYPE)
         rate, nsample, dt, cores = cpu_baseline(img_c, seeds, args.nbrw, args.sorw, nol,
                                                 args.cpu_budget)
-        walk_s = steps_total / rate
+        walk_s = n * args.nbrw * args.sorw / rate
         line["cpu_baseline"] = {
             "value": n_vox / walk_s, "unit": "voxels/s", "cores": cores, "kind": "port",
             "sample": "oracle ref_walk (C/OpenMP restatement of pyopencl_small.py randomWalk, "
                       "float MRG stream) on %d random seeds of this workload (%.1fs, %.3g steps/s); "
-                      "full walk extrapolated to %.0fs; smoothing/merge not counted"
+                      "full plain walk extrapolated to %.0fs; smoothing/merge not counted"
                       % (nsample, dt, rate, walk_s)}
     print(json.dumps(line), flush=True)
 

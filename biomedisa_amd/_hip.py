@@ -37,6 +37,7 @@ SIGNATURES = {
     "bmx_upload": (_i, [_p, _p, _p, C.c_size_t]),
     "bmx_download": (_i, [_p, _p, _p, C.c_size_t]),
     "bmx_zero": (_i, [_p, _p, C.c_size_t]),
+    "bmx_copy": (_i, [_p, _p, _p, C.c_size_t]),
     "bmx_plane_flags": (_i, [_p, _p, _i64, _i64, _i64, _i, _p]),
     "bmx_seeds": (_i, [_p, _p, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
@@ -64,6 +65,13 @@ SIGNATURES = {
     "bmx_adam": (_i, [_p, _p, _p, _p, _p, _p, _i64, C.c_float, C.c_float, C.c_float, C.c_float, _i,
                       C.c_float]),
     "bmx_f32_to_bf16": (_i, [_p, _p, _p, _i64]),
+    "bmx_add_bf16": (_i, [_p, _p, _p, _i64]),
+    "bmx_hits_target": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]),
+    "bmx_net_input": (_i, [_p, _p, _i, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
+    "bmx_head_predict": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _p, C.c_float, _p, _p]),
+    "bmx_guided_masks": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p]),
+    "bmx_merge_guided": (_i, [_p, _p, _i, _i64, _p, _p, _p, _p]),
+    "bmx_select_labels": (_i, [_p, _p, _p, _p, _i, _p, _i64, _p]),
 }
 
 _lib = None

@@ -130,6 +130,16 @@ int bmx_download(bmx_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes
     return BMX_OK;
 }
 
+int bmx_copy(bmx_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    if (bytes == 0) return BMX_OK;
+    BMX_CHECK(dst_dev && src_dev, "bmx_copy: null pointer");
+    BMX_HIP(hipMemcpyAsync(dst_dev, src_dev, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return BMX_OK;
+}
+
 int bmx_zero(bmx_ctx *ctx, void *dst_dev, size_t bytes)
 {
     int rc = bmx_set_device(ctx);

@@ -1,0 +1,280 @@
+// bmx_guide.hip -- surrogate-guided walking: encode the network input from
+// the image and the sparse labels, turn the network output into a label
+// proposal and an uncertain set, select the region whose seeds must walk, and
+// merge walker and network labels.  No counterpart in the reference.
+//
+// Guided rule: U = {voxels whose top softmax probability < thr} united with
+// the predicted label boundaries dilated by `brad` voxels (Chebyshev).  Seeds
+// within `srad` voxels (Chebyshev) of U walk; the result takes the walker's
+// argmax on U and the network's label elsewhere.
+#include "bmx_internal.h"
+
+namespace {
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Network input on a padded grid (Zp,Yp,Xp) >= (Z,Y,X), 16 bf16 channels:
+// 0: image/127.5 - 1; 1: 1 on seed voxels (labelled planes, label not
+// ignored); 2 + l: one-hot of label index l on seed voxels (l < 14).
+template <typename T>
+__global__ void k_net_input(const T *__restrict__ img, const uint8_t *__restrict__ lab,
+                            const uint8_t *__restrict__ flags, int Z, int Y, int X, int Zp, int Yp, int Xp,
+                            uint16_t *__restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np_ = (int64_t)Zp * Yp * Xp;
+    if (v >= np_) return;
+    const int x = (int)(v % Xp), y = (int)((v / Xp) % Yp), z = (int)(v / ((int64_t)Xp * Yp));
+    uint16_t ch[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) ch[c] = 0;
+    if (z < Z && y < Y && x < X) {
+        const int64_t i = ((int64_t)z * Y + y) * X + x;
+        ch[0] = f2bf((float)img[i] * (1.0f / 127.5f) - 1.0f);
+        const bool plane = flags[z] | flags[Z + y] | flags[Z + Y + x];
+        const uint8_t l = lab[i];
+        if (plane && l != BMX_LABEL_NONE) {
+            ch[1] = 0x3F80;  // 1.0
+            if (l < 14) ch[2 + l] = 0x3F80;
+        }
+    }
+    uint4 *o = (uint4 *)(out + v * 16);
+    o[0] = make_uint4(ch[0] | (ch[1] << 16), ch[2] | (ch[3] << 16), ch[4] | (ch[5] << 16), ch[6] | (ch[7] << 16));
+    o[1] = make_uint4(ch[8] | (ch[9] << 16), ch[10] | (ch[11] << 16), ch[12] | (ch[13] << 16),
+                      ch[14] | (ch[15] << 16));
+}
+
+// 1x1x1 head on the padded grid -> per voxel of the (Z,Y,X) volume: argmax
+// label index and an uncertainty flag (top probability < thr).
+__global__ void k_head_predict(const uint16_t *__restrict__ x, int Z, int Y, int X, int Zp, int Yp, int Xp,
+                               int L, const float *__restrict__ W, const float *__restrict__ b, float thr,
+                               uint8_t *__restrict__ pred, uint8_t *__restrict__ unc)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (int64_t)Z * Y * X) return;
+    const int xx = (int)(v % X), yy = (int)((v / X) % Y), zz = (int)(v / ((int64_t)X * Y));
+    const int64_t p = ((int64_t)zz * Yp + yy) * Xp + xx;
+    float xin[16];
+    for (int c = 0; c < 16; ++c) xin[c] = bf2f(x[p * 16 + c]);
+    float best = -1e30f, den = 0.0f;
+    int bl = 0;
+    float lg[16];
+    for (int l = 0; l < L; ++l) {
+        float s = b[l];
+        for (int c = 0; c < 16; ++c) s += W[l * 16 + c] * xin[c];
+        lg[l] = s;
+        if (s > best) { best = s; bl = l; }
+    }
+    for (int l = 0; l < L; ++l) den += __expf(lg[l] - best);
+    pred[v] = (uint8_t)bl;
+    unc[v] = (1.0f / den) < thr ? 1 : 0;
+}
+
+// U seed: voxels whose predicted label differs from a 6-neighbour's.
+__global__ void k_boundary(const uint8_t *__restrict__ pred, int Z, int Y, int X, uint8_t *__restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (int64_t)Z * Y * X) return;
+    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
+    const uint8_t p = pred[v];
+    const int64_t sy = X, sz = (int64_t)X * Y;
+    bool b = false;
+    if (x > 0) b |= pred[v - 1] != p;
+    if (x < X - 1) b |= pred[v + 1] != p;
+    if (y > 0) b |= pred[v - sy] != p;
+    if (y < Y - 1) b |= pred[v + sy] != p;
+    if (z > 0) b |= pred[v - sz] != p;
+    if (z < Z - 1) b |= pred[v + sz] != p;
+    out[v] = b ? 1 : 0;
+}
+
+// out = max over |k| <= r along one axis (0: z, 1: y, 2: x) of in.
+__global__ void k_dilate_axis(const uint8_t *__restrict__ in, int Z, int Y, int X, int axis, int r,
+                              uint8_t *__restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (int64_t)Z * Y * X) return;
+    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
+    const int c = axis == 0 ? z : (axis == 1 ? y : x);
+    const int n = axis == 0 ? Z : (axis == 1 ? Y : X);
+    const int64_t st = axis == 0 ? (int64_t)X * Y : (axis == 1 ? (int64_t)X : 1);
+    const int lo = max(c - r, 0), hi = min(c + r, n - 1);
+    uint8_t m = 0;
+    for (int k = lo; k <= hi && !m; ++k) m |= in[v + (int64_t)(k - c) * st];
+    out[v] = m ? 1 : 0;
+}
+
+__global__ void k_or(uint8_t *__restrict__ a, const uint8_t *__restrict__ b, int64_t n)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) a[v] = (a[v] | b[v]) ? 1 : 0;
+}
+
+struct LabelValues {
+    uint8_t v[256];
+};
+
+// out = U ? values[argmax_l hits] : values[pred]
+__global__ void k_merge_guided(const uint32_t *__restrict__ hits, int nlabels, int64_t V,
+                               const uint8_t *__restrict__ pred, const uint8_t *__restrict__ U, LabelValues lv,
+                               uint8_t *__restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    if (!U[v]) {
+        out[v] = lv.v[pred[v]];
+        return;
+    }
+    uint32_t b = hits[v];
+    int bi = 0;
+    for (int l = 1; l < nlabels; ++l) {
+        const uint32_t h = hits[(int64_t)l * V + v];
+        if (h > b) { b = h; bi = l; }
+    }
+    out[v] = lv.v[bi];
+}
+
+// out = U ? a : values[pred]  (a already holds label values)
+__global__ void k_select_labels(const uint8_t *__restrict__ a, const uint8_t *__restrict__ pred,
+                                const uint8_t *__restrict__ U, LabelValues lv, int64_t V,
+                                uint8_t *__restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    out[v] = U[v] ? a[v] : lv.v[pred[v]];
+}
+
+// Soft training targets from walker hits: t[v][l] = hits[l][v] / sum_l hits,
+// on a padded grid (zeros and mask 0 outside the volume or where no walker
+// arrived); L >= nlabels outputs (extra labels get 0).
+__global__ void k_hits_target(const uint32_t *__restrict__ hits, int nlabels, int Z, int Y, int X, int Zp,
+                              int Yp, int Xp, int L, float *__restrict__ target, uint8_t *__restrict__ vmask)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np_ = (int64_t)Zp * Yp * Xp;
+    if (v >= np_) return;
+    const int x = (int)(v % Xp), y = (int)((v / Xp) % Yp), z = (int)(v / ((int64_t)Xp * Yp));
+    const int64_t V = (int64_t)Z * Y * X;
+    float tot = 0.0f;
+    int64_t i = 0;
+    const bool in = z < Z && y < Y && x < X;
+    if (in) {
+        i = ((int64_t)z * Y + y) * X + x;
+        for (int l = 0; l < nlabels; ++l) tot += (float)hits[(int64_t)l * V + i];
+    }
+    const bool ok = in && tot > 0.0f;
+    for (int l = 0; l < L; ++l)
+        target[v * L + l] = (ok && l < nlabels) ? (float)hits[(int64_t)l * V + i] / tot : 0.0f;
+    vmask[v] = ok ? 1 : 0;
+}
+
+int dilate(bmx_ctx *ctx, uint8_t *buf, uint8_t *tmp, int Z, int Y, int X, int r)
+{
+    if (r <= 0) return BMX_OK;
+    const int64_t V = (int64_t)Z * Y * X;
+    k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 2, r, tmp);
+    k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(tmp, Z, Y, X, 1, r, buf);
+    k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 0, r, tmp);
+    BMX_LAUNCH_CHECK();
+    BMX_HIP(hipMemcpyAsync(buf, tmp, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
+    return BMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmx_net_input(bmx_ctx *ctx, const void *img, int img_dtype, const uint8_t *lab, const uint8_t *flags,
+                  int Z, int Y, int X, int Zp, int Yp, int Xp, uint16_t *out)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(img && lab && flags && out, "bmx_net_input: null pointer");
+    BMX_CHECK(Zp >= Z && Yp >= Y && Xp >= X, "bmx_net_input: padded grid smaller than the volume");
+    const int64_t np_ = (int64_t)Zp * Yp * Xp;
+    if (img_dtype == BMX_IMG_U8)
+        k_net_input<uint8_t><<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>((const uint8_t *)img, lab, flags, Z,
+                                                                          Y, X, Zp, Yp, Xp, out);
+    else
+        k_net_input<float><<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>((const float *)img, lab, flags, Z, Y,
+                                                                        X, Zp, Yp, Xp, out);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_head_predict(bmx_ctx *ctx, const uint16_t *x, int Z, int Y, int X, int Zp, int Yp, int Xp, int L,
+                     const float *W, const float *b, float thr, uint8_t *pred, uint8_t *unc)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(x && W && b && pred && unc && L >= 1 && L <= 16, "bmx_head_predict: bad arguments");
+    const int64_t V = (int64_t)Z * Y * X;
+    k_head_predict<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(x, Z, Y, X, Zp, Yp, Xp, L, W, b, thr, pred, unc);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_guided_masks(bmx_ctx *ctx, const uint8_t *pred, const uint8_t *unc, int Z, int Y, int X, int brad,
+                     int srad, uint8_t *U, uint8_t *need, uint8_t *tmp)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(pred && unc && U && need && tmp, "bmx_guided_masks: null pointer");
+    BMX_CHECK(brad >= 0 && srad >= 0, "bmx_guided_masks: negative radius");
+    const int64_t V = (int64_t)Z * Y * X;
+    k_boundary<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(pred, Z, Y, X, U);
+    BMX_LAUNCH_CHECK();
+    rc = dilate(ctx, U, tmp, Z, Y, X, brad);
+    if (rc) return rc;
+    k_or<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(U, unc, V);
+    BMX_LAUNCH_CHECK();
+    BMX_HIP(hipMemcpyAsync(need, U, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
+    return dilate(ctx, need, tmp, Z, Y, X, srad);
+}
+
+int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const uint8_t *values,
+                     const uint8_t *pred, const uint8_t *U, uint8_t *out)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(hits && values && pred && U && out && nlabels >= 1 && nlabels <= 255,
+              "bmx_merge_guided: bad arguments");
+    LabelValues lv;
+    for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
+    k_merge_guided<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(hits, nlabels, V, pred, U, lv, out);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_hits_target(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int Y, int X, int Zp, int Yp,
+                    int Xp, int L, float *target, uint8_t *vmask)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(hits && target && vmask && L >= nlabels, "bmx_hits_target: bad arguments");
+    const int64_t np_ = (int64_t)Zp * Yp * Xp;
+    k_hits_target<<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>(hits, nlabels, Z, Y, X, Zp, Yp, Xp, L, target,
+                                                                vmask);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_select_labels(bmx_ctx *ctx, const uint8_t *a, const uint8_t *pred, const uint8_t *U, int nlabels,
+                      const uint8_t *values, int64_t V, uint8_t *out)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(a && pred && U && values && out, "bmx_select_labels: null pointer");
+    LabelValues lv;
+    for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
+    k_select_labels<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(a, pred, U, lv, V, out);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+}  // extern "C"

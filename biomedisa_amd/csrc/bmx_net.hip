@@ -466,6 +466,12 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
     if (pbf) pbf[i] = f2bf(pi);
 }
 
+__global__ void k_add_bf16(uint16_t *__restrict__ y, const uint16_t *__restrict__ x, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(bf2f(y[i]) + bf2f(x[i]));
+}
+
 __global__ void k_f2bf(const float *__restrict__ a, uint16_t *__restrict__ b, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -652,6 +658,16 @@ int bmx_relu_mask(bmx_ctx *ctx, uint16_t *g, const uint16_t *a, int64_t n)
     if (rc) return rc;
     BMX_CHECK(g && a, "bmx_relu_mask: null pointer");
     k_relu_mask<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(g, a, n);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_add_bf16(bmx_ctx *ctx, uint16_t *y, const uint16_t *x, int64_t n)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(y && x, "bmx_add_bf16: null pointer");
+    k_add_bf16<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(y, x, n);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }

@@ -106,17 +106,87 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
     return out
 
 
+def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
+                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=16, timings=None, keep=False):
+    """Surrogate-guided pipeline on device-resident inputs.
+
+    The network proposes a label and a confidence for every voxel.  U = voxels
+    whose top probability is below `thr` plus the proposed label boundaries
+    dilated by `brad`; only seeds within `srad` (Chebyshev) of U walk, with the
+    same Philox streams as the plain walker, so those walks are exactly the
+    plain walker's.  The result takes the walker's argmax on U and the
+    network's label elsewhere ('smooth': the smoothed walker argmax on U).
+    Returns the dict of run_pipeline plus 'walked_seeds' and 'U_fraction'.
+    """
+    from . import surrogate as S
+    eng = vol.eng
+    Z, Y, X = vol.dims
+    V = vol.V
+    if nlabels > S.L_MAX:
+        raise PipelineError("guided mode supports at most %d labels" % S.L_MAX)
+    t = timings if timings is not None else {}
+    t0 = time.perf_counter()
+    flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis)
+    flags = flags_buf.download((Z + Y + X,), np.uint8)
+    if allaxis and adjacent_planes((flags[:Z], flags[Z:Z + Y], flags[Z + Y:])):
+        raise PipelineError("At least one empty slice between labels required.")
+    seeds, nseeds = eng.seeds(vol.lab, vol.img, vol.img_dtype, vol.dims, flags_buf)
+    t1 = time.perf_counter()
+    grid = (S.pad8(Z), S.pad8(Y), S.pad8(X))
+    x = net._get("input", int(np.prod(grid)) * S.CIN * 2)
+    _hip.call("bmx_net_input", eng.ctx, vol.img.ptr, vol.img_dtype, vol.lab.ptr, flags_buf.ptr, Z, Y, X,
+              *grid, x.ptr)
+    pred, unc = eng.alloc(V), eng.alloc(V)
+    net.predict(x, vol.dims, grid, nlabels, thr, pred, unc)
+    U, need, tmp = eng.alloc(V), eng.alloc(V), eng.alloc(V)
+    _hip.call("bmx_guided_masks", eng.ctx, pred.ptr, unc.ptr, Z, Y, X, int(brad), int(srad), U.ptr, need.ptr,
+              tmp.ptr)
+    sub, nsub = eng.select_seeds(seeds, nseeds, need)
+    eng.sync()
+    t2 = time.perf_counter()
+    hits = eng.alloc(nlabels * V * 4)
+    hits.zero()
+    accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, sub, nsub, nlabels, nbrw, sorw, _hip.RNG_PHILOX,
+                        key, hits)
+    t3 = time.perf_counter()
+    out = {"nseeds": nseeds, "walked_seeds": nsub, "accepted": accepted, "flags": flags}
+    values = np.ascontiguousarray(values, dtype=np.uint8)
+    regular = eng.alloc(V)
+    _hip.call("bmx_merge_guided", eng.ctx, hits.ptr, int(nlabels), V, values.ctypes.data, pred.ptr, U.ptr,
+              regular.ptr)
+    out["regular"] = regular
+    if smooth:
+        sm_all, sm = eng.alloc(V), eng.alloc(V)
+        eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm_all)
+        _hip.call("bmx_select_labels", eng.ctx, sm_all.ptr, pred.ptr, U.ptr, int(nlabels), values.ctypes.data,
+                  V, sm.ptr)
+        out["smooth"] = sm
+    eng.sync()
+    t4 = time.perf_counter()
+    if keep:
+        out.update(pred=pred, U=U, need=need, hits=hits)
+    phase_ms, kernel_ms, launches = eng.last_walk_stats()
+    t.update(seeds_s=t1 - t0, proposal_s=t2 - t1, walk_s=t3 - t2, merge_s=t4 - t3, walk_phase_ms=phase_ms,
+             walk_kernel_ms=kernel_ms, walk_launches=launches)
+    return out
+
+
 def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_alpha=1.0,
                         acwe_smooth=1, acwe_steps=3, path_to_data=None, path_to_labels=None,
                         denoise=False, uncertainty=False, platform=None, allaxis=False,
                         ignore='none', only='all', smooth=0, compression=True, return_hits=False,
                         img_id=None, label_id=None, remote=False, queue=0, clean=None, fill=None,
-                        slicer=False, rng="philox", key=DEFAULT_KEY, device=0):
+                        slicer=False, rng="philox", key=DEFAULT_KEY, device=0, method="plain",
+                        guide=None):
     """Smart interpolation of sparsely labelled slices (biomedisa/interpolation.py:43).
 
     Returns the reference's result dict ({'regular': uint8 (Z,Y,X)} plus
     'smooth', 'uncertainty', 'hits' when requested), or None when the input is
     rejected (after printing "Error: ..." as the reference does).
+
+    method='guided' runs the surrogate-guided pipeline (run_pipeline_guided;
+    Philox stream only, no 'uncertainty'/'hits'); `guide` is a dict of its
+    thr/brad/srad parameters.
     """
     if data is None or labelData is None:
         raise NotImplementedError("file loading (path_to_data/path_to_labels) is not part of this "
@@ -125,6 +195,10 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         raise NotImplementedError("acwe/clean/fill/denoise post-processing is not implemented")
     if rng not in _RNG:
         raise ValueError("rng must be 'philox' or 'reference'")
+    if method not in ("plain", "guided"):
+        raise ValueError("method must be 'plain' or 'guided'")
+    if method == "guided" and (rng != "philox" or uncertainty or return_hits):
+        raise ValueError("guided mode runs the Philox walker and returns 'regular'/'smooth' only")
 
     bm = Biomedisa()
     bm.TIC = time.time()
@@ -160,8 +234,13 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     eng = _hip.engine(device)
     vol = DeviceVolume(eng, img_c, labidx)
     try:
-        dev = run_pipeline(vol, bm.nol, values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
-                           bm.uncertainty, rng, key, keep_hits=bm.return_hits)
+        if method == "guided":
+            from . import surrogate
+            dev = run_pipeline_guided(vol, bm.nol, values, surrogate.net(device), bm.nbrw, bm.sorw,
+                                      bm.allaxis, bm.smooth, key, **(guide or {}))
+        else:
+            dev = run_pipeline(vol, bm.nol, values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
+                               bm.uncertainty, rng, key, keep_hits=bm.return_hits)
     except PipelineError as e:
         _error_(bm, str(e))
         return None

@@ -74,6 +74,8 @@ int bmx_free(bmx_ctx *ctx, void *dptr);
 int bmx_upload(bmx_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 int bmx_download(bmx_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
 int bmx_zero(bmx_ctx *ctx, void *dst_dev, size_t bytes);
+/* Device-to-device copy on the context stream. */
+int bmx_copy(bmx_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 
 /* Labelled planes: replaces read_labeled_slices / read_labeled_slices_allx /
  * read_indices_allx (biomedisa_helper.py:1015-1066).  flags (device, Z+Y+X
@@ -190,6 +192,32 @@ int bmx_head(bmx_ctx *ctx, const uint16_t *x, int64_t nvox, int L, const float *
 int bmx_adam(bmx_ctx *ctx, float *p, const float *g, float *m, float *v, uint16_t *pbf, int64_t n,
              float lr, float beta1, float beta2, float eps, int step, float weight_decay);
 int bmx_f32_to_bf16(bmx_ctx *ctx, const float *a, uint16_t *b, int64_t n);
+/* y += x elementwise (bf16). */
+int bmx_add_bf16(bmx_ctx *ctx, uint16_t *y, const uint16_t *x, int64_t n);
+/* Soft targets t[v][l] = hits[l][v] / sum_l hits[.][v] on a padded grid;
+ * vmask = 1 where the voxel is inside (Z,Y,X) and some walker arrived. */
+int bmx_hits_target(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int Y, int X, int Zp,
+                    int Yp, int Xp, int L, float *target, uint8_t *vmask);
+
+/* Surrogate-guided walking (no counterpart in the reference). */
+/* Network input on a padded grid (Zp,Yp,Xp) >= (Z,Y,X), 16 bf16 channels:
+ * image/127.5-1, seed indicator, one-hot label index (< 14) on seed voxels. */
+int bmx_net_input(bmx_ctx *ctx, const void *img, int img_dtype, const uint8_t *lab,
+                  const uint8_t *flags, int Z, int Y, int X, int Zp, int Yp, int Xp, uint16_t *out);
+/* Head on the padded grid -> per voxel of (Z,Y,X): argmax label index and
+ * uncertainty flag (top softmax probability < thr). */
+int bmx_head_predict(bmx_ctx *ctx, const uint16_t *x, int Z, int Y, int X, int Zp, int Yp, int Xp,
+                     int L, const float *W, const float *b, float thr, uint8_t *pred, uint8_t *unc);
+/* U = unc | dilate(label boundaries of pred, brad); need = dilate(U, srad)
+ * (Chebyshev dilations); tmp: V bytes of scratch. */
+int bmx_guided_masks(bmx_ctx *ctx, const uint8_t *pred, const uint8_t *unc, int Z, int Y, int X,
+                     int brad, int srad, uint8_t *U, uint8_t *need, uint8_t *tmp);
+/* out = U ? values[argmax hits] : values[pred]. */
+int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,
+                     const uint8_t *values, const uint8_t *pred, const uint8_t *U, uint8_t *out);
+/* out = U ? a : values[pred]  (a holds label values, e.g. a smoothed result). */
+int bmx_select_labels(bmx_ctx *ctx, const uint8_t *a, const uint8_t *pred, const uint8_t *U,
+                      int nlabels, const uint8_t *values, int64_t V, uint8_t *out);
 
 #ifdef __cplusplus
 }

@@ -88,3 +88,55 @@ def head(x, W, b, target=None, vmask=None, scale=1.0):
     dW = g.reshape(-1, g.shape[-1]).T @ x.reshape(-1, x.shape[-1])
     db = g.reshape(-1, g.shape[-1]).sum(0)
     return p, loss, dx, dW, db
+
+
+# ---------------------------------------------------------------------------
+# The whole encoder-decoder (same layer list as the GPU network), float64,
+# with optional bf16 rounding of activations and activation gradients at the
+# points where the GPU stores them in bf16.
+# ---------------------------------------------------------------------------
+UNET_LAYERS = [("enc0", 16, 16), ("enc1", 16, 32), ("enc2", 32, 64), ("bott", 64, 128),
+               ("dec2", 192, 64), ("dec1", 96, 32), ("dec0", 48, 16)]
+
+
+def unet_loss_and_grads(params, x, target, vmask, round_bf16=True):
+    """params: {name.w [Co,27,Ci], name.b, head.W [L,16], head.b}; x [N,D,H,W,16];
+    target [N,D,H,W,L]; vmask [N,D,H,W].  Returns (summed loss, grads dict)."""
+    r = bf16_round if round_bf16 else (lambda a: a)
+    P = {k: (bf16_round(v) if (round_bf16 and k.endswith(".w")) else v.astype(np.float64))
+         for k, v in params.items()}
+
+    def conv(name, inp):
+        return r(conv3d(inp, P[name + ".w"], P[name + ".b"], relu=True))
+
+    a0 = conv("enc0", x)
+    p0 = r(pool2(a0)); a1 = conv("enc1", p0)
+    p1 = r(pool2(a1)); a2 = conv("enc2", p1)
+    p2 = r(pool2(a2)); a3 = conv("bott", p2)
+    c2 = np.concatenate([a2, up2(a3)], -1); d2 = conv("dec2", c2)
+    c1 = np.concatenate([a1, up2(d2)], -1); d1 = conv("dec1", c1)
+    c0 = np.concatenate([a0, up2(d1)], -1); d0 = conv("dec0", c0)
+    nvox = float(np.prod(x.shape[:4]))
+    _, loss, g_d0, dW, db = head(d0, P["head.W"], P["head.b"], target, vmask, 1.0 / nvox)
+    G = {"head.W": dW, "head.b": db}
+
+    def back(name, inp, g_out, out, mask_in):
+        g = r(g_out) * (out > 0)
+        G[name + ".w"], G[name + ".b"] = conv3d_wgrad(inp, g)
+        gi = conv3d_dgrad(g, P[name + ".w"])
+        return r(gi * (inp > 0)) if mask_in else r(gi)
+
+    g_c0 = back("dec0", c0, g_d0, d0, True)
+    g_a0, g_d1 = g_c0[..., :16], r(up2_bwd(g_c0[..., 16:]))
+    g_c1 = back("dec1", c1, g_d1, d1, True)
+    g_a1, g_d2 = g_c1[..., :32], r(up2_bwd(g_c1[..., 32:]))
+    g_c2 = back("dec2", c2, g_d2, d2, True)
+    g_a2, g_a3 = g_c2[..., :64], r(up2_bwd(g_c2[..., 64:]))
+    g_p2 = back("bott", p2, g_a3, a3, False)
+    g_a2 = r(g_a2 + r(pool2_bwd(g_p2)))
+    g_p1 = back("enc2", p1, g_a2, a2, False)
+    g_a1 = r(g_a1 + r(pool2_bwd(g_p1)))
+    g_p0 = back("enc1", p0, g_a1, a1, False)
+    g_a0 = r(g_a0 + r(pool2_bwd(g_p0)))
+    back("enc0", x, g_a0, a0, False)
+    return loss, G

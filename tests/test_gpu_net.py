@@ -159,3 +159,36 @@ def test_adam_step(eng):
     got = pb.download((n,), np.float32)
     np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6)
     np.testing.assert_array_equal(bf.download((n,), np.uint16), O.to_bf16_bits(got))
+
+
+def test_whole_network_gradients_match_oracle(eng):
+    """Loss and every parameter gradient of the GPU encoder-decoder (forward,
+    head, backward through pooling, upsampling, skips and ReLUs) against the
+    float64 oracle network with bf16 rounding at the same points."""
+    from biomedisa_amd import surrogate as S
+    net = S.Net(eng, seed=3)
+    N, D, H, W = 1, 8, 8, 16
+    rng = np.random.default_rng(12)
+    x = O.bf16_round(np.concatenate([rng.standard_normal((N, D, H, W, 2)),
+                                     (rng.random((N, D, H, W, 14)) < 0.1)], -1))
+    t = rng.random((N, D, H, W, S.L_MAX)); t /= t.sum(-1, keepdims=True); t = t.astype(np.float32)
+    vm = (rng.random((N, D, H, W)) > 0.1).astype(np.uint8)
+    xb, tb, mb, lb = up_bf16(eng, x), eng.upload(t), eng.upload(vm), eng.alloc(4)
+    net.zero_grad()
+    lb.zero()
+    net.loss_and_grad(xb, tb, mb, N, D, H, W, lb)
+    params = net.state_dict()
+    loss, G = O.unet_loss_and_grads(params, x, t.astype(np.float64), vm)
+    np.testing.assert_allclose(lb.download((1,), np.float32)[0], loss, rtol=2e-2)
+    grads = {}
+    for name, _, _ in S.LAYERS:
+        w, b, _, _ = net.conv[name]
+        grads[name + ".w"] = w.g.download(w.shape, np.float32)
+        grads[name + ".b"] = b.g.download(b.shape, np.float32)
+    grads["head.W"] = net.hW.g.download(net.hW.shape, np.float32)
+    grads["head.b"] = net.hb.g.download(net.hb.shape, np.float32)
+    for k, ref in G.items():
+        got = grads[k].astype(np.float64)
+        cos = (got * ref).sum() / (np.linalg.norm(got) * np.linalg.norm(ref) + 1e-30)
+        rel = np.linalg.norm(got - ref) / (np.linalg.norm(ref) + 1e-30)
+        assert cos > 0.995 and rel < 0.1, (k, cos, rel)

@@ -80,3 +80,30 @@ def test_bf16_rounding_ties_to_even():
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 0.0], np.float32)
     r = O.bf16_round(x)
     np.testing.assert_array_equal(r, [1.0, 1.0, 1.0 + 2 ** -6, -2.5, 0.0])
+
+
+def test_whole_network_gradient_finite_differences():
+    """The oracle encoder-decoder's analytic gradients (float64, no rounding; they
+    are gradients of the mean loss, the loss returned is the sum) against central
+    differences of its loss, for a few weights of every layer."""
+    rng = np.random.default_rng(5)
+    params = {}
+    for n, ci, co in O.UNET_LAYERS:
+        params[n + ".w"] = rng.standard_normal((co, 27, ci)) * np.sqrt(2 / (27 * ci))
+        params[n + ".b"] = rng.standard_normal(co) * 0.1
+    params["head.W"] = rng.standard_normal((14, 16)) * 0.25
+    params["head.b"] = np.zeros(14)
+    x = rng.standard_normal((1, 8, 8, 8, 16))
+    t = rng.random((1, 8, 8, 8, 14)); t /= t.sum(-1, keepdims=True)
+    vm = np.ones((1, 8, 8, 8))
+    loss, G = O.unet_loss_and_grads(params, x, t, vm, round_bf16=False)
+    for k in ["enc0.w", "enc1.w", "enc2.w", "bott.w", "dec2.w", "dec1.w", "dec0.w", "dec2.b", "head.W"]:
+        flat = params[k].reshape(-1)
+        g = G[k].reshape(-1)
+        for i in rng.choice(flat.size, 3, replace=False):
+            old = flat[i]
+            flat[i] = old + 1e-5; lp, _ = O.unet_loss_and_grads(params, x, t, vm, False)
+            flat[i] = old - 1e-5; lm, _ = O.unet_loss_and_grads(params, x, t, vm, False)
+            flat[i] = old
+            fd = (lp - lm) / 2e-5 / x[..., 0].size
+            assert abs(fd - g[i]) <= 1e-4 * max(1e-2, abs(fd)), (k, i, fd, g[i])
