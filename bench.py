@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also launch bmx_f32_to_bf16 over 64 Mi floats (reads 256 MiB, writes "
+                         "128 MiB) once, as the known-traffic reference for HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; "
@@ -148,6 +151,14 @@ def main():
         res["guided"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], walked=out["walked_seeds"],
                              accepted=out["accepted"],
                              regular=out["regular"].download(dv.dims, np.uint8))
+
+    if args.calibrate_copy:
+        n_cal = 64 << 20
+        src, dst = eng.alloc(n_cal * 4), eng.alloc(n_cal * 2)
+        src.zero()
+        _hip.call("bmx_f32_to_bf16", eng.ctx, src.ptr, dst.ptr, n_cal)
+        eng.sync()
+        log("calibration copy: %d floats -> bf16" % n_cal)
 
     inner = (slice(1, -1),) * 3
     main_key = "guided" if "guided" in res else "plain"

@@ -66,7 +66,7 @@ SIGNATURES = {
                       C.c_float]),
     "bmx_f32_to_bf16": (_i, [_p, _p, _p, _i64]),
     "bmx_add_bf16": (_i, [_p, _p, _p, _i64]),
-    "bmx_hits_target": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]),
+    "bmx_hits_target": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]),
     "bmx_net_input": (_i, [_p, _p, _i, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
     "bmx_head_predict": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _p, C.c_float, _p, _p]),
     "bmx_guided_masks": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p]),

@@ -50,7 +50,10 @@ __global__ void k_net_input(const T *__restrict__ img, const uint8_t *__restrict
 }
 
 // 1x1x1 head on the padded grid -> per voxel of the (Z,Y,X) volume: argmax
-// label index and an uncertainty flag (top probability < thr).
+// label index (first maximum) and an uncertainty flag (top probability < thr).
+// Logits accumulate in channel order in FP32 (no contraction), the softmax
+// denominator in label order with the engine's exp, so the result is
+// reproducible on the CPU.
 __global__ void k_head_predict(const uint16_t *__restrict__ x, int Z, int Y, int X, int Zp, int Yp, int Xp,
                                int L, const float *__restrict__ W, const float *__restrict__ b, float thr,
                                uint8_t *__restrict__ pred, uint8_t *__restrict__ unc)
@@ -70,7 +73,7 @@ __global__ void k_head_predict(const uint16_t *__restrict__ x, int Z, int Y, int
         lg[l] = s;
         if (s > best) { best = s; bl = l; }
     }
-    for (int l = 0; l < L; ++l) den += __expf(lg[l] - best);
+    for (int l = 0; l < L; ++l) den += bmx_expf(lg[l] - best);
     pred[v] = (uint8_t)bl;
     unc[v] = (1.0f / den) < thr ? 1 : 0;
 }
@@ -153,20 +156,32 @@ __global__ void k_select_labels(const uint8_t *__restrict__ a, const uint8_t *__
 // on a padded grid (zeros and mask 0 outside the volume or where no walker
 // arrived); L >= nlabels outputs (extra labels get 0).
 __global__ void k_hits_target(const uint32_t *__restrict__ hits, int nlabels, int Z, int Y, int X, int Zp,
-                              int Yp, int Xp, int L, float *__restrict__ target, uint8_t *__restrict__ vmask)
+                              int Yp, int Xp, int L, int hard, float *__restrict__ target,
+                              uint8_t *__restrict__ vmask)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t np_ = (int64_t)Zp * Yp * Xp;
     if (v >= np_) return;
     const int x = (int)(v % Xp), y = (int)((v / Xp) % Yp), z = (int)(v / ((int64_t)Xp * Yp));
     const int64_t V = (int64_t)Z * Y * X;
-    float tot = 0.0f;
-    int64_t i = 0;
     const bool in = z < Z && y < Y && x < X;
-    if (in) {
-        i = ((int64_t)z * Y + y) * X + x;
-        for (int l = 0; l < nlabels; ++l) tot += (float)hits[(int64_t)l * V + i];
+    const int64_t i = in ? ((int64_t)z * Y + y) * X + x : 0;
+    if (hard) {
+        // one-hot of the walker's argmax: first maximum, index 0 when every count is 0
+        int best = 0;
+        uint32_t bh = 0;
+        if (in)
+            for (int l = 0; l < nlabels; ++l) {
+                const uint32_t h = hits[(int64_t)l * V + i];
+                if (h > bh) { bh = h; best = l; }
+            }
+        for (int l = 0; l < L; ++l) target[v * L + l] = (in && l == best) ? 1.0f : 0.0f;
+        vmask[v] = in ? 1 : 0;
+        return;
     }
+    float tot = 0.0f;
+    if (in)
+        for (int l = 0; l < nlabels; ++l) tot += (float)hits[(int64_t)l * V + i];
     const bool ok = in && tot > 0.0f;
     for (int l = 0; l < L; ++l)
         target[v * L + l] = (ok && l < nlabels) ? (float)hits[(int64_t)l * V + i] / tot : 0.0f;
@@ -197,6 +212,7 @@ int bmx_net_input(bmx_ctx *ctx, const void *img, int img_dtype, const uint8_t *l
     BMX_CHECK(img && lab && flags && out, "bmx_net_input: null pointer");
     BMX_CHECK(Zp >= Z && Yp >= Y && Xp >= X, "bmx_net_input: padded grid smaller than the volume");
     const int64_t np_ = (int64_t)Zp * Yp * Xp;
+    BMX_CHECK(np_ < ((int64_t)1 << 32) - 256, "bmx_net_input: grid exceeds one dispatch");
     if (img_dtype == BMX_IMG_U8)
         k_net_input<uint8_t><<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>((const uint8_t *)img, lab, flags, Z,
                                                                           Y, X, Zp, Yp, Xp, out);
@@ -214,6 +230,7 @@ int bmx_head_predict(bmx_ctx *ctx, const uint16_t *x, int Z, int Y, int X, int Z
     if (rc) return rc;
     BMX_CHECK(x && W && b && pred && unc && L >= 1 && L <= 16, "bmx_head_predict: bad arguments");
     const int64_t V = (int64_t)Z * Y * X;
+    BMX_CHECK(V < ((int64_t)1 << 32) - 256, "bmx_head_predict: volume exceeds one dispatch");
     k_head_predict<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(x, Z, Y, X, Zp, Yp, Xp, L, W, b, thr, pred, unc);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
@@ -227,6 +244,7 @@ int bmx_guided_masks(bmx_ctx *ctx, const uint8_t *pred, const uint8_t *unc, int 
     BMX_CHECK(pred && unc && U && need && tmp, "bmx_guided_masks: null pointer");
     BMX_CHECK(brad >= 0 && srad >= 0, "bmx_guided_masks: negative radius");
     const int64_t V = (int64_t)Z * Y * X;
+    BMX_CHECK(V < ((int64_t)1 << 32) - 256, "bmx_guided_masks: volume exceeds one dispatch");
     k_boundary<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(pred, Z, Y, X, U);
     BMX_LAUNCH_CHECK();
     rc = dilate(ctx, U, tmp, Z, Y, X, brad);
@@ -244,6 +262,7 @@ int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,
     if (rc) return rc;
     BMX_CHECK(hits && values && pred && U && out && nlabels >= 1 && nlabels <= 255,
               "bmx_merge_guided: bad arguments");
+    BMX_CHECK(V >= 0 && V < ((int64_t)1 << 32) - 256, "bmx_merge_guided: V out of range");
     LabelValues lv;
     for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
     k_merge_guided<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(hits, nlabels, V, pred, U, lv, out);
@@ -252,14 +271,15 @@ int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,
 }
 
 int bmx_hits_target(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int Y, int X, int Zp, int Yp,
-                    int Xp, int L, float *target, uint8_t *vmask)
+                    int Xp, int L, int hard, float *target, uint8_t *vmask)
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(hits && target && vmask && L >= nlabels, "bmx_hits_target: bad arguments");
     const int64_t np_ = (int64_t)Zp * Yp * Xp;
-    k_hits_target<<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>(hits, nlabels, Z, Y, X, Zp, Yp, Xp, L, target,
-                                                                vmask);
+    BMX_CHECK(np_ < ((int64_t)1 << 32) - 256, "bmx_hits_target: grid exceeds one dispatch");
+    k_hits_target<<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>(hits, nlabels, Z, Y, X, Zp, Yp, Xp, L, hard,
+                                                                target, vmask);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -270,6 +290,7 @@ int bmx_select_labels(bmx_ctx *ctx, const uint8_t *a, const uint8_t *pred, const
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(a && pred && U && values && out, "bmx_select_labels: null pointer");
+    BMX_CHECK(V >= 0 && V < ((int64_t)1 << 32) - 256, "bmx_select_labels: V out of range");
     LabelValues lv;
     for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
     k_select_labels<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(a, pred, U, lv, V, out);

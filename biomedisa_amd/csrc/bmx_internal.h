@@ -133,3 +133,16 @@ static inline unsigned bmx_blocks(int64_t n, int threads)
 {
     return (unsigned)((n + threads - 1) / threads);
 }
+
+// Element-wise kernels over ranges that may pass 2^32 elements stride over
+// them: one dispatch covers at most 2^32-1 work-items per grid dimension (the
+// dispatch packet's grid size is 32-bit), so such launches are capped at
+// kMaxStrideBlocks blocks and each thread loops with bmx_gtid/bmx_gstride.
+constexpr int64_t kMaxStrideBlocks = (int64_t)1 << 20;
+static inline unsigned bmx_stride_blocks(int64_t n, int threads)
+{
+    const int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b < 1 ? 1 : (b < kMaxStrideBlocks ? b : kMaxStrideBlocks));
+}
+__device__ __forceinline__ int64_t bmx_gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int64_t bmx_gstride() { return (int64_t)gridDim.x * blockDim.x; }

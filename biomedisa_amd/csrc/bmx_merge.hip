@@ -99,7 +99,7 @@ int bmx_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const
     if (rc) return rc;
     BMX_CHECK(hits && values && out, "bmx_argmax: null pointer");
     BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_argmax: nlabels %d not in [1,255]", nlabels);
-    BMX_CHECK(V >= 0, "bmx_argmax: negative V");
+    BMX_CHECK(V >= 0 && V < ((int64_t)1 << 32) - 256, "bmx_argmax: V %lld not in [0, 2^32)", (long long)V);
     if (V == 0) return BMX_OK;
     LabelValues lv;
     for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;

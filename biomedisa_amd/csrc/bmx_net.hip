@@ -262,8 +262,8 @@ __global__ __launch_bounds__(64) void k_conv_wgrad(const uint16_t *__restrict__ 
 // g = (a > 0) ? g : 0  (backward of a ReLU whose output is a).
 __global__ void k_relu_mask(uint16_t *__restrict__ g, const uint16_t *__restrict__ a, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && !(bf2f(a[i]) > 0.0f)) g[i] = 0;
+    for (int64_t i = bmx_gtid(); i < n; i += bmx_gstride())
+        if (!(bf2f(a[i]) > 0.0f)) g[i] = 0;
 }
 
 // [Co][27][Ci] -> [Ci][27][Co]
@@ -282,35 +282,36 @@ __global__ void k_pool2(const uint16_t *__restrict__ x, Shape s, int C, uint16_t
 {
     const int Do = s.D / 2, Ho = s.H / 2, Wo = s.W / 2;
     const int64_t n_out = (int64_t)s.N * Do * Ho * Wo * C;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_out) return;
-    const int c = (int)(i % C);
-    int64_t t = i / C;
-    const int xo = (int)(t % Wo); t /= Wo;
-    const int yo = (int)(t % Ho); t /= Ho;
-    const int zo = (int)(t % Do);
-    const int n = (int)(t / Do);
-    float acc = 0.0f;
-    for (int k = 0; k < 8; ++k) {
-        const int z = 2 * zo + (k >> 2), yy = 2 * yo + ((k >> 1) & 1), xx = 2 * xo + (k & 1);
-        acc += bf2f(x[((((int64_t)n * s.D + z) * s.H + yy) * s.W + xx) * C + c]);
+    for (int64_t i = bmx_gtid(); i < n_out; i += bmx_gstride()) {
+        const int c = (int)(i % C);
+        int64_t t = i / C;
+        const int xo = (int)(t % Wo); t /= Wo;
+        const int yo = (int)(t % Ho); t /= Ho;
+        const int zo = (int)(t % Do);
+        const int n = (int)(t / Do);
+        float acc = 0.0f;
+        for (int k = 0; k < 8; ++k) {
+            const int z = 2 * zo + (k >> 2), yy = 2 * yo + ((k >> 1) & 1), xx = 2 * xo + (k & 1);
+            acc += bf2f(x[((((int64_t)n * s.D + z) * s.H + yy) * s.W + xx) * C + c]);
+        }
+        y[i] = f2bf(acc * 0.125f);
     }
-    y[i] = f2bf(acc * 0.125f);
 }
 
 __global__ void k_pool2_bwd(const uint16_t *__restrict__ dy, Shape s, int C, uint16_t *__restrict__ dx)
 {
     const int64_t n_in = (int64_t)s.N * s.D * s.H * s.W * C;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_in) return;
-    const int c = (int)(i % C);
-    int64_t t = i / C;
-    const int xx = (int)(t % s.W); t /= s.W;
-    const int yy = (int)(t % s.H); t /= s.H;
-    const int z = (int)(t % s.D);
-    const int n = (int)(t / s.D);
-    const int64_t o = ((((int64_t)n * (s.D / 2) + z / 2) * (s.H / 2) + yy / 2) * (s.W / 2) + xx / 2) * C + c;
-    dx[i] = f2bf(bf2f(dy[o]) * 0.125f);
+    for (int64_t i = bmx_gtid(); i < n_in; i += bmx_gstride()) {
+        const int c = (int)(i % C);
+        int64_t t = i / C;
+        const int xx = (int)(t % s.W); t /= s.W;
+        const int yy = (int)(t % s.H); t /= s.H;
+        const int z = (int)(t % s.D);
+        const int n = (int)(t / s.D);
+        const int64_t o =
+            ((((int64_t)n * (s.D / 2) + z / 2) * (s.H / 2) + yy / 2) * (s.W / 2) + xx / 2) * C + c;
+        dx[i] = f2bf(bf2f(dy[o]) * 0.125f);
+    }
 }
 
 // Nearest 2x upsampling of x (shape s = the small grid) into y, and its
@@ -319,34 +320,34 @@ __global__ void k_up2(const uint16_t *__restrict__ x, Shape s, int C, uint16_t *
 {
     const int D2 = s.D * 2, H2 = s.H * 2, W2 = s.W * 2;
     const int64_t n_out = (int64_t)s.N * D2 * H2 * W2 * C;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_out) return;
-    const int c = (int)(i % C);
-    int64_t t = i / C;
-    const int xx = (int)(t % W2); t /= W2;
-    const int yy = (int)(t % H2); t /= H2;
-    const int z = (int)(t % D2);
-    const int n = (int)(t / D2);
-    y[i] = x[((((int64_t)n * s.D + z / 2) * s.H + yy / 2) * s.W + xx / 2) * C + c];
+    for (int64_t i = bmx_gtid(); i < n_out; i += bmx_gstride()) {
+        const int c = (int)(i % C);
+        int64_t t = i / C;
+        const int xx = (int)(t % W2); t /= W2;
+        const int yy = (int)(t % H2); t /= H2;
+        const int z = (int)(t % D2);
+        const int n = (int)(t / D2);
+        y[i] = x[((((int64_t)n * s.D + z / 2) * s.H + yy / 2) * s.W + xx / 2) * C + c];
+    }
 }
 
 __global__ void k_up2_bwd(const uint16_t *__restrict__ dy, Shape s, int C, uint16_t *__restrict__ dx)
 {
     const int64_t n_small = (int64_t)s.N * s.D * s.H * s.W * C;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_small) return;
-    const int c = (int)(i % C);
-    int64_t t = i / C;
-    const int xo = (int)(t % s.W); t /= s.W;
-    const int yo = (int)(t % s.H); t /= s.H;
-    const int zo = (int)(t % s.D);
-    const int n = (int)(t / s.D);
-    float acc = 0.0f;
-    for (int k = 0; k < 8; ++k) {
-        const int z = 2 * zo + (k >> 2), yy = 2 * yo + ((k >> 1) & 1), xx = 2 * xo + (k & 1);
-        acc += bf2f(dy[((((int64_t)n * 2 * s.D + z) * 2 * s.H + yy) * 2 * s.W + xx) * C + c]);
+    for (int64_t i = bmx_gtid(); i < n_small; i += bmx_gstride()) {
+        const int c = (int)(i % C);
+        int64_t t = i / C;
+        const int xo = (int)(t % s.W); t /= s.W;
+        const int yo = (int)(t % s.H); t /= s.H;
+        const int zo = (int)(t % s.D);
+        const int n = (int)(t / s.D);
+        float acc = 0.0f;
+        for (int k = 0; k < 8; ++k) {
+            const int z = 2 * zo + (k >> 2), yy = 2 * yo + ((k >> 1) & 1), xx = 2 * xo + (k & 1);
+            acc += bf2f(dy[((((int64_t)n * 2 * s.D + z) * 2 * s.H + yy) * 2 * s.W + xx) * C + c]);
+        }
+        dx[i] = f2bf(acc);
     }
-    dx[i] = f2bf(acc);
 }
 
 // y[v] = concat(a[v][0..Ca), b[v][0..Cb)); split is its backward.
@@ -354,27 +355,27 @@ __global__ void k_concat(const uint16_t *__restrict__ a, int Ca, const uint16_t 
                          int64_t nvox, uint16_t *__restrict__ y)
 {
     const int C = Ca + Cb;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nvox * C) return;
-    const int64_t v = i / C;
-    const int c = (int)(i - v * C);
-    y[i] = c < Ca ? a[v * Ca + c] : b[v * Cb + (c - Ca)];
+    for (int64_t i = bmx_gtid(); i < nvox * C; i += bmx_gstride()) {
+        const int64_t v = i / C;
+        const int c = (int)(i - v * C);
+        y[i] = c < Ca ? a[v * Ca + c] : b[v * Cb + (c - Ca)];
+    }
 }
 
 __global__ void k_split(const uint16_t *__restrict__ y, int64_t nvox, uint16_t *__restrict__ a, int Ca,
                         uint16_t *__restrict__ b, int Cb, int accumulate_a)
 {
     const int C = Ca + Cb;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nvox * C) return;
-    const int64_t v = i / C;
-    const int c = (int)(i - v * C);
-    if (c < Ca) {
-        float val = bf2f(y[i]);
-        if (accumulate_a) val += bf2f(a[v * Ca + c]);
-        a[v * Ca + c] = f2bf(val);
-    } else {
-        b[v * Cb + (c - Ca)] = y[i];
+    for (int64_t i = bmx_gtid(); i < nvox * C; i += bmx_gstride()) {
+        const int64_t v = i / C;
+        const int c = (int)(i - v * C);
+        if (c < Ca) {
+            float val = bf2f(y[i]);
+            if (accumulate_a) val += bf2f(a[v * Ca + c]);
+            a[v * Ca + c] = f2bf(val);
+        } else {
+            b[v * Cb + (c - Ca)] = y[i];
+        }
     }
 }
 
@@ -454,28 +455,26 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
                        float *__restrict__ v, uint16_t *__restrict__ pbf, int64_t n, float lr, float b1,
                        float b2, float eps, float c1, float c2, float wd)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i] + wd * p[i];
-    const float mi = b1 * m[i] + (1.0f - b1) * gi;
-    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-    p[i] = pi;
-    if (pbf) pbf[i] = f2bf(pi);
+    for (int64_t i = bmx_gtid(); i < n; i += bmx_gstride()) {
+        const float gi = g[i] + wd * p[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] = pi;
+        if (pbf) pbf[i] = f2bf(pi);
+    }
 }
 
 __global__ void k_add_bf16(uint16_t *__restrict__ y, const uint16_t *__restrict__ x, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = f2bf(bf2f(y[i]) + bf2f(x[i]));
+    for (int64_t i = bmx_gtid(); i < n; i += bmx_gstride()) y[i] = f2bf(bf2f(y[i]) + bf2f(x[i]));
 }
 
 __global__ void k_f2bf(const float *__restrict__ a, uint16_t *__restrict__ b, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = f2bf(a[i]);
+    for (int64_t i = bmx_gtid(); i < n; i += bmx_gstride()) b[i] = f2bf(a[i]);
 }
 
 template <int NB>
@@ -505,6 +504,8 @@ int check_conv_shape(int N, int D, int H, int W, int Ci, int Co)
     BMX_CHECK(N >= 1 && D >= 1 && H >= 1 && W >= 1, "conv3d: bad grid shape");
     BMX_CHECK(Ci >= 16 && Ci % 16 == 0 && Co >= 16 && Co % 16 == 0, "conv3d: channels must be multiples of 16");
     BMX_CHECK((int64_t)N * D * H * W * (Ci > Co ? Ci : Co) < ((int64_t)1 << 40), "conv3d: tensor too large");
+    // two work-items per voxel in the forward/dgrad grids; one dispatch holds < 2^32
+    BMX_CHECK((int64_t)N * D * H * W < ((int64_t)1 << 31) - 128, "conv3d: grid exceeds one dispatch");
     return BMX_OK;
 }
 
@@ -566,7 +567,7 @@ int bmx_pool2(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int C
     if (rc) return rc;
     BMX_CHECK(x && y && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "bmx_pool2: bad arguments");
     const int64_t n = (int64_t)N * (D / 2) * (H / 2) * (W / 2) * C;
-    k_pool2<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
+    k_pool2<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -577,7 +578,7 @@ int bmx_pool2_bwd(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, 
     if (rc) return rc;
     BMX_CHECK(dy && dx && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "bmx_pool2_bwd: bad arguments");
     const int64_t n = (int64_t)N * D * H * W * C;
-    k_pool2_bwd<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
+    k_pool2_bwd<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -588,7 +589,7 @@ int bmx_up2(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int C, 
     if (rc) return rc;
     BMX_CHECK(x && y, "bmx_up2: null pointer");
     const int64_t n = (int64_t)N * D * H * W * 8 * C;
-    k_up2<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
+    k_up2<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -599,7 +600,7 @@ int bmx_up2_bwd(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, in
     if (rc) return rc;
     BMX_CHECK(dy && dx, "bmx_up2_bwd: null pointer");
     const int64_t n = (int64_t)N * D * H * W * C;
-    k_up2_bwd<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
+    k_up2_bwd<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -609,7 +610,7 @@ int bmx_concat(bmx_ctx *ctx, const uint16_t *a, int Ca, const uint16_t *b, int C
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(a && b && y, "bmx_concat: null pointer");
-    k_concat<<<bmx_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(a, Ca, b, Cb, nvox, y);
+    k_concat<<<bmx_stride_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(a, Ca, b, Cb, nvox, y);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -620,7 +621,7 @@ int bmx_split(bmx_ctx *ctx, const uint16_t *y, int64_t nvox, uint16_t *a, int Ca
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(y && a && b, "bmx_split: null pointer");
-    k_split<<<bmx_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(y, nvox, a, Ca, b, Cb, accumulate_a);
+    k_split<<<bmx_stride_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(y, nvox, a, Ca, b, Cb, accumulate_a);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -633,6 +634,8 @@ int bmx_head(bmx_ctx *ctx, const uint16_t *x, int64_t nvox, int L, const float *
     if (rc) return rc;
     BMX_CHECK(x && W && b && L >= 1 && L <= 16, "bmx_head: bad arguments (L <= 16)");
     BMX_CHECK(!target || (dx && dW && db && loss), "bmx_head: training needs dx, dW, db, loss");
+    BMX_CHECK(nvox >= 0 && nvox < ((int64_t)1 << 32) - 256, "bmx_head: nvox %lld exceeds one dispatch",
+              (long long)nvox);
     k_head<<<bmx_blocks(nvox, 256), 256, 0, ctx->stream>>>(x, nvox, L, W, b, prob, target, vmask, scale, dx,
                                                           dW, db, loss);
     BMX_LAUNCH_CHECK();
@@ -646,7 +649,7 @@ int bmx_adam(bmx_ctx *ctx, float *p, const float *g, float *m, float *v, uint16_
     if (rc) return rc;
     BMX_CHECK(p && g && m && v && step >= 1, "bmx_adam: bad arguments");
     const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
-    k_adam<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(p, g, m, v, pbf, n, lr, beta1, beta2, eps, c1, c2,
+    k_adam<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(p, g, m, v, pbf, n, lr, beta1, beta2, eps, c1, c2,
                                                        weight_decay);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
@@ -657,7 +660,7 @@ int bmx_relu_mask(bmx_ctx *ctx, uint16_t *g, const uint16_t *a, int64_t n)
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(g && a, "bmx_relu_mask: null pointer");
-    k_relu_mask<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(g, a, n);
+    k_relu_mask<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(g, a, n);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -667,7 +670,7 @@ int bmx_add_bf16(bmx_ctx *ctx, uint16_t *y, const uint16_t *x, int64_t n)
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(y && x, "bmx_add_bf16: null pointer");
-    k_add_bf16<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(y, x, n);
+    k_add_bf16<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(y, x, n);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -677,7 +680,7 @@ int bmx_f32_to_bf16(bmx_ctx *ctx, const float *a, uint16_t *b, int64_t n)
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(a && b, "bmx_f32_to_bf16: null pointer");
-    k_f2bf<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(a, b, n);
+    k_f2bf<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(a, b, n);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }

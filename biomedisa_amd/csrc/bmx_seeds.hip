@@ -350,7 +350,7 @@ extern "C" int bmx_select_seeds(bmx_ctx *ctx, const bmx_seed *seeds, int64_t nse
     if (rc) return rc;
     BMX_CHECK(nout != nullptr, "bmx_select_seeds: nout is null");
     BMX_CHECK(nseeds == 0 || (seeds && mask && out), "bmx_select_seeds: null pointer");
-    BMX_CHECK(nseeds >= 0, "bmx_select_seeds: negative nseeds");
+    BMX_CHECK(nseeds >= 0 && nseeds < ((int64_t)1 << 32) - 256, "bmx_select_seeds: nseeds out of range");
     BMX_HIP(hipMemsetAsync(ctx->d_counter + 2, 0, sizeof(uint64_t), ctx->stream));
     if (nseeds > 0) {
         k_select_seeds<<<bmx_blocks(nseeds, 256), 256, 0, ctx->stream>>>(

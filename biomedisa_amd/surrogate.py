@@ -9,9 +9,10 @@ concatenations, 1x1x1 softmax head):
     [a0, up d1](48) -dec0-> d0(16) -head-> softmax over L = 14 label indices
 
 Input channels (bmx_net_input): image/127.5 - 1, seed indicator, one-hot of
-the label index on seed voxels.  The network is trained to reproduce the
-plain walker's per-voxel label distribution (hits / sum of hits) on seeded
-synthetic volumes.  BF16 weights and activations, FP32 accumulation, FP32
+the label index on seed voxels.  The network is trained with cross-entropy
+against the plain walker's per-voxel label (argmax of its hits) on seeded
+synthetic volumes, so its top probability estimates how likely the plain
+walker is to agree with its proposal.  BF16 weights and activations, FP32 accumulation, FP32
 master weights and gradients; every forward/backward op is a libbmx_hip
 kernel (bmx_net.hip).  No counterpart in the reference.
 """
@@ -294,8 +295,12 @@ class Sample:
         self.x, self.target, self.vmask, self.grid = x, target, vmask, grid
 
 
-def make_sample(eng, vol, nbrw=10, sorw=4000, key=1):
-    """Walk a synthetic volume with the plain walker and build its training sample."""
+def make_sample(eng, vol, nbrw=10, sorw=4000, key=1, hard=False):
+    """Walk a synthetic volume with the plain walker and build its training sample.
+
+    hard=True trains towards the one-hot of the walker's argmax, so the network's
+    top probability estimates how likely the plain walker is to give that label;
+    hard=False towards the walker's per-voxel hit fractions."""
     from . import helpers as H
     from .interpolation import DeviceVolume, run_pipeline
     bm = H.Biomedisa()
@@ -317,20 +322,21 @@ def make_sample(eng, vol, nbrw=10, sorw=4000, key=1):
     _hip.call("bmx_net_input", eng.ctx, dv.img.ptr, dv.img_dtype, dv.lab.ptr, flags.ptr, Z, Y, X, *grid, x.ptr)
     target = eng.alloc(ng * L_MAX * 4)
     vmask = eng.alloc(ng)
-    _hip.call("bmx_hits_target", eng.ctx, out["hits"].ptr, bm.nol, Z, Y, X, *grid, L_MAX, target.ptr, vmask.ptr)
+    _hip.call("bmx_hits_target", eng.ctx, out["hits"].ptr, bm.nol, Z, Y, X, *grid, L_MAX, int(hard), target.ptr,
+              vmask.ptr)
     eng.sync()
     return Sample(x, target, vmask, grid)
 
 
 def train(n_volumes=2000, size=64, iters=1500, batch=4, lr=2e-3, seed=0, out=WEIGHTS_PATH, log=print,
-          device=0):
+          device=0, hard=False):
     """Train the network on the GPU with the project's own kernels; save FP32 weights."""
     eng = _hip.engine(device)
     t0 = time.perf_counter()
     samples = []
     s = seed
     while len(samples) < n_volumes:
-        smp = make_sample(eng, training_volume(s, size))
+        smp = make_sample(eng, training_volume(s, size), hard=hard)
         s += 1
         if smp is not None:
             samples.append(smp)
@@ -383,6 +389,7 @@ if __name__ == "__main__":
     ap.add_argument("--size", type=int, default=64)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=WEIGHTS_PATH)
+    ap.add_argument("--target", choices=["soft", "hard"], default="soft")
     a = ap.parse_args()
-    train(a.volumes, a.size, a.iters, a.batch, a.lr, out=a.out,
+    train(a.volumes, a.size, a.iters, a.batch, a.lr, out=a.out, hard=a.target == "hard",
           log=lambda m: print("[train] " + m, flush=True))

@@ -194,10 +194,13 @@ int bmx_adam(bmx_ctx *ctx, float *p, const float *g, float *m, float *v, uint16_
 int bmx_f32_to_bf16(bmx_ctx *ctx, const float *a, uint16_t *b, int64_t n);
 /* y += x elementwise (bf16). */
 int bmx_add_bf16(bmx_ctx *ctx, uint16_t *y, const uint16_t *x, int64_t n);
-/* Soft targets t[v][l] = hits[l][v] / sum_l hits[.][v] on a padded grid;
- * vmask = 1 where the voxel is inside (Z,Y,X) and some walker arrived. */
+/* Training targets on a padded grid from walker hit counts.
+ * hard = 0: t[v][l] = hits[l][v] / sum_l hits[.][v]; vmask = 1 where the voxel is
+ *           inside (Z,Y,X) and some walker arrived.
+ * hard = 1: t[v] = one-hot of the walker's argmax (first maximum; index 0 when all
+ *           counts are 0, as bmx_argmax); vmask = 1 inside (Z,Y,X). */
 int bmx_hits_target(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int Y, int X, int Zp,
-                    int Yp, int Xp, int L, float *target, uint8_t *vmask);
+                    int Yp, int Xp, int L, int hard, float *target, uint8_t *vmask);
 
 /* Surrogate-guided walking (no counterpart in the reference). */
 /* Network input on a padded grid (Zp,Yp,Xp) >= (Z,Y,X), 16 bf16 channels:

@@ -1,0 +1,169 @@
+"""Surrogate-guided kernels (bmx_guide.hip) against oracle/guide.py, bit for
+bit, and the exactness properties of the guided pipeline."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from biomedisa_amd import _hip
+from oracle import guide as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _prep(vol):
+    from biomedisa_amd import helpers as H
+    bm = H.Biomedisa()
+    bm.data, bm.labelData, bm.only, bm.ignore, bm.success = vol.img, vol.labels, "all", "none", True
+    bm = H.pre_processing(bm)
+    img = np.ascontiguousarray(np.pad(H.normalize_image(bm.data), 1))
+    lab = np.ascontiguousarray(np.pad(bm.labelData, 1))
+    return img, H.label_index_volume(lab, bm.allLabels), bm.allLabels.astype(np.uint8), bm.nol
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_net_input_bitexact(eng, dtype):
+    rng = np.random.default_rng(1)
+    Z, Y, X = 11, 13, 9
+    grid = (16, 16, 16)
+    img = rng.integers(0, 256, (Z, Y, X)).astype(dtype)
+    if dtype == np.float32:
+        img += rng.random((Z, Y, X)).astype(np.float32)
+    lab = rng.integers(0, 17, (Z, Y, X)).astype(np.uint8)
+    lab[rng.random((Z, Y, X)) < 0.1] = G.LABEL_NONE
+    flags = (rng.random(Z + Y + X) < 0.25).astype(np.uint8)
+    ib, lb, fb = eng.upload(img), eng.upload(lab), eng.upload(flags)
+    out = eng.alloc(int(np.prod(grid)) * 16 * 2)
+    _hip.call("bmx_net_input", eng.ctx, ib.ptr, _hip.IMG_U8 if dtype == np.uint8 else _hip.IMG_F32, lb.ptr,
+              fb.ptr, Z, Y, X, *grid, out.ptr)
+    np.testing.assert_array_equal(out.download(grid + (16,), np.uint16), G.net_input(img, lab, flags, grid))
+
+
+@pytest.mark.parametrize("L,thr", [(4, 0.9), (9, 0.5), (14, 0.95)])
+def test_head_predict_bitexact(eng, L, thr):
+    rng = np.random.default_rng(L)
+    dims, grid = (9, 10, 13), (16, 16, 16)
+    x = (rng.standard_normal(grid + (16,)) * 1.5).astype(np.float32)
+    xb = (x.view(np.uint32) >> 16).astype(np.uint16)
+    W = (rng.standard_normal((L, 16)) * 0.5).astype(np.float32)
+    b = rng.standard_normal(L).astype(np.float32)
+    xbuf, Wb, bb = eng.upload(xb), eng.upload(W), eng.upload(b)
+    V = int(np.prod(dims))
+    pred, unc = eng.alloc(V), eng.alloc(V)
+    _hip.call("bmx_head_predict", eng.ctx, xbuf.ptr, *dims, *grid, L, Wb.ptr, bb.ptr, C.c_float(thr), pred.ptr,
+              unc.ptr)
+    p_ref, u_ref = G.head_predict(xb, dims, W, b, thr)
+    np.testing.assert_array_equal(pred.download(dims, np.uint8), p_ref)
+    np.testing.assert_array_equal(unc.download(dims, np.uint8), u_ref)
+    assert 0 < u_ref.mean() < 1
+
+
+@pytest.mark.parametrize("brad,srad", [(0, 0), (1, 2), (3, 16), (2, 40)])
+def test_guided_masks_bitexact(eng, brad, srad):
+    rng = np.random.default_rng(brad * 10 + srad)
+    dims = (30, 37, 41)
+    pred = np.zeros(dims, np.uint8)
+    for k in range(1, 5):
+        c = rng.integers(0, dims)
+        r = rng.integers(3, 9, 3)
+        sl = tuple(slice(max(0, c[i] - r[i]), c[i] + r[i]) for i in range(3))
+        pred[sl] = k
+    unc = (rng.random(dims) < 0.001).astype(np.uint8)
+    V = int(np.prod(dims))
+    pb, ub = eng.upload(pred), eng.upload(unc)
+    U, need, tmp = eng.alloc(V), eng.alloc(V), eng.alloc(V)
+    _hip.call("bmx_guided_masks", eng.ctx, pb.ptr, ub.ptr, *dims, brad, srad, U.ptr, need.ptr, tmp.ptr)
+    U_ref, need_ref = G.guided_masks(pred, unc, brad, srad)
+    np.testing.assert_array_equal(U.download(dims, np.uint8), U_ref)
+    np.testing.assert_array_equal(need.download(dims, np.uint8), need_ref)
+
+
+def test_merge_guided_select_labels_bitexact(eng):
+    rng = np.random.default_rng(7)
+    nl, dims = 6, (7, 9, 11)
+    V = int(np.prod(dims))
+    hits = rng.integers(0, 4, (nl,) + dims).astype(np.uint32)
+    values = np.array([0, 3, 5, 8, 100, 255], np.uint8)
+    pred = rng.integers(0, nl, dims).astype(np.uint8)
+    U = (rng.random(dims) < 0.4).astype(np.uint8)
+    hb, pb, Ub = eng.upload(hits), eng.upload(pred), eng.upload(U)
+    out = eng.alloc(V)
+    _hip.call("bmx_merge_guided", eng.ctx, hb.ptr, nl, V, values.ctypes.data, pb.ptr, Ub.ptr, out.ptr)
+    np.testing.assert_array_equal(out.download(dims, np.uint8), G.merge_guided(hits, values, pred, U))
+    a = rng.integers(0, 256, dims).astype(np.uint8)
+    ab = eng.upload(a)
+    _hip.call("bmx_select_labels", eng.ctx, ab.ptr, pb.ptr, Ub.ptr, nl, values.ctypes.data, V, out.ptr)
+    np.testing.assert_array_equal(out.download(dims, np.uint8), G.select_labels(a, pred, U, values))
+
+
+@pytest.mark.parametrize("hard", [0, 1])
+def test_hits_target_bitexact(eng, hard):
+    rng = np.random.default_rng(8 + hard)
+    nl, dims, grid, L = 5, (6, 10, 7), (8, 16, 8), 14
+    hits = rng.integers(0, 3, (nl,) + dims).astype(np.uint32)
+    hits[:, 0, :3] = 0
+    hb = eng.upload(hits)
+    ng = int(np.prod(grid))
+    t, m = eng.alloc(ng * L * 4), eng.alloc(ng)
+    _hip.call("bmx_hits_target", eng.ctx, hb.ptr, nl, *dims, *grid, L, hard, t.ptr, m.ptr)
+    t_ref, m_ref = G.hits_target(hits, grid, L, hard)
+    np.testing.assert_array_equal(t.download(grid + (L,), np.float32), t_ref)
+    np.testing.assert_array_equal(m.download(grid, np.uint8), m_ref)
+
+
+def test_select_seeds_set(eng):
+    from biomedisa_amd.synth import ellipsoids_volume
+    vol = ellipsoids_volume(shape=(40, 44, 48), seed=5, semi=(6, 12), every=6)
+    img, labidx, _, _ = _prep(vol)
+    dims = img.shape
+    ib, lb = eng.upload(img), eng.upload(labidx)
+    flags = eng.plane_flags(lb, dims, False)
+    seeds, n = eng.seeds(lb, ib, _hip.IMG_U8, dims, flags)
+    rng = np.random.default_rng(9)
+    mask = (rng.random(dims) < 0.3).astype(np.uint8)
+    sub, ns = eng.select_seeds(seeds, n, eng.upload(mask))
+    all_s = seeds.download((n,), _hip.SEED_DTYPE)
+    got = np.sort(sub.download((ns,), _hip.SEED_DTYPE), order="pos")
+    np.testing.assert_array_equal(got, G.select_seeds(all_s, mask))
+
+
+def _pipelines(eng, vol, nbrw, sorw, smooth, net, **guide):
+    from biomedisa_amd.interpolation import DeviceVolume, run_pipeline, run_pipeline_guided
+    img, labidx, values, nol = _prep(vol)
+    dv = DeviceVolume(eng, img, labidx)
+    p = run_pipeline(dv, nol, values, nbrw=nbrw, sorw=sorw, allaxis=vol.allaxis, smooth=smooth)
+    g = run_pipeline_guided(dv, nol, values, net, nbrw=nbrw, sorw=sorw, allaxis=vol.allaxis, smooth=smooth,
+                            keep=True, **guide)
+    dl = lambda b: b.download(dv.dims, np.uint8)  # noqa: E731
+    return p, g, dl, values
+
+
+def test_guided_equals_plain_when_all_uncertain(eng):
+    """thr above 1 marks every voxel uncertain: every seed walks and the guided
+    result is the plain result, 'smooth' included."""
+    from biomedisa_amd import surrogate as S
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(48, 52, 56), seed=11, n_struct=4, radius=(4, 14), every=7, allaxis=True)
+    p, g, dl, _ = _pipelines(eng, vol, 3, 150, 5, S.Net(eng, seed=1), thr=1.01, brad=1, srad=1)
+    assert g["walked_seeds"] == g["nseeds"] == p["nseeds"]
+    assert g["accepted"] == p["accepted"]
+    np.testing.assert_array_equal(dl(g["regular"]), dl(p["regular"]))
+    np.testing.assert_array_equal(dl(g["smooth"]), dl(p["smooth"]))
+
+
+@pytest.mark.parametrize("trained", [False, True])
+def test_guided_exact_on_uncertain_set(eng, trained):
+    """With srad >= sorw no skipped seed's walk can reach U (a step moves one
+    voxel along one axis), so on U the guided labels are the plain labels, and
+    off U they are the network's."""
+    from biomedisa_amd import surrogate as S
+    from biomedisa_amd.synth import structures_volume
+    net = S.net(0) if trained else S.Net(eng, seed=2)
+    vol = structures_volume(shape=(56, 48, 60), seed=12, n_struct=5, radius=(4, 16), every=8, allaxis=False)
+    sorw = 24
+    p, g, dl, values = _pipelines(eng, vol, 4, sorw, 0, net, thr=0.9, brad=2, srad=sorw)
+    U = dl(g["U"]).astype(bool)
+    greg, preg = dl(g["regular"]), dl(p["regular"])
+    np.testing.assert_array_equal(greg[U], preg[U])
+    np.testing.assert_array_equal(greg[~U], values[dl(g["pred"])][~U])
+    assert g["walked_seeds"] <= g["nseeds"]

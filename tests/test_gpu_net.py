@@ -192,3 +192,33 @@ def test_whole_network_gradients_match_oracle(eng):
         cos = (got * ref).sum() / (np.linalg.norm(got) * np.linalg.norm(ref) + 1e-30)
         rel = np.linalg.norm(got - ref) / (np.linalg.norm(ref) + 1e-30)
         assert cos > 0.995 and rel < 0.1, (k, cos, rel)
+
+
+def test_concat_past_2pow32_elements(eng):
+    """Element-wise kernels stride over ranges beyond 2^32 elements (one dispatch
+    covers fewer work-items): upsample two tensors on the device and concatenate
+    them into 4.4e9 elements, then check the tail of the result."""
+    small = (1, 112, 128, 800)
+    nsmall = int(np.prod(small))
+    nbig = nsmall * 8
+    assert nbig * 48 > 2 ** 32
+    rng = np.random.default_rng(13)
+    xa = rng.integers(0, 65536, (nsmall, 16), dtype=np.uint16)
+    xb = rng.integers(0, 65536, (nsmall, 32), dtype=np.uint16)
+    ba, bb = eng.upload(xa), eng.upload(xb)
+    ua, ub = eng.alloc(nbig * 16 * 2), eng.alloc(nbig * 32 * 2)
+    _hip.call("bmx_up2", eng.ctx, ba.ptr, *small, 16, ua.ptr)
+    _hip.call("bmx_up2", eng.ctx, bb.ptr, *small, 32, ub.ptr)
+    del ba, bb
+    y = eng.alloc(nbig * 48 * 2)
+    _hip.call("bmx_concat", eng.ctx, ua.ptr, 16, ub.ptr, 32, nbig, y.ptr)
+    ntail = 1 << 16
+    tail = np.empty((ntail, 48), np.uint16)
+    _hip._check(_hip.load().bmx_download(eng.ctx, tail.ctypes.data, y.ptr + (nbig - ntail) * 96, tail.nbytes))
+    v = np.arange(nbig - ntail, nbig)
+    _, D, H, W = small
+    xx, t = v % (2 * W), v // (2 * W)
+    yy, zz = t % (2 * H), t // (2 * H)
+    src = ((zz // 2) * H + yy // 2) * W + xx // 2
+    np.testing.assert_array_equal(tail[:, :16], xa[src])
+    np.testing.assert_array_equal(tail[:, 16:], xb[src])
