@@ -7,17 +7,22 @@
 // same floats with half the traffic.  Interior expressions are written in
 // the reference's operation order (the build disables FP contraction).
 //
-// Tiling: a 64x4 (x,y) tile per block marches along a chunk of z planes; the
-// planes z-1, z, z+1 of the tile plus a 1-voxel halo rotate through three LDS
-// slots, so each input voxel is read from memory about once per pass.
+// Tiling: a 64x16 (x,y) tile per block (each thread owns 4 rows of a column)
+// marches along a chunk of 32 z planes; the planes z-1, z, z+1 of the tile plus
+// a 1-voxel halo rotate through three LDS slots and the next plane is fetched
+// into registers during the computation, so each input voxel is read from
+// memory about 1.15 times per pass.
 #include "bmx_internal.h"
 
 namespace {
 
-constexpr int TX = 64;
-constexpr int TY = 4;
-constexpr int kSmoothThreads = TX * TY;
-constexpr int kZChunk = 64;
+constexpr int TX = 64;            // x extent of a tile (one wave per row group)
+constexpr int TYT = 4;            // thread rows per block
+constexpr int RY = 4;             // output rows per thread (register blocking in y)
+constexpr int TY = TYT * RY;      // y extent of a tile
+constexpr int kSmoothThreads = TX * TYT;
+constexpr int kZChunk = 32;
+constexpr int LW = TX + 2, LH = TY + 2;   // LDS plane with a 1-voxel halo
 
 template <typename T> __device__ __forceinline__ float ldf(const T *p, int64_t i);
 template <> __device__ __forceinline__ float ldf<float>(const float *p, int64_t i) { return p[i]; }
@@ -26,69 +31,121 @@ template <> __device__ __forceinline__ float ldf<uint32_t>(const uint32_t *p, in
     return (float)p[i];
 }
 
+// Per-thread share of one tile plane plus halo: RY interior rows at column tx,
+// and up to two halo values (rows 0 / LH-1 for thread rows 0 / 1, the left and
+// right halo columns for thread row 2).
+struct PlaneRegs {
+    float v[RY];
+    float h0, h1;
+};
+
+template <typename Tin>
+__device__ __forceinline__ void fetch_plane(const Tin *__restrict__ in, int zz, int Z, int Y, int X, int x0,
+                                            int y0, int tx, int ty, PlaneRegs &r)
+{
+    zz = min(max(zz, 0), Z - 1);
+    const int64_t base = (int64_t)zz * Y * X;
+    const int gx = min(max(x0 + tx, 0), X - 1);
+#pragma unroll
+    for (int k = 0; k < RY; ++k) {
+        const int gy = min(max(y0 + ty * RY + k, 0), Y - 1);
+        r.v[k] = ldf(in, base + (int64_t)gy * X + gx);
+    }
+    r.h0 = 0.0f;
+    r.h1 = 0.0f;
+    if (ty < 2) {
+        // halo rows: ly = 0 (gy = y0-1) for ty 0, ly = LH-1 (gy = y0+TY) for ty 1
+        const int gy = min(max(ty == 0 ? y0 - 1 : y0 + TY, 0), Y - 1);
+        r.h0 = ldf(in, base + (int64_t)gy * X + gx);
+    } else if (ty == 2 && tx < 2 * LH) {
+        // halo columns: lx = 0 (gx = x0-1) and lx = LW-1 (gx = x0+TX), rows ly = 0..LH-1
+        const int ly = tx % LH;
+        const int gy = min(max(y0 + ly - 1, 0), Y - 1);
+        const int gxh = min(max(tx < LH ? x0 - 1 : x0 + TX, 0), X - 1);
+        r.h0 = ldf(in, base + (int64_t)gy * X + gxh);
+    }
+}
+
+__device__ __forceinline__ void store_plane(float (*s)[LW], int tx, int ty, const PlaneRegs &r)
+{
+#pragma unroll
+    for (int k = 0; k < RY; ++k) s[1 + ty * RY + k][1 + tx] = r.v[k];
+    if (ty < 2) {
+        s[ty == 0 ? 0 : LH - 1][1 + tx] = r.h0;
+    } else if (ty == 2 && tx < 2 * LH) {
+        s[tx % LH][tx < LH ? 0 : LW - 1] = r.h0;
+    }
+}
+
+// One round phi_{t+1} = phi_t + 0.01 kappa(phi_t) over a 64x16 (x,y) tile and
+// a chunk of z planes.  Planes z-1, z, z+1 live in three LDS slots; the plane
+// z+2 is fetched into registers while plane z is computed.  Each thread
+// computes RY consecutive rows of one column.
 template <typename Tin>
 __global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__restrict__ in,
                                                                float *__restrict__ out, int Z, int Y,
                                                                int X)
 {
-    __shared__ float s[3][TY + 2][TX + 2];
+    __shared__ float s[3][LH][LW];
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int x = blockIdx.x * TX + tx, y = blockIdx.y * TY + ty;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x = x0 + tx;
     const int z_begin = blockIdx.z * kZChunk;
     const int z_end = min(Z, z_begin + kZChunk);
     const int64_t sy = X, sz = (int64_t)X * Y;
 
-    auto load_plane = [&](int slot, int zz) {
-        zz = min(max(zz, 0), Z - 1);
-        for (int k = threadIdx.x; k < (TY + 2) * (TX + 2); k += kSmoothThreads) {
-            const int ly = k / (TX + 2), lx = k - ly * (TX + 2);
-            const int gy = min(max((int)(blockIdx.y * TY) + ly - 1, 0), Y - 1);
-            const int gx = min(max((int)(blockIdx.x * TX) + lx - 1, 0), X - 1);
-            s[slot][ly][lx] = ldf(in, (int64_t)zz * sz + (int64_t)gy * sy + gx);
-        }
-    };
-
+    PlaneRegs r;
+    fetch_plane(in, z_begin - 1, Z, Y, X, x0, y0, tx, ty, r);
+    store_plane(s[0], tx, ty, r);
+    fetch_plane(in, z_begin, Z, Y, X, x0, y0, tx, ty, r);
+    store_plane(s[1], tx, ty, r);
+    fetch_plane(in, z_begin + 1, Z, Y, X, x0, y0, tx, ty, r);
     int sm = 0, sc = 1, sp = 2;
-    load_plane(sm, z_begin - 1);
-    load_plane(sc, z_begin);
-    const bool inside_xy = x > 0 && x < X - 1 && y > 0 && y < Y - 1;
-    const int ly = ty + 1, lx = tx + 1;
+    const bool inside_x = x > 0 && x < X - 1;
+    const int lx = tx + 1;
     for (int z = z_begin; z < z_end; ++z) {
-        load_plane(sp, z + 1);
+        store_plane(s[sp], tx, ty, r);
         __syncthreads();
-        if (x < X && y < Y) {
-            const float pc = s[sc][ly][lx];
-            float res = pc;
-            if (inside_xy && z > 0 && z < Z - 1) {
-                const float eps = 1e-10f;
-                const float xm = s[sc][ly][lx - 1], xp = s[sc][ly][lx + 1];
-                const float ym = s[sc][ly - 1][lx], yp = s[sc][ly + 1][lx];
-                const float zm = s[sm][ly][lx], zp = s[sp][ly][lx];
-                const float dx = (xm - xp) * 0.5f;
-                const float dxx = (xm - 2.0f * pc + xp);
-                const float dx2 = dx * dx;
-                const float dy = (ym - yp) * 0.5f;
-                const float dyy = (ym - 2.0f * pc + yp);
-                const float dy2 = dy * dy;
-                const float dz = (zm - zp) * 0.5f;
-                const float dzz = (zm - 2.0f * pc + zp);
-                const float dz2 = dz * dz;
-                const float dxy = (s[sc][ly - 1][lx - 1] + s[sc][ly + 1][lx + 1]
-                                   - s[sc][ly - 1][lx + 1] - s[sc][ly + 1][lx - 1]) * 0.25f;
-                const float dxz = (s[sp][ly][lx - 1] + s[sm][ly][lx + 1]
-                                   - s[sp][ly][lx + 1] - s[sm][ly][lx - 1]) * 0.25f;
-                const float dyz = (s[sp][ly - 1][lx] + s[sm][ly + 1][lx]
-                                   - s[sp][ly + 1][lx] - s[sm][ly - 1][lx]) * 0.25f;
-                const float curv = (dxx * (dy2 + dz2)
-                                    + dyy * (dx2 + dz2)
-                                    + dzz * (dx2 + dy2)
-                                    - 2.0f * dx * dy * dxy
-                                    - 2.0f * dx * dz * dxz
-                                    - 2.0f * dy * dz * dyz)
-                                   / (dx2 + dy2 + dz2 + eps);
-                res = pc + 0.01f * curv;
+        if (z + 1 < z_end) fetch_plane(in, z + 2, Z, Y, X, x0, y0, tx, ty, r);
+        const bool inside_z = z > 0 && z < Z - 1;
+#pragma unroll
+        for (int k = 0; k < RY; ++k) {
+            const int ly = 1 + ty * RY + k;
+            const int y = y0 + ty * RY + k;
+            if (x < X && y < Y) {
+                const float pc = s[sc][ly][lx];
+                float res = pc;
+                if (inside_x && inside_z && y > 0 && y < Y - 1) {
+                    const float eps = 1e-10f;
+                    const float xm = s[sc][ly][lx - 1], xp = s[sc][ly][lx + 1];
+                    const float ym = s[sc][ly - 1][lx], yp = s[sc][ly + 1][lx];
+                    const float zm = s[sm][ly][lx], zp = s[sp][ly][lx];
+                    const float dx = (xm - xp) * 0.5f;
+                    const float dxx = (xm - 2.0f * pc + xp);
+                    const float dx2 = dx * dx;
+                    const float dy = (ym - yp) * 0.5f;
+                    const float dyy = (ym - 2.0f * pc + yp);
+                    const float dy2 = dy * dy;
+                    const float dz = (zm - zp) * 0.5f;
+                    const float dzz = (zm - 2.0f * pc + zp);
+                    const float dz2 = dz * dz;
+                    const float dxy = (s[sc][ly - 1][lx - 1] + s[sc][ly + 1][lx + 1]
+                                       - s[sc][ly - 1][lx + 1] - s[sc][ly + 1][lx - 1]) * 0.25f;
+                    const float dxz = (s[sp][ly][lx - 1] + s[sm][ly][lx + 1]
+                                       - s[sp][ly][lx + 1] - s[sm][ly][lx - 1]) * 0.25f;
+                    const float dyz = (s[sp][ly - 1][lx] + s[sm][ly + 1][lx]
+                                       - s[sp][ly + 1][lx] - s[sm][ly - 1][lx]) * 0.25f;
+                    const float curv = (dxx * (dy2 + dz2)
+                                        + dyy * (dx2 + dz2)
+                                        + dzz * (dx2 + dy2)
+                                        - 2.0f * dx * dy * dxy
+                                        - 2.0f * dx * dz * dxz
+                                        - 2.0f * dy * dz * dyz)
+                                       / (dx2 + dy2 + dz2 + eps);
+                    res = pc + 0.01f * curv;
+                }
+                out[(int64_t)z * sz + (int64_t)y * sy + x] = res;
             }
-            out[(int64_t)z * sz + (int64_t)y * sy + x] = res;
         }
         __syncthreads();
         const int t = sm; sm = sc; sc = sp; sp = t;

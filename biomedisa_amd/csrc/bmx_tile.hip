@@ -131,16 +131,19 @@ __device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
 // One Philox block (4 steps) of one walker.  Local coordinates (lz,ly,lx)
 // address the LDS tile, whose voxel (1,1,1) is the region origin; the region
 // proper is [1,R]^3 and the tile adds one voxel on each side for neighbour
-// reads.  li is the walker's index in the tile.  EDGE: the tile may reach the
-// volume boundary, so moves are checked against [lmin, lmax] per axis (the
-// local range of interior voxels); otherwise every move is interior.
+// reads.  li is the walker's index in the tile; the global voxel index is
+// rebuilt from the coordinates only on the rare steps that leave the tile.
+// EDGE: the tile may reach the volume boundary, so moves are checked against
+// [lmin, lmax] per axis (the local range of interior voxels); otherwise every
+// move is interior.  A step that starts inside the region lands inside the
+// tile, so its hit always goes to the LDS counters.
 template <typename T, bool EDGE>
 __device__ __forceinline__ void tile_block(const T *__restrict__ img, const WalkGeom &g, const T *simg,
                                            uint32_t *shits, uint32_t *__restrict__ hg, int sorw,
-                                           const uint32_t rr[4], float B, float div1, int &step,
-                                           uint32_t &pos, int &li, int &lz, int &ly, int &lx,
-                                           int lmin0, int lmin1, int lmin2, int lmax0, int lmax1,
-                                           int lmax2, uint64_t &acc)
+                                           const uint32_t rr[4], float B, float div1, int &step, int &li,
+                                           int &lz, int &ly, int &lx, int bz, int by, int bx, int lmin0,
+                                           int lmin1, int lmin2, int lmax0, int lmax1, int lmax2,
+                                           uint32_t &acc)
 {
     constexpr int R = TileCfg<T>::C + 2 * TileCfg<T>::H, RI = R + 2;
 #pragma unroll
@@ -157,12 +160,13 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
                 a4 = (float)simg[li + 1];
                 a5 = (float)simg[li - 1];
             } else {
-                a0 = bmx_ld(img, (int64_t)pos + g.sz);
-                a1 = bmx_ld(img, (int64_t)pos - g.sz);
-                a2 = bmx_ld(img, (int64_t)pos + g.sy);
-                a3 = bmx_ld(img, (int64_t)pos - g.sy);
-                a4 = bmx_ld(img, (int64_t)pos + 1);
-                a5 = bmx_ld(img, (int64_t)pos - 1);
+                const int64_t pos = ((int64_t)(bz + lz) * g.sz + (int64_t)(by + ly) * g.sy) + (bx + lx);
+                a0 = bmx_ld(img, pos + g.sz);
+                a1 = bmx_ld(img, pos - g.sz);
+                a2 = bmx_ld(img, pos + g.sy);
+                a3 = bmx_ld(img, pos - g.sy);
+                a4 = bmx_ld(img, pos + 1);
+                a5 = bmx_ld(img, pos - 1);
             }
             const int d = choose(weight_nonpos(B, a0, div1), weight_nonpos(B, a1, div1),
                                  weight_nonpos(B, a2, div1), weight_nonpos(B, a3, div1),
@@ -182,12 +186,14 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
                 lx += axis == 2 ? sgn : 0;
                 const int lmag = axis == 0 ? RI * RI : (axis == 1 ? RI : 1);
                 li += sgn * lmag;
-                pos += step_delta(d, g.sz, g.sy);
                 ++acc;
-                if (max(max((unsigned)lz, (unsigned)ly), (unsigned)lx) < (unsigned)RI)
+                if (in || max(max((unsigned)lz, (unsigned)ly), (unsigned)lx) < (unsigned)RI) {
                     atomicAdd(&shits[li], 1u);
-                else
+                } else {
+                    const int64_t pos =
+                        ((int64_t)(bz + lz) * g.sz + (int64_t)(by + ly) * g.sy) + (bx + lx);
                     atomicAdd(&hg[pos], 1u);
+                }
             }
             ++step;
         }
@@ -237,9 +243,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
                       (lmax1 < R + 4) | (lmax2 < R + 4);
 
     uint32_t *__restrict__ hg = hits + (int64_t)label * g.V;
-    uint64_t acc = 0;
+    uint32_t acc = 0;  // accepted moves of this thread in this launch (< 2^32: the launch is bounded)
     bool have = false;
-    uint32_t j = 0, pos = 0, p0 = 0, seed = 0, wl = 0;
+    uint32_t j = 0, p0 = 0, seed = 0, wl = 0;
     int step = 0, lz = 0, ly = 0, lx = 0, li = 0, taken = 0;
     float B = 0.0f, div1 = 0.0f;
     while (true) {
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
             j = atomicAdd(&next, 1u);
             if (j >= cnt) break;
             const uint4 w4 = st[off + j];
-            pos = w4.x;
+            const uint32_t pos = w4.x;
             seed = w4.y;
             wl = w4.z;
             step = (int)w4.w;
@@ -269,20 +275,21 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
         const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
         const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
         if (edge)
-            tile_block<T, true>(img, g, simg, shits, hg, sorw, rr, B, div1, step, pos, li, lz, ly, lx,
+            tile_block<T, true>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by, bx,
                                 lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, acc);
         else
-            tile_block<T, false>(img, g, simg, shits, hg, sorw, rr, B, div1, step, pos, li, lz, ly, lx,
+            tile_block<T, false>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by, bx,
                                  lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, acc);
         taken += 4;
         const bool inside =
             max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
         if (step >= sorw || !inside || taken >= cap) {
+            const uint32_t pos = (uint32_t)(bz + lz) * g.sz + (uint32_t)(by + ly) * g.sy + (uint32_t)(bx + lx);
             st[off + j] = make_uint4(pos, seed, wl, (uint32_t)step);
             have = false;
         }
     }
-    wave_add_u64(accepted, acc);
+    wave_add_u64(accepted, (uint64_t)acc);
     __syncthreads();
     for (int k = threadIdx.x; k < NT; k += kTileThreads) {
         const uint32_t c = shits[k];

@@ -82,6 +82,8 @@ class Net:
         self.hb = Param(eng, np.zeros(L_MAX))
         self._bufs = {}
         self.step_count = 0
+        self.T = 1.0             # softmax temperature used by predict (fitted by calibrate)
+        self._head_inf = None    # head weights / T on the device, rebuilt after changes
         self.scratch = eng.alloc(max(ci * co * 27 for _, ci, co in LAYERS) * 2)
 
     # -- parameters -----------------------------------------------------------
@@ -98,6 +100,7 @@ class Net:
             w, b, _, _ = self.conv[name]
             d[name + ".w"], d[name + ".b"] = w.get(), b.get()
         d["head.W"], d["head.b"] = self.hW.get(), self.hb.get()
+        d["head.T"] = np.array(self.T, np.float32)
         return d
 
     def load_state_dict(self, d):
@@ -107,6 +110,8 @@ class Net:
             b.set(d[name + ".b"])
         self.hW.set(d["head.W"])
         self.hb.set(d["head.b"])
+        self.T = float(d["head.T"]) if "head.T" in d else 1.0
+        self._head_inf = None
 
     def save(self, path=WEIGHTS_PATH):
         np.savez_compressed(path, **self.state_dict())
@@ -233,18 +238,39 @@ class Net:
 
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
         self.step_count += 1
+        self._head_inf = None
         for p in self.params():
             _hip.call("bmx_adam", self.eng.ctx, p.w.ptr, p.g.ptr, p.m.ptr, p.v.ptr,
                       None if p.bf is None else p.bf.ptr, p.n, _f(lr), _f(beta1), _f(beta2), _f(eps),
                       self.step_count, _f(wd))
 
+    def _head_for_inference(self):
+        """Head weights divided by the temperature: logits / T = (W/T) x + b/T."""
+        if self._head_inf is None:
+            self._head_inf = (self.eng.upload((self.hW.get() / self.T).astype(np.float32)),
+                              self.eng.upload((self.hb.get() / self.T).astype(np.float32)))
+        return self._head_inf
+
     def predict(self, x, dims, grid, nlabels, thr, pred, unc):
         """Labels (index) and uncertainty flags of a (Z,Y,X) volume whose input x lives on
-        the padded grid (Zp,Yp,Xp)."""
+        the padded grid (Zp,Yp,Xp): flag = top softmax(logits / T) probability < thr."""
         Zp, Yp, Xp = grid
         d0 = self.forward(x, 1, Zp, Yp, Xp)
+        W, b = self._head_for_inference()
         _hip.call("bmx_head_predict", self.eng.ctx, d0.ptr, *dims, Zp, Yp, Xp, int(nlabels),
-                  self.hW.w.ptr, self.hb.w.ptr, _f(thr), pred.ptr, unc.ptr)
+                  W.ptr, b.ptr, _f(thr), pred.ptr, unc.ptr)
+
+    def log_probs(self, x, grid, nlabels):
+        """log softmax over the first nlabels logits (temperature 1) on the padded grid,
+        float32 [Zp,Yp,Xp,nlabels] on the host."""
+        Zp, Yp, Xp = grid
+        d0 = self.forward(x, 1, Zp, Yp, Xp)
+        n = Zp * Yp * Xp
+        prob = self._get("prob", n * nlabels * 4)
+        _hip.call("bmx_head", self.eng.ctx, d0.ptr, n, int(nlabels), self.hW.w.ptr, self.hb.w.ptr, prob.ptr,
+                  None, None, _f(1.0), None, None, None, None)
+        p = prob.download((Zp, Yp, Xp, nlabels), np.float32)
+        return np.log(np.maximum(p, 1e-38))
 
     def flops_forward(self, N, D, H, W):
         v0 = N * D * H * W
@@ -373,6 +399,100 @@ def train(n_volumes=2000, size=64, iters=1500, batch=4, lr=2e-3, seed=0, out=WEI
     return model, hist, time.perf_counter() - t1
 
 
+def validation_volume(seed, size=256):
+    """Held-out volume of the 512^3 benchmark's kind scaled to `size` (8 structures,
+    ellipsoids and tubes, means 25 apart over 30..230, noise 12, bias 15, labels on
+    every 10th plane of all axes).  Calibration seeds are disjoint from the seed of
+    the benchmark volume."""
+    from .synth import structures_volume
+    return structures_volume(shape=(size, size, size), seed=int(seed), n_struct=8,
+                             radius=(5, max(8, 60 * size // 512)), every=10, allaxis=True)
+
+
+def calibrate(net, n_volumes=2, size=512, seed0=7001, thr=0.9, brad=3, target_recall=0.95, min_count=500,
+              log=print):
+    """Fit the softmax temperature T used by predict on held-out volumes.
+
+    For each volume the plain walker gives the reference labels; voxels where the
+    network's argmax disagrees with them and that lie outside the predicted
+    boundaries dilated by `brad` must be caught by the confidence rule
+    (top softmax(logits/T) probability < thr).  A disagreement voxel counts for
+    both labels involved.  T is the smallest value on a 0.05 grid for which, for
+    every label with at least `min_count` disagreements, the rule plus the
+    boundary band covers `target_recall` of them.  Sets net.T and returns
+    (T, worst per-label recall, stats)."""
+    from . import helpers as H
+    from .interpolation import DeviceVolume, run_pipeline
+    eng = net.eng
+    per_vol = []       # (log-probs, labels involved) of disagreements outside the band
+    n_dis = n_band = 0
+    tot_l = {}         # label -> disagreements
+    band_l = {}        # label -> disagreements inside the band
+    for i in range(n_volumes):
+        vol = validation_volume(seed0 + i, size)
+        bm = H.Biomedisa()
+        bm.data, bm.labelData, bm.only, bm.ignore, bm.success = vol.img, vol.labels, "all", "none", True
+        bm = H.pre_processing(bm)
+        img = np.pad(H.normalize_image(bm.data), 1)
+        lab = np.pad(bm.labelData, 1)
+        z0, z1, y0, y1, x0, x1 = H.predict_blocksize(lab)
+        img = np.ascontiguousarray(img[z0:z1, y0:y1, x0:x1])
+        labidx = H.label_index_volume(np.ascontiguousarray(lab[z0:z1, y0:y1, x0:x1]), bm.allLabels)
+        nol = bm.nol
+        dv = DeviceVolume(eng, img, labidx)
+        out = run_pipeline(dv, nol, bm.allLabels.astype(np.uint8), nbrw=10, sorw=4000, allaxis=True,
+                           keep_hits=True)
+        Z, Y, X = dv.dims
+        V = dv.V
+        ref = eng.alloc(V)
+        eng.argmax(out["hits"], nol, V, np.arange(nol, dtype=np.uint8), ref)
+        A = ref.download(dv.dims, np.uint8)
+        grid = (pad8(Z), pad8(Y), pad8(X))
+        x = net._get("input", int(np.prod(grid)) * CIN * 2)
+        flags = eng.upload(out["flags"])
+        _hip.call("bmx_net_input", eng.ctx, dv.img.ptr, dv.img_dtype, dv.lab.ptr, flags.ptr, Z, Y, X, *grid,
+                  x.ptr)
+        lp = net.log_probs(x, grid, nol)[:Z, :Y, :X]
+        pred = np.ascontiguousarray(lp.argmax(-1).astype(np.uint8))
+        pb, zb = eng.upload(pred), eng.alloc(V)
+        zb.zero()
+        band, need, tmp = eng.alloc(V), eng.alloc(V), eng.alloc(V)
+        _hip.call("bmx_guided_masks", eng.ctx, pb.ptr, zb.ptr, Z, Y, X, int(brad), 0, band.ptr, need.ptr, tmp.ptr)
+        bnd = band.download(dv.dims, np.uint8).astype(bool)
+        D = pred != A
+        n_dis += int(D.sum())
+        n_band += int((D & bnd).sum())
+        vals = bm.allLabels
+        for l in range(1, nol):
+            m = D & ((A == l) | (pred == l))
+            tot_l[int(vals[l])] = tot_l.get(int(vals[l]), 0) + int(m.sum())
+            band_l[int(vals[l])] = band_l.get(int(vals[l]), 0) + int((m & bnd).sum())
+        out_m = D & ~bnd
+        per_vol.append((lp[out_m].astype(np.float64), vals[A[out_m]], vals[pred[out_m]]))
+        log("calibration volume %d: %s, %d labels, %d disagreements (%d outside the band)"
+            % (i, dv.dims, nol, int(D.sum()), int((D & ~bnd).sum())))
+    labels = [l for l, c in tot_l.items() if c >= min_count]
+    recall, T = 0.0, 1.0
+    for T in np.arange(1.0, 40.0001, 0.05):
+        caught = dict(band_l)
+        for lp, la, lb in per_vol:
+            if lp.size == 0:
+                continue
+            z = lp / T
+            z -= z.max(-1, keepdims=True)
+            flag = (1.0 / np.exp(z).sum(-1)) < thr
+            for l in labels:
+                caught[l] += int((flag & ((la == l) | (lb == l))).sum())
+        recall = min([caught[l] / tot_l[l] for l in labels] or [1.0])
+        if recall >= target_recall:
+            break
+    net.T = float(T)
+    net._head_inf = None
+    log("temperature %.2f: worst per-label recall %.4f over labels %s; %d disagreements, %d in the boundary band"
+        % (T, recall, sorted(labels), n_dis, n_band))
+    return net.T, recall, {"disagreements": n_dis, "in_band": n_band, "per_label": tot_l}
+
+
 def _copy(eng, dst, dst_off, src, nbytes):
     """Device-to-device copy into dst at byte offset dst_off."""
     assert dst_off + nbytes <= dst.nbytes and nbytes <= src.nbytes
@@ -382,7 +502,7 @@ def _copy(eng, dst, dst_off, src, nbytes):
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("cmd", choices=["train", "calibrate"])
     ap.add_argument("--volumes", type=int, default=2000)
     ap.add_argument("--iters", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=4)
@@ -390,6 +510,14 @@ if __name__ == "__main__":
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=WEIGHTS_PATH)
     ap.add_argument("--target", choices=["soft", "hard"], default="soft")
+    ap.add_argument("--weights", default=WEIGHTS_PATH, help="calibrate: weights to calibrate")
+    ap.add_argument("--recall", type=float, default=0.95)
     a = ap.parse_args()
-    train(a.volumes, a.size, a.iters, a.batch, a.lr, out=a.out, hard=a.target == "hard",
-          log=lambda m: print("[train] " + m, flush=True))
+    lg = lambda m: print("[%s] %s" % (a.cmd, m), flush=True)  # noqa: E731
+    if a.cmd == "train":
+        train(a.volumes, a.size, a.iters, a.batch, a.lr, out=a.out, hard=a.target == "hard", log=lg)
+    else:
+        model = Net(_hip.engine(0))
+        model.load(a.weights)
+        calibrate(model, n_volumes=a.volumes, size=a.size, target_recall=a.recall, log=lg)
+        model.save(a.out)
