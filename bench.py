@@ -107,7 +107,9 @@ def main():
     ap.add_argument("--method", choices=["both", "plain", "guided"], default="both")
     ap.add_argument("--thr", type=float, default=0.9)
     ap.add_argument("--brad", type=int, default=3)
-    ap.add_argument("--srad", type=int, default=16)
+    ap.add_argument("--srad", type=int, default=8)
+    ap.add_argument("--far", type=int, default=32)
+    ap.add_argument("--far-walks", type=int, default=1)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -146,9 +148,11 @@ def main():
         def guided_step(tm):
             return run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw,
                                        allaxis=True, smooth=args.smooth, thr=args.thr,
-                                       brad=args.brad, srad=args.srad, timings=tm)
+                                       brad=args.brad, srad=args.srad, far=args.far,
+                                       far_walks=args.far_walks, timings=tm)
         out, tms, sec = time_pipeline(guided_step, eng, args.warmup, args.steps, "guided")
         res["guided"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], walked=out["walked_seeds"],
+                             ring=out["ring_seeds"],
                              accepted=out["accepted"],
                              regular=out["regular"].download(dv.dims, np.uint8))
 
@@ -170,7 +174,8 @@ def main():
     if "guided" in res:
         g = res["guided"]
         config.update(guided_seconds=g["sec"], walked_seed_fraction=g["walked"] / g["nseeds"],
-                      thr=args.thr, brad=args.brad, srad=args.srad,
+                      ring_seed_fraction=g["ring"] / g["nseeds"], far=args.far, far_walks=args.far_walks,
+                      temperature=net.T, thr=args.thr, brad=args.brad, srad=args.srad,
                       guided_timings=g["tms"][-1])
         config["min_dice_vs_ground_truth_guided"] = min(per_label_dice(g["regular"][inner], vol.gt).values())
     if "plain" in res:
@@ -203,7 +208,7 @@ def main():
     rk = res["plain"] if "plain" in res else res["guided"]
     tm = rk["tms"][-1]
     walked = rk.get("walked", rk["nseeds"])
-    steps_total = walked * args.nbrw * args.sorw
+    steps_total = (walked * args.nbrw + rk.get("ring", 0) * args.far_walks) * args.sorw
     walk_bytes = steps_total * 6 * 1 + rk["accepted"] * 8
     launches = max(int(tm["walk_launches"]), 1)
     kernel_s = tm["walk_kernel_ms"] / 1000.0

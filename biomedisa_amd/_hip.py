@@ -20,7 +20,8 @@ LABEL_NONE = 0xFF
 
 
 class WalkOpts(C.Structure):
-    _fields_ = [("algorithm", C.c_int), ("segment_steps", C.c_int), ("brick_log2", C.c_int)]
+    _fields_ = [("algorithm", C.c_int), ("segment_steps", C.c_int), ("brick_log2", C.c_int),
+                ("hit_weight", C.c_int)]
 
 SEED_DTYPE = np.dtype([("pos", "<u4"), ("label", "<u4"), ("var", "<f4"), ("reserved", "<u4")])
 
@@ -67,6 +68,7 @@ SIGNATURES = {
     "bmx_f32_to_bf16": (_i, [_p, _p, _p, _i64]),
     "bmx_add_bf16": (_i, [_p, _p, _p, _i64]),
     "bmx_hits_target": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]),
+    "bmx_ring_mask": (_i, [_p, _p, _i, _i, _i, _i, _p, _p]),
     "bmx_net_input": (_i, [_p, _p, _i, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
     "bmx_head_predict": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _p, C.c_float, _p, _p]),
     "bmx_guided_masks": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p]),
@@ -213,11 +215,12 @@ class Engine:
         return buf, n.value
 
     def walk(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits,
-             algorithm=WALK_AUTO, segment_steps=0, brick_log2=0):
-        """Accumulate the walks of `nseeds` seeds into `hits`; returns the accepted moves."""
+             algorithm=WALK_AUTO, segment_steps=0, brick_log2=0, hit_weight=1):
+        """Accumulate the walks of `nseeds` seeds into `hits` (each accepted move adds
+        `hit_weight`); returns the accepted moves."""
         Z, Y, X = dims
         acc = C.c_uint64()
-        opts = WalkOpts(int(algorithm), int(segment_steps), int(brick_log2))
+        opts = WalkOpts(int(algorithm), int(segment_steps), int(brick_log2), int(hit_weight))
         _check(load().bmx_walk_ex(self.ctx, img.ptr, img_dtype, Z, Y, X,
                                   seeds.ptr if seeds is not None else None, int(nseeds),
                                   int(nlabels), int(nbrw), int(sorw), int(rng),

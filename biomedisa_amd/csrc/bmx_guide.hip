@@ -6,7 +6,10 @@
 // Guided rule: U = {voxels whose top softmax probability < thr} united with
 // the predicted label boundaries dilated by `brad` voxels (Chebyshev).  Seeds
 // within `srad` voxels (Chebyshev) of U walk; the result takes the walker's
-// argmax on U and the network's label elsewhere.
+// argmax on U and the network's label elsewhere.  Far field: seeds in the ring
+// between srad and a wider radius walk k times fewer walks with hits weighted by
+// k (bmx_walk_opts.hit_weight), so the hits on U also carry an unbiased estimate
+// of the contributions of the seeds beyond srad.
 #include "bmx_internal.h"
 
 namespace {
@@ -116,6 +119,12 @@ __global__ void k_or(uint8_t *__restrict__ a, const uint8_t *__restrict__ b, int
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n) a[v] = (a[v] | b[v]) ? 1 : 0;
+}
+
+__global__ void k_andnot(uint8_t *__restrict__ a, const uint8_t *__restrict__ b, int64_t n)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) a[v] = (a[v] && !b[v]) ? 1 : 0;
 }
 
 struct LabelValues {
@@ -253,6 +262,22 @@ int bmx_guided_masks(bmx_ctx *ctx, const uint8_t *pred, const uint8_t *unc, int 
     BMX_LAUNCH_CHECK();
     BMX_HIP(hipMemcpyAsync(need, U, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
     return dilate(ctx, need, tmp, Z, Y, X, srad);
+}
+
+int bmx_ring_mask(bmx_ctx *ctx, const uint8_t *need, int Z, int Y, int X, int r, uint8_t *ring, uint8_t *tmp)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(need && ring && tmp, "bmx_ring_mask: null pointer");
+    BMX_CHECK(r >= 0, "bmx_ring_mask: negative radius");
+    const int64_t V = (int64_t)Z * Y * X;
+    BMX_CHECK(V < ((int64_t)1 << 32) - 256, "bmx_ring_mask: volume exceeds one dispatch");
+    BMX_HIP(hipMemcpyAsync(ring, need, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
+    rc = dilate(ctx, ring, tmp, Z, Y, X, r);
+    if (rc) return rc;
+    k_andnot<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(ring, need, V);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
 }
 
 int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const uint8_t *values,

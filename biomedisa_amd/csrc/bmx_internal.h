@@ -59,8 +59,8 @@ int bmx_set_device(bmx_ctx *ctx);
 // Tile-owner walker (bmx_tile.hip), called by bmx_walk_ex.
 int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
                    const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
-                   uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc, hipEvent_t *ev,
-                   int max_timed, int *launches);
+                   uint32_t weight, uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc,
+                   hipEvent_t *ev, int max_timed, int *launches);
 
 // Device arithmetic -----------------------------------------------------------
 

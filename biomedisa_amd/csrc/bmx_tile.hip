@@ -143,7 +143,7 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
                                            const uint32_t rr[4], float B, float div1, int &step, int &li,
                                            int &lz, int &ly, int &lx, int bz, int by, int bx, int lmin0,
                                            int lmin1, int lmin2, int lmax0, int lmax1, int lmax2,
-                                           uint32_t &acc)
+                                           uint32_t weight, uint32_t &acc)
 {
     constexpr int R = TileCfg<T>::C + 2 * TileCfg<T>::H, RI = R + 2;
 #pragma unroll
@@ -188,11 +188,11 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
                 li += sgn * lmag;
                 ++acc;
                 if (in || max(max((unsigned)lz, (unsigned)ly), (unsigned)lx) < (unsigned)RI) {
-                    atomicAdd(&shits[li], 1u);
+                    atomicAdd(&shits[li], weight);
                 } else {
                     const int64_t pos =
                         ((int64_t)(bz + lz) * g.sz + (int64_t)(by + ly) * g.sy) + (bx + lx);
-                    atomicAdd(&hg[pos], 1u);
+                    atomicAdd(&hg[pos], weight);
                 }
             }
             ++step;
@@ -204,7 +204,7 @@ template <typename T>
 __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     const T *__restrict__ img, WalkGeom g, TileGeom tg, const bmx_seed *__restrict__ seeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
-    const uint32_t *__restrict__ bcnt, int sorw, int cap, uint32_t k0, uint32_t k1,
+    const uint32_t *__restrict__ bcnt, int sorw, int cap, uint32_t weight, uint32_t k0, uint32_t k1,
     uint32_t *__restrict__ hits, unsigned long long *accepted)
 {
     constexpr int C = TileCfg<T>::C, H = TileCfg<T>::H, R = C + 2 * H, RI = R + 2;
@@ -276,10 +276,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
         const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
         if (edge)
             tile_block<T, true>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by, bx,
-                                lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, acc);
+                                lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
         else
             tile_block<T, false>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by, bx,
-                                 lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, acc);
+                                 lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
         taken += 4;
         const bool inside =
             max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
@@ -303,8 +303,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
 
 template <typename T>
 int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *seeds, int64_t nseeds,
-                 int nlabels, int nbrw, int sorw, int cap, uint32_t k0, uint32_t k1, uint32_t *hits,
-                 unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches)
+                 int nlabels, int nbrw, int sorw, int cap, uint32_t weight, uint32_t k0, uint32_t k1,
+                 uint32_t *hits, unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches)
 {
     constexpr int C = TileCfg<T>::C;
     const int64_t n0 = nseeds * (int64_t)nbrw;
@@ -359,7 +359,7 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *
         if (live == 0) break;
         if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round], ctx->stream));
         k_tile_walk<T><<<nb, kTileThreads, lds, ctx->stream>>>(img, g, tg, seeds, Bst, bkey, boff, bcnt,
-                                                              sorw, cap, k0, k1, hits, acc);
+                                                              sorw, cap, weight, k0, k1, hits, acc);
         BMX_LAUNCH_CHECK();
         if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round + 1], ctx->stream));
         uint4 *t = A; A = Bst; Bst = t;
@@ -376,13 +376,13 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *
 // Entry from bmx_walk_ex (bmx_walk.hip).
 int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
                    const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
-                   uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc, hipEvent_t *ev,
-                   int max_timed, int *launches)
+                   uint32_t weight, uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc,
+                   hipEvent_t *ev, int max_timed, int *launches)
 {
     WalkGeom g{Z, Y, X, (uint32_t)X, (uint32_t)X * (uint32_t)Y, (int64_t)Z * Y * X};
     if (img_dtype == BMX_IMG_U8)
         return walk_tiled_t<uint8_t>(ctx, (const uint8_t *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap,
-                                     k0, k1, hits, acc, ev, max_timed, launches);
-    return walk_tiled_t<float>(ctx, (const float *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap, k0,
-                               k1, hits, acc, ev, max_timed, launches);
+                                     weight, k0, k1, hits, acc, ev, max_timed, launches);
+    return walk_tiled_t<float>(ctx, (const float *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap, weight,
+                               k0, k1, hits, acc, ev, max_timed, launches);
 }

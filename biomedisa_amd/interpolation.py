@@ -107,16 +107,21 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=16, timings=None, keep=False):
+                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=8, far=32, far_walks=1, cap=0, far_cap=64,
+                        timings=None, keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
 
     The network proposes a label and a confidence for every voxel.  U = voxels
-    whose top probability is below `thr` plus the proposed label boundaries
-    dilated by `brad`; only seeds within `srad` (Chebyshev) of U walk, with the
-    same Philox streams as the plain walker, so those walks are exactly the
-    plain walker's.  The result takes the walker's argmax on U and the
-    network's label elsewhere ('smooth': the smoothed walker argmax on U).
-    Returns the dict of run_pipeline plus 'walked_seeds' and 'U_fraction'.
+    whose top (temperature-scaled) probability is below `thr` plus the proposed
+    label boundaries dilated by `brad`.  Seeds within `srad` (Chebyshev) of U
+    walk all `nbrw` walks, with the same Philox streams as the plain walker, so
+    those walks are exactly the plain walker's.  Seeds in the ring from `srad`
+    to `far` walk their first `far_walks` walks with every hit counted
+    nbrw/far_walks times: an unbiased estimate of what they add to U
+    (far=0 disables the ring).  cap / far_cap: steps per walker per round of the
+    tile walker for the near and ring walks (0 = library default).  The result takes the walker's argmax on U and
+    the network's label elsewhere ('smooth': the smoothed walker argmax on U).
+    Returns the dict of run_pipeline plus 'walked_seeds' and 'ring_seeds'.
     """
     from . import surrogate as S
     eng = vol.eng
@@ -124,6 +129,8 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     V = vol.V
     if nlabels > S.L_MAX:
         raise PipelineError("guided mode supports at most %d labels" % S.L_MAX)
+    if far > srad and (far_walks < 1 or nbrw % far_walks):
+        raise PipelineError("far_walks must divide nbrw")
     t = timings if timings is not None else {}
     t0 = time.perf_counter()
     flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis)
@@ -142,14 +149,25 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     _hip.call("bmx_guided_masks", eng.ctx, pred.ptr, unc.ptr, Z, Y, X, int(brad), int(srad), U.ptr, need.ptr,
               tmp.ptr)
     sub, nsub = eng.select_seeds(seeds, nseeds, need)
+    ring_sub, nring = None, 0
+    if far > srad:
+        ring = eng.alloc(V)
+        _hip.call("bmx_ring_mask", eng.ctx, need.ptr, Z, Y, X, int(far - srad), ring.ptr, tmp.ptr)
+        ring_sub, nring = eng.select_seeds(seeds, nseeds, ring)
     eng.sync()
     t2 = time.perf_counter()
     hits = eng.alloc(nlabels * V * 4)
     hits.zero()
     accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, sub, nsub, nlabels, nbrw, sorw, _hip.RNG_PHILOX,
-                        key, hits)
+                        key, hits, segment_steps=cap)
+    phase_ms, kernel_ms, launches = eng.last_walk_stats()
+    if nring:
+        accepted += eng.walk(vol.img, vol.img_dtype, vol.dims, ring_sub, nring, nlabels, far_walks, sorw,
+                             _hip.RNG_PHILOX, key, hits, segment_steps=far_cap, hit_weight=nbrw // far_walks)
+        p2, k2, l2 = eng.last_walk_stats()
+        phase_ms, kernel_ms, launches = phase_ms + p2, kernel_ms + k2, launches + l2
     t3 = time.perf_counter()
-    out = {"nseeds": nseeds, "walked_seeds": nsub, "accepted": accepted, "flags": flags}
+    out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "accepted": accepted, "flags": flags}
     values = np.ascontiguousarray(values, dtype=np.uint8)
     regular = eng.alloc(V)
     _hip.call("bmx_merge_guided", eng.ctx, hits.ptr, int(nlabels), V, values.ctypes.data, pred.ptr, U.ptr,
@@ -165,7 +183,6 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     t4 = time.perf_counter()
     if keep:
         out.update(pred=pred, U=U, need=need, hits=hits)
-    phase_ms, kernel_ms, launches = eng.last_walk_stats()
     t.update(seeds_s=t1 - t0, proposal_s=t2 - t1, walk_s=t3 - t2, merge_s=t4 - t3, walk_phase_ms=phase_ms,
              walk_kernel_ms=kernel_ms, walk_launches=launches)
     return out
@@ -186,7 +203,7 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
 
     method='guided' runs the surrogate-guided pipeline (run_pipeline_guided;
     Philox stream only, no 'uncertainty'/'hits'); `guide` is a dict of its
-    thr/brad/srad parameters.
+    thr/brad/srad/far/far_walks parameters.
     """
     if data is None or labelData is None:
         raise NotImplementedError("file loading (path_to_data/path_to_labels) is not part of this "

@@ -119,6 +119,8 @@ typedef struct bmx_walk_opts {
     int segment_steps; /* sorted: steps per launch; tiled: max steps per walker per round;
                           multiple of 4 (0 = 128 sorted, 256 tiled) */
     int brick_log2;    /* sorted: log2 of the sort-brick side in voxels (0 = 3) */
+    int hit_weight;    /* tiled: count added per accepted move (0 = 1); with nbrw = n/k walks
+                          and weight k, a seed subset's hits estimate its n-walk hits */
 } bmx_walk_opts;
 
 /* bmx_walk with a choice of algorithm (opts may be NULL = automatic).  No
@@ -216,6 +218,8 @@ int bmx_head_predict(bmx_ctx *ctx, const uint16_t *x, int Z, int Y, int X, int Z
 int bmx_guided_masks(bmx_ctx *ctx, const uint8_t *pred, const uint8_t *unc, int Z, int Y, int X,
                      int brad, int srad, uint8_t *U, uint8_t *need, uint8_t *tmp);
 /* out = U ? values[argmax hits] : values[pred]. */
+/* ring = dilate(need, r) & ~need (Chebyshev r): the seeds of the far-field ring. */
+int bmx_ring_mask(bmx_ctx *ctx, const uint8_t *need, int Z, int Y, int X, int r, uint8_t *ring, uint8_t *tmp);
 int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,
                      const uint8_t *values, const uint8_t *pred, const uint8_t *U, uint8_t *out);
 /* out = U ? a : values[pred]  (a holds label values, e.g. a smoothed result). */

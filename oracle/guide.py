@@ -115,6 +115,12 @@ def guided_masks(pred, unc, brad, srad):
     return U.astype(np.uint8), dilate(U, srad)
 
 
+def ring_mask(need, r):
+    """dilate(need, r) & ~need: the far-field ring around the near set."""
+    n = np.asarray(need) != 0
+    return (dilate(n, r).astype(bool) & ~n).astype(np.uint8)
+
+
 def merge_guided(hits, values, pred, U):
     """values[first argmax of hits] on U, values[pred] elsewhere (hits [L,...])."""
     values = np.asarray(values, np.uint8)

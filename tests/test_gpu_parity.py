@@ -278,3 +278,28 @@ def test_config1_shape_dice_vs_reference_stream():
     b = smart_interpolation(vol.img, vol.labels, nbrw=10, sorw=1000, rng="reference")["regular"]
     d = per_label_dice(a, b)
     assert min(d.values()) >= 0.98, d
+
+
+def test_hit_weight_scales_hits(eng):
+    """hit_weight k adds k per accepted move: walk(nbrw=1, k) == k * walk(nbrw=1, 1),
+    and only the tiled Philox walker accepts it."""
+    from biomedisa_amd.synth import ellipsoids_volume
+    vol = ellipsoids_volume(shape=(36, 40, 44), seed=31, semi=(6, 12), every=6)
+    img = np.ascontiguousarray(np.pad(vol.img, 1))
+    lab = np.ascontiguousarray(np.pad(vol.labels, 1))
+    dims, V = img.shape, img.size
+    ib, lb = eng.upload(img), eng.upload(lab)
+    flags = eng.plane_flags(lb, dims, False)
+    seeds, n = eng.seeds(lb, ib, _hip.IMG_U8, dims, flags)
+    out = []
+    for w in (1, 10):
+        h = eng.alloc(4 * V * 4)
+        h.zero()
+        acc = eng.walk(ib, _hip.IMG_U8, dims, seeds, n, 4, 1, 300, _hip.RNG_PHILOX, 99, h,
+                       algorithm=_hip.WALK_TILED, hit_weight=w)
+        out.append((h.download((4,) + dims, np.uint32), acc))
+    np.testing.assert_array_equal(out[1][0], 10 * out[0][0])
+    assert out[0][1] == out[1][1]
+    with pytest.raises(_hip.BmxError):
+        eng.walk(ib, _hip.IMG_U8, dims, seeds, n, 4, 1, 300, _hip.RNG_PHILOX, 99, h,
+                 algorithm=_hip.WALK_DIRECT, hit_weight=2)
