@@ -21,7 +21,7 @@ LABEL_NONE = 0xFF
 
 class WalkOpts(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("segment_steps", C.c_int), ("brick_log2", C.c_int),
-                ("hit_weight", C.c_int)]
+                ("hit_weight", C.c_int), ("seed_overrides", C.c_int)]
 
 SEED_DTYPE = np.dtype([("pos", "<u4"), ("label", "<u4"), ("var", "<f4"), ("reserved", "<u4")])
 
@@ -69,6 +69,7 @@ SIGNATURES = {
     "bmx_add_bf16": (_i, [_p, _p, _p, _i64]),
     "bmx_hits_target": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]),
     "bmx_ring_mask": (_i, [_p, _p, _i, _i, _i, _i, _p, _p]),
+    "bmx_seed_walks": (_i, [_p, _p, _i64, _i, _i]),
     "bmx_net_input": (_i, [_p, _p, _i, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
     "bmx_head_predict": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _p, C.c_float, _p, _p]),
     "bmx_guided_masks": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p]),
@@ -215,12 +216,14 @@ class Engine:
         return buf, n.value
 
     def walk(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits,
-             algorithm=WALK_AUTO, segment_steps=0, brick_log2=0, hit_weight=1):
+             algorithm=WALK_AUTO, segment_steps=0, brick_log2=0, hit_weight=1, seed_overrides=False):
         """Accumulate the walks of `nseeds` seeds into `hits` (each accepted move adds
-        `hit_weight`); returns the accepted moves."""
+        `hit_weight`, times the seed's own weight with seed_overrides); returns the
+        accepted moves."""
         Z, Y, X = dims
         acc = C.c_uint64()
-        opts = WalkOpts(int(algorithm), int(segment_steps), int(brick_log2), int(hit_weight))
+        opts = WalkOpts(int(algorithm), int(segment_steps), int(brick_log2), int(hit_weight),
+                        int(bool(seed_overrides)))
         _check(load().bmx_walk_ex(self.ctx, img.ptr, img_dtype, Z, Y, X,
                                   seeds.ptr if seeds is not None else None, int(nseeds),
                                   int(nlabels), int(nbrw), int(sorw), int(rng),
@@ -254,6 +257,17 @@ class Engine:
     def uncertainty(self, hits, nlabels, dims, out):
         Z, Y, X = dims
         _check(load().bmx_uncertainty(self.ctx, hits.ptr, int(nlabels), Z, Y, X, out.ptr))
+
+
+class BufferView:
+    """A sub-range of a DeviceBuffer (no ownership): .ptr / .nbytes like a buffer."""
+
+    def __init__(self, buf, offset, nbytes=None):
+        assert 0 <= offset <= buf.nbytes
+        self.base = buf
+        self.ptr = buf.ptr + int(offset)
+        self.nbytes = buf.nbytes - int(offset) if nbytes is None else int(nbytes)
+        assert offset + self.nbytes <= buf.nbytes
 
 
 def ptr(buf):

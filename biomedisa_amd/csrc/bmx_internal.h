@@ -30,6 +30,8 @@ struct bmx_ctx {
     size_t keys_cap;
     void *d_buckets;         // tiled walker: non-empty (key, offset, count) lists
     size_t buckets_cap;
+    void *d_scan;            // exclusive-scan block partials
+    size_t scan_cap;
     hipEvent_t *seg_ev;      // sorted walker: event pairs around each walk launch
     float last_walk_kernel_ms;
     int last_walk_launches;
@@ -59,8 +61,8 @@ int bmx_set_device(bmx_ctx *ctx);
 // Tile-owner walker (bmx_tile.hip), called by bmx_walk_ex.
 int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
                    const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
-                   uint32_t weight, uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc,
-                   hipEvent_t *ev, int max_timed, int *launches);
+                   uint32_t weight, int overrides, uint32_t k0, uint32_t k1, uint32_t *hits,
+                   unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches);
 
 // Device arithmetic -----------------------------------------------------------
 

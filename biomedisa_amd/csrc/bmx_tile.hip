@@ -59,15 +59,26 @@ __device__ __forceinline__ uint32_t tile_key(uint32_t pos, uint32_t label, const
     return (((z / C) * tg.nty + y / C) * tg.ntx + x / C) * tg.nlabels + label;
 }
 
-// Walker state: {pos, seed index, walk | label << 16, step}.
+// Walker state: {pos, seed index, walk | label << 16, step}.  With overrides a
+// seed running fewer than nbrw walks gets its extra walkers created finished
+// (step = sorw), so the first re-bucketing drops them.
 __global__ __launch_bounds__(256) void k_tile_init(const bmx_seed *__restrict__ seeds, int64_t nseeds,
-                                                   int nbrw, uint4 *__restrict__ st)
+                                                   int nbrw, int sorw, int overrides, uint4 *__restrict__ st)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nseeds * nbrw) return;
     const int64_t s = i / nbrw;
     const bmx_seed sd = seeds[s];
-    st[i] = make_uint4(sd.pos, (uint32_t)s, (uint32_t)(i - s * nbrw) | (sd.label << 16), 0u);
+    const uint32_t w = (uint32_t)(i - s * nbrw);
+    const uint32_t walks = overrides ? (sd.reserved & 0xFFFFu) : 0u;
+    const uint32_t step0 = (walks != 0u && w >= walks) ? (uint32_t)sorw : 0u;
+    st[i] = make_uint4(sd.pos, (uint32_t)s, w | (sd.label << 16), step0);
+}
+
+__global__ __launch_bounds__(256) void k_seed_walks(bmx_seed *__restrict__ seeds, int64_t n, uint32_t value)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) seeds[i].reserved = value;
 }
 
 template <int C>
@@ -204,8 +215,8 @@ template <typename T>
 __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     const T *__restrict__ img, WalkGeom g, TileGeom tg, const bmx_seed *__restrict__ seeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
-    const uint32_t *__restrict__ bcnt, int sorw, int cap, uint32_t weight, uint32_t k0, uint32_t k1,
-    uint32_t *__restrict__ hits, unsigned long long *accepted)
+    const uint32_t *__restrict__ bcnt, int sorw, int cap, uint32_t weight0, int overrides, uint32_t k0,
+    uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted)
 {
     constexpr int C = TileCfg<T>::C, H = TileCfg<T>::H, R = C + 2 * H, RI = R + 2;
     constexpr int NT = RI * RI * RI;
@@ -223,13 +234,33 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     // global coordinates of tile voxel (0,0,0)
     const int bz = tz * C - H - 1, by = ty * C - H - 1, bx = tx * C - H - 1;
 
-    for (int k = threadIdx.x; k < NT; k += kTileThreads) {
-        const int lz = k / (RI * RI), ly = (k / RI) % RI, lx = k % RI;
-        const int gz = min(max(bz + lz, 0), g.Z - 1);
-        const int gy = min(max(by + ly, 0), g.Y - 1);
-        const int gx = min(max(bx + lx, 0), g.X - 1);
-        simg[k] = img[(int64_t)gz * g.sz + (int64_t)gy * g.sy + gx];
-        shits[k] = 0u;
+    // Tile load: each thread issues a batch of image loads before storing them to
+    // LDS, so one memory latency is paid per batch rather than per element; the
+    // hit counters are cleared with 16-byte stores.
+    {
+        constexpr int BATCH = 8;
+        for (int base = 0; base < NT; base += BATCH * kTileThreads) {
+            T v[BATCH];
+#pragma unroll
+            for (int q = 0; q < BATCH; ++q) {
+                const int k = base + threadIdx.x + q * kTileThreads;
+                if (k < NT) {
+                    const int lz = k / (RI * RI), ly = (k / RI) % RI, lx = k % RI;
+                    const int gz = min(max(bz + lz, 0), g.Z - 1);
+                    const int gy = min(max(by + ly, 0), g.Y - 1);
+                    const int gx = min(max(bx + lx, 0), g.X - 1);
+                    v[q] = img[(int64_t)gz * g.sz + (int64_t)gy * g.sy + gx];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BATCH; ++q) {
+                const int k = base + threadIdx.x + q * kTileThreads;
+                if (k < NT) simg[k] = v[q];
+            }
+        }
+        uint4 *sh4 = (uint4 *)shits;
+        for (int k = threadIdx.x; k < NT / 4; k += kTileThreads) sh4[k] = make_uint4(0u, 0u, 0u, 0u);
+        for (int k = (NT / 4) * 4 + threadIdx.x; k < NT; k += kTileThreads) shits[k] = 0u;
     }
     if (threadIdx.x == 0) next = 0u;
     __syncthreads();
@@ -245,7 +276,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
     uint32_t *__restrict__ hg = hits + (int64_t)label * g.V;
     uint32_t acc = 0;  // accepted moves of this thread in this launch (< 2^32: the launch is bounded)
     bool have = false;
-    uint32_t j = 0, p0 = 0, seed = 0, wl = 0;
+    uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = weight0;
     int step = 0, lz = 0, ly = 0, lx = 0, li = 0, taken = 0;
     float B = 0.0f, div1 = 0.0f;
     while (true) {
@@ -261,6 +292,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
             p0 = sd.pos;
             B = bmx_ld(img, (int64_t)p0);
             div1 = 1.0f / (2.0f * sd.var);
+            weight = weight0;
+            if (overrides && (sd.reserved >> 16) != 0u) weight = weight0 * (sd.reserved >> 16);
             const int z = (int)(pos / g.sz);
             const uint32_t r = pos - (uint32_t)z * g.sz;
             const int y = (int)(r / g.sy);
@@ -303,8 +336,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_walk(
 
 template <typename T>
 int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *seeds, int64_t nseeds,
-                 int nlabels, int nbrw, int sorw, int cap, uint32_t weight, uint32_t k0, uint32_t k1,
-                 uint32_t *hits, unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches)
+                 int nlabels, int nbrw, int sorw, int cap, uint32_t weight, int overrides, uint32_t k0,
+                 uint32_t k1, uint32_t *hits, unsigned long long *acc, hipEvent_t *ev, int max_timed,
+                 int *launches)
 {
     constexpr int C = TileCfg<T>::C;
     const int64_t n0 = nseeds * (int64_t)nbrw;
@@ -333,17 +367,19 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *
     uint32_t *counts = (uint32_t *)ctx->d_keys;
     uint32_t *bkey = (uint32_t *)ctx->d_buckets, *boff = bkey + nkeys, *bcnt = boff + nkeys;
     uint32_t *d_nb = (uint32_t *)(ctx->d_counter + 1);
-    k_tile_init<<<bmx_blocks(n0, 256), 256, 0, ctx->stream>>>(seeds, nseeds, nbrw, A);
+    k_tile_init<<<bmx_blocks(n0, 256), 256, 0, ctx->stream>>>(seeds, nseeds, nbrw, sorw, overrides, A);
     BMX_LAUNCH_CHECK();
     int64_t n = n0;
     int round = 0;
+    // BMX_WALK_TRACE=1 prints one line per round (live walkers, buckets, kernel ms) to stderr.
+    static const bool trace = getenv("BMX_WALK_TRACE") && getenv("BMX_WALK_TRACE")[0] == '1';
     while (n > 0) {
         BMX_HIP(hipMemsetAsync(counts, 0, (size_t)(nkeys + 1) * sizeof(uint32_t), ctx->stream));
         BMX_HIP(hipMemsetAsync(d_nb, 0, sizeof(uint32_t), ctx->stream));
         k_tile_hist<C><<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(A, n, g, tg, sorw, counts);
         BMX_LAUNCH_CHECK();
-        k_scan_u32<<<1, 1024, 0, ctx->stream>>>(counts, nkeys + 1);
-        BMX_LAUNCH_CHECK();
+        rc = scan_u32(ctx, counts, nkeys + 1);
+        if (rc) return rc;
         k_bucket_list<<<bmx_blocks(nkeys, 256), 256, 0, ctx->stream>>>(counts, nkeys, bkey, boff, bcnt,
                                                                        d_nb);
         BMX_LAUNCH_CHECK();
@@ -359,9 +395,17 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *
         if (live == 0) break;
         if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round], ctx->stream));
         k_tile_walk<T><<<nb, kTileThreads, lds, ctx->stream>>>(img, g, tg, seeds, Bst, bkey, boff, bcnt,
-                                                              sorw, cap, weight, k0, k1, hits, acc);
+                                                              sorw, cap, weight, overrides, k0, k1, hits,
+                                                              acc);
         BMX_LAUNCH_CHECK();
         if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round + 1], ctx->stream));
+        if (trace && round < max_timed) {
+            float ms = 0.0f;
+            BMX_HIP(hipEventSynchronize(ev[2 * round + 1]));
+            BMX_HIP(hipEventElapsedTime(&ms, ev[2 * round], ev[2 * round + 1]));
+            fprintf(stderr, "[bmx_walk] round %d live %lld buckets %u walkers/bucket %.0f kernel %.3f ms\n", round,
+                    (long long)live, nb, (double)live / (nb ? nb : 1), ms);
+        }
         uint4 *t = A; A = Bst; Bst = t;
         n = live;
         ++round;
@@ -373,16 +417,32 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *
 
 }  // namespace
 
+extern "C" int bmx_seed_walks(bmx_ctx *ctx, bmx_seed *seeds, int64_t n, int walks, int weight)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(n == 0 || seeds, "bmx_seed_walks: null seeds");
+    BMX_CHECK(n >= 0 && n < ((int64_t)1 << 32) - 256, "bmx_seed_walks: n out of range");
+    BMX_CHECK(walks >= 0 && walks <= 65535 && weight >= 0 && weight <= 65535,
+              "bmx_seed_walks: walks / weight not in [0, 65535]");
+    if (n > 0) {
+        k_seed_walks<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(seeds, n,
+                                                                 (uint32_t)walks | ((uint32_t)weight << 16));
+        BMX_LAUNCH_CHECK();
+    }
+    return BMX_OK;
+}
+
 // Entry from bmx_walk_ex (bmx_walk.hip).
 int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
                    const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
-                   uint32_t weight, uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc,
-                   hipEvent_t *ev, int max_timed, int *launches)
+                   uint32_t weight, int overrides, uint32_t k0, uint32_t k1, uint32_t *hits,
+                   unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches)
 {
     WalkGeom g{Z, Y, X, (uint32_t)X, (uint32_t)X * (uint32_t)Y, (int64_t)Z * Y * X};
     if (img_dtype == BMX_IMG_U8)
         return walk_tiled_t<uint8_t>(ctx, (const uint8_t *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap,
-                                     weight, k0, k1, hits, acc, ev, max_timed, launches);
+                                     weight, overrides, k0, k1, hits, acc, ev, max_timed, launches);
     return walk_tiled_t<float>(ctx, (const float *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap, weight,
-                               k0, k1, hits, acc, ev, max_timed, launches);
+                               overrides, k0, k1, hits, acc, ev, max_timed, launches);
 }

@@ -469,8 +469,8 @@ int walk_sorted(bmx_ctx *ctx, const void *img, int img_dtype, const WalkGeom &g,
         BMX_LAUNCH_CHECK();
         if (k < kMaxTimedLaunches) BMX_HIP(hipEventRecord(ctx->seg_ev[2 * k + 1], ctx->stream));
         if (last) break;
-        k_scan_u32<<<1, 1024, 0, ctx->stream>>>(counts, (int64_t)nkeys);
-        BMX_LAUNCH_CHECK();
+        rc = scan_u32(ctx, counts, (int64_t)nkeys);
+        if (rc) return rc;
         k_seg_scatter<<<blocks, 256, 0, ctx->stream>>>(A, n, Bst, counts, sg, sorw);
         BMX_LAUNCH_CHECK();
         uint4 *t = A; A = Bst; Bst = t;
@@ -503,6 +503,7 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
     int blog = (opts && opts->brick_log2 > 0) ? opts->brick_log2 : 3;
     const int weight = (opts && opts->hit_weight > 0) ? opts->hit_weight : 1;
     BMX_CHECK(!opts || opts->hit_weight >= 0, "bmx_walk: negative hit_weight");
+    const int overrides = (opts && opts->seed_overrides) ? 1 : 0;
     BMX_CHECK(algo == BMX_WALK_AUTO || algo == BMX_WALK_DIRECT || algo == BMX_WALK_SORTED ||
                   algo == BMX_WALK_TILED,
               "bmx_walk: bad algorithm %d", algo);
@@ -514,10 +515,11 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
     const bool run = nseeds > 0 && nbrw > 0 && sorw > 0;
     const int64_t nwalkers = nseeds * (int64_t)nbrw;
     if (algo == BMX_WALK_AUTO) algo = nwalkers >= ((int64_t)1 << 16) ? BMX_WALK_TILED : BMX_WALK_DIRECT;
-    if (algo == BMX_WALK_AUTO && weight > 1) algo = BMX_WALK_TILED;
+    if (algo == BMX_WALK_AUTO && (weight > 1 || overrides)) algo = BMX_WALK_TILED;
     if (rng != BMX_RNG_PHILOX) algo = BMX_WALK_DIRECT;
-    BMX_CHECK(weight == 1 || (algo == BMX_WALK_TILED && rng == BMX_RNG_PHILOX),
-              "bmx_walk: hit_weight %d needs the tiled walker and the Philox stream", weight);
+    BMX_CHECK((weight == 1 && !overrides) || (algo == BMX_WALK_TILED && rng == BMX_RNG_PHILOX),
+              "bmx_walk: hit_weight / seed_overrides need the tiled walker and the Philox stream");
+    BMX_CHECK(nbrw <= 65535, "bmx_walk: nbrw %d above 65535", nbrw);
     if (algo == BMX_WALK_TILED && !(opts && opts->segment_steps > 0)) seg = 256;
     if (run && rng == BMX_RNG_PHILOX && algo != BMX_WALK_TILED) {
         const size_t rec_bytes = (size_t)g.V * (img_dtype == BMX_IMG_U8 ? 8 : sizeof(RecF32));
@@ -539,8 +541,8 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
             rc = ensure_events(ctx);
             if (rc) return rc;
             rc = bmx_walk_tiled(ctx, img, img_dtype, g.Z, g.Y, g.X, seeds, nseeds, nlabels, nbrw, sorw, seg,
-                                (uint32_t)weight, k0, k1, hits, acc, ctx->seg_ev, kMaxTimedLaunches,
-                                &launches);
+                                (uint32_t)weight, overrides, k0, k1, hits, acc, ctx->seg_ev,
+                                kMaxTimedLaunches, &launches);
             if (rc) return rc;
         } else if (rng == BMX_RNG_PHILOX && algo == BMX_WALK_SORTED) {
             rc = walk_sorted(ctx, img, img_dtype, g, seeds, nseeds, nbrw, sorw, k0, k1, hits, acc, seg,

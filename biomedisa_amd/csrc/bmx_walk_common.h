@@ -84,30 +84,90 @@ __device__ __forceinline__ uint32_t agg_atomic_inc(uint32_t *base, uint32_t key,
 
 
 // Exclusive scan of n uint32 counts in place by one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_u32(uint32_t *counts, int64_t n)
+// In-place exclusive scan of counts[0..n) in three passes: per-block totals of
+// kScanTile elements, a scan of those totals (one block), and the block-local
+// scans plus their offsets.  Loads are coalesced (4 consecutive per thread).
+constexpr int kScanThreads = 1024;
+constexpr int kScanTile = 4 * kScanThreads;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *part, uint32_t *total)
 {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t lo = t * per, hi = (lo + per < n) ? lo + per : n;
-    uint32_t s = 0;
-    for (int64_t i = lo; i < hi; ++i) s += counts[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = (t >= off) ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    // inclusive scan within each wave, then across the 16 wave totals
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    uint32_t run = part[t] - s;
-    for (int64_t i = lo; i < hi; ++i) {
-        const uint32_t c = counts[i];
-        counts[i] = run;
-        run += c;
+    if (lane == 63) part[wave] = x;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int nw = kScanThreads / 64;
+        uint32_t w = threadIdx.x < nw ? part[threadIdx.x] : 0u;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(w, off, 64);
+            if ((int)threadIdx.x >= off) w += y;
+        }
+        if (threadIdx.x < nw) part[threadIdx.x] = w;
+        if (threadIdx.x == nw - 1) *total = w;
+    }
+    __syncthreads();
+    const uint32_t before = wave > 0 ? part[wave - 1] : 0u;
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_totals(const uint32_t *__restrict__ counts, int64_t n,
+                                                              uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t part[kScanThreads / 64];
+    __shared__ uint32_t total;
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + 4 * threadIdx.x;
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (base + q < n) s += counts[base + q];
+    (void)block_excl_scan(s, part, &total);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t *__restrict__ totals, int64_t nb)
+{
+    __shared__ uint32_t part[kScanThreads / 64];
+    __shared__ uint32_t total;
+    uint32_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+        const int64_t b = b0 + threadIdx.x;
+        const uint32_t v = b < nb ? totals[b] : 0u;
+        const uint32_t ex = block_excl_scan(v, part, &total);
+        if (b < nb) totals[b] = carry + ex;
+        carry += total;
+        __syncthreads();
     }
 }
 
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t *__restrict__ counts, int64_t n,
+                                                             const uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t part[kScanThreads / 64];
+    __shared__ uint32_t total;
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + 4 * threadIdx.x;
+    uint32_t v[4];
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v[q] = base + q < n ? counts[base + q] : 0u;
+        s += v[q];
+    }
+    uint32_t run = totals[blockIdx.x] + block_excl_scan(s, part, &total);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (base + q < n) {
+            counts[base + q] = run;
+            run += v[q];
+        }
+}
 
 template <typename P>
 int grow(bmx_ctx *ctx, P **ptr, size_t *cap, size_t bytes)
@@ -124,5 +184,21 @@ int grow(bmx_ctx *ctx, P **ptr, size_t *cap, size_t bytes)
     return BMX_OK;
 }
 
+// Exclusive scan of counts[0..n) on the context stream (block partials in ctx->d_scan).
+inline int scan_u32(bmx_ctx *ctx, uint32_t *counts, int64_t n)
+{
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    int rc = grow(ctx, &ctx->d_scan, &ctx->scan_cap, (size_t)(nb > 0 ? nb : 1) * sizeof(uint32_t));
+    if (rc) return rc;
+    if (n <= 0) return BMX_OK;
+    uint32_t *totals = (uint32_t *)ctx->d_scan;
+    k_scan_totals<<<(unsigned)nb, kScanThreads, 0, ctx->stream>>>(counts, n, totals);
+    BMX_LAUNCH_CHECK();
+    k_scan_top<<<1, kScanThreads, 0, ctx->stream>>>(totals, nb);
+    BMX_LAUNCH_CHECK();
+    k_scan_apply<<<(unsigned)nb, kScanThreads, 0, ctx->stream>>>(counts, n, totals);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
 
 }  // namespace

@@ -107,7 +107,7 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=8, far=32, far_walks=1, cap=0, far_cap=64,
+                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=12, far=48, far_walks=2, cap=0,
                         timings=None, keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
 
@@ -118,8 +118,10 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     those walks are exactly the plain walker's.  Seeds in the ring from `srad`
     to `far` walk their first `far_walks` walks with every hit counted
     nbrw/far_walks times: an unbiased estimate of what they add to U
-    (far=0 disables the ring).  cap / far_cap: steps per walker per round of the
-    tile walker for the near and ring walks (0 = library default).  The result takes the walker's argmax on U and
+    (far=0 disables the ring).  Near and ring seeds walk in one call (per-seed
+    walk counts and weights), so ring walkers fill the same tile buckets.  cap:
+    steps per walker per round of the tile walker (0 = library default).  The
+    result takes the walker's argmax on U and
     the network's label elsewhere ('smooth': the smoothed walker argmax on U).
     Returns the dict of run_pipeline plus 'walked_seeds' and 'ring_seeds'.
     """
@@ -148,24 +150,23 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     U, need, tmp = eng.alloc(V), eng.alloc(V), eng.alloc(V)
     _hip.call("bmx_guided_masks", eng.ctx, pred.ptr, unc.ptr, Z, Y, X, int(brad), int(srad), U.ptr, need.ptr,
               tmp.ptr)
-    sub, nsub = eng.select_seeds(seeds, nseeds, need)
-    ring_sub, nring = None, 0
+    sub = eng.alloc(max(1, nseeds) * _hip.SEED_DTYPE.itemsize)
+    _, nsub = eng.select_seeds(seeds, nseeds, need, out=sub)
+    nring = 0
     if far > srad:
+        # ring seeds follow the near seeds in the same list (the two sets are disjoint)
         ring = eng.alloc(V)
         _hip.call("bmx_ring_mask", eng.ctx, need.ptr, Z, Y, X, int(far - srad), ring.ptr, tmp.ptr)
-        ring_sub, nring = eng.select_seeds(seeds, nseeds, ring)
+        tail = _hip.BufferView(sub, nsub * _hip.SEED_DTYPE.itemsize)
+        _, nring = eng.select_seeds(seeds, nseeds, ring, out=tail)
+        _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, nring, int(far_walks), nbrw // far_walks)
     eng.sync()
     t2 = time.perf_counter()
     hits = eng.alloc(nlabels * V * 4)
     hits.zero()
-    accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, sub, nsub, nlabels, nbrw, sorw, _hip.RNG_PHILOX,
-                        key, hits, segment_steps=cap)
+    accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, sub, nsub + nring, nlabels, nbrw, sorw,
+                        _hip.RNG_PHILOX, key, hits, segment_steps=cap, seed_overrides=nring > 0)
     phase_ms, kernel_ms, launches = eng.last_walk_stats()
-    if nring:
-        accepted += eng.walk(vol.img, vol.img_dtype, vol.dims, ring_sub, nring, nlabels, far_walks, sorw,
-                             _hip.RNG_PHILOX, key, hits, segment_steps=far_cap, hit_weight=nbrw // far_walks)
-        p2, k2, l2 = eng.last_walk_stats()
-        phase_ms, kernel_ms, launches = phase_ms + p2, kernel_ms + k2, launches + l2
     t3 = time.perf_counter()
     out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "accepted": accepted, "flags": flags}
     values = np.ascontiguousarray(values, dtype=np.uint8)

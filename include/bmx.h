@@ -48,7 +48,9 @@ extern "C" {
 typedef struct bmx_ctx bmx_ctx;
 
 /* One walker seed: voxel index, label index, intensity variance of the
- * seed's neighbourhood (_calc_var, pyopencl_small.py:140-185). */
+ * seed's neighbourhood (_calc_var, pyopencl_small.py:140-185).  `reserved` is 0
+ * as written by bmx_seeds; with bmx_walk_opts.seed_overrides the tiled walker
+ * reads it as (walks | hit_weight << 16), see bmx_seed_walks. */
 typedef struct bmx_seed {
     uint32_t pos;
     uint32_t label;
@@ -121,6 +123,9 @@ typedef struct bmx_walk_opts {
     int brick_log2;    /* sorted: log2 of the sort-brick side in voxels (0 = 3) */
     int hit_weight;    /* tiled: count added per accepted move (0 = 1); with nbrw = n/k walks
                           and weight k, a seed subset's hits estimate its n-walk hits */
+    int seed_overrides; /* tiled: 1 = each seed's `reserved` gives its walk count (low 16
+                           bits, 0 = nbrw, at most nbrw) and a hit weight (high 16 bits,
+                           0 = 1) multiplying hit_weight */
 } bmx_walk_opts;
 
 /* bmx_walk with a choice of algorithm (opts may be NULL = automatic).  No
@@ -218,6 +223,9 @@ int bmx_head_predict(bmx_ctx *ctx, const uint16_t *x, int Z, int Y, int X, int Z
 int bmx_guided_masks(bmx_ctx *ctx, const uint8_t *pred, const uint8_t *unc, int Z, int Y, int X,
                      int brad, int srad, uint8_t *U, uint8_t *need, uint8_t *tmp);
 /* out = U ? values[argmax hits] : values[pred]. */
+/* seeds[i].reserved = walks | weight << 16 for i < n (per-seed walk count and hit
+ * weight read by bmx_walk_ex with seed_overrides).  No counterpart in the reference. */
+int bmx_seed_walks(bmx_ctx *ctx, bmx_seed *seeds, int64_t n, int walks, int weight);
 /* ring = dilate(need, r) & ~need (Chebyshev r): the seeds of the far-field ring. */
 int bmx_ring_mask(bmx_ctx *ctx, const uint8_t *need, int Z, int Y, int X, int r, uint8_t *ring, uint8_t *tmp);
 int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,

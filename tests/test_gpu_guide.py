@@ -155,7 +155,8 @@ def test_guided_equals_plain_when_all_uncertain(eng):
     from biomedisa_amd import surrogate as S
     from biomedisa_amd.synth import structures_volume
     vol = structures_volume(shape=(48, 52, 56), seed=11, n_struct=4, radius=(4, 14), every=7, allaxis=True)
-    p, g, dl, _ = _pipelines(eng, vol, 3, 150, 5, S.Net(eng, seed=1), thr=1.01, brad=1, srad=1, far=9)
+    p, g, dl, _ = _pipelines(eng, vol, 3, 150, 5, S.Net(eng, seed=1), thr=1.01, brad=1, srad=1, far=9,
+                             far_walks=3)
     assert g["walked_seeds"] == g["nseeds"] == p["nseeds"] and g["ring_seeds"] == 0
     assert g["accepted"] == p["accepted"]
     np.testing.assert_array_equal(dl(g["regular"]), dl(p["regular"]))

@@ -303,3 +303,62 @@ def test_hit_weight_scales_hits(eng):
     with pytest.raises(_hip.BmxError):
         eng.walk(ib, _hip.IMG_U8, dims, seeds, n, 4, 1, 300, _hip.RNG_PHILOX, 99, h,
                  algorithm=_hip.WALK_DIRECT, hit_weight=2)
+
+
+def test_seed_overrides_equal_separate_walks(eng):
+    """Per-seed walk counts and weights (bmx_seed_walks + seed_overrides) give the
+    hits of the two seed groups walked separately: group A all nbrw walks, group B
+    its first walk counted nbrw times."""
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(40, 36, 44), seed=32, n_struct=3, radius=(4, 10), every=5, allaxis=True)
+    img = np.ascontiguousarray(np.pad(vol.img, 1))
+    lab = np.ascontiguousarray(np.pad(vol.labels, 1))
+    dims, V = img.shape, img.size
+    ib, lb = eng.upload(img), eng.upload(lab)
+    flags = eng.plane_flags(lb, dims, True)
+    seeds, n = eng.seeds(lb, ib, _hip.IMG_U8, dims, flags)
+    nl, nbrw, sorw, key = 4, 5, 200, 7
+    na = n // 2
+    tail = _hip.BufferView(seeds, na * _hip.SEED_DTYPE.itemsize)
+    _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, n - na, 1, nbrw)
+    both = eng.alloc(nl * V * 4)
+    both.zero()
+    eng.walk(ib, _hip.IMG_U8, dims, seeds, n, nl, nbrw, sorw, _hip.RNG_PHILOX, key, both, seed_overrides=True)
+    _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, n - na, 0, 0)
+    ha, hb = eng.alloc(nl * V * 4), eng.alloc(nl * V * 4)
+    ha.zero()
+    hb.zero()
+    eng.walk(ib, _hip.IMG_U8, dims, seeds, na, nl, nbrw, sorw, _hip.RNG_PHILOX, key, ha,
+             algorithm=_hip.WALK_TILED)
+    eng.walk(ib, _hip.IMG_U8, dims, tail, n - na, nl, 1, sorw, _hip.RNG_PHILOX, key, hb,
+             algorithm=_hip.WALK_TILED)
+    shape = (nl,) + dims
+    ref = ha.download(shape, np.uint32) + nbrw * hb.download(shape, np.uint32)
+    np.testing.assert_array_equal(both.download(shape, np.uint32), ref)
+
+
+@pytest.mark.parametrize("w", [16, 20])
+def test_tile_counter_overflow_path(eng, w):
+    """Heavy weights and many walks from two seeds in a small uniform volume push
+    single voxels far past 65535 hits within one round; totals must equal w times
+    the unweighted walk."""
+    dims = (14, 14, 14)
+    img = np.full(dims, 100, np.uint8)
+    lab = np.zeros(dims, np.uint8)
+    lab[7, 6, 6] = 1
+    lab[7, 7, 7] = 2
+    lab[7, 6, 7] = 255
+    flags = np.zeros(sum(dims), np.uint8)
+    flags[7] = 1
+    ib, lb, fb = eng.upload(img), eng.upload(lab), eng.upload(flags)
+    seeds, n = eng.seeds(lb, ib, _hip.IMG_U8, dims, fb)
+    V = int(np.prod(dims))
+    out = []
+    for weight in (1, w):
+        h = eng.alloc(3 * V * 4)
+        h.zero()
+        eng.walk(ib, _hip.IMG_U8, dims, seeds, n, 3, 3000, 400, _hip.RNG_PHILOX, 5, h,
+                 algorithm=_hip.WALK_TILED, hit_weight=weight)
+        out.append(h.download((3,) + dims, np.uint32).astype(np.uint64))
+    assert out[1].max() > 3 * 65536
+    np.testing.assert_array_equal(out[1], w * out[0])
