@@ -53,6 +53,7 @@ SIGNATURES = {
     "bmx_smooth_argmax": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p]),
     "bmx_uncertainty": (_i, [_p, _p, _i, _i64, _i64, _i64, _p]),
     "bmx_conv3d_fwd": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _i, _i]),
+    "bmx_conv3d_fwd_cat": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _i, _p, _p, _p, _i, _i]),
     "bmx_conv3d_dgrad": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _i, _p]),
     "bmx_conv3d_wgrad": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _i, _p, _p]),
     "bmx_pool2": (_i, [_p, _p, _i, _i, _i, _i, _i, _p]),

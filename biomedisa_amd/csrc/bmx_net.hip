@@ -52,13 +52,20 @@ struct Shape {
 // Forward: y = act(conv(x, w) + b).  grid.x: voxel groups of 128 (4 waves x
 // 32), grid.y: output-channel groups of 16*NB.  Each lane decodes its own
 // voxel's coordinates, so any grid shape works; requires Ci % 16 == 0 and
-// Co % (16*NB) == 0.
-template <int NB>
+// Co % (16*NB) == 0.  With a second source xb (Cb channels on the half-size
+// grid), the input is concat(x, up2(xb)) with Ci = Ca + Cb, read in place: a
+// 16-channel chunk comes from x below Ca and from xb at the voxel's parent
+// (z/2, y/2, x/2) above, so neither the upsampled tensor nor the concatenation
+// is materialised.
+template <int NB, bool CAT>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x, Shape s, int Ci,
                                                   const uint16_t *__restrict__ w,
                                                   const float *__restrict__ bias,
-                                                  uint16_t *__restrict__ y, int Co, int relu)
+                                                  uint16_t *__restrict__ y, int Co, int relu,
+                                                  const uint16_t *__restrict__ xb, int Ca)
 {
+    const int Cb = Ci - Ca;
+    const int Dh = s.D / 2, Hh = s.H / 2, Wh = s.W / 2;
     const int lane = threadIdx.x & 63;
     const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
     const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
@@ -83,23 +90,27 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x
         for (int nb = 0; nb < NB; ++nb) acc[rb][nb] = float4_t{0.f, 0.f, 0.f, 0.f};
     for (int tap = 0; tap < 27; ++tap) {
         const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const uint16_t *src[2];
+        const uint16_t *src[2], *srcb[2];
         bool ok[2];
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) {
             const int z = cz[rb] + dz, yy = cy[rb] + dy, xx = cx[rb] + dx;
             ok[rb] = vin[rb] && (unsigned)z < (unsigned)s.D && (unsigned)yy < (unsigned)s.H &&
                      (unsigned)xx < (unsigned)s.W;
-            const int64_t nv = (((int64_t)cn[rb] * s.D + (ok[rb] ? z : 0)) * s.H + (ok[rb] ? yy : 0)) * s.W +
-                               (ok[rb] ? xx : 0);
-            src[rb] = x + (ok[rb] ? nv : 0) * Ci + kq;
+            const int zc = ok[rb] ? z : 0, yc = ok[rb] ? yy : 0, xc = ok[rb] ? xx : 0;
+            const int64_t nv = (((int64_t)cn[rb] * s.D + zc) * s.H + yc) * s.W + xc;
+            src[rb] = x + nv * (CAT ? Ca : Ci) + kq;
+            if (CAT) {
+                const int64_t nh = (((int64_t)cn[rb] * Dh + zc / 2) * Hh + yc / 2) * Wh + xc / 2;
+                srcb[rb] = xb + nh * Cb + kq;
+            }
         }
         const uint16_t *wt = w + ((int64_t)(co0 + r) * 27 + tap) * Ci + kq;
         for (int c0 = 0; c0 < Ci; c0 += 16) {
             short4_t a[2];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
-                a[rb] = ld4(src[rb] + c0);
+                a[rb] = (CAT && c0 >= Ca) ? ld4(srcb[rb] + (c0 - Ca)) : ld4(src[rb] + c0);
                 if (!ok[rb]) a[rb] = short4_t{0, 0, 0, 0};
             }
 #pragma unroll
@@ -479,11 +490,14 @@ __global__ void k_f2bf(const float *__restrict__ a, uint16_t *__restrict__ b, in
 
 template <int NB>
 int launch_fwd(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y,
-               int Co, int relu)
+               int Co, int relu, const uint16_t *xb = nullptr, int Ca = 0)
 {
     const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
     dim3 grid((unsigned)((nvox + 127) / 128), (unsigned)(Co / (16 * NB)));
-    k_conv_fwd<NB><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu);
+    if (xb)
+        k_conv_fwd<NB, true><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, xb, Ca);
+    else
+        k_conv_fwd<NB, false><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, nullptr, Ci);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -525,6 +539,23 @@ int bmx_conv3d_fwd(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, 
     if (Co % 64 == 0) return launch_fwd<4>(ctx, x, s, Ci, w, b, y, Co, relu);
     if (Co % 32 == 0) return launch_fwd<2>(ctx, x, s, Ci, w, b, y, Co, relu);
     return launch_fwd<1>(ctx, x, s, Ci, w, b, y, Co, relu);
+}
+
+int bmx_conv3d_fwd_cat(bmx_ctx *ctx, const uint16_t *a, int N, int D, int H, int W, int Ca, const uint16_t *b,
+                       int Cb, const uint16_t *w, const float *bias, uint16_t *y, int Co, int relu)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    rc = check_conv_shape(N, D, H, W, Ca + Cb, Co);
+    if (rc) return rc;
+    BMX_CHECK(a && b && w && y, "bmx_conv3d_fwd_cat: null pointer");
+    BMX_CHECK(Ca >= 16 && Ca % 16 == 0 && Cb >= 16 && Cb % 16 == 0, "bmx_conv3d_fwd_cat: channels");
+    BMX_CHECK(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "bmx_conv3d_fwd_cat: grid must be even");
+    Shape s{N, D, H, W};
+    const int Ci = Ca + Cb;
+    if (Co % 64 == 0) return launch_fwd<4>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
+    if (Co % 32 == 0) return launch_fwd<2>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
+    return launch_fwd<1>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
 }
 
 int bmx_conv3d_dgrad(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, int Co, const uint16_t *w,

@@ -14,6 +14,9 @@
 // memory about 1.15 times per pass.
 #include "bmx_internal.h"
 
+#include <climits>
+#include <vector>
+
 namespace {
 
 constexpr int TX = 64;            // x extent of a tile (one wave per row group)
@@ -31,6 +34,13 @@ template <> __device__ __forceinline__ float ldf<uint32_t>(const uint32_t *p, in
     return (float)p[i];
 }
 
+// Box [z0,z1) x [y0,y1) x [x0,x1) of the volume that a launch updates; voxels
+// outside it read as 0 (smooth_argmax restricts each label to the box of its
+// non-zero hits grown by the iteration count, outside which phi stays 0).
+struct Box {
+    int z0, z1, y0, y1, x0, x1;
+};
+
 // Per-thread share of one tile plane plus halo: RY interior rows at column tx,
 // and up to two halo values (rows 0 / LH-1 for thread rows 0 / 1, the left and
 // right halo columns for thread row 2).
@@ -39,30 +49,34 @@ struct PlaneRegs {
     float h0, h1;
 };
 
+// Value of voxel (z, y, x) as the stencil sees it: clamped to the volume (the
+// clamped values only feed boundary voxels, which are copied), and 0 outside
+// the box.
+template <typename Tin>
+__device__ __forceinline__ float box_ld(const Tin *__restrict__ in, int z, int y, int x, int Y, int X,
+                                        const Box &bx, int Z)
+{
+    z = min(max(z, 0), Z - 1);
+    y = min(max(y, 0), Y - 1);
+    x = min(max(x, 0), X - 1);
+    const bool inb = z >= bx.z0 && z < bx.z1 && y >= bx.y0 && y < bx.y1 && x >= bx.x0 && x < bx.x1;
+    return inb ? ldf(in, ((int64_t)z * Y + y) * X + x) : 0.0f;
+}
+
 template <typename Tin>
 __device__ __forceinline__ void fetch_plane(const Tin *__restrict__ in, int zz, int Z, int Y, int X, int x0,
-                                            int y0, int tx, int ty, PlaneRegs &r)
+                                            int y0, int tx, int ty, const Box &bx, PlaneRegs &r)
 {
-    zz = min(max(zz, 0), Z - 1);
-    const int64_t base = (int64_t)zz * Y * X;
-    const int gx = min(max(x0 + tx, 0), X - 1);
 #pragma unroll
-    for (int k = 0; k < RY; ++k) {
-        const int gy = min(max(y0 + ty * RY + k, 0), Y - 1);
-        r.v[k] = ldf(in, base + (int64_t)gy * X + gx);
-    }
+    for (int k = 0; k < RY; ++k) r.v[k] = box_ld(in, zz, y0 + ty * RY + k, x0 + tx, Y, X, bx, Z);
     r.h0 = 0.0f;
     r.h1 = 0.0f;
     if (ty < 2) {
-        // halo rows: ly = 0 (gy = y0-1) for ty 0, ly = LH-1 (gy = y0+TY) for ty 1
-        const int gy = min(max(ty == 0 ? y0 - 1 : y0 + TY, 0), Y - 1);
-        r.h0 = ldf(in, base + (int64_t)gy * X + gx);
+        // halo rows: ly = 0 (y0-1) for ty 0, ly = LH-1 (y0+TY) for ty 1
+        r.h0 = box_ld(in, zz, ty == 0 ? y0 - 1 : y0 + TY, x0 + tx, Y, X, bx, Z);
     } else if (ty == 2 && tx < 2 * LH) {
-        // halo columns: lx = 0 (gx = x0-1) and lx = LW-1 (gx = x0+TX), rows ly = 0..LH-1
-        const int ly = tx % LH;
-        const int gy = min(max(y0 + ly - 1, 0), Y - 1);
-        const int gxh = min(max(tx < LH ? x0 - 1 : x0 + TX, 0), X - 1);
-        r.h0 = ldf(in, base + (int64_t)gy * X + gxh);
+        // halo columns: lx = 0 (x0-1) and lx = LW-1 (x0+TX), rows ly = 0..LH-1
+        r.h0 = box_ld(in, zz, y0 + tx % LH - 1, tx < LH ? x0 - 1 : x0 + TX, Y, X, bx, Z);
     }
 }
 
@@ -84,35 +98,35 @@ __device__ __forceinline__ void store_plane(float (*s)[LW], int tx, int ty, cons
 template <typename Tin>
 __global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__restrict__ in,
                                                                float *__restrict__ out, int Z, int Y,
-                                                               int X)
+                                                               int X, Box bx)
 {
     __shared__ float s[3][LH][LW];
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x0 = bx.x0 + blockIdx.x * TX, y0 = bx.y0 + blockIdx.y * TY;
     const int x = x0 + tx;
-    const int z_begin = blockIdx.z * kZChunk;
-    const int z_end = min(Z, z_begin + kZChunk);
+    const int z_begin = bx.z0 + blockIdx.z * kZChunk;
+    const int z_end = min(bx.z1, z_begin + kZChunk);
     const int64_t sy = X, sz = (int64_t)X * Y;
 
     PlaneRegs r;
-    fetch_plane(in, z_begin - 1, Z, Y, X, x0, y0, tx, ty, r);
+    fetch_plane(in, z_begin - 1, Z, Y, X, x0, y0, tx, ty, bx, r);
     store_plane(s[0], tx, ty, r);
-    fetch_plane(in, z_begin, Z, Y, X, x0, y0, tx, ty, r);
+    fetch_plane(in, z_begin, Z, Y, X, x0, y0, tx, ty, bx, r);
     store_plane(s[1], tx, ty, r);
-    fetch_plane(in, z_begin + 1, Z, Y, X, x0, y0, tx, ty, r);
+    fetch_plane(in, z_begin + 1, Z, Y, X, x0, y0, tx, ty, bx, r);
     int sm = 0, sc = 1, sp = 2;
     const bool inside_x = x > 0 && x < X - 1;
     const int lx = tx + 1;
     for (int z = z_begin; z < z_end; ++z) {
         store_plane(s[sp], tx, ty, r);
         __syncthreads();
-        if (z + 1 < z_end) fetch_plane(in, z + 2, Z, Y, X, x0, y0, tx, ty, r);
+        if (z + 1 < z_end) fetch_plane(in, z + 2, Z, Y, X, x0, y0, tx, ty, bx, r);
         const bool inside_z = z > 0 && z < Z - 1;
 #pragma unroll
         for (int k = 0; k < RY; ++k) {
             const int ly = 1 + ty * RY + k;
             const int y = y0 + ty * RY + k;
-            if (x < X && y < Y) {
+            if (x < bx.x1 && y < bx.y1) {
                 const float pc = s[sc][ly][lx];
                 float res = pc;
                 if (inside_x && inside_z && y > 0 && y < Y - 1) {
@@ -154,11 +168,15 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__res
 
 __global__ __launch_bounds__(256) void k_argmax_update(const float *__restrict__ phi,
                                                        float *__restrict__ best,
-                                                       uint8_t *__restrict__ bidx, int label, int64_t V)
+                                                       uint8_t *__restrict__ bidx, int label, int Z, int Y,
+                                                       int X, Box bx)
 {
+    const int64_t V = (int64_t)Z * Y * X;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    const float p = phi[v];
+    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
+    const bool inb = z >= bx.z0 && z < bx.z1 && y >= bx.y0 && y < bx.y1 && x >= bx.x0 && x < bx.x1;
+    const float p = inb ? phi[v] : 0.0f;
     if (label == 0 || p > best[v]) {
         best[v] = p;
         bidx[v] = (uint8_t)label;
@@ -169,6 +187,43 @@ struct LabelValues {
     uint8_t v[256];
 };
 
+// Bounding box of the non-zero hits of one label: box[0..5] = min z, max z,
+// min y, max y, min x, max x (min fields start at INT_MAX, max at -1).
+__global__ __launch_bounds__(256) void k_hits_box(const uint32_t *__restrict__ hits, int Z, int Y, int X,
+                                                  int *__restrict__ box)
+{
+    const int64_t V = (int64_t)Z * Y * X;
+    int b[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        if (hits[v]) {
+            const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
+            b[0] = min(b[0], z); b[1] = max(b[1], z);
+            b[2] = min(b[2], y); b[3] = max(b[3], y);
+            b[4] = min(b[4], x); b[5] = max(b[5], x);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 6; k += 2) {
+            b[k] = min(b[k], __shfl_xor(b[k], off, 64));
+            b[k + 1] = max(b[k + 1], __shfl_xor(b[k + 1], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0 && b[1] >= 0) {
+        for (int k = 0; k < 6; k += 2) {
+            atomicMin(&box[k], b[k]);
+            atomicMax(&box[k + 1], b[k + 1]);
+        }
+    }
+}
+
+__global__ void k_box_init(int *box, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) box[i] = (i & 1) ? -1 : INT_MAX;
+}
+
 __global__ __launch_bounds__(256) void k_map_values(const uint8_t *__restrict__ bidx, LabelValues lv,
                                                     uint8_t *__restrict__ out, int64_t V)
 {
@@ -178,10 +233,10 @@ __global__ __launch_bounds__(256) void k_map_values(const uint8_t *__restrict__ 
 }
 
 template <typename Tin>
-int launch_step(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X)
+int launch_step(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, const Box &bx)
 {
-    dim3 grid((X + TX - 1) / TX, (Y + TY - 1) / TY, (Z + kZChunk - 1) / kZChunk);
-    k_smooth_step<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X);
+    dim3 grid((bx.x1 - bx.x0 + TX - 1) / TX, (bx.y1 - bx.y0 + TY - 1) / TY, (bx.z1 - bx.z0 + kZChunk - 1) / kZChunk);
+    k_smooth_step<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X, bx);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -190,13 +245,13 @@ int launch_step(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X)
 // (one of a / b).
 template <typename Tin>
 int smooth_rounds(bmx_ctx *ctx, const Tin *in, float *a, float *b, int Z, int Y, int X, int iters,
-                  float **result)
+                  const Box &bx, float **result)
 {
-    int rc = launch_step<Tin>(ctx, in, a, Z, Y, X);
+    int rc = launch_step<Tin>(ctx, in, a, Z, Y, X, bx);
     if (rc) return rc;
     float *cur = a, *nxt = b;
     for (int it = 1; it < iters; ++it) {
-        rc = launch_step<float>(ctx, cur, nxt, Z, Y, X);
+        rc = launch_step<float>(ctx, cur, nxt, Z, Y, X, bx);
         if (rc) return rc;
         float *t = cur; cur = nxt; nxt = t;
     }
@@ -216,8 +271,9 @@ int bmx_smooth(bmx_ctx *ctx, float *phi, float *tmp, int64_t Z, int64_t Y, int64
     BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_smooth: bad dims");
     BMX_CHECK(iters >= 0, "bmx_smooth: negative iters");
     float *cur = phi, *nxt = tmp;
+    const Box whole{0, (int)Z, 0, (int)Y, 0, (int)X};
     for (int it = 0; it < iters; ++it) {
-        rc = launch_step<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X);
+        rc = launch_step<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X, whole);
         if (rc) return rc;
         float *t = cur; cur = nxt; nxt = t;
     }
@@ -239,17 +295,39 @@ int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z
     const int64_t V = Z * Y * X;
     float *a = nullptr, *b = nullptr, *best = nullptr;
     uint8_t *bidx = nullptr;
+    int *dbox = nullptr;
     hipError_t e = hipMalloc((void **)&a, (size_t)V * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&b, (size_t)V * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&best, (size_t)V * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&bidx, (size_t)V);
+    if (e == hipSuccess) e = hipMalloc((void **)&dbox, (size_t)nlabels * 6 * sizeof(int));
     if (e == hipSuccess) {
+        // Each label is smoothed only inside the box of its non-zero hits grown by
+        // `iters`: where all 19 stencil inputs are 0 the update gives exactly 0,
+        // so phi's support grows by at most one voxel per round and stays inside.
+        std::vector<int> box((size_t)nlabels * 6);
+        k_box_init<<<bmx_blocks(nlabels * 6, 256), 256, 0, ctx->stream>>>(dbox, nlabels * 6);
+        for (int l = 0; l < nlabels; ++l)
+            k_hits_box<<<2048, 256, 0, ctx->stream>>>(hits + (int64_t)l * V, (int)Z, (int)Y, (int)X,
+                                                     dbox + 6 * l);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(box.data(), dbox, box.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: hit boxes failed");
         for (int l = 0; l < nlabels && rc == BMX_OK; ++l) {
-            float *res = nullptr;
-            rc = smooth_rounds<uint32_t>(ctx, hits + (int64_t)l * V, a, b, (int)Z, (int)Y, (int)X,
-                                         iters, &res);
+            const int *h = &box[(size_t)6 * l];
+            Box bx{0, 0, 0, 0, 0, 0};
+            float *res = a;
+            if (h[1] >= 0) {
+                bx = Box{max(h[0] - iters, 0), min(h[1] + 1 + iters, (int)Z), max(h[2] - iters, 0),
+                         min(h[3] + 1 + iters, (int)Y), max(h[4] - iters, 0), min(h[5] + 1 + iters, (int)X)};
+                rc = smooth_rounds<uint32_t>(ctx, hits + (int64_t)l * V, a, b, (int)Z, (int)Y, (int)X, iters,
+                                             bx, &res);
+            }
             if (rc == BMX_OK) {
-                k_argmax_update<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(res, best, bidx, l, V);
+                k_argmax_update<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(res, best, bidx, l, (int)Z, (int)Y,
+                                                                            (int)X, bx);
                 if (hipGetLastError() != hipSuccess)
                     rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
             }
@@ -273,6 +351,7 @@ int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z
     if (b) (void)hipFree(b);
     if (best) (void)hipFree(best);
     if (bidx) (void)hipFree(bidx);
+    if (dbox) (void)hipFree(dbox);
     return rc;
 }
 

@@ -132,9 +132,10 @@ class Net:
         s0 = N * D * H * W
         s1, s2, s3 = s0 // 8, s0 // 64, s0 // 512
         sizes = {"a0": s0 * 16, "p0": s1 * 16, "a1": s1 * 32, "p1": s2 * 32, "a2": s2 * 64, "p2": s3 * 64,
-                 "a3": s3 * 128, "u2": s2 * 128, "c2": s2 * 192, "d2": s2 * 64, "u1": s1 * 64, "c1": s1 * 96,
-                 "d1": s1 * 32, "u0": s0 * 32, "c0": s0 * 48, "d0": s0 * 16}
+                 "a3": s3 * 128, "d2": s2 * 64, "d1": s1 * 32, "d0": s0 * 16}
         if grads:
+            sizes.update({"u2": s2 * 128, "c2": s2 * 192, "u1": s1 * 64, "c1": s1 * 96, "u0": s0 * 32,
+                          "c0": s0 * 48})
             sizes.update({"g_d0": s0 * 16, "g_c0": s0 * 48, "g_a0": s0 * 16, "g_u0": s0 * 32,
                           "g_d1": s1 * 32, "g_c1": s1 * 96, "g_a1": s1 * 32, "g_u1": s1 * 64,
                           "g_d2": s2 * 64, "g_c2": s2 * 192, "g_a2": s2 * 64, "g_u2": s2 * 128,
@@ -146,6 +147,11 @@ class Net:
     def _conv(self, name, x, shape, y):
         w, b, ci, co = self.conv[name]
         _hip.call("bmx_conv3d_fwd", self.eng.ctx, x.ptr, *shape, ci, w.bf.ptr, b.w.ptr, y.ptr, co, 1)
+
+    def _conv_cat(self, name, a, ca, b, cb, shape, y):
+        w, bias, ci, co = self.conv[name]
+        assert ci == ca + cb
+        _hip.call("bmx_conv3d_fwd_cat", self.eng.ctx, a.ptr, *shape, ca, b.ptr, cb, w.bf.ptr, bias.w.ptr, y.ptr, co, 1)
 
     def forward(self, x, N, D, H, W, grads=False):
         """x: bf16 [N,D,H,W,16] (D,H,W multiples of 8) -> d0 buffer (bf16 [.., 16])."""
@@ -163,15 +169,22 @@ class Net:
         self._conv("enc2", a["p1"], S2, a["a2"])
         _hip.call("bmx_pool2", ctx, a["a2"].ptr, *S2, 64, a["p2"].ptr)
         self._conv("bott", a["p2"], S3, a["a3"])
-        _hip.call("bmx_up2", ctx, a["a3"].ptr, *S3, 128, a["u2"].ptr)
-        _hip.call("bmx_concat", ctx, a["a2"].ptr, 64, a["u2"].ptr, 128, n2, a["c2"].ptr)
-        self._conv("dec2", a["c2"], S2, a["d2"])
-        _hip.call("bmx_up2", ctx, a["d2"].ptr, *S2, 64, a["u1"].ptr)
-        _hip.call("bmx_concat", ctx, a["a1"].ptr, 32, a["u1"].ptr, 64, n1, a["c1"].ptr)
-        self._conv("dec1", a["c1"], S1, a["d1"])
-        _hip.call("bmx_up2", ctx, a["d1"].ptr, *S1, 32, a["u0"].ptr)
-        _hip.call("bmx_concat", ctx, a["a0"].ptr, 16, a["u0"].ptr, 32, n0, a["c0"].ptr)
-        self._conv("dec0", a["c0"], S0, a["d0"])
+        if grads:
+            # training keeps the concatenated decoder inputs for the weight gradients
+            _hip.call("bmx_up2", ctx, a["a3"].ptr, *S3, 128, a["u2"].ptr)
+            _hip.call("bmx_concat", ctx, a["a2"].ptr, 64, a["u2"].ptr, 128, n2, a["c2"].ptr)
+            self._conv("dec2", a["c2"], S2, a["d2"])
+            _hip.call("bmx_up2", ctx, a["d2"].ptr, *S2, 64, a["u1"].ptr)
+            _hip.call("bmx_concat", ctx, a["a1"].ptr, 32, a["u1"].ptr, 64, n1, a["c1"].ptr)
+            self._conv("dec1", a["c1"], S1, a["d1"])
+            _hip.call("bmx_up2", ctx, a["d1"].ptr, *S1, 32, a["u0"].ptr)
+            _hip.call("bmx_concat", ctx, a["a0"].ptr, 16, a["u0"].ptr, 32, n0, a["c0"].ptr)
+            self._conv("dec0", a["c0"], S0, a["d0"])
+        else:
+            # inference reads concat(skip, up2(coarser)) in place
+            self._conv_cat("dec2", a["a2"], 64, a["a3"], 128, S2, a["d2"])
+            self._conv_cat("dec1", a["a1"], 32, a["d2"], 64, S1, a["d1"])
+            self._conv_cat("dec0", a["a0"], 16, a["d1"], 32, S0, a["d0"])
         self._last = (x, (N, D, H, W), a)
         return a["d0"]
 

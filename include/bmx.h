@@ -175,6 +175,12 @@ int bmx_uncertainty(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, 
 int bmx_conv3d_fwd(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int Ci,
                    const uint16_t *w, const float *b, uint16_t *y, int Co, int relu);
 /* dx = conv^T(dy, w) [* (xact > 0) if xact]; wt_scratch: Co*27*Ci bf16. */
+/* Forward convolution of concat(a, up2(b)) (b: Cb channels on the half-size grid,
+ * nearest 2x upsampling), without materialising either: the decoder input of the
+ * encoder-decoder at inference time.  Same output as bmx_up2 + bmx_concat +
+ * bmx_conv3d_fwd.  No counterpart in the reference. */
+int bmx_conv3d_fwd_cat(bmx_ctx *ctx, const uint16_t *a, int N, int D, int H, int W, int Ca, const uint16_t *b,
+                       int Cb, const uint16_t *w, const float *bias, uint16_t *y, int Co, int relu);
 int bmx_conv3d_dgrad(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, int Co,
                      const uint16_t *w, const uint16_t *xact, uint16_t *dx, int Ci,
                      uint16_t *wt_scratch);

@@ -222,3 +222,26 @@ def test_concat_past_2pow32_elements(eng):
     src = ((zz // 2) * H + yy // 2) * W + xx // 2
     np.testing.assert_array_equal(tail[:, :16], xa[src])
     np.testing.assert_array_equal(tail[:, 16:], xb[src])
+
+
+@pytest.mark.parametrize("ca,cb,co", [(16, 32, 16), (32, 64, 32), (64, 128, 64)])
+def test_conv_fwd_cat_equals_materialised(eng, ca, cb, co):
+    """bmx_conv3d_fwd_cat(a, b) == bmx_conv3d_fwd(concat(a, up2(b))), bit for bit."""
+    rng = np.random.default_rng(ca + cb)
+    shape = (2, 4, 6, 8)
+    half = (2, 2, 3, 4)
+    a = O.bf16_round(rng.standard_normal(shape + (ca,)))
+    b = O.bf16_round(rng.standard_normal(half + (cb,)))
+    w = O.bf16_round(rng.standard_normal((co, 27, ca + cb)) * 0.1)
+    bias = rng.standard_normal(co).astype(np.float32)
+    ab, bb, wb, biasb = up_bf16(eng, a), up_bf16(eng, b), up_bf16(eng, w), eng.upload(bias)
+    nv = int(np.prod(shape))
+    ub, cb_buf = eng.alloc(nv * cb * 2), eng.alloc(nv * (ca + cb) * 2)
+    _hip.call("bmx_up2", eng.ctx, bb.ptr, *half, cb, ub.ptr)
+    _hip.call("bmx_concat", eng.ctx, ab.ptr, ca, ub.ptr, cb, nv, cb_buf.ptr)
+    y1, y2 = eng.alloc(nv * co * 2), eng.alloc(nv * co * 2)
+    _hip.call("bmx_conv3d_fwd", eng.ctx, cb_buf.ptr, *shape, ca + cb, wb.ptr, biasb.ptr, y1.ptr, co, 1)
+    _hip.call("bmx_conv3d_fwd_cat", eng.ctx, ab.ptr, *shape, ca, bb.ptr, cb, wb.ptr, biasb.ptr, y2.ptr, co, 1)
+    np.testing.assert_array_equal(y2.download(shape + (co,), np.uint16), y1.download(shape + (co,), np.uint16))
+    ref = O.conv3d(np.concatenate([a, O.up2(b)], -1), w, bias.astype(np.float64), relu=True)
+    _close_bf16(down_bf16(y2, shape + (co,)), ref)
