@@ -16,6 +16,13 @@ algorithmic bytes (6 neighbour reads per step + one 8-byte hit RMW per
 accepted move) over the summed HIP-event time of its launches; `cpu_baseline`
 times the C restatement of the reference kernel (oracle ref_walk, float MRG
 stream) on a bounded sample of the same seeds and extrapolates.
+
+--workload batch measures the batched plain method instead: 64 uint8 volumes
+(shapes uniform in 96^3 .. 256x320x320, 2-12 structures, noise 8-20, labels on
+every 6th-12th XY plane), nbrw=10, sorw=4000, no smoothing, all walked in one
+bmx_walk_batch call with autotuned tile shape, round cap and round driver.
+`value` is batch voxels/s; the config carries the same volumes interpolated
+one call at a time on the same GPU and checks that both give identical labels.
 """
 import argparse
 import json
@@ -121,12 +128,14 @@ def main():
     ap.add_argument("--method", choices=["both", "plain", "guided"], default="both")
     ap.add_argument("--thr", type=float, default=0.9)
     ap.add_argument("--brad", type=int, default=3)
-    ap.add_argument("--srad", type=int, default=12)
+    ap.add_argument("--srad", type=int, default=8)
     ap.add_argument("--far", type=int, default=48)
     ap.add_argument("--far-walks", type=int, default=2)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["512", "batch"], default="512")
+    ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch bmx_f32_to_bf16 over 64 Mi floats (reads 256 MiB, writes "
                          "128 MiB) once, as the known-traffic reference for HBM counters")
@@ -135,6 +144,8 @@ def main():
         print("bench.py measures one GPU: every box in this setup has one MI355X; "
               "multi-GPU sharding is described, not run", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     from biomedisa_amd import _hip
     from biomedisa_amd.helpers import per_label_dice
@@ -248,6 +259,85 @@ def main():
                       "float MRG stream) on %d random seeds of this workload (%.1fs, %.3g steps/s); "
                       "full plain walk extrapolated to %.0fs; smoothing/merge not counted"
                       % (nsample, dt, rate, walk_s)}
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """The batched plain method: one bmx_walk_batch over every volume."""
+    from biomedisa_amd.synth import batch_volumes
+    t = time.perf_counter()
+    vols = batch_volumes(seed=3, n=args.batch_n, workers=8)
+    log("generated %d volumes in %.1fs" % (len(vols), time.perf_counter() - t))
+    from biomedisa_amd import _hip
+    from biomedisa_amd.autotune import tune_walk
+    from biomedisa_amd.batch import DeviceBatch, run_batch
+    from biomedisa_amd.interpolation import DeviceVolume, _prepare, run_pipeline
+    preps = [_prepare(v.img, v.labels, args.nbrw, args.sorw, False, None, False, "none", "all", 0, False,
+                      False) for v in vols]
+    assert all(p is not None for p in preps)
+    eng = _hip.engine(0)
+    batch = DeviceBatch(eng, [(p.img, p.labidx, p.bm.nol, False) for p in preps])
+    values = [p.values for p in preps]
+    n_vox = sum(int(np.prod(v.img.shape)) for v in vols)
+    log("batch: %d volumes, %d voxels (%d in label boxes), %d seeds"
+        % (len(vols), n_vox, batch.total_voxels, sum(batch.nseeds)))
+    t = time.perf_counter()
+    opts = tune_walk(batch, args.nbrw, args.sorw, 0x5EED0B10D1CE0001, log=log)
+    tune_s = time.perf_counter() - t
+    log("autotune: %s in %.1fs" % (opts, tune_s))
+    hits = eng.alloc(batch.hits_elems * 4)
+
+    def step(tm, o=opts):
+        return run_batch(batch, values, args.nbrw, args.sorw, 0, walk_opts=o, timings=tm, hits=hits)
+    out, tms, sec = time_pipeline(step, eng, args.warmup, args.steps, "batch")
+    labels = [o["regular"].download(p.img.shape, np.uint8) for o, p in zip(out, preps)]
+    _, tms_def, sec_def = time_pipeline(lambda tm: step(tm, {}), eng, 0, 1, "batch default shape")
+    # the same volumes one pipeline call at a time (what looping over smart_interpolation runs)
+    t = time.perf_counter()
+    same = True
+    for p, lab in zip(preps, labels):
+        dv = DeviceVolume(eng, p.img, p.labidx)
+        o = run_pipeline(dv, p.bm.nol, p.values, nbrw=args.nbrw, sorw=args.sorw, allaxis=False, smooth=0)
+        same &= bool(np.array_equal(o["regular"].download(p.img.shape, np.uint8), lab))
+        o["regular"].free()
+        dv.img.free()
+        dv.lab.free()
+    eng.sync()
+    loop_s = time.perf_counter() - t
+    log("per-volume loop: %.2fs, identical labels: %s" % (loop_s, same))
+    tm = tms[-1]
+    steps_total = sum(batch.nseeds) * args.nbrw * args.sorw
+    walk_bytes = steps_total * 6 + tm["accepted"] * 8
+    launches = max(int(tm["walk_launches"]), 1)
+    kernel_s = tm["walk_kernel_ms"] / 1000.0
+    achieved = walk_bytes / (tm["walk_phase_ms"] / 1000.0) / 1e9
+    line = {
+        "metric": "segmented voxels/s, 64-volume batch, plain walker",
+        "value": n_vox / sec,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * sec,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "%d uint8 volumes 96^3..256x320x320, 2-12 structures, labels every 6-12th XY "
+                               "plane, nbrw=%d sorw=%d smooth=0" % (len(vols), args.nbrw, args.sorw),
+                   "voxels": n_vox, "seeds": int(sum(batch.nseeds)), "walk_options": opts,
+                   "autotune_seconds": tune_s, "autotune_table": [[o, s] for o, s in tune_walk.last_table],
+                   "default_options_seconds": sec_def, "per_volume_loop_seconds": loop_s,
+                   "batched_vs_loop": loop_s / sec, "identical_to_per_volume_calls": same,
+                   "timings": tm},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tile_walk<uint8_t> (walk phase, graph-driven rounds time as one)",
+                     "walker_steps": steps_total, "accepted_moves": tm["accepted"], "launches": launches,
+                     "walk_kernel_ms": tm["walk_kernel_ms"], "walk_phase_ms": tm["walk_phase_ms"]},
+    }
+    hits.free()
     print(json.dumps(line), flush=True)
 
 

@@ -21,7 +21,14 @@ LABEL_NONE = 0xFF
 
 class WalkOpts(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("segment_steps", C.c_int), ("brick_log2", C.c_int),
-                ("hit_weight", C.c_int), ("seed_overrides", C.c_int)]
+                ("hit_weight", C.c_int), ("seed_overrides", C.c_int), ("tile_shape", C.c_int),
+                ("graph_rounds", C.c_int)]
+
+TILE_SHAPES = 4   # BMX_TILE_SHAPES
+
+# bmx_volume: one volume of a batched walk.
+VOLUME_DTYPE = np.dtype([("offset", "<u8"), ("Z", "<i4"), ("Y", "<i4"), ("X", "<i4"), ("nlabels", "<i4"),
+                         ("hits_offset", "<u8"), ("seed_offset", "<u8"), ("nseeds", "<u8")])
 
 SEED_DTYPE = np.dtype([("pos", "<u4"), ("label", "<u4"), ("var", "<f4"), ("reserved", "<u4")])
 
@@ -46,6 +53,8 @@ SIGNATURES = {
                       C.POINTER(C.c_uint64)]),
     "bmx_walk_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                          C.POINTER(C.c_uint64), C.POINTER(WalkOpts)]),
+    "bmx_walk_batch": (_i, [_p, _p, _i, _p, _i, _p, _i, _i, C.c_uint64, _p, C.POINTER(C.c_uint64),
+                            C.POINTER(WalkOpts)]),
     "bmx_last_walk_ms": (C.c_float, [_p]),
     "bmx_last_walk_stats": (_i, [_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(_i)]),
     "bmx_argmax": (_i, [_p, _p, _i, _i64, _p, _p]),
@@ -63,12 +72,13 @@ SIGNATURES = {
     "bmx_concat": (_i, [_p, _p, _i, _p, _i, _i64, _p]),
     "bmx_split": (_i, [_p, _p, _i64, _p, _i, _p, _i, _i]),
     "bmx_relu_mask": (_i, [_p, _p, _p, _i64]),
-    "bmx_head": (_i, [_p, _p, _i64, _i, _p, _p, _p, _p, _p, C.c_float, _p, _p, _p, _p]),
+    "bmx_head": (_i, [_p, _p, _i64, _i, _p, _p, _p, _p, _p, _p, C.c_float, _p, _p, _p, _p]),
     "bmx_adam": (_i, [_p, _p, _p, _p, _p, _p, _i64, C.c_float, C.c_float, C.c_float, C.c_float, _i,
                       C.c_float]),
     "bmx_f32_to_bf16": (_i, [_p, _p, _p, _i64]),
     "bmx_add_bf16": (_i, [_p, _p, _p, _i64]),
     "bmx_hits_target": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]),
+    "bmx_hits_index": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _p]),
     "bmx_ring_mask": (_i, [_p, _p, _i, _i, _i, _i, _p, _p]),
     "bmx_seed_walks": (_i, [_p, _p, _i64, _i, _i]),
     "bmx_net_input": (_i, [_p, _p, _i, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
@@ -217,19 +227,34 @@ class Engine:
         return buf, n.value
 
     def walk(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits,
-             algorithm=WALK_AUTO, segment_steps=0, brick_log2=0, hit_weight=1, seed_overrides=False):
+             algorithm=WALK_AUTO, segment_steps=0, brick_log2=0, hit_weight=1, seed_overrides=False,
+             tile_shape=0, graph_rounds=0):
         """Accumulate the walks of `nseeds` seeds into `hits` (each accepted move adds
         `hit_weight`, times the seed's own weight with seed_overrides); returns the
         accepted moves."""
         Z, Y, X = dims
         acc = C.c_uint64()
         opts = WalkOpts(int(algorithm), int(segment_steps), int(brick_log2), int(hit_weight),
-                        int(bool(seed_overrides)))
+                        int(bool(seed_overrides)), int(tile_shape), int(graph_rounds))
         _check(load().bmx_walk_ex(self.ctx, img.ptr, img_dtype, Z, Y, X,
                                   seeds.ptr if seeds is not None else None, int(nseeds),
                                   int(nlabels), int(nbrw), int(sorw), int(rng),
                                   C.c_uint64(int(key) & (2**64 - 1)), hits.ptr, C.byref(acc),
                                   C.byref(opts)))
+        return acc.value
+
+    def walk_batch(self, img, img_dtype, volumes, seeds, nbrw, sorw, key, hits, segment_steps=0,
+                   hit_weight=1, seed_overrides=False, tile_shape=0, graph_rounds=0):
+        """Tiled walk of every volume of a batch (`volumes`: VOLUME_DTYPE array, image,
+        seeds and hit blocks at the offsets it gives); returns the accepted moves."""
+        vols = np.ascontiguousarray(volumes, dtype=VOLUME_DTYPE)
+        acc = C.c_uint64()
+        opts = WalkOpts(WALK_TILED, int(segment_steps), 0, int(hit_weight), int(bool(seed_overrides)),
+                        int(tile_shape), int(graph_rounds))
+        _check(load().bmx_walk_batch(self.ctx, img.ptr, img_dtype, vols.ctypes.data, len(vols),
+                                     seeds.ptr if seeds is not None else None, int(nbrw), int(sorw),
+                                     C.c_uint64(int(key) & (2**64 - 1)), hits.ptr, C.byref(acc),
+                                     C.byref(opts)))
         return acc.value
 
     def last_walk_ms(self):
@@ -269,6 +294,12 @@ class BufferView:
         self.ptr = buf.ptr + int(offset)
         self.nbytes = buf.nbytes - int(offset) if nbytes is None else int(nbytes)
         assert offset + self.nbytes <= buf.nbytes
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(load().bmx_download(self.base.engine.ctx, out.ctypes.data, self.ptr, out.nbytes))
+        return out
 
 
 def ptr(buf):

@@ -43,7 +43,7 @@ int bmx_open(int device, bmx_ctx **out)
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-    if (e == hipSuccess) e = hipMalloc((void **)&c->d_counter, 8 * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_counter, 16 * sizeof(uint64_t));
     if (e != hipSuccess) {
         free(c);
         return bmx_set_error(BMX_E_DEVICE, "bmx_open: %s", hipGetErrorString(e));
@@ -63,6 +63,7 @@ int bmx_close(bmx_ctx *ctx)
     if (ctx->d_keys) BMX_HIP(hipFree(ctx->d_keys));
     if (ctx->d_buckets) BMX_HIP(hipFree(ctx->d_buckets));
     if (ctx->d_scan) BMX_HIP(hipFree(ctx->d_scan));
+    if (ctx->d_vols) BMX_HIP(hipFree(ctx->d_vols));
     if (ctx->seg_ev) {
         for (int k = 0; k < 2 * 1024; ++k) BMX_HIP(hipEventDestroy(ctx->seg_ev[k]));
         free(ctx->seg_ev);

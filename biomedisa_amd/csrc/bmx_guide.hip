@@ -21,9 +21,10 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Network input on a padded grid (Zp,Yp,Xp) >= (Z,Y,X), 16 bf16 channels:
+// Network input on a padded grid (Zp,Yp,Xp) >= (Z,Y,X), 32 bf16 channels:
 // 0: image/127.5 - 1; 1: 1 on seed voxels (labelled planes, label not
-// ignored); 2 + l: one-hot of label index l on seed voxels (l < 14).
+// ignored); 2 + l: one-hot of label index l on seed voxels (l < 30).
+constexpr int kNetIn = 32;
 template <typename T>
 __global__ void k_net_input(const T *__restrict__ img, const uint8_t *__restrict__ lab,
                             const uint8_t *__restrict__ flags, int Z, int Y, int X, int Zp, int Yp, int Xp,
@@ -33,9 +34,9 @@ __global__ void k_net_input(const T *__restrict__ img, const uint8_t *__restrict
     const int64_t np_ = (int64_t)Zp * Yp * Xp;
     if (v >= np_) return;
     const int x = (int)(v % Xp), y = (int)((v / Xp) % Yp), z = (int)(v / ((int64_t)Xp * Yp));
-    uint16_t ch[16];
+    uint16_t ch[kNetIn];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) ch[c] = 0;
+    for (int c = 0; c < kNetIn; ++c) ch[c] = 0;
     if (z < Z && y < Y && x < X) {
         const int64_t i = ((int64_t)z * Y + y) * X + x;
         ch[0] = f2bf((float)img[i] * (1.0f / 127.5f) - 1.0f);
@@ -43,13 +44,14 @@ __global__ void k_net_input(const T *__restrict__ img, const uint8_t *__restrict
         const uint8_t l = lab[i];
         if (plane && l != BMX_LABEL_NONE) {
             ch[1] = 0x3F80;  // 1.0
-            if (l < 14) ch[2 + l] = 0x3F80;
+            if (l < kNetIn - 2) ch[2 + l] = 0x3F80;
         }
     }
-    uint4 *o = (uint4 *)(out + v * 16);
-    o[0] = make_uint4(ch[0] | (ch[1] << 16), ch[2] | (ch[3] << 16), ch[4] | (ch[5] << 16), ch[6] | (ch[7] << 16));
-    o[1] = make_uint4(ch[8] | (ch[9] << 16), ch[10] | (ch[11] << 16), ch[12] | (ch[13] << 16),
-                      ch[14] | (ch[15] << 16));
+    uint4 *o = (uint4 *)(out + v * kNetIn);
+#pragma unroll
+    for (int q = 0; q < kNetIn / 8; ++q)
+        o[q] = make_uint4(ch[8 * q] | (ch[8 * q + 1] << 16), ch[8 * q + 2] | (ch[8 * q + 3] << 16),
+                          ch[8 * q + 4] | (ch[8 * q + 5] << 16), ch[8 * q + 6] | (ch[8 * q + 7] << 16));
 }
 
 // 1x1x1 head on the padded grid -> per voxel of the (Z,Y,X) volume: argmax
@@ -69,7 +71,7 @@ __global__ void k_head_predict(const uint16_t *__restrict__ x, int Z, int Y, int
     for (int c = 0; c < 16; ++c) xin[c] = bf2f(x[p * 16 + c]);
     float best = -1e30f, den = 0.0f;
     int bl = 0;
-    float lg[16];
+    float lg[32];
     for (int l = 0; l < L; ++l) {
         float s = b[l];
         for (int c = 0; c < 16; ++c) s += W[l * 16 + c] * xin[c];
@@ -197,6 +199,29 @@ __global__ void k_hits_target(const uint32_t *__restrict__ hits, int nlabels, in
     vmask[v] = ok ? 1 : 0;
 }
 
+// Training label of every grid voxel: the first argmax index of the walker's
+// hits (0 when all counts are 0, as bmx_argmax), and the in-volume mask.
+__global__ void k_hits_index(const uint32_t *__restrict__ hits, int nlabels, int Z, int Y, int X, int Zp, int Yp,
+                             int Xp, uint8_t *__restrict__ idx, uint8_t *__restrict__ vmask)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np_ = (int64_t)Zp * Yp * Xp;
+    if (v >= np_) return;
+    const int x = (int)(v % Xp), y = (int)((v / Xp) % Yp), z = (int)(v / ((int64_t)Xp * Yp));
+    const bool in = z < Z && y < Y && x < X;
+    int best = 0;
+    if (in) {
+        const int64_t V = (int64_t)Z * Y * X, i = ((int64_t)z * Y + y) * X + x;
+        uint32_t bh = 0;
+        for (int l = 0; l < nlabels; ++l) {
+            const uint32_t h = hits[(int64_t)l * V + i];
+            if (h > bh) { bh = h; best = l; }
+        }
+    }
+    idx[v] = (uint8_t)best;
+    vmask[v] = in ? 1 : 0;
+}
+
 int dilate(bmx_ctx *ctx, uint8_t *buf, uint8_t *tmp, int Z, int Y, int X, int r)
 {
     if (r <= 0) return BMX_OK;
@@ -237,7 +262,7 @@ int bmx_head_predict(bmx_ctx *ctx, const uint16_t *x, int Z, int Y, int X, int Z
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
-    BMX_CHECK(x && W && b && pred && unc && L >= 1 && L <= 16, "bmx_head_predict: bad arguments");
+    BMX_CHECK(x && W && b && pred && unc && L >= 1 && L <= 32, "bmx_head_predict: bad arguments");
     const int64_t V = (int64_t)Z * Y * X;
     BMX_CHECK(V < ((int64_t)1 << 32) - 256, "bmx_head_predict: volume exceeds one dispatch");
     k_head_predict<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(x, Z, Y, X, Zp, Yp, Xp, L, W, b, thr, pred, unc);
@@ -305,6 +330,20 @@ int bmx_hits_target(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int 
     BMX_CHECK(np_ < ((int64_t)1 << 32) - 256, "bmx_hits_target: grid exceeds one dispatch");
     k_hits_target<<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>(hits, nlabels, Z, Y, X, Zp, Yp, Xp, L, hard,
                                                                 target, vmask);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_hits_index(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int Y, int X, int Zp, int Yp, int Xp,
+                   uint8_t *idx, uint8_t *vmask)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(hits && idx && vmask && nlabels >= 1 && nlabels <= 255, "bmx_hits_index: bad arguments");
+    BMX_CHECK(Zp >= Z && Yp >= Y && Xp >= X, "bmx_hits_index: padded grid smaller than the volume");
+    const int64_t np_ = (int64_t)Zp * Yp * Xp;
+    BMX_CHECK(np_ < ((int64_t)1 << 32) - 256, "bmx_hits_index: grid exceeds one dispatch");
+    k_hits_index<<<bmx_blocks(np_, 256), 256, 0, ctx->stream>>>(hits, nlabels, Z, Y, X, Zp, Yp, Xp, idx, vmask);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }

@@ -19,7 +19,7 @@ struct bmx_ctx {
     hipStream_t stream;
     hipEvent_t ev0, ev1;     // bracket the last walker launch (bmx_last_walk_ms)
     float last_walk_ms;
-    uint64_t *d_counter;     // 8 x uint64 scratch counters (accepted moves, seed counts)
+    uint64_t *d_counter;     // 16 x uint64 scratch: accepted moves, seed counts, tiled-walk round control
     int64_t *d_block_counts; // seed compaction scratch
     int64_t block_counts_cap;
     void *d_rec;             // packed neighbour records of the walker
@@ -32,6 +32,8 @@ struct bmx_ctx {
     size_t buckets_cap;
     void *d_scan;            // exclusive-scan block partials
     size_t scan_cap;
+    void *d_vols;            // tiled walker: volume table of the walk
+    size_t vols_cap;
     hipEvent_t *seg_ev;      // sorted walker: event pairs around each walk launch
     float last_walk_kernel_ms;
     int last_walk_launches;
@@ -58,11 +60,16 @@ int bmx_set_error(int code, const char *fmt, ...);
 
 int bmx_set_device(bmx_ctx *ctx);
 
-// Tile-owner walker (bmx_tile.hip), called by bmx_walk_ex.
-int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
+// Tile-owner walker (bmx_tile.hip), called by bmx_walk_ex (one volume) and
+// bmx_walk_batch.  vols is a host array of nvol validated volumes.  On return
+// *launches = walk rounds; *kernel_ms = -1 for host-driven rounds (walk
+// launches bracketed by ev[2k], ev[2k+1]), else the summed walk-kernel time of
+// the graph-driven rounds (device wall clock), or -2 where it is unknown.
+int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volume *vols, int nvol,
                    const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
                    uint32_t weight, int overrides, uint32_t k0, uint32_t k1, uint32_t *hits,
-                   unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches);
+                   unsigned long long *acc, int shape, int graph_pairs, hipEvent_t *ev, int max_timed,
+                   int *launches, float *kernel_ms);
 
 // Device arithmetic -----------------------------------------------------------
 

@@ -397,11 +397,13 @@ __global__ void k_split(const uint16_t *__restrict__ y, int64_t nvox, uint16_t *
 __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, int64_t nvox, int L,
                                               const float *__restrict__ W, const float *__restrict__ b,
                                               float *__restrict__ prob, const float *__restrict__ target,
+                                              const uint8_t *__restrict__ tidx,
                                               const uint8_t *__restrict__ vmask, float scale,
                                               uint16_t *__restrict__ dx, float *__restrict__ dW,
                                               float *__restrict__ db, float *__restrict__ loss)
 {
-    constexpr int C = 16, LM = 16;
+    constexpr int C = 16, LM = 32;
+    const bool train = target || tidx;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = v < nvox;
     float xin[C];
@@ -430,15 +432,15 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, in
         if (l < L) {
             const float p = lg[l] / den;
             if (act && prob) prob[v * L + l] = p;
-            if (target) {
+            if (train) {
                 const float w = (act && (!vmask || vmask[v])) ? 1.0f : 0.0f;
-                const float t = act ? target[v * L + l] : 0.0f;
+                const float t = !act ? 0.0f : (target ? target[v * L + l] : (tidx[v] == l ? 1.0f : 0.0f));
                 gl[l] = w * scale * (p - t);
                 if (w > 0.0f && t > 0.0f) ls -= t * __logf(fmaxf(p, 1e-30f));
             }
         }
     }
-    if (!target) return;
+    if (!train) return;
     if (act) {
         for (int c = 0; c < C; ++c) {
             float s = 0.0f;
@@ -658,17 +660,18 @@ int bmx_split(bmx_ctx *ctx, const uint16_t *y, int64_t nvox, uint16_t *a, int Ca
 }
 
 int bmx_head(bmx_ctx *ctx, const uint16_t *x, int64_t nvox, int L, const float *W, const float *b, float *prob,
-             const float *target, const uint8_t *vmask, float scale, uint16_t *dx, float *dW, float *db,
-             float *loss)
+             const float *target, const uint8_t *target_idx, const uint8_t *vmask, float scale, uint16_t *dx,
+             float *dW, float *db, float *loss)
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
-    BMX_CHECK(x && W && b && L >= 1 && L <= 16, "bmx_head: bad arguments (L <= 16)");
-    BMX_CHECK(!target || (dx && dW && db && loss), "bmx_head: training needs dx, dW, db, loss");
+    BMX_CHECK(x && W && b && L >= 1 && L <= 32, "bmx_head: bad arguments (L <= 32)");
+    BMX_CHECK(!(target && target_idx), "bmx_head: give soft targets or target indices, not both");
+    BMX_CHECK(!(target || target_idx) || (dx && dW && db && loss), "bmx_head: training needs dx, dW, db, loss");
     BMX_CHECK(nvox >= 0 && nvox < ((int64_t)1 << 32) - 256, "bmx_head: nvox %lld exceeds one dispatch",
               (long long)nvox);
-    k_head<<<bmx_blocks(nvox, 256), 256, 0, ctx->stream>>>(x, nvox, L, W, b, prob, target, vmask, scale, dx,
-                                                          dW, db, loss);
+    k_head<<<bmx_blocks(nvox, 256), 256, 0, ctx->stream>>>(x, nvox, L, W, b, prob, target, target_idx, vmask,
+                                                          scale, dx, dW, db, loss);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }

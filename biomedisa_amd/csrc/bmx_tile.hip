@@ -21,58 +21,116 @@
 // Philox block is used whole within one round.
 #include "bmx_walk_common.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace {
 
-constexpr int kTileThreads = 1024;
+// Tile shapes: core side C, halo H and workgroup size NTH.  Shape 0 is the
+// default (one 135 KB / 140 KB workgroup per CU); the others trade fewer
+// voxels per round for more resident workgroups, which can pay on small
+// volumes.  BMX_TILE_SHAPES of them are compiled; bmx_walk_opts.tile_shape
+// picks one (the walk autotuner times them).
+template <typename T, int S> struct TileShape;
+template <> struct TileShape<uint8_t, 0> { static constexpr int C = 16, H = 6, NTH = 1024; };
+template <> struct TileShape<uint8_t, 1> { static constexpr int C = 12, H = 5, NTH = 1024; };
+template <> struct TileShape<uint8_t, 2> { static constexpr int C = 12, H = 5, NTH = 512; };
+template <> struct TileShape<uint8_t, 3> { static constexpr int C = 8, H = 4, NTH = 512; };
+template <> struct TileShape<float, 0> { static constexpr int C = 16, H = 4, NTH = 1024; };
+template <> struct TileShape<float, 1> { static constexpr int C = 12, H = 3, NTH = 1024; };
+template <> struct TileShape<float, 2> { static constexpr int C = 12, H = 3, NTH = 512; };
+template <> struct TileShape<float, 3> { static constexpr int C = 8, H = 3, NTH = 512; };
+static_assert(BMX_TILE_SHAPES == 4, "one TileShape per shape index");
 
-template <typename T> struct TileCfg;
-template <> struct TileCfg<uint8_t> {
-    static constexpr int C = 16, H = 6;
-};
-template <> struct TileCfg<float> {
-    static constexpr int C = 16, H = 4;
-};
+template <typename T, int S> constexpr int tile_RI() { return TileShape<T, S>::C + 2 * TileShape<T, S>::H + 2; }
+template <typename T, int S> constexpr size_t tile_img_bytes()
+{
+    return ((size_t)tile_RI<T, S>() * tile_RI<T, S>() * tile_RI<T, S>() * sizeof(T) + 15) & ~(size_t)15;
+}
+template <typename T, int S> constexpr size_t tile_lds_bytes()
+{
+    return tile_img_bytes<T, S>() + (size_t)tile_RI<T, S>() * tile_RI<T, S>() * tile_RI<T, S>() * sizeof(uint32_t);
+}
 
-template <typename T> constexpr int tile_R() { return TileCfg<T>::C + 2 * TileCfg<T>::H; }
-template <typename T> constexpr size_t tile_img_bytes()
-{
-    return ((size_t)(tile_R<T>() + 2) * (tile_R<T>() + 2) * (tile_R<T>() + 2) * sizeof(T) + 15) & ~(size_t)15;
-}
-template <typename T> constexpr size_t tile_lds_bytes()
-{
-    return tile_img_bytes<T>() +
-           (size_t)(tile_R<T>() + 2) * (tile_R<T>() + 2) * (tile_R<T>() + 2) * sizeof(uint32_t);
-}
+// One volume of a walk (a batch walks many at once): voxels [base, base + V)
+// of the concatenated image, z-major with x fastest; tiles [tile_base,
+// tile_base + ntz*nty*ntx) of the batch's tile numbering; seeds [seed_base,
+// next volume's seed_base) of the seed array, with volume-local positions; its
+// hit block [labels][V] at element offset hits_base.  Walker positions are
+// batch-global (base + local position).
+struct VolGeom {
+    uint32_t base;
+    int Z, Y, X;
+    uint32_t sy, sz, V;
+    uint32_t tile_base, nty, ntx;
+    uint32_t seed_base, pad;
+    uint64_t hits_base;
+};
 
 struct TileGeom {
-    uint32_t ntz, nty, ntx, nlabels;
+    const VolGeom *vols;
+    int nvol;
+    uint32_t nlabels;   // label slots per tile key (the largest label count of the batch)
 };
 
-template <int C>
-__device__ __forceinline__ uint32_t tile_key(uint32_t pos, uint32_t label, const WalkGeom &g,
-                                             const TileGeom &tg)
+// Last volume whose first voxel (resp. first tile) is <= pos (resp. tile).
+__device__ __forceinline__ int vol_of_pos(const VolGeom *__restrict__ vols, int nvol, uint32_t pos)
 {
-    const uint32_t z = pos / g.sz;
-    const uint32_t r = pos - z * g.sz;
-    const uint32_t y = r / g.sy;
-    const uint32_t x = r - y * g.sy;
-    return (((z / C) * tg.nty + y / C) * tg.ntx + x / C) * tg.nlabels + label;
+    int lo = 0, hi = nvol - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vols[mid].base <= pos) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int vol_of_seed(const VolGeom *__restrict__ vols, int nvol, uint32_t seed)
+{
+    int lo = 0, hi = nvol - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vols[mid].seed_base <= seed) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int vol_of_tile(const VolGeom *__restrict__ vols, int nvol, uint32_t tile)
+{
+    int lo = 0, hi = nvol - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vols[mid].tile_base <= tile) lo = mid; else hi = mid - 1;
+    }
+    return lo;
 }
 
-// Walker state: {pos, seed index, walk | label << 16, step}.  With overrides a
-// seed running fewer than nbrw walks gets its extra walkers created finished
-// (step = sorw), so the first re-bucketing drops them.
+template <int C>
+__device__ __forceinline__ uint32_t tile_key(uint32_t pos, uint32_t label, const TileGeom &tg)
+{
+    const VolGeom &g = tg.vols[vol_of_pos(tg.vols, tg.nvol, pos)];
+    const uint32_t r0 = pos - g.base;
+    const uint32_t z = r0 / g.sz;
+    const uint32_t r = r0 - z * g.sz;
+    const uint32_t y = r / g.sy;
+    const uint32_t x = r - y * g.sy;
+    return (g.tile_base + ((z / C) * g.nty + y / C) * g.ntx + x / C) * tg.nlabels + label;
+}
+
+// Walker state: {batch-global position, seed index, walk | label << 16, step},
+// one per (seed, walk), seed-major.  With overrides a seed running fewer than
+// nbrw walks gets its extra walkers created finished (step = sorw), so the
+// first re-bucketing drops them.
 __global__ __launch_bounds__(256) void k_tile_init(const bmx_seed *__restrict__ seeds, int64_t nseeds,
-                                                   int nbrw, int sorw, int overrides, uint4 *__restrict__ st)
+                                                   int nbrw, int sorw, int overrides, TileGeom tg,
+                                                   uint4 *__restrict__ st)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nseeds * nbrw) return;
     const int64_t s = i / nbrw;
     const bmx_seed sd = seeds[s];
+    const uint32_t base = tg.vols[vol_of_seed(tg.vols, tg.nvol, (uint32_t)s)].base;
     const uint32_t w = (uint32_t)(i - s * nbrw);
     const uint32_t walks = overrides ? (sd.reserved & 0xFFFFu) : 0u;
     const uint32_t step0 = (walks != 0u && w >= walks) ? (uint32_t)sorw : 0u;
-    st[i] = make_uint4(sd.pos, (uint32_t)s, w | (sd.label << 16), step0);
+    st[i] = make_uint4(base + sd.pos, (uint32_t)s, w | (sd.label << 16), step0);
 }
 
 __global__ __launch_bounds__(256) void k_seed_walks(bmx_seed *__restrict__ seeds, int64_t n, uint32_t value)
@@ -81,44 +139,92 @@ __global__ __launch_bounds__(256) void k_seed_walks(bmx_seed *__restrict__ seeds
     if (i < n) seeds[i].reserved = value;
 }
 
+// Round bookkeeping in device memory, so that a round needs no host values
+// and a sequence of rounds can be captured in one graph: n = walkers in the
+// current state array, nb = non-empty buckets, bnext = next bucket to walk.
+struct RoundCtl {
+    uint32_t nb, bnext, n, done;
+    // persistent walk kernels: first start and last end of the current round
+    // (device wall clock), and the summed spans of the rounds so far
+    unsigned long long t0, t1, ticks;
+};
+
+// Start of a round: clear the key counts and the bucket counters.
+__global__ __launch_bounds__(256) void k_round_begin(uint32_t *__restrict__ counts, int64_t ncounts,
+                                                     RoundCtl *__restrict__ ctl)
+{
+    for (int64_t i = bmx_gtid(); i < ncounts; i += bmx_gstride()) counts[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->nb = 0u;
+        ctl->bnext = 0u;
+    }
+}
+
+// End of a round's re-bucketing: the live walkers (the key-count total) become
+// the next round's n.
+__global__ void k_round_end(const uint32_t *__restrict__ counts, int64_t nkeys, RoundCtl *__restrict__ ctl)
+{
+    if (threadIdx.x == 0) ctl->n = counts[nkeys];
+}
+
+__global__ void k_set_count(RoundCtl *__restrict__ ctl, uint32_t n)
+{
+    if (threadIdx.x == 0) {
+        ctl->n = n;
+        ctl->done = 0u;
+        ctl->t0 = ~0ull;
+        ctl->t1 = 0ull;
+        ctl->ticks = 0ull;
+    }
+}
+
+// Walker counts per key.  The loop bound is uniform per block, as
+// agg_atomic_inc needs every lane of a wave.
 template <int C>
-__global__ __launch_bounds__(256) void k_tile_hist(const uint4 *__restrict__ st, int64_t n, WalkGeom g,
+__global__ __launch_bounds__(256) void k_tile_hist(const uint4 *__restrict__ st, const RoundCtl *__restrict__ ctl,
                                                    TileGeom tg, int sorw, uint32_t *counts)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    bool live = false;
-    uint32_t key = 0;
-    if (i < n) {
-        const uint4 w4 = st[i];
-        live = (int)w4.w < sorw;
-        key = tile_key<C>(w4.x, w4.z >> 16, g, tg);
+    const int64_t n = ctl->n;
+    for (int64_t b0 = (int64_t)blockIdx.x * 256; b0 < n; b0 += (int64_t)gridDim.x * 256) {
+        const int64_t i = b0 + threadIdx.x;
+        bool live = false;
+        uint32_t key = 0;
+        if (i < n) {
+            const uint4 w4 = st[i];
+            live = (int)w4.w < sorw;
+            key = tile_key<C>(w4.x, w4.z >> 16, tg);
+        }
+        (void)agg_atomic_inc(counts, key, live);
     }
-    (void)agg_atomic_inc(counts, key, live);
 }
 
 template <int C>
-__global__ __launch_bounds__(256) void k_tile_scatter(const uint4 *__restrict__ in, int64_t n,
-                                                      uint4 *__restrict__ out, uint32_t *cursor,
-                                                      WalkGeom g, TileGeom tg, int sorw)
+__global__ __launch_bounds__(256) void k_tile_scatter(const uint4 *__restrict__ in, const RoundCtl *__restrict__ ctl,
+                                                      uint4 *__restrict__ out, uint32_t *cursor, TileGeom tg,
+                                                      int sorw)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    uint4 w4 = make_uint4(0, 0, 0, 0);
-    bool live = false;
-    uint32_t key = 0;
-    if (i < n) {
-        w4 = in[i];
-        live = (int)w4.w < sorw;
-        key = tile_key<C>(w4.x, w4.z >> 16, g, tg);
+    const int64_t n = ctl->n;
+    for (int64_t b0 = (int64_t)blockIdx.x * 256; b0 < n; b0 += (int64_t)gridDim.x * 256) {
+        const int64_t i = b0 + threadIdx.x;
+        uint4 w4 = make_uint4(0, 0, 0, 0);
+        bool live = false;
+        uint32_t key = 0;
+        if (i < n) {
+            w4 = in[i];
+            live = (int)w4.w < sorw;
+            key = tile_key<C>(w4.x, w4.z >> 16, tg);
+        }
+        const uint32_t dst = agg_atomic_inc(cursor, key, live);
+        if (live) out[dst] = w4;
     }
-    const uint32_t dst = agg_atomic_inc(cursor, key, live);
-    if (live) out[dst] = w4;
 }
 
 // Non-empty buckets of the exclusive offsets off[0..nkeys] -> (key, offset, count).
 __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict__ off, int64_t nkeys,
                                                      uint32_t *bkey, uint32_t *boff, uint32_t *bcnt,
-                                                     uint32_t *nb)
+                                                     RoundCtl *ctl)
 {
+    uint32_t *nb = &ctl->nb;
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0, o = 0;
     if (k < nkeys) {
@@ -148,7 +254,7 @@ __device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
 // [lmin, lmax] per axis (the local range of interior voxels); otherwise every
 // move is interior.  A step that starts inside the region lands inside the
 // tile, so its hit always goes to the LDS counters.
-template <typename T, bool EDGE>
+template <typename T, int S, bool EDGE>
 __device__ __forceinline__ void tile_block(const T *__restrict__ img, const WalkGeom &g, const T *simg,
                                            uint32_t *shits, uint32_t *__restrict__ hg, int sorw,
                                            const uint32_t rr[4], float B, float div1, int &step, int &li,
@@ -156,7 +262,7 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
                                            int lmin1, int lmin2, int lmax0, int lmax1, int lmax2,
                                            uint32_t weight, uint32_t &acc)
 {
-    constexpr int R = TileCfg<T>::C + 2 * TileCfg<T>::H, RI = R + 2;
+    constexpr int R = TileShape<T, S>::C + 2 * TileShape<T, S>::H, RI = R + 2;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (step < sorw) {
@@ -211,193 +317,413 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kTileThreads) void k_tile_walk(
-    const T *__restrict__ img, WalkGeom g, TileGeom tg, const bmx_seed *__restrict__ seeds,
+// Walks the round's buckets as described at the top of this file.  PERSIST =
+// false: workgroup b walks bucket b (the grid is ctl->nb).  PERSIST = true: a
+// grid of resident workgroups takes buckets from ctl->bnext until all ctl->nb
+// are taken (a fixed grid, as a captured graph needs).  The loop keeps more
+// values live across buckets (about 80 VGPRs against 44), which costs nothing
+// where LDS already limits a CU to one workgroup.
+template <typename T, int S, bool PERSIST>
+__global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
+    const T *__restrict__ img_all, TileGeom tg, const bmx_seed *__restrict__ seeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
-    const uint32_t *__restrict__ bcnt, int sorw, int cap, uint32_t weight0, int overrides, uint32_t k0,
-    uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted)
+    const uint32_t *__restrict__ bcnt, RoundCtl *__restrict__ ctl, int sorw, int cap, uint32_t weight0,
+    int overrides, uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted)
 {
-    constexpr int C = TileCfg<T>::C, H = TileCfg<T>::H, R = C + 2 * H, RI = R + 2;
+    using Sh = TileShape<T, S>;
+    constexpr int C = Sh::C, H = Sh::H, R = C + 2 * H, RI = R + 2, NTH = Sh::NTH;
     constexpr int NT = RI * RI * RI;
     extern __shared__ __align__(16) unsigned char smem[];
     T *simg = (T *)smem;
-    uint32_t *shits = (uint32_t *)(smem + tile_img_bytes<T>());
-    __shared__ uint32_t next;
+    uint32_t *shits = (uint32_t *)(smem + tile_img_bytes<T, S>());
+    __shared__ uint32_t next, bucket;
 
-    const uint32_t key = bkey[blockIdx.x];
-    const uint32_t off = boff[blockIdx.x], cnt = bcnt[blockIdx.x];
-    const uint32_t label = key % tg.nlabels;
-    const uint32_t tile = key / tg.nlabels;
-    const int tx = (int)(tile % tg.ntx), ty = (int)((tile / tg.ntx) % tg.nty),
-              tz = (int)(tile / (tg.ntx * tg.nty));
-    // global coordinates of tile voxel (0,0,0)
-    const int bz = tz * C - H - 1, by = ty * C - H - 1, bx = tx * C - H - 1;
+    const uint32_t nb = ctl->nb;
+    if (PERSIST && threadIdx.x == 0) atomicMin(&ctl->t0, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+    while (true) {
+        if (threadIdx.x == 0) {
+            bucket = PERSIST ? atomicAdd(&ctl->bnext, 1u) : blockIdx.x;
+            next = 0u;
+        }
+        __syncthreads();
+        // the same value in every thread: made wave-uniform so that the bucket's
+        // geometry lives in scalar registers
+        const uint32_t bi = PERSIST ? __builtin_amdgcn_readfirstlane(bucket) : blockIdx.x;
+        if (bi >= nb) break;
+        const uint32_t key = bkey[bi];
+        const uint32_t off = boff[bi], cnt = bcnt[bi];
+        const uint32_t label = key % tg.nlabels;
+        const uint32_t tile = key / tg.nlabels;
+        // the bucket's volume: geometry, image and hit block in volume-local indices
+        const VolGeom gv = tg.vols[vol_of_tile(tg.vols, tg.nvol, tile)];
+        const WalkGeom g{gv.Z, gv.Y, gv.X, gv.sy, gv.sz, (int64_t)gv.V};
+        const T *__restrict__ img = img_all + gv.base;
+        const uint32_t lt = tile - gv.tile_base;
+        const int tx = (int)(lt % gv.ntx), ty = (int)((lt / gv.ntx) % gv.nty),
+                  tz = (int)(lt / (gv.ntx * gv.nty));
+        // volume coordinates of tile voxel (0,0,0)
+        const int bz = tz * C - H - 1, by = ty * C - H - 1, bx = tx * C - H - 1;
 
-    // Tile load: each thread issues a batch of image loads before storing them to
-    // LDS, so one memory latency is paid per batch rather than per element; the
-    // hit counters are cleared with 16-byte stores.
-    {
-        constexpr int BATCH = 8;
-        for (int base = 0; base < NT; base += BATCH * kTileThreads) {
-            T v[BATCH];
+        // Tile load: each thread issues a batch of image loads before storing them
+        // to LDS, so one memory latency is paid per batch rather than per element;
+        // the hit counters are cleared with 16-byte stores.
+        // tid: threadIdx.x made opaque to the compiler, so that the per-thread tile
+        // addresses below are recomputed per bucket rather than hoisted out of
+        // the bucket loop, where they would hold registers through the walk
+        int tid;
+        asm volatile("" : "=v"(tid) : "0"((int)threadIdx.x));
+        {
+            constexpr int BATCH = 8;
+            for (int base = 0; base < NT; base += BATCH * NTH) {
+                T v[BATCH];
 #pragma unroll
-            for (int q = 0; q < BATCH; ++q) {
-                const int k = base + threadIdx.x + q * kTileThreads;
-                if (k < NT) {
-                    const int lz = k / (RI * RI), ly = (k / RI) % RI, lx = k % RI;
-                    const int gz = min(max(bz + lz, 0), g.Z - 1);
-                    const int gy = min(max(by + ly, 0), g.Y - 1);
-                    const int gx = min(max(bx + lx, 0), g.X - 1);
-                    v[q] = img[(int64_t)gz * g.sz + (int64_t)gy * g.sy + gx];
+                for (int q = 0; q < BATCH; ++q) {
+                    const int k = base + tid + q * NTH;
+                    if (k < NT) {
+                        const int lz = k / (RI * RI), ly = (k / RI) % RI, lx = k % RI;
+                        const int gz = min(max(bz + lz, 0), g.Z - 1);
+                        const int gy = min(max(by + ly, 0), g.Y - 1);
+                        const int gx = min(max(bx + lx, 0), g.X - 1);
+                        v[q] = img[(int64_t)gz * g.sz + (int64_t)gy * g.sy + gx];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < BATCH; ++q) {
+                    const int k = base + tid + q * NTH;
+                    if (k < NT) simg[k] = v[q];
                 }
             }
-#pragma unroll
-            for (int q = 0; q < BATCH; ++q) {
-                const int k = base + threadIdx.x + q * kTileThreads;
-                if (k < NT) simg[k] = v[q];
+            uint4 *sh4 = (uint4 *)shits;
+            for (int k = tid; k < NT / 4; k += NTH) sh4[k] = make_uint4(0u, 0u, 0u, 0u);
+            for (int k = (NT / 4) * 4 + tid; k < NT; k += NTH) shits[k] = 0u;
+        }
+        __syncthreads();
+
+        // Interior voxels are 1..dim-2; in tile coordinates 1-b..dim-2-b.  A
+        // walker is never more than 3 steps outside the region, so a tile whose
+        // region grown by 4 stays inside the interior needs no move checks.
+        const int lmin0 = 1 - bz, lmin1 = 1 - by, lmin2 = 1 - bx;
+        const int lmax0 = g.Z - 2 - bz, lmax1 = g.Y - 2 - by, lmax2 = g.X - 2 - bx;
+        const bool edge = (lmin0 > -3) | (lmin1 > -3) | (lmin2 > -3) | (lmax0 < R + 4) |
+                          (lmax1 < R + 4) | (lmax2 < R + 4);
+
+        uint32_t *__restrict__ hg = hits + gv.hits_base + (int64_t)label * gv.V;
+        uint32_t acc = 0;  // accepted moves of this thread in this bucket (< cnt * cap < 2^32)
+        bool have = false;
+        uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = weight0;
+        int step = 0, lz = 0, ly = 0, lx = 0, li = 0, taken = 0;
+        float B = 0.0f, div1 = 0.0f;
+        while (true) {
+            if (!have) {
+                j = atomicAdd(&next, 1u);
+                if (j >= cnt) break;
+                const uint4 w4 = st[off + j];
+                const uint32_t pos = w4.x - gv.base;
+                seed = w4.y;
+                wl = w4.z;
+                step = (int)w4.w;
+                const bmx_seed sd = seeds[seed];
+                p0 = sd.pos;   // volume-local: a seed's Philox stream does not depend on the batch
+                B = bmx_ld(img, (int64_t)p0);
+                div1 = 1.0f / (2.0f * sd.var);
+                weight = weight0;
+                if (overrides && (sd.reserved >> 16) != 0u) weight = weight0 * (sd.reserved >> 16);
+                const int z = (int)(pos / g.sz);
+                const uint32_t r = pos - (uint32_t)z * g.sz;
+                const int y = (int)(r / g.sy);
+                const int x = (int)(r - (uint32_t)y * g.sy);
+                lz = z - bz;
+                ly = y - by;
+                lx = x - bx;
+                li = (lz * RI + ly) * RI + lx;
+                taken = 0;
+                have = true;
+            }
+            const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
+            const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
+            if (edge)
+                tile_block<T, S, true>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by,
+                                       bx, lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
+            else
+                tile_block<T, S, false>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by,
+                                        bx, lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
+            taken += 4;
+            const bool inside =
+                max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
+            if (step >= sorw || !inside || taken >= cap) {
+                const uint32_t pos =
+                    (uint32_t)(bz + lz) * g.sz + (uint32_t)(by + ly) * g.sy + (uint32_t)(bx + lx);
+                st[off + j] = make_uint4(gv.base + pos, seed, wl, (uint32_t)step);
+                have = false;
             }
         }
-        uint4 *sh4 = (uint4 *)shits;
-        for (int k = threadIdx.x; k < NT / 4; k += kTileThreads) sh4[k] = make_uint4(0u, 0u, 0u, 0u);
-        for (int k = (NT / 4) * 4 + threadIdx.x; k < NT; k += kTileThreads) shits[k] = 0u;
-    }
-    if (threadIdx.x == 0) next = 0u;
-    __syncthreads();
-
-    // Interior voxels are 1..dim-2; in tile coordinates 1-b..dim-2-b.  A walker
-    // is never more than 3 steps outside the region, so a tile whose region
-    // grown by 4 stays inside the interior needs no move checks.
-    const int lmin0 = 1 - bz, lmin1 = 1 - by, lmin2 = 1 - bx;
-    const int lmax0 = g.Z - 2 - bz, lmax1 = g.Y - 2 - by, lmax2 = g.X - 2 - bx;
-    const bool edge = (lmin0 > -3) | (lmin1 > -3) | (lmin2 > -3) | (lmax0 < R + 4) |
-                      (lmax1 < R + 4) | (lmax2 < R + 4);
-
-    uint32_t *__restrict__ hg = hits + (int64_t)label * g.V;
-    uint32_t acc = 0;  // accepted moves of this thread in this launch (< 2^32: the launch is bounded)
-    bool have = false;
-    uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = weight0;
-    int step = 0, lz = 0, ly = 0, lx = 0, li = 0, taken = 0;
-    float B = 0.0f, div1 = 0.0f;
-    while (true) {
-        if (!have) {
-            j = atomicAdd(&next, 1u);
-            if (j >= cnt) break;
-            const uint4 w4 = st[off + j];
-            const uint32_t pos = w4.x;
-            seed = w4.y;
-            wl = w4.z;
-            step = (int)w4.w;
-            const bmx_seed sd = seeds[seed];
-            p0 = sd.pos;
-            B = bmx_ld(img, (int64_t)p0);
-            div1 = 1.0f / (2.0f * sd.var);
-            weight = weight0;
-            if (overrides && (sd.reserved >> 16) != 0u) weight = weight0 * (sd.reserved >> 16);
-            const int z = (int)(pos / g.sz);
-            const uint32_t r = pos - (uint32_t)z * g.sz;
-            const int y = (int)(r / g.sy);
-            const int x = (int)(r - (uint32_t)y * g.sy);
-            lz = z - bz;
-            ly = y - by;
-            lx = x - bx;
-            li = (lz * RI + ly) * RI + lx;
-            taken = 0;
-            have = true;
+        wave_add_u64(accepted, (uint64_t)acc);
+        __syncthreads();
+        for (int k = tid; k < NT; k += NTH) {
+            const uint32_t c = shits[k];
+            if (c) {
+                const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
+                const int64_t gi = (int64_t)(bz + lz2) * g.sz + (int64_t)(by + ly2) * g.sy + (bx + lx2);
+                atomicAdd(&hg[gi], c);
+            }
         }
-        const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
-        const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
-        if (edge)
-            tile_block<T, true>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by, bx,
-                                lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
-        else
-            tile_block<T, false>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by, bx,
-                                 lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
-        taken += 4;
-        const bool inside =
-            max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
-        if (step >= sorw || !inside || taken >= cap) {
-            const uint32_t pos = (uint32_t)(bz + lz) * g.sz + (uint32_t)(by + ly) * g.sy + (uint32_t)(bx + lx);
-            st[off + j] = make_uint4(pos, seed, wl, (uint32_t)step);
-            have = false;
-        }
+        if (!PERSIST) break;
+        __syncthreads();   // the LDS tile and `bucket` are reused by the next bucket
     }
-    wave_add_u64(accepted, (uint64_t)acc);
-    __syncthreads();
-    for (int k = threadIdx.x; k < NT; k += kTileThreads) {
-        const uint32_t c = shits[k];
-        if (c) {
-            const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
-            const int64_t gi = (int64_t)(bz + lz2) * g.sz + (int64_t)(by + ly2) * g.sy + (bx + lx2);
-            atomicAdd(&hg[gi], c);
+    // The span of a persistent launch, first workgroup start to last workgroup
+    // end, is added to ctl->ticks by the workgroup that ends last, which then
+    // clears the round's fields for the next launch.
+    if (PERSIST && threadIdx.x == 0) {
+        atomicMax(&ctl->t1, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+        __threadfence();
+        if (atomicAdd(&ctl->done, 1u) == gridDim.x - 1) {
+            __threadfence();
+            const unsigned long long t0 = atomicAdd(&ctl->t0, 0ull), t1 = atomicAdd(&ctl->t1, 0ull);
+            ctl->ticks += t1 > t0 ? t1 - t0 : 0ull;
+            ctl->t0 = ~0ull;
+            ctl->t1 = 0ull;
+            ctl->done = 0u;
         }
     }
 }
 
-template <typename T>
-int walk_tiled_t(bmx_ctx *ctx, const T *img, const WalkGeom &g, const bmx_seed *seeds, int64_t nseeds,
-                 int nlabels, int nbrw, int sorw, int cap, uint32_t weight, int overrides, uint32_t k0,
-                 uint32_t k1, uint32_t *hits, unsigned long long *acc, hipEvent_t *ev, int max_timed,
-                 int *launches)
+// Resident workgroups of the persistent k_tile_walk<T, S> on the context's
+// device: the grid of graph-driven rounds.
+template <typename T, int S>
+int walk_grid(bmx_ctx *ctx, int *grid)
 {
-    constexpr int C = TileCfg<T>::C;
+    const size_t lds = tile_lds_bytes<T, S>();
+    // Raise the dynamic-LDS limit where the runtime wants it; an error here is
+    // left to the launch itself to report.
+    if (hipFuncSetAttribute((const void *)k_tile_walk<T, S, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+        (void)hipGetLastError();
+    if (hipFuncSetAttribute((const void *)k_tile_walk<T, S, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+        (void)hipGetLastError();
+    int per_cu = 0, cus = 0;
+    BMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_tile_walk<T, S, true>,
+                                                         TileShape<T, S>::NTH, lds));
+    BMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    *grid = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    return BMX_OK;
+}
+
+// Everything a round launches, fixed for the whole walk.
+template <typename T>
+struct RoundArgs {
+    const T *img;
+    TileGeom tg;
+    const bmx_seed *seeds;
+    uint32_t *counts, *bkey, *boff, *bcnt;
+    RoundCtl *ctl;
+    int64_t nkeys;
+    int sorw, cap, overrides;
+    uint32_t weight, k0, k1;
+    uint32_t *hits;
+    unsigned long long *acc;
+};
+
+// Re-bucketing of one round: walkers of `in` (ctl->n of them) counted per
+// (tile, label) key, listed as buckets and scattered live into `out`; ctl->n
+// becomes the live count.  `grid` blocks run the per-walker kernels.
+template <typename T, int S>
+int round_bucket(bmx_ctx *ctx, const RoundArgs<T> &a, const uint4 *in, uint4 *out, unsigned grid)
+{
+    constexpr int C = TileShape<T, S>::C;
+    k_round_begin<<<bmx_stride_blocks(a.nkeys + 1, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys + 1, a.ctl);
+    BMX_LAUNCH_CHECK();
+    k_tile_hist<C><<<grid, 256, 0, ctx->stream>>>(in, a.ctl, a.tg, a.sorw, a.counts);
+    BMX_LAUNCH_CHECK();
+    int rc = scan_u32(ctx, a.counts, a.nkeys + 1);
+    if (rc) return rc;
+    k_bucket_list<<<bmx_blocks(a.nkeys, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys, a.bkey, a.boff, a.bcnt,
+                                                                     a.ctl);
+    BMX_LAUNCH_CHECK();
+    k_tile_scatter<C><<<grid, 256, 0, ctx->stream>>>(in, a.ctl, out, a.counts, a.tg, a.sorw);
+    BMX_LAUNCH_CHECK();
+    k_round_end<<<1, 64, 0, ctx->stream>>>(a.counts, a.nkeys, a.ctl);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+template <typename T, int S, bool PERSIST>
+int round_walk(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *st, unsigned grid)
+{
+    k_tile_walk<T, S, PERSIST><<<grid, TileShape<T, S>::NTH, tile_lds_bytes<T, S>(), ctx->stream>>>(
+        a.img, a.tg, a.seeds, st, a.bkey, a.boff, a.bcnt, a.ctl, a.sorw, a.cap, a.weight, a.overrides, a.k0,
+        a.k1, a.hits, a.acc);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+// Captures 2 * pairs rounds (state A -> B -> A ...) as one HIP graph.
+template <typename T, int S>
+int capture_rounds(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *A, uint4 *Bst, unsigned grid, unsigned wgrid,
+                   int pairs, hipGraphExec_t *exec)
+{
+    hipGraph_t graph = nullptr;
+    BMX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = BMX_OK;
+    for (int r = 0; r < 2 * pairs && rc == BMX_OK; ++r) {
+        uint4 *in = (r & 1) ? Bst : A, *out = (r & 1) ? A : Bst;
+        rc = round_bucket<T, S>(ctx, a, in, out, grid);
+        if (rc == BMX_OK) rc = round_walk<T, S, true>(ctx, a, out, wgrid);
+    }
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc == BMX_OK && e != hipSuccess)
+        rc = bmx_set_error(BMX_E_DEVICE, "bmx_walk: graph capture failed: %s", hipGetErrorString(e));
+    if (rc == BMX_OK) {
+        const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess)
+            rc = bmx_set_error(BMX_E_DEVICE, "bmx_walk: graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+}
+
+// Rounds driven from the device: 2 * pairs rounds captured once in a HIP graph
+// (no host value enters a round; a round with no live walker does no work),
+// replayed until no walker is live, the host reading one count per replay.
+// An even round count returns the state to A.  *kernel_ms receives the summed
+// spans of the walk launches, which the persistent kernel measures itself on
+// the device wall clock (events cannot bracket launches inside a graph).
+template <typename T, int S>
+int rounds_graph(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *A, uint4 *Bst, unsigned grid, unsigned wgrid,
+                 int pairs, int *rounds, float *kernel_ms)
+{
+    int khz = 0;
+    BMX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    hipGraphExec_t exec = nullptr;
+    int rc = capture_rounds<T, S>(ctx, a, A, Bst, grid, wgrid, pairs, &exec);
+    if (rc) return rc;
+    const int max_rounds = a.sorw / 4 + 2 + 2 * pairs;
+    RoundCtl host{};
+    while (rc == BMX_OK) {
+        hipError_t el = hipGraphLaunch(exec, ctx->stream);
+        if (el == hipSuccess)
+            el = hipMemcpyAsync(&host, a.ctl, sizeof(host), hipMemcpyDeviceToHost, ctx->stream);
+        if (el == hipSuccess) el = hipStreamSynchronize(ctx->stream);
+        if (el != hipSuccess) {
+            rc = bmx_set_error(BMX_E_DEVICE, "bmx_walk: graph replay failed: %s", hipGetErrorString(el));
+            break;
+        }
+        *rounds += 2 * pairs;
+        if (host.n == 0) break;
+        if (*rounds > max_rounds) rc = bmx_set_error(BMX_E_INVALID, "bmx_walk: tiled walker did not converge");
+    }
+    (void)hipGraphExecDestroy(exec);
+    *kernel_ms = khz > 0 ? (float)((double)host.ticks / (double)khz) : -2.0f;
+    return rc;
+}
+
+template <typename T, int S>
+int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, const bmx_seed *seeds,
+                 int64_t nseeds, int nlabels, int nbrw, int sorw, int cap, uint32_t weight, int overrides,
+                 uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc, int graph_pairs,
+                 hipEvent_t *ev, int max_timed, int *launches, float *kernel_ms)
+{
+    constexpr int C = TileShape<T, S>::C;
     const int64_t n0 = nseeds * (int64_t)nbrw;
     BMX_CHECK(n0 < ((int64_t)1 << 32), "bmx_walk: %lld walkers exceed 2^32", (long long)n0);
     BMX_CHECK(nbrw <= 65535, "bmx_walk: tiled walker needs nbrw <= 65535");
-    TileGeom tg;
-    tg.ntz = (uint32_t)((g.Z + C - 1) / C);
-    tg.nty = (uint32_t)((g.Y + C - 1) / C);
-    tg.ntx = (uint32_t)((g.X + C - 1) / C);
-    tg.nlabels = (uint32_t)nlabels;
-    const int64_t nkeys = (int64_t)tg.ntz * tg.nty * tg.ntx * nlabels;
+    BMX_CHECK(nvol >= 1, "bmx_walk: no volumes");
+    std::vector<VolGeom> vg((size_t)nvol);
+    int64_t ntiles = 0;
+    for (int v = 0; v < nvol; ++v) {
+        const bmx_volume &b = vols[v];
+        VolGeom &g = vg[(size_t)v];
+        g.base = (uint32_t)b.offset;
+        g.Z = b.Z;
+        g.Y = b.Y;
+        g.X = b.X;
+        g.sy = (uint32_t)b.X;
+        g.sz = (uint32_t)b.X * (uint32_t)b.Y;
+        g.V = (uint32_t)((int64_t)b.Z * b.Y * b.X);
+        g.nty = (uint32_t)((b.Y + C - 1) / C);
+        g.ntx = (uint32_t)((b.X + C - 1) / C);
+        g.tile_base = (uint32_t)ntiles;
+        g.hits_base = b.hits_offset;
+        g.seed_base = (uint32_t)b.seed_offset;
+        g.pad = 0;
+        ntiles += (int64_t)((b.Z + C - 1) / C) * g.nty * g.ntx;
+    }
+    const int64_t nkeys = ntiles * nlabels;
     BMX_CHECK(nkeys < ((int64_t)1 << 31), "bmx_walk: too many tiles");
-    int rc = grow(ctx, &ctx->d_state, &ctx->state_cap, (size_t)n0 * 2 * sizeof(uint4));
+    int rc = grow(ctx, &ctx->d_vols, &ctx->vols_cap, (size_t)nvol * sizeof(VolGeom));
+    if (rc) return rc;
+    BMX_HIP(hipMemcpyAsync(ctx->d_vols, vg.data(), (size_t)nvol * sizeof(VolGeom), hipMemcpyHostToDevice,
+                           ctx->stream));
+    rc = grow(ctx, &ctx->d_state, &ctx->state_cap, (size_t)n0 * 2 * sizeof(uint4));
     if (rc) return rc;
     rc = grow(ctx, &ctx->d_keys, &ctx->keys_cap, (size_t)(nkeys + 1) * sizeof(uint32_t));
     if (rc) return rc;
     rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap, (size_t)nkeys * 3 * sizeof(uint32_t));
     if (rc) return rc;
-    const size_t lds = tile_lds_bytes<T>();
-    // Raise the dynamic-LDS limit where the runtime wants it; an error here is
-    // left to the launch itself to report.
-    if (hipFuncSetAttribute((const void *)k_tile_walk<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
-        (void)hipGetLastError();
+    // the scan's block partials, allocated before any capture
+    rc = grow(ctx, &ctx->d_scan, &ctx->scan_cap,
+              (size_t)((nkeys + 1 + kScanTile - 1) / kScanTile) * sizeof(uint32_t));
+    if (rc) return rc;
+    int wgrid_max = 0;
+    rc = walk_grid<T, S>(ctx, &wgrid_max);
+    if (rc) return rc;
+
+    RoundArgs<T> a;
+    a.img = img;
+    a.tg = TileGeom{(const VolGeom *)ctx->d_vols, nvol, (uint32_t)nlabels};
+    a.seeds = seeds;
+    a.counts = (uint32_t *)ctx->d_keys;
+    a.bkey = (uint32_t *)ctx->d_buckets;
+    a.boff = a.bkey + nkeys;
+    a.bcnt = a.boff + nkeys;
+    a.ctl = (RoundCtl *)(ctx->d_counter + 8);
+    static_assert(sizeof(RoundCtl) <= 8 * sizeof(uint64_t), "RoundCtl fits the context's scratch counters");
+    a.nkeys = nkeys;
+    a.sorw = sorw;
+    a.cap = cap;
+    a.overrides = overrides;
+    a.weight = weight;
+    a.k0 = k0;
+    a.k1 = k1;
+    a.hits = hits;
+    a.acc = acc;
     uint4 *A = (uint4 *)ctx->d_state, *Bst = A + n0;
-    uint32_t *counts = (uint32_t *)ctx->d_keys;
-    uint32_t *bkey = (uint32_t *)ctx->d_buckets, *boff = bkey + nkeys, *bcnt = boff + nkeys;
-    uint32_t *d_nb = (uint32_t *)(ctx->d_counter + 1);
-    k_tile_init<<<bmx_blocks(n0, 256), 256, 0, ctx->stream>>>(seeds, nseeds, nbrw, sorw, overrides, A);
+    k_tile_init<<<bmx_blocks(n0, 256), 256, 0, ctx->stream>>>(seeds, nseeds, nbrw, sorw, overrides, a.tg, A);
     BMX_LAUNCH_CHECK();
-    int64_t n = n0;
+    k_set_count<<<1, 64, 0, ctx->stream>>>(a.ctl, (uint32_t)n0);
+    BMX_LAUNCH_CHECK();
+    constexpr int64_t kMaxWalkerBlocks = (int64_t)1 << 16;
+    const auto walker_grid = [](int64_t n) {
+        const int64_t b = (n + 255) / 256;
+        return (unsigned)(b < 1 ? 1 : (b < kMaxWalkerBlocks ? b : kMaxWalkerBlocks));
+    };
     int round = 0;
+    *launches = 0;
+    *kernel_ms = -1.0f;
+    if (graph_pairs > 0) {
+        rc = rounds_graph<T, S>(ctx, a, A, Bst, walker_grid(n0), (unsigned)wgrid_max, graph_pairs, &round,
+                                kernel_ms);
+        *launches = round;
+        return rc;
+    }
+
+    // Host-driven rounds: the host reads the live and bucket counts after each
+    // re-bucketing and sizes the next launches by them.
+    int64_t n = n0;
     // BMX_WALK_TRACE=1 prints one line per round (live walkers, buckets, kernel ms) to stderr.
     static const bool trace = getenv("BMX_WALK_TRACE") && getenv("BMX_WALK_TRACE")[0] == '1';
     while (n > 0) {
-        BMX_HIP(hipMemsetAsync(counts, 0, (size_t)(nkeys + 1) * sizeof(uint32_t), ctx->stream));
-        BMX_HIP(hipMemsetAsync(d_nb, 0, sizeof(uint32_t), ctx->stream));
-        k_tile_hist<C><<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(A, n, g, tg, sorw, counts);
-        BMX_LAUNCH_CHECK();
-        rc = scan_u32(ctx, counts, nkeys + 1);
+        rc = round_bucket<T, S>(ctx, a, A, Bst, walker_grid(n));
         if (rc) return rc;
-        k_bucket_list<<<bmx_blocks(nkeys, 256), 256, 0, ctx->stream>>>(counts, nkeys, bkey, boff, bcnt,
-                                                                       d_nb);
-        BMX_LAUNCH_CHECK();
-        k_tile_scatter<C><<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(A, n, Bst, counts, g, tg, sorw);
-        BMX_LAUNCH_CHECK();
-        uint32_t host[2] = {0, 0};
-        BMX_HIP(hipMemcpyAsync(&host[0], counts + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        BMX_HIP(hipMemcpyAsync(&host[1], d_nb, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        RoundCtl host{};
+        BMX_HIP(hipMemcpyAsync(&host, a.ctl, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
         BMX_HIP(hipStreamSynchronize(ctx->stream));
-        const int64_t live = host[0];
-        const uint32_t nb = host[1];
+        const int64_t live = host.n;
+        const uint32_t nb = host.nb;
         if (live == 0) break;
         if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round], ctx->stream));
-        k_tile_walk<T><<<nb, kTileThreads, lds, ctx->stream>>>(img, g, tg, seeds, Bst, bkey, boff, bcnt,
-                                                              sorw, cap, weight, overrides, k0, k1, hits,
-                                                              acc);
-        BMX_LAUNCH_CHECK();
+        rc = round_walk<T, S, false>(ctx, a, Bst, nb);
+        if (rc) return rc;
         if (round < max_timed) BMX_HIP(hipEventRecord(ev[2 * round + 1], ctx->stream));
         if (trace && round < max_timed) {
             float ms = 0.0f;
@@ -433,16 +759,36 @@ extern "C" int bmx_seed_walks(bmx_ctx *ctx, bmx_seed *seeds, int64_t n, int walk
     return BMX_OK;
 }
 
-// Entry from bmx_walk_ex (bmx_walk.hip).
-int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, int Z, int Y, int X,
+// Entry from bmx_walk_ex / bmx_walk_batch (bmx_walk.hip).  A single volume is a
+// one-entry table.
+int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volume *vols, int nvol,
                    const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int cap,
                    uint32_t weight, int overrides, uint32_t k0, uint32_t k1, uint32_t *hits,
-                   unsigned long long *acc, hipEvent_t *ev, int max_timed, int *launches)
+                   unsigned long long *acc, int shape, int graph_pairs, hipEvent_t *ev, int max_timed,
+                   int *launches, float *kernel_ms)
 {
-    WalkGeom g{Z, Y, X, (uint32_t)X, (uint32_t)X * (uint32_t)Y, (int64_t)Z * Y * X};
-    if (img_dtype == BMX_IMG_U8)
-        return walk_tiled_t<uint8_t>(ctx, (const uint8_t *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap,
-                                     weight, overrides, k0, k1, hits, acc, ev, max_timed, launches);
-    return walk_tiled_t<float>(ctx, (const float *)img, g, seeds, nseeds, nlabels, nbrw, sorw, cap, weight,
-                               overrides, k0, k1, hits, acc, ev, max_timed, launches);
+    BMX_CHECK(shape >= 0 && shape < BMX_TILE_SHAPES, "bmx_walk: tile_shape %d not in [0,%d)", shape,
+              BMX_TILE_SHAPES);
+#define BMX_TILED_CASE(TT, SS)                                                                              \
+    case SS:                                                                                                \
+        return walk_tiled_t<TT, SS>(ctx, (const TT *)img, vols, nvol, seeds, nseeds, nlabels, nbrw, sorw, cap, \
+                                    weight, overrides, k0, k1, hits, acc, graph_pairs, ev, max_timed, launches, \
+                                    kernel_ms);
+    if (img_dtype == BMX_IMG_U8) {
+        switch (shape) {
+            BMX_TILED_CASE(uint8_t, 0)
+            BMX_TILED_CASE(uint8_t, 1)
+            BMX_TILED_CASE(uint8_t, 2)
+            BMX_TILED_CASE(uint8_t, 3)
+        }
+    } else {
+        switch (shape) {
+            BMX_TILED_CASE(float, 0)
+            BMX_TILED_CASE(float, 1)
+            BMX_TILED_CASE(float, 2)
+            BMX_TILED_CASE(float, 3)
+        }
+    }
+#undef BMX_TILED_CASE
+    return bmx_set_error(BMX_E_INVALID, "bmx_walk: bad tile shape");
 }

@@ -481,6 +481,47 @@ int walk_sorted(bmx_ctx *ctx, const void *img, int img_dtype, const WalkGeom &g,
 
 }  // namespace
 
+// Reads the accepted-move count and the walk timings of the launch bracketed
+// by ctx->ev0 / ev1 (per-round events in ctx->seg_ev for `launches` rounds).
+// Round pairs per graph replay of the tiled walker: graph_rounds -1 = host-driven
+// rounds (0 pairs), 0 = the default of 8 rounds, k = k rounds rounded up to even.
+static int graph_pairs_of(const bmx_walk_opts *opts)
+{
+    const int gr = opts ? opts->graph_rounds : 0;
+    return gr < 0 ? 0 : (gr == 0 ? 4 : (gr + 1) / 2);
+}
+
+static int finish_walk(bmx_ctx *ctx, bool run, int launches, uint64_t *accepted, float graph_kernel_ms = -1.0f)
+{
+    if (accepted) {
+        uint64_t a = 0;
+        BMX_HIP(hipMemcpyAsync(&a, ctx->d_counter, sizeof(a), hipMemcpyDeviceToHost, ctx->stream));
+        BMX_HIP(hipStreamSynchronize(ctx->stream));
+        *accepted = a;
+        float ms = 0.0f;
+        BMX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_walk_ms = ms;
+        ctx->last_walk_kernel_ms = ms;
+        ctx->last_walk_launches = run ? 1 : 0;
+        if (graph_kernel_ms >= 0.0f || graph_kernel_ms == -2.0f) {
+            // graph-driven rounds: the summed walk kernels, or the whole phase
+            if (graph_kernel_ms >= 0.0f) ctx->last_walk_kernel_ms = graph_kernel_ms;
+            ctx->last_walk_launches = launches;
+        } else if (launches > 0) {
+            float sum = 0.0f;
+            for (int k = 0; k < launches; ++k) {
+                float t = 0.0f;
+                BMX_HIP(hipEventElapsedTime(&t, ctx->seg_ev[2 * k], ctx->seg_ev[2 * k + 1]));
+                sum += t;
+            }
+            ctx->last_walk_kernel_ms = sum;
+            ctx->last_walk_launches = launches;
+        }
+    }
+    return BMX_OK;
+}
+
+
 extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y,
                            int64_t X, const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw,
                            int sorw, int rng, uint64_t key, uint32_t *hits, uint64_t *accepted,
@@ -504,6 +545,10 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
     const int weight = (opts && opts->hit_weight > 0) ? opts->hit_weight : 1;
     BMX_CHECK(!opts || opts->hit_weight >= 0, "bmx_walk: negative hit_weight");
     const int overrides = (opts && opts->seed_overrides) ? 1 : 0;
+    const int shape = opts ? opts->tile_shape : 0;
+    BMX_CHECK(!opts || (opts->graph_rounds >= -1 && opts->graph_rounds <= 1024),
+              "bmx_walk: graph_rounds %d not in [-1, 1024]", opts ? opts->graph_rounds : 0);
+    const int graph_pairs = graph_pairs_of(opts);
     BMX_CHECK(algo == BMX_WALK_AUTO || algo == BMX_WALK_DIRECT || algo == BMX_WALK_SORTED ||
                   algo == BMX_WALK_TILED,
               "bmx_walk: bad algorithm %d", algo);
@@ -534,15 +579,17 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
         BMX_LAUNCH_CHECK();
     }
     int launches = 0;
+    float graph_ms = -1.0f;
     BMX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (run) {
         const uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
         if (rng == BMX_RNG_PHILOX && algo == BMX_WALK_TILED) {
             rc = ensure_events(ctx);
             if (rc) return rc;
-            rc = bmx_walk_tiled(ctx, img, img_dtype, g.Z, g.Y, g.X, seeds, nseeds, nlabels, nbrw, sorw, seg,
-                                (uint32_t)weight, overrides, k0, k1, hits, acc, ctx->seg_ev,
-                                kMaxTimedLaunches, &launches);
+            const bmx_volume vol{0, g.Z, g.Y, g.X, nlabels, 0, 0, (uint64_t)nseeds};
+            rc = bmx_walk_tiled(ctx, img, img_dtype, &vol, 1, seeds, nseeds, nlabels, nbrw, sorw, seg,
+                                (uint32_t)weight, overrides, k0, k1, hits, acc, shape, graph_pairs,
+                                ctx->seg_ev, kMaxTimedLaunches, &launches, &graph_ms);
             if (rc) return rc;
         } else if (rng == BMX_RNG_PHILOX && algo == BMX_WALK_SORTED) {
             rc = walk_sorted(ctx, img, img_dtype, g, seeds, nseeds, nbrw, sorw, k0, k1, hits, acc, seg,
@@ -582,28 +629,7 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
         }
     }
     BMX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-    if (accepted) {
-        uint64_t a = 0;
-        BMX_HIP(hipMemcpyAsync(&a, ctx->d_counter, sizeof(a), hipMemcpyDeviceToHost, ctx->stream));
-        BMX_HIP(hipStreamSynchronize(ctx->stream));
-        *accepted = a;
-        float ms = 0.0f;
-        BMX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        ctx->last_walk_ms = ms;
-        ctx->last_walk_kernel_ms = ms;
-        ctx->last_walk_launches = run ? 1 : 0;
-        if (launches > 0) {
-            float sum = 0.0f;
-            for (int k = 0; k < launches; ++k) {
-                float t = 0.0f;
-                BMX_HIP(hipEventElapsedTime(&t, ctx->seg_ev[2 * k], ctx->seg_ev[2 * k + 1]));
-                sum += t;
-            }
-            ctx->last_walk_kernel_ms = sum;
-            ctx->last_walk_launches = launches;
-        }
-    }
-    return BMX_OK;
+    return finish_walk(ctx, run, launches, accepted, graph_ms);
 }
 
 extern "C" int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,
@@ -612,6 +638,63 @@ extern "C" int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z,
 {
     return bmx_walk_ex(ctx, img, img_dtype, Z, Y, X, seeds, nseeds, nlabels, nbrw, sorw, rng, key, hits,
                        accepted, nullptr);
+}
+
+extern "C" int bmx_walk_batch(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volume *vols, int nvol,
+                              const bmx_seed *seeds, int nbrw, int sorw, uint64_t key, uint32_t *hits,
+                              uint64_t *accepted, const bmx_walk_opts *opts)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(img && hits && vols, "bmx_walk_batch: null pointer");
+    BMX_CHECK(img_dtype == BMX_IMG_U8 || img_dtype == BMX_IMG_F32, "bmx_walk_batch: bad image dtype %d",
+              img_dtype);
+    BMX_CHECK(nvol >= 1 && nvol <= (1 << 20), "bmx_walk_batch: nvol %d not in [1, 2^20]", nvol);
+    BMX_CHECK(nbrw >= 0 && nbrw <= 65535 && sorw >= 0, "bmx_walk_batch: nbrw / sorw out of range");
+    const int algo = opts ? opts->algorithm : BMX_WALK_AUTO;
+    BMX_CHECK(algo == BMX_WALK_AUTO || algo == BMX_WALK_TILED, "bmx_walk_batch: algorithm must be tiled");
+    BMX_CHECK(!opts || opts->hit_weight >= 0, "bmx_walk_batch: negative hit_weight");
+    const int seg = (opts && opts->segment_steps > 0) ? opts->segment_steps : 256;
+    BMX_CHECK(seg % 4 == 0, "bmx_walk_batch: segment_steps %d must be a multiple of 4", seg);
+    const int weight = (opts && opts->hit_weight > 0) ? opts->hit_weight : 1;
+    const int overrides = (opts && opts->seed_overrides) ? 1 : 0;
+    const int shape = opts ? opts->tile_shape : 0;
+    BMX_CHECK(!opts || (opts->graph_rounds >= -1 && opts->graph_rounds <= 1024),
+              "bmx_walk_batch: graph_rounds %d not in [-1, 1024]", opts ? opts->graph_rounds : 0);
+    const int graph_pairs = graph_pairs_of(opts);
+    uint64_t img_end = 0, seed_end = 0;
+    int nlabels = 1;
+    for (int v = 0; v < nvol; ++v) {
+        const bmx_volume &b = vols[v];
+        BMX_CHECK(b.Z >= 3 && b.Y >= 3 && b.X >= 3, "bmx_walk_batch: volume %d: bad dims %d x %d x %d", v, b.Z,
+                  b.Y, b.X);
+        BMX_CHECK(b.nlabels >= 1 && b.nlabels <= 255, "bmx_walk_batch: volume %d: nlabels %d not in [1,255]", v,
+                  b.nlabels);
+        BMX_CHECK(b.offset >= img_end, "bmx_walk_batch: volume %d overlaps or precedes volume %d", v, v - 1);
+        BMX_CHECK(b.seed_offset == seed_end, "bmx_walk_batch: volume %d: seeds must follow volume %d's", v, v - 1);
+        img_end = b.offset + (uint64_t)b.Z * (uint64_t)b.Y * (uint64_t)b.X;
+        seed_end = b.seed_offset + b.nseeds;
+        BMX_CHECK(img_end < ((uint64_t)1 << 32), "bmx_walk_batch: batch image exceeds 2^32 elements");
+        if (b.nlabels > nlabels) nlabels = b.nlabels;
+    }
+    const int64_t nseeds = (int64_t)seed_end;
+    BMX_CHECK(nseeds == 0 || seeds, "bmx_walk_batch: null seeds");
+    BMX_HIP(hipMemsetAsync(ctx->d_counter, 0, sizeof(uint64_t), ctx->stream));
+    const bool run = nseeds > 0 && nbrw > 0 && sorw > 0;
+    int launches = 0;
+    float graph_ms = -1.0f;
+    BMX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (run) {
+        rc = ensure_events(ctx);
+        if (rc) return rc;
+        rc = bmx_walk_tiled(ctx, img, img_dtype, vols, nvol, seeds, nseeds, nlabels, nbrw, sorw, seg,
+                            (uint32_t)weight, overrides, (uint32_t)key, (uint32_t)(key >> 32), hits,
+                            (unsigned long long *)ctx->d_counter, shape, graph_pairs, ctx->seg_ev,
+                            kMaxTimedLaunches, &launches, &graph_ms);
+        if (rc) return rc;
+    }
+    BMX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    return finish_walk(ctx, run, launches, accepted, graph_ms);
 }
 
 extern "C" int bmx_last_walk_stats(bmx_ctx *ctx, float *phase_ms, float *kernel_ms, int *launches)

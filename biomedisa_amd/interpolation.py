@@ -26,6 +26,9 @@ from .helpers import (Biomedisa, Dice_score, _error_, adjacent_planes, label_ind
                       normalize_image, pre_processing, predict_blocksize)
 
 DEFAULT_KEY = 0x5EED0B10D1CE0001
+# The guided walk's sparser, ring-weighted seed set runs best with 128 steps
+# per walker per round (the plain walk with the library's 256).
+GUIDED_WALK_OPTS = {"segment_steps": 128}
 _RNG = {"philox": _hip.RNG_PHILOX, "reference": _hip.RNG_REF}
 
 
@@ -56,8 +59,12 @@ class PipelineError(ValueError):
 
 
 def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                 uncertainty=False, rng="philox", key=DEFAULT_KEY, keep_hits=False, timings=None):
+                 uncertainty=False, rng="philox", key=DEFAULT_KEY, keep_hits=False, timings=None,
+                 walk_opts=None):
     """Seeds, walk, merge (and optional smoothing / uncertainty) on device-resident inputs.
+
+    walk_opts: keyword options of Engine.walk (algorithm, segment_steps,
+    tile_shape, graph_rounds); they change the speed of the walk, not its hits.
 
     Returns a dict of device buffers: 'regular' (uint8 V), optional 'smooth',
     'uncertainty', 'hits' (uint32 [nlabels][V]), plus host ints 'nseeds',
@@ -79,7 +86,7 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
     hits = eng.alloc(nlabels * V * 4)
     hits.zero()
     accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, seeds, nseeds, nlabels, nbrw, sorw,
-                        _RNG[rng], key, hits)
+                        _RNG[rng], key, hits, **(walk_opts or {}))
     t2 = time.perf_counter()
     out = {"nseeds": nseeds, "accepted": accepted, "flags": flags}
     regular = eng.alloc(V)
@@ -107,7 +114,7 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=12, far=48, far_walks=2, cap=0,
+                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=8, far=48, far_walks=2, walk_opts=None,
                         timings=None, keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
 
@@ -119,9 +126,8 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     to `far` walk their first `far_walks` walks with every hit counted
     nbrw/far_walks times: an unbiased estimate of what they add to U
     (far=0 disables the ring).  Near and ring seeds walk in one call (per-seed
-    walk counts and weights), so ring walkers fill the same tile buckets.  cap:
-    steps per walker per round of the tile walker (0 = library default).  The
-    result takes the walker's argmax on U and
+    walk counts and weights), so ring walkers fill the same tile buckets.
+    walk_opts: as for run_pipeline.  The result takes the walker's argmax on U and
     the network's label elsewhere ('smooth': the smoothed walker argmax on U).
     Returns the dict of run_pipeline plus 'walked_seeds' and 'ring_seeds'.
     """
@@ -141,12 +147,8 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
         raise PipelineError("At least one empty slice between labels required.")
     seeds, nseeds = eng.seeds(vol.lab, vol.img, vol.img_dtype, vol.dims, flags_buf)
     t1 = time.perf_counter()
-    grid = (S.pad8(Z), S.pad8(Y), S.pad8(X))
-    x = net._get("input", int(np.prod(grid)) * S.CIN * 2)
-    _hip.call("bmx_net_input", eng.ctx, vol.img.ptr, vol.img_dtype, vol.lab.ptr, flags_buf.ptr, Z, Y, X,
-              *grid, x.ptr)
     pred, unc = eng.alloc(V), eng.alloc(V)
-    net.predict(x, vol.dims, grid, nlabels, thr, pred, unc)
+    net.predict_volume(vol, flags, nlabels, thr, pred, unc)
     U, need, tmp = eng.alloc(V), eng.alloc(V), eng.alloc(V)
     _hip.call("bmx_guided_masks", eng.ctx, pred.ptr, unc.ptr, Z, Y, X, int(brad), int(srad), U.ptr, need.ptr,
               tmp.ptr)
@@ -164,8 +166,9 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     t2 = time.perf_counter()
     hits = eng.alloc(nlabels * V * 4)
     hits.zero()
+    wopts = dict(GUIDED_WALK_OPTS, **(walk_opts or {}))
     accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, sub, nsub + nring, nlabels, nbrw, sorw,
-                        _hip.RNG_PHILOX, key, hits, segment_steps=cap, seed_overrides=nring > 0)
+                        _hip.RNG_PHILOX, key, hits, seed_overrides=nring > 0, **wopts)
     phase_ms, kernel_ms, launches = eng.last_walk_stats()
     t3 = time.perf_counter()
     out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "accepted": accepted, "flags": flags}
@@ -218,6 +221,39 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     if method == "guided" and (rng != "philox" or uncertainty or return_hits):
         raise ValueError("guided mode runs the Philox walker and returns 'regular'/'smooth' only")
 
+    prep = _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
+                    return_hits, slicer)
+    if prep is None:
+        return None
+    bm = prep.bm
+    eng = _hip.engine(device)
+    vol = DeviceVolume(eng, prep.img, prep.labidx)
+    try:
+        if method == "guided":
+            from . import surrogate
+            dev = run_pipeline_guided(vol, bm.nol, prep.values, surrogate.net(device), bm.nbrw, bm.sorw,
+                                      bm.allaxis, bm.smooth, key, **(guide or {}))
+        else:
+            dev = run_pipeline(vol, bm.nol, prep.values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
+                               bm.uncertainty, rng, key, keep_hits=bm.return_hits)
+    except PipelineError as e:
+        _error_(bm, str(e))
+        return None
+    return _results(prep, dev)
+
+
+class _Prepared:
+    """A validated input on the host: the padded, cropped image and label-index
+    volume the pipeline runs on, and what is needed to place its results."""
+
+    def __init__(self, bm, img, lab, labidx, values, box):
+        self.bm, self.img, self.lab_c, self.labidx, self.values, self.box = bm, img, lab, labidx, values, box
+
+
+def _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
+             return_hits, slicer):
+    """Checks and pre-processing of smart_interpolation (interpolation.py:77-204);
+    None (after printing the error) when the input is rejected."""
     bm = Biomedisa()
     bm.TIC = time.time()
     bm.success = True
@@ -240,7 +276,8 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     img = np.pad(bm.data, 1, mode="constant", constant_values=0)
     lab = np.pad(bm.labelData, 1, mode="constant", constant_values=0)
     bm.zsh, bm.ysh, bm.xsh = img.shape
-    z0, z1, y0, y1, x0, x1 = predict_blocksize(lab)
+    box = predict_blocksize(lab)
+    z0, z1, y0, y1, x0, x1 = box
     img_c = np.ascontiguousarray(img[z0:z1, y0:y1, x0:x1])
     lab_c = np.ascontiguousarray(lab[z0:z1, y0:y1, x0:x1])
     labidx = label_index_volume(lab_c, bm.allLabels)
@@ -248,22 +285,15 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     if bm.nol > 255:
         _error_(bm, "At most 255 labels (including background) are supported.")
         return None
+    return _Prepared(bm, img_c, lab_c, labidx, values, box)
 
-    eng = _hip.engine(device)
-    vol = DeviceVolume(eng, img_c, labidx)
-    try:
-        if method == "guided":
-            from . import surrogate
-            dev = run_pipeline_guided(vol, bm.nol, values, surrogate.net(device), bm.nbrw, bm.sorw,
-                                      bm.allaxis, bm.smooth, key, **(guide or {}))
-        else:
-            dev = run_pipeline(vol, bm.nol, values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
-                               bm.uncertainty, rng, key, keep_hits=bm.return_hits)
-    except PipelineError as e:
-        _error_(bm, str(e))
-        return None
 
-    shape_c = vol.dims
+def _results(prep, dev):
+    """The result dictionary of smart_interpolation from the pipeline's device
+    buffers (rw_small.py:195-281)."""
+    bm = prep.bm
+    z0, z1, y0, y1, x0, x1 = prep.box
+    shape_c = prep.img.shape
     results = {}
 
     def place(buf):
@@ -284,8 +314,8 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         results["hits"] = hits[:, 1:-1, 1:-1, 1:-1]
 
     # validate result (rw_small.py:203-211)
-    mask = lab_c > 0
-    dice = Dice_score(lab_c, final_crop * mask)
+    mask = prep.lab_c > 0
+    dice = Dice_score(prep.lab_c, final_crop * mask)
     if dice < 0.3:
         print('Warning: Bad result! Use "--allaxis" if you labeled axes other than the xy-plane.')
 

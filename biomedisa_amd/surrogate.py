@@ -4,12 +4,12 @@ Architecture (4 levels, 16/32/64/128 channels, one 3x3x3 conv + ReLU per
 level and path, 2x average pooling down, 2x nearest upsampling up, skip
 concatenations, 1x1x1 softmax head):
 
-    x(16) -enc0-> a0(16) -pool-> -enc1-> a1(32) -pool-> -enc2-> a2(64) -pool-> -bott-> a3(128)
+    x(32) -enc0-> a0(16) -pool-> -enc1-> a1(32) -pool-> -enc2-> a2(64) -pool-> -bott-> a3(128)
     [a2, up a3](192) -dec2-> d2(64);  [a1, up d2](96) -dec1-> d1(32);
-    [a0, up d1](48) -dec0-> d0(16) -head-> softmax over L = 14 label indices
+    [a0, up d1](48) -dec0-> d0(16) -head-> softmax over L = 30 label indices
 
-Input channels (bmx_net_input): image/127.5 - 1, seed indicator, one-hot of
-the label index on seed voxels.  The network is trained with cross-entropy
+Input channels (bmx_net_input, 32): image/127.5 - 1, seed indicator, one-hot
+of the label index (< 30) on seed voxels.  The network is trained with cross-entropy
 against the plain walker's per-voxel label (argmax of its hits) on seeded
 synthetic volumes, so its top probability estimates how likely the plain
 walker is to agree with its proposal.  BF16 weights and activations, FP32 accumulation, FP32
@@ -24,9 +24,9 @@ import numpy as np
 
 from . import _hip
 
-L_MAX = 14        # label indices the input encoding and the head support
-CIN = 16          # padded input channels
-LAYERS = [("enc0", 16, 16), ("enc1", 16, 32), ("enc2", 32, 64), ("bott", 64, 128),
+L_MAX = 30        # label indices the input encoding and the head support
+CIN = 32          # input channels: image, seed indicator, one-hot of 30 label indices
+LAYERS = [("enc0", 32, 16), ("enc1", 16, 32), ("enc2", 32, 64), ("bott", 64, 128),
           ("dec2", 192, 64), ("dec1", 96, 32), ("dec0", 48, 16)]
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_weights.npz")
 
@@ -154,7 +154,7 @@ class Net:
         _hip.call("bmx_conv3d_fwd_cat", self.eng.ctx, a.ptr, *shape, ca, b.ptr, cb, w.bf.ptr, bias.w.ptr, y.ptr, co, 1)
 
     def forward(self, x, N, D, H, W, grads=False):
-        """x: bf16 [N,D,H,W,16] (D,H,W multiples of 8) -> d0 buffer (bf16 [.., 16])."""
+        """x: bf16 [N,D,H,W,CIN] (D,H,W multiples of 8) -> d0 buffer (bf16 [.., 16])."""
         assert D % 8 == 0 and H % 8 == 0 and W % 8 == 0
         a = self._acts(N, D, H, W, grads)
         ctx = self.eng.ctx
@@ -238,15 +238,16 @@ class Net:
         for p in self.params():
             p.g.zero()
 
-    def loss_and_grad(self, x, target, vmask, N, D, H, W, loss_buf):
-        """Forward, softmax cross-entropy against soft targets, backward.  Returns nothing;
-        the summed loss accumulates into loss_buf (float)."""
+    def loss_and_grad(self, x, target, vmask, N, D, H, W, loss_buf, target_idx=None):
+        """Forward, softmax cross-entropy against soft targets (float [.., L_MAX]) or
+        label indices (uint8 target_idx), backward.  The summed loss accumulates
+        into loss_buf (float)."""
         d0 = self.forward(x, N, D, H, W, grads=True)
         a = self._last[2]
         nvox = N * D * H * W
         _hip.call("bmx_head", self.eng.ctx, d0.ptr, nvox, L_MAX, self.hW.w.ptr, self.hb.w.ptr, None,
-                  target.ptr, vmask.ptr, _f(1.0 / nvox), a["g_d0"].ptr, self.hW.g.ptr, self.hb.g.ptr,
-                  loss_buf.ptr)
+                  None if target is None else target.ptr, None if target_idx is None else target_idx.ptr,
+                  vmask.ptr, _f(1.0 / nvox), a["g_d0"].ptr, self.hW.g.ptr, self.hb.g.ptr, loss_buf.ptr)
         self.backward()
 
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
@@ -273,6 +274,43 @@ class Net:
         _hip.call("bmx_head_predict", self.eng.ctx, d0.ptr, *dims, Zp, Yp, Xp, int(nlabels),
                   W.ptr, b.ptr, _f(thr), pred.ptr, unc.ptr)
 
+    # Slab-wise inference: a slab of the volume plus HALO planes on each side
+    # (more than the network's 29-voxel receptive field) gives exact predictions
+    # for its core planes; volumes whose padded grid exceeds MAX_GRID voxels
+    # are predicted slab by slab along z.
+    HALO = 32
+    MAX_GRID = 300_000_000
+
+    def predict_volume(self, vol, flags, nlabels, thr, pred, unc, max_grid=None):
+        """Labels and uncertainty flags of a DeviceVolume (flags: host uint8 plane
+        flags [Z+Y+X]) into the device buffers pred / unc (uint8, V each)."""
+        Z, Y, X = vol.dims
+        Yp, Xp = pad8(Y), pad8(X)
+        max_grid = self.MAX_GRID if max_grid is None else max_grid
+        eng = self.eng
+        if pad8(Z) * Yp * Xp <= max_grid:
+            core = Z
+        else:
+            core = (max_grid // (Yp * Xp) - 2 * self.HALO) // 8 * 8
+            if core < 8:
+                raise ValueError("volume too wide for slab-wise inference (%d x %d)" % (Y, X))
+        item = 1 if vol.img_dtype == _hip.IMG_U8 else 4
+        flags = np.ascontiguousarray(flags, np.uint8)
+        W, b = self._head_for_inference()
+        for z0 in range(0, Z, core):
+            zs, ze = max(0, z0 - self.HALO), min(Z, z0 + core + self.HALO)
+            nz = ze - zs
+            grid = (pad8(nz), Yp, Xp)
+            x = self._get("input", int(np.prod(grid)) * CIN * 2)
+            fs = eng.upload(np.concatenate([flags[zs:ze], flags[Z:Z + Y], flags[Z + Y:]]))
+            _hip.call("bmx_net_input", eng.ctx, vol.img.ptr + zs * Y * X * item, vol.img_dtype,
+                      vol.lab.ptr + zs * Y * X, fs.ptr, nz, Y, X, *grid, x.ptr)
+            d0 = self.forward(x, 1, *grid)
+            nc = min(core, Z - z0)
+            off = (z0 - zs) * Yp * Xp * 16 * 2   # first core plane of the slab's decoder output
+            _hip.call("bmx_head_predict", eng.ctx, d0.ptr + off, nc, Y, X, grid[0], Yp, Xp, int(nlabels),
+                      W.ptr, b.ptr, _f(thr), pred.ptr + z0 * Y * X, unc.ptr + z0 * Y * X)
+
     def log_probs(self, x, grid, nlabels):
         """log softmax over the first nlabels logits (temperature 1) on the padded grid,
         float32 [Zp,Yp,Xp,nlabels] on the host."""
@@ -281,7 +319,7 @@ class Net:
         n = Zp * Yp * Xp
         prob = self._get("prob", n * nlabels * 4)
         _hip.call("bmx_head", self.eng.ctx, d0.ptr, n, int(nlabels), self.hW.w.ptr, self.hb.w.ptr, prob.ptr,
-                  None, None, _f(1.0), None, None, None, None)
+                  None, None, None, _f(1.0), None, None, None, None)
         p = prob.download((Zp, Yp, Xp, nlabels), np.float32)
         return np.log(np.maximum(p, 1e-38))
 
@@ -318,7 +356,7 @@ def training_volume(seed, size=64):
     bias and label spacing drawn from the seed)."""
     from .synth import structures_volume
     rng = np.random.default_rng(seed)
-    n_struct = int(rng.integers(2, 9))
+    n_struct = int(rng.integers(2, 9)) if rng.random() < 0.7 else int(rng.integers(9, 17))
     every = int(rng.integers(6, 13))
     allaxis = bool(rng.random() < 0.7)
     return structures_volume(shape=(size, size, size), seed=int(seed) + 1000003, n_struct=n_struct,
@@ -328,13 +366,15 @@ def training_volume(seed, size=64):
 
 
 class Sample:
-    """Device-resident training sample: input x, soft target, mask on a padded grid."""
+    """Device-resident training sample on a padded grid: input x, the walker's label
+    index per voxel (tidx, hard targets) or its hit fractions (target, soft
+    targets), and the in-volume mask."""
 
-    def __init__(self, x, target, vmask, grid):
-        self.x, self.target, self.vmask, self.grid = x, target, vmask, grid
+    def __init__(self, x, target, tidx, vmask, grid):
+        self.x, self.target, self.tidx, self.vmask, self.grid = x, target, tidx, vmask, grid
 
 
-def make_sample(eng, vol, nbrw=10, sorw=4000, key=1, hard=False):
+def make_sample(eng, vol, nbrw=10, sorw=4000, key=1, hard=True):
     """Walk a synthetic volume with the plain walker and build its training sample.
 
     hard=True trains towards the one-hot of the walker's argmax, so the network's
@@ -359,16 +399,21 @@ def make_sample(eng, vol, nbrw=10, sorw=4000, key=1, hard=False):
     x = eng.alloc(ng * CIN * 2)
     flags = eng.upload(out["flags"])
     _hip.call("bmx_net_input", eng.ctx, dv.img.ptr, dv.img_dtype, dv.lab.ptr, flags.ptr, Z, Y, X, *grid, x.ptr)
-    target = eng.alloc(ng * L_MAX * 4)
     vmask = eng.alloc(ng)
-    _hip.call("bmx_hits_target", eng.ctx, out["hits"].ptr, bm.nol, Z, Y, X, *grid, L_MAX, int(hard), target.ptr,
-              vmask.ptr)
+    target = tidx = None
+    if hard:
+        tidx = eng.alloc(ng)
+        _hip.call("bmx_hits_index", eng.ctx, out["hits"].ptr, bm.nol, Z, Y, X, *grid, tidx.ptr, vmask.ptr)
+    else:
+        target = eng.alloc(ng * L_MAX * 4)
+        _hip.call("bmx_hits_target", eng.ctx, out["hits"].ptr, bm.nol, Z, Y, X, *grid, L_MAX, 0, target.ptr,
+                  vmask.ptr)
     eng.sync()
-    return Sample(x, target, vmask, grid)
+    return Sample(x, target, tidx, vmask, grid)
 
 
 def train(n_volumes=2000, size=64, iters=1500, batch=4, lr=2e-3, seed=0, out=WEIGHTS_PATH, log=print,
-          device=0, hard=False):
+          device=0, hard=True):
     """Train the network on the GPU with the project's own kernels; save FP32 weights."""
     eng = _hip.engine(device)
     t0 = time.perf_counter()
@@ -385,7 +430,8 @@ def train(n_volumes=2000, size=64, iters=1500, batch=4, lr=2e-3, seed=0, out=WEI
     ng = int(np.prod(grid))
     model = Net(eng, seed=seed)
     xb = eng.alloc(batch * ng * CIN * 2)
-    tb = eng.alloc(batch * ng * L_MAX * 4)
+    tb = None if hard else eng.alloc(batch * ng * L_MAX * 4)
+    ib = eng.alloc(batch * ng) if hard else None
     mb = eng.alloc(batch * ng)
     lossb = eng.alloc(4)
     rng = np.random.default_rng(seed)
@@ -396,11 +442,14 @@ def train(n_volumes=2000, size=64, iters=1500, batch=4, lr=2e-3, seed=0, out=WEI
         for k, i in enumerate(idx):
             smp = samples[i]
             _copy(eng, xb, k * ng * CIN * 2, smp.x, ng * CIN * 2)
-            _copy(eng, tb, k * ng * L_MAX * 4, smp.target, ng * L_MAX * 4)
+            if hard:
+                _copy(eng, ib, k * ng, smp.tidx, ng)
+            else:
+                _copy(eng, tb, k * ng * L_MAX * 4, smp.target, ng * L_MAX * 4)
             _copy(eng, mb, k * ng, smp.vmask, ng)
         model.zero_grad()
         lossb.zero()
-        model.loss_and_grad(xb, tb, mb, batch, *grid, lossb)
+        model.loss_and_grad(xb, tb, mb, batch, *grid, lossb, target_idx=ib)
         lr_t = lr * 0.5 * (1 + np.cos(np.pi * it / iters))
         model.adam(lr_t)
         if it % 50 == 0 or it == iters - 1:
@@ -522,7 +571,7 @@ if __name__ == "__main__":
     ap.add_argument("--size", type=int, default=64)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=WEIGHTS_PATH)
-    ap.add_argument("--target", choices=["soft", "hard"], default="soft")
+    ap.add_argument("--target", choices=["soft", "hard"], default="hard")
     ap.add_argument("--weights", default=WEIGHTS_PATH, help="calibrate: weights to calibrate")
     ap.add_argument("--recall", type=float, default=0.95)
     a = ap.parse_args()

@@ -136,3 +136,43 @@ def structures_volume(shape=(512, 512, 512), seed=2, n_struct=8, radius=(5, 60),
     img = _finish(img, shape, rng, bias, sigma, max_value, dtype)
     return Volume(img, gt, sparse_labels(gt, every, allaxis), allaxis)
 
+
+
+def large_volume(seed=5, size=1024, every=16):
+    """Large-volume workload: size^3 uint16 with 12-bit values, 16 overlapping
+    ellipsoids and tubes (radius 10-120 voxels at 1024^3, scaled with size), means
+    spread over 480..3680, noise sigma 40, bias field +-200, labels on every
+    `every`-th plane of all three axes."""
+    f = size / 1024.0
+    return structures_volume(shape=(size, size, size), seed=seed, n_struct=16,
+                             radius=(max(3, int(round(10 * f))), max(6, int(round(120 * f)))),
+                             mean_range=(480, 3680), sigma=40.0, bias=200.0, every=every, allaxis=True,
+                             dtype=np.uint16, max_value=4095)
+
+
+def _batch_member(args):
+    shape, seed, n_struct, sigma, every = args
+    return structures_volume(shape=shape, seed=seed, n_struct=n_struct, radius=(5, max(8, min(shape) // 4)),
+                             mean_range=(30, 230), sigma=sigma, bias=15.0, every=every, allaxis=False)
+
+
+def batch_volumes(seed=3, n=64, size_lo=96, size_hi=(256, 320, 320), workers=1):
+    """Batched-interpolation workload: n uint8 volumes, shape per axis uniform in
+    [size_lo, size_hi] (z up to 256, y and x up to 320), each with 2-12
+    structures (ellipsoids and tubes, radius 5 .. a quarter of the smallest
+    side), noise sigma uniform in [8, 20] and labels on every k-th XY plane,
+    k uniform in [6, 12].  workers > 1 generates them in that many processes
+    (same volumes)."""
+    rng = _rng(seed)
+    jobs = []
+    for i in range(n):
+        shape = tuple(int(rng.integers(size_lo, hi + 1)) for hi in size_hi)
+        n_struct = int(rng.integers(2, 13))
+        sigma = float(rng.uniform(8, 20))
+        every = int(rng.integers(6, 13))
+        jobs.append((shape, int(seed) * 1000 + i, n_struct, sigma, every))
+    if workers <= 1:
+        return [_batch_member(j) for j in jobs]
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_batch_member, jobs)

@@ -126,13 +126,48 @@ typedef struct bmx_walk_opts {
     int seed_overrides; /* tiled: 1 = each seed's `reserved` gives its walk count (low 16
                            bits, 0 = nbrw, at most nbrw) and a hit weight (high 16 bits,
                            0 = 1) multiplying hit_weight */
+    int tile_shape;     /* tiled: tile shape index in [0, BMX_TILE_SHAPES) (core side, halo,
+                           workgroup size; 0 = the default shape) */
+    int graph_rounds;   /* tiled: rounds per replay of one captured HIP graph whose walk
+                           kernel is persistent (resident workgroups take buckets from a
+                           device counter), replayed until every walk has ended; k > 0 =
+                           k rounds rounded up to even, 0 = 8; -1 = rounds driven by the
+                           host, one walk workgroup launched per bucket */
 } bmx_walk_opts;
+
+/* Tile shapes compiled into the tiled walker. */
+#define BMX_TILE_SHAPES 4
 
 /* bmx_walk with a choice of algorithm (opts may be NULL = automatic).  No
  * counterpart in the reference, which has one kernel per backend. */
 int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,
                 const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, int rng,
                 uint64_t key, uint32_t *hits, uint64_t *accepted, const bmx_walk_opts *opts);
+
+/* One volume of a batched walk.  Its image occupies elements [offset, offset +
+ * Z*Y*X) of the batch image (z-major, x fastest); its seeds are
+ * seeds[seed_offset, seed_offset + nseeds) with volume-local positions and
+ * labels below nlabels, as bmx_seeds writes them; its hits accumulate into the
+ * uint32 block [nlabels][Z*Y*X] at element hits_offset of the batch hits. */
+typedef struct bmx_volume {
+    uint64_t offset;
+    int32_t Z, Y, X;
+    int32_t nlabels;
+    uint64_t hits_offset;
+    uint64_t seed_offset;
+    uint64_t nseeds;
+} bmx_volume;
+
+/* Random walks of many volumes of uneven size and label count in one tiled
+ * walk: per volume the hits equal those of bmx_walk_ex with the same key and
+ * the Philox stream (bit-exact).  Volumes are given in ascending, disjoint
+ * offset and seed order, the batch image below 2^32 elements.  opts as for
+ * bmx_walk_ex (algorithm must be AUTO or TILED).  Replaces the per-volume
+ * calls a caller of smart_interpolation (biomedisa/interpolation.py:77-203)
+ * would make; no counterpart in the reference. */
+int bmx_walk_batch(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volume *vols, int nvol,
+                   const bmx_seed *seeds, int nbrw, int sorw, uint64_t key, uint32_t *hits,
+                   uint64_t *accepted, const bmx_walk_opts *opts);
 
 /* Milliseconds of the last bmx_walk (whole walk phase: record packing
  * excluded, sorting included), measured with HIP events on the context
@@ -196,12 +231,13 @@ int bmx_concat(bmx_ctx *ctx, const uint16_t *a, int Ca, const uint16_t *b, int C
 int bmx_split(bmx_ctx *ctx, const uint16_t *y, int64_t nvox, uint16_t *a, int Ca, uint16_t *b,
               int Cb, int accumulate_a);
 int bmx_relu_mask(bmx_ctx *ctx, uint16_t *g, const uint16_t *a, int64_t n);
-/* 1x1x1 softmax head over 16 channels, L <= 16 outputs.  With target (soft
- * labels [v][L]) also dx, dW, db (accumulated) and the summed cross-entropy
- * (accumulated into *loss); vmask (may be NULL) excludes voxels. */
+/* 1x1x1 softmax head over 16 channels, L <= 32 outputs.  With target (soft
+ * labels [v][L]) or target_idx (one label index per voxel) also dx, dW, db
+ * (accumulated) and the summed cross-entropy (accumulated into *loss); vmask
+ * (may be NULL) excludes voxels. */
 int bmx_head(bmx_ctx *ctx, const uint16_t *x, int64_t nvox, int L, const float *W, const float *b,
-             float *prob, const float *target, const uint8_t *vmask, float scale, uint16_t *dx,
-             float *dW, float *db, float *loss);
+             float *prob, const float *target, const uint8_t *target_idx, const uint8_t *vmask, float scale,
+             uint16_t *dx, float *dW, float *db, float *loss);
 int bmx_adam(bmx_ctx *ctx, float *p, const float *g, float *m, float *v, uint16_t *pbf, int64_t n,
              float lr, float beta1, float beta2, float eps, int step, float weight_decay);
 int bmx_f32_to_bf16(bmx_ctx *ctx, const float *a, uint16_t *b, int64_t n);
@@ -212,12 +248,16 @@ int bmx_add_bf16(bmx_ctx *ctx, uint16_t *y, const uint16_t *x, int64_t n);
  *           inside (Z,Y,X) and some walker arrived.
  * hard = 1: t[v] = one-hot of the walker's argmax (first maximum; index 0 when all
  *           counts are 0, as bmx_argmax); vmask = 1 inside (Z,Y,X). */
+/* Training labels on a padded grid: idx = first argmax index of the hits (0 when
+ * all counts are 0), vmask = 1 inside (Z,Y,X).  No counterpart in the reference. */
+int bmx_hits_index(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int Y, int X, int Zp, int Yp, int Xp,
+                   uint8_t *idx, uint8_t *vmask);
 int bmx_hits_target(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int Z, int Y, int X, int Zp,
                     int Yp, int Xp, int L, int hard, float *target, uint8_t *vmask);
 
 /* Surrogate-guided walking (no counterpart in the reference). */
-/* Network input on a padded grid (Zp,Yp,Xp) >= (Z,Y,X), 16 bf16 channels:
- * image/127.5-1, seed indicator, one-hot label index (< 14) on seed voxels. */
+/* Network input on a padded grid (Zp,Yp,Xp) >= (Z,Y,X), 32 bf16 channels:
+ * image/127.5-1, seed indicator, one-hot label index (< 30) on seed voxels. */
 int bmx_net_input(bmx_ctx *ctx, const void *img, int img_dtype, const uint8_t *lab,
                   const uint8_t *flags, int Z, int Y, int X, int Zp, int Yp, int Xp, uint16_t *out);
 /* Head on the padded grid -> per voxel of (Z,Y,X): argmax label index and

@@ -19,8 +19,9 @@ from . import orc
 from .net import to_bf16_bits
 
 LABEL_NONE = 0xFF
-N_CH = 16
-L_ONEHOT = 14
+N_CH = 32          # network input channels
+L_ONEHOT = 30      # label indices with a one-hot input channel
+HEAD_CH = 16       # channels of the decoder output read by the head
 
 
 def net_input(img, lab, flags, grid):
@@ -52,7 +53,7 @@ def head_logits(xbits, W, b):
     lg = np.empty(x.shape[:-1] + (L,), np.float32)
     for l in range(L):
         s = np.full(x.shape[:-1], b[l], np.float32)
-        for c in range(N_CH):
+        for c in range(HEAD_CH):
             s = s + W[l, c] * x[..., c]
         lg[..., l] = s
     return lg
@@ -154,6 +155,17 @@ def hits_target(hits, grid, L, hard):
         t[:Z, :Y, :X, l] = np.where(ok, h[l].astype(np.float32) / safe, np.float32(0))
     m[:Z, :Y, :X] = ok
     return t, m
+
+
+def hits_index(hits, grid):
+    """Training labels [Zp,Yp,Xp] u8 (first argmax index, 0 for all-zero) and mask."""
+    h = np.asarray(hits, np.uint32)
+    nl, Z, Y, X = h.shape
+    idx = np.zeros(grid, np.uint8)
+    m = np.zeros(grid, np.uint8)
+    idx[:Z, :Y, :X] = orc.argmax(np.ascontiguousarray(h), np.arange(nl, dtype=np.uint8))
+    m[:Z, :Y, :X] = 1
+    return idx, m
 
 
 def select_seeds(seeds, mask):

@@ -95,12 +95,12 @@ def head(x, W, b, target=None, vmask=None, scale=1.0):
 # with optional bf16 rounding of activations and activation gradients at the
 # points where the GPU stores them in bf16.
 # ---------------------------------------------------------------------------
-UNET_LAYERS = [("enc0", 16, 16), ("enc1", 16, 32), ("enc2", 32, 64), ("bott", 64, 128),
+UNET_LAYERS = [("enc0", 32, 16), ("enc1", 16, 32), ("enc2", 32, 64), ("bott", 64, 128),
                ("dec2", 192, 64), ("dec1", 96, 32), ("dec0", 48, 16)]
 
 
 def unet_loss_and_grads(params, x, target, vmask, round_bf16=True):
-    """params: {name.w [Co,27,Ci], name.b, head.W [L,16], head.b}; x [N,D,H,W,16];
+    """params: {name.w [Co,27,Ci], name.b, head.W [L,16], head.b}; x [N,D,H,W,Ci of enc0];
     target [N,D,H,W,L]; vmask [N,D,H,W].  Returns (summed loss, grads dict)."""
     r = bf16_round if round_bf16 else (lambda a: a)
     P = {k: (bf16_round(v) if (round_bf16 and k.endswith(".w")) else v.astype(np.float64))

@@ -42,3 +42,33 @@ def test_last_error_starts_empty_and_version():
     lib = _hip.load()
     assert lib.bmx_version() >= 1
     assert isinstance(lib.bmx_last_error(), bytes)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """bmx_seed, bmx_walk_opts and bmx_volume as the bindings lay them out: sizes
+    and field offsets from a C program compiled against include/bmx.h."""
+    import ctypes
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "bmx.h"\n'
+        '#define F(T, m) printf(#T "." #m " %zu\\n", offsetof(T, m))\n'
+        'int main(void) {\n'
+        ' printf("seed %zu opts %zu volume %zu\\n", sizeof(bmx_seed), sizeof(bmx_walk_opts), sizeof(bmx_volume));\n'
+        ' F(bmx_seed, var); F(bmx_seed, reserved);\n'
+        ' F(bmx_walk_opts, seed_overrides); F(bmx_walk_opts, tile_shape); F(bmx_walk_opts, graph_rounds);\n'
+        ' F(bmx_volume, Z); F(bmx_volume, nlabels); F(bmx_volume, hits_offset); F(bmx_volume, seed_offset);\n'
+        ' F(bmx_volume, nseeds);\n'
+        ' return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    sizes = dict(zip(out[0].split()[0::2], map(int, out[0].split()[1::2])))
+    assert sizes == {"seed": _hip.SEED_DTYPE.itemsize, "opts": ctypes.sizeof(_hip.WalkOpts),
+                     "volume": _hip.VOLUME_DTYPE.itemsize}
+    offs = dict(line.split() for line in out[1:] if line)
+    assert int(offs["bmx_seed.var"]) == _hip.SEED_DTYPE.fields["var"][1]
+    assert int(offs["bmx_seed.reserved"]) == _hip.SEED_DTYPE.fields["reserved"][1]
+    for f in ("seed_overrides", "tile_shape", "graph_rounds"):
+        assert int(offs["bmx_walk_opts." + f]) == getattr(_hip.WalkOpts, f).offset
+    for f in ("Z", "nlabels", "hits_offset", "seed_offset", "nseeds"):
+        assert int(offs["bmx_volume." + f]) == _hip.VOLUME_DTYPE.fields[f][1]

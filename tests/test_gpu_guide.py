@@ -29,17 +29,17 @@ def test_net_input_bitexact(eng, dtype):
     img = rng.integers(0, 256, (Z, Y, X)).astype(dtype)
     if dtype == np.float32:
         img += rng.random((Z, Y, X)).astype(np.float32)
-    lab = rng.integers(0, 17, (Z, Y, X)).astype(np.uint8)
+    lab = rng.integers(0, 34, (Z, Y, X)).astype(np.uint8)
     lab[rng.random((Z, Y, X)) < 0.1] = G.LABEL_NONE
     flags = (rng.random(Z + Y + X) < 0.25).astype(np.uint8)
     ib, lb, fb = eng.upload(img), eng.upload(lab), eng.upload(flags)
-    out = eng.alloc(int(np.prod(grid)) * 16 * 2)
+    out = eng.alloc(int(np.prod(grid)) * G.N_CH * 2)
     _hip.call("bmx_net_input", eng.ctx, ib.ptr, _hip.IMG_U8 if dtype == np.uint8 else _hip.IMG_F32, lb.ptr,
               fb.ptr, Z, Y, X, *grid, out.ptr)
-    np.testing.assert_array_equal(out.download(grid + (16,), np.uint16), G.net_input(img, lab, flags, grid))
+    np.testing.assert_array_equal(out.download(grid + (G.N_CH,), np.uint16), G.net_input(img, lab, flags, grid))
 
 
-@pytest.mark.parametrize("L,thr", [(4, 0.9), (9, 0.5), (14, 0.95)])
+@pytest.mark.parametrize("L,thr", [(4, 0.9), (9, 0.5), (14, 0.95), (30, 0.9)])
 def test_head_predict_bitexact(eng, L, thr):
     rng = np.random.default_rng(L)
     dims, grid = (9, 10, 13), (16, 16, 16)
@@ -94,6 +94,20 @@ def test_merge_guided_select_labels_bitexact(eng):
     ab = eng.upload(a)
     _hip.call("bmx_select_labels", eng.ctx, ab.ptr, pb.ptr, Ub.ptr, nl, values.ctypes.data, V, out.ptr)
     np.testing.assert_array_equal(out.download(dims, np.uint8), G.select_labels(a, pred, U, values))
+
+
+def test_hits_index_bitexact(eng):
+    rng = np.random.default_rng(19)
+    nl, dims, grid = 6, (6, 10, 7), (8, 16, 8)
+    hits = rng.integers(0, 3, (nl,) + dims).astype(np.uint32)
+    hits[:, 0, :3] = 0
+    hb = eng.upload(hits)
+    ng = int(np.prod(grid))
+    ib, mb = eng.alloc(ng), eng.alloc(ng)
+    _hip.call("bmx_hits_index", eng.ctx, hb.ptr, nl, *dims, *grid, ib.ptr, mb.ptr)
+    i_ref, m_ref = G.hits_index(hits, grid)
+    np.testing.assert_array_equal(ib.download(grid, np.uint8), i_ref)
+    np.testing.assert_array_equal(mb.download(grid, np.uint8), m_ref)
 
 
 @pytest.mark.parametrize("hard", [0, 1])
@@ -214,3 +228,23 @@ def test_trained_network_gain(eng):
         print(name, "walk cost %.3f of plain, min per-label Dice %.4f" % res[name])
     assert res["trained"][0] < 0.4 and res["trained"][1] >= 0.99
     assert res["untrained"][1] < 0.99 or res["untrained"][0] > 0.9
+
+
+def test_slab_inference_equals_whole_volume(eng):
+    """Slab-wise prediction (z-slabs with a 32-plane halo, 8-aligned) gives the
+    whole-volume prediction bit for bit: the receptive field is 29 voxels."""
+    from biomedisa_amd import surrogate as S
+    from biomedisa_amd.interpolation import DeviceVolume
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(70, 40, 48), seed=15, n_struct=4, radius=(4, 12), every=6, allaxis=True)
+    img, labidx, values, nol = _prep(vol)
+    dv = DeviceVolume(eng, img, labidx)
+    flags = eng.plane_flags(dv.lab, dv.dims, True).download((sum(dv.dims),), np.uint8)
+    net = S.Net(eng, seed=4)
+    out = []
+    for max_grid in (None, (8 + 2 * S.Net.HALO) * S.pad8(dv.dims[1]) * S.pad8(dv.dims[2])):
+        pred, unc = eng.alloc(dv.V), eng.alloc(dv.V)
+        net.predict_volume(dv, flags, nol, 0.9, pred, unc, max_grid=max_grid)
+        out.append((pred.download(dv.dims, np.uint8), unc.download(dv.dims, np.uint8)))
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    np.testing.assert_array_equal(out[1][1], out[0][1])

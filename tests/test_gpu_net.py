@@ -116,23 +116,29 @@ def test_pool_up_concat_split(eng):
     np.testing.assert_array_equal(down_bf16(b2, shape + (32,)), y)
 
 
-def test_head_softmax_ce(eng):
-    rng = np.random.default_rng(10)
-    nv, L = 300, 9
+@pytest.mark.parametrize("L,index_targets", [(9, False), (30, False), (30, True)])
+def test_head_softmax_ce(eng, L, index_targets):
+    rng = np.random.default_rng(10 + L)
+    nv = 300
     x = O.bf16_round(rng.standard_normal((nv, 16)))
     W = (rng.standard_normal((L, 16)) * 0.3).astype(np.float32)
     b = rng.standard_normal(L).astype(np.float32)
-    t = rng.random((nv, L)); t /= t.sum(-1, keepdims=True)
+    if index_targets:
+        ti = rng.integers(0, L, nv).astype(np.uint8)
+        t = np.eye(L)[ti]
+    else:
+        t = rng.random((nv, L)); t /= t.sum(-1, keepdims=True)
     vm = (rng.random(nv) > 0.2).astype(np.uint8)
     xb, Wb, bb = up_bf16(eng, x), eng.upload(W), eng.upload(b)
-    tb, mb = eng.upload(t.astype(np.float32)), eng.upload(vm)
+    tb, mb = (None, eng.upload(vm)) if index_targets else (eng.upload(t.astype(np.float32)), eng.upload(vm))
+    tib = eng.upload(ti) if index_targets else None
     pb, dxb = eng.alloc(nv * L * 4), eng.alloc(nv * 16 * 2)
     dWb, dbb, lb = eng.alloc(L * 16 * 4), eng.alloc(L * 4), eng.alloc(4)
     for bf in (dWb, dbb, lb):
         bf.zero()
     scale = 1.0 / nv
-    _hip.call("bmx_head", eng.ctx, xb.ptr, nv, L, Wb.ptr, bb.ptr, pb.ptr, tb.ptr, mb.ptr, C.c_float(scale),
-              dxb.ptr, dWb.ptr, dbb.ptr, lb.ptr)
+    _hip.call("bmx_head", eng.ctx, xb.ptr, nv, L, Wb.ptr, bb.ptr, pb.ptr, None if tb is None else tb.ptr,
+              None if tib is None else tib.ptr, mb.ptr, C.c_float(scale), dxb.ptr, dWb.ptr, dbb.ptr, lb.ptr)
     p, loss, dx, dW, db = O.head(x, W.astype(np.float64), b.astype(np.float64),
                                  t.astype(np.float32).astype(np.float64), vm, scale)
     np.testing.assert_allclose(pb.download((nv, L), np.float32), p, rtol=1e-4, atol=1e-6)
@@ -170,7 +176,7 @@ def test_whole_network_gradients_match_oracle(eng):
     N, D, H, W = 1, 8, 8, 16
     rng = np.random.default_rng(12)
     x = O.bf16_round(np.concatenate([rng.standard_normal((N, D, H, W, 2)),
-                                     (rng.random((N, D, H, W, 14)) < 0.1)], -1))
+                                     (rng.random((N, D, H, W, S.CIN - 2)) < 0.1)], -1))
     t = rng.random((N, D, H, W, S.L_MAX)); t /= t.sum(-1, keepdims=True); t = t.astype(np.float32)
     vm = (rng.random((N, D, H, W)) > 0.1).astype(np.uint8)
     xb, tb, mb, lb = up_bf16(eng, x), eng.upload(t), eng.upload(vm), eng.alloc(4)

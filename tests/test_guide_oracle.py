@@ -87,12 +87,13 @@ def test_net_input_channels():
     rng = np.random.default_rng(3)
     Z, Y, X = 5, 6, 7
     img = rng.integers(0, 256, (Z, Y, X)).astype(np.uint8)
-    lab = rng.integers(0, 16, (Z, Y, X)).astype(np.uint8)
+    lab = rng.integers(0, 34, (Z, Y, X)).astype(np.uint8)
     lab[0, 0, :3] = G.LABEL_NONE
     flags = np.zeros(Z + Y + X, np.uint8)
     flags[0] = 1          # z = 0 plane
     flags[Z + 3] = 1      # y = 3 plane
     out = G.net_input(img, lab, flags, (8, 8, 8))
+    assert out.shape == (8, 8, 8, G.N_CH)
     x = from_bf16_bits(out).astype(np.float64)
     np.testing.assert_allclose(x[:Z, :Y, :X, 0], img / 127.5 - 1, atol=2 ** -8)
     assert not x[Z:].any() and not x[:, Y:].any() and not x[:, :, X:].any()
@@ -101,8 +102,8 @@ def test_net_input_channels():
             for xx in range(X):
                 seed = (z == 0 or y == 3) and lab[z, y, xx] != G.LABEL_NONE
                 assert x[z, y, xx, 1] == (1.0 if seed else 0.0)
-                hot = np.zeros(14)
-                if seed and lab[z, y, xx] < 14:
+                hot = np.zeros(G.L_ONEHOT)
+                if seed and lab[z, y, xx] < G.L_ONEHOT:
                     hot[lab[z, y, xx]] = 1
                 np.testing.assert_array_equal(x[z, y, xx, 2:], hot)
 

@@ -80,3 +80,18 @@ def test_normalize_image():
     u16 = np.array([10, 20, 30], np.uint16)
     f = H.normalize_image(u16)
     assert f.dtype == np.float32 and f.tolist() == [0.0, 127.5, 255.0]
+
+
+def test_batch_volumes_workload_shape():
+    """The batched workload: seeded, shapes within [96, (256, 320, 320)], 2-12
+    structures, XY-plane labels only."""
+    from biomedisa_amd.synth import batch_volumes
+    a = batch_volumes(seed=3, n=3)
+    b = batch_volumes(seed=3, n=3)
+    for va, vb in zip(a, b):
+        np.testing.assert_array_equal(va.img, vb.img)
+        assert va.img.dtype == np.uint8 and not va.allaxis
+        assert all(96 <= s <= hi for s, hi in zip(va.img.shape, (256, 320, 320)))
+        assert 2 <= int(va.gt.max()) <= 12
+        zs = np.nonzero(va.labels.reshape(va.labels.shape[0], -1).any(axis=1))[0]
+        assert len(zs) >= 2 and any(np.all(zs % k == 0) for k in range(6, 13))

@@ -93,7 +93,7 @@ def test_whole_network_gradient_finite_differences():
         params[n + ".b"] = rng.standard_normal(co) * 0.1
     params["head.W"] = rng.standard_normal((14, 16)) * 0.25
     params["head.b"] = np.zeros(14)
-    x = rng.standard_normal((1, 8, 8, 8, 16))
+    x = rng.standard_normal((1, 8, 8, 8, O.UNET_LAYERS[0][1]))
     t = rng.random((1, 8, 8, 8, 14)); t /= t.sum(-1, keepdims=True)
     vm = np.ones((1, 8, 8, 8))
     loss, G = O.unet_loss_and_grads(params, x, t, vm, round_bf16=False)
