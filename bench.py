@@ -242,7 +242,9 @@ def main():
                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
                         "traffic_source": "profiles/k_tile_walk_512_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                           "scaled by a copy-kernel calibration)",
-                        "kernel": "k_tile_walk<uint8_t>", "walker_steps": steps_total,
+                        "kernel": "k_tile_walk<uint8_t, 0, true> (persistent; rounds replayed in a HIP graph, "
+                                  "kernel time = summed launch spans on the device wall clock)",
+                        "walker_steps": steps_total,
                         "accepted_moves": rk["accepted"], "launches": launches,
                         "launch_avg_ms": tm["walk_kernel_ms"] / launches,
                         "algorithmic_bytes_per_launch": walk_bytes / launches}
