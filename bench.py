@@ -23,6 +23,12 @@ every 6th-12th XY plane), nbrw=10, sorw=4000, no smoothing, all walked in one
 bmx_walk_batch call with autotuned tile shape, round cap and round driver.
 `value` is batch voxels/s; the config carries the same volumes interpolated
 one call at a time on the same GPU and checks that both give identical labels.
+
+--workload large measures the large volume: 1024^3 uint16 (12-bit), 16
+structures, noise sigma 40, bias +-200, labels on every 16th plane of all axes,
+nbrw=10, sorw=4000, no smoothing; plain and surrogate-guided on the same GPU
+with every uint32 hit volume resident in HBM.  `value` is the guided voxels/s;
+the config carries the plain time and the per-label Dice of guided vs plain.
 """
 import argparse
 import json
@@ -134,7 +140,7 @@ def main():
     ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["512", "batch"], default="512")
+    ap.add_argument("--workload", choices=["512", "batch", "large"], default="512")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch bmx_f32_to_bf16 over 64 Mi floats (reads 256 MiB, writes "
@@ -146,6 +152,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "large":
+        return bench_large(args)
 
     from biomedisa_amd import _hip
     from biomedisa_amd.helpers import per_label_dice
@@ -340,6 +348,103 @@ def bench_batch(args):
                      "walk_kernel_ms": tm["walk_kernel_ms"], "walk_phase_ms": tm["walk_phase_ms"]},
     }
     hits.free()
+    print(json.dumps(line), flush=True)
+
+
+def dice_by_confusion(a, b, nbins=256):
+    """Per-label Dice of two uint8 label volumes from one confusion matrix,
+    accumulated over z-slabs ({label >= 1 present in either: dice})."""
+    cm = np.zeros((nbins, nbins), dtype=np.int64)
+    for z in range(0, a.shape[0], 64):
+        idx = a[z:z + 64].ravel().astype(np.int32) * nbins + b[z:z + 64].ravel()
+        cm += np.bincount(idx, minlength=nbins * nbins).reshape(nbins, nbins)
+    out = {}
+    for l in range(1, nbins):
+        s = int(cm[l, :].sum() + cm[:, l].sum())
+        if s:
+            out[l] = 2.0 * int(cm[l, l]) / s
+    return out
+
+
+def bench_large(args):
+    """1024^3 uint16: plain vs surrogate-guided, one step each."""
+    import gc
+    from biomedisa_amd import _hip, surrogate
+    from biomedisa_amd import helpers as H
+    from biomedisa_amd.interpolation import DeviceVolume, run_pipeline, run_pipeline_guided
+    from biomedisa_amd.synth import large_volume
+    t = time.perf_counter()
+    vol = large_volume(seed=5, size=1024, log=log)
+    log("volume generated in %.0fs" % (time.perf_counter() - t))
+    bm = H.Biomedisa()
+    bm.data, bm.labelData, bm.only, bm.ignore, bm.success = vol.img, vol.labels, "all", "none", True
+    bm = H.pre_processing(bm)
+    nol = bm.nol
+    img = np.pad(H.normalize_image(bm.data), 1)
+    lab = np.pad(bm.labelData, 1)
+    z0, z1, y0, y1, x0, x1 = H.predict_blocksize(lab)
+    img = np.ascontiguousarray(img[z0:z1, y0:y1, x0:x1])
+    labidx = H.label_index_volume(np.ascontiguousarray(lab[z0:z1, y0:y1, x0:x1]), bm.allLabels)
+    values = bm.allLabels.astype(np.uint8)
+    gt = vol.gt
+    del vol, bm, lab
+    gc.collect()
+    eng = _hip.engine(0)
+    dv = DeviceVolume(eng, img, labidx)
+    del img, labidx
+    gc.collect()
+    n_vox = 1024 ** 3
+    log("uploaded %s float32 image, %d labels" % (dv.dims, nol))
+    res = {}
+    for method in ("plain", "guided"):
+        tm = {}
+        t = time.perf_counter()
+        if method == "plain":
+            out = run_pipeline(dv, nol, values, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0, timings=tm)
+        else:
+            net = surrogate.net(0)
+            out = run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0,
+                                      thr=args.thr, brad=args.brad, srad=args.srad, far=args.far,
+                                      far_walks=args.far_walks, timings=tm)
+        eng.sync()
+        sec = time.perf_counter() - t
+        log("%s: %.1fs %s" % (method, sec, {k: round(v, 3) for k, v in tm.items()}))
+        res[method] = dict(sec=sec, tm=tm, nseeds=out["nseeds"], walked=out.get("walked_seeds"),
+                           ring=out.get("ring_seeds"), accepted=out["accepted"],
+                           regular=out["regular"].download(dv.dims, np.uint8))
+        out["regular"].free()
+        del out
+        gc.collect()
+    p, g = res["plain"], res["guided"]
+    d = dice_by_confusion(g["regular"], p["regular"])
+    inner = (slice(1, -1),) * 3
+    d_gt = dice_by_confusion(np.ascontiguousarray(p["regular"][inner]), gt)
+    log("Dice computed")
+    line = {
+        "metric": "segmented voxels/s at 1024^3 uint16, 16 structures; surrogate-guided vs plain",
+        "value": n_vox / g["sec"],
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": 1000.0 * g["sec"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "1024^3 uint16 (12-bit, normalised to float32), 16 ellipsoids/tubes, noise 40, "
+                               "bias 200, labels every 16th plane allaxis, nbrw=%d sorw=%d smooth=0"
+                               % (args.nbrw, args.sorw),
+                   "labels": nol, "seeds": p["nseeds"], "plain_seconds": p["sec"], "guided_seconds": g["sec"],
+                   "speedup_vs_plain": p["sec"] / g["sec"], "walked_seed_fraction": g["walked"] / g["nseeds"],
+                   "ring_seed_fraction": g["ring"] / g["nseeds"],
+                   "min_dice_guided_vs_plain": min(d.values()),
+                   "dice_guided_vs_plain": {str(k): round(v, 5) for k, v in d.items()},
+                   "min_dice_vs_ground_truth_plain": min(d_gt.values()),
+                   "plain_timings": p["tm"], "guided_timings": g["tm"],
+                   "hit_volumes_resident_GB": nol * int(np.prod(dv.dims)) * 4 / 1e9},
+    }
     print(json.dumps(line), flush=True)
 
 

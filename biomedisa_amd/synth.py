@@ -115,8 +115,9 @@ def ellipsoids_volume(shape=(128, 128, 128), seed=1, means=(60, 120, 190), backg
 
 def structures_volume(shape=(512, 512, 512), seed=2, n_struct=8, radius=(5, 60), mean_range=(30, 230),
                       background=None, sigma=12.0, bias=15.0, every=10, allaxis=True, dtype=np.uint8,
-                      max_value=255):
-    """Overlapping ellipsoids and tubes (half each), distinct means spread over mean_range."""
+                      max_value=255, log=None):
+    """Overlapping ellipsoids and tubes (half each), distinct means spread over mean_range.
+    log: optional callable, told of each structure drawn (large volumes take minutes)."""
     rng = _rng(seed)
     lo, hi = mean_range
     means = np.linspace(lo, hi, n_struct + 1)
@@ -133,12 +134,16 @@ def structures_volume(shape=(512, 512, 512), seed=2, n_struct=8, radius=(5, 60),
             sl, inside = _tube(shape, rng, r_lo, min(r_hi, 0.25 * min(shape)))
         img[sl][inside] = struct_means[k]
         gt[sl][inside] = k + 1
+        if log:
+            log("structure %d/%d drawn" % (k + 1, n_struct))
     img = _finish(img, shape, rng, bias, sigma, max_value, dtype)
+    if log:
+        log("bias field and noise added")
     return Volume(img, gt, sparse_labels(gt, every, allaxis), allaxis)
 
 
 
-def large_volume(seed=5, size=1024, every=16):
+def large_volume(seed=5, size=1024, every=16, log=None):
     """Large-volume workload: size^3 uint16 with 12-bit values, 16 overlapping
     ellipsoids and tubes (radius 10-120 voxels at 1024^3, scaled with size), means
     spread over 480..3680, noise sigma 40, bias field +-200, labels on every
@@ -147,7 +152,7 @@ def large_volume(seed=5, size=1024, every=16):
     return structures_volume(shape=(size, size, size), seed=seed, n_struct=16,
                              radius=(max(3, int(round(10 * f))), max(6, int(round(120 * f)))),
                              mean_range=(480, 3680), sigma=40.0, bias=200.0, every=every, allaxis=True,
-                             dtype=np.uint16, max_value=4095)
+                             dtype=np.uint16, max_value=4095, log=log)
 
 
 def _batch_member(args):

@@ -2,12 +2,16 @@
 plane of all axes; plain and surrogate-guided on the same GPU with all uint32 hit
 volumes resident in HBM, compared by per-label Dice and time."""
 import gc
+import os
 import time
 
 import numpy as np
 import pytest
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(os.environ.get("BMX_LARGE") != "1",
+                                 reason="the 1024^3 workload takes about five minutes; run with BMX_LARGE=1 "
+                                        "(bench.py --workload large measures it)")]
 
 
 def test_large_uint16_guided_vs_plain(eng):
@@ -16,7 +20,7 @@ def test_large_uint16_guided_vs_plain(eng):
     from biomedisa_amd.interpolation import DeviceVolume, run_pipeline, run_pipeline_guided
     from biomedisa_amd.synth import large_volume
     t0 = time.perf_counter()
-    vol = large_volume(seed=5, size=1024)
+    vol = large_volume(seed=5, size=1024, log=lambda m: print(m, flush=True))
     bm = H.Biomedisa()
     bm.data, bm.labelData, bm.only, bm.ignore, bm.success = vol.img, vol.labels, "all", "none", True
     bm = H.pre_processing(bm)
