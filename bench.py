@@ -396,11 +396,15 @@ def bench_large(args):
     n_vox = 1024 ** 3
     log("uploaded %s float32 image, %d labels" % (dv.dims, nol))
     res = {}
-    for method in ("plain", "guided"):
+    for method in ("plain", "guided", "plain_large_path"):
         tm = {}
         t = time.perf_counter()
         if method == "plain":
             out = run_pipeline(dv, nol, values, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0, timings=tm)
+        elif method == "plain_large_path":
+            # the reference's own driver at this size (adaptive walks)
+            out = run_pipeline(dv, nol, values, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0, timings=tm,
+                               path="large")
         else:
             net = surrogate.net(0)
             out = run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0,
@@ -415,8 +419,9 @@ def bench_large(args):
         out["regular"].free()
         del out
         gc.collect()
-    p, g = res["plain"], res["guided"]
+    p, g, pl = res["plain"], res["guided"], res["plain_large_path"]
     d = dice_by_confusion(g["regular"], p["regular"])
+    d_large = dice_by_confusion(pl["regular"], p["regular"])
     inner = (slice(1, -1),) * 3
     d_gt = dice_by_confusion(np.ascontiguousarray(p["regular"][inner]), gt)
     log("Dice computed")
@@ -443,6 +448,9 @@ def bench_large(args):
                    "dice_guided_vs_plain": {str(k): round(v, 5) for k, v in d.items()},
                    "min_dice_vs_ground_truth_plain": min(d_gt.values()),
                    "plain_timings": p["tm"], "guided_timings": g["tm"],
+                   "large_path": {"seconds": pl["sec"], "walked_seed_fraction": pl["walked"] / pl["nseeds"],
+                                  "min_dice_vs_plain": min(d_large.values()),
+                                  "speedup_vs_plain": p["sec"] / pl["sec"], "timings": pl["tm"]},
                    "hit_volumes_resident_GB": nol * int(np.prod(dv.dims)) * 4 / 1e9},
     }
     print(json.dumps(line), flush=True)

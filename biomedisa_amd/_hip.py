@@ -48,6 +48,9 @@ SIGNATURES = {
     "bmx_copy": (_i, [_p, _p, _p, C.c_size_t]),
     "bmx_plane_flags": (_i, [_p, _p, _i64, _i64, _i64, _i, _p]),
     "bmx_seeds": (_i, [_p, _p, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
+    "bmx_plane_flags_ex": (_i, [_p, _p, _i64, _i64, _i64, _i, _i, _p]),
+    "bmx_adaptive_mask": (_i, [_p, _p, _i64, _i64, _i64, _p, _p, _p]),
+    "bmx_smooth_argmax_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p, _i]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),
@@ -194,11 +197,25 @@ class Engine:
         _check(load().bmx_upload(self.ctx, buf.ptr, arr.ctypes.data, arr.nbytes))
 
     # path -----------------------------------------------------------------
-    def plane_flags(self, lab, dims, allaxis, out=None):
+    def plane_flags(self, lab, dims, allaxis, out=None, large=False):
         Z, Y, X = dims
         flags = out if out is not None else DeviceBuffer(self, Z + Y + X)
-        _check(load().bmx_plane_flags(self.ctx, lab.ptr, Z, Y, X, int(bool(allaxis)), flags.ptr))
+        _check(load().bmx_plane_flags_ex(self.ctx, lab.ptr, Z, Y, X, int(bool(allaxis)), int(bool(large)),
+                                         flags.ptr))
         return flags
+
+    def adaptive_mask(self, lab, dims, flags, out=None):
+        """uint8 mask of the voxels whose seeds walk in the large-volume path."""
+        Z, Y, X = dims
+        V = Z * Y * X
+        mask = out if out is not None else DeviceBuffer(self, V)
+        tmp = DeviceBuffer(self, 4 * V)
+        try:
+            _check(load().bmx_adaptive_mask(self.ctx, lab.ptr, Z, Y, X, flags.ptr, tmp.ptr, mask.ptr))
+        finally:
+            self.sync()
+            tmp.free()
+        return mask
 
     def count_seeds(self, lab, dims, flags):
         Z, Y, X = dims
@@ -274,11 +291,11 @@ class Engine:
         Z, Y, X = dims
         _check(load().bmx_smooth(self.ctx, phi.ptr, tmp.ptr, Z, Y, X, int(iters)))
 
-    def smooth_argmax(self, hits, nlabels, dims, iters, values, out):
+    def smooth_argmax(self, hits, nlabels, dims, iters, values, out, clamp_first=False):
         Z, Y, X = dims
         values = np.ascontiguousarray(values, dtype=np.uint8)
-        _check(load().bmx_smooth_argmax(self.ctx, hits.ptr, int(nlabels), Z, Y, X, int(iters),
-                                        values.ctypes.data, out.ptr))
+        _check(load().bmx_smooth_argmax_ex(self.ctx, hits.ptr, int(nlabels), Z, Y, X, int(iters),
+                                           values.ctypes.data, out.ptr, int(bool(clamp_first))))
 
     def uncertainty(self, hits, nlabels, dims, out):
         Z, Y, X = dims

@@ -10,6 +10,14 @@
 //   seed test:                pyopencl_small.py:205-207 (interior voxel whose
 //                             mask value is a label)
 //   variance:                 _calc_var, pyopencl_small.py:140-185
+// Large-volume variants (bmx_plane_flags_ex large=1, bmx_adaptive_mask):
+//   planes (XY):              biomedisa_helper.py:1068-1080 (a labelled plane
+//                             hides the next 4 planes)
+//   planes (allaxis):         biomedisa_helper.py:1082-1103 (as above, but only
+//                             positive labels count)
+//   adaptive walks:           pyopencl_large.py:570-590 (a seed walks if it lies
+//                             on the sub-lattice (x+z, y+z even) or another label
+//                             is seen on the stride-5 grid within +-100 voxels)
 #include "bmx_internal.h"
 
 namespace {
@@ -18,25 +26,35 @@ constexpr int kSeedThreads = 256;
 constexpr int kSeedPerThread = 16;
 constexpr int kSeedChunk = kSeedThreads * kSeedPerThread;
 
-__device__ __forceinline__ bool nz(const uint8_t *lab, int64_t i) { return lab[i] != 0; }
+// Voxel counts for plane detection: any non-zero label index (negative labels,
+// BMX_LABEL_NONE, included), or with POS only positive labels (index 1..254).
+template <bool POS>
+__device__ __forceinline__ bool nz(const uint8_t *lab, int64_t i)
+{
+    const uint8_t v = lab[i];
+    return POS ? (v != 0 && v != BMX_LABEL_NONE) : v != 0;
+}
 
 // Voxel survives the first thin-line pass along an axis of extent na, stride sa.
+template <bool POS>
 __device__ __forceinline__ bool pass1(const uint8_t *lab, int64_t i, int a, int na, int64_t sa)
 {
-    if (!nz(lab, i)) return false;
-    if (a > 0 && a < na - 1 && !nz(lab, i - sa) && !nz(lab, i + sa)) return false;
+    if (!nz<POS>(lab, i)) return false;
+    if (a > 0 && a < na - 1 && !nz<POS>(lab, i - sa) && !nz<POS>(lab, i + sa)) return false;
     return true;
 }
 
+template <bool POS>
 __device__ __forceinline__ bool pass2(const uint8_t *lab, int64_t i, int a, int na, int64_t sa,
                                       int b, int nb, int64_t sb)
 {
-    if (!pass1(lab, i, a, na, sa)) return false;
-    if (b > 0 && b < nb - 1 && !pass1(lab, i - sb, a, na, sa) && !pass1(lab, i + sb, a, na, sa))
+    if (!pass1<POS>(lab, i, a, na, sa)) return false;
+    if (b > 0 && b < nb - 1 && !pass1<POS>(lab, i - sb, a, na, sa) && !pass1<POS>(lab, i + sb, a, na, sa))
         return false;
     return true;
 }
 
+template <bool POS>
 __global__ __launch_bounds__(256) void k_plane_flags(const uint8_t *__restrict__ lab, int Z, int Y,
                                                      int X, int allaxis, uint8_t *fz, uint8_t *fy,
                                                      uint8_t *fx)
@@ -44,7 +62,7 @@ __global__ __launch_bounds__(256) void k_plane_flags(const uint8_t *__restrict__
     const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
-        if (!nz(lab, i)) continue;
+        if (!nz<false>(lab, i)) continue;
         const int z = (int)(i / sz);
         const int64_t r = i - z * sz;
         const int y = (int)(r / sy);
@@ -53,9 +71,25 @@ __global__ __launch_bounds__(256) void k_plane_flags(const uint8_t *__restrict__
             if (!fz[z]) fz[z] = 1;
             continue;
         }
-        if (!fz[z] && pass2(lab, i, y, Y, sy, x, X, 1)) fz[z] = 1;
-        if (!fy[y] && pass2(lab, i, z, Z, sz, x, X, 1)) fy[y] = 1;
-        if (!fx[x] && pass2(lab, i, z, Z, sz, y, Y, sy)) fx[x] = 1;
+        if (!fz[z] && pass2<POS>(lab, i, y, Y, sy, x, X, 1)) fz[z] = 1;
+        if (!fy[y] && pass2<POS>(lab, i, z, Z, sz, x, X, 1)) fy[y] = 1;
+        if (!fx[x] && pass2<POS>(lab, i, z, Z, sz, y, Y, sy)) fx[x] = 1;
+    }
+}
+
+// XY planes of the large-volume path: scanning z upwards, a labelled plane is
+// kept and the 4 planes after it are skipped (biomedisa_helper.py:1068-1080).
+__global__ void k_skip_planes(uint8_t *fz, int Z)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int z = 0;
+    while (z < Z) {
+        if (fz[z]) {
+            for (int k = z + 1; k < z + 5 && k < Z; ++k) fz[k] = 0;
+            z += 5;
+        } else {
+            ++z;
+        }
     }
 }
 
@@ -250,8 +284,8 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_write(const uint8_t *__re
 
 extern "C" {
 
-int bmx_plane_flags(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, int allaxis,
-                    uint8_t *flags)
+int bmx_plane_flags_ex(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, int allaxis,
+                       int large, uint8_t *flags)
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
@@ -263,10 +297,24 @@ int bmx_plane_flags(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int6
     const int64_t V = Z * Y * X;
     int64_t blocks = (V + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    k_plane_flags<<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, (int)Z, (int)Y, (int)X, allaxis,
-                                                            flags, flags + Z, flags + Z + Y);
+    if (large)
+        k_plane_flags<true><<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, (int)Z, (int)Y, (int)X, allaxis,
+                                                                      flags, flags + Z, flags + Z + Y);
+    else
+        k_plane_flags<false><<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, (int)Z, (int)Y, (int)X, allaxis,
+                                                                       flags, flags + Z, flags + Z + Y);
     BMX_LAUNCH_CHECK();
+    if (large && !allaxis) {
+        k_skip_planes<<<1, 64, 0, ctx->stream>>>(flags, (int)Z);
+        BMX_LAUNCH_CHECK();
+    }
     return BMX_OK;
+}
+
+int bmx_plane_flags(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, int allaxis,
+                    uint8_t *flags)
+{
+    return bmx_plane_flags_ex(ctx, lab, Z, Y, X, allaxis, 0, flags);
 }
 
 int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype, int64_t Z, int64_t Y,
@@ -341,7 +389,108 @@ __global__ __launch_bounds__(256) void k_select_seeds(const bmx_seed *__restrict
     if (keep) out[base + __popcll(ballot & ((1ull << lane) - 1ull))] = s;
 }
 
+// Adaptive walks of the large-volume path (pyopencl_large.py:570-590): the
+// seed at p (label L) walks if x+z and y+z are even, or if some interior voxel
+// q = p + 5*(i,j,k), |i|,|j|,|k| <= 20, carries a label other than L on a
+// labelled plane.  p itself is in that window with label L, so "another label
+// is seen" is "the window's smallest and largest label differ".  The window is
+// a product of one stride-5 set per axis (the interior test is per axis too),
+// so min and max are taken in three separable passes; BMX_LABEL_NONE (0xFF,
+// off-plane or ignored) is the identity of min and is mapped to 0 for max.
+constexpr int kAdaptStep = 5, kAdaptTaps = 20;   // offsets -100..100 in steps of 5
+
+struct MinMax {
+    uint8_t lo, hi;
+};
+
+// Pass along x: the label seen at q (labelled plane, else none), min/max over
+// the stride-5 window of x.
+__global__ __launch_bounds__(256) void k_adapt_x(const uint8_t *__restrict__ lab, const uint8_t *__restrict__ fz,
+                                                 const uint8_t *__restrict__ fy, const uint8_t *__restrict__ fx,
+                                                 int Z, int Y, int X, MinMax *__restrict__ out)
+{
+    const int64_t sz = (int64_t)X * Y, V = sz * Z;
+    for (int64_t i = bmx_gtid(); i < V; i += bmx_gstride()) {
+        const int z = (int)(i / sz);
+        const int64_t r = i - z * sz;
+        const int y = (int)(r / X);
+        const int x = (int)(r - (int64_t)y * X);
+        uint8_t lo = 0xFF, hi = 0;
+        if (fz[z] | fy[y]) {
+            const int64_t row = i - x;
+            for (int k = -kAdaptTaps; k <= kAdaptTaps; ++k) {
+                const int qx = x + kAdaptStep * k;
+                if (qx <= 0 || qx >= X - 1) continue;
+                const uint8_t v = lab[row + qx];
+                lo = min(lo, v);
+                hi = max(hi, v == BMX_LABEL_NONE ? (uint8_t)0 : v);
+            }
+        } else {
+            // only x-planes can be labelled on this row
+            for (int k = -kAdaptTaps; k <= kAdaptTaps; ++k) {
+                const int qx = x + kAdaptStep * k;
+                if (qx <= 0 || qx >= X - 1 || !fx[qx]) continue;
+                const uint8_t v = lab[i - x + qx];
+                lo = min(lo, v);
+                hi = max(hi, v == BMX_LABEL_NONE ? (uint8_t)0 : v);
+            }
+        }
+        out[i] = MinMax{lo, hi};
+    }
+}
+
+// Pass along y (stride X) or z (stride X*Y) of the (min, max) pairs.  The z
+// pass writes the mask: 1 where the seed walks.
+template <bool FINAL>
+__global__ __launch_bounds__(256) void k_adapt_axis(const MinMax *__restrict__ in, int Z, int Y, int X, int axis,
+                                                    MinMax *__restrict__ out, uint8_t *__restrict__ mask)
+{
+    const int64_t sz = (int64_t)X * Y, V = sz * Z;
+    const int64_t st = axis == 1 ? (int64_t)X : sz;
+    const int n = axis == 1 ? Y : Z;
+    for (int64_t i = bmx_gtid(); i < V; i += bmx_gstride()) {
+        const int z = (int)(i / sz);
+        const int64_t r = i - z * sz;
+        const int y = (int)(r / X);
+        const int x = (int)(r - (int64_t)y * X);
+        const int c = axis == 1 ? y : z;
+        uint8_t lo = 0xFF, hi = 0;
+        for (int k = -kAdaptTaps; k <= kAdaptTaps; ++k) {
+            const int q = c + kAdaptStep * k;
+            if (q <= 0 || q >= n - 1) continue;
+            const MinMax m = in[i + (int64_t)(kAdaptStep * k) * st];
+            lo = min(lo, m.lo);
+            hi = max(hi, m.hi);
+        }
+        if (FINAL) {
+            const bool lattice = ((x + z) % 2 == 0) && ((y + z) % 2 == 0);
+            mask[i] = (lattice || lo != hi) ? 1 : 0;
+        } else {
+            out[i] = MinMax{lo, hi};
+        }
+    }
+}
+
 }  // namespace
+
+extern "C" int bmx_adaptive_mask(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
+                                 const uint8_t *flags, void *tmp, uint8_t *mask)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(lab && flags && tmp && mask, "bmx_adaptive_mask: null pointer");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_adaptive_mask: bad dims");
+    const int64_t V = Z * Y * X;
+    MinMax *a = (MinMax *)tmp, *b = a + V;
+    const unsigned grid = bmx_stride_blocks(V, 256);
+    k_adapt_x<<<grid, 256, 0, ctx->stream>>>(lab, flags, flags + Z, flags + Z + Y, (int)Z, (int)Y, (int)X, a);
+    BMX_LAUNCH_CHECK();
+    k_adapt_axis<false><<<grid, 256, 0, ctx->stream>>>(a, (int)Z, (int)Y, (int)X, 1, b, nullptr);
+    BMX_LAUNCH_CHECK();
+    k_adapt_axis<true><<<grid, 256, 0, ctx->stream>>>(b, (int)Z, (int)Y, (int)X, 2, nullptr, mask);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
 
 extern "C" int bmx_select_seeds(bmx_ctx *ctx, const bmx_seed *seeds, int64_t nseeds, const uint8_t *mask,
                                 bmx_seed *out, int64_t *nout)

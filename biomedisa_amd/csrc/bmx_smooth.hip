@@ -169,14 +169,17 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__res
 __global__ __launch_bounds__(256) void k_argmax_update(const float *__restrict__ phi,
                                                        float *__restrict__ best,
                                                        uint8_t *__restrict__ bidx, int label, int Z, int Y,
-                                                       int X, Box bx)
+                                                       int X, Box bx, int clamp_first)
 {
     const int64_t V = (int64_t)Z * Y * X;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
     const bool inb = z >= bx.z0 && z < bx.z1 && y >= bx.y0 && y < bx.y1 && x >= bx.x0 && x < bx.x1;
-    const float p = inb ? phi[v] : 0.0f;
+    float p = inb ? phi[v] : 0.0f;
+    // large-volume path: the first label's smoothed map has its negative values
+    // set to 0 before the running maximum (pyopencl_large.py:464-469)
+    if (clamp_first && label == 0 && p < 0.0f) p = 0.0f;
     if (label == 0 || p > best[v]) {
         best[v] = p;
         bidx[v] = (uint8_t)label;
@@ -286,6 +289,12 @@ int bmx_smooth(bmx_ctx *ctx, float *phi, float *tmp, int64_t Z, int64_t Y, int64
 int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y, int64_t X,
                       int iters, const uint8_t *values, uint8_t *out)
 {
+    return bmx_smooth_argmax_ex(ctx, hits, nlabels, Z, Y, X, iters, values, out, 0);
+}
+
+int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y, int64_t X,
+                         int iters, const uint8_t *values, uint8_t *out, int clamp_first)
+{
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(hits && values && out, "bmx_smooth_argmax: null pointer");
@@ -327,7 +336,7 @@ int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z
             }
             if (rc == BMX_OK) {
                 k_argmax_update<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(res, best, bidx, l, (int)Z, (int)Y,
-                                                                            (int)X, bx);
+                                                                            (int)X, bx, clamp_first);
                 if (hipGetLastError() != hipSuccess)
                     rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
             }

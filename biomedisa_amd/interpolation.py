@@ -58,13 +58,30 @@ class PipelineError(ValueError):
     """Input rejected by the pipeline (the reference's _error_ cases)."""
 
 
+def select_path(dims, nlabels):
+    """The reference's choice between its small- and large-volume drivers
+    (interpolation.py:192-196): 'small' when the cropped box as float32 times the
+    label count stays below 1e10 bytes and the box below 2e9 bytes."""
+    nbytes = int(np.prod(dims)) * 4
+    return "small" if nbytes * nlabels < 1e10 and nbytes < 2e9 else "large"
+
+
 def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth=0,
                  uncertainty=False, rng="philox", key=DEFAULT_KEY, keep_hits=False, timings=None,
-                 walk_opts=None):
+                 walk_opts=None, path="small"):
     """Seeds, walk, merge (and optional smoothing / uncertainty) on device-resident inputs.
 
     walk_opts: keyword options of Engine.walk (algorithm, segment_steps,
     tile_shape, graph_rounds); they change the speed of the walk, not its hits.
+
+    path: 'small' follows rw_small / pyopencl_small (every seed walks); 'large'
+    the large-volume driver on one GPU (rw_large.py:40-123, pyopencl_large.py:
+    243-496): its plane readers (a labelled XY plane hides the next 4; allaxis
+    planes count positive labels only), adaptive walks (a seed walks if it lies
+    on the sub-lattice x+z, y+z even, or sees another label on the stride-5 grid
+    within +-100 voxels) and the clamp of the first label's smoothed map at 0.
+    Volumes beyond the reference's OpenCL range (1.8e9 voxels), which it splits
+    into z-subdomains that confine the walks, are walked whole here.
 
     Returns a dict of device buffers: 'regular' (uint8 V), optional 'smooth',
     'uncertainty', 'hits' (uint32 [nlabels][V]), plus host ints 'nseeds',
@@ -77,24 +94,38 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
         raise PipelineError("at most 255 labels (including background) are supported")
     t = timings if timings is not None else {}
     t0 = time.perf_counter()
+    if path not in ("small", "large"):
+        raise ValueError("path must be 'small' or 'large'")
+    large = path == "large"
     flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis)
     flags = flags_buf.download((Z + Y + X,), np.uint8)
     if allaxis and adjacent_planes((flags[:Z], flags[Z:Z + Y], flags[Z + Y:])):
         raise PipelineError("At least one empty slice between labels required.")
+    if large:
+        eng.plane_flags(vol.lab, vol.dims, allaxis, out=flags_buf, large=True)
+        flags = flags_buf.download((Z + Y + X,), np.uint8)
     seeds, nseeds = eng.seeds(vol.lab, vol.img, vol.img_dtype, vol.dims, flags_buf)
+    nwalk = nseeds
+    if large and nseeds:
+        mask = eng.adaptive_mask(vol.lab, vol.dims, flags_buf)
+        walkers = eng.alloc(max(1, nseeds) * _hip.SEED_DTYPE.itemsize)
+        _, nwalk = eng.select_seeds(seeds, nseeds, mask, out=walkers)
+        mask.free()
+        seeds.free()
+        seeds = walkers
     t1 = time.perf_counter()
     hits = eng.alloc(nlabels * V * 4)
     hits.zero()
-    accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, seeds, nseeds, nlabels, nbrw, sorw,
+    accepted = eng.walk(vol.img, vol.img_dtype, vol.dims, seeds, nwalk, nlabels, nbrw, sorw,
                         _RNG[rng], key, hits, **(walk_opts or {}))
     t2 = time.perf_counter()
-    out = {"nseeds": nseeds, "accepted": accepted, "flags": flags}
+    out = {"nseeds": nseeds, "walked_seeds": nwalk, "accepted": accepted, "flags": flags, "path": path}
     regular = eng.alloc(V)
     eng.argmax(hits, nlabels, V, values, regular)
     out["regular"] = regular
     if smooth:
         sm = eng.alloc(V)
-        eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm)
+        eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm, clamp_first=large)
         out["smooth"] = sm
     if uncertainty:
         uq = eng.alloc(V)
@@ -198,7 +229,7 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
                         ignore='none', only='all', smooth=0, compression=True, return_hits=False,
                         img_id=None, label_id=None, remote=False, queue=0, clean=None, fill=None,
                         slicer=False, rng="philox", key=DEFAULT_KEY, device=0, method="plain",
-                        guide=None):
+                        guide=None, path="auto"):
     """Smart interpolation of sparsely labelled slices (biomedisa/interpolation.py:43).
 
     Returns the reference's result dict ({'regular': uint8 (Z,Y,X)} plus
@@ -208,6 +239,10 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     method='guided' runs the surrogate-guided pipeline (run_pipeline_guided;
     Philox stream only, no 'uncertainty'/'hits'); `guide` is a dict of its
     thr/brad/srad/far/far_walks parameters.
+
+    path: 'auto' (default) takes the reference's small- or large-volume driver
+    by its size rule (interpolation.py:192-196), 'small' or 'large' forces one;
+    see run_pipeline.  The guided method always walks as the small driver.
     """
     if data is None or labelData is None:
         raise NotImplementedError("file loading (path_to_data/path_to_labels) is not part of this "
@@ -218,6 +253,8 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         raise ValueError("rng must be 'philox' or 'reference'")
     if method not in ("plain", "guided"):
         raise ValueError("method must be 'plain' or 'guided'")
+    if path not in ("auto", "small", "large"):
+        raise ValueError("path must be 'auto', 'small' or 'large'")
     if method == "guided" and (rng != "philox" or uncertainty or return_hits):
         raise ValueError("guided mode runs the Philox walker and returns 'regular'/'smooth' only")
 
@@ -234,8 +271,10 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
             dev = run_pipeline_guided(vol, bm.nol, prep.values, surrogate.net(device), bm.nbrw, bm.sorw,
                                       bm.allaxis, bm.smooth, key, **(guide or {}))
         else:
+            if path == "auto":
+                path = select_path(vol.dims, bm.nol)
             dev = run_pipeline(vol, bm.nol, prep.values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
-                               bm.uncertainty, rng, key, keep_hits=bm.return_hits)
+                               bm.uncertainty, rng, key, keep_hits=bm.return_hits, path=path)
     except PipelineError as e:
         _error_(bm, str(e))
         return None

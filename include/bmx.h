@@ -85,6 +85,22 @@ int bmx_copy(bmx_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 int bmx_plane_flags(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
                     int allaxis, uint8_t *flags);
 
+/* bmx_plane_flags with large = 1: the large-volume path's planes, replacing
+ * read_labeled_slices_large (biomedisa_helper.py:1068-1080: scanning z, a
+ * labelled plane hides the 4 planes after it) and read_labeled_slices_allx_large
+ * (:1082-1103: as allaxis, but only positive labels count). */
+int bmx_plane_flags_ex(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
+                       int allaxis, int large, uint8_t *flags);
+
+/* Adaptive walks of the large-volume path: replaces the `found` test of the
+ * randomWalk kernel in pyopencl_large.py:570-590.  mask[p] (uint8 over the
+ * volume) = 1 where a seed at p walks: x+z and y+z even, or an interior voxel
+ * on the stride-5 grid within +-100 voxels of p (per axis) lies on a labelled
+ * plane with a label other than p's.  tmp: device scratch of 4*Z*Y*X bytes.
+ * Select the walking seeds with bmx_select_seeds. */
+int bmx_adaptive_mask(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
+                      const uint8_t *flags, void *tmp, uint8_t *mask);
+
 /* Seed extraction: replaces the label-volume rebuild of pyopencl_small.py:
  * 89-107 and the per-work-item seed test and _calc_var of :140-185, :205-228.
  * Writes the seeds in ascending voxel order.  seeds == NULL counts only.
@@ -193,6 +209,13 @@ int bmx_smooth(bmx_ctx *ctx, float *phi, float *tmp, int64_t Z, int64_t Y, int64
  * `iters` times, then first-maximum argmax mapped through values. */
 int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y,
                       int64_t X, int iters, const uint8_t *values, uint8_t *out);
+
+/* bmx_smooth_argmax with clamp_first = 1: the large-volume path's 'smooth'
+ * (pyopencl_large.py:455-469), where the first label's smoothed map has its
+ * negative values set to 0 before the running maximum (max_to_label, :233-241:
+ * strictly greater wins, as the first-maximum argmax). */
+int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y,
+                         int64_t X, int iters, const uint8_t *values, uint8_t *out, int clamp_first);
 
 /* The 'uncertainty' result of rw_small.py:152-176 (MAX_SRC + UNCERTAINTY_SRC,
  * opencl_kernels.py:54-133): uint8(255 * (1 - (1-m0/m2)(1-m1/m2))) from the

@@ -496,6 +496,39 @@ void orc_uncertainty(const float *hits, int nlabels, int Z, int Y, int X, float 
             }
 }
 
+/* The `found` test of the large-volume kernel, as written in
+ * pyopencl_large.py:570-588: a seed at (plane, row, column) walks if
+ * (column + plane) and (row + plane) are even; otherwise it scans the offsets
+ * -100..100 in steps of 5 along each axis and walks if an interior voxel there
+ * holds a label of the seeds volume (labelled planes; ignore elsewhere) that is
+ * neither its own nor the ignore value.  out[k] = 1 if seed k walks. */
+void orc_adaptive(const uint8_t *lab, int Z, int Y, int X, const uint8_t *fz, const uint8_t *fy,
+                  const uint8_t *fx, const int64_t *pos, int64_t n, uint8_t *out)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t p = pos[k];
+        const int plane = (int)(p / sz), row = (int)((p % sz) / sy), column = (int)(p % sy);
+        const uint8_t segment = orc_mask(lab, fz, fy, fx, plane, row, column, p);
+        int found = 0;
+        if ((column + plane) % 2 == 0 && (row + plane) % 2 == 0) {
+            found = 1;
+        } else {
+            for (int z = -100; z < 101; z = z + 5)
+                for (int y = -100; y < 101; y = y + 5)
+                    for (int x = -100; x < 101; x = x + 5)
+                        if (plane + z > 0 && row + y > 0 && column + x > 0 && plane + z < Z - 1 &&
+                            row + y < Y - 1 && column + x < X - 1) {
+                            const int64_t q = (int64_t)(plane + z) * sz + (int64_t)(row + y) * sy + column + x;
+                            const uint8_t v = orc_mask(lab, fz, fy, fx, plane + z, row + y, column + x, q);
+                            if (v != segment && v != ORC_IGN) found = 1;
+                        }
+        }
+        out[k] = (uint8_t)found;
+    }
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

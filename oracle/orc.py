@@ -48,6 +48,7 @@ def lib():
         L.orc_argmax_f32.argtypes = [_p, C.c_int, _i64, _p, _p]
         L.orc_smooth.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_uncertainty.argtypes = [_p, C.c_int, C.c_int, C.c_int, C.c_int, _p]
+        L.orc_adaptive.argtypes = [_p, C.c_int, C.c_int, C.c_int, _p, _p, _p, _p, _i64, _p]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -110,6 +111,17 @@ def seeds(labidx, img, flags):
     L.orc_seeds(_ptr(labidx), _ptr(img), _img_dtype(img), Z, Y, X,
                 _ptr(fz), _ptr(fy), _ptr(fx), _ptr(pos), _ptr(lab), _ptr(var))
     return pos, lab, var
+
+
+def adaptive(labidx, flags, pos):
+    """The large-volume kernel's `found` test per seed (bool array)."""
+    labidx = np.ascontiguousarray(labidx, dtype=np.uint8)
+    Z, Y, X = labidx.shape
+    fz, fy, fx = [np.ascontiguousarray(f, dtype=np.uint8) for f in flags]
+    pos = np.ascontiguousarray(pos, np.int64)
+    out = np.zeros(pos.size, np.uint8)
+    lib().orc_adaptive(_ptr(labidx), Z, Y, X, _ptr(fz), _ptr(fy), _ptr(fx), _ptr(pos), pos.size, _ptr(out))
+    return out.astype(bool)
 
 
 def philox_walk(img, pos, lab, var, nlabels, nbrw, sorw, key):

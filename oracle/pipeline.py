@@ -3,7 +3,9 @@
 Follows biomedisa/interpolation.py:135-204 (dtype handling, pad, bounding-box
 crop), biomedisa_helper.py:639-724 (label set), the backend flow of
 pyopencl_small.py:80-135 and rw_small.py:114-201 (walk, smoothing,
-uncertainty, argmax, get_labels).  Heavy loops run in libbmx_oracle.so.
+uncertainty, argmax, get_labels), and for path='large' that of
+pyopencl_large.py:243-496 with rw_large.py:40-123 on one GPU (planes,
+adaptive seeds, running maxima).  Heavy loops run in libbmx_oracle.so.
 Written independently of biomedisa_amd so that the two can check each other.
 """
 import numpy as np
@@ -67,6 +69,56 @@ def prepare(data, labelData):
                 crop=(z0, z1, y0, y1, x0, x1), padded_shape=img.shape, nol=len(vals))
 
 
+def _thin_planes_positive(arr):
+    """read_labeled_slices_allx_large (biomedisa_helper.py:1082-1103): indices of
+    the planes of arr (axis 0) holding a positive label that survives the removal
+    of 1-voxel-thin lines along axis 1, then axis 2."""
+    ones = (arr > 0).astype(np.int8)
+    grad = np.zeros_like(ones)
+    t = np.abs(ones[:, :-1] - ones[:, 1:])
+    grad[:, :-1] += t
+    grad[:, 1:] += t
+    ones[grad == 2] = 0
+    grad.fill(0)
+    t = np.abs(ones[:, :, :-1] - ones[:, :, 1:])
+    grad[:, :, :-1] += t
+    grad[:, :, 1:] += t
+    ones[grad == 2] = 0
+    return [k for k in range(arr.shape[0]) if np.any(ones[k])]
+
+
+def plane_flags_large(lab, allaxis):
+    """Labelled planes of the large-volume path on the label VALUE volume `lab`
+    (negative = ignored): XY planes by read_labeled_slices_large
+    (biomedisa_helper.py:1068-1080: a labelled plane hides the next 4), allaxis
+    planes by read_labeled_slices_allx_large on the volume and its two swapped
+    views (rw_large.py:67-99)."""
+    Z, Y, X = lab.shape
+    fz, fy, fx = np.zeros(Z, np.uint8), np.zeros(Y, np.uint8), np.zeros(X, np.uint8)
+    if not allaxis:
+        i = 0
+        while i < Z:
+            if np.any(lab[i]):
+                fz[i] = 1
+                i += 5
+            else:
+                i += 1
+        return fz, fy, fx
+    fz[_thin_planes_positive(lab)] = 1
+    t = np.ascontiguousarray(np.swapaxes(lab, 0, 1))
+    fy[_thin_planes_positive(t)] = 1
+    t = np.ascontiguousarray(np.swapaxes(t, 0, 2))
+    fx[_thin_planes_positive(t)] = 1
+    return fz, fy, fx
+
+
+def select_path(p):
+    """interpolation.py:192-196: the small path when the cropped box's float32
+    volume times the label count is below 1e10 bytes and the volume below 2e9."""
+    nbytes = int(np.prod(p["img"].shape)) * 4
+    return "small" if nbytes * p["nol"] < 1e10 and nbytes < 2e9 else "large"
+
+
 def _uncrop(vol_c, p, dtype=np.uint8):
     z0, z1, y0, y1, x0, x1 = p["crop"]
     full = np.zeros(vol_c.shape[:-3] + tuple(p["padded_shape"]), dtype=dtype)
@@ -75,22 +127,32 @@ def _uncrop(vol_c, p, dtype=np.uint8):
 
 
 def smart_interpolation(data, labelData, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                        uncertainty=False, return_hits=False, rng="philox", key=PHILOX_KEY):
-    """CPU oracle of the drop-in -> result dict (+ '_hits_u32', '_accepted', '_nseeds')."""
+                        uncertainty=False, return_hits=False, rng="philox", key=PHILOX_KEY, path="small"):
+    """CPU oracle of the drop-in -> result dict (+ '_hits_u32', '_accepted', '_nseeds').
+    path: 'small', 'large' or 'auto' (the reference's size rule)."""
     p = prepare(data, labelData)
     flags = orc.plane_flags(p["labidx"], allaxis)
     if allaxis and any(np.any(f[1:] & f[:-1]) for f in flags):
         return None
+    if path == "auto":
+        path = select_path(p)
+    if path == "large":
+        flags = plane_flags_large(p["lab"], allaxis)
     pos, lab, var = orc.seeds(p["labidx"], p["img"], flags)
+    if path == "large":
+        keep = orc.adaptive(p["labidx"], flags, pos)
+        pos, lab, var = pos[keep], lab[keep], var[keep]
     nol = p["nol"]
     if rng == "philox":
         hits, acc = orc.philox_walk(p["img"], pos, lab, var, nol, nbrw, sorw, key)
     else:
         hits, acc = orc.ref_walk(p["img"], pos, lab, var, nol, nbrw, sorw)
-    res = {"_hits_u32": hits, "_accepted": acc, "_nseeds": pos.size, "_prep": p}
+    res = {"_hits_u32": hits, "_accepted": acc, "_nseeds": pos.size, "_prep": p, "_path": path}
     res["regular"] = _uncrop(orc.argmax(hits, p["values"]), p)
     if smooth:
         phi = np.stack([orc.smooth(h.astype(np.float32), smooth) for h in hits])
+        if path == "large":
+            phi[0][phi[0] < 0] = 0   # pyopencl_large.py:466
         res["smooth"] = _uncrop(orc.argmax(phi, p["values"]), p)
     if uncertainty:
         u = orc.uncertainty(hits.astype(np.float32))
