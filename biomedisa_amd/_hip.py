@@ -51,6 +51,8 @@ SIGNATURES = {
     "bmx_plane_flags_ex": (_i, [_p, _p, _i64, _i64, _i64, _i, _i, _p]),
     "bmx_adaptive_mask": (_i, [_p, _p, _i64, _i64, _i64, _p, _p, _p]),
     "bmx_smooth_argmax_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p, _i]),
+    "bmx_clean": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
+    "bmx_fill": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),
@@ -296,6 +298,16 @@ class Engine:
         values = np.ascontiguousarray(values, dtype=np.uint8)
         _check(load().bmx_smooth_argmax_ex(self.ctx, hits.ptr, int(nlabels), Z, Y, X, int(iters),
                                            values.ctypes.data, out.ptr, int(bool(clamp_first))))
+
+    def clean(self, vol, dims, threshold, out):
+        """Outlier removal of a uint8 label-value volume (device) into out."""
+        Z, Y, X = dims
+        _check(load().bmx_clean(self.ctx, vol.ptr, Z, Y, X, float(threshold), out.ptr))
+
+    def fill(self, vol, dims, threshold, out):
+        """Hole filling of a uint8 label-value volume (device) into out."""
+        Z, Y, X = dims
+        _check(load().bmx_fill(self.ctx, vol.ptr, Z, Y, X, float(threshold), out.ptr))
 
     def uncertainty(self, hits, nlabels, dims, out):
         Z, Y, X = dims

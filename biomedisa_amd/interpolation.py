@@ -247,8 +247,8 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     if data is None or labelData is None:
         raise NotImplementedError("file loading (path_to_data/path_to_labels) is not part of this "
                                   "engine: pass NumPy arrays")
-    if acwe or clean or fill or denoise:
-        raise NotImplementedError("acwe/clean/fill/denoise post-processing is not implemented")
+    if acwe or denoise:
+        raise NotImplementedError("acwe/denoise post-processing is not implemented")
     if rng not in _RNG:
         raise ValueError("rng must be 'philox' or 'reference'")
     if method not in ("plain", "guided"):
@@ -278,7 +278,39 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     except PipelineError as e:
         _error_(bm, str(e))
         return None
-    return _results(prep, dev)
+    results = _results(prep, dev)
+    post_process(results, clean, fill, device)
+    return results
+
+
+def post_process(results, clean=None, fill=None, device=0):
+    """The reference's label post-processing (rw_small.py:222-241) on the device:
+    'cleaned' (and 'smooth_cleaned') by bmx_clean, 'filled' by bmx_fill, and
+    'cleaned_filled' = cleaned + (filled - regular) in uint8 arithmetic, as the
+    reference computes it."""
+    if not (clean or fill):
+        return results
+    eng = _hip.engine(device)
+
+    def run(name, vol, threshold):
+        buf = eng.upload(np.ascontiguousarray(vol, dtype=np.uint8))
+        out = eng.alloc(vol.size)
+        getattr(eng, name)(buf, vol.shape, threshold, out)
+        res = out.download(vol.shape, np.uint8)
+        buf.free()
+        out.free()
+        return res
+
+    final = results["regular"]
+    if clean:
+        results["cleaned"] = run("clean", final, clean)
+        if "smooth" in results:
+            results["smooth_cleaned"] = run("clean", results["smooth"], clean)
+    if fill:
+        results["filled"] = run("fill", final, fill)
+    if clean and fill:
+        results["cleaned_filled"] = results["cleaned"] + (results["filled"] - final)
+    return results
 
 
 class _Prepared:

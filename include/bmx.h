@@ -217,6 +217,22 @@ int bmx_smooth_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z
 int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, int64_t Y,
                          int64_t X, int iters, const uint8_t *values, uint8_t *out, int clamp_first);
 
+/* Outlier removal: replaces clean (biomedisa/features/remove_outlier.py:60-95)
+ * on a label VALUE volume (device uint8, Z*Y*X).  For every present value but
+ * the smallest, 6-connected components of that value smaller than or equal to
+ * threshold x its largest component become 0.  out may equal in. */
+int bmx_clean(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, double threshold,
+              uint8_t *out);
+
+/* Hole filling: replaces fill (remove_outlier.py:97-139).  For every present
+ * value k but the smallest, in ascending order and on the volume as changed by
+ * the values before it: in the box of k grown by one voxel, 6-connected
+ * components of "not k" other than the largest entry of [voxels of k,
+ * component sizes] become k unless larger than threshold x (voxels of k);
+ * when the voxels of k are that largest entry they become 0.  out may equal in. */
+int bmx_fill(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, double threshold,
+             uint8_t *out);
+
 /* The 'uncertainty' result of rw_small.py:152-176 (MAX_SRC + UNCERTAINTY_SRC,
  * opencl_kernels.py:54-133): uint8(255 * (1 - (1-m0/m2)(1-m1/m2))) from the
  * three largest hit counts, interior voxels; 0 elsewhere. */

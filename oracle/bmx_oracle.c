@@ -529,6 +529,38 @@ void orc_adaptive(const uint8_t *lab, int Z, int Y, int X, const uint8_t *fz, co
     }
 }
 
+/* 6-connected components of mask (non-zero = foreground), numbered 1, 2, ...
+ * in the raster order of their first voxel (breadth-first from each unvisited
+ * foreground voxel in raster order); labels[i] = 0 for background.  Returns the
+ * component count.  queue: scratch of Z*Y*X int64. */
+int64_t orc_label6(const uint8_t *mask, int Z, int Y, int X, int32_t *labels, int64_t *queue)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
+    for (int64_t i = 0; i < V; ++i) labels[i] = 0;
+    int32_t n = 0;
+    for (int64_t s = 0; s < V; ++s) {
+        if (!mask[s] || labels[s]) continue;
+        ++n;
+        int64_t head = 0, tail = 0;
+        queue[tail++] = s;
+        labels[s] = n;
+        while (head < tail) {
+            const int64_t i = queue[head++];
+            const int z = (int)(i / sz), y = (int)((i % sz) / sy), x = (int)(i % sy);
+            const int64_t nb[6] = {x > 0 ? i - 1 : -1, x < X - 1 ? i + 1 : -1, y > 0 ? i - sy : -1,
+                                   y < Y - 1 ? i + sy : -1, z > 0 ? i - sz : -1, z < Z - 1 ? i + sz : -1};
+            for (int k = 0; k < 6; ++k) {
+                const int64_t j = nb[k];
+                if (j >= 0 && mask[j] && !labels[j]) {
+                    labels[j] = n;
+                    queue[tail++] = j;
+                }
+            }
+        }
+    }
+    return n;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

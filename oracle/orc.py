@@ -49,6 +49,8 @@ def lib():
         L.orc_smooth.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_uncertainty.argtypes = [_p, C.c_int, C.c_int, C.c_int, C.c_int, _p]
         L.orc_adaptive.argtypes = [_p, C.c_int, C.c_int, C.c_int, _p, _p, _p, _p, _i64, _p]
+        L.orc_label6.restype = _i64
+        L.orc_label6.argtypes = [_p, C.c_int, C.c_int, C.c_int, _p, _p]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -111,6 +113,17 @@ def seeds(labidx, img, flags):
     L.orc_seeds(_ptr(labidx), _ptr(img), _img_dtype(img), Z, Y, X,
                 _ptr(fz), _ptr(fy), _ptr(fx), _ptr(pos), _ptr(lab), _ptr(var))
     return pos, lab, var
+
+
+def label6(mask):
+    """6-connected components -> (int32 labels numbered in raster order of each
+    component's first voxel, count)."""
+    mask = np.ascontiguousarray(mask, dtype=np.uint8)
+    Z, Y, X = mask.shape
+    labels = np.empty(mask.shape, np.int32)
+    queue = np.empty(mask.size, np.int64)
+    n = lib().orc_label6(_ptr(mask), Z, Y, X, _ptr(labels), _ptr(queue))
+    return labels, int(n)
 
 
 def adaptive(labidx, flags, pos):

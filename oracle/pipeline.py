@@ -11,6 +11,7 @@ Written independently of biomedisa_amd so that the two can check each other.
 import numpy as np
 
 from . import orc
+from . import post
 
 PHILOX_KEY = 0x5EED0B10D1CE0001
 
@@ -127,7 +128,8 @@ def _uncrop(vol_c, p, dtype=np.uint8):
 
 
 def smart_interpolation(data, labelData, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                        uncertainty=False, return_hits=False, rng="philox", key=PHILOX_KEY, path="small"):
+                        uncertainty=False, return_hits=False, rng="philox", key=PHILOX_KEY, path="small",
+                        clean=None, fill=None):
     """CPU oracle of the drop-in -> result dict (+ '_hits_u32', '_accepted', '_nseeds').
     path: 'small', 'large' or 'auto' (the reference's size rule)."""
     p = prepare(data, labelData)
@@ -159,4 +161,13 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, allaxis=False, smoo
         res["uncertainty"] = _uncrop((u * np.float32(255.0)).astype(np.uint8), p)
     if return_hits:
         res["hits"] = _uncrop(hits.astype(np.float32), p, np.float32)
+    # rw_small.py:222-241
+    if clean:
+        res["cleaned"] = post.clean(res["regular"], clean)
+        if smooth:
+            res["smooth_cleaned"] = post.clean(res["smooth"], clean)
+    if fill:
+        res["filled"] = post.fill(res["regular"], fill)
+    if clean and fill:
+        res["cleaned_filled"] = post.cleaned_filled(res["regular"], res["cleaned"], res["filled"])
     return res

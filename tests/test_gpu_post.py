@@ -1,0 +1,82 @@
+"""clean / fill on the GPU (bmx_clean, bmx_fill: union-find components) against
+the oracle's restatement of remove_outlier.py, bit-exact."""
+import numpy as np
+import pytest
+
+import oracle.pipeline as opipe
+from oracle import post
+from biomedisa_amd import _hip
+from tests.test_post_oracle import _hollow_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _gpu(eng, name, vol, threshold):
+    buf = eng.upload(np.ascontiguousarray(vol, dtype=np.uint8))
+    out = eng.alloc(vol.size)
+    getattr(eng, name)(buf, vol.shape, threshold, out)
+    return out.download(vol.shape, np.uint8)
+
+
+def _blobs(rng, shape, nlab, density):
+    """Random label volume: smoothed noise thresholded into several labels, so
+    there are large components, small outliers and holes."""
+    from scipy import ndimage
+    noise = ndimage.uniform_filter(rng.random(shape), size=3)
+    q = np.quantile(noise, np.linspace(1 - density, 1, nlab + 1))
+    vol = np.zeros(shape, np.uint8)
+    for k in range(nlab):
+        vol[(noise >= q[k]) & (noise < q[k + 1])] = k + 1
+    speck = rng.random(shape) < 0.01
+    vol[speck] = rng.integers(0, nlab + 1, int(speck.sum()))
+    return vol
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_clean_and_fill_random_bitexact(eng, case):
+    rng = np.random.default_rng(300 + case)
+    shape = tuple(int(s) for s in rng.integers(10, 48, 3))
+    vol = _blobs(rng, shape, int(rng.integers(1, 6)), float(rng.uniform(0.3, 0.9)))
+    for name, t in (("clean", 0.1), ("clean", 0.01), ("fill", 0.9), ("fill", 0.05)):
+        _say("%s threshold %g case %d shape %s" % (name, t, case, shape))
+        got = _gpu(eng, name, vol, t)
+        want = getattr(post, name)(vol, t)
+        np.testing.assert_array_equal(got, want, err_msg="%s %g" % (name, t))
+
+
+def test_fill_hand_cases(eng):
+    for vol, t in ((_hollow_cube(12, 2, 10, 4), 0.9), (_hollow_cube(14, 1, 13, 2), 0.9),
+                   (_hollow_cube(14, 1, 13, 2), 1.5)):
+        vol = vol.copy()
+        vol[vol.shape[0] // 2, 5, 5] = 6
+        np.testing.assert_array_equal(_gpu(eng, "fill", vol, t), post.fill(vol, t))
+    cube = np.zeros((12, 12, 12), np.uint8)
+    cube[1:11, 1:11, 1:11] = 1        # the label outnumbers its outside: removed, shell filled
+    np.testing.assert_array_equal(_gpu(eng, "fill", cube, 0.9), post.fill(cube, 0.9))
+
+
+def test_components_in_a_large_box(eng):
+    """A 160 x 200 x 220 volume (7 M voxels) of many components: the union-find
+    across all XCDs gives the oracle's components."""
+    rng = np.random.default_rng(9)
+    vol = _blobs(rng, (160, 200, 220), 3, 0.6)
+    _say("clean/fill on %s" % (vol.shape,))
+    np.testing.assert_array_equal(_gpu(eng, "clean", vol, 0.05), post.clean(vol, 0.05))
+    np.testing.assert_array_equal(_gpu(eng, "fill", vol, 0.5), post.fill(vol, 0.5))
+
+
+def test_smart_interpolation_clean_fill_keys():
+    """smart_interpolation(clean=, fill=) returns the reference's post-processing
+    keys equal to the oracle pipeline's."""
+    from biomedisa_amd.interpolation import smart_interpolation
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(36, 40, 44), seed=12, n_struct=3, radius=(4, 10), every=4, allaxis=False)
+    kw = dict(nbrw=3, sorw=150, smooth=2, clean=0.1, fill=0.9)
+    got = smart_interpolation(vol.img, vol.labels, **kw)
+    want = opipe.smart_interpolation(vol.img, vol.labels, **kw)
+    for k in ("regular", "smooth", "cleaned", "smooth_cleaned", "filled", "cleaned_filled"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
