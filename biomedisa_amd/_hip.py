@@ -53,6 +53,7 @@ SIGNATURES = {
     "bmx_smooth_argmax_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p, _i]),
     "bmx_clean": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
     "bmx_fill": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
+    "bmx_acwe": (_i, [_p, _p, _p, _i64, _i64, _i64, _p, _i, C.c_double, _i, _i, _p, _p]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),
@@ -308,6 +309,13 @@ class Engine:
         """Hole filling of a uint8 label-value volume (device) into out."""
         Z, Y, X = dims
         _check(load().bmx_fill(self.ctx, vol.ptr, Z, Y, X, float(threshold), out.ptr))
+
+    def acwe(self, img, lab, dims, labels, alpha, smooth, steps, out, tmp):
+        """Active-contour refinement of the label volume lab (device) into out."""
+        Z, Y, X = dims
+        labels = np.ascontiguousarray(labels, dtype=np.uint8)
+        _check(load().bmx_acwe(self.ctx, img.ptr, lab.ptr, Z, Y, X, labels.ctypes.data, len(labels),
+                               float(alpha), int(smooth), int(steps), out.ptr, tmp.ptr))
 
     def uncertainty(self, hits, nlabels, dims, out):
         Z, Y, X = dims

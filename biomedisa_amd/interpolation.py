@@ -247,8 +247,8 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     if data is None or labelData is None:
         raise NotImplementedError("file loading (path_to_data/path_to_labels) is not part of this "
                                   "engine: pass NumPy arrays")
-    if acwe or denoise:
-        raise NotImplementedError("acwe/denoise post-processing is not implemented")
+    if denoise:
+        raise NotImplementedError("denoise pre-processing is not implemented")
     if rng not in _RNG:
         raise ValueError("rng must be 'philox' or 'reference'")
     if method not in ("plain", "guided"):
@@ -280,6 +280,15 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         return None
     results = _results(prep, dev)
     post_process(results, clean, fill, device)
+    if acwe:
+        # rw_small.py:244-253: the (normalised) image of the walk's box, zeros
+        # elsewhere, and the regular result
+        from .post import active_contour
+        z0, z1, y0, y1, x0, x1 = prep.box
+        data = np.zeros((bm.zsh, bm.ysh, bm.xsh), dtype=prep.img.dtype)
+        data[z0:z1, y0:y1, x0:x1] = prep.img
+        data = data[1:-1, 1:-1, 1:-1]
+        results["acwe"] = active_contour(data, results["regular"], acwe_alpha, acwe_smooth, acwe_steps, device)
     return results
 
 

@@ -233,6 +233,17 @@ int bmx_clean(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, 
 int bmx_fill(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, double threshold,
              uint8_t *out);
 
+/* Active contour: replaces the acwe loop of activeContour
+ * (biomedisa/features/active_contour.py:158-169) with evolution and curvop
+ * (curvop_numba.py:32-148).  img (uint8) and lab (label values) are the
+ * padded, cropped volumes the reference works on; labels[0..nlabels) its
+ * allLabels (ascending).  steps times: label means, evolution (alpha weights
+ * label 0), then `smooth` erosion/dilation rounds per label.  out receives the
+ * labels; tmp is device scratch of Z*Y*X bytes. */
+int bmx_acwe(bmx_ctx *ctx, const uint8_t *img, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
+             const uint8_t *labels, int nlabels, double alpha, int smooth, int steps, uint8_t *out,
+             uint8_t *tmp);
+
 /* The 'uncertainty' result of rw_small.py:152-176 (MAX_SRC + UNCERTAINTY_SRC,
  * opencl_kernels.py:54-133): uint8(255 * (1 - (1-m0/m2)(1-m1/m2))) from the
  * three largest hit counts, interior voxels; 0 elsewhere. */

@@ -561,6 +561,144 @@ int64_t orc_label6(const uint8_t *mask, int Z, int Y, int X, int32_t *labels, in
     return n;
 }
 
+/* evolution of curvop_numba.py:32-57, as written: B = A, then every interior
+ * voxel takes the first neighbour label (27 positions, dz/dy/dx -1..1) with a
+ * strictly smaller cost |mean[l] - img| (alpha x that for label 0) than the
+ * best so far, starting from its own label.  Costs in double. */
+void orc_evolution(const float *mean, const uint8_t *A, const uint8_t *data, int Z, int Y, int X, double alpha,
+                   uint8_t *B)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
+    for (int64_t i = 0; i < V; ++i) B[i] = A[i];
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k < Z - 1; ++k)
+        for (int l = 1; l < Y - 1; ++l)
+            for (int m = 1; m < X - 1; ++m) {
+                const int64_t i = k * sz + l * sy + m;
+                const uint8_t refLabel = A[i];
+                uint8_t finLabel = refLabel;
+                const double img = data[i];
+                double ref = refLabel == 0 ? alpha * fabs((double)mean[refLabel] - img)
+                                           : fabs((double)mean[refLabel] - img);
+                for (int n = -1; n < 2; ++n)
+                    for (int o = -1; o < 2; ++o)
+                        for (int p = -1; p < 2; ++p) {
+                            const uint8_t tmpLabel = A[i + n * sz + o * sy + p];
+                            if (tmpLabel != refLabel) {
+                                const double val = tmpLabel == 0 ? alpha * fabs((double)mean[tmpLabel] - img)
+                                                                 : fabs((double)mean[tmpLabel] - img);
+                                if (val < ref) {
+                                    ref = val;
+                                    finLabel = tmpLabel;
+                                }
+                            }
+                        }
+                B[i] = finLabel;
+            }
+}
+
+/* The planar patterns _P3 of curvop (curvop_numba.py:130-138), P3[t][k][l][m]. */
+static void orc_patterns(int P3[9][3][3][3])
+{
+    for (int t = 0; t < 9; ++t)
+        for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l)
+                for (int m = 0; m < 3; ++m) P3[t][k][l][m] = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            P3[0][a][b][1] = 1;
+            P3[1][a][1][b] = 1;
+            P3[2][1][a][b] = 1;
+            P3[3][a][b][b] = 1;
+            P3[4][a][b][2 - b] = 1;
+            P3[5][b][a][b] = 1;
+            P3[6][b][a][2 - b] = 1;
+            P3[7][b][b][a] = 1;
+            P3[8][b][2 - b][a] = 1;
+        }
+}
+
+/* erosion and dilation of curvop_numba.py:59-121, as written (reads start,
+ * writes final). */
+void orc_erosion(const uint8_t *start, uint8_t *final_, int zsh, int ysh, int xsh, int label)
+{
+    int P3[9][3][3][3];
+    orc_patterns(P3);
+    const int64_t sy = xsh, sz = (int64_t)xsh * ysh;
+#pragma omp parallel for schedule(static)
+    for (int plane = 1; plane < zsh - 1; ++plane)
+        for (int row = 1; row < ysh - 1; ++row)
+            for (int column = 1; column < xsh - 1; ++column) {
+                const int64_t c = plane * sz + row * sy + column;
+                int found = 0;
+                for (int n = -1; n < 2; ++n)
+                    for (int o = -1; o < 2; ++o)
+                        for (int p = -1; p < 2; ++p)
+                            if (start[c + n * sz + o * sy + p] != label) found = 1;
+                if (start[c] == label && found == 1) {
+                    int t = 0;
+                    found = 0;
+                    while (t < 9 && found == 0) {
+                        long value = 0;
+                        for (int k = 0; k < 3; ++k)
+                            for (int l = 0; l < 3; ++l)
+                                for (int m = 0; m < 3; ++m)
+                                    if (P3[t][k][l][m] == 1)
+                                        value += labs((long)start[c + (k - 1) * sz + (l - 1) * sy + (m - 1)] - label);
+                        if (value == 0) found = 1;
+                        t += 1;
+                    }
+                    if (found == 0) {
+                        int subLabel = label;
+                        for (int n = -1; n < 2; ++n)
+                            for (int o = -1; o < 2; ++o)
+                                for (int p = -1; p < 2; ++p) {
+                                    const int tmpLabel = start[c + n * sz + o * sy + p];
+                                    if (tmpLabel != label) subLabel = tmpLabel;
+                                }
+                        final_[c] = (uint8_t)subLabel;
+                    }
+                }
+            }
+}
+
+void orc_dilation(const uint8_t *start, uint8_t *final_, int zsh, int ysh, int xsh, int label)
+{
+    int P3[9][3][3][3];
+    orc_patterns(P3);
+    const int64_t sy = xsh, sz = (int64_t)xsh * ysh;
+#pragma omp parallel for schedule(static)
+    for (int plane = 1; plane < zsh - 1; ++plane)
+        for (int row = 1; row < ysh - 1; ++row)
+            for (int column = 1; column < xsh - 1; ++column) {
+                const int64_t c = plane * sz + row * sy + column;
+                int found = 0;
+                for (int n = -1; n < 2; ++n)
+                    for (int o = -1; o < 2; ++o)
+                        for (int p = -1; p < 2; ++p)
+                            if (start[c + n * sz + o * sy + p] == label) found = 1;
+                if (start[c] != label && found == 1) {
+                    int t = 0;
+                    found = 0;
+                    while (t < 9 && found == 0) {
+                        long value = 0;
+                        for (int k = 0; k < 3; ++k)
+                            for (int l = 0; l < 3; ++l)
+                                for (int m = 0; m < 3; ++m) {
+                                    if (P3[t][k][l][m] == 1 &&
+                                        start[c + (k - 1) * sz + (l - 1) * sy + (m - 1)] != label)
+                                        value += 0;
+                                    else
+                                        value += 1;
+                                }
+                        if (value == 0) found = 1;
+                        t += 1;
+                    }
+                    if (found == 0) final_[c] = (uint8_t)label;
+                }
+            }
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -51,6 +51,9 @@ def lib():
         L.orc_adaptive.argtypes = [_p, C.c_int, C.c_int, C.c_int, _p, _p, _p, _p, _i64, _p]
         L.orc_label6.restype = _i64
         L.orc_label6.argtypes = [_p, C.c_int, C.c_int, C.c_int, _p, _p]
+        L.orc_evolution.argtypes = [_p, _p, _p, C.c_int, C.c_int, C.c_int, C.c_double, _p]
+        L.orc_erosion.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_dilation.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -124,6 +127,24 @@ def label6(mask):
     queue = np.empty(mask.size, np.int64)
     n = lib().orc_label6(_ptr(mask), Z, Y, X, _ptr(labels), _ptr(queue))
     return labels, int(n)
+
+
+def evolution(mean, A, data, alpha):
+    mean = np.ascontiguousarray(mean, np.float32)
+    A = np.ascontiguousarray(A, np.uint8)
+    data = np.ascontiguousarray(data, np.uint8)
+    B = np.empty_like(A)
+    lib().orc_evolution(_ptr(mean), _ptr(A), _ptr(data), *A.shape, float(alpha), _ptr(B))
+    return B
+
+
+def erosion(start, final, label):
+    """Reads start, writes into final (in place), as curvop_numba.erosion."""
+    lib().orc_erosion(_ptr(start), _ptr(final), *start.shape, int(label))
+
+
+def dilation(start, final, label):
+    lib().orc_dilation(_ptr(start), _ptr(final), *start.shape, int(label))
 
 
 def adaptive(labidx, flags, pos):

@@ -129,7 +129,7 @@ def _uncrop(vol_c, p, dtype=np.uint8):
 
 def smart_interpolation(data, labelData, nbrw=10, sorw=4000, allaxis=False, smooth=0,
                         uncertainty=False, return_hits=False, rng="philox", key=PHILOX_KEY, path="small",
-                        clean=None, fill=None):
+                        clean=None, fill=None, acwe=False, acwe_alpha=1.0, acwe_smooth=1, acwe_steps=3):
     """CPU oracle of the drop-in -> result dict (+ '_hits_u32', '_accepted', '_nseeds').
     path: 'small', 'large' or 'auto' (the reference's size rule)."""
     p = prepare(data, labelData)
@@ -170,4 +170,10 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, allaxis=False, smoo
         res["filled"] = post.fill(res["regular"], fill)
     if clean and fill:
         res["cleaned_filled"] = post.cleaned_filled(res["regular"], res["cleaned"], res["filled"])
+    # rw_small.py:244-253
+    if acwe:
+        z0, z1, y0, y1, x0, x1 = p["crop"]
+        data = np.zeros(p["padded_shape"], dtype=p["img"].dtype)
+        data[z0:z1, y0:y1, x0:x1] = p["img"]
+        res["acwe"] = post.acwe(data[1:-1, 1:-1, 1:-1], res["regular"], acwe_alpha, acwe_smooth, acwe_steps)
     return res

@@ -72,3 +72,79 @@ def fill(image, threshold=0.9):
 def cleaned_filled(final, cleaned, filled):
     """rw_small.py:238: cleaned + (filled - final) in uint8 arithmetic."""
     return (cleaned + (filled - final)).astype(np.uint8)
+
+
+def img_to_uint8(img):
+    """biomedisa_helper.py:345-352."""
+    if img.dtype != np.uint8:
+        img = img.astype(np.float32)
+        img -= np.amin(img)
+        img /= np.amax(img)
+        img *= 255.0
+        img = img.astype(np.uint8)
+    return img
+
+
+def _acwe_box(labels):
+    """reduce_blocksize of active_contour.py:91-110: bbox of the non-zero labels
+    grown by 100 and clamped."""
+    zsh, ysh, xsh = labels.shape
+    nz = np.nonzero(labels)
+    lo = lambda v: v - 100 if v - 100 > 0 else 0
+    hi = lambda v, n: v + 100 if v + 100 < n else n
+    return (slice(lo(int(nz[0].min())), hi(int(nz[0].max()), zsh)),
+            slice(lo(int(nz[1].min())), hi(int(nz[1].max()), ysh)),
+            slice(lo(int(nz[2].min())), hi(int(nz[2].max()), xsh)))
+
+
+def curvop(start, steps, label):
+    """curvop_numba.py:124-148 with its start/final copies."""
+    start = np.ascontiguousarray(start, np.uint8).copy()
+    final = start.copy()
+    for k in range(steps):
+        if k % 2 == 0:
+            orc.erosion(start, final, label)
+            start = final.copy()
+            orc.dilation(final, start, label)
+            final = start.copy()
+        else:
+            orc.dilation(start, final, label)
+            start = final.copy()
+            orc.erosion(final, start, label)
+            final = start.copy()
+    return start
+
+
+def acwe_labels(data, labels, alpha=1.0, smooth=1, steps=3):
+    """The padded, cropped inputs and the acwe loop of active_contour.py:146-169
+    -> (image box, label box before, label box after, box, padded shape)."""
+    data = img_to_uint8(data)
+    data = np.pad(data, 1)
+    labels = np.pad(labels, 1)
+    box = _acwe_box(labels)
+    img = np.ascontiguousarray(data[box])
+    lab = np.ascontiguousarray(labels[box])
+    all_labels = np.unique(labels)
+    all_labels = all_labels[all_labels >= 0]
+    if not np.any(all_labels == 0):
+        all_labels = np.concatenate([[0], all_labels])
+    cur = lab.copy()
+    for n in range(steps):
+        mean = np.zeros(256, dtype=np.float32)
+        for k in all_labels:
+            inside = cur == k
+            if np.any(inside):
+                mean[k] = np.mean(img[inside])
+        cur = orc.evolution(mean, cur, img, alpha)
+        for k in all_labels:
+            cur = curvop(cur, smooth, k)
+    return img, lab, cur, box, data.shape, all_labels
+
+
+def acwe(data, labels, alpha=1.0, smooth=1, steps=3):
+    """activeContour(data, labels, alpha, smooth, steps) with simple=False
+    (active_contour.py:112-184): the refined labels in the input's shape."""
+    _, _, cur, box, shape, _ = acwe_labels(data, labels, alpha, smooth, steps)
+    final = np.zeros(shape, np.uint8)
+    final[box] = cur
+    return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])

@@ -80,3 +80,33 @@ def test_smart_interpolation_clean_fill_keys():
     want = opipe.smart_interpolation(vol.img, vol.labels, **kw)
     for k in ("regular", "smooth", "cleaned", "smooth_cleaned", "filled", "cleaned_filled"):
         np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_acwe_bitexact(eng, case):
+    """bmx_acwe (means, evolution, per-label erosion/dilation rounds) against the
+    oracle's restatement of curvop_numba.py on random label volumes."""
+    from biomedisa_amd.post import active_contour
+    rng = np.random.default_rng(400 + case)
+    shape = tuple(int(s) for s in rng.integers(14, 40, 3))
+    labels = _blobs(rng, shape, int(rng.integers(1, 5)), float(rng.uniform(0.3, 0.8)))
+    img = (labels.astype(np.float32) * 40 + rng.normal(0, 25, shape)).clip(0, 255).astype(np.uint8)
+    if case == 3:
+        img = img.astype(np.float32) * 3.7   # float input: img_to_uint8 on both sides
+    alpha, smooth, steps = [(1.0, 1, 3), (0.5, 2, 2), (2.0, 1, 1), (1.0, 3, 2)][case]
+    _say("acwe case %d shape %s alpha %g smooth %d steps %d" % (case, shape, alpha, smooth, steps))
+    got = active_contour(img, labels, alpha, smooth, steps)
+    want = post.acwe(img, labels, alpha, smooth, steps)
+    assert (want != labels).any()
+    np.testing.assert_array_equal(got, want)
+
+
+def test_smart_interpolation_acwe_key():
+    from biomedisa_amd.interpolation import smart_interpolation
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(30, 36, 40), seed=14, n_struct=2, radius=(4, 9), every=4, allaxis=False)
+    kw = dict(nbrw=3, sorw=120, acwe=True, acwe_steps=2)
+    got = smart_interpolation(vol.img, vol.labels, **kw)
+    want = opipe.smart_interpolation(vol.img, vol.labels, **kw)
+    np.testing.assert_array_equal(got["regular"], want["regular"])
+    np.testing.assert_array_equal(got["acwe"], want["acwe"])

@@ -109,3 +109,79 @@ def test_cleaned_filled_wraps_in_uint8():
     cleaned = np.array([5, 0, 0, 7], np.uint8)
     filled = np.array([5, 9, 9, 7], np.uint8)
     np.testing.assert_array_equal(post.cleaned_filled(final, cleaned, filled), [5, 4, 9, 7])
+
+
+# --- active contour (curvop_numba.py) -------------------------------------
+
+def test_evolution_takes_the_cheapest_neighbour_label():
+    A = np.ones((5, 5, 5), np.uint8)
+    data = np.full(A.shape, 100, np.uint8)
+    mean = np.zeros(256, np.float32)
+    mean[1], mean[2], mean[3] = 100.0, 12.0, 8.0
+    data[2, 2, 2] = 10
+    A[1, 2, 2] = 2                     # cost |12 - 10| = 2
+    A[3, 3, 3] = 3                     # cost |8 - 10| = 2: a tie, later in order
+    B = orc.evolution(mean, A, data, 1.0)
+    assert B[2, 2, 2] == 2             # the first strict minimum in (dz, dy, dx) order
+    A[1, 2, 2] = 0                     # label 0 costs alpha * |mean[0] - 10| = alpha * 10
+    B = orc.evolution(mean, A, data, 1.0)
+    assert B[2, 2, 2] == 3
+    B = orc.evolution(mean, A, data, 0.1)   # 0.1 * 10 = 1 < 2
+    assert B[2, 2, 2] == 0
+    assert (B[0] == A[0]).all() and (B[:, 0] == A[:, 0]).all()   # only interior voxels change
+
+
+def test_erosion_of_a_cube_leaves_a_cross_and_dilation_grows_a_voxel():
+    """A 3^3 cube: only voxels with a whole planar 3x3 pattern of the label keep
+    it - the centre and the six face centres; corners and edges take the other
+    label.  Dilation turns the 26-neighbourhood of a voxel into the label."""
+    start = np.zeros((7, 7, 7), np.uint8)
+    start[2:5, 2:5, 2:5] = 1
+    final = start.copy()
+    orc.erosion(start, final, 1)
+    want = np.zeros_like(start)
+    want[3, 3, 3] = 1
+    for d in range(3):
+        for s in (-1, 1):
+            c = [3, 3, 3]
+            c[d] += s
+            want[tuple(c)] = 1
+    np.testing.assert_array_equal(final, want)
+    one = np.zeros((5, 5, 5), np.uint8)
+    one[2, 2, 2] = 4
+    out = one.copy()
+    orc.dilation(one, out, 4)
+    want = np.zeros_like(one)
+    want[1:4, 1:4, 1:4] = 4
+    np.testing.assert_array_equal(out, want)
+
+
+def test_curvop_rounds_alternate():
+    """Even rounds erode then dilate, odd rounds dilate then erode: one round
+    removes a lone voxel, two rounds of a 3^3 cube end where erosion of its
+    opening leaves it."""
+    v = np.zeros((9, 9, 9), np.uint8)
+    v[4, 4, 4] = 1
+    np.testing.assert_array_equal(post.curvop(v, 1, 1), np.zeros_like(v))
+    cube = np.zeros((9, 9, 9), np.uint8)
+    cube[3:6, 3:6, 3:6] = 2
+    r1 = post.curvop(cube, 1, 2)
+    e = cube.copy()
+    orc.erosion(cube, e, 2)
+    d = e.copy()
+    orc.dilation(e, d, 2)
+    np.testing.assert_array_equal(r1, d)
+    r2 = post.curvop(cube, 2, 2)
+    d2 = d.copy()
+    orc.dilation(d, d2, 2)
+    e2 = d2.copy()
+    orc.erosion(d2, e2, 2)
+    np.testing.assert_array_equal(r2, e2)
+
+
+def test_acwe_means_are_float32_of_the_float64_mean():
+    data = np.array([[[1, 2], [2, 2]]], np.uint8)
+    labels = np.array([[[1, 1], [1, 0]]], np.uint8)
+    img, lab, cur, box, shape, all_labels = post.acwe_labels(data, labels, 1.0, 0, 1)
+    assert list(all_labels) == [0, 1]
+    assert np.float32(np.mean(np.array([1, 2, 2], np.uint8))) == np.float32(5.0 / 3.0)
