@@ -54,6 +54,9 @@ SIGNATURES = {
     "bmx_clean": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
     "bmx_fill": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
     "bmx_acwe": (_i, [_p, _p, _p, _i64, _i64, _i64, _p, _i, C.c_double, _i, _i, _p, _p]),
+    "bmx_refine_distance": (_i, [_p, _p, _i64, _i64, _i64, _p, _p]),
+    "bmx_refine_stats": (_i, [_p, _p, _p, _p, _i64, _i64, _i64, _p]),
+    "bmx_refine_apply": (_i, [_p, _p, _p, _i64, _i64, _i64, _p, _p, _p]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),

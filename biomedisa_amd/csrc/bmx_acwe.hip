@@ -179,7 +179,157 @@ int launch_morph(bmx_ctx *ctx, bool erosion, const uint8_t *in, uint8_t *out, co
     return BMX_OK;
 }
 
+// --- refinement (active_contour.py:193-225) --------------------------------
+//
+// geodis (active_contour.py:42-89): one forward and one backward raster scan of
+// a chamfer distance with weights 1, sqrt2, sqrt3 over float32 values, each
+// voxel's new value min(neighbour + weight, own) computed in float64 and stored
+// as float32.  A scan reads voxels it has already updated, so it is sequential;
+// in the forward scan voxel (z, y, x) reads only voxels of smaller
+// T = 4z + 2y + x, so all voxels of one T can be updated together and the
+// wavefronts T = 0, 1, ... reproduce the scan bit for bit (the backward scan
+// likewise with T taken from the far corner).
+constexpr double kSqrt2 = 1.4142135623730951, kSqrt3 = 1.7320508075688772;   // np.sqrt(2), np.sqrt(3)
+
+__global__ __launch_bounds__(256) void k_geo_init(const uint8_t *__restrict__ lab, int64_t V, float *__restrict__ c)
+{
+    for (int64_t i = bmx_gtid(); i < V; i += bmx_gstride()) c[i] = lab[i] == 0 ? 100.0f : 0.0f;
+}
+
+template <bool FORWARD>
+__global__ __launch_bounds__(256) void k_geo_front(float *__restrict__ c, Geo g, int T)
+{
+    const int64_t n = (int64_t)g.Z * g.Y;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int z = (int)(j / g.Y), y = (int)(j - (int64_t)z * g.Y);
+    int x;
+    if (FORWARD) {
+        // z in [1, Z-1], y in [1, Y-2], x in [1, X-2]
+        x = T - 4 * z - 2 * y;
+        if (z < 1 || y < 1 || y > g.Y - 2 || x < 1 || x > g.X - 2) return;
+    } else {
+        // z in [0, Z-2], y in [1, Y-2], x in [1, X-2]; T counted from the far corner
+        x = (g.X - 1) - (T - 4 * (g.Z - 1 - z) - 2 * (g.Y - 1 - y));
+        if (z > g.Z - 2 || y < 1 || y > g.Y - 2 || x < 1 || x > g.X - 2) return;
+    }
+    const int64_t i = (int64_t)z * g.sz + (int64_t)y * g.sy + x;
+    const int dz = FORWARD ? -1 : 1;
+    const int64_t pz = i + dz * g.sz;
+    double m = (double)c[i];
+    // neighbours in the plane z -/+ 1
+    for (int o = -1; o <= 1; ++o)
+        for (int p = -1; p <= 1; ++p) {
+            const int k = (o != 0) + (p != 0);
+            const double w = k == 0 ? 1.0 : (k == 1 ? kSqrt2 : kSqrt3);
+            m = fmin(m, (double)c[pz + o * g.sy + p] + w);
+        }
+    // neighbours in the same plane already scanned
+    const int s = FORWARD ? -1 : 1;
+    m = fmin(m, (double)c[i + s * g.sy - 1] + kSqrt2);
+    m = fmin(m, (double)c[i + s * g.sy] + 1.0);
+    m = fmin(m, (double)c[i + s * g.sy + 1] + kSqrt2);
+    m = fmin(m, (double)c[i + s] + 1.0);
+    c[i] = (float)m;
+}
+
+// d = 0 where the distance is at most 10, else 1 (active_contour.py:206-208).
+__global__ __launch_bounds__(256) void k_geo_threshold(const float *__restrict__ c, int64_t V, uint8_t *__restrict__ d)
+{
+    for (int64_t i = bmx_gtid(); i < V; i += bmx_gstride()) d[i] = c[i] <= 10.0f ? 0 : 1;
+}
+
+// Per z-plane, over the voxels with d == 0: count, sum and sum of squares of
+// the image per label value.  One block per plane.
+__global__ __launch_bounds__(256) void k_refine_stats(const uint8_t *__restrict__ img, const uint8_t *__restrict__ lab,
+                                                      const uint8_t *__restrict__ d, Geo g,
+                                                      unsigned long long *__restrict__ stats)
+{
+    __shared__ unsigned long long h[256 * 3];
+    for (int k = threadIdx.x; k < 256 * 3; k += blockDim.x) h[k] = 0ull;
+    __syncthreads();
+    const int z = blockIdx.x;
+    const int64_t base = (int64_t)z * g.sz;
+    for (int64_t r = threadIdx.x; r < g.sz; r += blockDim.x) {
+        if (d[base + r]) continue;
+        const uint8_t l = lab[base + r];
+        const unsigned long long v = img[base + r];
+        atomicAdd(&h[3 * l + 0], 1ull);
+        atomicAdd(&h[3 * l + 1], v);
+        atomicAdd(&h[3 * l + 2], v * v);
+    }
+    __syncthreads();
+    unsigned long long *out = stats + (int64_t)z * 256 * 3;
+    for (int k = threadIdx.x; k < 256 * 3; k += blockDim.x) out[k] = h[k];
+}
+
+// out = (d == 0 and lut[z][img]) ? label[z] : 0 (label[z] = 0: none).
+__global__ __launch_bounds__(256) void k_refine_apply(const uint8_t *__restrict__ img, const uint8_t *__restrict__ d,
+                                                      Geo g, const uint8_t *__restrict__ lut,
+                                                      const uint8_t *__restrict__ label, uint8_t *__restrict__ out)
+{
+    for (int64_t i = bmx_gtid(); i < g.V; i += bmx_gstride()) {
+        const int z = (int)(i / g.sz);
+        const uint8_t l = label[z];
+        out[i] = (l != 0 && d[i] == 0 && lut[(int64_t)z * 256 + img[i]]) ? l : (uint8_t)0;
+    }
+}
+
 }  // namespace
+
+extern "C" int bmx_refine_distance(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, float *dist,
+                                   uint8_t *d)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(lab && dist && d, "bmx_refine_distance: null pointer");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < ((int64_t)1 << 32) && Z * Y < ((int64_t)1 << 31),
+              "bmx_refine_distance: bad dims");
+    Geo g{(int)Z, (int)Y, (int)X, X, X * Y, Z * Y * X};
+    k_geo_init<<<bmx_stride_blocks(g.V, 256), 256, 0, ctx->stream>>>(lab, g.V, dist);
+    BMX_LAUNCH_CHECK();
+    const unsigned blocks = bmx_blocks(Z * Y, 256);
+    const int Tmax = 4 * (int)(Z - 1) + 2 * (int)(Y - 1) + (int)(X - 1);
+    if (Y >= 3 && X >= 3) {
+        for (int T = 0; T <= Tmax; ++T) {
+            k_geo_front<true><<<blocks, 256, 0, ctx->stream>>>(dist, g, T);
+            BMX_LAUNCH_CHECK();
+        }
+        for (int T = 0; T <= Tmax; ++T) {
+            k_geo_front<false><<<blocks, 256, 0, ctx->stream>>>(dist, g, T);
+            BMX_LAUNCH_CHECK();
+        }
+    }
+    k_geo_threshold<<<bmx_stride_blocks(g.V, 256), 256, 0, ctx->stream>>>(dist, g.V, d);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+extern "C" int bmx_refine_stats(bmx_ctx *ctx, const uint8_t *img, const uint8_t *lab, const uint8_t *d, int64_t Z,
+                                int64_t Y, int64_t X, uint64_t *stats)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(img && lab && d && stats, "bmx_refine_stats: null pointer");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < ((int64_t)1 << 32), "bmx_refine_stats: bad dims");
+    Geo g{(int)Z, (int)Y, (int)X, X, X * Y, Z * Y * X};
+    k_refine_stats<<<(unsigned)Z, 256, 0, ctx->stream>>>(img, lab, d, g, (unsigned long long *)stats);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+extern "C" int bmx_refine_apply(bmx_ctx *ctx, const uint8_t *img, const uint8_t *d, int64_t Z, int64_t Y,
+                                int64_t X, const uint8_t *lut, const uint8_t *label, uint8_t *out)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(img && d && lut && label && out, "bmx_refine_apply: null pointer");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < ((int64_t)1 << 32), "bmx_refine_apply: bad dims");
+    Geo g{(int)Z, (int)Y, (int)X, X, X * Y, Z * Y * X};
+    k_refine_apply<<<bmx_stride_blocks(g.V, 256), 256, 0, ctx->stream>>>(img, d, g, lut, label, out);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
 
 extern "C" int bmx_acwe(bmx_ctx *ctx, const uint8_t *img, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
                         const uint8_t *labels, int nlabels, double alpha, int smooth, int steps, uint8_t *out,

@@ -283,12 +283,13 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     if acwe:
         # rw_small.py:244-253: the (normalised) image of the walk's box, zeros
         # elsewhere, and the regular result
-        from .post import active_contour
+        from .post import active_contour, refinement
         z0, z1, y0, y1, x0, x1 = prep.box
         data = np.zeros((bm.zsh, bm.ysh, bm.xsh), dtype=prep.img.dtype)
         data[z0:z1, y0:y1, x0:x1] = prep.img
         data = data[1:-1, 1:-1, 1:-1]
         results["acwe"] = active_contour(data, results["regular"], acwe_alpha, acwe_smooth, acwe_steps, device)
+        results["refined"] = refinement(data, results["regular"], device)
     return results
 
 

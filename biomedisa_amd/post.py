@@ -80,3 +80,72 @@ def active_contour(data, labels, alpha=1.0, smooth=1, steps=3, device=0):
     final = np.zeros(shape, np.uint8)
     final[box] = res
     return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])
+
+
+def _std_from_moments(n, s, q):
+    """Population standard deviation of n integers with sum s and sum of squares
+    q: the exact variance (n q - s^2) / n^2, rounded once, then sqrt.  np.std
+    sums rounded squared deviations instead, which can differ in the last bits."""
+    from fractions import Fraction
+    return float(np.sqrt(float(Fraction(n * q - s * s, n * n))))
+
+
+def _plane_rule(stats, all_labels):
+    """refinement's per-plane decision (active_contour.py:211-224), on the
+    per-plane moments: for each label l (in order) with voxels of l and of other
+    values near the labels (d == 0), plane k becomes
+    (d == 0 and p1 > p2) * l, p1 / p2 the Gaussian densities of the two groups
+    at the voxel's value; a later label overwrites the whole plane.  Returns
+    (lut uint8 [Z][256], label uint8 [Z])."""
+    Z = stats.shape[0]
+    lut = np.zeros((Z, 256), np.uint8)
+    plane_label = np.zeros(Z, np.uint8)
+    img = np.arange(256, dtype=np.uint8)
+    total = stats[:, :, :].sum(axis=1, dtype=np.uint64)
+    for l in all_labels[1:]:
+        for k in range(Z):
+            n1, s1_, q1 = (int(v) for v in stats[k, l])
+            n2, s2_, q2 = (int(t) - int(v) for t, v in zip(total[k], stats[k, l]))
+            if n1 == 0 or n2 == 0:
+                continue
+            m1, m2 = np.float64(s1_ / n1), np.float64(s2_ / n2)
+            s1 = max(np.float64(_std_from_moments(n1, s1_, q1)), 1)
+            s2 = max(np.float64(_std_from_moments(n2, s2_, q2)), 1)
+            p1 = np.exp(-(img - m1) ** 2 / (2 * s1 ** 2)) / np.sqrt(2 * np.pi * s1 ** 2)
+            p2 = np.exp(-(img - m2) ** 2 / (2 * s2 ** 2)) / np.sqrt(2 * np.pi * s2 ** 2)
+            lut[k] = p1 > p2
+            plane_label[k] = l
+    return lut, plane_label
+
+
+def refinement(data, labels, device=0):
+    """activeContour(data, labels, simple=True): the labels refined plane by
+    plane near their boundaries (active_contour.py:193-225).  The chamfer
+    distance, the plane moments and the final assignment run on the device; the
+    per-plane Gaussian decision is taken on the host for the 256 image values.
+    None where the reference's pre_processing rejects the labels."""
+    prep = _prepare(data, labels)
+    if prep is None:
+        return None
+    img, lab, box, shape, all_labels = prep
+    Z, Y, X = lab.shape
+    eng = _hip.engine(device)
+    ib, lb = eng.upload(img), eng.upload(lab)
+    dist, d = eng.alloc(4 * lab.size), eng.alloc(lab.size)
+    st, out = eng.alloc(Z * 256 * 3 * 8), eng.alloc(lab.size)
+    bufs = [ib, lb, dist, d, st, out]
+    try:
+        _hip.call("bmx_refine_distance", eng.ctx, lb.ptr, Z, Y, X, dist.ptr, d.ptr)
+        _hip.call("bmx_refine_stats", eng.ctx, ib.ptr, lb.ptr, d.ptr, Z, Y, X, st.ptr)
+        stats = st.download((Z, 256, 3), np.uint64)
+        lut, plane_label = _plane_rule(stats, all_labels)
+        lt, pl = eng.upload(lut), eng.upload(plane_label)
+        bufs += [lt, pl]
+        _hip.call("bmx_refine_apply", eng.ctx, ib.ptr, d.ptr, Z, Y, X, lt.ptr, pl.ptr, out.ptr)
+        res = out.download(lab.shape, np.uint8)
+    finally:
+        for b in bufs:
+            b.free()
+    final = np.zeros(shape, np.uint8)
+    final[box] = res
+    return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])

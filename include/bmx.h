@@ -244,6 +244,22 @@ int bmx_acwe(bmx_ctx *ctx, const uint8_t *img, const uint8_t *lab, int64_t Z, in
              const uint8_t *labels, int nlabels, double alpha, int smooth, int steps, uint8_t *out,
              uint8_t *tmp);
 
+/* Refinement (active_contour.py:193-225, activeContour with simple=True), in
+ * three steps around a host-side decision per plane:
+ *   bmx_refine_distance: dist (float32, Z*Y*X) = geodis of (lab == 0 ? 100 : 0),
+ *     the chamfer scans of active_contour.py:42-89 reproduced bit for bit;
+ *     d (uint8) = dist <= 10 ? 0 : 1.
+ *   bmx_refine_stats: per z-plane and label value, over the voxels with d == 0,
+ *     count, sum and sum of squares of img: stats[z][256][3] (uint64).
+ *   bmx_refine_apply: out = (d == 0 and lut[z][img]) ? label[z] : 0, with the
+ *     host's per-plane label (0 = none) and uint8 lut[z][256]. */
+int bmx_refine_distance(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, float *dist,
+                        uint8_t *d);
+int bmx_refine_stats(bmx_ctx *ctx, const uint8_t *img, const uint8_t *lab, const uint8_t *d, int64_t Z,
+                     int64_t Y, int64_t X, uint64_t *stats);
+int bmx_refine_apply(bmx_ctx *ctx, const uint8_t *img, const uint8_t *d, int64_t Z, int64_t Y, int64_t X,
+                     const uint8_t *lut, const uint8_t *label, uint8_t *out);
+
 /* The 'uncertainty' result of rw_small.py:152-176 (MAX_SRC + UNCERTAINTY_SRC,
  * opencl_kernels.py:54-133): uint8(255 * (1 - (1-m0/m2)(1-m1/m2))) from the
  * three largest hit counts, interior voxels; 0 elsewhere. */

@@ -699,6 +699,46 @@ void orc_dilation(const uint8_t *start, uint8_t *final_, int zsh, int ysh, int x
             }
 }
 
+/* geodis of active_contour.py:42-89, as written: `iterations` forward and
+ * backward raster scans in place; every candidate in double, the minimum stored
+ * as float. */
+void orc_geodis(float *c, int zsh, int ysh, int xsh, double sqrt2, double sqrt3, int iterations)
+{
+    const int64_t sy = xsh, sz = (int64_t)xsh * ysh;
+#define C3(z, y, x) ((double)c[(int64_t)(z) * sz + (int64_t)(y) * sy + (x)])
+    for (int it = 0; it < iterations; ++it) {
+        for (int z = 1; z < zsh; ++z)
+            for (int y = 1; y < ysh - 1; ++y)
+                for (int x = 1; x < xsh - 1; ++x) {
+                    double m = C3(z, y, x);
+                    const double a[13] = {C3(z - 1, y - 1, x - 1) + sqrt3, C3(z - 1, y - 1, x) + sqrt2,
+                                          C3(z - 1, y - 1, x + 1) + sqrt3, C3(z - 1, y, x - 1) + sqrt2,
+                                          C3(z - 1, y, x) + 1, C3(z - 1, y, x + 1) + sqrt2,
+                                          C3(z - 1, y + 1, x - 1) + sqrt3, C3(z - 1, y + 1, x) + sqrt2,
+                                          C3(z - 1, y + 1, x + 1) + sqrt3, C3(z, y - 1, x - 1) + sqrt2,
+                                          C3(z, y - 1, x) + 1, C3(z, y - 1, x + 1) + sqrt2, C3(z, y, x - 1) + 1};
+                    for (int k = 0; k < 13; ++k)
+                        if (a[k] < m) m = a[k];
+                    c[(int64_t)z * sz + (int64_t)y * sy + x] = (float)m;
+                }
+        for (int z = zsh - 2; z > -1; --z)
+            for (int y = ysh - 2; y > 0; --y)
+                for (int x = xsh - 2; x > 0; --x) {
+                    double m = C3(z, y, x);
+                    const double a[13] = {C3(z + 1, y - 1, x - 1) + sqrt3, C3(z + 1, y - 1, x) + sqrt2,
+                                          C3(z + 1, y - 1, x + 1) + sqrt3, C3(z + 1, y, x - 1) + sqrt2,
+                                          C3(z + 1, y, x) + 1, C3(z + 1, y, x + 1) + sqrt2,
+                                          C3(z + 1, y + 1, x - 1) + sqrt3, C3(z + 1, y + 1, x) + sqrt2,
+                                          C3(z + 1, y + 1, x + 1) + sqrt3, C3(z, y + 1, x + 1) + sqrt2,
+                                          C3(z, y + 1, x) + 1, C3(z, y + 1, x - 1) + sqrt2, C3(z, y, x + 1) + 1};
+                    for (int k = 0; k < 13; ++k)
+                        if (a[k] < m) m = a[k];
+                    c[(int64_t)z * sz + (int64_t)y * sy + x] = (float)m;
+                }
+    }
+#undef C3
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -54,6 +54,7 @@ def lib():
         L.orc_evolution.argtypes = [_p, _p, _p, C.c_int, C.c_int, C.c_int, C.c_double, _p]
         L.orc_erosion.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_dilation.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_geodis.argtypes = [_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -145,6 +146,13 @@ def erosion(start, final, label):
 
 def dilation(start, final, label):
     lib().orc_dilation(_ptr(start), _ptr(final), *start.shape, int(label))
+
+
+def geodis(c, sqrt2, sqrt3, iterations):
+    """In place on a float32 volume (active_contour.py:42-89)."""
+    assert c.dtype == np.float32 and c.flags.c_contiguous
+    lib().orc_geodis(_ptr(c), *c.shape, float(sqrt2), float(sqrt3), int(iterations))
+    return c
 
 
 def adaptive(labidx, flags, pos):

@@ -176,4 +176,5 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, allaxis=False, smoo
         data = np.zeros(p["padded_shape"], dtype=p["img"].dtype)
         data[z0:z1, y0:y1, x0:x1] = p["img"]
         res["acwe"] = post.acwe(data[1:-1, 1:-1, 1:-1], res["regular"], acwe_alpha, acwe_smooth, acwe_steps)
+        res["refined"] = post.refinement(data[1:-1, 1:-1, 1:-1], res["regular"])
     return res

@@ -148,3 +148,48 @@ def acwe(data, labels, alpha=1.0, smooth=1, steps=3):
     final = np.zeros(shape, np.uint8)
     final[box] = cur
     return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])
+
+
+def refinement_labels(img, lab, all_labels):
+    """refinement of active_contour.py:193-225 as written, on the padded,
+    cropped uint8 image and labels."""
+    zsh, ysh, xsh = img.shape
+    distance = np.zeros(img.shape, dtype=np.float32)
+    distance[lab == 0] = 100
+    distance = orc.geodis(distance, np.sqrt(2), np.sqrt(3), 1)
+    distance[distance <= 10] = 0
+    distance[distance > 10] = 1
+    distance = distance.astype(np.uint8)
+    result = np.zeros_like(lab)
+    for l in all_labels[1:]:
+        for k in range(zsh):
+            im = img[k]
+            mask = lab[k]
+            d = distance[k]
+            if np.any(np.logical_and(d == 0, mask == l)) and np.any(np.logical_and(d == 0, mask != l)):
+                m1, s1 = np.mean(im[np.logical_and(d == 0, mask == l)]), np.std(im[np.logical_and(d == 0, mask == l)])
+                m2, s2 = np.mean(im[np.logical_and(d == 0, mask != l)]), np.std(im[np.logical_and(d == 0, mask != l)])
+                s1 = max(s1, 1)
+                s2 = max(s2, 1)
+                p1 = np.exp(-(im - m1) ** 2 / (2 * s1 ** 2)) / np.sqrt(2 * np.pi * s1 ** 2)
+                p2 = np.exp(-(im - m2) ** 2 / (2 * s2 ** 2)) / np.sqrt(2 * np.pi * s2 ** 2)
+                result[k] = np.logical_and(d == 0, p1 > p2) * l
+    return result
+
+
+def refinement(data, labels):
+    """activeContour(data, labels, simple=True) (active_contour.py:112-184)."""
+    data = img_to_uint8(data)
+    data = np.pad(data, 1)
+    labels = np.pad(labels, 1)
+    box = _acwe_box(labels)
+    img = np.ascontiguousarray(data[box])
+    lab = np.ascontiguousarray(labels[box])
+    all_labels = np.unique(labels)
+    all_labels = all_labels[all_labels >= 0]
+    if not np.any(all_labels == 0):
+        all_labels = np.concatenate([[0], all_labels])
+    res = refinement_labels(img, lab, all_labels)
+    final = np.zeros(data.shape, np.uint8)
+    final[box] = res
+    return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])

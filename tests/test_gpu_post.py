@@ -110,3 +110,48 @@ def test_smart_interpolation_acwe_key():
     want = opipe.smart_interpolation(vol.img, vol.labels, **kw)
     np.testing.assert_array_equal(got["regular"], want["regular"])
     np.testing.assert_array_equal(got["acwe"], want["acwe"])
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_refine_distance_bitexact(eng, case):
+    """The wavefront order of bmx_refine_distance reproduces the sequential
+    chamfer scans bit for bit (float32 distances), rim voxels included."""
+    from oracle import orc
+    rng = np.random.default_rng(500 + case)
+    shape = tuple(int(s) for s in rng.integers(3, 36, 3))
+    lab = (rng.random(shape) < [0.02, 0.2, 0.001, 0.5][case]).astype(np.uint8) * 3
+    _say("refine distance case %d shape %s" % (case, shape))
+    lb = eng.upload(np.where(lab == 0, 0, lab).astype(np.uint8))
+    dist, d = eng.alloc(4 * lab.size), eng.alloc(lab.size)
+    _hip.call("bmx_refine_distance", eng.ctx, lb.ptr, *shape, dist.ptr, d.ptr)
+    got = dist.download(shape, np.float32)
+    want = np.zeros(shape, np.float32)
+    want[lab == 0] = 100
+    orc.geodis(want, np.sqrt(2), np.sqrt(3), 1)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    np.testing.assert_array_equal(d.download(shape, np.uint8), (want > 10).astype(np.uint8))
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_refinement_matches_oracle(eng, case):
+    from biomedisa_amd.post import refinement
+    rng = np.random.default_rng(600 + case)
+    shape = tuple(int(s) for s in rng.integers(20, 50, 3))
+    labels = _blobs(rng, shape, int(rng.integers(1, 5)), float(rng.uniform(0.2, 0.6)))
+    img = (labels.astype(np.float32) * 45 + rng.normal(0, 30, shape)).clip(0, 255).astype(np.uint8)
+    _say("refinement case %d shape %s" % (case, shape))
+    got = refinement(img, labels)
+    want = post.refinement(img, labels)
+    assert want.any()
+    np.testing.assert_array_equal(got, want)
+
+
+def test_smart_interpolation_refined_key():
+    from biomedisa_amd.interpolation import smart_interpolation
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(30, 36, 40), seed=15, n_struct=3, radius=(4, 9), every=4, allaxis=False)
+    kw = dict(nbrw=3, sorw=120, acwe=True)
+    got = smart_interpolation(vol.img, vol.labels, **kw)
+    want = opipe.smart_interpolation(vol.img, vol.labels, **kw)
+    for k in ("acwe", "refined"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)

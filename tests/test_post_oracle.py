@@ -185,3 +185,53 @@ def test_acwe_means_are_float32_of_the_float64_mean():
     img, lab, cur, box, shape, all_labels = post.acwe_labels(data, labels, 1.0, 0, 1)
     assert list(all_labels) == [0, 1]
     assert np.float32(np.mean(np.array([1, 2, 2], np.uint8))) == np.float32(5.0 / 3.0)
+
+
+# --- refinement (active_contour.py:193-225) --------------------------------
+
+def _chamfer(off):
+    a, b, c = sorted((abs(int(v)) for v in off), reverse=True)
+    return c * np.sqrt(3) + (b - c) * np.sqrt(2) + (a - b)
+
+
+def test_geodis_is_the_chamfer_distance_from_a_source():
+    """One forward and one backward scan give the 1/sqrt2/sqrt3 chamfer distance
+    (largest offset minus middle, plus sqrt2 x middle minus smallest, plus sqrt3 x
+    smallest) from a single source, to float32 rounding of the partial sums."""
+    c = np.full((9, 9, 9), 100, np.float32)
+    c[4, 4, 4] = 0
+    orc.geodis(c, np.sqrt(2), np.sqrt(3), 1)
+    for z in range(9):
+        for y in range(1, 8):
+            for x in range(1, 8):
+                assert abs(c[z, y, x] - _chamfer((z - 4, y - 4, x - 4))) < 1e-5, (z, y, x)
+    assert (c[:, 0, :] == 100).all() and (c[:, :, 8] == 100).all()   # the rim is never scanned
+
+
+def test_geodis_two_sources_take_the_nearer():
+    c = np.full((12, 12, 12), 100, np.float32)
+    c[3, 3, 3] = c[8, 9, 7] = 0
+    orc.geodis(c, np.sqrt(2), np.sqrt(3), 1)
+    for z in range(1, 11):
+        for y in range(1, 11):
+            for x in range(1, 11):
+                want = min(_chamfer((z - 3, y - 3, x - 3)), _chamfer((z - 8, y - 9, x - 7)))
+                assert abs(c[z, y, x] - want) < 1e-5
+
+
+def test_refinement_last_label_takes_the_plane():
+    """Labels 1 and 2 side by side in one plane, both with near and far pixels:
+    the plane's result is that of label 2 alone (its assignment overwrites the
+    plane); pixels far from every label (distance > 10) stay 0."""
+    img = np.zeros((3, 40, 40), np.uint8)
+    lab = np.zeros((3, 40, 40), np.uint8)
+    lab[1, 5:20, 5:15] = 1
+    lab[1, 5:20, 15:25] = 2
+    img[1] = 30
+    img[1, 5:20, 5:15] = 100
+    img[1, 5:20, 15:25] = 200
+    res = post.refinement(img, lab)
+    assert set(np.unique(res[1])) <= {0, 2}
+    assert (res[1][lab[1] == 2] == 2).all()
+    assert (res[1][lab[1] == 1] == 0).all()
+    assert (res[1, 35:, 35:] == 0).all()
