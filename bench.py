@@ -140,7 +140,7 @@ def main():
     ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["512", "batch", "large"], default="512")
+    ap.add_argument("--workload", choices=["512", "batch", "large", "post"], default="512")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch bmx_f32_to_bf16 over 64 Mi floats (reads 256 MiB, writes "
@@ -154,6 +154,8 @@ def main():
         return bench_batch(args)
     if args.workload == "large":
         return bench_large(args)
+    if args.workload == "post":
+        return bench_post(args)
 
     from biomedisa_amd import _hip
     from biomedisa_amd.helpers import per_label_dice
@@ -348,6 +350,64 @@ def bench_batch(args):
                      "walk_kernel_ms": tm["walk_kernel_ms"], "walk_phase_ms": tm["walk_phase_ms"]},
     }
     hits.free()
+    print(json.dumps(line), flush=True)
+
+
+def bench_post(args):
+    """Post-processing of the 512^3 plain result: clean (0.1), fill (0.9), acwe
+    (alpha 1, smooth 1, 3 steps) and refined on the device, each timed once after
+    a warm-up; the oracle's CPU restatements timed on a 256^3 corner of the same
+    volumes."""
+    from biomedisa_amd import _hip
+    from biomedisa_amd import post as P
+    from biomedisa_amd.interpolation import DeviceVolume, run_pipeline, post_process
+    vol, img_c, labidx, values, nol = prepare(args)
+    eng = _hip.engine(0)
+    dv = DeviceVolume(eng, img_c, labidx)
+    out = run_pipeline(dv, nol, values, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0)
+    regular = out["regular"].download(dv.dims, np.uint8)[1:-1, 1:-1, 1:-1]
+    data = img_c[1:-1, 1:-1, 1:-1]
+    log("plain result %s" % (regular.shape,))
+    steps = {
+        "clean": lambda: post_process({"regular": regular}, clean=0.1)["cleaned"],
+        "fill": lambda: post_process({"regular": regular}, fill=0.9)["filled"],
+        "acwe": lambda: P.active_contour(data, regular, 1.0, 1, 3),
+        "refined": lambda: P.refinement(data, regular),
+    }
+    gpu = {}
+    for name, f in steps.items():
+        f()
+        eng.sync()
+        t = time.perf_counter()
+        r = f()
+        eng.sync()
+        gpu[name] = time.perf_counter() - t
+        log("%s on the GPU: %.3fs (%d voxels changed)" % (name, gpu[name], int((r != regular).sum())))
+    import oracle.post as OP
+    crop = (slice(0, 256),) * 3
+    rc, dc = np.ascontiguousarray(regular[crop]), np.ascontiguousarray(data[crop])
+    cpu_steps = {"clean": lambda: OP.clean(rc, 0.1), "fill": lambda: OP.fill(rc, 0.9),
+                 "acwe": lambda: OP.acwe(dc, rc, 1.0, 1, 3), "refined": lambda: OP.refinement(dc, rc)}
+    cpu = {}
+    for name, f in cpu_steps.items():
+        t = time.perf_counter()
+        f()
+        cpu[name] = time.perf_counter() - t
+        log("%s, oracle on 256^3: %.2fs" % (name, cpu[name]))
+    n_vox = int(np.prod(regular.shape))
+    line = {"metric": "post-processing voxels/s at 512^3 (clean, fill, acwe, refined)",
+            "value": n_vox / sum(gpu.values()), "unit": "voxels/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": 1000.0 * sum(gpu.values()), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "512^3 plain result of the benchmark volume", "gpu_seconds": gpu,
+                       "oracle_seconds_256cube": cpu,
+                       "oracle_voxels_per_s": {k: 256 ** 3 / v for k, v in cpu.items()},
+                       "gpu_voxels_per_s": {k: n_vox / v for k, v in gpu.items()}}}
+    from oracle import orc
+    line["cpu_baseline"] = {"value": 256 ** 3 / sum(cpu.values()), "unit": "voxels/s",
+                            "cores": int(orc.lib().orc_num_threads()), "kind": "port",
+                            "sample": "the oracle's C/NumPy restatements of clean, fill, acwe and refined "
+                                      "on the 256^3 corner of the same volumes"}
     print(json.dumps(line), flush=True)
 
 

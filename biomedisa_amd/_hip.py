@@ -57,6 +57,7 @@ SIGNATURES = {
     "bmx_refine_distance": (_i, [_p, _p, _i64, _i64, _i64, _p, _p]),
     "bmx_refine_stats": (_i, [_p, _p, _p, _p, _i64, _i64, _i64, _p]),
     "bmx_refine_apply": (_i, [_p, _p, _p, _i64, _i64, _i64, _p, _p, _p]),
+    "bmx_histogram": (_i, [_p, _p, _i64, _p]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),
@@ -312,6 +313,12 @@ class Engine:
         """Hole filling of a uint8 label-value volume (device) into out."""
         Z, Y, X = dims
         _check(load().bmx_fill(self.ctx, vol.ptr, Z, Y, X, float(threshold), out.ptr))
+
+    def histogram(self, vol, n):
+        """Counts of the 256 values of a uint8 device buffer."""
+        counts = np.zeros(256, np.uint64)
+        _check(load().bmx_histogram(self.ctx, vol.ptr, int(n), counts.ctypes.data))
+        return counts
 
     def acwe(self, img, lab, dims, labels, alpha, smooth, steps, out, tmp):
         """Active-contour refinement of the label volume lab (device) into out."""

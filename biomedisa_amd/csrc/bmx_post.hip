@@ -60,11 +60,27 @@ __device__ __forceinline__ uint32_t find_root(const uint32_t *par, uint32_t i)
     return i;
 }
 
+// find_root with path halving: each visited voxel's parent is lowered to its
+// grandparent with atomicMin.  A grandparent is an ancestor, parents only
+// decrease, and an ancestor stays an ancestor, so the tree keeps its root and
+// no concurrent link is undone; chains shorten as the merge proceeds.
+__device__ __forceinline__ uint32_t find_halve(uint32_t *par, uint32_t i)
+{
+    uint32_t p = ld_parent(par, i);
+    while (p != i) {
+        const uint32_t gp = ld_parent(par, p);
+        if (gp < p) atomicMin(par + i, gp);
+        i = p;
+        p = ld_parent(par, i);
+    }
+    return i;
+}
+
 __device__ __forceinline__ void unite(uint32_t *par, uint32_t a, uint32_t b)
 {
     while (true) {
-        a = find_root(par, a);
-        b = find_root(par, b);
+        a = find_halve(par, a);
+        b = find_halve(par, b);
         if (a == b) return;
         if (a < b) {
             const uint32_t t = a;
@@ -84,17 +100,44 @@ __device__ __forceinline__ bool fg(const uint8_t *v, int64_t vi, uint8_t k)
     return INV ? v[vi] != k : v[vi] == k;
 }
 
+// One wave per x-row of the box, 64 voxels at a time: every foreground voxel
+// points at the first voxel of its run along x (the run's smallest index, a
+// root), found from the wave's ballot of non-foreground lanes below it, or
+// carried over from the previous 64 voxels when the run crosses them.
 template <bool INV>
 __global__ __launch_bounds__(256) void k_cc_init(const uint8_t *__restrict__ vol, BoxG b, uint8_t k,
                                                  uint32_t *__restrict__ par, uint32_t *__restrict__ size)
 {
-    for (int64_t i = bmx_gtid(); i < b.Vb; i += bmx_gstride()) {
-        par[i] = fg<INV>(vol, vol_index(b, i), k) ? (uint32_t)i : kNone;
-        size[i] = 0u;
+    const int lane = threadIdx.x & 63;
+    const int64_t rows = (int64_t)b.Zb * b.Yb;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t row = wave0; row < rows; row += nwaves) {   // the same row in every lane of the wave
+        const int64_t i0 = row * b.Xb;
+        const int64_t v0 = vol_index(b, i0);
+        uint32_t carry = kNone;
+        for (int x0 = 0; x0 < b.Xb; x0 += 64) {
+            const int x = x0 + lane;
+            const bool f = x < b.Xb && fg<INV>(vol, v0 + x, k);
+            const uint64_t gaps = ~__ballot(f);                    // lanes that end a run
+            const uint64_t below = gaps & ((1ull << lane) - 1ull);  // gaps below this lane
+            uint32_t start;
+            if (below) start = (uint32_t)(i0 + x0 + (63 - __clzll(below)) + 1);
+            else start = carry != kNone ? carry : (uint32_t)(i0 + x0);
+            if (x < b.Xb) {
+                par[i0 + x] = f ? start : kNone;
+                size[i0 + x] = 0u;
+            }
+            // the run through lane 63 continues into the next 64 voxels
+            const uint32_t last = __shfl(start, 63, 64);
+            carry = (gaps >> 63) & 1ull ? kNone : last;
+        }
     }
 }
 
-// Unions with the three backward neighbours (-x, -y, -z) that are foreground.
+// Unions with the backward neighbours in y and z that are foreground (x
+// neighbours are already joined by the row runs); a voxel whose -y (-z)
+// neighbour continues the same pair of runs as its -x neighbour skips it.
 __global__ __launch_bounds__(256) void k_cc_merge(BoxG b, uint32_t *par)
 {
     const int64_t plane = (int64_t)b.Yb * b.Xb;
@@ -104,19 +147,49 @@ __global__ __launch_bounds__(256) void k_cc_merge(BoxG b, uint32_t *par)
         const int64_t r = i - (int64_t)z * plane;
         const int y = (int)(r / b.Xb);
         const int x = (int)(r - (int64_t)y * b.Xb);
-        if (x > 0 && par[i - 1] != kNone) unite(par, (uint32_t)i, (uint32_t)(i - 1));
-        if (y > 0 && par[i - b.Xb] != kNone) unite(par, (uint32_t)i, (uint32_t)(i - b.Xb));
-        if (z > 0 && par[i - plane] != kNone) unite(par, (uint32_t)i, (uint32_t)(i - plane));
+        const bool left = x > 0 && par[i - 1] != kNone;
+        if (y > 0 && par[i - b.Xb] != kNone && !(left && par[i - b.Xb - 1] != kNone))
+            unite(par, (uint32_t)i, (uint32_t)(i - b.Xb));
+        if (z > 0 && par[i - plane] != kNone && !(left && par[i - plane - 1] != kNone))
+            unite(par, (uint32_t)i, (uint32_t)(i - plane));
     }
 }
 
-// Every foreground voxel points at its root; roots count their voxels.
-__global__ __launch_bounds__(256) void k_cc_flatten(BoxG b, uint32_t *par, uint32_t *size)
+// Every parent is a run start (a voxel's parent starts as its run's first
+// voxel; unions link roots, which are run starts; halving moves a parent to a
+// grandparent).  Run starts are first pointed straight at their roots, so that
+// every voxel then reaches its root in two steps.
+__global__ __launch_bounds__(256) void k_cc_compress(BoxG b, uint32_t *par)
 {
     for (int64_t i = bmx_gtid(); i < b.Vb; i += bmx_gstride()) {
         if (par[i] == kNone) continue;
-        const uint32_t r = find_root(par, (uint32_t)i);
-        atomicAdd(size + r, 1u);
+        const int x = (int)(i % b.Xb);
+        if (x > 0 && par[i - 1] != kNone) continue;   // not a run start
+        const uint32_t r = find_halve(par, (uint32_t)i);
+        atomicMin(par + i, r);
+    }
+}
+
+// Roots count their voxels.  Neighbouring voxels mostly share a root, so the
+// lanes of a wave add per distinct root, one atomic each (a single root of a
+// large component would otherwise take one atomic per voxel, serialised on one
+// address).  The loop bound is uniform per block, as the ballots need every
+// lane of the wave.
+__global__ __launch_bounds__(256) void k_cc_flatten(BoxG b, uint32_t *par, uint32_t *size)
+{
+    const int lane = threadIdx.x & 63;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < b.Vb; b0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = b0 + threadIdx.x;
+        const bool f = i < b.Vb && par[i] != kNone;
+        const uint32_t r = f ? find_root(par, (uint32_t)i) : kNone;
+        uint64_t todo = __ballot(f);
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t lr = __shfl(r, leader, 64);
+            const uint64_t grp = __ballot(f && r == lr) & todo;
+            if (lane == leader) atomicAdd(size + lr, (uint32_t)__popcll(grp));
+            todo &= ~grp;
+        }
     }
 }
 
@@ -257,6 +330,8 @@ int post_label(bmx_ctx *ctx, uint8_t *vol, int Z, int Y, int X, uint8_t k, doubl
     BMX_LAUNCH_CHECK();
     k_cc_merge<<<grid, 256, 0, ctx->stream>>>(b, pb.par);
     BMX_LAUNCH_CHECK();
+    k_cc_compress<<<grid, 256, 0, ctx->stream>>>(b, pb.par);
+    BMX_LAUNCH_CHECK();
     k_cc_flatten<<<grid, 256, 0, ctx->stream>>>(b, pb.par, pb.size);
     BMX_LAUNCH_CHECK();
     k_cc_relabel<<<grid, 256, 0, ctx->stream>>>(b, pb.par);
@@ -332,4 +407,27 @@ extern "C" int bmx_fill(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, i
                         uint8_t *out)
 {
     return post_volume<true>(ctx, in, Z, Y, X, threshold, out);
+}
+
+extern "C" int bmx_histogram(bmx_ctx *ctx, const uint8_t *vol, int64_t n, uint64_t *counts)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(vol && counts, "bmx_histogram: null pointer");
+    BMX_CHECK(n >= 0, "bmx_histogram: negative size");
+    uint32_t *d = nullptr;
+    BMX_HIP(hipMalloc((void **)&d, 256 * sizeof(uint32_t)));
+    hipError_t e = hipMemsetAsync(d, 0, 256 * sizeof(uint32_t), ctx->stream);
+    if (e == hipSuccess && n > 0) {
+        const unsigned grid = bmx_stride_blocks(n, 256);
+        k_hist256<<<grid < 1024 ? grid : 1024, 256, 0, ctx->stream>>>(vol, n, d);
+        e = hipGetLastError();
+    }
+    uint32_t h[256];
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return bmx_set_error(BMX_E_DEVICE, "bmx_histogram: %s", hipGetErrorString(e));
+    for (int k = 0; k < 256; ++k) counts[k] = h[k];
+    return BMX_OK;
 }

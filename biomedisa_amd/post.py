@@ -20,43 +20,61 @@ def img_to_uint8(img):
     return img
 
 
-def _all_labels(labels):
-    """pre_processing's label set (biomedisa_helper.py:671-694): unique values,
-    0 first."""
-    vals = np.unique(labels)
-    vals = vals[(vals >= 0) & (vals <= 255)]
-    if not np.any(vals == 0):
-        vals = np.concatenate([[0], vals])
-    return vals.astype(np.uint8)
-
-
-def _box(labels):
+def _box(mask):
     """reduce_blocksize of active_contour.py:91-110: bbox of the non-zero labels
-    grown by 100 and clamped."""
-    zsh, ysh, xsh = labels.shape
-    nz = [np.nonzero(np.any(labels != 0, axis=tuple(a for a in range(3) if a != d)))[0] for d in range(3)]
+    (mask) grown by 100 and clamped."""
+    zsh, ysh, xsh = mask.shape
+    nz = [np.nonzero(np.any(mask, axis=tuple(a for a in range(3) if a != d)))[0] for d in range(3)]
     lo = lambda v: v - 100 if v - 100 > 0 else 0
     hi = lambda v, n: v + 100 if v + 100 < n else n
     return tuple(slice(lo(int(n.min())), hi(int(n.max()), s)) for n, s in zip(nz, (zsh, ysh, xsh)))
 
 
-def _prepare(data, labels):
-    """activeContour's checks and inputs (active_contour.py:112-156): None when
-    pre_processing rejects them, else (uint8 image box, label box, box, padded
-    shape, label set)."""
-    if data.shape != labels.shape or labels.ndim != 3:
+class _Inputs:
+    """activeContour's inputs on the device (active_contour.py:112-156): the
+    padded, cropped uint8 image and labels, the box, the padded shape and the
+    label set."""
+
+    def __init__(self, eng, data, labels):
+        labels = np.asarray(labels)
+        if labels.dtype.kind == "f":
+            labels = labels.astype(np.int32)
+        img = np.pad(img_to_uint8(data), 1)
+        lab = np.pad(labels, 1)
+        self.box = _box(lab != 0)
+        self.shape = img.shape
+        img_b = np.ascontiguousarray(img[self.box])
+        lab_b = np.ascontiguousarray(lab[self.box])
+        self.dims = lab_b.shape
+        self.img = eng.upload(img_b)
+        if lab_b.dtype == np.uint8:
+            self.lab = eng.upload(lab_b)
+            counts = eng.histogram(self.lab, lab_b.size)
+            vals = np.flatnonzero(counts)
+        else:
+            vals = np.unique(lab_b)
+            vals = vals[(vals >= 0) & (vals <= 255)]
+            self.lab = eng.upload(lab_b.astype(np.uint8))
+        if not np.any(vals == 0):
+            vals = np.concatenate([[0], vals])
+        self.all_labels = vals.astype(np.uint8)
+
+    def place(self, res):
+        final = np.zeros(self.shape, np.uint8)
+        final[self.box] = res
+        return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])
+
+    def free(self):
+        self.img.free()
+        self.lab.free()
+
+
+def _inputs(eng, data, labels):
+    """None where the reference's pre_processing rejects the labels: shapes
+    differ, or no non-zero label (the label set then has fewer than two values)."""
+    if data.shape != labels.shape or labels.ndim != 3 or not np.any(labels != 0):
         return None
-    labels = np.asarray(labels)
-    if labels.dtype.kind == "f":
-        labels = labels.astype(np.int32)
-    all_labels = _all_labels(labels)
-    if len(all_labels) < 2:
-        return None
-    img = np.pad(img_to_uint8(data), 1)
-    lab = np.pad(labels, 1)
-    box = _box(lab)
-    return (np.ascontiguousarray(img[box]), np.ascontiguousarray(lab[box]).astype(np.uint8), box, img.shape,
-            all_labels)
+    return _Inputs(eng, data, labels)
 
 
 def active_contour(data, labels, alpha=1.0, smooth=1, steps=3, device=0):
@@ -64,22 +82,20 @@ def active_contour(data, labels, alpha=1.0, smooth=1, steps=3, device=0):
     acwe loop on the device (bmx_acwe).  Returns the uint8 labels in the
     input's shape, or None where the reference's pre_processing rejects the
     labels (fewer than two values)."""
-    prep = _prepare(data, labels)
-    if prep is None:
-        return None
-    img, lab, box, shape, all_labels = prep
     eng = _hip.engine(device)
-    ib, lb = eng.upload(img), eng.upload(lab)
-    out, tmp = eng.alloc(lab.size), eng.alloc(lab.size)
+    inp = _inputs(eng, data, labels)
+    if inp is None:
+        return None
+    n = int(np.prod(inp.dims))
+    out, tmp = eng.alloc(n), eng.alloc(n)
     try:
-        eng.acwe(ib, lb, lab.shape, all_labels, alpha, smooth, steps, out, tmp)
-        res = out.download(lab.shape, np.uint8)
+        eng.acwe(inp.img, inp.lab, inp.dims, inp.all_labels, alpha, smooth, steps, out, tmp)
+        res = out.download(inp.dims, np.uint8)
     finally:
-        for b in (ib, lb, out, tmp):
+        for b in (out, tmp):
             b.free()
-    final = np.zeros(shape, np.uint8)
-    final[box] = res
-    return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])
+        inp.free()
+    return inp.place(res)
 
 
 def _std_from_moments(n, s, q):
@@ -124,28 +140,26 @@ def refinement(data, labels, device=0):
     distance, the plane moments and the final assignment run on the device; the
     per-plane Gaussian decision is taken on the host for the 256 image values.
     None where the reference's pre_processing rejects the labels."""
-    prep = _prepare(data, labels)
-    if prep is None:
-        return None
-    img, lab, box, shape, all_labels = prep
-    Z, Y, X = lab.shape
     eng = _hip.engine(device)
-    ib, lb = eng.upload(img), eng.upload(lab)
-    dist, d = eng.alloc(4 * lab.size), eng.alloc(lab.size)
-    st, out = eng.alloc(Z * 256 * 3 * 8), eng.alloc(lab.size)
-    bufs = [ib, lb, dist, d, st, out]
+    inp = _inputs(eng, data, labels)
+    if inp is None:
+        return None
+    Z, Y, X = inp.dims
+    n = Z * Y * X
+    dist, d = eng.alloc(4 * n), eng.alloc(n)
+    st, out = eng.alloc(Z * 256 * 3 * 8), eng.alloc(n)
+    bufs = [dist, d, st, out]
     try:
-        _hip.call("bmx_refine_distance", eng.ctx, lb.ptr, Z, Y, X, dist.ptr, d.ptr)
-        _hip.call("bmx_refine_stats", eng.ctx, ib.ptr, lb.ptr, d.ptr, Z, Y, X, st.ptr)
+        _hip.call("bmx_refine_distance", eng.ctx, inp.lab.ptr, Z, Y, X, dist.ptr, d.ptr)
+        _hip.call("bmx_refine_stats", eng.ctx, inp.img.ptr, inp.lab.ptr, d.ptr, Z, Y, X, st.ptr)
         stats = st.download((Z, 256, 3), np.uint64)
-        lut, plane_label = _plane_rule(stats, all_labels)
+        lut, plane_label = _plane_rule(stats, inp.all_labels)
         lt, pl = eng.upload(lut), eng.upload(plane_label)
         bufs += [lt, pl]
-        _hip.call("bmx_refine_apply", eng.ctx, ib.ptr, d.ptr, Z, Y, X, lt.ptr, pl.ptr, out.ptr)
-        res = out.download(lab.shape, np.uint8)
+        _hip.call("bmx_refine_apply", eng.ctx, inp.img.ptr, d.ptr, Z, Y, X, lt.ptr, pl.ptr, out.ptr)
+        res = out.download(inp.dims, np.uint8)
     finally:
         for b in bufs:
             b.free()
-    final = np.zeros(shape, np.uint8)
-    final[box] = res
-    return np.ascontiguousarray(final[1:-1, 1:-1, 1:-1])
+        inp.free()
+    return inp.place(res)

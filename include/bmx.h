@@ -233,6 +233,10 @@ int bmx_clean(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, 
 int bmx_fill(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, double threshold,
              uint8_t *out);
 
+/* Counts of the 256 values of a uint8 device array (host uint64[256] out);
+ * below 2^32 per value.  Used for the label sets of the post-processing. */
+int bmx_histogram(bmx_ctx *ctx, const uint8_t *vol, int64_t n, uint64_t *counts);
+
 /* Active contour: replaces the acwe loop of activeContour
  * (biomedisa/features/active_contour.py:158-169) with evolution and curvop
  * (curvop_numba.py:32-148).  img (uint8) and lab (label values) are the
