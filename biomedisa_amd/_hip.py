@@ -58,6 +58,7 @@ SIGNATURES = {
     "bmx_refine_stats": (_i, [_p, _p, _p, _p, _i64, _i64, _i64, _p]),
     "bmx_refine_apply": (_i, [_p, _p, _p, _i64, _i64, _i64, _p, _p, _p]),
     "bmx_histogram": (_i, [_p, _p, _i64, _p]),
+    "bmx_denoise": (_i, [_p, _p, _i, _i64, _i64, _i64, _p]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),
@@ -313,6 +314,11 @@ class Engine:
         """Hole filling of a uint8 label-value volume (device) into out."""
         Z, Y, X = dims
         _check(load().bmx_fill(self.ctx, vol.ptr, Z, Y, X, float(threshold), out.ptr))
+
+    def denoise(self, img, dtype, dims, out):
+        """3x3x3 mean filter of a uint8/float32 device image into out."""
+        Z, Y, X = dims
+        _check(load().bmx_denoise(self.ctx, img.ptr, int(dtype), Z, Y, X, out.ptr))
 
     def histogram(self, vol, n):
         """Counts of the 256 values of a uint8 device buffer."""

@@ -431,3 +431,48 @@ extern "C" int bmx_histogram(bmx_ctx *ctx, const uint8_t *vol, int64_t n, uint64
     for (int k = 0; k < 256; ++k) counts[k] = h[k];
     return BMX_OK;
 }
+
+namespace {
+
+// smooth_img_3x3 (features/biomedisa_helper.py:309-323): every voxel becomes the mean of
+// its 3x3x3 neighbourhood clipped to the volume, summed in (dz, dy, dx) order
+// in double and stored with the image's type (uint8: truncated).
+template <typename T>
+__global__ __launch_bounds__(256) void k_denoise(const T *__restrict__ in, int Z, int Y, int X, T *__restrict__ out)
+{
+    const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
+    for (int64_t i = bmx_gtid(); i < V; i += bmx_gstride()) {
+        const int z = (int)(i / sz);
+        const int64_t r = i - (int64_t)z * sz;
+        const int y = (int)(r / sy), x = (int)(r - (int64_t)y * sy);
+        double tmp = 0.0;
+        int n = 0;
+        for (int k = -1; k < 2; ++k)
+            for (int l = -1; l < 2; ++l)
+                for (int m = -1; m < 2; ++m)
+                    if (z + k >= 0 && z + k < Z && y + l >= 0 && y + l < Y && x + m >= 0 && x + m < X) {
+                        tmp += (double)in[i + k * sz + l * sy + m];
+                        ++n;
+                    }
+        out[i] = (T)(tmp / (double)n);
+    }
+}
+
+}  // namespace
+
+extern "C" int bmx_denoise(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X, void *out)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(img && out && img != out, "bmx_denoise: null or aliased pointers");
+    BMX_CHECK(img_dtype == BMX_IMG_U8 || img_dtype == BMX_IMG_F32, "bmx_denoise: bad image dtype");
+    BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < ((int64_t)1 << 32), "bmx_denoise: bad dims");
+    const unsigned grid = bmx_stride_blocks(Z * Y * X, 256);
+    if (img_dtype == BMX_IMG_U8)
+        k_denoise<uint8_t><<<grid, 256, 0, ctx->stream>>>((const uint8_t *)img, (int)Z, (int)Y, (int)X,
+                                                          (uint8_t *)out);
+    else
+        k_denoise<float><<<grid, 256, 0, ctx->stream>>>((const float *)img, (int)Z, (int)Y, (int)X, (float *)out);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}

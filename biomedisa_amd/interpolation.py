@@ -244,11 +244,27 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     by its size rule (interpolation.py:192-196), 'small' or 'large' forces one;
     see run_pipeline.  The guided method always walks as the small driver.
     """
+    files = None
     if data is None or labelData is None:
-        raise NotImplementedError("file loading (path_to_data/path_to_labels) is not part of this "
-                                  "engine: pass NumPy arrays")
-    if denoise:
-        raise NotImplementedError("denoise pre-processing is not implemented")
+        # pre_processing (features/biomedisa_helper.py:639-655): read what was not
+        # passed; result files are written only when the image came from a file
+        # (interpolation.py:76-79)
+        from . import io as bio
+        final_image_type = ".tif"
+        image_from_file = data is None
+        if data is None:
+            data = bio.load_data(path_to_data)[0] if path_to_data is not None else None
+            if data is None:
+                print("Error: Invalid image data.")
+                return None
+        if labelData is None:
+            if path_to_labels is not None:
+                labelData, final_image_type = bio.load_data(path_to_labels)
+            if labelData is None:
+                print("Error: Invalid label data.")
+                return None
+        if image_from_file:
+            files = bio.result_paths(path_to_data, final_image_type)
     if rng not in _RNG:
         raise ValueError("rng must be 'philox' or 'reference'")
     if method not in ("plain", "guided"):
@@ -259,7 +275,7 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         raise ValueError("guided mode runs the Philox walker and returns 'regular'/'smooth' only")
 
     prep = _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
-                    return_hits, slicer)
+                    return_hits, slicer, denoise, device)
     if prep is None:
         return None
     bm = prep.bm
@@ -290,6 +306,12 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         data = data[1:-1, 1:-1, 1:-1]
         results["acwe"] = active_contour(data, results["regular"], acwe_alpha, acwe_smooth, acwe_steps, device)
         results["refined"] = refinement(data, results["regular"], device)
+    if files is not None:
+        # the result files of rw_small.py:212-253, named as interpolation.py:115-133
+        from . import io as bio
+        for key, path in files.items():
+            if results.get(key) is not None:
+                bio.save_data(path, results[key], compression)
     return results
 
 
@@ -331,8 +353,25 @@ class _Prepared:
         self.bm, self.img, self.lab_c, self.labidx, self.values, self.box = bm, img, lab, labidx, values, box
 
 
+def denoise_image(img, device=0):
+    """smooth_img_3x3 (features/biomedisa_helper.py:309-323) on the device
+    (bmx_denoise): the mean of each voxel's 3x3x3 neighbourhood clipped to the
+    volume, in the image's type (uint8 or float32, as normalize_image leaves it)."""
+    img = np.ascontiguousarray(img)
+    if img.dtype not in (np.uint8, np.float32):
+        raise TypeError("denoise_image takes uint8 or float32 images")
+    eng = _hip.engine(device)
+    src = eng.upload(img)
+    out = eng.alloc(img.nbytes)
+    eng.denoise(src, _hip.IMG_U8 if img.dtype == np.uint8 else _hip.IMG_F32, img.shape, out)
+    res = out.download(img.shape, img.dtype)
+    src.free()
+    out.free()
+    return res
+
+
 def _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
-             return_hits, slicer):
+             return_hits, slicer, denoise=False, device=0):
     """Checks and pre-processing of smart_interpolation (interpolation.py:77-204);
     None (after printing the error) when the input is rejected."""
     bm = Biomedisa()
@@ -352,6 +391,9 @@ def _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore
     if bm.labelData.dtype in ["uint32", "int64", "uint64"]:
         print(f"Warning: Potential label loss during conversion from {bm.labelData.dtype} to int32.")
         bm.labelData = bm.labelData.astype(np.int32)
+    if denoise:
+        # interpolation.py:149-151
+        bm.data = denoise_image(bm.data, device)
 
     # 1-voxel pad (:157-159) and label bounding box +-100 (:192, :203-204)
     img = np.pad(bm.data, 1, mode="constant", constant_values=0)

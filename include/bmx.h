@@ -233,6 +233,11 @@ int bmx_clean(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, 
 int bmx_fill(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, double threshold,
              uint8_t *out);
 
+/* Image denoising: replaces smooth_img_3x3 (features/biomedisa_helper.py:309-323), the
+ * mean of each voxel's 3x3x3 neighbourhood clipped to the volume (summed in
+ * double, stored in the image's type; uint8 truncates).  out != img. */
+int bmx_denoise(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X, void *out);
+
 /* Counts of the 256 values of a uint8 device array (host uint64[256] out);
  * below 2^32 per value.  Used for the label sets of the post-processing. */
 int bmx_histogram(bmx_ctx *ctx, const uint8_t *vol, int64_t n, uint64_t *counts);

@@ -739,6 +739,33 @@ void orc_geodis(float *c, int zsh, int ysh, int xsh, double sqrt2, double sqrt3,
 #undef C3
 }
 
+/* smooth_img_3x3 of features/biomedisa_helper.py:309-323, as written; dtype 0 = uint8
+ * (the mean truncated into uint8), 1 = float32. */
+void orc_smooth_img(const void *img, int dtype, int zsh, int ysh, int xsh, void *out)
+{
+    const int64_t sy = xsh, sz = (int64_t)xsh * ysh;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < zsh; ++z)
+        for (int y = 0; y < ysh; ++y)
+            for (int x = 0; x < xsh; ++x) {
+                double tmp = 0;
+                int i = 0;
+                for (int k = -1; k < 2; ++k)
+                    for (int l = -1; l < 2; ++l)
+                        for (int m = -1; m < 2; ++m)
+                            if (0 <= z + k && z + k < zsh && 0 <= y + l && y + l < ysh && 0 <= x + m && x + m < xsh) {
+                                const int64_t q = (z + k) * sz + (y + l) * sy + x + m;
+                                tmp += dtype == 0 ? (double)((const uint8_t *)img)[q] : (double)((const float *)img)[q];
+                                i += 1;
+                            }
+                const int64_t c = z * sz + y * sy + x;
+                if (dtype == 0)
+                    ((uint8_t *)out)[c] = (uint8_t)(tmp / i);
+                else
+                    ((float *)out)[c] = (float)(tmp / i);
+            }
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -55,6 +55,7 @@ def lib():
         L.orc_erosion.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_dilation.argtypes = [_p, _p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_geodis.argtypes = [_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.orc_smooth_img.argtypes = [_p, C.c_int, C.c_int, C.c_int, C.c_int, _p]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -153,6 +154,14 @@ def geodis(c, sqrt2, sqrt3, iterations):
     assert c.dtype == np.float32 and c.flags.c_contiguous
     lib().orc_geodis(_ptr(c), *c.shape, float(sqrt2), float(sqrt3), int(iterations))
     return c
+
+
+def smooth_img(img):
+    """smooth_img_3x3 (uint8 or float32)."""
+    img = np.ascontiguousarray(img)
+    out = np.empty_like(img)
+    lib().orc_smooth_img(_ptr(img), 0 if img.dtype == np.uint8 else 1, *img.shape, _ptr(out))
+    return out
 
 
 def adaptive(labidx, flags, pos):

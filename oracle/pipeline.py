@@ -43,8 +43,10 @@ def _bbox(lab):
     return lo(amin_z), hi(amax_z, zsh), lo(amin_y), hi(amax_y, ysh), lo(amin_x), hi(amax_x, xsh)
 
 
-def prepare(data, labelData):
-    """-> dict(img, labidx, values, crop, padded_shape, nol) exactly as the walker sees them."""
+def prepare(data, labelData, denoise=False):
+    """-> dict(img, labidx, values, crop, padded_shape, nol) exactly as the walker
+    sees them; denoise applies smooth_img_3x3 after the dtype handling
+    (interpolation.py:149-151)."""
     labels = np.array(labelData)
     if labels.dtype.kind == "f" or labels.dtype in (np.uint32, np.int64, np.uint64):
         labels = labels.astype(np.int32)
@@ -58,6 +60,8 @@ def prepare(data, labelData):
         img -= np.amin(img)
         img /= np.amax(img)
         img *= 255.0
+    if denoise:
+        img = orc.smooth_img(img)
     img = np.pad(img, 1)
     labels = np.pad(labels, 1)
     z0, z1, y0, y1, x0, x1 = _bbox(labels)
@@ -129,10 +133,11 @@ def _uncrop(vol_c, p, dtype=np.uint8):
 
 def smart_interpolation(data, labelData, nbrw=10, sorw=4000, allaxis=False, smooth=0,
                         uncertainty=False, return_hits=False, rng="philox", key=PHILOX_KEY, path="small",
-                        clean=None, fill=None, acwe=False, acwe_alpha=1.0, acwe_smooth=1, acwe_steps=3):
+                        clean=None, fill=None, acwe=False, acwe_alpha=1.0, acwe_smooth=1, acwe_steps=3,
+                        denoise=False):
     """CPU oracle of the drop-in -> result dict (+ '_hits_u32', '_accepted', '_nseeds').
     path: 'small', 'large' or 'auto' (the reference's size rule)."""
-    p = prepare(data, labelData)
+    p = prepare(data, labelData, denoise)
     flags = orc.plane_flags(p["labidx"], allaxis)
     if allaxis and any(np.any(f[1:] & f[:-1]) for f in flags):
         return None

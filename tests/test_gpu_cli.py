@@ -1,0 +1,67 @@
+"""Image denoising (bmx_denoise) bit-exact against the oracle's smooth_img_3x3,
+smart_interpolation(denoise=True) against the oracle pipeline, and the command
+line end to end: files read, results written next to the image and equal to
+the in-memory call's."""
+import numpy as np
+import pytest
+
+import oracle.pipeline as opipe
+from oracle import orc
+from biomedisa_amd import _hip
+from biomedisa_amd import io as bio
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 3, 70), (33, 17, 65), (64, 64, 64)])
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_denoise_bitexact(eng, shape, dtype):
+    rng = np.random.default_rng(sum(shape))
+    img = (rng.random(shape) * 255).astype(dtype)
+    _say("denoise %s %s" % (shape, np.dtype(dtype).name))
+    src = eng.upload(img)
+    out = eng.alloc(img.nbytes)
+    eng.denoise(src, _hip.IMG_U8 if dtype == np.uint8 else _hip.IMG_F32, shape, out)
+    got = out.download(shape, dtype)
+    want = orc.smooth_img(img)
+    np.testing.assert_array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_smart_interpolation_denoise(dtype):
+    from biomedisa_amd.interpolation import smart_interpolation
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(30, 34, 38), seed=21, n_struct=3, radius=(4, 9), every=4, allaxis=False)
+    img = vol.img if dtype == np.uint8 else vol.img.astype(np.uint16) * 200
+    kw = dict(nbrw=3, sorw=120, smooth=2, denoise=True)
+    got = smart_interpolation(img, vol.labels, **kw)
+    want = opipe.smart_interpolation(img, vol.labels, **kw)
+    for k in ("regular", "smooth"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    plain = smart_interpolation(img, vol.labels, nbrw=3, sorw=120)
+    assert (plain["regular"] != got["regular"]).any()
+
+
+@pytest.mark.parametrize("label_ext", [".tif", ".nrrd"])
+def test_cli_end_to_end(tmp_path, label_ext):
+    from biomedisa_amd.__main__ import main
+    from biomedisa_amd.interpolation import smart_interpolation
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(28, 32, 36), seed=22, n_struct=3, radius=(4, 9), every=4, allaxis=False)
+    img_path = str(tmp_path / "scan.tif")
+    lab_path = str(tmp_path / ("scan_labels" + label_ext))
+    bio.save_data(img_path, vol.img)
+    bio.save_data(lab_path, vol.labels)
+    _say("cli %s" % label_ext)
+    assert main([img_path, lab_path, "--nbrw", "3", "--sorw", "120", "-s", "2", "-u", "-c", "-f"]) == 0
+    want = smart_interpolation(vol.img, vol.labels, nbrw=3, sorw=120, smooth=2, uncertainty=True,
+                               clean=0.1, fill=0.9)
+    files = bio.result_paths(img_path, label_ext)
+    assert files["regular"] == str(tmp_path / ("final.scan" + label_ext))
+    for k in ("regular", "smooth", "uncertainty", "cleaned", "smooth_cleaned", "filled", "cleaned_filled"):
+        got = bio.load_data(files[k])[0]
+        np.testing.assert_array_equal(got, want[k], err_msg=k)
