@@ -134,9 +134,15 @@ def main():
     ap.add_argument("--method", choices=["both", "plain", "guided"], default="both")
     ap.add_argument("--thr", type=float, default=0.9)
     ap.add_argument("--brad", type=int, default=3)
-    ap.add_argument("--srad", type=int, default=8)
+    ap.add_argument("--srad", type=int, default=6)
     ap.add_argument("--far", type=int, default=48)
-    ap.add_argument("--far-walks", type=int, default=2)
+    ap.add_argument("--far-walks", type=int, default=1)
+    ap.add_argument("--repair", type=int, default=2, help="repair radius of the guided method; -1 disables")
+    ap.add_argument("--repair-min", type=int, default=10)
+    ap.add_argument("--repair-far", type=int, default=24)
+    ap.add_argument("--repair-far-walks", type=int, default=-1, help="-1: as --far-walks")
+    ap.add_argument("--no-robust", action="store_true",
+                    help="skip the second guided measurement with the robust preset")
     ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -184,12 +190,21 @@ def main():
             return run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw,
                                        allaxis=True, smooth=args.smooth, thr=args.thr,
                                        brad=args.brad, srad=args.srad, far=args.far,
-                                       far_walks=args.far_walks, timings=tm)
+                                       far_walks=args.far_walks, timings=tm, **repair_kw(args))
         out, tms, sec = time_pipeline(guided_step, eng, args.warmup, args.steps, "guided")
         res["guided"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], walked=out["walked_seeds"],
-                             ring=out["ring_seeds"],
-                             accepted=out["accepted"],
+                             ring=out["ring_seeds"], repair=out["repair_seeds"], repair_voxels=out["repair_voxels"],
+                             repair_ring=out["repair_ring_seeds"], accepted=out["accepted"],
                              regular=out["regular"].download(dv.dims, np.uint8))
+        if not args.no_robust:
+            from biomedisa_amd.interpolation import GUIDED_ROBUST
+
+            def robust_step(tm):
+                return run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw, allaxis=True,
+                                           smooth=args.smooth, thr=args.thr, brad=args.brad, far=args.far,
+                                           repair_min=args.repair_min, timings=tm, **GUIDED_ROBUST)
+            out, tms, sec = time_pipeline(robust_step, eng, args.warmup, args.steps, "guided (robust preset)")
+            res["robust"] = dict(sec=sec, regular=out["regular"].download(dv.dims, np.uint8))
 
     if args.calibrate_copy:
         n_cal = 64 << 20
@@ -210,6 +225,9 @@ def main():
         g = res["guided"]
         config.update(guided_seconds=g["sec"], walked_seed_fraction=g["walked"] / g["nseeds"],
                       ring_seed_fraction=g["ring"] / g["nseeds"], far=args.far, far_walks=args.far_walks,
+                      repair=args.repair, repair_min=args.repair_min, repair_far=args.repair_far,
+                      repair_far_walks=args.repair_far_walks if args.repair_far_walks >= 0 else args.far_walks,
+                      repair_seed_fraction=g["repair"] / g["nseeds"], repair_voxels=g["repair_voxels"],
                       temperature=net.T, thr=args.thr, brad=args.brad, srad=args.srad,
                       guided_timings=g["tms"][-1])
         config["min_dice_vs_ground_truth_guided"] = min(per_label_dice(g["regular"][inner], vol.gt).values())
@@ -223,6 +241,12 @@ def main():
         config.update(speedup_vs_plain=res["plain"]["sec"] / res["guided"]["sec"],
                       min_dice_guided_vs_plain=min(d.values()),
                       dice_guided_vs_plain={str(k): round(v, 5) for k, v in d.items()})
+        if "robust" in res:
+            from biomedisa_amd.interpolation import GUIDED_ROBUST
+            dr = per_label_dice(res["robust"]["regular"], res["plain"]["regular"])
+            config["guided_robust_preset"] = dict(GUIDED_ROBUST, seconds=res["robust"]["sec"],
+                                                  speedup_vs_plain=res["plain"]["sec"] / res["robust"]["sec"],
+                                                  min_dice_vs_plain=min(dr.values()))
 
     line = {
         "metric": "segmented voxels/s at 512^3, 8 labels; surrogate-guided vs plain at Dice>=0.99",
@@ -243,7 +267,9 @@ def main():
     rk = res["plain"] if "plain" in res else res["guided"]
     tm = rk["tms"][-1]
     walked = rk.get("walked", rk["nseeds"])
-    steps_total = (walked * args.nbrw + rk.get("ring", 0) * args.far_walks) * args.sorw
+    rfw = args.repair_far_walks if args.repair_far_walks >= 0 else args.far_walks
+    steps_total = ((walked + (rk.get("repair") or 0)) * args.nbrw + (rk.get("ring") or 0) * args.far_walks
+                   + (rk.get("repair_ring") or 0) * rfw) * args.sorw
     walk_bytes = steps_total * 6 * 1 + rk["accepted"] * 8
     launches = max(int(tm["walk_launches"]), 1)
     kernel_s = tm["walk_kernel_ms"] / 1000.0
@@ -272,6 +298,13 @@ def main():
                       "full plain walk extrapolated to %.0fs; smoothing/merge not counted"
                       % (nsample, dt, rate, walk_s)}
     print(json.dumps(line), flush=True)
+
+
+def repair_kw(args):
+    """Repair options of the guided pipeline from the command line."""
+    return dict(repair=None if args.repair < 0 else args.repair, repair_min=args.repair_min,
+                repair_far=args.repair_far,
+                repair_far_walks=None if args.repair_far_walks < 0 else args.repair_far_walks)
 
 
 def bench_batch(args):
@@ -469,12 +502,12 @@ def bench_large(args):
             net = surrogate.net(0)
             out = run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0,
                                       thr=args.thr, brad=args.brad, srad=args.srad, far=args.far,
-                                      far_walks=args.far_walks, timings=tm)
+                                      far_walks=args.far_walks, timings=tm, **repair_kw(args))
         eng.sync()
         sec = time.perf_counter() - t
         log("%s: %.1fs %s" % (method, sec, {k: round(v, 3) for k, v in tm.items()}))
         res[method] = dict(sec=sec, tm=tm, nseeds=out["nseeds"], walked=out.get("walked_seeds"),
-                           ring=out.get("ring_seeds"), accepted=out["accepted"],
+                           ring=out.get("ring_seeds"), repair=out.get("repair_seeds"), accepted=out["accepted"],
                            regular=out["regular"].download(dv.dims, np.uint8))
         out["regular"].free()
         del out
@@ -504,6 +537,7 @@ def bench_large(args):
                    "labels": nol, "seeds": p["nseeds"], "plain_seconds": p["sec"], "guided_seconds": g["sec"],
                    "speedup_vs_plain": p["sec"] / g["sec"], "walked_seed_fraction": g["walked"] / g["nseeds"],
                    "ring_seed_fraction": g["ring"] / g["nseeds"],
+                   "repair_seed_fraction": (g["repair"] or 0) / g["nseeds"],
                    "min_dice_guided_vs_plain": min(d.values()),
                    "dice_guided_vs_plain": {str(k): round(v, 5) for k, v in d.items()},
                    "min_dice_vs_ground_truth_plain": min(d_gt.values()),

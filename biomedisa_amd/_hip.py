@@ -59,6 +59,9 @@ SIGNATURES = {
     "bmx_refine_apply": (_i, [_p, _p, _p, _i64, _i64, _i64, _p, _p, _p]),
     "bmx_histogram": (_i, [_p, _p, _i64, _p]),
     "bmx_denoise": (_i, [_p, _p, _i, _i64, _i64, _i64, _p]),
+    "bmx_guided_disagree": (_i, [_p, _p, _i, _i64, _p, _p, C.c_uint32, _p]),
+    "bmx_repair_masks": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
+    "bmx_repair_hits": (_i, [_p, _p, _p, _i, _i64, _p, _p]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),
     "bmx_walk": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                       C.POINTER(C.c_uint64)]),

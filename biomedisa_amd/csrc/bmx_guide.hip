@@ -153,6 +153,39 @@ __global__ void k_merge_guided(const uint32_t *__restrict__ hits, int nlabels, i
     out[v] = lv.v[bi];
 }
 
+// Repair test of the guided result: a voxel outside U whose walker hits (at
+// least minhits in all) have a first-maximum label other than the proposal's.
+__global__ void k_guided_disagree(const uint32_t *__restrict__ hits, int nlabels, int64_t V,
+                                  const uint8_t *__restrict__ pred, const uint8_t *__restrict__ U,
+                                  uint32_t minhits, uint8_t *__restrict__ D)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint8_t d = 0;
+    if (!U[v]) {
+        uint64_t tot = 0;
+        uint32_t b = 0;
+        int bi = 0;
+        for (int l = 0; l < nlabels; ++l) {
+            const uint32_t h = hits[(int64_t)l * V + v];
+            tot += h;
+            if (h > b) { b = h; bi = l; }
+        }
+        d = (tot >= minhits && tot > 0 && bi != pred[v]) ? 1 : 0;
+    }
+    D[v] = d;
+}
+
+// hits[:, v] = hits2[:, v] and U[v] = 1 on the repaired voxels U2.
+__global__ void k_repair_hits(uint32_t *__restrict__ hits, const uint32_t *__restrict__ hits2, int nlabels,
+                              int64_t V, const uint8_t *__restrict__ U2, uint8_t *__restrict__ U)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V || !U2[v]) return;
+    for (int l = 0; l < nlabels; ++l) hits[(int64_t)l * V + v] = hits2[(int64_t)l * V + v];
+    U[v] = 1;
+}
+
 // out = U ? a : values[pred]  (a already holds label values)
 __global__ void k_select_labels(const uint8_t *__restrict__ a, const uint8_t *__restrict__ pred,
                                 const uint8_t *__restrict__ U, LabelValues lv, int64_t V,
@@ -301,6 +334,49 @@ int bmx_ring_mask(bmx_ctx *ctx, const uint8_t *need, int Z, int Y, int X, int r,
     rc = dilate(ctx, ring, tmp, Z, Y, X, r);
     if (rc) return rc;
     k_andnot<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(ring, need, V);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_guided_disagree(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const uint8_t *pred,
+                        const uint8_t *U, uint32_t minhits, uint8_t *D)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(hits && pred && U && D && nlabels >= 1 && nlabels <= 255, "bmx_guided_disagree: bad arguments");
+    BMX_CHECK(V >= 0 && V < ((int64_t)1 << 32) - 256, "bmx_guided_disagree: V out of range");
+    if (V == 0) return BMX_OK;
+    k_guided_disagree<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(hits, nlabels, V, pred, U, minhits, D);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+int bmx_repair_masks(bmx_ctx *ctx, uint8_t *D, const uint8_t *U, int Z, int Y, int X, int r, int srad,
+                     uint8_t *need, uint8_t *tmp)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(D && U && need && tmp, "bmx_repair_masks: null pointer");
+    BMX_CHECK(r >= 0 && srad >= 0, "bmx_repair_masks: negative radius");
+    const int64_t V = (int64_t)Z * Y * X;
+    BMX_CHECK(V < ((int64_t)1 << 32) - 256, "bmx_repair_masks: volume exceeds one dispatch");
+    rc = dilate(ctx, D, tmp, Z, Y, X, r);
+    if (rc) return rc;
+    k_andnot<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(D, U, V);
+    BMX_LAUNCH_CHECK();
+    BMX_HIP(hipMemcpyAsync(need, D, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
+    return dilate(ctx, need, tmp, Z, Y, X, srad);
+}
+
+int bmx_repair_hits(bmx_ctx *ctx, uint32_t *hits, const uint32_t *hits2, int nlabels, int64_t V,
+                    const uint8_t *U2, uint8_t *U)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(hits && hits2 && U2 && U && nlabels >= 1 && nlabels <= 255, "bmx_repair_hits: bad arguments");
+    BMX_CHECK(V >= 0 && V < ((int64_t)1 << 32) - 256, "bmx_repair_hits: V out of range");
+    if (V == 0) return BMX_OK;
+    k_repair_hits<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(hits, hits2, nlabels, V, U2, U);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }

@@ -29,6 +29,9 @@ DEFAULT_KEY = 0x5EED0B10D1CE0001
 # The guided walk's sparser, ring-weighted seed set runs best with 128 steps
 # per walker per round (the plain walk with the library's 256).
 GUIDED_WALK_OPTS = {"segment_steps": 128}
+# Guided parameters that trade speed for agreement with the plain walker:
+# wider near set, two ring walks, a wider repair ring (pass as `guide=`).
+GUIDED_ROBUST = {"srad": 8, "far_walks": 2, "repair": 2, "repair_far_walks": 1, "repair_far": 32}
 _RNG = {"philox": _hip.RNG_PHILOX, "reference": _hip.RNG_REF}
 
 
@@ -145,8 +148,9 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=8, far=48, far_walks=2, walk_opts=None,
-                        timings=None, keep=False):
+                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=6, far=48, far_walks=1, repair=2, repair_min=10,
+                        repair_far_walks=None, repair_far=24, walk_opts=None, repair_walk_opts=None, timings=None,
+                        keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
 
     The network proposes a label and a confidence for every voxel.  U = voxels
@@ -158,9 +162,19 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     nbrw/far_walks times: an unbiased estimate of what they add to U
     (far=0 disables the ring).  Near and ring seeds walk in one call (per-seed
     walk counts and weights), so ring walkers fill the same tile buckets.
+    Repair (repair = a radius; None disables it): outside U, a voxel whose hits
+    from this walk number at least `repair_min` and whose walker argmax is not the
+    proposed label marks a confident network error.  Those voxels dilated by
+    `repair` (minus U) form U2; every seed within `srad` of U2 walks all its
+    walks again into a second hit volume, whose counts replace the first walk's
+    on U2, and U2 joins U.  The repair walk has its own ring out to
+    `repair_far` (None: far), whose seeds walk `repair_far_walks` walks (None:
+    far_walks); repair_walk_opts overrides walk_opts for it.  GUIDED_ROBUST
+    holds slower, more conservative settings.
     walk_opts: as for run_pipeline.  The result takes the walker's argmax on U and
     the network's label elsewhere ('smooth': the smoothed walker argmax on U).
-    Returns the dict of run_pipeline plus 'walked_seeds' and 'ring_seeds'.
+    Returns the dict of run_pipeline plus 'walked_seeds', 'ring_seeds',
+    'repair_seeds' and 'repair_voxels'.
     """
     from . import surrogate as S
     eng = vol.eng
@@ -183,16 +197,23 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     U, need, tmp = eng.alloc(V), eng.alloc(V), eng.alloc(V)
     _hip.call("bmx_guided_masks", eng.ctx, pred.ptr, unc.ptr, Z, Y, X, int(brad), int(srad), U.ptr, need.ptr,
               tmp.ptr)
-    sub = eng.alloc(max(1, nseeds) * _hip.SEED_DTYPE.itemsize)
-    _, nsub = eng.select_seeds(seeds, nseeds, need, out=sub)
-    nring = 0
-    if far > srad:
-        # ring seeds follow the near seeds in the same list (the two sets are disjoint)
-        ring = eng.alloc(V)
-        _hip.call("bmx_ring_mask", eng.ctx, need.ptr, Z, Y, X, int(far - srad), ring.ptr, tmp.ptr)
-        tail = _hip.BufferView(sub, nsub * _hip.SEED_DTYPE.itemsize)
-        _, nring = eng.select_seeds(seeds, nseeds, ring, out=tail)
-        _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, nring, int(far_walks), nbrw // far_walks)
+    ring = eng.alloc(V) if far > srad else None
+
+    def walkers(near, far_walks, far=far):
+        """Seeds in `near` with all walks, then the ring seeds out to `far` with
+        far_walks weighted walks -> (list, near count, ring count)."""
+        sub = eng.alloc(max(1, nseeds) * _hip.SEED_DTYPE.itemsize)
+        _, nsub = eng.select_seeds(seeds, nseeds, near, out=sub)
+        nring = 0
+        if ring is not None and far > srad:
+            # ring seeds follow the near seeds in the same list (the two sets are disjoint)
+            _hip.call("bmx_ring_mask", eng.ctx, near.ptr, Z, Y, X, int(far - srad), ring.ptr, tmp.ptr)
+            tail = _hip.BufferView(sub, nsub * _hip.SEED_DTYPE.itemsize)
+            _, nring = eng.select_seeds(seeds, nseeds, ring, out=tail)
+            _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, nring, int(far_walks), nbrw // far_walks)
+        return sub, nsub, nring
+
+    sub, nsub, nring = walkers(need, far_walks)
     eng.sync()
     t2 = time.perf_counter()
     hits = eng.alloc(nlabels * V * 4)
@@ -202,7 +223,39 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
                         _hip.RNG_PHILOX, key, hits, seed_overrides=nring > 0, **wopts)
     phase_ms, kernel_ms, launches = eng.last_walk_stats()
     t3 = time.perf_counter()
-    out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "accepted": accepted, "flags": flags}
+    nrep, nrep_ring, rep_vox = 0, 0, 0
+    if repair is not None:
+        U2 = eng.alloc(V)
+        _hip.call("bmx_guided_disagree", eng.ctx, hits.ptr, int(nlabels), V, pred.ptr, U.ptr, int(repair_min),
+                  U2.ptr)
+        need2 = eng.alloc(V)
+        _hip.call("bmx_repair_masks", eng.ctx, U2.ptr, U.ptr, Z, Y, X, int(repair), int(srad), need2.ptr, tmp.ptr)
+        rfw = far_walks if repair_far_walks is None else repair_far_walks
+        if far > srad and (rfw < 1 or nbrw % rfw):
+            raise PipelineError("repair_far_walks must divide nbrw")
+        sub2, nrep, nrep_ring = walkers(need2, rfw, far if repair_far is None else repair_far)
+        need2.free()
+        t["repair_masks_s"] = time.perf_counter() - t3
+        if nrep:
+            rep_vox = int(eng.histogram(U2, V)[1])
+            hits2 = eng.alloc(nlabels * V * 4)
+            hits2.zero()
+            ropts = wopts if repair_walk_opts is None else dict(GUIDED_WALK_OPTS, **repair_walk_opts)
+            accepted += eng.walk(vol.img, vol.img_dtype, vol.dims, sub2, nrep + nrep_ring, nlabels, nbrw, sorw,
+                                 _hip.RNG_PHILOX, key, hits2, seed_overrides=nrep_ring > 0, **ropts)
+            p2, k2, l2 = eng.last_walk_stats()
+            t["repair_walk_ms"] = p2
+            phase_ms += p2
+            kernel_ms = kernel_ms + k2 if kernel_ms >= 0 and k2 >= 0 else -2.0
+            launches += l2
+            _hip.call("bmx_repair_hits", eng.ctx, hits.ptr, hits2.ptr, int(nlabels), V, U2.ptr, U.ptr)
+            eng.sync()
+            hits2.free()
+        sub2.free()
+        U2.free()
+    t3r = time.perf_counter()
+    out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "repair_seeds": nrep,
+           "repair_ring_seeds": nrep_ring, "repair_voxels": rep_vox, "accepted": accepted, "flags": flags}
     values = np.ascontiguousarray(values, dtype=np.uint8)
     regular = eng.alloc(V)
     _hip.call("bmx_merge_guided", eng.ctx, hits.ptr, int(nlabels), V, values.ctypes.data, pred.ptr, U.ptr,
@@ -218,8 +271,8 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     t4 = time.perf_counter()
     if keep:
         out.update(pred=pred, U=U, need=need, hits=hits)
-    t.update(seeds_s=t1 - t0, proposal_s=t2 - t1, walk_s=t3 - t2, merge_s=t4 - t3, walk_phase_ms=phase_ms,
-             walk_kernel_ms=kernel_ms, walk_launches=launches)
+    t.update(seeds_s=t1 - t0, proposal_s=t2 - t1, walk_s=t3 - t2, repair_s=t3r - t3, merge_s=t4 - t3r,
+             walk_phase_ms=phase_ms, walk_kernel_ms=kernel_ms, walk_launches=launches)
     return out
 
 

@@ -351,6 +351,16 @@ int bmx_seed_walks(bmx_ctx *ctx, bmx_seed *seeds, int64_t n, int walks, int weig
 int bmx_ring_mask(bmx_ctx *ctx, const uint8_t *need, int Z, int Y, int X, int r, uint8_t *ring, uint8_t *tmp);
 int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,
                      const uint8_t *values, const uint8_t *pred, const uint8_t *U, uint8_t *out);
+/* Repair of the guided result (no counterpart in the reference).
+ * D = !U & (sum of hits >= minhits, and > 0) & (first argmax of hits != pred). */
+int bmx_guided_disagree(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const uint8_t *pred,
+                        const uint8_t *U, uint32_t minhits, uint8_t *D);
+/* D <- dilate(D, r) & ~U (the repaired voxels U2); need = dilate(U2, srad). */
+int bmx_repair_masks(bmx_ctx *ctx, uint8_t *D, const uint8_t *U, int Z, int Y, int X, int r, int srad,
+                     uint8_t *need, uint8_t *tmp);
+/* On U2: hits[:, v] = hits2[:, v] and U[v] = 1. */
+int bmx_repair_hits(bmx_ctx *ctx, uint32_t *hits, const uint32_t *hits2, int nlabels, int64_t V,
+                    const uint8_t *U2, uint8_t *U);
 /* out = U ? a : values[pred]  (a holds label values, e.g. a smoothed result). */
 int bmx_select_labels(bmx_ctx *ctx, const uint8_t *a, const uint8_t *pred, const uint8_t *U,
                       int nlabels, const uint8_t *values, int64_t V, uint8_t *out);

@@ -122,6 +122,29 @@ def ring_mask(need, r):
     return (dilate(n, r).astype(bool) & ~n).astype(np.uint8)
 
 
+def guided_disagree(hits, pred, U, minhits):
+    """Outside U: total hits >= minhits (and > 0) and first argmax != pred."""
+    h = np.asarray(hits, np.uint32)
+    tot = h.sum(0, dtype=np.uint64)
+    am = np.argmax(h, axis=0)
+    d = (np.asarray(U) == 0) & (tot >= max(int(minhits), 1)) & (am != np.asarray(pred))
+    return d.astype(np.uint8)
+
+
+def repair_masks(D, U, r, srad):
+    """U2 = dilate(D, r) & ~U;  need = dilate(U2, srad)."""
+    U2 = dilate(D, r).astype(bool) & (np.asarray(U) == 0)
+    return U2.astype(np.uint8), dilate(U2, srad)
+
+
+def repair_hits(hits, hits2, U2, U):
+    """hits with hits2's counts on U2, and U | U2."""
+    m = np.asarray(U2) != 0
+    h = np.array(hits, np.uint32)
+    h[:, m] = np.asarray(hits2, np.uint32)[:, m]
+    return h, ((np.asarray(U) != 0) | m).astype(np.uint8)
+
+
 def merge_guided(hits, values, pred, U):
     """values[first argmax of hits] on U, values[pred] elsewhere (hits [L,...])."""
     values = np.asarray(values, np.uint8)

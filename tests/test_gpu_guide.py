@@ -96,6 +96,41 @@ def test_merge_guided_select_labels_bitexact(eng):
     np.testing.assert_array_equal(out.download(dims, np.uint8), G.select_labels(a, pred, U, values))
 
 
+@pytest.mark.parametrize("minh", [0, 1, 6])
+def test_guided_disagree_bitexact(eng, minh):
+    rng = np.random.default_rng(40 + minh)
+    nl, dims = 7, (9, 11, 13)
+    V = int(np.prod(dims))
+    hits = rng.integers(0, 3, (nl,) + dims).astype(np.uint32)
+    hits[:, 2, :4] = 0
+    pred = rng.integers(0, nl, dims).astype(np.uint8)
+    U = (rng.random(dims) < 0.3).astype(np.uint8)
+    hb, pb, Ub, D = eng.upload(hits), eng.upload(pred), eng.upload(U), eng.alloc(V)
+    _hip.call("bmx_guided_disagree", eng.ctx, hb.ptr, nl, V, pb.ptr, Ub.ptr, minh, D.ptr)
+    np.testing.assert_array_equal(D.download(dims, np.uint8), G.guided_disagree(hits, pred, U, minh))
+
+
+@pytest.mark.parametrize("r,srad", [(0, 0), (2, 8), (4, 20)])
+def test_repair_masks_and_hits_bitexact(eng, r, srad):
+    rng = np.random.default_rng(50 + r)
+    nl, dims = 5, (21, 25, 27)
+    V = int(np.prod(dims))
+    D = (rng.random(dims) < 0.002).astype(np.uint8)
+    U = G.dilate(rng.random(dims) < 0.003, 2)
+    Db, Ub, need, tmp = eng.upload(D), eng.upload(U), eng.alloc(V), eng.alloc(V)
+    _hip.call("bmx_repair_masks", eng.ctx, Db.ptr, Ub.ptr, *dims, r, srad, need.ptr, tmp.ptr)
+    U2_ref, need_ref = G.repair_masks(D, U, r, srad)
+    np.testing.assert_array_equal(Db.download(dims, np.uint8), U2_ref)
+    np.testing.assert_array_equal(need.download(dims, np.uint8), need_ref)
+    hits = rng.integers(0, 50, (nl,) + dims).astype(np.uint32)
+    hits2 = rng.integers(0, 50, (nl,) + dims).astype(np.uint32)
+    hb, h2b = eng.upload(hits), eng.upload(hits2)
+    _hip.call("bmx_repair_hits", eng.ctx, hb.ptr, h2b.ptr, nl, V, Db.ptr, Ub.ptr)
+    h_ref, U_ref = G.repair_hits(hits, hits2, U2_ref, U)
+    np.testing.assert_array_equal(hb.download((nl,) + dims, np.uint32), h_ref)
+    np.testing.assert_array_equal(Ub.download(dims, np.uint8), U_ref)
+
+
 def test_hits_index_bitexact(eng):
     rng = np.random.default_rng(19)
     nl, dims, grid = 6, (6, 10, 7), (8, 16, 8)
@@ -195,6 +230,58 @@ def test_guided_exact_on_uncertain_set(eng, trained):
     assert g["walked_seeds"] <= g["nseeds"]
 
 
+class _PlantedProposal:
+    """A proposal with a known confident error: the plain result's label indices
+    with a cube given another label, and no voxel flagged uncertain."""
+
+    def __init__(self, pred):
+        self.pred = np.ascontiguousarray(pred, np.uint8)
+
+    def predict_volume(self, vol, flags, nlabels, thr, pred, unc):
+        vol.eng.upload_into(pred, self.pred)
+        unc.zero()
+
+
+def test_repair_exact_on_repaired_set(eng):
+    """A confident error inside a structure lies outside U (only boundaries are
+    uncertain).  The repair walk finds it from the first walk's hits; with
+    srad >= sorw the repaired voxels (now part of U) carry the plain labels
+    exactly, the cube's error is gone, and voxels outside U keep the proposal."""
+    from biomedisa_amd.interpolation import DeviceVolume, run_pipeline, run_pipeline_guided
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(50, 46, 58), seed=16, n_struct=5, radius=(8, 14), every=6, allaxis=False)
+    img, labidx, values, nol = _prep(vol)
+    dv = DeviceVolume(eng, img, labidx)
+    sorw = 20
+    p = run_pipeline(dv, nol, values, nbrw=4, sorw=sorw, allaxis=False)
+    preg = p["regular"].download(dv.dims, np.uint8)
+    idx = np.searchsorted(values, preg).astype(np.uint8)
+    assert np.array_equal(values[idx], preg)
+    # a 5^3 cube centred on the deepest voxel of the most frequent structure label;
+    # with brad = 0 its central 3^3 voxels are outside U
+    from scipy import ndimage
+    lab = int(np.bincount(idx[idx > 0]).argmax())
+    depth = ndimage.distance_transform_cdt(idx == lab, metric="chessboard")
+    c = np.unravel_index(int(depth.argmax()), depth.shape)
+    assert depth[c] >= 4
+    planted = idx.copy()
+    cube = tuple(slice(k - 2, k + 3) for k in c)
+    planted[cube] = (lab % (nol - 1)) + 1
+    kw = dict(nbrw=4, sorw=sorw, allaxis=False, thr=0.9, brad=0, srad=sorw, far=0, keep=True)
+    g0 = run_pipeline_guided(dv, nol, values, _PlantedProposal(planted), repair=None, **kw)
+    g1 = run_pipeline_guided(dv, nol, values, _PlantedProposal(planted), repair=2, repair_min=1, **kw)
+    dl = lambda b: b.download(dv.dims, np.uint8)  # noqa: E731
+    U0, U1 = dl(g0["U"]).astype(bool), dl(g1["U"]).astype(bool)
+    assert g0["repair_seeds"] == 0 and g1["repair_seeds"] > 0
+    assert g1["repair_voxels"] == int((U1 & ~U0).sum()) > 0
+    greg = dl(g1["regular"])
+    np.testing.assert_array_equal(greg[U1], preg[U1])
+    np.testing.assert_array_equal(greg[~U1], values[planted][~U1])
+    inner = tuple(slice(k - 1, k + 2) for k in c)
+    assert (dl(g0["regular"])[inner] != preg[inner]).all()
+    np.testing.assert_array_equal(greg[cube], preg[cube])
+
+
 def test_far_ring_estimates_skipped_walks(eng):
     """With srad = 0 and a ring that reaches every seed, the hits are the first
     walk of every seed counted nbrw times; with far_walks = nbrw the ring walks
@@ -204,7 +291,7 @@ def test_far_ring_estimates_skipped_walks(eng):
     vol = structures_volume(shape=(40, 44, 48), seed=14, n_struct=4, radius=(4, 12), every=6, allaxis=False)
     sorw = 60
     p, g, dl, values = _pipelines(eng, vol, 4, sorw, 0, S.Net(eng, seed=3), thr=0.9, brad=2, srad=0, far=200,
-                                  far_walks=4)
+                                  far_walks=4, repair=None)
     U = dl(g["U"]).astype(bool)
     assert g["walked_seeds"] + g["ring_seeds"] == g["nseeds"]
     np.testing.assert_array_equal(dl(g["regular"])[U], dl(p["regular"])[U])
@@ -222,7 +309,8 @@ def test_trained_network_gain(eng):
     res = {}
     for name, net in (("trained", S.net(0)), ("untrained", S.Net(eng, seed=5))):
         p, g, dl, values = _pipelines(eng, vol, 10, 4000, 0, net)
-        cost = (g["walked_seeds"] * 10 + g["ring_seeds"] * 1) / (10.0 * g["nseeds"])
+        cost = (g["walked_seeds"] * 10 + g["ring_seeds"] + g["repair_seeds"] * 10 + g["repair_ring_seeds"]) \
+            / (10.0 * g["nseeds"])
         d = per_label_dice(dl(g["regular"]), dl(p["regular"]))
         res[name] = (cost, min(d.values()))
         print(name, "walk cost %.3f of plain, min per-label Dice %.4f" % res[name])

@@ -148,3 +148,44 @@ def test_select_seeds():
     mask = np.zeros(40, np.uint8)
     mask[[0, 9, 27, 28]] = 1
     np.testing.assert_array_equal(G.select_seeds(s, mask)["pos"], [0, 9, 27])
+
+
+def test_guided_disagree_brute_force():
+    """Voxel by voxel: outside U, at least minhits hits, and the first label with
+    the largest count differs from the proposal (ties go to the lower label)."""
+    rng = np.random.default_rng(31)
+    nl, dims = 4, (5, 6, 7)
+    hits = rng.integers(0, 4, (nl,) + dims).astype(np.uint32)
+    hits[:, 0, 0, :3] = 0
+    hits[:, 1, 1, 1] = [2, 2, 0, 0]
+    pred = rng.integers(0, nl, dims).astype(np.uint8)
+    pred[1, 1, 1] = 1
+    U = (rng.random(dims) < 0.3).astype(np.uint8)
+    U[1, 1, 1] = 0
+    for minh in (0, 1, 5):
+        ref = np.zeros(dims, np.uint8)
+        for z, y, x in np.ndindex(*dims):
+            h = [int(hits[l, z, y, x]) for l in range(nl)]
+            best = max(range(nl), key=lambda l: (h[l], -l))
+            ref[z, y, x] = int(not U[z, y, x] and sum(h) >= max(minh, 1) and best != pred[z, y, x])
+        np.testing.assert_array_equal(G.guided_disagree(hits, pred, U, minh), ref)
+    assert G.guided_disagree(hits, pred, U, 1)[1, 1, 1] == 1   # tie 2/2 -> label 0 != 1
+
+
+def test_repair_masks_and_hits():
+    rng = np.random.default_rng(32)
+    dims = (12, 13, 14)
+    D = rng.random(dims) < 0.004
+    U = rng.random(dims) < 0.2
+    U2, need = G.repair_masks(D, U, 2, 3)
+    ref = ndimage.binary_dilation(D, structure=np.ones((5, 5, 5), bool)) & ~U
+    np.testing.assert_array_equal(U2, ref.astype(np.uint8))
+    np.testing.assert_array_equal(need, ndimage.binary_dilation(ref, structure=np.ones((7,) * 3, bool))
+                                  .astype(np.uint8))
+    hits = rng.integers(0, 9, (3,) + dims).astype(np.uint32)
+    hits2 = rng.integers(0, 9, (3,) + dims).astype(np.uint32)
+    h, U3 = G.repair_hits(hits, hits2, U2, U)
+    m = U2.astype(bool)
+    np.testing.assert_array_equal(h[:, m], hits2[:, m])
+    np.testing.assert_array_equal(h[:, ~m], hits[:, ~m])
+    np.testing.assert_array_equal(U3, (U | m).astype(np.uint8))
