@@ -8,10 +8,17 @@ the formats this package reads and writes itself.
                  compress is set, BigTIFF above 2 GB, as the reference's tifffile
                  call does (biomedisa_helper.py:835-843).
     .nrrd        NRRD with raw or gzip encoding.
+    .am          AmiraMesh / Avizo binary lattices (either byte order; raw,
+                 HxByteRLE or HxZip data; the first data stream).  Results are
+                 written with the label file's header, as the reference does
+                 (amira_to_np/amira_data_stream.py:476-515), or a minimal one.
+    .nii/.nii.gz NIfTI-1 single files (scl_slope applied as SimpleITK does).
+    .mha/.mhd    MetaImage, data inline or in a separate raw file, optionally
+                 zlib-compressed.
     .npy/.npz    NumPy arrays (.npz: the first array).
 
-Other extensions (.am, .nc, .nii, .mhd, .zarr, directories of slices) need
-libraries this package does not carry and raise NotImplementedError.
+Other extensions (.nc, .hdr/.img, .zip, directories of slices) raise
+NotImplementedError.
 """
 import gzip
 import os
@@ -205,6 +212,250 @@ def _write_nrrd(path, arr, compress=True):
         f.write(gzip.compress(data, 6) if compress else data)
 
 
+_AM_TYPES = {"byte": "u1", "ubyte": "u1", "sbyte": "i1", "short": "i2", "ushort": "u2", "int": "i4",
+             "uint": "u4", "float": "f4", "double": "f8"}
+
+
+def _rle_decode(data, n):
+    from . import _hip
+    src = np.frombuffer(data, np.uint8)
+    out = np.empty(n, np.uint8)
+    got = _hip.load().bmx_hxbyterle_decode(src.ctypes.data, src.size, out.ctypes.data, n)
+    if got != n:
+        raise ValueError(_hip.load().bmx_last_error().decode(errors="replace"))
+    return out
+
+
+def _rle_encode(arr):
+    from . import _hip
+    src = np.ascontiguousarray(arr, np.uint8).ravel()
+    out = np.empty(src.size + (src.size + 126) // 127, np.uint8)
+    n = _hip.load().bmx_hxbyterle_encode(src.ctypes.data, src.size, out.ctypes.data, out.size)
+    if n < 0:
+        raise ValueError(_hip.load().bmx_last_error().decode(errors="replace"))
+    return out[:n].tobytes()
+
+
+def _am_split(buf):
+    """-> (header bytes up to the data section, data section bytes)."""
+    at = buf.find(b"\n@1\n")
+    if at < 0:
+        raise ValueError("AmiraMesh: no data section @1")
+    return buf[:at], buf[at + 4:]
+
+
+def _read_am(path):
+    with open(path, "rb") as f:
+        buf = f.read()
+    head, body = _am_split(buf)
+    text = head.decode("latin-1")
+    first = text.split("\n", 1)[0]
+    if "AmiraMesh" not in first and "Avizo" not in first:
+        raise ValueError("%s: not an AmiraMesh file" % path)
+    if "BINARY" not in first:
+        raise NotImplementedError("%s: ASCII AmiraMesh" % path)
+    bo = "<" if "LITTLE-ENDIAN" in first else ">"
+    import re
+    dims = re.search(r"define\s+Lattice\s+(\d+)\s+(\d+)\s+(\d+)", text)
+    if dims is None:
+        raise NotImplementedError("%s: no 3-D Lattice" % path)
+    X, Y, Z = (int(v) for v in dims.groups())
+    m = re.search(r"Lattice\s*\{\s*(\w+)\s+\w+\s*\}\s*@1(?:\((\w+),(\d+)\))?", text)
+    if m is None:
+        raise NotImplementedError("%s: no Lattice data stream @1" % path)
+    typ, enc, enc_len = m.group(1), m.group(2), m.group(3)
+    if typ not in _AM_TYPES:
+        raise NotImplementedError("%s: lattice type %s" % (path, typ))
+    dt = np.dtype(bo + _AM_TYPES[typ])
+    n = X * Y * Z
+    if enc is None:
+        arr = np.frombuffer(body[:n * dt.itemsize], dtype=dt)
+    elif enc == "HxByteRLE":
+        arr = _rle_decode(body[:int(enc_len)], n * dt.itemsize).view(dt)
+    elif enc == "HxZip":
+        arr = np.frombuffer(zlib.decompress(body[:int(enc_len)]), dtype=dt)
+    else:
+        raise NotImplementedError("%s: AmiraMesh encoding %s" % (path, enc))
+    if arr.size != n:
+        raise ValueError("%s: data section holds %d of %d values" % (path, arr.size, n))
+    return arr.reshape(Z, Y, X).astype(dt.newbyteorder("="))
+
+
+def am_header(path):
+    """The header of an AmiraMesh file (bytes up to its data section), as the
+    reference keeps it to write results with the label file's materials."""
+    with open(path, "rb") as f:
+        return _am_split(f.read())[0]
+
+
+def _write_am(path, arr, compress=True, header=None):
+    import re
+    arr = np.ascontiguousarray(arr)
+    names = {"u1": "byte", "i1": "sbyte", "i2": "short", "u2": "ushort", "i4": "int", "u4": "uint",
+             "f4": "float", "f8": "double"}
+    key = arr.dtype.kind + str(arr.dtype.itemsize)
+    Z, Y, X = arr.shape
+    if header is None:
+        enc = "HxByteRLE" if compress and key == "u1" else None
+        head = ("# AmiraMesh BINARY-LITTLE-ENDIAN 2.1\n\n\ndefine Lattice %d %d %d\n\n"
+                "Parameters {\n    Content \"%dx%dx%d %s, uniform coordinates\",\n    CoordType \"uniform\"\n}\n\n"
+                "Lattice { %s Data } @1%s\n\n# Data section follows"
+                % (X, Y, Z, X, Y, Z, names[key], names[key], "(HxByteRLE,0)" if enc else ""))
+        head = head.encode("latin-1")
+        bo = "<"
+    else:
+        head = bytes(header)
+        text = head.decode("latin-1")
+        first = text.split("\n", 1)[0]
+        if "BINARY" not in first:
+            raise NotImplementedError("writing ASCII AmiraMesh")
+        bo = "<" if "LITTLE-ENDIAN" in first else ">"
+        m = re.search(r"Lattice\s*\{\s*(\w+)\s+\w+\s*\}\s*@1(?:\((\w+),(\d+)\))?", text)
+        dims = re.search(r"define\s+Lattice\s+(\d+)\s+(\d+)\s+(\d+)", text)
+        if m is None or dims is None or tuple(int(v) for v in dims.groups()) != (X, Y, Z):
+            raise ValueError("AmiraMesh header does not describe a %dx%dx%d lattice" % (X, Y, Z))
+        if _AM_TYPES.get(m.group(1)) != key:
+            arr = arr.astype(_AM_TYPES[m.group(1)])
+        enc = m.group(2)
+    raw = arr.astype(arr.dtype.newbyteorder(bo)).tobytes()
+    if enc == "HxByteRLE":
+        data = _rle_encode(np.frombuffer(raw, np.uint8))
+    elif enc == "HxZip":
+        data = zlib.compress(raw, 6)
+    elif enc is None:
+        data = raw
+    else:
+        raise NotImplementedError("AmiraMesh encoding %s" % enc)
+    if enc is not None:
+        # the encoded length in the stream's definition (amira_data_stream.py:497-503)
+        head = re.sub(rb"@1\(" + enc.encode() + rb",\d+\)", b"@1(" + enc.encode() + b",%d)" % len(data), head,
+                      count=1)
+    with open(path, "wb") as f:
+        f.write(head)
+        f.write(b"\n@1\n")
+        f.write(data)
+        f.write(b"\n")
+
+
+_NII_TYPES = {2: "u1", 4: "i2", 8: "i4", 16: "f4", 64: "f8", 256: "i1", 512: "u2", 768: "u4", 1024: "i8", 1280: "u8"}
+
+
+def _read_nii(path):
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rb") as f:
+        buf = f.read()
+    if len(buf) < 348:
+        raise ValueError("%s: not a NIfTI-1 file" % path)
+    bo = "<" if struct.unpack("<i", buf[:4])[0] == 348 else ">"
+    if struct.unpack(bo + "i", buf[:4])[0] != 348:
+        raise ValueError("%s: not a NIfTI-1 file" % path)
+    dim = struct.unpack(bo + "8h", buf[40:56])
+    dtype_code = struct.unpack(bo + "h", buf[70:72])[0]
+    vox_offset = int(struct.unpack(bo + "f", buf[108:112])[0])
+    slope, inter = struct.unpack(bo + "2f", buf[112:120])
+    if buf[344:347] != b"n+1":
+        raise NotImplementedError("%s: NIfTI header without data (.hdr/.img pair)" % path)
+    if dtype_code not in _NII_TYPES:
+        raise NotImplementedError("%s: NIfTI datatype %d" % (path, dtype_code))
+    nd = dim[0]
+    shape = [dim[k] if k <= nd else 1 for k in range(1, 4)]
+    if nd > 3 and any(d > 1 for d in dim[4:nd + 1]):
+        raise NotImplementedError("%s: %d-D NIfTI image" % (path, nd))
+    dt = np.dtype(bo + _NII_TYPES[dtype_code])
+    n = shape[0] * shape[1] * shape[2]
+    arr = np.frombuffer(buf[vox_offset:vox_offset + n * dt.itemsize], dtype=dt)
+    arr = arr.reshape(shape[2], shape[1], shape[0]).astype(dt.newbyteorder("="))
+    if slope not in (0.0, 1.0) or inter != 0.0:
+        arr = arr.astype(np.float64) * (slope if slope != 0.0 else 1.0) + inter
+    return arr
+
+
+def _write_nii(path, arr, compress=True):
+    arr = np.ascontiguousarray(arr)
+    codes = {v: k for k, v in _NII_TYPES.items()}
+    key = arr.dtype.kind + str(arr.dtype.itemsize)
+    if key not in codes:
+        raise NotImplementedError("NIfTI: dtype %s" % arr.dtype)
+    Z, Y, X = arr.shape
+    hdr = bytearray(352)
+    struct.pack_into("<i", hdr, 0, 348)
+    struct.pack_into("<8h", hdr, 40, 3, X, Y, Z, 1, 1, 1, 1)
+    struct.pack_into("<2h", hdr, 70, codes[key], arr.dtype.itemsize * 8)
+    struct.pack_into("<8f", hdr, 76, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+    struct.pack_into("<3f", hdr, 108, 352.0, 1.0, 0.0)
+    struct.pack_into("<B", hdr, 123, 10)      # xyzt_units: mm, s
+    hdr[344:348] = b"n+1\0"
+    data = bytes(hdr) + arr.astype(arr.dtype.newbyteorder("<")).tobytes()
+    if path.endswith(".gz"):
+        data = gzip.compress(data, 6)
+    with open(path, "wb") as f:
+        f.write(data)
+
+
+_MET_TYPES = {"MET_UCHAR": "u1", "MET_CHAR": "i1", "MET_USHORT": "u2", "MET_SHORT": "i2", "MET_UINT": "u4",
+              "MET_INT": "i4", "MET_ULONG_LONG": "u8", "MET_LONG_LONG": "i8", "MET_FLOAT": "f4",
+              "MET_DOUBLE": "f8"}
+
+
+def _read_meta(path):
+    with open(path, "rb") as f:
+        buf = f.read()
+    fields, pos = {}, 0
+    while True:
+        end = buf.find(b"\n", pos)
+        if end < 0:
+            raise ValueError("%s: MetaImage header without ElementDataFile" % path)
+        line = buf[pos:end].decode("latin-1").strip()
+        pos = end + 1
+        if "=" in line:
+            k, v = line.split("=", 1)
+            fields[k.strip()] = v.strip()
+            if k.strip() == "ElementDataFile":
+                break
+    dims = [int(v) for v in fields["DimSize"].split()]
+    if len(dims) != 3 or int(fields.get("ElementNumberOfChannels", "1")) != 1:
+        raise NotImplementedError("%s: only 3-D single-channel MetaImages" % path)
+    msb = (fields.get("BinaryDataByteOrderMSB", fields.get("ElementByteOrderMSB", "False")).lower() == "true")
+    dt = np.dtype((">" if msb else "<") + _MET_TYPES[fields["ElementType"]])
+    src = fields["ElementDataFile"]
+    if src == "LOCAL":
+        body = buf[pos:]
+    else:
+        with open(os.path.join(os.path.dirname(path), src), "rb") as f:
+            body = f.read()
+    if fields.get("CompressedData", "False").lower() == "true":
+        body = zlib.decompress(body)
+    X, Y, Z = dims
+    arr = np.frombuffer(body[:X * Y * Z * dt.itemsize], dtype=dt).reshape(Z, Y, X)
+    return arr.astype(dt.newbyteorder("="))
+
+
+def _write_meta(path, arr, compress=True):
+    arr = np.ascontiguousarray(arr)
+    names = {v: k for k, v in _MET_TYPES.items()}
+    key = arr.dtype.kind + str(arr.dtype.itemsize)
+    if key not in names:
+        raise NotImplementedError("MetaImage: dtype %s" % arr.dtype)
+    Z, Y, X = arr.shape
+    data = arr.astype(arr.dtype.newbyteorder("<")).tobytes()
+    if compress:
+        data = zlib.compress(data, 6)
+    local = path.lower().endswith(".mha")
+    raw_name = os.path.splitext(os.path.basename(path))[0] + (".zraw" if compress else ".raw")
+    head = ("ObjectType = Image\nNDims = 3\nBinaryData = True\nBinaryDataByteOrderMSB = False\n"
+            "CompressedData = %s\n%sTransformMatrix = 1 0 0 0 1 0 0 0 1\nOffset = 0 0 0\n"
+            "ElementSpacing = 1 1 1\nDimSize = %d %d %d\nElementType = %s\nElementDataFile = %s\n"
+            % ("True" if compress else "False", "CompressedDataSize = %d\n" % len(data) if compress else "",
+               X, Y, Z, names[key], "LOCAL" if local else raw_name))
+    with open(path, "wb") as f:
+        f.write(head.encode("latin-1"))
+        if local:
+            f.write(data)
+    if not local:
+        with open(os.path.join(os.path.dirname(path), raw_name), "wb") as f:
+            f.write(data)
+
+
 def extension(path):
     """The file type as load_data names it (biomedisa_helper.py:390-394)."""
     ext = os.path.splitext(path)[1]
@@ -224,6 +475,12 @@ def load_data(path):
         return _read_tiff(path), ext
     if ext == ".nrrd":
         return _read_nrrd(path), ext
+    if ext == ".am":
+        return _read_am(path), ext
+    if ext in (".nii", ".nii.gz"):
+        return _read_nii(path), ext
+    if ext in (".mha", ".mhd"):
+        return _read_meta(path), ext
     if ext == ".npy":
         return np.load(path, allow_pickle=False), ext
     if ext == ".npz":
@@ -232,12 +489,20 @@ def load_data(path):
     raise NotImplementedError("reading %s files is not part of this package" % ext)
 
 
-def save_data(path, arr, compress=True):
+def save_data(path, arr, compress=True, header=None):
+    """Writes arr with the type its extension names; header: an AmiraMesh
+    header (am_header) for .am results."""
     ext = extension(path)
     if ext in (".tif", ".tiff"):
         _write_tiff(path, arr, compress)
     elif ext == ".nrrd":
         _write_nrrd(path, arr, compress)
+    elif ext == ".am":
+        _write_am(path, arr, compress, header)
+    elif ext in (".nii", ".nii.gz"):
+        _write_nii(path, arr, compress)
+    elif ext in (".mha", ".mhd"):
+        _write_meta(path, arr, compress)
     elif ext == ".npy":
         np.save(path, arr, allow_pickle=False)
     elif ext == ".npz":

@@ -233,6 +233,14 @@ int bmx_clean(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, 
 int bmx_fill(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, double threshold,
              uint8_t *out);
 
+/* HxByteRLE codec of AmiraMesh byte lattices (host memory; no device work),
+ * replacing byterle_decoder / byterle_encoder
+ * (features/amira_to_np/amira_data_stream.py:21-141).  decode: n_out bytes
+ * from n_in encoded bytes, returns n_out or a negative BMX_E_*; encode: needs
+ * cap >= n + ceil(n / 127), returns the encoded length. */
+int64_t bmx_hxbyterle_decode(const uint8_t *in, int64_t n_in, uint8_t *out, int64_t n_out);
+int64_t bmx_hxbyterle_encode(const uint8_t *in, int64_t n, uint8_t *out, int64_t cap);
+
 /* Image denoising: replaces smooth_img_3x3 (features/biomedisa_helper.py:309-323), the
  * mean of each voxel's 3x3x3 neighbourhood clipped to the volume (summed in
  * double, stored in the image's type; uint8 truncates).  out != img. */

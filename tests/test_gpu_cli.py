@@ -46,7 +46,25 @@ def test_smart_interpolation_denoise(dtype):
     assert (plain["regular"] != got["regular"]).any()
 
 
-@pytest.mark.parametrize("label_ext", [".tif", ".nrrd"])
+_AM_HEADER = """# AmiraMesh BINARY-LITTLE-ENDIAN 2.1
+
+define Lattice %d %d %d
+
+Parameters {
+    Materials {
+        Exterior { id 0 }
+        Tissue { Color 0.1 0.6 0.3, id 1 }
+    }
+    Content "%dx%dx%d byte, uniform coordinates",
+    CoordType "uniform"
+}
+
+Lattice { byte Labels } @1(HxByteRLE,0)
+
+# Data section follows"""
+
+
+@pytest.mark.parametrize("label_ext", [".tif", ".nrrd", ".am", ".nii.gz", ".mha"])
 def test_cli_end_to_end(tmp_path, label_ext):
     from biomedisa_amd.__main__ import main
     from biomedisa_amd.interpolation import smart_interpolation
@@ -55,7 +73,11 @@ def test_cli_end_to_end(tmp_path, label_ext):
     img_path = str(tmp_path / "scan.tif")
     lab_path = str(tmp_path / ("scan_labels" + label_ext))
     bio.save_data(img_path, vol.img)
-    bio.save_data(lab_path, vol.labels)
+    if label_ext == ".am":
+        Z, Y, X = vol.labels.shape
+        bio.save_data(lab_path, vol.labels, header=(_AM_HEADER % (X, Y, Z, X, Y, Z)).encode())
+    else:
+        bio.save_data(lab_path, vol.labels)
     _say("cli %s" % label_ext)
     assert main([img_path, lab_path, "--nbrw", "3", "--sorw", "120", "-s", "2", "-u", "-c", "-f"]) == 0
     want = smart_interpolation(vol.img, vol.labels, nbrw=3, sorw=120, smooth=2, uncertainty=True,
@@ -65,3 +87,6 @@ def test_cli_end_to_end(tmp_path, label_ext):
     for k in ("regular", "smooth", "uncertainty", "cleaned", "smooth_cleaned", "filled", "cleaned_filled"):
         got = bio.load_data(files[k])[0]
         np.testing.assert_array_equal(got, want[k], err_msg=k)
+    if label_ext == ".am":
+        with open(files["regular"], "rb") as f:
+            assert b"Tissue { Color 0.1 0.6 0.3, id 1 }" in f.read()

@@ -2,7 +2,9 @@
 and NRRD files written and read back, files built byte by byte from the format
 specifications (big-endian TIFF with horizontal differencing, strips stored out
 of line, BigTIFF; NRRD with gzip and comments), the result file names of
-interpolation.py:115-133, and the reference's command-line defaults."""
+interpolation.py:115-133, and the reference's command-line defaults.  AmiraMesh
+(HxByteRLE checked against a decoder written from its definition), NIfTI-1 and
+MetaImage files likewise."""
 import gzip
 import struct
 import zlib
@@ -123,8 +125,11 @@ def test_missing_and_unsupported(tmp_path, capsys):
     got, ext = bio.load_data(str(tmp_path / "none.tif"))
     assert got is None and ext == ".tif"
     assert "Error" in capsys.readouterr().out
-    (tmp_path / "x.am").write_bytes(b"# AmiraMesh")
+    (tmp_path / "x.nc").write_bytes(b"CDF")
     with pytest.raises(NotImplementedError):
+        bio.load_data(str(tmp_path / "x.nc"))
+    (tmp_path / "x.am").write_bytes(b"# AmiraMesh")
+    with pytest.raises(ValueError):
         bio.load_data(str(tmp_path / "x.am"))
 
 
@@ -164,3 +169,177 @@ def test_missing_file_returns_none(tmp_path, capsys):
     assert smart_interpolation(None, None, path_to_data=str(tmp_path / "no.tif"),
                                path_to_labels=str(tmp_path / "no_l.tif")) is None
     assert "Invalid image data" in capsys.readouterr().out
+
+
+# --- HxByteRLE, AmiraMesh, NIfTI, MetaImage ---------------------------------
+
+def _rle_decode_py(stream, n):
+    """HxByteRLE by its definition: control byte c; c & 0x80: (c & 0x7f)
+    literal bytes follow; otherwise the next byte repeats c times."""
+    out, i = [], 0
+    while len(out) < n:
+        c = stream[i]
+        i += 1
+        k = c & 0x7F
+        if c & 0x80:
+            out.extend(stream[i:i + k])
+            i += k
+        else:
+            out.extend([stream[i]] * k)
+            i += 1
+    return np.array(out, np.uint8)
+
+
+def test_hxbyterle_known_stream_and_round_trip():
+    stream = bytes([3, 7, 0x82, 1, 2, 0x7F, 0, 0x81, 9])
+    want = np.array([7, 7, 7, 1, 2] + [0] * 127 + [9], np.uint8)
+    np.testing.assert_array_equal(bio._rle_decode(stream, want.size), want)
+    rng = np.random.default_rng(4)
+    for n in (0, 1, 2, 3, 127, 128, 129, 1000, 50000):
+        runs = rng.integers(1, 300, n)
+        data = np.repeat(rng.integers(0, 4, n).astype(np.uint8), runs)[:n]
+        noisy = rng.random(n) < 0.2
+        data[noisy] = rng.integers(0, 256, int(noisy.sum()))
+        enc = bio._rle_encode(data)
+        assert len(enc) <= n + (n + 126) // 127
+        np.testing.assert_array_equal(_rle_decode_py(enc, n), data)
+        np.testing.assert_array_equal(bio._rle_decode(enc, n), data)
+    noise = rng.integers(0, 256, 10000).astype(np.uint8)
+    np.testing.assert_array_equal(bio._rle_decode(bio._rle_encode(noise), noise.size), noise)
+    with pytest.raises(ValueError):
+        bio._rle_decode(bytes([0x85, 1, 2]), 5)          # truncated literal record
+    with pytest.raises(ValueError):
+        bio._rle_decode(bytes([10, 1]), 5)               # stream longer than the lattice
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint16, np.int32, np.float32, np.float64])
+@pytest.mark.parametrize("compress", [False, True])
+def test_amira_round_trip(tmp_path, dtype, compress):
+    arr = (np.random.default_rng(5).random((6, 7, 8)) * 100).astype(dtype)
+    arr[2:4] = 0
+    path = str(tmp_path / "v.am")
+    bio.save_data(path, arr, compress)
+    got, ext = bio.load_data(path)
+    assert ext == ".am" and got.dtype == arr.dtype
+    np.testing.assert_array_equal(got, arr)
+
+
+_AM_LABEL_HEADER = b"""# AmiraMesh BINARY-LITTLE-ENDIAN 2.1
+
+
+define Lattice 5 4 3
+
+Parameters {
+    Materials {
+        Exterior {
+            id 0
+        }
+        Bone {
+            Color 0.8 0.7 0.2,
+            id 1
+        }
+    }
+    Content "5x4x3 byte, uniform coordinates",
+    CoordType "uniform"
+}
+
+Lattice { byte Labels } @1(HxByteRLE,7)
+
+# Data section follows"""
+
+
+def test_amira_hand_built_files(tmp_path):
+    vol = np.zeros((3, 4, 5), np.uint8)
+    vol[1, 1:3, 1:4] = 1
+    # HxByteRLE byte lattice with materials, the length in the stream definition
+    enc = bio._rle_encode(vol)
+    head = _AM_LABEL_HEADER.replace(b"@1(HxByteRLE,7)", b"@1(HxByteRLE,%d)" % len(enc))
+    (tmp_path / "l.am").write_bytes(head + b"\n@1\n" + enc + b"\n")
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "l.am"))[0], vol)
+    # big-endian ushort, raw
+    u = (np.arange(60, dtype=np.uint16) * 1000).reshape(3, 4, 5)
+    head = (b"# AmiraMesh 3D BINARY 2.0\n\ndefine Lattice 5 4 3\n\nLattice { ushort Data } @1\n\n"
+            b"# Data section follows")
+    (tmp_path / "be.am").write_bytes(head + b"\n@1\n" + u.astype(">u2").tobytes() + b"\n")
+    got = bio.load_data(str(tmp_path / "be.am"))[0]
+    assert got.dtype == np.uint16
+    np.testing.assert_array_equal(got, u)
+    # HxZip float
+    f = np.linspace(-1, 1, 60, dtype=np.float32).reshape(3, 4, 5)
+    z = zlib.compress(f.tobytes())
+    head = (b"# Avizo BINARY-LITTLE-ENDIAN 2.1\ndefine Lattice 5 4 3\nLattice { float Data } @1(HxZip,%d)\n"
+            b"# Data section follows" % len(z))
+    (tmp_path / "z.am").write_bytes(head + b"\n@1\n" + z + b"\n")
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "z.am"))[0], f)
+
+
+def test_amira_result_keeps_the_label_header(tmp_path):
+    """Results are written with the label file's header (materials kept) and the
+    new encoded length, as amira_data_stream.py:476-515 does."""
+    vol = np.zeros((3, 4, 5), np.uint8)
+    vol[0, :2] = 1
+    enc = bio._rle_encode(vol)
+    head = _AM_LABEL_HEADER.replace(b"@1(HxByteRLE,7)", b"@1(HxByteRLE,%d)" % len(enc))
+    (tmp_path / "l.am").write_bytes(head + b"\n@1\n" + enc + b"\n")
+    header = bio.am_header(str(tmp_path / "l.am"))
+    res = np.zeros((3, 4, 5), np.uint8)
+    res[1:] = 1
+    res[2, 3, 4] = 0
+    bio.save_data(str(tmp_path / "r.am"), res, True, header=header)
+    raw = (tmp_path / "r.am").read_bytes()
+    assert b"Bone" in raw and b"Color 0.8 0.7 0.2" in raw
+    assert b"@1(HxByteRLE,%d)" % len(bio._rle_encode(res)) in raw
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "r.am"))[0], res)
+    with pytest.raises(ValueError):
+        bio.save_data(str(tmp_path / "bad.am"), np.zeros((2, 2, 2), np.uint8), True, header=header)
+
+
+@pytest.mark.parametrize("name", ["v.nii", "v.nii.gz"])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.int16, np.uint16, np.int32, np.float32, np.float64])
+def test_nifti_round_trip(tmp_path, name, dtype):
+    arr = (np.random.default_rng(6).random((4, 5, 6)) * 100).astype(dtype)
+    path = str(tmp_path / name)
+    bio.save_data(path, arr)
+    got, ext = bio.load_data(path)
+    assert ext == (".nii.gz" if name.endswith(".gz") else ".nii") and got.dtype == arr.dtype
+    np.testing.assert_array_equal(got, arr)
+
+
+def test_nifti_hand_built(tmp_path):
+    """NIfTI-1: x fastest in the file; dim[1..3] = X, Y, Z; vox_offset; scl_slope
+    and scl_inter applied; big-endian headers recognised by sizeof_hdr."""
+    vol = np.arange(2 * 3 * 4, dtype=np.uint16).reshape(2, 3, 4)
+    for bo in "<>":
+        hdr = bytearray(400)
+        struct.pack_into(bo + "i", hdr, 0, 348)
+        struct.pack_into(bo + "8h", hdr, 40, 3, 4, 3, 2, 1, 1, 1, 1)
+        struct.pack_into(bo + "2h", hdr, 70, 512, 16)
+        struct.pack_into(bo + "3f", hdr, 108, 400.0, 2.0, -1.0)
+        hdr[344:348] = b"n+1\0"
+        (tmp_path / "h.nii").write_bytes(bytes(hdr) + vol.astype(bo + "u2").tobytes())
+        got = bio.load_data(str(tmp_path / "h.nii"))[0]
+        np.testing.assert_array_equal(got, vol.astype(np.float64) * 2.0 - 1.0)
+
+
+@pytest.mark.parametrize("name", ["v.mha", "v.mhd"])
+@pytest.mark.parametrize("compress", [False, True])
+def test_metaimage_round_trip(tmp_path, name, compress):
+    arr = (np.random.default_rng(7).random((4, 5, 6)) * 100).astype(np.int16)
+    path = str(tmp_path / name)
+    bio.save_data(path, arr, compress)
+    np.testing.assert_array_equal(bio.load_data(path)[0], arr)
+    if name.endswith(".mhd"):
+        assert (tmp_path / ("v.zraw" if compress else "v.raw")).exists()
+
+
+def test_metaimage_hand_built(tmp_path):
+    vol = (np.arange(24, dtype=np.int16) - 12).reshape(2, 3, 4)
+    head = (b"ObjectType = Image\nNDims = 3\nDimSize = 4 3 2\nElementType = MET_SHORT\n"
+            b"BinaryDataByteOrderMSB = True\nElementDataFile = LOCAL\n")
+    (tmp_path / "h.mha").write_bytes(head + vol.astype(">i2").tobytes())
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "h.mha"))[0], vol)
+    (tmp_path / "d.raw").write_bytes(zlib.compress(vol.astype("<i2").tobytes()))
+    head = (b"ObjectType = Image\nNDims = 3\nDimSize = 4 3 2\nElementType = MET_SHORT\n"
+            b"CompressedData = True\nElementDataFile = d.raw\n")
+    (tmp_path / "h.mhd").write_bytes(head)
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "h.mhd"))[0], vol)
