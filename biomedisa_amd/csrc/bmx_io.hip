@@ -62,3 +62,38 @@ extern "C" int64_t bmx_hxbyterle_encode(const uint8_t *in, int64_t n, uint8_t *o
     }
     return o;
 }
+
+// PNG scanline filters (ISO/IEC 15948, section 9): rows x (1 + rowbytes)
+// filtered bytes -> rows x rowbytes raw bytes; bpp = bytes per complete pixel.
+extern "C" int bmx_png_unfilter(const uint8_t *in, int64_t rows, int64_t rowbytes, int bpp, uint8_t *out)
+{
+    BMX_CHECK(in && out && rows >= 0 && rowbytes >= 0 && bpp >= 1 && bpp <= 8, "bmx_png_unfilter: bad arguments");
+    for (int64_t r = 0; r < rows; ++r) {
+        const uint8_t *src = in + r * (rowbytes + 1);
+        const uint8_t ft = src[0];
+        ++src;
+        uint8_t *dst = out + r * rowbytes;
+        const uint8_t *up = r > 0 ? dst - rowbytes : nullptr;
+        for (int64_t i = 0; i < rowbytes; ++i) {
+            const int a = i >= bpp ? dst[i - bpp] : 0;
+            const int b = up ? up[i] : 0;
+            const int c = (up && i >= bpp) ? up[i - bpp] : 0;
+            int pred;
+            switch (ft) {
+            case 0: pred = 0; break;
+            case 1: pred = a; break;
+            case 2: pred = b; break;
+            case 3: pred = (a + b) >> 1; break;
+            case 4: {
+                const int p = a + b - c;
+                const int pa = p > a ? p - a : a - p, pb = p > b ? p - b : b - p, pc = p > c ? p - c : c - p;
+                pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+                break;
+            }
+            default: return bmx_set_error(BMX_E_INVALID, "bmx_png_unfilter: filter type %d in row %lld", ft, (long long)r);
+            }
+            dst[i] = (uint8_t)(src[i] + pred);
+        }
+    }
+    return BMX_OK;
+}

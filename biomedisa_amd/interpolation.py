@@ -303,7 +303,7 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         # passed; result files are written only when the image came from a file
         # (interpolation.py:76-79)
         from . import io as bio
-        final_image_type, am_header = ".tif", None
+        final_image_type, label_header = ".tif", None
         image_from_file = data is None
         if data is None:
             data = bio.load_data(path_to_data)[0] if path_to_data is not None else None
@@ -313,8 +313,8 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         if labelData is None:
             if path_to_labels is not None:
                 labelData, final_image_type = bio.load_data(path_to_labels)
-                if labelData is not None and final_image_type == ".am":
-                    am_header = bio.am_header(path_to_labels)
+                if labelData is not None:
+                    label_header = bio.read_header(path_to_labels)
             if labelData is None:
                 print("Error: Invalid label data.")
                 return None
@@ -366,8 +366,11 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
         from . import io as bio
         for key, path in files.items():
             if results.get(key) is not None:
-                bio.save_data(path, results[key], compression,
-                              header=am_header if bio.extension(path) == ".am" else None)
+                if key == "uncertainty":
+                    bio.save_data(path, results[key], compression)
+                else:
+                    bio.save_data(path, results[key], compression, header=label_header,
+                                  file_type=final_image_type)
     return results
 
 

@@ -16,8 +16,11 @@ the formats this package reads and writes itself.
     .mha/.mhd    MetaImage, data inline or in a separate raw file, optionally
                  zlib-compressed.
     .npy/.npz    NumPy arrays (.npz: the first array).
+    directory / .zip   a stack of 2-D TIFF or grayscale PNG slices in natural
+                 file-name order (biomedisa_helper.py:505-591); results are
+                 written as slices with the label stack's file names.
 
-Other extensions (.nc, .hdr/.img, .zip, directories of slices) raise
+Other extensions (.nc, .hdr/.img, DICOM/JPEG/BMP slices) raise
 NotImplementedError.
 """
 import gzip
@@ -456,6 +459,159 @@ def _write_meta(path, arr, compress=True):
             f.write(data)
 
 
+_PNG_SIG = b"\x89PNG\r\n\x1a\n"
+
+
+def _read_png(buf, name="PNG"):
+    """Grayscale (optionally with alpha, which is dropped) 8/16-bit,
+    non-interlaced PNG -> (rows, cols) array."""
+    from . import _hip
+    if buf[:8] != _PNG_SIG:
+        raise ValueError("%s: not a PNG file" % name)
+    pos, idat, ihdr = 8, [], None
+    while pos < len(buf):
+        n, typ = struct.unpack(">I4s", buf[pos:pos + 8])
+        body = buf[pos + 8:pos + 8 + n]
+        if typ == b"IHDR":
+            ihdr = struct.unpack(">IIBBBBB", body)
+        elif typ == b"IDAT":
+            idat.append(body)
+        elif typ == b"IEND":
+            break
+        pos += 12 + n
+    if ihdr is None:
+        raise ValueError("%s: no IHDR chunk" % name)
+    W, H, depth, ctype, _, _, interlace = ihdr
+    if ctype not in (0, 4) or depth not in (8, 16) or interlace:
+        raise NotImplementedError("%s: PNG colour type %d, depth %d, interlace %d" % (name, ctype, depth, interlace))
+    chans = 1 if ctype == 0 else 2
+    bpp = chans * depth // 8
+    raw = np.frombuffer(zlib.decompress(b"".join(idat)), np.uint8)
+    rowbytes = W * bpp
+    if raw.size < H * (rowbytes + 1):
+        raise ValueError("%s: truncated image data" % name)
+    out = np.empty(H * rowbytes, np.uint8)
+    rc = _hip.load().bmx_png_unfilter(raw.ctypes.data, H, rowbytes, bpp, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(_hip.load().bmx_last_error().decode(errors="replace"))
+    img = out.view(">u2" if depth == 16 else "u1").reshape(H, W, chans)[:, :, 0]
+    return img.astype(img.dtype.newbyteorder("="))
+
+
+def _png_bytes(img, compress=True):
+    img = np.ascontiguousarray(img)
+    if img.dtype not in (np.uint8, np.uint16) or img.ndim != 2:
+        raise NotImplementedError("PNG slices hold 2-D uint8 or uint16 data, not %s %s" % (img.dtype, img.shape))
+    H, W = img.shape
+    depth = 8 * img.dtype.itemsize
+    rows = img.astype(img.dtype.newbyteorder(">")).view(np.uint8).reshape(H, -1)
+    data = np.concatenate([np.zeros((H, 1), np.uint8), rows], axis=1).tobytes()
+
+    def chunk(typ, body):
+        return struct.pack(">I", len(body)) + typ + body + struct.pack(">I", zlib.crc32(typ + body) & 0xFFFFFFFF)
+    return (_PNG_SIG + chunk(b"IHDR", struct.pack(">IIBBBBB", W, H, depth, 0, 0, 0, 0))
+            + chunk(b"IDAT", zlib.compress(data, 6 if compress else 0)) + chunk(b"IEND", b""))
+
+
+def _natural_key(s):
+    """File-name order of the reference (biomedisa_helper.py:377-379)."""
+    import re
+    return [int(t) if t.isdigit() else t.lower() for t in re.split(r"(\d+)", s)]
+
+
+_SLICE_TYPES = (".png", ".tif", ".tiff")
+_OTHER_SLICES = (".dcm", ".dicom", ".bmp", ".jpg", ".jpeg")
+
+
+def _slice_entries(path):
+    """-> sorted [(name, bytes)] of the slice files of a directory or zip archive."""
+    import zipfile
+    entries = []
+    if os.path.isdir(path):
+        for root, _, names in os.walk(path):
+            for n in names:
+                full = os.path.join(root, n)
+                entries.append((full, lambda full=full: open(full, "rb").read()))
+    else:
+        z = zipfile.ZipFile(path)
+        for info in z.infolist():
+            if not info.is_dir():
+                entries.append((info.filename, lambda info=info, z=z: z.read(info)))
+    keep = [(n, rd) for n, rd in entries if not os.path.basename(n).startswith(".")
+            and os.path.splitext(n)[1].lower() in _SLICE_TYPES]
+    if not keep and any(os.path.splitext(n)[1].lower() in _OTHER_SLICES for n, _ in entries):
+        raise NotImplementedError("%s: only TIFF and PNG slices are read" % path)
+    if not keep:
+        raise ValueError("%s: no image slices" % path)
+    keep.sort(key=lambda e: _natural_key(e[0]))
+    return keep
+
+
+def _read_slices(path):
+    planes, names = [], []
+    for name, rd in _slice_entries(path):
+        buf = rd()
+        if os.path.splitext(name)[1].lower() == ".png":
+            img = _read_png(buf, name)
+        else:
+            import tempfile
+            with tempfile.NamedTemporaryFile(suffix=".tif") as t:
+                t.write(buf)
+                t.flush()
+                img = _read_tiff(t.name)
+            if img.shape[0] != 1:
+                raise NotImplementedError("%s: multi-page TIFF slice" % name)
+            img = img[0]
+        if planes and (img.shape != planes[0].shape or img.dtype != planes[0].dtype):
+            raise ValueError("%s: slice %s differs in shape or type from the first" % (path, name))
+        planes.append(img)
+        names.append(os.path.basename(name))
+    return np.stack(planes), {"slices": names, "dtype": planes[0].dtype}
+
+
+def _slice_file_bytes(name, img, compress):
+    if os.path.splitext(name)[1].lower() == ".png":
+        return _png_bytes(img, compress)
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix=".tif") as t:
+        _write_tiff(t.name, img, compress)
+        return open(t.name, "rb").read()
+
+
+def _write_slices(path, arr, compress=True, header=None):
+    """Slices written with the label stack's file names and sample type
+    (biomedisa_helper.py:758-783): a folder, or a zip archive for .zip."""
+    import zipfile
+    arr = np.asarray(arr)
+    if header is not None:
+        names, dtype = header["slices"], header["dtype"]
+        if len(names) != arr.shape[0]:
+            raise ValueError("%d slice names for %d planes" % (len(names), arr.shape[0]))
+        arr = arr.astype(dtype)
+    else:
+        names = ["slice_%05d.tif" % k for k in range(arr.shape[0])]
+    if path.lower().endswith(".zip"):
+        with zipfile.ZipFile(path, "w") as z:
+            for k, n in enumerate(names):
+                z.writestr(n, _slice_file_bytes(n, arr[k], compress))
+    else:
+        os.makedirs(path, exist_ok=True)
+        for k, n in enumerate(names):
+            with open(os.path.join(path, n), "wb") as f:
+                f.write(_slice_file_bytes(n, arr[k], compress))
+
+
+def read_header(path):
+    """What writing results in the type of `path` keeps from it: the AmiraMesh
+    header, or the slice names and sample type of a slice stack; else None."""
+    ext = extension(path)
+    if ext == ".am":
+        return am_header(path)
+    if os.path.isdir(path) or ext == ".zip":
+        return _read_slices(path)[1]
+    return None
+
+
 def extension(path):
     """The file type as load_data names it (biomedisa_helper.py:390-394)."""
     ext = os.path.splitext(path)[1]
@@ -471,6 +627,10 @@ def load_data(path):
     if not os.path.exists(path):
         print(f"Error: No such file or directory '{path}'")
         return None, ext
+    if os.path.isdir(path):
+        return _read_slices(path)[0], ""
+    if ext == ".zip":
+        return _read_slices(path)[0], ext
     if ext in (".tif", ".tiff"):
         return _read_tiff(path), ext
     if ext == ".nrrd":
@@ -489,11 +649,14 @@ def load_data(path):
     raise NotImplementedError("reading %s files is not part of this package" % ext)
 
 
-def save_data(path, arr, compress=True, header=None):
-    """Writes arr with the type its extension names; header: an AmiraMesh
-    header (am_header) for .am results."""
-    ext = extension(path)
-    if ext in (".tif", ".tiff"):
+def save_data(path, arr, compress=True, header=None, file_type=None):
+    """Writes arr with the type file_type (default: the extension of path; ''
+    is a folder of slices), as save_data(..., final_image_type) does;
+    header: read_header of the label file, for .am and slice stacks."""
+    ext = extension(path) if file_type is None else file_type
+    if ext in ("", ".zip"):
+        _write_slices(path, arr, compress, header)
+    elif ext in (".tif", ".tiff"):
         _write_tiff(path, arr, compress)
     elif ext == ".nrrd":
         _write_nrrd(path, arr, compress)

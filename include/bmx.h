@@ -240,6 +240,10 @@ int bmx_fill(bmx_ctx *ctx, const uint8_t *in, int64_t Z, int64_t Y, int64_t X, d
  * cap >= n + ceil(n / 127), returns the encoded length. */
 int64_t bmx_hxbyterle_decode(const uint8_t *in, int64_t n_in, uint8_t *out, int64_t n_out);
 int64_t bmx_hxbyterle_encode(const uint8_t *in, int64_t n, uint8_t *out, int64_t cap);
+/* PNG scanline unfiltering (host memory), for slice stacks read by load_data
+ * (biomedisa_helper.py:505-591): rows x (1 + rowbytes) filtered bytes -> rows x
+ * rowbytes; bpp = bytes per complete pixel (1..8). */
+int bmx_png_unfilter(const uint8_t *in, int64_t rows, int64_t rowbytes, int bpp, uint8_t *out);
 
 /* Image denoising: replaces smooth_img_3x3 (features/biomedisa_helper.py:309-323), the
  * mean of each voxel's 3x3x3 neighbourhood clipped to the volume (summed in

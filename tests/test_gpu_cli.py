@@ -64,7 +64,7 @@ Lattice { byte Labels } @1(HxByteRLE,0)
 # Data section follows"""
 
 
-@pytest.mark.parametrize("label_ext", [".tif", ".nrrd", ".am", ".nii.gz", ".mha"])
+@pytest.mark.parametrize("label_ext", [".tif", ".nrrd", ".am", ".nii.gz", ".mha", "", ".zip"])
 def test_cli_end_to_end(tmp_path, label_ext):
     from biomedisa_amd.__main__ import main
     from biomedisa_amd.interpolation import smart_interpolation
@@ -76,6 +76,10 @@ def test_cli_end_to_end(tmp_path, label_ext):
     if label_ext == ".am":
         Z, Y, X = vol.labels.shape
         bio.save_data(lab_path, vol.labels, header=(_AM_HEADER % (X, Y, Z, X, Y, Z)).encode())
+    elif label_ext in ("", ".zip"):
+        # a stack of PNG slices, named so that natural order differs from plain order
+        names = ["lab_%d.png" % k for k in range(vol.labels.shape[0])]
+        bio.save_data(lab_path, vol.labels, header={"slices": names, "dtype": np.uint8}, file_type=label_ext)
     else:
         bio.save_data(lab_path, vol.labels)
     _say("cli %s" % label_ext)
@@ -87,6 +91,8 @@ def test_cli_end_to_end(tmp_path, label_ext):
     for k in ("regular", "smooth", "uncertainty", "cleaned", "smooth_cleaned", "filled", "cleaned_filled"):
         got = bio.load_data(files[k])[0]
         np.testing.assert_array_equal(got, want[k], err_msg=k)
+    if label_ext in ("", ".zip"):
+        assert bio.read_header(files["regular"])["slices"][:3] == ["lab_0.png", "lab_1.png", "lab_2.png"]
     if label_ext == ".am":
         with open(files["regular"], "rb") as f:
             assert b"Tissue { Color 0.1 0.6 0.3, id 1 }" in f.read()

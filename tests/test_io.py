@@ -343,3 +343,109 @@ def test_metaimage_hand_built(tmp_path):
             b"CompressedData = True\nElementDataFile = d.raw\n")
     (tmp_path / "h.mhd").write_bytes(head)
     np.testing.assert_array_equal(bio.load_data(str(tmp_path / "h.mhd"))[0], vol)
+
+
+# --- slice stacks (directories, zip archives; TIFF and PNG slices) ---------
+
+def _png_filtered(img, filters):
+    """A PNG whose rows use the given filter types, filtered here from the PNG
+    specification (section 9), independent of the reader."""
+    img = np.ascontiguousarray(img)
+    H, W = img.shape[:2]
+    raw = img.astype(img.dtype.newbyteorder(">")).view(np.uint8).reshape(H, -1).astype(np.int64)
+    bpp = img.dtype.itemsize * (img.shape[2] if img.ndim == 3 else 1)
+    rows = []
+    for r in range(H):
+        t = filters[r % len(filters)]
+        out = []
+        for i in range(raw.shape[1]):
+            x = raw[r, i]
+            a = raw[r, i - bpp] if i >= bpp else 0
+            b = raw[r - 1, i] if r > 0 else 0
+            c = raw[r - 1, i - bpp] if r > 0 and i >= bpp else 0
+            if t == 0:
+                p = 0
+            elif t == 1:
+                p = a
+            elif t == 2:
+                p = b
+            elif t == 3:
+                p = (a + b) // 2
+            else:
+                q = a + b - c
+                pa, pb, pc = abs(q - a), abs(q - b), abs(q - c)
+                p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+            out.append((x - p) % 256)
+        rows.append(bytes([t]) + bytes(out))
+    depth = 8 * img.dtype.itemsize
+    ctype = 4 if img.ndim == 3 else 0
+
+    def chunk(typ, body):
+        return struct.pack(">I", len(body)) + typ + body + struct.pack(">I", zlib.crc32(typ + body) & 0xFFFFFFFF)
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", W, H, depth, ctype, 0, 0, 0))
+            + chunk(b"IDAT", zlib.compress(b"".join(rows))) + chunk(b"IEND", b""))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_png_every_filter_type(dtype):
+    rng = np.random.default_rng(8)
+    img = rng.integers(0, np.iinfo(dtype).max, (7, 9)).astype(dtype)
+    np.testing.assert_array_equal(bio._read_png(_png_filtered(img, [0, 1, 2, 3, 4])), img)
+    np.testing.assert_array_equal(bio._read_png(_png_filtered(img, [4, 3, 2, 1])), img)
+    ga = np.stack([img, img[::-1]], axis=2)            # gray + alpha: the alpha is dropped
+    np.testing.assert_array_equal(bio._read_png(_png_filtered(ga, [4, 1])), img)
+    np.testing.assert_array_equal(bio._read_png(bio._png_bytes(img)), img)
+
+
+def test_slice_directory_natural_order(tmp_path):
+    vol = (np.random.default_rng(9).random((12, 5, 6)) * 250).astype(np.uint8)
+    d = tmp_path / "stack"
+    d.mkdir()
+    for k in range(12):
+        (d / ("img_%d.png" % k)).write_bytes(_png_filtered(vol[k], [k % 5]))
+    (d / ".hidden.png").write_bytes(b"not an image")
+    (d / "notes.txt").write_text("ignored")
+    got, ext = bio.load_data(str(d))
+    assert ext == ""
+    np.testing.assert_array_equal(got, vol)             # img_2 before img_10
+    hdr = bio.read_header(str(d))
+    assert hdr["slices"][:3] == ["img_0.png", "img_1.png", "img_2.png"] and hdr["dtype"] == np.uint8
+
+
+def test_slice_zip_and_results(tmp_path):
+    import zipfile
+    vol = (np.random.default_rng(10).random((3, 4, 5)) * 1000).astype(np.uint16)
+    with zipfile.ZipFile(tmp_path / "s.zip", "w") as z:
+        for k in range(3):
+            bio._write_tiff(str(tmp_path / "t.tif"), vol[k], compress=bool(k % 2))
+            z.write(tmp_path / "t.tif", "slice%02d.tif" % k)
+    got, ext = bio.load_data(str(tmp_path / "s.zip"))
+    assert ext == ".zip"
+    np.testing.assert_array_equal(got, vol)
+    hdr = bio.read_header(str(tmp_path / "s.zip"))
+    res = (vol > 500).astype(np.uint8)
+    # results keep the label stack's names and sample type (biomedisa_helper.py:758-783)
+    bio.save_data(str(tmp_path / "final.s.zip"), res, True, header=hdr)
+    assert sorted(zipfile.ZipFile(tmp_path / "final.s.zip").namelist()) == ["slice00.tif", "slice01.tif",
+                                                                           "slice02.tif"]
+    back = bio.load_data(str(tmp_path / "final.s.zip"))[0]
+    assert back.dtype == np.uint16
+    np.testing.assert_array_equal(back, res)
+    bio.save_data(str(tmp_path / "final.dir"), res, True, header=hdr, file_type="")
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "final.dir"))[0], res)
+
+
+def test_slice_errors(tmp_path):
+    d = tmp_path / "mixed"
+    d.mkdir()
+    (d / "a.png").write_bytes(bio._png_bytes(np.zeros((4, 5), np.uint8)))
+    (d / "b.png").write_bytes(bio._png_bytes(np.zeros((4, 6), np.uint8)))
+    with pytest.raises(ValueError):
+        bio.load_data(str(d))
+    e = tmp_path / "dicom"
+    e.mkdir()
+    (e / "x.dcm").write_bytes(b"DICM")
+    with pytest.raises(NotImplementedError):
+        bio.load_data(str(e))
+    with pytest.raises(ValueError):
+        bio._read_png(b"\x89PNG\r\n\x1a\n" + b"\0" * 20)
