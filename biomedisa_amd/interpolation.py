@@ -149,8 +149,8 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
                         key=DEFAULT_KEY, thr=0.9, brad=3, srad=6, far=48, far_walks=1, repair=2, repair_min=10,
-                        repair_far_walks=None, repair_far=24, walk_opts=None, repair_walk_opts=None, timings=None,
-                        keep=False):
+                        repair_far_walks=None, repair_far=24, plain_above=0.5, walk_opts=None,
+                        repair_walk_opts=None, timings=None, keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
 
     The network proposes a label and a confidence for every voxel.  U = voxels
@@ -171,6 +171,10 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     `repair_far` (None: far), whose seeds walk `repair_far_walks` walks (None:
     far_walks); repair_walk_opts overrides walk_opts for it.  GUIDED_ROBUST
     holds slower, more conservative settings.
+    plain_above: when more than this fraction of the seeds falls in the near
+    set (the proposal is uncertain almost everywhere, as on volumes unlike its
+    training data), the plain pipeline runs instead: its result is exact and
+    costs little more (None: never).
     walk_opts: as for run_pipeline.  The result takes the walker's argmax on U and
     the network's label elsewhere ('smooth': the smoothed walker argmax on U).
     Returns the dict of run_pipeline plus 'walked_seeds', 'ring_seeds',
@@ -215,6 +219,16 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
 
     sub, nsub, nring = walkers(need, far_walks)
     eng.sync()
+    if plain_above is not None and nseeds and nsub > plain_above * nseeds:
+        out = run_pipeline(vol, nlabels, values, nbrw, sorw, allaxis, smooth, key=key, keep_hits=keep,
+                           walk_opts=walk_opts, timings=t)
+        out.update(walked_seeds=nseeds, ring_seeds=0, repair_seeds=0, repair_ring_seeds=0, repair_voxels=0,
+                   fallback=True)
+        if keep:
+            every = eng.upload(np.ones(V, np.uint8))     # every voxel carries the walker's label
+            out.update(pred=pred, U=every, need=every)
+        t.update(proposal_s=time.perf_counter() - t1)
+        return out
     t2 = time.perf_counter()
     hits = eng.alloc(nlabels * V * 4)
     hits.zero()
@@ -255,7 +269,8 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
         U2.free()
     t3r = time.perf_counter()
     out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "repair_seeds": nrep,
-           "repair_ring_seeds": nrep_ring, "repair_voxels": rep_vox, "accepted": accepted, "flags": flags}
+           "repair_ring_seeds": nrep_ring, "repair_voxels": rep_vox, "accepted": accepted, "flags": flags,
+           "fallback": False}
     values = np.ascontiguousarray(values, dtype=np.uint8)
     regular = eng.alloc(V)
     _hip.call("bmx_merge_guided", eng.ctx, hits.ptr, int(nlabels), V, values.ctypes.data, pred.ptr, U.ptr,

@@ -205,7 +205,8 @@ def test_guided_equals_plain_when_all_uncertain(eng):
     from biomedisa_amd.synth import structures_volume
     vol = structures_volume(shape=(48, 52, 56), seed=11, n_struct=4, radius=(4, 14), every=7, allaxis=True)
     p, g, dl, _ = _pipelines(eng, vol, 3, 150, 5, S.Net(eng, seed=1), thr=1.01, brad=1, srad=1, far=9,
-                             far_walks=3)
+                             far_walks=3, plain_above=None)
+    assert not g["fallback"]
     assert g["walked_seeds"] == g["nseeds"] == p["nseeds"] and g["ring_seeds"] == 0
     assert g["accepted"] == p["accepted"]
     np.testing.assert_array_equal(dl(g["regular"]), dl(p["regular"]))
@@ -222,7 +223,7 @@ def test_guided_exact_on_uncertain_set(eng, trained):
     net = S.net(0) if trained else S.Net(eng, seed=2)
     vol = structures_volume(shape=(56, 48, 60), seed=12, n_struct=5, radius=(4, 16), every=8, allaxis=False)
     sorw = 24
-    p, g, dl, values = _pipelines(eng, vol, 4, sorw, 0, net, thr=0.9, brad=2, srad=sorw, far=0)
+    p, g, dl, values = _pipelines(eng, vol, 4, sorw, 0, net, thr=0.9, brad=2, srad=sorw, far=0, plain_above=None)
     U = dl(g["U"]).astype(bool)
     greg, preg = dl(g["regular"]), dl(p["regular"])
     np.testing.assert_array_equal(greg[U], preg[U])
@@ -267,7 +268,7 @@ def test_repair_exact_on_repaired_set(eng):
     planted = idx.copy()
     cube = tuple(slice(k - 2, k + 3) for k in c)
     planted[cube] = (lab % (nol - 1)) + 1
-    kw = dict(nbrw=4, sorw=sorw, allaxis=False, thr=0.9, brad=0, srad=sorw, far=0, keep=True)
+    kw = dict(nbrw=4, sorw=sorw, allaxis=False, thr=0.9, brad=0, srad=sorw, far=0, keep=True, plain_above=None)
     g0 = run_pipeline_guided(dv, nol, values, _PlantedProposal(planted), repair=None, **kw)
     g1 = run_pipeline_guided(dv, nol, values, _PlantedProposal(planted), repair=2, repair_min=1, **kw)
     dl = lambda b: b.download(dv.dims, np.uint8)  # noqa: E731
@@ -291,10 +292,24 @@ def test_far_ring_estimates_skipped_walks(eng):
     vol = structures_volume(shape=(40, 44, 48), seed=14, n_struct=4, radius=(4, 12), every=6, allaxis=False)
     sorw = 60
     p, g, dl, values = _pipelines(eng, vol, 4, sorw, 0, S.Net(eng, seed=3), thr=0.9, brad=2, srad=0, far=200,
-                                  far_walks=4, repair=None)
+                                  far_walks=4, repair=None, plain_above=None)
     U = dl(g["U"]).astype(bool)
     assert g["walked_seeds"] + g["ring_seeds"] == g["nseeds"]
     np.testing.assert_array_equal(dl(g["regular"])[U], dl(p["regular"])[U])
+
+
+def test_plain_fallback_when_the_proposal_is_uncertain(eng):
+    """thr above 1 flags every voxel: the near set holds every seed, so the plain
+    pipeline runs instead and the result is the plain result."""
+    from biomedisa_amd import surrogate as S
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(40, 44, 48), seed=17, n_struct=4, radius=(4, 12), every=6, allaxis=True)
+    p, g, dl, _ = _pipelines(eng, vol, 3, 120, 4, S.Net(eng, seed=7), thr=1.01)
+    assert g["fallback"] and g["walked_seeds"] == g["nseeds"] and g["ring_seeds"] == 0
+    assert g["accepted"] == p["accepted"]
+    np.testing.assert_array_equal(dl(g["regular"]), dl(p["regular"]))
+    np.testing.assert_array_equal(dl(g["smooth"]), dl(p["smooth"]))
+    assert dl(g["U"]).all()
 
 
 def test_trained_network_gain(eng):
