@@ -11,7 +11,9 @@
 // marches along a chunk of 32 z planes; the planes z-1, z, z+1 of the tile plus
 // a 1-voxel halo rotate through three LDS slots and the next plane is fetched
 // into registers during the computation, so each input voxel is read from
-// memory about 1.15 times per pass.
+// memory about 1.15 times per pass.  k_smooth_pair does two rounds per pass
+// (a 2-voxel halo, the middle round kept in LDS); an odd count ends with one
+// k_smooth_step pass.
 #include "bmx_internal.h"
 
 #include <climits>
@@ -91,6 +93,40 @@ __device__ __forceinline__ void store_plane(float (*s)[LW], int tx, int ty, cons
     }
 }
 
+// phi + 0.01 kappa(phi) at an interior voxel from the planes z-1 (m), z (c) and
+// z+1 (p) held in LDS rows of width W, in the reference's operation order
+// (opencl_kernels.py:136-224).
+template <int W>
+__device__ __forceinline__ float curvature_update(const float (*m)[W], const float (*c)[W], const float (*p)[W],
+                                                  int ly, int lx)
+{
+    const float eps = 1e-10f;
+    const float pc = c[ly][lx];
+    const float xm = c[ly][lx - 1], xp = c[ly][lx + 1];
+    const float ym = c[ly - 1][lx], yp = c[ly + 1][lx];
+    const float zm = m[ly][lx], zp = p[ly][lx];
+    const float dx = (xm - xp) * 0.5f;
+    const float dxx = (xm - 2.0f * pc + xp);
+    const float dx2 = dx * dx;
+    const float dy = (ym - yp) * 0.5f;
+    const float dyy = (ym - 2.0f * pc + yp);
+    const float dy2 = dy * dy;
+    const float dz = (zm - zp) * 0.5f;
+    const float dzz = (zm - 2.0f * pc + zp);
+    const float dz2 = dz * dz;
+    const float dxy = (c[ly - 1][lx - 1] + c[ly + 1][lx + 1] - c[ly - 1][lx + 1] - c[ly + 1][lx - 1]) * 0.25f;
+    const float dxz = (p[ly][lx - 1] + m[ly][lx + 1] - p[ly][lx + 1] - m[ly][lx - 1]) * 0.25f;
+    const float dyz = (p[ly - 1][lx] + m[ly + 1][lx] - p[ly + 1][lx] - m[ly - 1][lx]) * 0.25f;
+    const float curv = (dxx * (dy2 + dz2)
+                        + dyy * (dx2 + dz2)
+                        + dzz * (dx2 + dy2)
+                        - 2.0f * dx * dy * dxy
+                        - 2.0f * dx * dz * dxz
+                        - 2.0f * dy * dz * dyz)
+                       / (dx2 + dy2 + dz2 + eps);
+    return pc + 0.01f * curv;
+}
+
 // One round phi_{t+1} = phi_t + 0.01 kappa(phi_t) over a 64x16 (x,y) tile and
 // a chunk of z planes.  Planes z-1, z, z+1 live in three LDS slots; the plane
 // z+2 is fetched into registers while plane z is computed.  Each thread
@@ -128,41 +164,140 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__res
             const int y = y0 + ty * RY + k;
             if (x < bx.x1 && y < bx.y1) {
                 const float pc = s[sc][ly][lx];
-                float res = pc;
-                if (inside_x && inside_z && y > 0 && y < Y - 1) {
-                    const float eps = 1e-10f;
-                    const float xm = s[sc][ly][lx - 1], xp = s[sc][ly][lx + 1];
-                    const float ym = s[sc][ly - 1][lx], yp = s[sc][ly + 1][lx];
-                    const float zm = s[sm][ly][lx], zp = s[sp][ly][lx];
-                    const float dx = (xm - xp) * 0.5f;
-                    const float dxx = (xm - 2.0f * pc + xp);
-                    const float dx2 = dx * dx;
-                    const float dy = (ym - yp) * 0.5f;
-                    const float dyy = (ym - 2.0f * pc + yp);
-                    const float dy2 = dy * dy;
-                    const float dz = (zm - zp) * 0.5f;
-                    const float dzz = (zm - 2.0f * pc + zp);
-                    const float dz2 = dz * dz;
-                    const float dxy = (s[sc][ly - 1][lx - 1] + s[sc][ly + 1][lx + 1]
-                                       - s[sc][ly - 1][lx + 1] - s[sc][ly + 1][lx - 1]) * 0.25f;
-                    const float dxz = (s[sp][ly][lx - 1] + s[sm][ly][lx + 1]
-                                       - s[sp][ly][lx + 1] - s[sm][ly][lx - 1]) * 0.25f;
-                    const float dyz = (s[sp][ly - 1][lx] + s[sm][ly + 1][lx]
-                                       - s[sp][ly + 1][lx] - s[sm][ly - 1][lx]) * 0.25f;
-                    const float curv = (dxx * (dy2 + dz2)
-                                        + dyy * (dx2 + dz2)
-                                        + dzz * (dx2 + dy2)
-                                        - 2.0f * dx * dy * dxy
-                                        - 2.0f * dx * dz * dxz
-                                        - 2.0f * dy * dz * dyz)
-                                       / (dx2 + dy2 + dz2 + eps);
-                    res = pc + 0.01f * curv;
-                }
+                const float res = (inside_x && inside_z && y > 0 && y < Y - 1)
+                                      ? curvature_update<LW>(s[sm], s[sc], s[sp], ly, lx)
+                                      : pc;
                 out[(int64_t)z * sz + (int64_t)y * sy + x] = res;
             }
         }
         __syncthreads();
         const int t = sm; sm = sc; sc = sp; sp = t;
+    }
+}
+
+// Two rounds in one pass: phi_{t+2} from phi_t over a 64x16 (x,y) tile and a
+// chunk of z planes.  Round t+1 is computed on the tile grown by one voxel in x
+// and y (its planes z-1..z+1 rotate through three LDS slots, like the input's),
+// so round t+2 reads only values of its own block; each input voxel is read
+// about 1.5 times per two rounds.  Every voxel follows the single-round rules:
+// 0 outside the box, copied on the volume boundary, the stencil elsewhere, so
+// the results are those of two k_smooth_step passes bit for bit.
+constexpr int PW = TX + 4, PH = TY + 4;        // input plane with a 2-voxel halo
+constexpr int kPairLoads = (PW * PH + kSmoothThreads - 1) / kSmoothThreads;
+constexpr int kPairZChunk = 32;               // output planes per block
+
+__device__ __forceinline__ int slot3(int z) { return (z + 3) % 3; }
+
+template <typename Tin>
+__global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__restrict__ in,
+                                                               float *__restrict__ out, int Z, int Y,
+                                                               int X, Box bx)
+{
+    __shared__ float L0[3][PH][PW];   // round t, planes z-1..z+1 (frame: y0-2.., x0-2..)
+    __shared__ float L1[3][PH][PW];   // round t+1, rows 1..PH-2, columns 1..PW-2
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int x0 = bx.x0 + blockIdx.x * TX, y0 = bx.y0 + blockIdx.y * TY;
+    const int zb = bx.z0 + blockIdx.z * kPairZChunk;
+    const int ze = min(bx.z1, zb + kPairZChunk);
+    const int64_t sy = X, sz = (int64_t)X * Y;
+
+    // Per-thread constants of the z-independent parts: the in-plane offset and
+    // in-plane validity (inside the volume and the box) of the thread's input
+    // voxels, and of the voxels it computes in each round.
+    auto plane_ok = [&](int y, int x) {
+        return y >= 0 && y < Y && x >= 0 && x < X && y >= bx.y0 && y < bx.y1 && x >= bx.x0 && x < bx.x1;
+    };
+    auto z_ok = [&](int z) { return z >= 0 && z < Z && z >= bx.z0 && z < bx.z1; };
+    uint32_t off[kPairLoads];   // y * X + x < Y * X <= V < 2^32
+    uint32_t okmask = 0;
+#pragma unroll
+    for (int k = 0; k < kPairLoads; ++k) {
+        const int i = tid + k * kSmoothThreads;
+        const int ly = i / PW, lx = i - (i / PW) * PW;
+        const int y = y0 - 2 + ly, x = x0 - 2 + lx;
+        const bool ok = i < PW * PH && plane_ok(y, x);
+        off[k] = ok ? (uint32_t)y * (uint32_t)X + (uint32_t)x : 0u;
+        okmask |= (ok ? 1u : 0u) << k;
+    }
+    // one round at (z, frame row, frame column) from three planes of the round
+    // before; pin: in-plane validity, pint: in-plane interior (0 < y < Y-1, 0 < x < X-1)
+    auto round_at = [&](const float (*m)[PW], const float (*c)[PW], const float (*p)[PW], bool zin, bool zint,
+                        bool pin, bool pint, int ly, int lx) -> float {
+        if (!(zin && pin)) return 0.0f;
+        return (zint && pint) ? curvature_update<PW>(m, c, p, ly, lx) : c[ly][lx];
+    };
+    auto interior = [&](int y, int x) { return y > 0 && y < Y - 1 && x > 0 && x < X - 1; };
+    // the thread's core voxels (rows 2 + ty*RY + k, column 2 + tx of the frame)
+    const int xc = x0 + tx;
+    bool core_in[RY], core_int[RY];
+#pragma unroll
+    for (int k = 0; k < RY; ++k) {
+        const int y = y0 + ty * RY + k;
+        core_in[k] = plane_ok(y, xc);
+        core_int[k] = interior(y, xc);
+    }
+    // the thread's ring voxel of round t+1 (rows 1 and PH-2, columns 1 and PW-2)
+    constexpr int kRing = 2 * (PW - 2) + 2 * TY;
+    int rly = 0, rlx = 0;
+    if (tid < PW - 2) {
+        rly = 1; rlx = 1 + tid;
+    } else if (tid < 2 * (PW - 2)) {
+        rly = PH - 2; rlx = 1 + tid - (PW - 2);
+    } else if (tid < 2 * (PW - 2) + TY) {
+        rly = 2 + tid - 2 * (PW - 2); rlx = 1;
+    } else if (tid < kRing) {
+        rly = 2 + tid - 2 * (PW - 2) - TY; rlx = PW - 2;
+    }
+    const bool ring_in = tid < kRing && plane_ok(y0 - 2 + rly, x0 - 2 + rlx);
+    const bool ring_int = interior(y0 - 2 + rly, x0 - 2 + rlx);
+
+    float regs[kPairLoads];
+    auto fetch = [&](int z) {
+        const bool zv = z_ok(z);
+        const Tin *plane = in + (int64_t)(zv ? z : 0) * sz;
+#pragma unroll
+        for (int k = 0; k < kPairLoads; ++k)
+            regs[k] = (zv && ((okmask >> k) & 1u)) ? ldf(plane, (int64_t)off[k]) : 0.0f;
+    };
+    fetch(zb - 2);
+    for (int q = zb - 2; q <= ze + 1; ++q) {
+        float *l0 = &L0[slot3(q)][0][0];
+#pragma unroll
+        for (int k = 0; k < kPairLoads; ++k) {
+            const int i = tid + k * kSmoothThreads;
+            if (i < PW * PH) l0[i] = regs[k];
+        }
+        __syncthreads();
+        if (q + 1 <= ze + 1) fetch(q + 1);
+        // round t+1 at plane q-1 over frame rows 1..PH-2, columns 1..PW-2
+        const int r1 = q - 1;
+        if (r1 >= zb - 1) {
+            const float (*m)[PW] = L0[slot3(r1 - 1)], (*c)[PW] = L0[slot3(r1)], (*p)[PW] = L0[slot3(r1 + 1)];
+            float (*o)[PW] = L1[slot3(r1)];
+            const bool zin = z_ok(r1), zint = r1 > 0 && r1 < Z - 1;
+#pragma unroll
+            for (int k = 0; k < RY; ++k) {
+                const int ly = 2 + ty * RY + k, lx = 2 + tx;
+                o[ly][lx] = round_at(m, c, p, zin, zint, core_in[k], core_int[k], ly, lx);
+            }
+            // the ring around the 64x16 core
+            if (tid < kRing) o[rly][rlx] = round_at(m, c, p, zin, zint, ring_in, ring_int, rly, rlx);
+        }
+        __syncthreads();
+        // round t+2 at plane q-2 over the 64x16 core, written out inside the box
+        const int r2 = q - 2;
+        if (r2 >= zb && r2 < ze) {
+            const float (*m)[PW] = L1[slot3(r2 - 1)], (*c)[PW] = L1[slot3(r2)], (*p)[PW] = L1[slot3(r2 + 1)];
+            const bool zint = r2 > 0 && r2 < Z - 1;
+#pragma unroll
+            for (int k = 0; k < RY; ++k) {
+                const int ly = 2 + ty * RY + k, lx = 2 + tx;
+                const int y = y0 + ty * RY + k;
+                if (core_in[k])
+                    out[(int64_t)r2 * sz + (int64_t)y * sy + xc] = round_at(m, c, p, true, zint, true, core_int[k],
+                                                                            ly, lx);
+            }
+        }
     }
 }
 
@@ -244,17 +379,28 @@ int launch_step(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, co
     return BMX_OK;
 }
 
-// iters >= 1 rounds starting from `in`; the result lands in *result
-// (one of a / b).
+template <typename Tin>
+int launch_pair(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, const Box &bx)
+{
+    dim3 grid((bx.x1 - bx.x0 + TX - 1) / TX, (bx.y1 - bx.y0 + TY - 1) / TY,
+              (bx.z1 - bx.z0 + kPairZChunk - 1) / kPairZChunk);
+    k_smooth_pair<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X, bx);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+// iters >= 1 rounds starting from `in`, two per pass and a single pass for an
+// odd count; the result lands in *result (one of a / b).
 template <typename Tin>
 int smooth_rounds(bmx_ctx *ctx, const Tin *in, float *a, float *b, int Z, int Y, int X, int iters,
                   const Box &bx, float **result)
 {
-    int rc = launch_step<Tin>(ctx, in, a, Z, Y, X, bx);
+    int rc = iters >= 2 ? launch_pair<Tin>(ctx, in, a, Z, Y, X, bx) : launch_step<Tin>(ctx, in, a, Z, Y, X, bx);
     if (rc) return rc;
     float *cur = a, *nxt = b;
-    for (int it = 1; it < iters; ++it) {
-        rc = launch_step<float>(ctx, cur, nxt, Z, Y, X, bx);
+    for (int it = iters >= 2 ? 2 : 1; it < iters; it += 2) {
+        rc = iters - it >= 2 ? launch_pair<float>(ctx, cur, nxt, Z, Y, X, bx)
+                             : launch_step<float>(ctx, cur, nxt, Z, Y, X, bx);
         if (rc) return rc;
         float *t = cur; cur = nxt; nxt = t;
     }
@@ -275,8 +421,9 @@ int bmx_smooth(bmx_ctx *ctx, float *phi, float *tmp, int64_t Z, int64_t Y, int64
     BMX_CHECK(iters >= 0, "bmx_smooth: negative iters");
     float *cur = phi, *nxt = tmp;
     const Box whole{0, (int)Z, 0, (int)Y, 0, (int)X};
-    for (int it = 0; it < iters; ++it) {
-        rc = launch_step<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X, whole);
+    for (int it = 0; it < iters; it += 2) {
+        rc = iters - it >= 2 ? launch_pair<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X, whole)
+                             : launch_step<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X, whole);
         if (rc) return rc;
         float *t = cur; cur = nxt; nxt = t;
     }

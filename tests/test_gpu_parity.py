@@ -206,6 +206,38 @@ def test_smooth_bitexact_tall(eng):
                                   orc.smooth(phi, 3).view(np.uint32))
 
 
+@pytest.mark.parametrize("iters", [3, 6, 7])
+def test_smooth_argmax_small_boxes_bitexact(eng, iters):
+    """Labels whose hits sit in small clusters: each is smoothed in its own box
+    (hits box grown by iters, clipped to the volume), two rounds per pass and a
+    single pass for an odd count, against the oracle's whole-volume rounds."""
+    rng = np.random.default_rng(30 + iters)
+    shape = (40, 45, 140)
+    nl = 4
+    h = np.zeros((nl,) + shape, np.uint32)
+    for l, (z, y, x) in enumerate([(20, 22, 70), (1, 2, 3), (38, 40, 135), (10, 30, 100)]):
+        sl = tuple(slice(max(0, c - 4), c + 5) for c in (z, y, x))
+        h[l][sl] = rng.integers(0, 30, h[l][sl].shape)
+    values = np.array([0, 2, 4, 9], np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(h[0].size)
+    eng.smooth_argmax(hb, nl, shape, iters, values, out)
+    phi = np.stack([orc.smooth(x.astype(np.float32), iters) for x in h])
+    np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
+
+
+@pytest.mark.parametrize("iters", [2, 3, 4])
+def test_smooth_pairs_bitexact_wide(eng, iters):
+    """A volume wider than one tile in x and y and taller than one z chunk."""
+    shape = (70, 35, 130)
+    phi = (np.random.default_rng(40 + iters).random(shape) * 20).astype(np.float32)
+    a = eng.upload(phi)
+    b = eng.alloc(phi.nbytes)
+    eng.smooth(a, b, shape, iters)
+    np.testing.assert_array_equal(a.download(shape, np.float32).view(np.uint32),
+                                  orc.smooth(phi, iters).view(np.uint32))
+
+
 def test_smooth_argmax_and_uncertainty_bitexact(eng):
     rng = np.random.default_rng(4)
     shape = (21, 30, 66)
