@@ -117,6 +117,34 @@ __global__ void k_dilate_axis(const uint8_t *__restrict__ in, int Z, int Y, int 
     out[v] = m ? 1 : 0;
 }
 
+// The same dilation along z (axis 0) or y (axis 1) by sweeps: one thread per
+// line, a forward pass marking voxels within r after a set voxel and a backward
+// pass marking those within r before one (adjacent threads own adjacent x, so
+// every step is a coalesced row access).  out != in.
+__global__ void k_dilate_lines(const uint8_t *__restrict__ in, int Z, int Y, int X, int axis, int r,
+                               uint8_t *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nlines = axis == 0 ? (int64_t)Y * X : (int64_t)Z * X;
+    if (t >= nlines) return;
+    const int n = axis == 0 ? Z : Y;
+    const int64_t st = axis == 0 ? (int64_t)X * Y : (int64_t)X;
+    // first voxel of the line: (0, y, x) for z lines, (z, 0, x) for y lines
+    const int64_t base = axis == 0 ? t : (t / X) * (int64_t)X * Y + t % X;
+    int last = -r - 1;
+#pragma unroll 8
+    for (int i = 0; i < n; ++i) {
+        if (in[base + i * st]) last = i;
+        out[base + i * st] = (i - last <= r) ? 1 : 0;
+    }
+    int next = n + r;
+#pragma unroll 8
+    for (int i = n - 1; i >= 0; --i) {
+        if (in[base + i * st]) next = i;
+        if (next - i <= r) out[base + i * st] = 1;
+    }
+}
+
 __global__ void k_or(uint8_t *__restrict__ a, const uint8_t *__restrict__ b, int64_t n)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -260,8 +288,8 @@ int dilate(bmx_ctx *ctx, uint8_t *buf, uint8_t *tmp, int Z, int Y, int X, int r)
     if (r <= 0) return BMX_OK;
     const int64_t V = (int64_t)Z * Y * X;
     k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 2, r, tmp);
-    k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(tmp, Z, Y, X, 1, r, buf);
-    k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 0, r, tmp);
+    k_dilate_lines<<<bmx_blocks((int64_t)Z * X, 256), 256, 0, ctx->stream>>>(tmp, Z, Y, X, 1, r, buf);
+    k_dilate_lines<<<bmx_blocks((int64_t)Y * X, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 0, r, tmp);
     BMX_LAUNCH_CHECK();
     BMX_HIP(hipMemcpyAsync(buf, tmp, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
     return BMX_OK;
