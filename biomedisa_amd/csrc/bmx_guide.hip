@@ -117,20 +117,21 @@ __global__ void k_dilate_axis(const uint8_t *__restrict__ in, int Z, int Y, int 
     out[v] = m ? 1 : 0;
 }
 
-// The same dilation along z (axis 0) or y (axis 1) by sweeps: one thread per
-// line, a forward pass marking voxels within r after a set voxel and a backward
-// pass marking those within r before one (adjacent threads own adjacent x, so
-// every step is a coalesced row access).  out != in.
+// The same dilation by sweeps: one thread per line along z (axis 0), y (1) or
+// x (2), a forward pass marking voxels within r after a set voxel and a
+// backward pass marking those within r before one.  Along z and y adjacent
+// threads own adjacent x, so every step is a coalesced row access; along x
+// each thread reads its own row, which pays for large r only.  out != in.
 __global__ void k_dilate_lines(const uint8_t *__restrict__ in, int Z, int Y, int X, int axis, int r,
                                uint8_t *__restrict__ out)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nlines = axis == 0 ? (int64_t)Y * X : (int64_t)Z * X;
+    const int64_t nlines = axis == 0 ? (int64_t)Y * X : (axis == 1 ? (int64_t)Z * X : (int64_t)Z * Y);
     if (t >= nlines) return;
-    const int n = axis == 0 ? Z : Y;
-    const int64_t st = axis == 0 ? (int64_t)X * Y : (int64_t)X;
-    // first voxel of the line: (0, y, x) for z lines, (z, 0, x) for y lines
-    const int64_t base = axis == 0 ? t : (t / X) * (int64_t)X * Y + t % X;
+    const int n = axis == 0 ? Z : (axis == 1 ? Y : X);
+    const int64_t st = axis == 0 ? (int64_t)X * Y : (axis == 1 ? (int64_t)X : 1);
+    // first voxel of the line: (0, y, x) for z lines, (z, 0, x) for y lines, (z, y, 0) for x lines
+    const int64_t base = axis == 0 ? t : (axis == 1 ? (t / X) * (int64_t)X * Y + t % X : t * X);
     int last = -r - 1;
 #pragma unroll 8
     for (int i = 0; i < n; ++i) {
@@ -287,7 +288,10 @@ int dilate(bmx_ctx *ctx, uint8_t *buf, uint8_t *tmp, int Z, int Y, int X, int r)
 {
     if (r <= 0) return BMX_OK;
     const int64_t V = (int64_t)Z * Y * X;
-    k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 2, r, tmp);
+    if (r >= 8)
+        k_dilate_lines<<<bmx_blocks((int64_t)Z * Y, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 2, r, tmp);
+    else
+        k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 2, r, tmp);
     k_dilate_lines<<<bmx_blocks((int64_t)Z * X, 256), 256, 0, ctx->stream>>>(tmp, Z, Y, X, 1, r, buf);
     k_dilate_lines<<<bmx_blocks((int64_t)Y * X, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 0, r, tmp);
     BMX_LAUNCH_CHECK();
