@@ -138,6 +138,89 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x
         }
 }
 
+// Forward for 16 output channels from a plain input of at most 48 channels
+// (the first encoder layer) with the input staged in LDS: a block owns a 32 (x)
+// by 4 (y) tile of output rows and marches along ZC planes of z; the input
+// planes z-1, z, z+1 of the tile plus a 1-voxel halo rotate through three LDS
+// slots, so each input voxel is read from memory about 1.6 times instead of
+// once per tap.  Wave w computes the 32 voxels of row y0 + w with the same MFMA
+// sequence as k_conv_fwd<1, false> (taps outer, 16-channel chunks inner), so
+// the results are bit-identical to it.
+constexpr int kZmTX = 32, kZmTY = 4, kZmZC = 16;
+constexpr int kZmPX = kZmTX + 2, kZmPY = kZmTY + 2, kZmPlane = kZmPX * kZmPY;
+constexpr int kZmMaxCi = 48;
+
+__host__ __device__ constexpr int zm_pitch(int Ci) { return Ci + 4; }   // bf16 per LDS voxel (bank spread)
+
+__global__ __launch_bounds__(256) void k_conv_fwd_zm(const uint16_t *__restrict__ x, Shape s, int Ci,
+                                                     const uint16_t *__restrict__ w,
+                                                     const float *__restrict__ bias,
+                                                     uint16_t *__restrict__ y, int relu)
+{
+    extern __shared__ __align__(16) uint16_t zm_lds[];
+    const int P = zm_pitch(Ci);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+    const int x0 = blockIdx.x * kZmTX, y0 = blockIdx.y * kZmTY;
+    const int nzc = (s.D + kZmZC - 1) / kZmZC;
+    const int n = blockIdx.z / nzc;
+    const int zb = (blockIdx.z % nzc) * kZmZC, ze = min(s.D, zb + kZmZC);
+    const int groups = Ci / 4;                       // 8-byte channel groups per voxel
+
+    // input plane z into LDS slot (z + 3) % 3: zeros outside the grid
+    auto load_plane = [&](int z) {
+        uint16_t *dst = zm_lds + ((z + 3) % 3) * kZmPlane * P;
+        for (int u = tid; u < kZmPlane * groups; u += 256) {
+            const int p = u / groups, c = (u - p * groups) * 4;
+            const int ly = p / kZmPX, lx = p - ly * kZmPX;
+            const int yy = y0 - 1 + ly, xx = x0 - 1 + lx;
+            uint2 v = make_uint2(0u, 0u);
+            if ((unsigned)z < (unsigned)s.D && (unsigned)yy < (unsigned)s.H && (unsigned)xx < (unsigned)s.W) {
+                const int64_t vox = (((int64_t)n * s.D + z) * s.H + yy) * s.W + xx;
+                v = *(const uint2 *)(x + vox * Ci + c);
+            }
+            *(uint2 *)(dst + p * P + c) = v;
+        }
+    };
+
+    const int yrow = y0 + wv;
+    const float bb = bias ? bias[r] : 0.0f;
+    load_plane(zb - 1);
+    load_plane(zb);
+    for (int z = zb; z < ze; ++z) {
+        load_plane(z + 1);
+        __syncthreads();
+        float4_t acc[2] = {float4_t{0.f, 0.f, 0.f, 0.f}, float4_t{0.f, 0.f, 0.f, 0.f}};
+        for (int tap = 0; tap < 27; ++tap) {
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const uint16_t *pl = zm_lds + ((z + dz + 3) % 3) * kZmPlane * P;
+            const uint16_t *a0 = pl + ((wv + 1 + dy) * kZmPX + (1 + r + dx)) * P + kq;
+            const uint16_t *a1 = a0 + 16 * P;
+            const uint16_t *wt = w + ((int64_t)r * 27 + tap) * Ci + kq;
+            for (int c0 = 0; c0 < Ci; c0 += 16) {
+                const short4_t av0 = ld4(a0 + c0), av1 = ld4(a1 + c0);
+                const short4_t b = ld4(wt + c0);
+                acc[0] = mfma16(av0, b, acc[0]);
+                acc[1] = mfma16(av1, b, acc[1]);
+            }
+        }
+        if (yrow < s.H) {
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int xx = x0 + rb * 16 + kq + i;
+                    if (xx >= s.W) continue;
+                    float val = acc[rb][i] + bb;
+                    if (relu) val = fmaxf(val, 0.0f);
+                    const int64_t v = (((int64_t)n * s.D + z) * s.H + yrow) * s.W + xx;
+                    y[v * 16 + r] = f2bf(val);
+                }
+        }
+        __syncthreads();
+    }
+}
+
 // Input gradient: dx[v][ci] = sum_tap sum_co dy[v - off(tap)][co] * w[co][tap][ci],
 // times (xact[v][ci] > 0) when xact is given (ReLU of the layer below).
 // wt: weights re-laid out as [Ci][27][Co].
@@ -504,6 +587,23 @@ int launch_fwd(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t 
     return BMX_OK;
 }
 
+// The 16-output-channel forward: staged in LDS (k_conv_fwd_zm) for a plain
+// input of at most 48 channels, where it measured 1.4x faster at 512^3 (first
+// encoder layer: 27.9 against 39.6 ms); k_conv_fwd otherwise (staging the
+// decoder's concatenated input measured slower: 50.1 against 44.4 ms).
+int launch_fwd16(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y,
+                 int relu, const uint16_t *xb = nullptr, int Ca = 0)
+{
+    if (Ci > kZmMaxCi || xb) return launch_fwd<1>(ctx, x, s, Ci, w, b, y, 16, relu, xb, Ca);
+    const int nzc = (s.D + kZmZC - 1) / kZmZC;
+    BMX_CHECK((int64_t)s.N * nzc < 65536, "conv3d: too many z chunks for one dispatch");
+    dim3 grid((unsigned)((s.W + kZmTX - 1) / kZmTX), (unsigned)((s.H + kZmTY - 1) / kZmTY), (unsigned)(s.N * nzc));
+    const size_t lds = (size_t)3 * kZmPlane * zm_pitch(Ci) * sizeof(uint16_t);
+    k_conv_fwd_zm<<<grid, 256, lds, ctx->stream>>>(x, s, Ci, w, b, y, relu);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
 template <int NB>
 int launch_dgrad(bmx_ctx *ctx, const uint16_t *dy, Shape s, int Co, const uint16_t *wt, const uint16_t *xact,
                  uint16_t *dx, int Ci)
@@ -540,7 +640,7 @@ int bmx_conv3d_fwd(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, 
     Shape s{N, D, H, W};
     if (Co % 64 == 0) return launch_fwd<4>(ctx, x, s, Ci, w, b, y, Co, relu);
     if (Co % 32 == 0) return launch_fwd<2>(ctx, x, s, Ci, w, b, y, Co, relu);
-    return launch_fwd<1>(ctx, x, s, Ci, w, b, y, Co, relu);
+    return launch_fwd16(ctx, x, s, Ci, w, b, y, relu);
 }
 
 int bmx_conv3d_fwd_cat(bmx_ctx *ctx, const uint16_t *a, int N, int D, int H, int W, int Ca, const uint16_t *b,
@@ -557,7 +657,7 @@ int bmx_conv3d_fwd_cat(bmx_ctx *ctx, const uint16_t *a, int N, int D, int H, int
     const int Ci = Ca + Cb;
     if (Co % 64 == 0) return launch_fwd<4>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
     if (Co % 32 == 0) return launch_fwd<2>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
-    return launch_fwd<1>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
+    return launch_fwd16(ctx, a, s, Ci, w, bias, y, relu, b, Ca);
 }
 
 int bmx_conv3d_dgrad(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, int Co, const uint16_t *w,

@@ -251,3 +251,38 @@ def test_conv_fwd_cat_equals_materialised(eng, ca, cb, co):
     np.testing.assert_array_equal(y2.download(shape + (co,), np.uint16), y1.download(shape + (co,), np.uint16))
     ref = O.conv3d(np.concatenate([a, O.up2(b)], -1), w, bias.astype(np.float64), relu=True)
     _close_bf16(down_bf16(y2, shape + (co,)), ref)
+
+
+@pytest.mark.parametrize("shape,ci,cat", [((2, 20, 9, 70), 32, False), ((1, 34, 10, 66), 16, False),
+                                          ((1, 34, 10, 66), 48, True), ((2, 18, 6, 40), 48, True)])
+def test_conv_fwd16_lds_bit_identical(eng, shape, ci, cat):
+    """The 16-output-channel forward (staged in LDS, k_conv_fwd_zm, for plain
+    inputs) runs the generic kernel's MFMA sequence per output channel, so it
+    equals channels 0..15 of a 32-channel convolution (generic kernel) whose
+    first 16 filters are its filters, bit for bit, across x/y tiles, z chunks
+    and N; the concatenated-input cases check the same of the generic kernel."""
+    rng = np.random.default_rng(sum(shape) + ci)
+    w32 = O.bf16_round(rng.standard_normal((32, 27, ci)) * 0.1)
+    b32 = rng.standard_normal(32).astype(np.float32)
+    nv = int(np.prod(shape))
+    wb32, bb32 = up_bf16(eng, w32), eng.upload(b32)
+    wb16, bb16 = up_bf16(eng, w32[:16]), eng.upload(b32[:16].copy())
+    y32, y16 = eng.alloc(nv * 32 * 2), eng.alloc(nv * 16 * 2)
+    if cat:
+        ca, cb = 16, ci - 16
+        half = (shape[0], shape[1] // 2, shape[2] // 2, shape[3] // 2)
+        a = O.bf16_round(rng.standard_normal(shape + (ca,)))
+        b = O.bf16_round(rng.standard_normal(half + (cb,)))
+        ab, bb = up_bf16(eng, a), up_bf16(eng, b)
+        _hip.call("bmx_conv3d_fwd_cat", eng.ctx, ab.ptr, *shape, ca, bb.ptr, cb, wb32.ptr, bb32.ptr, y32.ptr, 32, 1)
+        _hip.call("bmx_conv3d_fwd_cat", eng.ctx, ab.ptr, *shape, ca, bb.ptr, cb, wb16.ptr, bb16.ptr, y16.ptr, 16, 1)
+        ref = O.conv3d(np.concatenate([a, O.up2(b)], -1), w32[:16], b32[:16].astype(np.float64), relu=True)
+    else:
+        x = O.bf16_round(rng.standard_normal(shape + (ci,)))
+        xb = up_bf16(eng, x)
+        _hip.call("bmx_conv3d_fwd", eng.ctx, xb.ptr, *shape, ci, wb32.ptr, bb32.ptr, y32.ptr, 32, 0)
+        _hip.call("bmx_conv3d_fwd", eng.ctx, xb.ptr, *shape, ci, wb16.ptr, bb16.ptr, y16.ptr, 16, 0)
+        ref = O.conv3d(x, w32[:16], b32[:16].astype(np.float64), relu=False)
+    got16 = y16.download(shape + (16,), np.uint16)
+    np.testing.assert_array_equal(got16, y32.download(shape + (32,), np.uint16)[..., :16])
+    _close_bf16(O.from_bf16_bits(got16).astype(np.float64), ref)
