@@ -11,8 +11,9 @@
 //
 // MFMA tiling (v_mfma_f32_16x16x16bf16_1k): lane l supplies A[l%16][4(l/16)..+3]
 // and B[4(l/16)..+3][l%16] and receives D[4(l/16)+i][l%16], i = 0..3.  A wave
-// computes 32 voxels (two 16-voxel row blocks along x) by 16*NB channels; the
-// K loop runs over the 27 taps and 16-channel chunks of the input.
+// computes 16*RB voxels (RB 16-voxel row blocks along x; RB = 2 in the
+// gradient kernels, 4 or 8 in the forward) by 16*NB channels; the K loop runs
+// over the 27 taps and 16-channel chunks of the input.
 #include "bmx_internal.h"
 
 namespace {
@@ -49,15 +50,15 @@ struct Shape {
     int N, D, H, W;
 };
 
-// Forward: y = act(conv(x, w) + b).  grid.x: voxel groups of 128 (4 waves x
-// 32), grid.y: output-channel groups of 16*NB.  Each lane decodes its own
+// Forward: y = act(conv(x, w) + b).  grid.x: voxel groups of 64*RB (4 waves x
+// 16*RB), grid.y: output-channel groups of 16*NB.  Each lane decodes its own
 // voxel's coordinates, so any grid shape works; requires Ci % 16 == 0 and
 // Co % (16*NB) == 0.  With a second source xb (Cb channels on the half-size
 // grid), the input is concat(x, up2(xb)) with Ci = Ca + Cb, read in place: a
 // 16-channel chunk comes from x below Ca and from xb at the voxel's parent
 // (z/2, y/2, x/2) above, so neither the upsampled tensor nor the concatenation
 // is materialised.
-template <int NB, bool CAT>
+template <int NB, bool CAT, int RB = 2>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x, Shape s, int Ci,
                                                   const uint16_t *__restrict__ w,
                                                   const float *__restrict__ bias,
@@ -68,14 +69,14 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x
     const int Dh = s.D / 2, Hh = s.H / 2, Wh = s.W / 2;
     const int lane = threadIdx.x & 63;
     const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
-    const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+    const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * RB);
     if (v0 >= nvox) return;
     const int co0 = blockIdx.y * 16 * NB;
     const int r = lane & 15, kq = (lane >> 4) * 4;
-    int cz[2], cy[2], cx[2], cn[2];
-    bool vin[2];
+    int cz[RB], cy[RB], cx[RB], cn[RB];
+    bool vin[RB];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
+    for (int rb = 0; rb < RB; ++rb) {
         int64_t v = v0 + rb * 16 + r;
         vin[rb] = v < nvox;
         cx[rb] = (int)(v % s.W); v /= s.W;
@@ -83,17 +84,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x
         cz[rb] = (int)(v % s.D);
         cn[rb] = (int)(v / s.D);
     }
-    float4_t acc[2][NB];
+    float4_t acc[RB][NB];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[rb][nb] = float4_t{0.f, 0.f, 0.f, 0.f};
     for (int tap = 0; tap < 27; ++tap) {
         const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const uint16_t *src[2], *srcb[2];
-        bool ok[2];
+        const uint16_t *src[RB], *srcb[RB];
+        bool ok[RB];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
+        for (int rb = 0; rb < RB; ++rb) {
             const int z = cz[rb] + dz, yy = cy[rb] + dy, xx = cx[rb] + dx;
             ok[rb] = vin[rb] && (unsigned)z < (unsigned)s.D && (unsigned)yy < (unsigned)s.H &&
                      (unsigned)xx < (unsigned)s.W;
@@ -107,9 +108,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x
         }
         const uint16_t *wt = w + ((int64_t)(co0 + r) * 27 + tap) * Ci + kq;
         for (int c0 = 0; c0 < Ci; c0 += 16) {
-            short4_t a[2];
+            short4_t a[RB];
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
+            for (int rb = 0; rb < RB; ++rb) {
                 a[rb] = (CAT && c0 >= Ca) ? ld4(srcb[rb] + (c0 - Ca)) : ld4(src[rb] + c0);
                 if (!ok[rb]) a[rb] = short4_t{0, 0, 0, 0};
             }
@@ -117,12 +118,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x
             for (int nb = 0; nb < NB; ++nb) {
                 const short4_t b = ld4(wt + (int64_t)nb * 16 * 27 * Ci + c0);
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb) acc[rb][nb] = mfma16(a[rb], b, acc[rb][nb]);
+                for (int rb = 0; rb < RB; ++rb) acc[rb][nb] = mfma16(a[rb], b, acc[rb][nb]);
             }
         }
     }
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             const int co = co0 + nb * 16 + r;
@@ -577,12 +578,17 @@ template <int NB>
 int launch_fwd(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y,
                int Co, int relu, const uint16_t *xb = nullptr, int Ca = 0)
 {
+    // 16-voxel row blocks per wave: RB * NB independent accumulator chains in
+    // flight (the MFMA latency, not its throughput, bounds these layers); the
+    // counts are the fastest measured at 512^3 (two row blocks: 44 / 21 / 13 ms
+    // for NB = 1 / 2 / 4 in the decoder; these: 38 / 15.5 / 8.4 ms)
+    constexpr int RB = NB == 2 ? 8 : 4;
     const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
-    dim3 grid((unsigned)((nvox + 127) / 128), (unsigned)(Co / (16 * NB)));
+    dim3 grid((unsigned)((nvox + 64 * RB - 1) / (64 * RB)), (unsigned)(Co / (16 * NB)));
     if (xb)
-        k_conv_fwd<NB, true><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, xb, Ca);
+        k_conv_fwd<NB, true, RB><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, xb, Ca);
     else
-        k_conv_fwd<NB, false><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, nullptr, Ci);
+        k_conv_fwd<NB, false, RB><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, nullptr, Ci);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
