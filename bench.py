@@ -140,6 +140,7 @@ def main():
     ap.add_argument("--repair", type=int, default=2, help="repair radius of the guided method; -1 disables")
     ap.add_argument("--repair-min", type=int, default=10)
     ap.add_argument("--repair-far", type=int, default=24)
+    ap.add_argument("--repair-srad", type=int, default=3)
     ap.add_argument("--repair-far-walks", type=int, default=-1, help="-1: as --far-walks")
     ap.add_argument("--no-robust", action="store_true",
                     help="skip the second guided measurement with the robust preset")
@@ -226,6 +227,7 @@ def main():
         config.update(guided_seconds=g["sec"], walked_seed_fraction=g["walked"] / g["nseeds"],
                       ring_seed_fraction=g["ring"] / g["nseeds"], far=args.far, far_walks=args.far_walks,
                       repair=args.repair, repair_min=args.repair_min, repair_far=args.repair_far,
+                      repair_srad=args.repair_srad,
                       repair_far_walks=args.repair_far_walks if args.repair_far_walks >= 0 else args.far_walks,
                       repair_seed_fraction=g["repair"] / g["nseeds"], repair_voxels=g["repair_voxels"],
                       temperature=net.T, thr=args.thr, brad=args.brad, srad=args.srad,
@@ -303,7 +305,7 @@ def main():
 def repair_kw(args):
     """Repair options of the guided pipeline from the command line."""
     return dict(repair=None if args.repair < 0 else args.repair, repair_min=args.repair_min,
-                repair_far=args.repair_far,
+                repair_far=args.repair_far, repair_srad=args.repair_srad,
                 repair_far_walks=None if args.repair_far_walks < 0 else args.repair_far_walks)
 
 

@@ -31,7 +31,7 @@ DEFAULT_KEY = 0x5EED0B10D1CE0001
 GUIDED_WALK_OPTS = {"segment_steps": 128}
 # Guided parameters that trade speed for agreement with the plain walker:
 # wider near set, two ring walks, a wider repair ring (pass as `guide=`).
-GUIDED_ROBUST = {"srad": 8, "far_walks": 2, "repair": 2, "repair_far_walks": 1, "repair_far": 32}
+GUIDED_ROBUST = {"srad": 8, "far_walks": 2, "repair": 2, "repair_far_walks": 1, "repair_far": 32, "repair_srad": 8}
 _RNG = {"philox": _hip.RNG_PHILOX, "reference": _hip.RNG_REF}
 
 
@@ -149,7 +149,7 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
                         key=DEFAULT_KEY, thr=0.9, brad=3, srad=6, far=48, far_walks=1, repair=2, repair_min=10,
-                        repair_far_walks=None, repair_far=24, plain_above=0.5, walk_opts=None,
+                        repair_far_walks=None, repair_far=24, repair_srad=3, plain_above=0.5, walk_opts=None,
                         repair_walk_opts=None, timings=None, keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
 
@@ -165,8 +165,8 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     Repair (repair = a radius; None disables it): outside U, a voxel whose hits
     from this walk number at least `repair_min` and whose walker argmax is not the
     proposed label marks a confident network error.  Those voxels dilated by
-    `repair` (minus U) form U2; every seed within `srad` of U2 walks all its
-    walks again into a second hit volume, whose counts replace the first walk's
+    `repair` (minus U) form U2; every seed within `repair_srad` (None: srad)
+    of U2 walks all its walks again into a second hit volume, whose counts replace the first walk's
     on U2, and U2 joins U.  The repair walk has its own ring out to
     `repair_far` (None: far), whose seeds walk `repair_far_walks` walks (None:
     far_walks); repair_walk_opts overrides walk_opts for it.  GUIDED_ROBUST
@@ -203,15 +203,15 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
               tmp.ptr)
     ring = eng.alloc(V) if far > srad else None
 
-    def walkers(near, far_walks, far=far):
+    def walkers(near, far_walks, far=far, inner=srad):
         """Seeds in `near` with all walks, then the ring seeds out to `far` with
         far_walks weighted walks -> (list, near count, ring count)."""
         sub = eng.alloc(max(1, nseeds) * _hip.SEED_DTYPE.itemsize)
         _, nsub = eng.select_seeds(seeds, nseeds, near, out=sub)
         nring = 0
-        if ring is not None and far > srad:
+        if ring is not None and far > inner:
             # ring seeds follow the near seeds in the same list (the two sets are disjoint)
-            _hip.call("bmx_ring_mask", eng.ctx, near.ptr, Z, Y, X, int(far - srad), ring.ptr, tmp.ptr)
+            _hip.call("bmx_ring_mask", eng.ctx, near.ptr, Z, Y, X, int(far - inner), ring.ptr, tmp.ptr)
             tail = _hip.BufferView(sub, nsub * _hip.SEED_DTYPE.itemsize)
             _, nring = eng.select_seeds(seeds, nseeds, ring, out=tail)
             _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, nring, int(far_walks), nbrw // far_walks)
@@ -243,11 +243,12 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
         _hip.call("bmx_guided_disagree", eng.ctx, hits.ptr, int(nlabels), V, pred.ptr, U.ptr, int(repair_min),
                   U2.ptr)
         need2 = eng.alloc(V)
-        _hip.call("bmx_repair_masks", eng.ctx, U2.ptr, U.ptr, Z, Y, X, int(repair), int(srad), need2.ptr, tmp.ptr)
+        rsrad = srad if repair_srad is None else repair_srad
+        _hip.call("bmx_repair_masks", eng.ctx, U2.ptr, U.ptr, Z, Y, X, int(repair), int(rsrad), need2.ptr, tmp.ptr)
         rfw = far_walks if repair_far_walks is None else repair_far_walks
         if far > srad and (rfw < 1 or nbrw % rfw):
             raise PipelineError("repair_far_walks must divide nbrw")
-        sub2, nrep, nrep_ring = walkers(need2, rfw, far if repair_far is None else repair_far)
+        sub2, nrep, nrep_ring = walkers(need2, rfw, far if repair_far is None else repair_far, rsrad)
         need2.free()
         t["repair_masks_s"] = time.perf_counter() - t3
         if nrep:
