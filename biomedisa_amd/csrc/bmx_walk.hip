@@ -559,8 +559,9 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
     unsigned long long *acc = (unsigned long long *)ctx->d_counter;
     const bool run = nseeds > 0 && nbrw > 0 && sorw > 0;
     const int64_t nwalkers = nseeds * (int64_t)nbrw;
-    if (algo == BMX_WALK_AUTO) algo = nwalkers >= ((int64_t)1 << 16) ? BMX_WALK_TILED : BMX_WALK_DIRECT;
-    if (algo == BMX_WALK_AUTO && (weight > 1 || overrides)) algo = BMX_WALK_TILED;
+    // automatic choice: weights and per-seed overrides need the tiled walker
+    if (algo == BMX_WALK_AUTO)
+        algo = (weight > 1 || overrides || nwalkers >= ((int64_t)1 << 16)) ? BMX_WALK_TILED : BMX_WALK_DIRECT;
     if (rng != BMX_RNG_PHILOX) algo = BMX_WALK_DIRECT;
     BMX_CHECK((weight == 1 && !overrides) || (algo == BMX_WALK_TILED && rng == BMX_RNG_PHILOX),
               "bmx_walk: hit_weight / seed_overrides need the tiled walker and the Philox stream");

@@ -127,7 +127,8 @@ int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y,
              int rng, uint64_t key, uint32_t *hits, uint64_t *accepted);
 
 /* Walker algorithms (identical hit counts; they differ only in speed). */
-#define BMX_WALK_AUTO 0   /* tiled for >= 2^16 walkers (Philox), direct otherwise */
+#define BMX_WALK_AUTO 0   /* tiled for >= 2^16 walkers, hit weights or per-seed overrides
+                             (Philox), direct otherwise */
 #define BMX_WALK_DIRECT 1 /* one launch, each thread walks its whole walk */
 #define BMX_WALK_SORTED 2 /* lockstep segments, walkers re-sorted by brick between them */
 #define BMX_WALK_TILED 3  /* tile-owner rounds, hit counters privatised in LDS */

@@ -369,6 +369,33 @@ def test_seed_overrides_equal_separate_walks(eng):
     np.testing.assert_array_equal(both.download(shape, np.uint32), ref)
 
 
+def test_auto_walker_with_overrides_on_few_seeds(eng):
+    """Fewer than 2^16 walkers with per-seed overrides: the automatic choice takes
+    the tiled walker (the direct one has no weights) and the hits equal the
+    explicitly tiled walk."""
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(24, 26, 28), seed=33, n_struct=2, radius=(3, 6), every=8, allaxis=False)
+    img = np.ascontiguousarray(np.pad(vol.img, 1))
+    lab = np.ascontiguousarray(np.pad(vol.labels, 1))
+    dims, V = img.shape, img.size
+    ib, lb = eng.upload(img), eng.upload(lab)
+    flags = eng.plane_flags(lb, dims, False)
+    seeds, n = eng.seeds(lb, ib, _hip.IMG_U8, dims, flags)
+    nl, nbrw, sorw, key = 3, 4, 150, 11
+    m = min(n, 300)
+    assert m * nbrw < 1 << 16
+    tail = _hip.BufferView(seeds, (m // 2) * _hip.SEED_DTYPE.itemsize)
+    _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, m - m // 2, 2, 2)
+    h_auto, h_tiled = eng.alloc(nl * V * 4), eng.alloc(nl * V * 4)
+    h_auto.zero()
+    h_tiled.zero()
+    a1 = eng.walk(ib, _hip.IMG_U8, dims, seeds, m, nl, nbrw, sorw, _hip.RNG_PHILOX, key, h_auto, seed_overrides=True)
+    a2 = eng.walk(ib, _hip.IMG_U8, dims, seeds, m, nl, nbrw, sorw, _hip.RNG_PHILOX, key, h_tiled,
+                  seed_overrides=True, algorithm=_hip.WALK_TILED)
+    assert a1 == a2 > 0
+    np.testing.assert_array_equal(h_auto.download((nl,) + dims, np.uint32), h_tiled.download((nl,) + dims, np.uint32))
+
+
 @pytest.mark.parametrize("w", [16, 20])
 def test_tile_counter_overflow_path(eng, w):
     """Heavy weights and many walks from two seeds in a small uniform volume push
