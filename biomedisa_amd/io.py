@@ -40,7 +40,11 @@ _TYPES = {1: ("B", 1), 2: ("s", 1), 3: ("H", 2), 4: ("I", 4), 5: ("II", 8), 6: (
 
 def _read_tiff(path):
     with open(path, "rb") as f:
-        buf = f.read()
+        return _parse_tiff(f.read(), path)
+
+
+def _parse_tiff(buf, path="TIFF"):
+    """TIFF file bytes -> (pages, rows, columns) array."""
     bo = {b"II": "<", b"MM": ">"}.get(buf[:2])
     if bo is None:
         raise ValueError("%s: not a TIFF file" % path)
@@ -90,15 +94,16 @@ def _read_tiff(path):
         dt = np.dtype(bo + kind + str(bps // 8)) if bps >= 8 else None
         if dt is None:
             raise NotImplementedError("%s: %d bits per sample" % (path, bps))
-        raw = b""
+        parts = []
         for o, c in zip(t[_OFFS], t[_COUNTS]):
             chunk = buf[o:o + c]
             if comp == 1:
-                raw += chunk
+                parts.append(chunk)
             elif comp in (8, 32946):
-                raw += zlib.decompress(chunk)
+                parts.append(zlib.decompress(chunk))
             else:
                 raise NotImplementedError("%s: TIFF compression %d" % (path, comp))
+        raw = b"".join(parts)
         plane = np.frombuffer(raw[:w * h * dt.itemsize], dtype=dt).reshape(h, w)
         if pred == 2:
             plane = np.cumsum(plane, axis=1, dtype=plane.dtype)
@@ -111,6 +116,11 @@ def _read_tiff(path):
 def _write_tiff(path, arr, compress=True, bigtiff=None):
     """One page per z-plane, one strip per page; BigTIFF when bigtiff is set or,
     by default, when the data exceed 2 GB."""
+    with open(path, "wb") as f:
+        f.write(_tiff_bytes(arr, compress, bigtiff))
+
+
+def _tiff_bytes(arr, compress=True, bigtiff=None):
     arr = np.ascontiguousarray(arr)
     if arr.ndim == 2:
         arr = arr[None]
@@ -160,8 +170,7 @@ def _write_tiff(path, arr, compress=True, bigtiff=None):
         for a, b in zip(ifd_pos, ifd_pos[1:]):
             n = struct.unpack_from("<H", out, a)[0]
             struct.pack_into("<I", out, a + 2 + 12 * n, b)
-    with open(path, "wb") as f:
-        f.write(out)
+    return bytes(out)
 
 
 _NRRD_TYPES = {"uchar": "u1", "unsigned char": "u1", "uint8": "u1", "uint8_t": "u1",
@@ -523,45 +532,52 @@ _SLICE_TYPES = (".png", ".tif", ".tiff")
 _OTHER_SLICES = (".dcm", ".dicom", ".bmp", ".jpg", ".jpeg")
 
 
-def _slice_entries(path):
-    """-> sorted [(name, bytes)] of the slice files of a directory or zip archive."""
+def _slice_names(path):
+    """Sorted names of the slice files of a directory (full paths) or of a zip
+    archive (member names)."""
     import zipfile
-    entries = []
     if os.path.isdir(path):
-        for root, _, names in os.walk(path):
-            for n in names:
-                full = os.path.join(root, n)
-                entries.append((full, lambda full=full: open(full, "rb").read()))
+        names = [os.path.join(root, n) for root, _, files in os.walk(path) for n in files]
     else:
-        z = zipfile.ZipFile(path)
-        for info in z.infolist():
-            if not info.is_dir():
-                entries.append((info.filename, lambda info=info, z=z: z.read(info)))
-    keep = [(n, rd) for n, rd in entries if not os.path.basename(n).startswith(".")
+        with zipfile.ZipFile(path) as z:
+            names = [i.filename for i in z.infolist() if not i.is_dir()]
+    keep = [n for n in names if not os.path.basename(n).startswith(".")
             and os.path.splitext(n)[1].lower() in _SLICE_TYPES]
-    if not keep and any(os.path.splitext(n)[1].lower() in _OTHER_SLICES for n, _ in entries):
+    if not keep and any(os.path.splitext(n)[1].lower() in _OTHER_SLICES for n in names):
         raise NotImplementedError("%s: only TIFF and PNG slices are read" % path)
     if not keep:
         raise ValueError("%s: no image slices" % path)
-    keep.sort(key=lambda e: _natural_key(e[0]))
+    keep.sort(key=_natural_key)
     return keep
+
+
+def _slice_bytes(path):
+    """-> iterator over (name, file bytes) of the slices in natural order."""
+    import zipfile
+    names = _slice_names(path)
+    if os.path.isdir(path):
+        for n in names:
+            with open(n, "rb") as f:
+                yield n, f.read()
+    else:
+        with zipfile.ZipFile(path) as z:
+            for n in names:
+                yield n, z.read(n)
+
+
+def _decode_slice(name, buf):
+    if os.path.splitext(name)[1].lower() == ".png":
+        return _read_png(buf, name)
+    img = _parse_tiff(buf, name)
+    if img.shape[0] != 1:
+        raise NotImplementedError("%s: multi-page TIFF slice" % name)
+    return img[0]
 
 
 def _read_slices(path):
     planes, names = [], []
-    for name, rd in _slice_entries(path):
-        buf = rd()
-        if os.path.splitext(name)[1].lower() == ".png":
-            img = _read_png(buf, name)
-        else:
-            import tempfile
-            with tempfile.NamedTemporaryFile(suffix=".tif") as t:
-                t.write(buf)
-                t.flush()
-                img = _read_tiff(t.name)
-            if img.shape[0] != 1:
-                raise NotImplementedError("%s: multi-page TIFF slice" % name)
-            img = img[0]
+    for name, buf in _slice_bytes(path):
+        img = _decode_slice(name, buf)
         if planes and (img.shape != planes[0].shape or img.dtype != planes[0].dtype):
             raise ValueError("%s: slice %s differs in shape or type from the first" % (path, name))
         planes.append(img)
@@ -572,10 +588,7 @@ def _read_slices(path):
 def _slice_file_bytes(name, img, compress):
     if os.path.splitext(name)[1].lower() == ".png":
         return _png_bytes(img, compress)
-    import tempfile
-    with tempfile.NamedTemporaryFile(suffix=".tif") as t:
-        _write_tiff(t.name, img, compress)
-        return open(t.name, "rb").read()
+    return _tiff_bytes(img, compress)
 
 
 def _write_slices(path, arr, compress=True, header=None):
@@ -608,7 +621,12 @@ def read_header(path):
     if ext == ".am":
         return am_header(path)
     if os.path.isdir(path) or ext == ".zip":
-        return _read_slices(path)[1]
+        # the names of every slice and the sample type of the first
+        slices = _slice_bytes(path)
+        name, buf = next(slices)
+        slices.close()
+        return {"slices": [os.path.basename(n) for n in _slice_names(path)],
+                "dtype": _decode_slice(name, buf).dtype}
     return None
 
 
