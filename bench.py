@@ -133,7 +133,7 @@ def main():
     ap.add_argument("--smooth", type=int, default=100)
     ap.add_argument("--method", choices=["both", "plain", "guided"], default="both")
     ap.add_argument("--thr", type=float, default=0.9)
-    ap.add_argument("--brad", type=int, default=3)
+    ap.add_argument("--brad", type=int, default=4)
     ap.add_argument("--srad", type=int, default=6)
     ap.add_argument("--far", type=int, default=48)
     ap.add_argument("--far-walks", type=int, default=1)

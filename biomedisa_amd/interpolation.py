@@ -148,7 +148,7 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
-                        key=DEFAULT_KEY, thr=0.9, brad=3, srad=6, far=48, far_walks=1, repair=2, repair_min=10,
+                        key=DEFAULT_KEY, thr=0.9, brad=4, srad=6, far=48, far_walks=1, repair=2, repair_min=10,
                         repair_far_walks=None, repair_far=24, repair_srad=3, plain_above=0.5, walk_opts=None,
                         repair_walk_opts=None, timings=None, keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
