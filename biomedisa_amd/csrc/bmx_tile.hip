@@ -142,8 +142,11 @@ __global__ __launch_bounds__(256) void k_seed_walks(bmx_seed *__restrict__ seeds
 // Round bookkeeping in device memory, so that a round needs no host values
 // and a sequence of rounds can be captured in one graph: n = walkers in the
 // current state array, nb = non-empty buckets, bnext = next bucket to walk.
+// The bucket list holds the nheavy buckets of at least one walker per thread
+// at its front and the nlight others at its back (nb = nheavy + nlight), so a
+// round starts its longest buckets first and ends on short ones.
 struct RoundCtl {
-    uint32_t nb, bnext, n, done;
+    uint32_t nb, bnext, n, done, nheavy, nlight;
     // persistent walk kernels: first start and last end of the current round
     // (device wall clock), and the summed spans of the rounds so far
     unsigned long long t0, t1, ticks;
@@ -157,6 +160,8 @@ __global__ __launch_bounds__(256) void k_round_begin(uint32_t *__restrict__ coun
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->nb = 0u;
         ctl->bnext = 0u;
+        ctl->nheavy = 0u;
+        ctl->nlight = 0u;
     }
 }
 
@@ -164,7 +169,17 @@ __global__ __launch_bounds__(256) void k_round_begin(uint32_t *__restrict__ coun
 // the next round's n.
 __global__ void k_round_end(const uint32_t *__restrict__ counts, int64_t nkeys, RoundCtl *__restrict__ ctl)
 {
-    if (threadIdx.x == 0) ctl->n = counts[nkeys];
+    if (threadIdx.x == 0) {
+        ctl->n = counts[nkeys];
+        ctl->nb = ctl->nheavy + ctl->nlight;
+    }
+}
+
+// Slot in the bucket arrays of the round's bucket i (heavy ones from the front,
+// light ones from the back).
+__device__ __forceinline__ uint32_t bucket_slot(uint32_t i, uint32_t nheavy, uint32_t nkeys)
+{
+    return i < nheavy ? i : nkeys - 1u - (i - nheavy);
 }
 
 __global__ void k_set_count(RoundCtl *__restrict__ ctl, uint32_t n)
@@ -219,20 +234,24 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const uint4 *__restrict__ 
     }
 }
 
-// Non-empty buckets of the exclusive offsets off[0..nkeys] -> (key, offset, count).
+// Non-empty buckets of the exclusive offsets off[0..nkeys] -> (key, offset,
+// count): buckets of at least `heavy` walkers from the front of the arrays,
+// the others from the back (nheavy + nlight <= nkeys, so the two never meet).
 __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict__ off, int64_t nkeys,
                                                      uint32_t *bkey, uint32_t *boff, uint32_t *bcnt,
-                                                     RoundCtl *ctl)
+                                                     RoundCtl *ctl, uint32_t heavy)
 {
-    uint32_t *nb = &ctl->nb;
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0, o = 0;
     if (k < nkeys) {
         o = off[k];
         c = off[k + 1] - o;
     }
-    const uint32_t idx = agg_atomic_inc(nb, 0u, c > 0);
+    const bool is_heavy = c >= heavy;
+    const uint32_t ih = agg_atomic_inc(&ctl->nheavy, 0u, c > 0 && is_heavy);
+    const uint32_t il = agg_atomic_inc(&ctl->nlight, 0u, c > 0 && !is_heavy);
     if (c > 0) {
+        const uint32_t idx = is_heavy ? ih : (uint32_t)nkeys - 1u - il;
         bkey[idx] = (uint32_t)k;
         boff[idx] = o;
         bcnt[idx] = c;
@@ -327,8 +346,9 @@ template <typename T, int S, bool PERSIST>
 __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
     const T *__restrict__ img_all, TileGeom tg, const bmx_seed *__restrict__ seeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
-    const uint32_t *__restrict__ bcnt, RoundCtl *__restrict__ ctl, int sorw, int cap, uint32_t weight0,
-    int overrides, uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted)
+    const uint32_t *__restrict__ bcnt, RoundCtl *__restrict__ ctl, uint32_t nkeys, int sorw, int cap,
+    uint32_t weight0, int overrides, uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits,
+    unsigned long long *accepted)
 {
     using Sh = TileShape<T, S>;
     constexpr int C = Sh::C, H = Sh::H, R = C + 2 * H, RI = R + 2, NTH = Sh::NTH;
@@ -338,7 +358,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
     uint32_t *shits = (uint32_t *)(smem + tile_img_bytes<T, S>());
     __shared__ uint32_t next, bucket;
 
-    const uint32_t nb = ctl->nb;
+    const uint32_t nb = ctl->nb, nheavy = ctl->nheavy;
     if (PERSIST && threadIdx.x == 0) atomicMin(&ctl->t0, (unsigned long long)__builtin_amdgcn_s_memrealtime());
     while (true) {
         if (threadIdx.x == 0) {
@@ -350,8 +370,9 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
         // geometry lives in scalar registers
         const uint32_t bi = PERSIST ? __builtin_amdgcn_readfirstlane(bucket) : blockIdx.x;
         if (bi >= nb) break;
-        const uint32_t key = bkey[bi];
-        const uint32_t off = boff[bi], cnt = bcnt[bi];
+        const uint32_t slot = bucket_slot(bi, nheavy, nkeys);
+        const uint32_t key = bkey[slot];
+        const uint32_t off = boff[slot], cnt = bcnt[slot];
         const uint32_t label = key % tg.nlabels;
         const uint32_t tile = key / tg.nlabels;
         // the bucket's volume: geometry, image and hit block in volume-local indices
@@ -538,7 +559,7 @@ int round_bucket(bmx_ctx *ctx, const RoundArgs<T> &a, const uint4 *in, uint4 *ou
     int rc = scan_u32(ctx, a.counts, a.nkeys + 1);
     if (rc) return rc;
     k_bucket_list<<<bmx_blocks(a.nkeys, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys, a.bkey, a.boff, a.bcnt,
-                                                                     a.ctl);
+                                                                     a.ctl, (uint32_t)TileShape<T, S>::NTH);
     BMX_LAUNCH_CHECK();
     k_tile_scatter<C><<<grid, 256, 0, ctx->stream>>>(in, a.ctl, out, a.counts, a.tg, a.sorw);
     BMX_LAUNCH_CHECK();
@@ -551,8 +572,8 @@ template <typename T, int S, bool PERSIST>
 int round_walk(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *st, unsigned grid)
 {
     k_tile_walk<T, S, PERSIST><<<grid, TileShape<T, S>::NTH, tile_lds_bytes<T, S>(), ctx->stream>>>(
-        a.img, a.tg, a.seeds, st, a.bkey, a.boff, a.bcnt, a.ctl, a.sorw, a.cap, a.weight, a.overrides, a.k0,
-        a.k1, a.hits, a.acc);
+        a.img, a.tg, a.seeds, st, a.bkey, a.boff, a.bcnt, a.ctl, (uint32_t)a.nkeys, a.sorw, a.cap, a.weight,
+        a.overrides, a.k0, a.k1, a.hits, a.acc);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
