@@ -13,7 +13,7 @@ import sys
 
 
 def parser():
-    p = argparse.ArgumentParser(description="Biomedisa interpolation on one MI355X.",
+    p = argparse.ArgumentParser(prog="python -m biomedisa_amd", description="Biomedisa interpolation on one MI355X.",
                                 formatter_class=argparse.ArgumentDefaultsHelpFormatter)
     p.add_argument("path_to_data", type=str, metavar="PATH_TO_IMAGE", help="Location of image data")
     p.add_argument("path_to_labels", type=str, metavar="PATH_TO_LABELS", help="Location of label data")
