@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--srad", type=int, default=6)
     ap.add_argument("--far", type=int, default=48)
     ap.add_argument("--far-walks", type=int, default=1)
+    ap.add_argument("--mid", type=int, default=16, help="ring seeds within this radius walk --mid-walks; -1: none")
+    ap.add_argument("--mid-walks", type=int, default=2)
     ap.add_argument("--repair", type=int, default=2, help="repair radius of the guided method; -1 disables")
     ap.add_argument("--repair-min", type=int, default=10)
     ap.add_argument("--repair-far", type=int, default=24)
@@ -191,10 +193,11 @@ def main():
             return run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw,
                                        allaxis=True, smooth=args.smooth, thr=args.thr,
                                        brad=args.brad, srad=args.srad, far=args.far,
-                                       far_walks=args.far_walks, timings=tm, **repair_kw(args))
+                                       far_walks=args.far_walks, timings=tm, **guided_kw(args))
         out, tms, sec = time_pipeline(guided_step, eng, args.warmup, args.steps, "guided")
         res["guided"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], walked=out["walked_seeds"],
-                             ring=out["ring_seeds"], repair=out["repair_seeds"], repair_voxels=out["repair_voxels"],
+                             ring=out["ring_seeds"], mid_ring=out["mid_ring_seeds"], repair=out["repair_seeds"],
+                             repair_voxels=out["repair_voxels"],
                              repair_ring=out["repair_ring_seeds"], accepted=out["accepted"],
                              regular=out["regular"].download(dv.dims, np.uint8))
         if not args.no_robust:
@@ -226,6 +229,7 @@ def main():
         g = res["guided"]
         config.update(guided_seconds=g["sec"], walked_seed_fraction=g["walked"] / g["nseeds"],
                       ring_seed_fraction=g["ring"] / g["nseeds"], far=args.far, far_walks=args.far_walks,
+                      mid_ring_seed_fraction=g["mid_ring"] / g["nseeds"], mid=args.mid, mid_walks=args.mid_walks,
                       repair=args.repair, repair_min=args.repair_min, repair_far=args.repair_far,
                       repair_srad=args.repair_srad,
                       repair_far_walks=args.repair_far_walks if args.repair_far_walks >= 0 else args.far_walks,
@@ -270,8 +274,9 @@ def main():
     tm = rk["tms"][-1]
     walked = rk.get("walked", rk["nseeds"])
     rfw = args.repair_far_walks if args.repair_far_walks >= 0 else args.far_walks
-    steps_total = ((walked + (rk.get("repair") or 0)) * args.nbrw + (rk.get("ring") or 0) * args.far_walks
-                   + (rk.get("repair_ring") or 0) * rfw) * args.sorw
+    nmid = rk.get("mid_ring") or 0
+    steps_total = ((walked + (rk.get("repair") or 0)) * args.nbrw + ((rk.get("ring") or 0) - nmid) * args.far_walks
+                   + nmid * args.mid_walks + (rk.get("repair_ring") or 0) * rfw) * args.sorw
     walk_bytes = steps_total * 6 * 1 + rk["accepted"] * 8
     launches = max(int(tm["walk_launches"]), 1)
     kernel_s = tm["walk_kernel_ms"] / 1000.0
@@ -302,9 +307,10 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def repair_kw(args):
-    """Repair options of the guided pipeline from the command line."""
-    return dict(repair=None if args.repair < 0 else args.repair, repair_min=args.repair_min,
+def guided_kw(args):
+    """Inner-ring and repair options of the guided pipeline from the command line."""
+    return dict(mid=None if args.mid < 0 else args.mid, mid_walks=args.mid_walks,
+                repair=None if args.repair < 0 else args.repair, repair_min=args.repair_min,
                 repair_far=args.repair_far, repair_srad=args.repair_srad,
                 repair_far_walks=None if args.repair_far_walks < 0 else args.repair_far_walks)
 
@@ -504,7 +510,7 @@ def bench_large(args):
             net = surrogate.net(0)
             out = run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=0,
                                       thr=args.thr, brad=args.brad, srad=args.srad, far=args.far,
-                                      far_walks=args.far_walks, timings=tm, **repair_kw(args))
+                                      far_walks=args.far_walks, timings=tm, **guided_kw(args))
         eng.sync()
         sec = time.perf_counter() - t
         log("%s: %.1fs %s" % (method, sec, {k: round(v, 3) for k, v in tm.items()}))

@@ -63,6 +63,7 @@ SIGNATURES = {
     "bmx_hxbyterle_encode": (_i64, [_p, _i64, _p, _i64]),
     "bmx_png_unfilter": (_i, [_p, _i64, _i64, _i, _p]),
     "bmx_guided_disagree": (_i, [_p, _p, _i, _i64, _p, _p, C.c_uint32, _p]),
+    "bmx_mask_andnot": (_i, [_p, _p, _p, _i64]),
     "bmx_repair_masks": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
     "bmx_repair_hits": (_i, [_p, _p, _p, _i, _i64, _p, _p]),
     "bmx_select_seeds": (_i, [_p, _p, _i64, _p, _p, C.POINTER(_i64)]),

@@ -370,6 +370,18 @@ int bmx_ring_mask(bmx_ctx *ctx, const uint8_t *need, int Z, int Y, int X, int r,
     return BMX_OK;
 }
 
+int bmx_mask_andnot(bmx_ctx *ctx, uint8_t *a, const uint8_t *b, int64_t n)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(a && b && a != b, "bmx_mask_andnot: null or aliased pointers");
+    BMX_CHECK(n >= 0 && n < ((int64_t)1 << 32) - 256, "bmx_mask_andnot: n out of range");
+    if (n == 0) return BMX_OK;
+    k_andnot<<<bmx_blocks(n, 256), 256, 0, ctx->stream>>>(a, b, n);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
 int bmx_guided_disagree(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const uint8_t *pred,
                         const uint8_t *U, uint32_t minhits, uint8_t *D)
 {

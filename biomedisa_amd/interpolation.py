@@ -149,7 +149,8 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
 
 def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=False, smooth=0,
                         key=DEFAULT_KEY, thr=0.9, brad=4, srad=6, far=48, far_walks=1, repair=2, repair_min=10,
-                        repair_far_walks=None, repair_far=24, repair_srad=3, plain_above=0.5, walk_opts=None,
+                        repair_far_walks=None, repair_far=24, repair_srad=3, mid=16, mid_walks=None,
+                        plain_above=0.5, walk_opts=None,
                         repair_walk_opts=None, timings=None, keep=False):
     """Surrogate-guided pipeline on device-resident inputs.
 
@@ -160,7 +161,10 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     those walks are exactly the plain walker's.  Seeds in the ring from `srad`
     to `far` walk their first `far_walks` walks with every hit counted
     nbrw/far_walks times: an unbiased estimate of what they add to U
-    (far=0 disables the ring).  Near and ring seeds walk in one call (per-seed
+    (far=0 disables the ring); ring seeds within `mid` walk `mid_walks` walks
+    instead (mid=None: none; mid_walks=None: 2 when nbrw is even, else
+    far_walks), which lowers the noise of the estimate where it weighs most,
+    next to U.  Near and ring seeds walk in one call (per-seed
     walk counts and weights), so ring walkers fill the same tile buckets.
     Repair (repair = a radius; None disables it): outside U, a voxel whose hits
     from this walk number at least `repair_min` and whose walker argmax is not the
@@ -177,8 +181,9 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     costs little more (None: never).
     walk_opts: as for run_pipeline.  The result takes the walker's argmax on U and
     the network's label elsewhere ('smooth': the smoothed walker argmax on U).
-    Returns the dict of run_pipeline plus 'walked_seeds', 'ring_seeds',
-    'repair_seeds' and 'repair_voxels'.
+    Returns the dict of run_pipeline plus 'walked_seeds', 'ring_seeds' (of
+    which 'mid_ring_seeds' walk mid_walks walks), 'repair_seeds',
+    'repair_ring_seeds', 'repair_voxels' and 'fallback'.
     """
     from . import surrogate as S
     eng = vol.eng
@@ -188,6 +193,12 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
         raise PipelineError("guided mode supports at most %d labels" % S.L_MAX)
     if far > srad and (far_walks < 1 or nbrw % far_walks):
         raise PipelineError("far_walks must divide nbrw")
+    if mid_walks is None:
+        mid_walks = 2 if nbrw % 2 == 0 else far_walks
+    # the inner ring reaches min(mid, far); it is empty when mid <= srad
+    mid = None if far <= srad or mid is None or mid <= srad else min(mid, far)
+    if mid is not None and (mid_walks < 1 or nbrw % mid_walks):
+        raise PipelineError("mid_walks must divide nbrw")
     t = timings if timings is not None else {}
     t0 = time.perf_counter()
     flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis)
@@ -202,27 +213,37 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     _hip.call("bmx_guided_masks", eng.ctx, pred.ptr, unc.ptr, Z, Y, X, int(brad), int(srad), U.ptr, need.ptr,
               tmp.ptr)
     ring = eng.alloc(V) if far > srad else None
+    inner_ring = eng.alloc(V) if mid is not None else None
 
-    def walkers(near, far_walks, far=far, inner=srad):
+    def walkers(near, far_walks, far=far, inner=srad, mid=None):
         """Seeds in `near` with all walks, then the ring seeds out to `far` with
-        far_walks weighted walks -> (list, near count, ring count)."""
+        far_walks weighted walks; with `mid`, the ring seeds out to mid walk
+        mid_walks weighted walks instead -> (list, near count, ring count, of
+        which out to mid)."""
         sub = eng.alloc(max(1, nseeds) * _hip.SEED_DTYPE.itemsize)
         _, nsub = eng.select_seeds(seeds, nseeds, near, out=sub)
-        nring = 0
+        nring = nmid = 0
         if ring is not None and far > inner:
-            # ring seeds follow the near seeds in the same list (the two sets are disjoint)
+            # ring seeds follow the near seeds in the same list (the sets are disjoint)
             _hip.call("bmx_ring_mask", eng.ctx, near.ptr, Z, Y, X, int(far - inner), ring.ptr, tmp.ptr)
-            tail = _hip.BufferView(sub, nsub * _hip.SEED_DTYPE.itemsize)
-            _, nring = eng.select_seeds(seeds, nseeds, ring, out=tail)
-            _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, nring, int(far_walks), nbrw // far_walks)
-        return sub, nsub, nring
+            if mid is not None:
+                _hip.call("bmx_ring_mask", eng.ctx, near.ptr, Z, Y, X, int(mid - inner), inner_ring.ptr, tmp.ptr)
+                _hip.call("bmx_mask_andnot", eng.ctx, ring.ptr, inner_ring.ptr, V)
+                tail = _hip.BufferView(sub, nsub * _hip.SEED_DTYPE.itemsize)
+                _, nmid = eng.select_seeds(seeds, nseeds, inner_ring, out=tail)
+                _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, nmid, int(mid_walks), nbrw // mid_walks)
+            tail = _hip.BufferView(sub, (nsub + nmid) * _hip.SEED_DTYPE.itemsize)
+            _, nout = eng.select_seeds(seeds, nseeds, ring, out=tail)
+            _hip.call("bmx_seed_walks", eng.ctx, tail.ptr, nout, int(far_walks), nbrw // far_walks)
+            nring = nmid + nout
+        return sub, nsub, nring, nmid
 
-    sub, nsub, nring = walkers(need, far_walks)
+    sub, nsub, nring, nmid = walkers(need, far_walks, mid=mid)
     eng.sync()
     if plain_above is not None and nseeds and nsub > plain_above * nseeds:
         out = run_pipeline(vol, nlabels, values, nbrw, sorw, allaxis, smooth, key=key, keep_hits=keep,
                            walk_opts=walk_opts, timings=t)
-        out.update(walked_seeds=nseeds, ring_seeds=0, repair_seeds=0, repair_ring_seeds=0, repair_voxels=0,
+        out.update(walked_seeds=nseeds, ring_seeds=0, mid_ring_seeds=0, repair_seeds=0, repair_ring_seeds=0, repair_voxels=0,
                    fallback=True)
         if keep:
             every = eng.upload(np.ones(V, np.uint8))     # every voxel carries the walker's label
@@ -248,7 +269,7 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
         rfw = far_walks if repair_far_walks is None else repair_far_walks
         if far > srad and (rfw < 1 or nbrw % rfw):
             raise PipelineError("repair_far_walks must divide nbrw")
-        sub2, nrep, nrep_ring = walkers(need2, rfw, far if repair_far is None else repair_far, rsrad)
+        sub2, nrep, nrep_ring, _ = walkers(need2, rfw, far if repair_far is None else repair_far, rsrad)
         need2.free()
         t["repair_masks_s"] = time.perf_counter() - t3
         if nrep:
@@ -269,7 +290,7 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
         sub2.free()
         U2.free()
     t3r = time.perf_counter()
-    out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "repair_seeds": nrep,
+    out = {"nseeds": nseeds, "walked_seeds": nsub, "ring_seeds": nring, "mid_ring_seeds": nmid, "repair_seeds": nrep,
            "repair_ring_seeds": nrep_ring, "repair_voxels": rep_vox, "accepted": accepted, "flags": flags,
            "fallback": False}
     values = np.ascontiguousarray(values, dtype=np.uint8)
@@ -307,7 +328,7 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
 
     method='guided' runs the surrogate-guided pipeline (run_pipeline_guided;
     Philox stream only, no 'uncertainty'/'hits'); `guide` is a dict of its
-    thr/brad/srad/far/far_walks parameters.
+    keyword parameters (thr, brad, srad, far, far_walks, mid, repair, ...).
 
     path: 'auto' (default) takes the reference's small- or large-volume driver
     by its size rule (interpolation.py:192-196), 'small' or 'large' forces one;

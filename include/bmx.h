@@ -364,6 +364,9 @@ int bmx_seed_walks(bmx_ctx *ctx, bmx_seed *seeds, int64_t n, int walks, int weig
 int bmx_ring_mask(bmx_ctx *ctx, const uint8_t *need, int Z, int Y, int X, int r, uint8_t *ring, uint8_t *tmp);
 int bmx_merge_guided(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V,
                      const uint8_t *values, const uint8_t *pred, const uint8_t *U, uint8_t *out);
+/* a = a & ~b over n mask bytes (0/1), e.g. to split the far-field ring
+ * (no counterpart in the reference). */
+int bmx_mask_andnot(bmx_ctx *ctx, uint8_t *a, const uint8_t *b, int64_t n);
 /* Repair of the guided result (no counterpart in the reference).
  * D = !U & (sum of hits >= minhits, and > 0) & (first argmax of hits != pred). */
 int bmx_guided_disagree(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const uint8_t *pred,

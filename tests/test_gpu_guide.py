@@ -171,6 +171,17 @@ def test_ring_mask_bitexact(eng, r):
     np.testing.assert_array_equal(ring.download(dims, np.uint8), G.ring_mask(need, r))
 
 
+def test_mask_andnot(eng):
+    rng = np.random.default_rng(21)
+    n = 70001
+    a, b = (rng.random(n) < 0.5).astype(np.uint8), (rng.random(n) < 0.3).astype(np.uint8)
+    da, db = eng.upload(a), eng.upload(b)
+    _hip.call("bmx_mask_andnot", eng.ctx, da.ptr, db.ptr, n)
+    np.testing.assert_array_equal(da.download((n,), np.uint8), (a & (1 - b)).astype(np.uint8))
+    with pytest.raises(_hip.BmxError):
+        _hip.call("bmx_mask_andnot", eng.ctx, da.ptr, da.ptr, n)
+
+
 def test_select_seeds_set(eng):
     from biomedisa_amd.synth import ellipsoids_volume
     vol = ellipsoids_volume(shape=(40, 44, 48), seed=5, semi=(6, 12), every=6)
@@ -283,19 +294,47 @@ def test_repair_exact_on_repaired_set(eng):
     np.testing.assert_array_equal(greg[cube], preg[cube])
 
 
-def test_far_ring_estimates_skipped_walks(eng):
-    """With srad = 0 and a ring that reaches every seed, the hits are the first
-    walk of every seed counted nbrw times; with far_walks = nbrw the ring walks
-    are the plain walks themselves, so the result on U is the plain result."""
-    from biomedisa_amd import surrogate as S
+def _ring_case(eng, **guide):
+    """Plain and guided runs on a small volume whose proposal is the ground
+    truth with no voxel uncertain: U is the label boundaries dilated by one,
+    and with srad = 0 most seeds fall in the ring."""
     from biomedisa_amd.synth import structures_volume
     vol = structures_volume(shape=(40, 44, 48), seed=14, n_struct=4, radius=(4, 12), every=6, allaxis=False)
-    sorw = 60
-    p, g, dl, values = _pipelines(eng, vol, 4, sorw, 0, S.Net(eng, seed=3), thr=0.9, brad=2, srad=0, far=200,
-                                  far_walks=4, repair=None, plain_above=None)
-    U = dl(g["U"]).astype(bool)
+    values = _prep(vol)[2]
+    assert np.isin(vol.gt, values).all()
+    net = _PlantedProposal(np.pad(np.searchsorted(values, vol.gt), 1))
+    p, g, dl, _ = _pipelines(eng, vol, 4, 60, 0, net, thr=0.9, brad=1, srad=0, far=200, repair=None,
+                             plain_above=None, **guide)
     assert g["walked_seeds"] + g["ring_seeds"] == g["nseeds"]
-    np.testing.assert_array_equal(dl(g["regular"])[U], dl(p["regular"])[U])
+    assert 0 < g["walked_seeds"] < g["nseeds"] // 2
+    U = dl(g["U"]).astype(bool)
+    return g, dl(g["regular"])[U], dl(p["regular"])[U]
+
+
+def test_far_ring_estimates_skipped_walks(eng):
+    """With srad = 0 and a ring that reaches every seed, the ring seeds walk
+    their first far_walks walks, counted nbrw/far_walks times; with far_walks =
+    nbrw the ring walks are the plain walks themselves, so the result on U is
+    the plain result (and with one walk it is not)."""
+    g, got, want = _ring_case(eng, far_walks=4, mid=None)
+    assert g["mid_ring_seeds"] == 0
+    np.testing.assert_array_equal(got, want)
+    g, got, want = _ring_case(eng, far_walks=1, mid=None)
+    assert (got != want).any()
+
+
+@pytest.mark.parametrize("mid,far_walks", [(4, 4), (300, 1)])
+def test_inner_ring_split(eng, mid, far_walks):
+    """mid splits the ring into two disjoint seed sets (mid=4), or takes all of
+    it when it reaches past far (mid=300): with the inner ring walking all nbrw
+    walks and the outer ring far_walks, the seeds add up and, where every seed
+    walks every walk, the result on U is the plain result."""
+    g, got, want = _ring_case(eng, far_walks=far_walks, mid=mid, mid_walks=4)
+    if mid < 200:
+        assert 0 < g["mid_ring_seeds"] < g["ring_seeds"]
+    else:
+        assert g["mid_ring_seeds"] == g["ring_seeds"] > 0
+    np.testing.assert_array_equal(got, want)
 
 
 def test_plain_fallback_when_the_proposal_is_uncertain(eng):
@@ -324,8 +363,8 @@ def test_trained_network_gain(eng):
     res = {}
     for name, net in (("trained", S.net(0)), ("untrained", S.Net(eng, seed=5))):
         p, g, dl, values = _pipelines(eng, vol, 10, 4000, 0, net)
-        cost = (g["walked_seeds"] * 10 + g["ring_seeds"] + g["repair_seeds"] * 10 + g["repair_ring_seeds"]) \
-            / (10.0 * g["nseeds"])
+        cost = (g["walked_seeds"] * 10 + g["ring_seeds"] + g["mid_ring_seeds"] + g["repair_seeds"] * 10
+                + g["repair_ring_seeds"]) / (10.0 * g["nseeds"])
         d = per_label_dice(dl(g["regular"]), dl(p["regular"]))
         res[name] = (cost, min(d.values()))
         print(name, "walk cost %.3f of plain, min per-label Dice %.4f" % res[name])
