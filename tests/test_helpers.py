@@ -95,3 +95,29 @@ def test_batch_volumes_workload_shape():
         assert 2 <= int(va.gt.max()) <= 12
         zs = np.nonzero(va.labels.reshape(va.labels.shape[0], -1).any(axis=1))[0]
         assert len(zs) >= 2 and any(np.all(zs % k == 0) for k in range(6, 13))
+
+
+@pytest.mark.parametrize("kw", [dict(far_walks=3), dict(mid_walks=3), dict(mid_walks=0), dict(nbrw=5, mid_walks=2)])
+def test_guided_parameter_checks(kw):
+    """run_pipeline_guided rejects walk counts that do not divide nbrw before it
+    touches the device (the volume here has no engine at all)."""
+    from biomedisa_amd.interpolation import PipelineError, run_pipeline_guided
+
+    class NoDevice:
+        eng, dims, V = None, (4, 4, 4), 64
+
+    args = dict(nbrw=10, sorw=10)
+    args.update(kw)
+    with pytest.raises(PipelineError, match="must divide nbrw"):
+        run_pipeline_guided(NoDevice(), 2, np.arange(2, dtype=np.uint8), None, **args)
+
+
+def test_guided_label_limit():
+    from biomedisa_amd import surrogate as S
+    from biomedisa_amd.interpolation import PipelineError, run_pipeline_guided
+
+    class NoDevice:
+        eng, dims, V = None, (4, 4, 4), 64
+
+    with pytest.raises(PipelineError, match="at most"):
+        run_pipeline_guided(NoDevice(), S.L_MAX + 1, np.arange(S.L_MAX + 1, dtype=np.uint8), None)
