@@ -13,6 +13,11 @@ the formats this package reads and writes itself.
                  written with the label file's header, as the reference does
                  (amira_to_np/amira_data_stream.py:476-515), or a minimal one.
     .nii/.nii.gz NIfTI-1 single files (scl_slope applied as SimpleITK does).
+    .hdr         NIfTI-1 pairs and Analyze 7.5 (header in .hdr, data in .img);
+                 written as a NIfTI-1 pair.
+    .mrc         MRC2014 volumes (modes 0, 1, 2, 6, 12), flipped along y and
+                 answered with the result type .tif, as load_data does
+                 (biomedisa_helper.py:595-604); read only.
     .mha/.mhd    MetaImage, data inline or in a separate raw file, optionally
                  zlib-compressed.
     .npy/.npz    NumPy arrays (.npz: the first array).
@@ -20,7 +25,7 @@ the formats this package reads and writes itself.
                  file-name order (biomedisa_helper.py:505-591); results are
                  written as slices with the label stack's file names.
 
-Other extensions (.nc, .hdr/.img, DICOM/JPEG/BMP slices) raise
+Other extensions (.nc, .zarr, DICOM/JPEG/BMP slices) raise
 NotImplementedError.
 """
 import gzip
@@ -353,6 +358,10 @@ _NII_TYPES = {2: "u1", 4: "i2", 8: "i4", 16: "f4", 64: "f8", 256: "i1", 512: "u2
 
 
 def _read_nii(path):
+    """NIfTI-1 single file (.nii, .nii.gz), or a .hdr whose data lies in the
+    .img beside it: a NIfTI-1 pair (magic ni1) or Analyze 7.5 (no magic),
+    whose header is read with the NIfTI-1 offsets, as NIfTI readers do: the
+    fields read here sit at the same bytes."""
     opener = gzip.open if path.endswith(".gz") else open
     with opener(path, "rb") as f:
         buf = f.read()
@@ -365,8 +374,15 @@ def _read_nii(path):
     dtype_code = struct.unpack(bo + "h", buf[70:72])[0]
     vox_offset = int(struct.unpack(bo + "f", buf[108:112])[0])
     slope, inter = struct.unpack(bo + "2f", buf[112:120])
-    if buf[344:347] != b"n+1":
-        raise NotImplementedError("%s: NIfTI header without data (.hdr/.img pair)" % path)
+    if path.lower().endswith(".hdr"):
+        if buf[344:347] == b"n+1":
+            raise ValueError("%s: a single-file NIfTI header in a .hdr file" % path)
+        vox_offset = max(vox_offset, 0)
+        img_path = path[:-4] + (".img" if path[-4:] == ".hdr" else ".IMG")
+        with open(img_path, "rb") as f:
+            buf = f.read()
+    elif buf[344:347] != b"n+1":
+        raise ValueError("%s: not a single-file NIfTI-1 image" % path)
     if dtype_code not in _NII_TYPES:
         raise NotImplementedError("%s: NIfTI datatype %d" % (path, dtype_code))
     nd = dim[0]
@@ -375,6 +391,8 @@ def _read_nii(path):
         raise NotImplementedError("%s: %d-D NIfTI image" % (path, nd))
     dt = np.dtype(bo + _NII_TYPES[dtype_code])
     n = shape[0] * shape[1] * shape[2]
+    if len(buf) < vox_offset + n * dt.itemsize:
+        raise ValueError("%s: image data shorter than the header says" % path)
     arr = np.frombuffer(buf[vox_offset:vox_offset + n * dt.itemsize], dtype=dt)
     arr = arr.reshape(shape[2], shape[1], shape[0]).astype(dt.newbyteorder("="))
     if slope not in (0.0, 1.0) or inter != 0.0:
@@ -394,14 +412,51 @@ def _write_nii(path, arr, compress=True):
     struct.pack_into("<8h", hdr, 40, 3, X, Y, Z, 1, 1, 1, 1)
     struct.pack_into("<2h", hdr, 70, codes[key], arr.dtype.itemsize * 8)
     struct.pack_into("<8f", hdr, 76, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
-    struct.pack_into("<3f", hdr, 108, 352.0, 1.0, 0.0)
+    pair = path.lower().endswith(".hdr")
+    struct.pack_into("<3f", hdr, 108, 0.0 if pair else 352.0, 1.0, 0.0)
     struct.pack_into("<B", hdr, 123, 10)      # xyzt_units: mm, s
+    if pair:                                  # header and .img beside it
+        hdr[344:348] = b"ni1\0"
+        with open(path, "wb") as f:
+            f.write(bytes(hdr[:348]))
+        with open(path[:-4] + ".img", "wb") as f:
+            f.write(arr.astype(arr.dtype.newbyteorder("<")).tobytes())
+        return
     hdr[344:348] = b"n+1\0"
     data = bytes(hdr) + arr.astype(arr.dtype.newbyteorder("<")).tobytes()
     if path.endswith(".gz"):
         data = gzip.compress(data, 6)
     with open(path, "wb") as f:
         f.write(data)
+
+
+_MRC_TYPES = {0: "i1", 1: "i2", 2: "f4", 6: "u2", 12: "f2"}
+
+
+def _read_mrc(path):
+    """MRC2014: a 1024-byte header of 32-bit words (nx, ny, nz, mode first;
+    the extended header's size at word 24; the machine stamp at byte 212),
+    then nz planes of ny rows of nx samples."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    if len(buf) < 1024:
+        raise ValueError("%s: not an MRC file" % path)
+    stamp = buf[212]
+    bo = ">" if stamp == 0x11 else "<"
+    if stamp not in (0x11, 0x44, 0x41):       # no stamp: take the order whose mode is known
+        bo = "<" if struct.unpack("<i", buf[12:16])[0] in _MRC_TYPES else ">"
+    nx, ny, nz, mode = struct.unpack(bo + "4i", buf[:16])
+    next_ = struct.unpack(bo + "i", buf[92:96])[0]
+    if mode not in _MRC_TYPES:
+        raise NotImplementedError("%s: MRC mode %d" % (path, mode))
+    if min(nx, ny, nz) < 1 or next_ < 0:
+        raise ValueError("%s: not an MRC file" % path)
+    dt = np.dtype(bo + _MRC_TYPES[mode])
+    start, n = 1024 + next_, nx * ny * nz
+    if len(buf) < start + n * dt.itemsize:
+        raise ValueError("%s: MRC data shorter than the header says" % path)
+    arr = np.frombuffer(buf[start:start + n * dt.itemsize], dtype=dt).reshape(nz, ny, nx)
+    return np.ascontiguousarray(np.flip(arr, 1)).astype(dt.newbyteorder("="))
 
 
 _MET_TYPES = {"MET_UCHAR": "u1", "MET_CHAR": "i1", "MET_USHORT": "u2", "MET_SHORT": "i2", "MET_UINT": "u4",
@@ -655,10 +710,12 @@ def load_data(path):
         return _read_nrrd(path), ext
     if ext == ".am":
         return _read_am(path), ext
-    if ext in (".nii", ".nii.gz"):
+    if ext in (".nii", ".nii.gz", ".hdr"):
         return _read_nii(path), ext
     if ext in (".mha", ".mhd"):
         return _read_meta(path), ext
+    if ext == ".mrc":
+        return _read_mrc(path), ".tif"
     if ext == ".npy":
         return np.load(path, allow_pickle=False), ext
     if ext == ".npz":
@@ -680,7 +737,7 @@ def save_data(path, arr, compress=True, header=None, file_type=None):
         _write_nrrd(path, arr, compress)
     elif ext == ".am":
         _write_am(path, arr, compress, header)
-    elif ext in (".nii", ".nii.gz"):
+    elif ext in (".nii", ".nii.gz", ".hdr"):
         _write_nii(path, arr, compress)
     elif ext in (".mha", ".mhd"):
         _write_meta(path, arr, compress)

@@ -64,7 +64,7 @@ Lattice { byte Labels } @1(HxByteRLE,0)
 # Data section follows"""
 
 
-@pytest.mark.parametrize("label_ext", [".tif", ".nrrd", ".am", ".nii.gz", ".mha", "", ".zip"])
+@pytest.mark.parametrize("label_ext", [".tif", ".nrrd", ".am", ".nii.gz", ".hdr", ".mha", "", ".zip"])
 def test_cli_end_to_end(tmp_path, label_ext):
     from biomedisa_amd.__main__ import main
     from biomedisa_amd.interpolation import smart_interpolation
@@ -96,3 +96,19 @@ def test_cli_end_to_end(tmp_path, label_ext):
     if label_ext == ".am":
         with open(files["regular"], "rb") as f:
             assert b"Tissue { Color 0.1 0.6 0.3, id 1 }" in f.read()
+
+
+def test_cli_mrc_labels(tmp_path):
+    """MRC labels (int8, y flipped in the file) give TIFF results."""
+    from biomedisa_amd.__main__ import main
+    from biomedisa_amd.interpolation import smart_interpolation
+    from biomedisa_amd.synth import structures_volume
+    from tests.test_io import _mrc_bytes
+    vol = structures_volume(shape=(28, 32, 36), seed=23, n_struct=3, radius=(4, 9), every=4, allaxis=False)
+    img_path = str(tmp_path / "scan.tif")
+    bio.save_data(img_path, vol.img)
+    (tmp_path / "scan_labels.mrc").write_bytes(_mrc_bytes(np.ascontiguousarray(vol.labels[:, ::-1]).astype(np.int8), 0))
+    _say("cli .mrc labels")
+    assert main([img_path, str(tmp_path / "scan_labels.mrc"), "--nbrw", "3", "--sorw", "120"]) == 0
+    want = smart_interpolation(vol.img, vol.labels, nbrw=3, sorw=120)
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "final.scan.tif"))[0], want["regular"])

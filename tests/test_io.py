@@ -449,3 +449,73 @@ def test_slice_errors(tmp_path):
         bio.load_data(str(e))
     with pytest.raises(ValueError):
         bio._read_png(b"\x89PNG\r\n\x1a\n" + b"\0" * 20)
+
+
+def _mrc_bytes(arr, mode, bo="<", stamp=True, ext=b""):
+    """An MRC2014 file assembled field by field: words 1-4 (nx, ny, nz, mode),
+    word 24 (extended header bytes), the machine stamp at byte 212."""
+    nz, ny, nx = arr.shape
+    hdr = bytearray(1024)
+    struct.pack_into(bo + "4i", hdr, 0, nx, ny, nz, mode)
+    struct.pack_into(bo + "i", hdr, 92, len(ext))
+    hdr[208:212] = b"MAP "
+    if stamp:
+        hdr[212:214] = b"\x44\x44" if bo == "<" else b"\x11\x11"
+    return bytes(hdr) + ext + arr.astype(arr.dtype.newbyteorder(bo)).tobytes()
+
+
+@pytest.mark.parametrize("mode,dtype", [(0, np.int8), (1, np.int16), (2, np.float32), (6, np.uint16),
+                                        (12, np.float16)])
+@pytest.mark.parametrize("bo,stamp", [("<", True), (">", True), ("<", False)])
+def test_mrc_read_flips_y(tmp_path, mode, dtype, bo, stamp):
+    rng = np.random.default_rng(mode)
+    arr = (rng.random((3, 5, 7)) * 100).astype(dtype)
+    p = tmp_path / "v.mrc"
+    p.write_bytes(_mrc_bytes(arr, mode, bo, stamp, ext=b"x" * 80))
+    got, ext = bio.load_data(str(p))
+    assert ext == ".tif"                       # results of .mrc labels are written as TIFF
+    assert got.dtype == np.dtype(dtype) and got.dtype.isnative
+    np.testing.assert_array_equal(got, arr[:, ::-1, :])
+
+
+def test_mrc_rejects_unknown_mode_and_short_data(tmp_path):
+    arr = np.zeros((2, 3, 4), np.float32)
+    p = tmp_path / "v.mrc"
+    p.write_bytes(_mrc_bytes(arr, 4))
+    with pytest.raises(NotImplementedError):
+        bio.load_data(str(p))
+    p.write_bytes(_mrc_bytes(arr, 2)[:-8])
+    with pytest.raises(ValueError):
+        bio.load_data(str(p))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.float32, np.uint16])
+def test_hdr_img_pair_round_trip(tmp_path, dtype):
+    arr = (np.random.default_rng(1).random((4, 6, 9)) * 200).astype(dtype)
+    p = tmp_path / "labels.hdr"
+    bio.save_data(str(p), arr)
+    assert (tmp_path / "labels.img").stat().st_size == arr.nbytes
+    hdr = p.read_bytes()
+    assert len(hdr) == 348 and hdr[344:348] == b"ni1\0"
+    got, ext = bio.load_data(str(p))
+    assert ext == ".hdr" and got.dtype == arr.dtype
+    np.testing.assert_array_equal(got, arr)
+
+
+@pytest.mark.parametrize("bo", ["<", ">"])
+def test_analyze75_read(tmp_path, bo):
+    """An Analyze 7.5 header (no magic) written field by field: sizeof_hdr,
+    dim[8] at 40, datatype and bitpix at 70, vox_offset at 108; data in .img."""
+    arr = np.arange(2 * 3 * 5, dtype=np.int16).reshape(2, 3, 5) - 7
+    hdr = bytearray(348)
+    struct.pack_into(bo + "i", hdr, 0, 348)
+    struct.pack_into(bo + "8h", hdr, 40, 4, 5, 3, 2, 1, 0, 0, 0)
+    struct.pack_into(bo + "2h", hdr, 70, 4, 16)
+    struct.pack_into(bo + "f", hdr, 108, 16.0)
+    (tmp_path / "a.hdr").write_bytes(bytes(hdr))
+    (tmp_path / "a.img").write_bytes(b"\0" * 16 + arr.astype(arr.dtype.newbyteorder(bo)).tobytes())
+    got, _ = bio.load_data(str(tmp_path / "a.hdr"))
+    np.testing.assert_array_equal(got, arr)
+    (tmp_path / "a.img").write_bytes(b"\0" * 20)
+    with pytest.raises(ValueError):
+        bio.load_data(str(tmp_path / "a.hdr"))
