@@ -515,7 +515,8 @@ def bench_large(args):
         sec = time.perf_counter() - t
         log("%s: %.1fs %s" % (method, sec, {k: round(v, 3) for k, v in tm.items()}))
         res[method] = dict(sec=sec, tm=tm, nseeds=out["nseeds"], walked=out.get("walked_seeds"),
-                           ring=out.get("ring_seeds"), repair=out.get("repair_seeds"), accepted=out["accepted"],
+                           ring=out.get("ring_seeds"), mid_ring=out.get("mid_ring_seeds"),
+                           repair=out.get("repair_seeds"), accepted=out["accepted"],
                            regular=out["regular"].download(dv.dims, np.uint8))
         out["regular"].free()
         del out
@@ -545,6 +546,7 @@ def bench_large(args):
                    "labels": nol, "seeds": p["nseeds"], "plain_seconds": p["sec"], "guided_seconds": g["sec"],
                    "speedup_vs_plain": p["sec"] / g["sec"], "walked_seed_fraction": g["walked"] / g["nseeds"],
                    "ring_seed_fraction": g["ring"] / g["nseeds"],
+                   "mid_ring_seed_fraction": g["mid_ring"] / g["nseeds"],
                    "repair_seed_fraction": (g["repair"] or 0) / g["nseeds"],
                    "min_dice_guided_vs_plain": min(d.values()),
                    "dice_guided_vs_plain": {str(k): round(v, 5) for k, v in d.items()},
