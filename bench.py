@@ -156,8 +156,8 @@ def main():
                          "128 MiB) once, as the known-traffic reference for HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has one MI355X; "
-              "multi-GPU sharding is described, not run", file=sys.stderr)
+        print("bench.py measures a single MI355X (--gpus 1); sharding the walk over several GPUs "
+              "is a design, not implemented", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
