@@ -21,11 +21,12 @@ the formats this package reads and writes itself.
     .mha/.mhd    MetaImage, data inline or in a separate raw file, optionally
                  zlib-compressed.
     .npy/.npz    NumPy arrays (.npz: the first array).
-    directory / .zip   a stack of 2-D TIFF or grayscale PNG slices in natural
-                 file-name order (biomedisa_helper.py:505-591); results are
-                 written as slices with the label stack's file names.
+    directory / .zip   a stack of 2-D TIFF, PNG or BMP slices in natural
+                 file-name order (biomedisa_helper.py:505-591; colour slices
+                 become gray by the reference's rgb2gray, :357-364); results
+                 are written as slices with the label stack's file names.
 
-Other extensions (.nc, .zarr, DICOM/JPEG/BMP slices) raise
+Other extensions (.nc, .zarr, DICOM/JPEG slices) raise
 NotImplementedError.
 """
 import gzip
@@ -526,9 +527,15 @@ def _write_meta(path, arr, compress=True):
 _PNG_SIG = b"\x89PNG\r\n\x1a\n"
 
 
+def _rgb2gray(rgb):
+    """(rows, cols, 3) R, G, B -> gray as biomedisa_helper.py:357-364 computes
+    it: the weighted sum in double, truncated to the sample type."""
+    return (0.2989 * rgb[:, :, 0] + 0.587 * rgb[:, :, 1] + 0.114 * rgb[:, :, 2]).astype(rgb.dtype)
+
+
 def _read_png(buf, name="PNG"):
-    """Grayscale (optionally with alpha, which is dropped) 8/16-bit,
-    non-interlaced PNG -> (rows, cols) array."""
+    """Grayscale (optionally with alpha, which is dropped) or RGB (made gray
+    by _rgb2gray), 8/16-bit, non-interlaced PNG -> (rows, cols) array."""
     from . import _hip
     if buf[:8] != _PNG_SIG:
         raise ValueError("%s: not a PNG file" % name)
@@ -546,9 +553,9 @@ def _read_png(buf, name="PNG"):
     if ihdr is None:
         raise ValueError("%s: no IHDR chunk" % name)
     W, H, depth, ctype, _, _, interlace = ihdr
-    if ctype not in (0, 4) or depth not in (8, 16) or interlace:
+    if ctype not in (0, 2, 4) or depth not in (8, 16) or interlace:
         raise NotImplementedError("%s: PNG colour type %d, depth %d, interlace %d" % (name, ctype, depth, interlace))
-    chans = 1 if ctype == 0 else 2
+    chans = {0: 1, 2: 3, 4: 2}[ctype]
     bpp = chans * depth // 8
     raw = np.frombuffer(zlib.decompress(b"".join(idat)), np.uint8)
     rowbytes = W * bpp
@@ -558,8 +565,62 @@ def _read_png(buf, name="PNG"):
     rc = _hip.load().bmx_png_unfilter(raw.ctypes.data, H, rowbytes, bpp, out.ctypes.data)
     if rc != 0:
         raise ValueError(_hip.load().bmx_last_error().decode(errors="replace"))
-    img = out.view(">u2" if depth == 16 else "u1").reshape(H, W, chans)[:, :, 0]
-    return img.astype(img.dtype.newbyteorder("="))
+    img = out.view(">u2" if depth == 16 else "u1").reshape(H, W, chans)
+    img = img.astype(img.dtype.newbyteorder("="))
+    return _rgb2gray(img) if ctype == 2 else img[:, :, 0]
+
+
+def _read_bmp(buf, name="BMP"):
+    """Uncompressed Windows bitmap with a BITMAPINFOHEADER or later header:
+    8-bit with a palette (a gray palette gives its gray values, a colour one
+    goes through _rgb2gray) or 24-bit B, G, R; rows bottom-up unless the
+    height is negative, each padded to 4 bytes -> (rows, cols) uint8."""
+    if len(buf) < 54 or buf[:2] != b"BM":
+        raise ValueError("%s: not a BMP file" % name)
+    off, hsize = struct.unpack("<II", buf[10:18])
+    if hsize < 40:
+        raise NotImplementedError("%s: BMP core header" % name)
+    W, H, _, bits, comp = struct.unpack("<iiHHI", buf[18:34])
+    if comp != 0 or bits not in (8, 24):
+        raise NotImplementedError("%s: BMP with %d bits, compression %d" % (name, bits, comp))
+    top_down, H = H < 0, abs(H)
+    if W < 1 or H < 1:
+        raise ValueError("%s: BMP of %d x %d pixels" % (name, W, H))
+    stride = (W * bits // 8 + 3) & ~3
+    if len(buf) < off + stride * H:
+        raise ValueError("%s: truncated BMP data" % name)
+    rows = np.frombuffer(buf, np.uint8, stride * H, off).reshape(H, stride)[:, :W * bits // 8]
+    if not top_down:
+        rows = rows[::-1]
+    if bits == 24:
+        return _rgb2gray(np.ascontiguousarray(rows.reshape(H, W, 3)[:, :, ::-1]))
+    n = struct.unpack("<I", buf[46:50])[0] or 256
+    if n > 256 or 14 + hsize + 4 * n > off:
+        raise ValueError("%s: BMP palette of %d entries" % (name, n))
+    pal = np.frombuffer(buf, np.uint8, 4 * n, 14 + hsize).reshape(n, 4)      # B, G, R, 0
+    if int(rows.max()) >= n:
+        raise ValueError("%s: BMP index beyond its palette" % name)
+    px = pal[rows]
+    if (pal[:, 0] == pal[:, 1]).all() and (pal[:, 1] == pal[:, 2]).all():
+        return np.ascontiguousarray(px[:, :, 2])
+    return _rgb2gray(np.ascontiguousarray(px[:, :, 2::-1]))
+
+
+def _bmp_bytes(img):
+    """2-D uint8 -> 8-bit BMP with a gray-ramp palette, rows bottom-up."""
+    img = np.ascontiguousarray(img)
+    if img.dtype != np.uint8 or img.ndim != 2:
+        raise NotImplementedError("BMP slices hold 2-D uint8 data, not %s %s" % (img.dtype, img.shape))
+    H, W = img.shape
+    stride = (W + 3) & ~3
+    rows = np.zeros((H, stride), np.uint8)
+    rows[:, :W] = img[::-1]
+    ramp = np.repeat(np.arange(256, dtype=np.uint8), 4).reshape(256, 4)
+    ramp[:, 3] = 0
+    off = 14 + 40 + 1024
+    return (b"BM" + struct.pack("<IHHI", off + rows.size, 0, 0, off)
+            + struct.pack("<IiiHHIIiiII", 40, W, H, 1, 8, 0, rows.size, 2835, 2835, 256, 0)
+            + ramp.tobytes() + rows.tobytes())
 
 
 def _png_bytes(img, compress=True):
@@ -583,8 +644,8 @@ def _natural_key(s):
     return [int(t) if t.isdigit() else t.lower() for t in re.split(r"(\d+)", s)]
 
 
-_SLICE_TYPES = (".png", ".tif", ".tiff")
-_OTHER_SLICES = (".dcm", ".dicom", ".bmp", ".jpg", ".jpeg")
+_SLICE_TYPES = (".png", ".tif", ".tiff", ".bmp")
+_OTHER_SLICES = (".dcm", ".dicom", ".jpg", ".jpeg")
 
 
 def _slice_names(path):
@@ -621,8 +682,11 @@ def _slice_bytes(path):
 
 
 def _decode_slice(name, buf):
-    if os.path.splitext(name)[1].lower() == ".png":
+    ext = os.path.splitext(name)[1].lower()
+    if ext == ".png":
         return _read_png(buf, name)
+    if ext == ".bmp":
+        return _read_bmp(buf, name)
     img = _parse_tiff(buf, name)
     if img.shape[0] != 1:
         raise NotImplementedError("%s: multi-page TIFF slice" % name)
@@ -641,8 +705,11 @@ def _read_slices(path):
 
 
 def _slice_file_bytes(name, img, compress):
-    if os.path.splitext(name)[1].lower() == ".png":
+    ext = os.path.splitext(name)[1].lower()
+    if ext == ".png":
         return _png_bytes(img, compress)
+    if ext == ".bmp":
+        return _bmp_bytes(img)
     return _tiff_bytes(img, compress)
 
 

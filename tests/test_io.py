@@ -6,6 +6,7 @@ interpolation.py:115-133, and the reference's command-line defaults.  AmiraMesh
 (HxByteRLE checked against a decoder written from its definition), NIfTI-1 and
 MetaImage files likewise."""
 import gzip
+import os
 import struct
 import zlib
 
@@ -519,3 +520,82 @@ def test_analyze75_read(tmp_path, bo):
     (tmp_path / "a.img").write_bytes(b"\0" * 20)
     with pytest.raises(ValueError):
         bio.load_data(str(tmp_path / "a.hdr"))
+
+
+def _png_chunk(typ, body):
+    return struct.pack(">I", len(body)) + typ + body + struct.pack(">I", zlib.crc32(typ + body) & 0xFFFFFFFF)
+
+
+# R, G, B pixels and their gray values under 0.2989 R + 0.587 G + 0.114 B,
+# truncated: worked by hand (white is 0.9999 x 255 = 254.97 -> 254)
+_RGB_KNOWN = [((255, 0, 0), 76), ((0, 255, 0), 149), ((0, 0, 255), 29), ((255, 255, 255), 254),
+              ((0, 0, 0), 0), ((10, 20, 30), 18)]
+
+
+def _bmp_file(pixels_bgr_rows, bits, palette=None, top_down=False):
+    """A BMP assembled field by field: 14-byte file header, 40-byte info
+    header, optional palette (B, G, R, 0), rows padded to 4 bytes."""
+    H = len(pixels_bgr_rows)
+    W = len(pixels_bgr_rows[0]) // (bits // 8)
+    stride = (W * bits // 8 + 3) // 4 * 4
+    body = b"".join(bytes(r) + b"\0" * (stride - len(r)) for r in (pixels_bgr_rows if top_down
+                                                                     else pixels_bgr_rows[::-1]))
+    pal = b"".join(bytes(p) + b"\0" for p in (palette or []))
+    off = 14 + 40 + len(pal)
+    info = struct.pack("<IiiHHIIiiII", 40, W, -H if top_down else H, 1, bits, 0, len(body), 0, 0,
+                       len(palette or []), 0)
+    return b"BM" + struct.pack("<IHHI", off + len(body), 0, 0, off) + info + pal + body
+
+
+@pytest.mark.parametrize("top_down", [False, True])
+def test_bmp_24bit_gray_known_values(top_down):
+    row0 = [c for rgb, _ in _RGB_KNOWN[:3] for c in rgb[::-1]]      # 3 pixels: 9 bytes, 3 of padding
+    row1 = [c for rgb, _ in _RGB_KNOWN[3:] for c in rgb[::-1]]
+    img = bio._read_bmp(_bmp_file([row0, row1], 24, top_down=top_down))
+    assert img.dtype == np.uint8
+    np.testing.assert_array_equal(img, [[76, 149, 29], [254, 0, 18]])
+
+
+def test_bmp_8bit_palettes():
+    idx = [[0, 1, 2, 3, 4], [4, 3, 2, 1, 0]]
+    gray = [(v, v, v) for v in (0, 7, 100, 200, 255)]              # a gray palette, not a ramp
+    np.testing.assert_array_equal(bio._read_bmp(_bmp_file(idx, 8, gray)),
+                                  [[0, 7, 100, 200, 255], [255, 200, 100, 7, 0]])
+    colour = [tuple(rgb[::-1]) for rgb, _ in _RGB_KNOWN[:5]]         # B, G, R entries
+    want = [g for _, g in _RGB_KNOWN[:5]]
+    np.testing.assert_array_equal(bio._read_bmp(_bmp_file(idx, 8, colour)), [want, want[::-1]])
+    with pytest.raises(ValueError):
+        bio._read_bmp(_bmp_file([[0, 5]], 8, gray))                   # index beyond the palette
+
+
+def test_bmp_rejects_other_forms():
+    with pytest.raises(NotImplementedError):
+        bio._read_bmp(_bmp_file([[1, 2, 3, 4]], 32))
+    with pytest.raises(ValueError):
+        bio._read_bmp(b"BX" + b"\0" * 60)
+
+
+def test_bmp_slices_round_trip(tmp_path):
+    arr = (np.random.default_rng(3).random((4, 7, 13)) * 255).astype(np.uint8)
+    names = ["s_%d.bmp" % k for k in range(4)]
+    d = tmp_path / "stack"
+    bio.save_data(str(d), arr, header={"slices": names, "dtype": np.uint8}, file_type="")
+    assert sorted(os.listdir(d)) == sorted(names)
+    got, ext = bio.load_data(str(d))
+    assert ext == "" and got.dtype == np.uint8
+    np.testing.assert_array_equal(got, arr)
+
+
+@pytest.mark.parametrize("depth", [8, 16])
+def test_png_rgb_gray_known_values(depth):
+    scale = 1 if depth == 8 else 257
+    rgb = np.array([[c for c, _ in _RGB_KNOWN]], np.uint32) * scale     # 1 x 6 x 3
+    dt = ">u2" if depth == 16 else "u1"
+    raw = b"".join(b"\0" + r.astype(dt).tobytes() for r in rgb)
+    png = (b"\x89PNG\r\n\x1a\n" + _png_chunk(b"IHDR", struct.pack(">IIBBBBB", 6, 1, depth, 2, 0, 0, 0))
+           + _png_chunk(b"IDAT", zlib.compress(raw)) + _png_chunk(b"IEND", b""))
+    img = bio._read_png(png)
+    if depth == 8:
+        np.testing.assert_array_equal(img, [[g for _, g in _RGB_KNOWN]])
+    else:   # white: 0.9999 x 65535 = 65528.4 -> 65528; (2570, 5140, 7710) -> 4664.3 -> 4664
+        np.testing.assert_array_equal(img, [[19588, 38469, 7470, 65528, 0, 4664]])
