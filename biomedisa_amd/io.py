@@ -21,12 +21,12 @@ the formats this package reads and writes itself.
     .mha/.mhd    MetaImage, data inline or in a separate raw file, optionally
                  zlib-compressed.
     .npy/.npz    NumPy arrays (.npz: the first array).
-    directory / .zip   a stack of 2-D TIFF, PNG or BMP slices in natural
+    directory / .zip   a stack of 2-D TIFF, PNG, BMP or DICOM slices in natural
                  file-name order (biomedisa_helper.py:505-591; colour slices
                  become gray by the reference's rgb2gray, :357-364); results
                  are written as slices with the label stack's file names.
 
-Other extensions (.nc, .zarr, DICOM/JPEG slices) raise
+Other extensions (.nc, .zarr, JPEG slices, compressed DICOM) raise
 NotImplementedError.
 """
 import gzip
@@ -644,8 +644,161 @@ def _natural_key(s):
     return [int(t) if t.isdigit() else t.lower() for t in re.split(r"(\d+)", s)]
 
 
-_SLICE_TYPES = (".png", ".tif", ".tiff", ".bmp")
-_OTHER_SLICES = (".dcm", ".dicom", ".jpg", ".jpeg")
+# DICOM: value representations with a 2-byte reserved field and a 4-byte length
+# in explicit-VR encodings (PS3.5 section 7.1.2)
+_DCM_LONG_VR = {b"OB", b"OD", b"OF", b"OL", b"OV", b"OW", b"SQ", b"UC", b"UN", b"UR", b"UT", b"SV", b"UV"}
+_DCM_TS = {"1.2.840.10008.1.2": ("<", False), "1.2.840.10008.1.2.1": ("<", True),
+           "1.2.840.10008.1.2.2": (">", True), "1.2.840.10008.1.2.1.99": ("<", True)}
+_DCM_WANT = {(0x0028, 0x0002): "spp", (0x0028, 0x0006): "planar", (0x0028, 0x0008): "frames",
+             (0x0028, 0x0010): "rows", (0x0028, 0x0011): "cols", (0x0028, 0x0100): "alloc",
+             (0x0028, 0x0101): "stored", (0x0028, 0x0103): "signed", (0x0028, 0x1052): "inter",
+             (0x0028, 0x1053): "slope", (0x7FE0, 0x0010): "pixels"}
+
+
+def _dcm_elements(buf, pos, end, bo, explicit, out, depth=0):
+    """Walks data elements from pos to end (or to an item delimiter when end is
+    None), keeping the values of _DCM_WANT at the top level; sequences and
+    items of undefined length are walked to their delimiters.  -> end position."""
+    while pos + 8 <= len(buf) and (end is None or pos < end):
+        g, e = struct.unpack(bo + "HH", buf[pos:pos + 4])
+        if g == 0xFFFE:                                   # item / delimiters: implicit form
+            n = struct.unpack(bo + "I", buf[pos + 4:pos + 8])[0]
+            pos += 8
+            if e == 0xE00D or e == 0xE0DD:                # item / sequence delimitation
+                return pos
+            if n == 0xFFFFFFFF:
+                pos = _dcm_elements(buf, pos, None, bo, explicit, out, depth + 1)
+            else:
+                pos += n
+            continue
+        vr = buf[pos + 4:pos + 6] if explicit else None
+        if explicit and vr in _DCM_LONG_VR:
+            n = struct.unpack(bo + "I", buf[pos + 8:pos + 12])[0]
+            pos += 12
+        elif explicit:
+            n = struct.unpack(bo + "H", buf[pos + 6:pos + 8])[0]
+            pos += 8
+        else:
+            n = struct.unpack(bo + "I", buf[pos + 4:pos + 8])[0]
+            pos += 8
+        if n == 0xFFFFFFFF:
+            if (g, e) == (0x7FE0, 0x0010):
+                raise NotImplementedError("DICOM with encapsulated (compressed) pixel data")
+            pos = _dcm_elements(buf, pos, None, bo, explicit, out, depth + 1)
+            continue
+        key = _DCM_WANT.get((g, e)) if depth == 0 else None
+        if key is not None:
+            out[key] = (vr, buf[pos:pos + n])
+        pos += n
+    return pos
+
+
+def _read_dicom(buf, name="DICOM"):
+    """One uncompressed DICOM image (PS3.10 file, or a bare implicit-VR data
+    set): implicit or explicit VR little endian, explicit big endian or
+    deflated explicit little endian; 8/16/32-bit samples with the bits above
+    BitsStored masked (unsigned) or sign-extended (signed); rescale slope and
+    intercept applied as GDCM does (identity: the stored type; integral:
+    int32; else float64); three samples per pixel made gray by _rgb2gray."""
+    pos, ts = 0, "1.2.840.10008.1.2"
+    if buf[128:132] == b"DICM":
+        meta = {}
+        pos = 132
+        while pos + 8 <= len(buf) and struct.unpack("<H", buf[pos:pos + 2])[0] == 0x0002:
+            g, e = struct.unpack("<HH", buf[pos:pos + 4])
+            vr = buf[pos + 4:pos + 6]
+            if vr in _DCM_LONG_VR:
+                n, hd = struct.unpack("<I", buf[pos + 8:pos + 12])[0], 12
+            else:
+                n, hd = struct.unpack("<H", buf[pos + 6:pos + 8])[0], 8
+            meta[e] = buf[pos + hd:pos + hd + n]
+            pos += hd + n
+        ts = meta.get(0x0010, b"").rstrip(b"\0 ").decode("ascii", "replace") or ts
+    if ts not in _DCM_TS:
+        raise NotImplementedError("%s: DICOM transfer syntax %s (compressed)" % (name, ts))
+    bo, explicit = _DCM_TS[ts]
+    if ts.endswith(".99"):
+        buf, pos = zlib.decompress(buf[pos:], -15), 0
+    el = {}
+    _dcm_elements(buf, pos, len(buf), bo, explicit, el)
+
+    def num(key, default=None):
+        if key not in el:
+            if default is None:
+                raise ValueError("%s: DICOM without %s" % (name, key))
+            return default
+        vr, raw = el[key]
+        if key in ("inter", "slope", "frames"):               # DS / IS: decimal strings
+            txt = raw.rstrip(b"\0 ").decode("ascii", "replace").split("\\")[0].strip()
+            return float(txt) if txt else default
+        return struct.unpack(bo + "H", raw[:2])[0]            # US
+    rows, cols, alloc = num("rows"), num("cols"), num("alloc")
+    spp, signed, frames = num("spp", 1), num("signed", 0), int(num("frames", 1))
+    stored = num("stored", alloc)
+    if "pixels" not in el:
+        raise ValueError("%s: DICOM without pixel data" % name)
+    if alloc not in (8, 16, 32) or spp not in (1, 3) or frames != 1 or not 1 <= stored <= alloc:
+        raise NotImplementedError("%s: DICOM with %d-bit samples, %d per pixel, %d frames"
+                                  % (name, alloc, spp, frames))
+    dt = np.dtype(bo + ("i" if signed else "u") + str(alloc // 8))
+    n = rows * cols * spp
+    raw = el["pixels"][1]
+    if len(raw) < n * dt.itemsize:
+        raise ValueError("%s: DICOM pixel data shorter than its rows and columns" % name)
+    px = np.frombuffer(raw[:n * dt.itemsize], dt).astype(dt.newbyteorder("="))
+    if stored < alloc:
+        if signed:
+            shift = alloc - stored
+            px = ((px << shift) >> shift).astype(px.dtype)
+        else:
+            px = px & px.dtype.type((1 << stored) - 1)
+    if spp == 3:
+        px = px.reshape(3, rows, cols).transpose(1, 2, 0) if num("planar", 0) else px.reshape(rows, cols, 3)
+        px = _rgb2gray(np.ascontiguousarray(px))
+    else:
+        px = px.reshape(rows, cols)
+    slope, inter = num("slope", 1.0), num("inter", 0.0)
+    if slope == 1.0 and inter == 0.0:
+        return px
+    if float(slope).is_integer() and float(inter).is_integer():
+        return (px.astype(np.int64) * int(slope) + int(inter)).astype(np.int32)
+    return px.astype(np.float64) * slope + inter
+
+
+def _dicom_bytes(img):
+    """2-D uint8/uint16/int16 -> a minimal PS3.10 Secondary Capture file in
+    explicit VR little endian, MONOCHROME2."""
+    img = np.ascontiguousarray(img)
+    if img.ndim != 2 or img.dtype not in (np.uint8, np.uint16, np.int16):
+        raise NotImplementedError("DICOM slices hold 2-D uint8, uint16 or int16 data, not %s %s"
+                                  % (img.dtype, img.shape))
+    import uuid
+
+    def elem(g, e, vr, value):
+        if len(value) % 2:
+            value += b"\0" if vr in (b"UI", b"OB") else b" "
+        if vr in _DCM_LONG_VR:
+            return struct.pack("<HH2sHI", g, e, vr, 0, len(value)) + value
+        return struct.pack("<HH2sH", g, e, vr, len(value)) + value
+    uid = ("2.25.%d" % uuid.uuid4().int).encode()
+    sop_class = b"1.2.840.10008.5.1.4.1.1.7"
+    meta = (elem(2, 1, b"OB", b"\x00\x01") + elem(2, 2, b"UI", sop_class) + elem(2, 3, b"UI", uid)
+            + elem(2, 0x10, b"UI", b"1.2.840.10008.1.2.1"))
+    meta = elem(2, 0, b"UL", struct.pack("<I", len(meta))) + meta
+    bits = img.dtype.itemsize * 8
+    rows, cols = img.shape
+    ds = (elem(8, 0x16, b"UI", sop_class) + elem(8, 0x18, b"UI", uid)
+          + elem(0x28, 2, b"US", struct.pack("<H", 1)) + elem(0x28, 4, b"CS", b"MONOCHROME2")
+          + elem(0x28, 0x10, b"US", struct.pack("<H", rows)) + elem(0x28, 0x11, b"US", struct.pack("<H", cols))
+          + elem(0x28, 0x100, b"US", struct.pack("<H", bits)) + elem(0x28, 0x101, b"US", struct.pack("<H", bits))
+          + elem(0x28, 0x102, b"US", struct.pack("<H", bits - 1))
+          + elem(0x28, 0x103, b"US", struct.pack("<H", 1 if img.dtype == np.int16 else 0))
+          + elem(0x7FE0, 0x10, b"OB" if bits == 8 else b"OW", img.astype(img.dtype.newbyteorder("<")).tobytes()))
+    return b"\0" * 128 + b"DICM" + meta + ds
+
+
+_SLICE_TYPES = (".png", ".tif", ".tiff", ".bmp", ".dcm", ".dicom")
+_OTHER_SLICES = (".jpg", ".jpeg")
 
 
 def _slice_names(path):
@@ -687,6 +840,8 @@ def _decode_slice(name, buf):
         return _read_png(buf, name)
     if ext == ".bmp":
         return _read_bmp(buf, name)
+    if ext in (".dcm", ".dicom"):
+        return _read_dicom(buf, name)
     img = _parse_tiff(buf, name)
     if img.shape[0] != 1:
         raise NotImplementedError("%s: multi-page TIFF slice" % name)
@@ -710,6 +865,8 @@ def _slice_file_bytes(name, img, compress):
         return _png_bytes(img, compress)
     if ext == ".bmp":
         return _bmp_bytes(img)
+    if ext in (".dcm", ".dicom"):
+        return _dicom_bytes(img)
     return _tiff_bytes(img, compress)
 
 

@@ -443,11 +443,16 @@ def test_slice_errors(tmp_path):
     (d / "b.png").write_bytes(bio._png_bytes(np.zeros((4, 6), np.uint8)))
     with pytest.raises(ValueError):
         bio.load_data(str(d))
-    e = tmp_path / "dicom"
+    e = tmp_path / "jpeg"
     e.mkdir()
-    (e / "x.dcm").write_bytes(b"DICM")
+    (e / "x.jpg").write_bytes(b"\xff\xd8\xff")
     with pytest.raises(NotImplementedError):
         bio.load_data(str(e))
+    f = tmp_path / "dicom"
+    f.mkdir()
+    (f / "x.dcm").write_bytes(b"DICM")
+    with pytest.raises(ValueError):
+        bio.load_data(str(f))
     with pytest.raises(ValueError):
         bio._read_png(b"\x89PNG\r\n\x1a\n" + b"\0" * 20)
 
@@ -599,3 +604,96 @@ def test_png_rgb_gray_known_values(depth):
         np.testing.assert_array_equal(img, [[g for _, g in _RGB_KNOWN]])
     else:   # white: 0.9999 x 65535 = 65528.4 -> 65528; (2570, 5140, 7710) -> 4664.3 -> 4664
         np.testing.assert_array_equal(img, [[19588, 38469, 7470, 65528, 0, 4664]])
+
+
+def _dcm_el(g, e, value, vr=None, bo="<"):
+    """One data element (PS3.5 7.1): implicit VR when vr is None; explicit VR
+    with a 4-byte length for OB/OW/SQ/UN, else a 2-byte length."""
+    if vr is None:
+        return struct.pack(bo + "HHI", g, e, len(value)) + value
+    if vr in (b"OB", b"OW", b"SQ", b"UN"):
+        return struct.pack(bo + "HH2sHI", g, e, vr, 0, len(value)) + value
+    return struct.pack(bo + "HH2sH", g, e, vr, len(value)) + value
+
+
+def _dcm_image_elements(rows, cols, alloc, stored, signed, pixels, bo="<", explicit=True, spp=1,
+                        planar=0, slope=None, inter=None):
+    us = lambda v: struct.pack(bo + "H", v)  # noqa: E731
+    vr = (lambda v: v) if explicit else (lambda v: None)
+    out = _dcm_el(0x28, 2, us(spp), vr(b"US"), bo)
+    if spp == 3:
+        out += _dcm_el(0x28, 6, us(planar), vr(b"US"), bo)
+    out += (_dcm_el(0x28, 0x10, us(rows), vr(b"US"), bo) + _dcm_el(0x28, 0x11, us(cols), vr(b"US"), bo)
+            + _dcm_el(0x28, 0x100, us(alloc), vr(b"US"), bo) + _dcm_el(0x28, 0x101, us(stored), vr(b"US"), bo)
+            + _dcm_el(0x28, 0x103, us(signed), vr(b"US"), bo))
+    if inter is not None:
+        out += _dcm_el(0x28, 0x1052, inter, vr(b"DS"), bo)
+    if slope is not None:
+        out += _dcm_el(0x28, 0x1053, slope, vr(b"DS"), bo)
+    return out + _dcm_el(0x7FE0, 0x10, pixels, vr(b"OW"), bo)
+
+
+def _dcm_file(ts, dataset):
+    meta = _dcm_el(2, 0x10, ts + (b"\0" if len(ts) % 2 else b""), b"UI")
+    return b"\0" * 128 + b"DICM" + _dcm_el(2, 0, struct.pack("<I", len(meta)), b"UL") + meta + dataset
+
+
+def test_dicom_implicit_bare_masks_high_bits():
+    raw = np.array([[0xF123, 0x0FFF, 0x1000]], np.uint16)          # bits above BitsStored = 12 are dropped
+    ds = _dcm_image_elements(1, 3, 16, 12, 0, raw.tobytes(), explicit=False)
+    np.testing.assert_array_equal(bio._read_dicom(ds), [[0x0123, 0x0FFF, 0]])
+
+
+def test_dicom_explicit_sequence_rescale_and_sign():
+    # an undefined-length sequence whose item holds its own Rows: the walker
+    # skips it, the top-level Rows (2) counts
+    nested = _dcm_el(0x28, 0x10, struct.pack("<H", 99), b"US")
+    item = struct.pack("<HHI", 0xFFFE, 0xE000, 0xFFFFFFFF) + nested + struct.pack("<HHI", 0xFFFE, 0xE00D, 0)
+    seq = (struct.pack("<HH2sHI", 0x0008, 0x1115, b"SQ", 0, 0xFFFFFFFF) + item
+           + struct.pack("<HHI", 0xFFFE, 0xE0DD, 0))
+    raw = np.array([0x0FFF, 0x0800, 0x07FF, 0x0001], np.uint16)     # 12-bit two's complement
+    ds = seq + _dcm_image_elements(2, 2, 16, 12, 1, raw.tobytes(), slope=b"2 ", inter=b"-1024")
+    got = bio._read_dicom(_dcm_file(b"1.2.840.10008.1.2.1", ds))
+    assert got.dtype == np.int32
+    np.testing.assert_array_equal(got, np.array([[-1, -2048], [2047, 1]]) * 2 - 1024)
+
+
+def test_dicom_big_endian_deflate_and_float_rescale():
+    v = np.array([[1, 2, 300], [4, 5, 60000]], np.uint16)
+    be = _dcm_image_elements(2, 3, 16, 16, 0, v.astype(">u2").tobytes(), bo=">")
+    np.testing.assert_array_equal(bio._read_dicom(_dcm_file(b"1.2.840.10008.1.2.2", be)), v)
+    le = _dcm_image_elements(2, 3, 16, 16, 0, v.tobytes(), slope=b"0.5", inter=b"1")
+    c = zlib.compressobj(6, zlib.DEFLATED, -15)
+    got = bio._read_dicom(_dcm_file(b"1.2.840.10008.1.2.1.99", c.compress(le) + c.flush()))
+    assert got.dtype == np.float64
+    np.testing.assert_array_equal(got, v * 0.5 + 1)
+
+
+@pytest.mark.parametrize("planar", [0, 1])
+def test_dicom_rgb_gray_known_values(planar):
+    rgb = np.array([[c for c, _ in _RGB_KNOWN]], np.uint8)          # 1 x 6 x 3
+    px = rgb.transpose(2, 0, 1).tobytes() if planar else rgb.tobytes()
+    ds = _dcm_image_elements(1, 6, 8, 8, 0, px, spp=3, planar=planar)
+    np.testing.assert_array_equal(bio._read_dicom(_dcm_file(b"1.2.840.10008.1.2.1", ds)),
+                                  [[g for _, g in _RGB_KNOWN]])
+
+
+def test_dicom_rejects_compressed():
+    ds = _dcm_image_elements(1, 1, 8, 8, 0, b"\0\0")
+    with pytest.raises(NotImplementedError):
+        bio._read_dicom(_dcm_file(b"1.2.840.10008.1.2.4.50", ds))   # JPEG baseline
+    enc = (_dcm_image_elements(1, 1, 8, 8, 0, b"")[:-12]
+           + struct.pack("<HH2sHI", 0x7FE0, 0x10, b"OB", 0, 0xFFFFFFFF))
+    with pytest.raises(NotImplementedError):
+        bio._read_dicom(_dcm_file(b"1.2.840.10008.1.2.1", enc))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int16])
+def test_dicom_slices_round_trip(tmp_path, dtype):
+    arr = ((np.random.default_rng(4).random((3, 5, 9)) - 0.3) * 3000).astype(dtype)
+    names = ["img%02d.dcm" % k for k in range(3)]
+    d = tmp_path / "dcm"
+    bio.save_data(str(d), arr, header={"slices": names, "dtype": np.dtype(dtype)}, file_type="")
+    got, _ = bio.load_data(str(d))
+    assert got.dtype == np.dtype(dtype)
+    np.testing.assert_array_equal(got, arr)
