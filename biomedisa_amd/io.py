@@ -766,11 +766,11 @@ def _read_dicom(buf, name="DICOM"):
 
 
 def _dicom_bytes(img):
-    """2-D uint8/uint16/int16 -> a minimal PS3.10 Secondary Capture file in
+    """2-D 8/16/32-bit integers -> a minimal PS3.10 Secondary Capture file in
     explicit VR little endian, MONOCHROME2."""
     img = np.ascontiguousarray(img)
-    if img.ndim != 2 or img.dtype not in (np.uint8, np.uint16, np.int16):
-        raise NotImplementedError("DICOM slices hold 2-D uint8, uint16 or int16 data, not %s %s"
+    if img.ndim != 2 or img.dtype.kind not in "iu" or img.dtype.itemsize > 4 or img.dtype == np.int8:
+        raise NotImplementedError("DICOM slices hold 2-D 8/16/32-bit integers, not %s %s"
                                   % (img.dtype, img.shape))
     import uuid
 
@@ -792,7 +792,7 @@ def _dicom_bytes(img):
           + elem(0x28, 0x10, b"US", struct.pack("<H", rows)) + elem(0x28, 0x11, b"US", struct.pack("<H", cols))
           + elem(0x28, 0x100, b"US", struct.pack("<H", bits)) + elem(0x28, 0x101, b"US", struct.pack("<H", bits))
           + elem(0x28, 0x102, b"US", struct.pack("<H", bits - 1))
-          + elem(0x28, 0x103, b"US", struct.pack("<H", 1 if img.dtype == np.int16 else 0))
+          + elem(0x28, 0x103, b"US", struct.pack("<H", 1 if img.dtype.kind == "i" else 0))
           + elem(0x7FE0, 0x10, b"OB" if bits == 8 else b"OW", img.astype(img.dtype.newbyteorder("<")).tobytes()))
     return b"\0" * 128 + b"DICM" + meta + ds
 

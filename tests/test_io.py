@@ -688,7 +688,7 @@ def test_dicom_rejects_compressed():
         bio._read_dicom(_dcm_file(b"1.2.840.10008.1.2.1", enc))
 
 
-@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int16])
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.int16, np.int32])
 def test_dicom_slices_round_trip(tmp_path, dtype):
     arr = ((np.random.default_rng(4).random((3, 5, 9)) - 0.3) * 3000).astype(dtype)
     names = ["img%02d.dcm" % k for k in range(3)]
