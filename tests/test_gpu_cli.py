@@ -2,6 +2,8 @@
 smart_interpolation(denoise=True) against the oracle pipeline, and the command
 line end to end: files read, results written next to the image and equal to
 the in-memory call's."""
+import os
+
 import numpy as np
 import pytest
 
@@ -112,3 +114,24 @@ def test_cli_mrc_labels(tmp_path):
     assert main([img_path, str(tmp_path / "scan_labels.mrc"), "--nbrw", "3", "--sorw", "120"]) == 0
     want = smart_interpolation(vol.img, vol.labels, nbrw=3, sorw=120)
     np.testing.assert_array_equal(bio.load_data(str(tmp_path / "final.scan.tif"))[0], want["regular"])
+
+
+@pytest.mark.parametrize("suffix", [".dcm", ".bmp"])
+def test_cli_slice_folder_labels(tmp_path, suffix):
+    """Labels as a folder of DICOM or BMP slices: the results are folders of
+    slices with the label files' names."""
+    from biomedisa_amd.__main__ import main
+    from biomedisa_amd.interpolation import smart_interpolation
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(24, 30, 34), seed=24, n_struct=3, radius=(4, 8), every=4, allaxis=False)
+    img_path = str(tmp_path / "scan.tif")
+    bio.save_data(img_path, vol.img)
+    names = ["lab%03d%s" % (k, suffix) for k in range(vol.labels.shape[0])]
+    bio.save_data(str(tmp_path / "labels"), vol.labels, header={"slices": names, "dtype": np.uint8}, file_type="")
+    _say("cli %s slices" % suffix)
+    assert main([img_path, str(tmp_path / "labels"), "--nbrw", "3", "--sorw", "120", "-s", "2"]) == 0
+    want = smart_interpolation(vol.img, vol.labels, nbrw=3, sorw=120, smooth=2)
+    files = bio.result_paths(img_path, "")
+    for k in ("regular", "smooth"):
+        assert sorted(os.listdir(files[k])) == names
+        np.testing.assert_array_equal(bio.load_data(files[k])[0], want[k], err_msg=k)
