@@ -21,12 +21,17 @@ the formats this package reads and writes itself.
     .mha/.mhd    MetaImage, data inline or in a separate raw file, optionally
                  zlib-compressed.
     .npy/.npz    NumPy arrays (.npz: the first array).
+    .zarr        Zarr v2 and v3 arrays (a directory) with no compressor, zlib or
+                 gzip; written as Zarr v2, zlib level 5 when compress is set, in
+                 chunks of up to 100^3, as the reference's writer configures it
+                 (biomedisa_helper.py:784-833).
     directory / .zip   a stack of 2-D TIFF, PNG, BMP or DICOM slices in natural
                  file-name order (biomedisa_helper.py:505-591; colour slices
                  become gray by the reference's rgb2gray, :357-364); results
                  are written as slices with the label stack's file names.
 
-Other extensions (.nc, .zarr, JPEG slices, compressed DICOM) raise
+Other extensions (.nc, JPEG slices, compressed DICOM, Zarr with other
+codecs) raise
 NotImplementedError.
 """
 import gzip
@@ -458,6 +463,103 @@ def _read_mrc(path):
         raise ValueError("%s: MRC data shorter than the header says" % path)
     arr = np.frombuffer(buf[start:start + n * dt.itemsize], dtype=dt).reshape(nz, ny, nx)
     return np.ascontiguousarray(np.flip(arr, 1)).astype(dt.newbyteorder("="))
+
+
+_ZARR3_TYPES = {"bool": "b1", "int8": "i1", "uint8": "u1", "int16": "i2", "uint16": "u2", "int32": "i4",
+                "uint32": "u4", "int64": "i8", "uint64": "u8", "float32": "f4", "float64": "f8"}
+
+
+def _read_zarr(path):
+    """A Zarr array stored as a directory: v2 (.zarray; chunk keys i.j.k or
+    i/j/k; C or F order) or v3 (zarr.json; a regular chunk grid; the default
+    or v2 chunk-key encoding; a bytes codec then gzip or zlib).  Chunks on the
+    far edges are stored whole and cropped; absent chunks hold fill_value."""
+    import json
+    if os.path.isfile(os.path.join(path, ".zarray")):
+        with open(os.path.join(path, ".zarray")) as f:
+            meta = json.load(f)
+        dt, shape, chunks = np.dtype(meta["dtype"]), meta["shape"], meta["chunks"]
+        comp = meta.get("compressor")
+        codecs = [] if comp is None else [comp["id"]]
+        if meta.get("filters"):
+            raise NotImplementedError("%s: Zarr filters" % path)
+        order, sep = meta.get("order", "C"), meta.get("dimension_separator", ".")
+        key = lambda idx: sep.join(map(str, idx))  # noqa: E731
+        fill = meta.get("fill_value")
+    elif os.path.isfile(os.path.join(path, "zarr.json")):
+        with open(os.path.join(path, "zarr.json")) as f:
+            meta = json.load(f)
+        if meta.get("node_type") != "array":
+            raise NotImplementedError("%s: a Zarr group, not an array" % path)
+        grid = meta["chunk_grid"]
+        if grid.get("name") != "regular" or meta["data_type"] not in _ZARR3_TYPES:
+            raise NotImplementedError("%s: Zarr chunk grid %s, data type %s" % (path, grid.get("name"),
+                                                                                meta["data_type"]))
+        shape, chunks = meta["shape"], grid["configuration"]["chunk_shape"]
+        endian, codecs = "<", []
+        for c in meta.get("codecs", []):
+            if c["name"] == "bytes":
+                endian = ">" if (c.get("configuration") or {}).get("endian") == "big" else "<"
+            else:
+                codecs.append(c["name"])
+        dt = np.dtype(endian + _ZARR3_TYPES[meta["data_type"]])
+        enc = meta.get("chunk_key_encoding", {"name": "default"})
+        sep = (enc.get("configuration") or {}).get("separator", "/" if enc["name"] == "default" else ".")
+        prefix = "c" + sep if enc["name"] == "default" else ""
+        key = lambda idx: prefix + sep.join(map(str, idx))  # noqa: E731
+        order, fill = "C", meta.get("fill_value")
+    else:
+        raise ValueError("%s: no .zarray or zarr.json" % path)
+    codecs = [c.replace("numcodecs.", "") for c in codecs]
+    if any(c not in ("zlib", "gzip") for c in codecs):
+        raise NotImplementedError("%s: Zarr codecs %s" % (path, codecs))
+    if len(shape) != len(chunks) or min(chunks, default=1) < 1:
+        raise ValueError("%s: Zarr shape %s with chunks %s" % (path, shape, chunks))
+    if fill == "NaN":
+        fill = np.nan
+    elif fill is None or isinstance(fill, str):     # no fill value, or Infinity / base64 forms
+        fill = 0
+    out = np.full(shape, fill, dtype=dt.newbyteorder("="))
+    grid = [range(-(-s // c)) for s, c in zip(shape, chunks)]
+    for idx in np.ndindex(*[len(g) for g in grid]):
+        name = os.path.join(path, *key(idx).split("/"))
+        if not os.path.isfile(name):
+            continue
+        with open(name, "rb") as f:
+            data = f.read()
+        for c in reversed(codecs):
+            data = gzip.decompress(data) if c == "gzip" else zlib.decompress(data)
+        n = int(np.prod(chunks))
+        if len(data) != n * dt.itemsize:
+            raise ValueError("%s: Zarr chunk %s holds %d bytes" % (path, key(idx), len(data)))
+        block = np.frombuffer(data, dt).reshape(chunks, order=order)
+        sl = tuple(slice(i * c, min((i + 1) * c, s)) for i, c, s in zip(idx, chunks, shape))
+        out[sl] = block[tuple(slice(0, t.stop - t.start) for t in sl)]
+    return out
+
+
+def _write_zarr(path, arr, compress=True, chunks=(100, 100, 100)):
+    """Zarr v2 directory (replacing one that exists): C order, chunks of
+    `chunks`, zlib level 5 when compress is set."""
+    import json
+    import shutil
+    arr = np.ascontiguousarray(arr)
+    if os.path.exists(path):
+        shutil.rmtree(path)
+    os.makedirs(path)
+    chunks = tuple(min(c, max(s, 1)) for c, s in zip(chunks, arr.shape)) if arr.ndim == 3 else arr.shape
+    meta = {"zarr_format": 2, "shape": list(arr.shape), "chunks": list(chunks), "dtype": arr.dtype.str,
+            "compressor": {"id": "zlib", "level": 5} if compress else None, "fill_value": 0, "order": "C",
+            "filters": None, "dimension_separator": "."}
+    with open(os.path.join(path, ".zarray"), "w") as f:
+        json.dump(meta, f)
+    for idx in np.ndindex(*[-(-s // c) for s, c in zip(arr.shape, chunks)]):
+        block = np.zeros(chunks, arr.dtype)
+        sl = tuple(slice(i * c, min((i + 1) * c, s)) for i, c, s in zip(idx, chunks, arr.shape))
+        block[tuple(slice(0, t.stop - t.start) for t in sl)] = arr[sl]
+        data = block.tobytes()
+        with open(os.path.join(path, ".".join(map(str, idx))), "wb") as f:
+            f.write(zlib.compress(data, 5) if compress else data)
 
 
 _MET_TYPES = {"MET_UCHAR": "u1", "MET_CHAR": "i1", "MET_USHORT": "u2", "MET_SHORT": "i2", "MET_UINT": "u4",
@@ -899,6 +1001,8 @@ def read_header(path):
     ext = extension(path)
     if ext == ".am":
         return am_header(path)
+    if ext == ".zarr":
+        return None
     if os.path.isdir(path) or ext == ".zip":
         # the names of every slice and the sample type of the first
         slices = _slice_bytes(path)
@@ -924,6 +1028,8 @@ def load_data(path):
     if not os.path.exists(path):
         print(f"Error: No such file or directory '{path}'")
         return None, ext
+    if ext == ".zarr":
+        return _read_zarr(path), ext
     if os.path.isdir(path):
         return _read_slices(path)[0], ""
     if ext == ".zip":
@@ -965,6 +1071,8 @@ def save_data(path, arr, compress=True, header=None, file_type=None):
         _write_nii(path, arr, compress)
     elif ext in (".mha", ".mhd"):
         _write_meta(path, arr, compress)
+    elif ext == ".zarr":
+        _write_zarr(path, arr, compress)
     elif ext == ".npy":
         np.save(path, arr, allow_pickle=False)
     elif ext == ".npz":

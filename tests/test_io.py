@@ -697,3 +697,88 @@ def test_dicom_slices_round_trip(tmp_path, dtype):
     got, _ = bio.load_data(str(d))
     assert got.dtype == np.dtype(dtype)
     np.testing.assert_array_equal(got, arr)
+
+
+def _zarr_dir(root, meta_name, meta, chunks):
+    """A Zarr store written by hand: the metadata file and {key: bytes}."""
+    import json
+    root.mkdir()
+    (root / meta_name).write_text(json.dumps(meta))
+    for key, data in chunks.items():
+        p = root.joinpath(*key.split("/"))
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_bytes(data)
+
+
+def test_zarr_v2_fortran_order_edges_and_fill(tmp_path):
+    a = np.arange(5 * 3 * 4, dtype=np.uint16).reshape(5, 3, 4) * 3
+    chunks = (3, 3, 3)          # 2 x 1 x 2 chunks, the far ones cropped
+    store = {}
+    for i in range(2):
+        for k in range(2):
+            if (i, k) == (1, 1):
+                continue        # absent: holds fill_value
+            block = np.zeros(chunks, np.uint16)
+            part = a[i * 3:(i + 1) * 3, :, k * 3:(k + 1) * 3]
+            block[:part.shape[0], :, :part.shape[2]] = part
+            store["%d.0.%d" % (i, k)] = zlib.compress(block.astype("<u2").tobytes(order="F"))
+    meta = {"zarr_format": 2, "shape": [5, 3, 4], "chunks": list(chunks), "dtype": "<u2",
+            "compressor": {"id": "zlib", "level": 1}, "fill_value": 7, "order": "F", "filters": None}
+    _zarr_dir(tmp_path / "a.zarr", ".zarray", meta, store)
+    got, ext = bio.load_data(str(tmp_path / "a.zarr"))
+    want = a.copy()
+    want[3:, :, 3:] = 7
+    assert ext == ".zarr"
+    np.testing.assert_array_equal(got, want)
+
+
+def test_zarr_v2_nested_keys_uncompressed(tmp_path):
+    a = (np.arange(2 * 4 * 6) - 20).astype(np.int16).reshape(2, 4, 6)
+    store = {"0/%d/%d" % (j, k): a[:, j * 2:(j + 1) * 2, k * 3:(k + 1) * 3].astype("<i2").tobytes()
+             for j in range(2) for k in range(2)}
+    meta = {"zarr_format": 2, "shape": [2, 4, 6], "chunks": [2, 2, 3], "dtype": "<i2", "compressor": None,
+            "fill_value": 0, "order": "C", "filters": None, "dimension_separator": "/"}
+    _zarr_dir(tmp_path / "b.zarr", ".zarray", meta, store)
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "b.zarr"))[0], a)
+
+
+@pytest.mark.parametrize("enc,codec", [("default", "gzip"), ("v2", "numcodecs.zlib")])
+def test_zarr_v3(tmp_path, enc, codec):
+    a = (np.random.default_rng(5).random((3, 5, 2)) * 1000).astype(np.float32)
+    comp = (lambda b: gzip.compress(b)) if codec == "gzip" else (lambda b: zlib.compress(b))
+    store = {}
+    for j in range(2):
+        block = np.zeros((3, 4, 2), np.float32)
+        part = a[:, j * 4:(j + 1) * 4]
+        block[:, :part.shape[1]] = part
+        key = ("c/0/%d/0" % j) if enc == "default" else ("0.%d.0" % j)
+        store[key] = comp(block.astype(">f4").tobytes())
+    meta = {"zarr_format": 3, "node_type": "array", "shape": [3, 5, 2], "data_type": "float32",
+            "chunk_grid": {"name": "regular", "configuration": {"chunk_shape": [3, 4, 2]}},
+            "chunk_key_encoding": {"name": enc}, "fill_value": 0.0,
+            "codecs": [{"name": "bytes", "configuration": {"endian": "big"}}, {"name": codec}]}
+    _zarr_dir(tmp_path / "c.zarr", "zarr.json", meta, store)
+    got = bio.load_data(str(tmp_path / "c.zarr"))[0]
+    assert got.dtype == np.float32
+    np.testing.assert_array_equal(got, a)
+
+
+def test_zarr_rejects_other_codecs(tmp_path):
+    meta = {"zarr_format": 2, "shape": [1, 1, 1], "chunks": [1, 1, 1], "dtype": "|u1",
+            "compressor": {"id": "blosc", "cname": "lz4"}, "fill_value": 0, "order": "C", "filters": None}
+    _zarr_dir(tmp_path / "d.zarr", ".zarray", meta, {})
+    with pytest.raises(NotImplementedError):
+        bio.load_data(str(tmp_path / "d.zarr"))
+
+
+@pytest.mark.parametrize("compress", [True, False])
+def test_zarr_round_trip(tmp_path, compress):
+    import json
+    a = (np.random.default_rng(6).random((120, 9, 230)) * 255).astype(np.uint8)
+    p = str(tmp_path / "r.zarr")
+    bio.save_data(p, a, compress=compress)
+    meta = json.loads(open(os.path.join(p, ".zarray")).read())
+    assert meta["chunks"] == [100, 9, 100] and (meta["compressor"] is not None) == compress
+    got, _ = bio.load_data(p)
+    np.testing.assert_array_equal(got, a)
+    assert bio.read_header(p) is None
