@@ -21,6 +21,10 @@ the formats this package reads and writes itself.
     .mha/.mhd    MetaImage, data inline or in a separate raw file, optionally
                  zlib-compressed.
     .npy/.npz    NumPy arrays (.npz: the first array).
+    .nc          NetCDF classic (CDF-1, CDF-2, CDF-5), also .nc.bz2 and folders
+                 of .nc blocks stacked along z (nc_reader.py:143-215): the
+                 tomo, segmented or labels variable; results as CDF-2 files
+                 named and sized as the label blocks.
     .zarr        Zarr v2 and v3 arrays (a directory) with no compressor, zlib or
                  gzip; written as Zarr v2, zlib level 5 when compress is set, in
                  chunks of up to 100^3, as the reference's writer configures it
@@ -30,8 +34,8 @@ the formats this package reads and writes itself.
                  become gray by the reference's rgb2gray, :357-364); results
                  are written as slices with the label stack's file names.
 
-Other extensions (.nc, JPEG slices, compressed DICOM, Zarr with other
-codecs) raise
+Other extensions (NetCDF-4/HDF5, JPEG slices, compressed DICOM, Zarr with
+other codecs) raise
 NotImplementedError.
 """
 import gzip
@@ -562,6 +566,192 @@ def _write_zarr(path, arr, compress=True, chunks=(100, 100, 100)):
             f.write(zlib.compress(data, 5) if compress else data)
 
 
+_NC_TYPES = {1: "i1", 2: "S1", 3: "i2", 4: "i4", 5: "f4", 6: "f8", 7: "u1", 8: "u2", 9: "u4", 10: "i8", 11: "u8"}
+
+
+def _nc_parse(buf, name="NetCDF"):
+    """NetCDF classic header (CDF-1, 64-bit offset CDF-2, 64-bit data CDF-5;
+    big-endian throughout) -> (numrecs, {dim name: length},
+    {var name: (dim names, type, vsize, begin, attributes)}, record size)."""
+    if len(buf) < 8 or (buf[:3] == b"CDF" and buf[3] not in (1, 2, 5)):
+        raise ValueError("%s: not a NetCDF file" % name)
+    if buf[:3] != b"CDF":
+        raise NotImplementedError("%s: not a NetCDF classic file (NetCDF-4/HDF5 is not read)" % name)
+    ver = buf[3]
+    cnt = ">Q" if ver == 5 else ">I"               # element counts and lengths
+    off = ">Q" if ver in (2, 5) else ">I"          # variable offsets
+    pos = [4]
+
+    def take(fmt):
+        v = struct.unpack_from(fmt, buf, pos[0])[0]
+        pos[0] += struct.calcsize(fmt)
+        return v
+
+    def text():
+        n = take(cnt)
+        s = buf[pos[0]:pos[0] + n].decode("utf-8", "replace")
+        pos[0] += (n + 3) // 4 * 4
+        return s
+
+    def attrs():
+        tag, n = take(">I"), take(cnt)
+        if tag not in (0, 0x0C):
+            raise ValueError("%s: bad attribute list" % name)
+        out = {}
+        for _ in range(n):
+            key, typ = text(), take(">I")
+            k = take(cnt)
+            dt = np.dtype(">" + _NC_TYPES[typ]) if typ != 2 else np.dtype("S1")
+            raw = buf[pos[0]:pos[0] + k * dt.itemsize]
+            pos[0] += (k * dt.itemsize + 3) // 4 * 4
+            out[key] = raw.decode("utf-8", "replace") if typ == 2 else np.frombuffer(raw, dt)
+        return out
+    numrecs = take(cnt)
+    tag, n = take(">I"), take(cnt)
+    dims = []
+    for _ in range(n if tag == 0x0A else 0):
+        dims.append((text(), take(cnt)))
+    attrs()                                        # global attributes
+    tag, n = take(">I"), take(cnt)
+    variables = {}
+    for _ in range(n if tag == 0x0B else 0):
+        vname = text()
+        ids = [take(cnt) for _ in range(take(cnt))]
+        vatts = attrs()
+        typ, vsize, begin = take(">I"), take(cnt), take(off)
+        if typ not in _NC_TYPES:
+            raise NotImplementedError("%s: NetCDF type %d" % (name, typ))
+        variables[vname] = ([dims[i] for i in ids], typ, vsize, begin, vatts)
+    recsize = sum(v[2] for v in variables.values() if v[0] and v[0][0][1] == 0)
+    if len([v for v in variables.values() if v[0] and v[0][0][1] == 0]) == 1:
+        recsize = next(int(np.prod([d[1] for d in v[0][1:]])) * np.dtype(_NC_TYPES[v[1]]).itemsize
+                       for v in variables.values() if v[0] and v[0][0][1] == 0)
+    return numrecs, dict(dims), variables, recsize
+
+
+def _nc_pick(variables, name="NetCDF", first_allowed=True):
+    """The variable nc_to_np reads (nc_reader.py:162-165, 199-201): 'tomo', else
+    'segmented', else 'labels', else (single files) the first one."""
+    for n in ("tomo", "segmented", "labels"):
+        if n in variables:
+            return n
+    if first_allowed and variables:
+        return next(iter(variables))
+    raise ValueError("%s: no labels, segmented or tomo variable" % name)
+
+
+def _nc_array(buf, name="NetCDF", first_allowed=True):
+    """-> (variable name, array) of one NetCDF classic file: record variables
+    gathered record by record; _Unsigned, scale_factor and add_offset applied
+    as netCDF4 reads them by default."""
+    try:
+        numrecs, _, variables, recsize = _nc_parse(buf, name)
+    except (struct.error, IndexError, KeyError) as e:
+        raise ValueError("%s: truncated or malformed NetCDF header (%s)" % (name, e)) from None
+    vname = _nc_pick(variables, name, first_allowed)
+    vdims, typ, vsize, begin, vatts = variables[vname]
+    dt = np.dtype(">" + _NC_TYPES[typ])
+    shape = [d[1] for d in vdims]
+    if shape and shape[0] == 0:                    # a record variable
+        shape[0] = numrecs
+        per = int(np.prod(shape[1:])) * dt.itemsize
+        if begin + (numrecs - 1) * recsize + per > len(buf) and numrecs:
+            raise ValueError("%s: NetCDF data shorter than its header says" % name)
+        raw = b"".join(buf[begin + r * recsize:begin + r * recsize + per] for r in range(numrecs))
+    else:
+        n = int(np.prod(shape)) * dt.itemsize
+        if begin + n > len(buf):
+            raise ValueError("%s: NetCDF data shorter than its header says" % name)
+        raw = buf[begin:begin + n]
+    arr = np.frombuffer(raw, dt).reshape(shape).astype(dt.newbyteorder("="))
+    if str(vatts.get("_Unsigned", "")).lower() == "true" and arr.dtype.kind == "i":
+        arr = arr.view(arr.dtype.str.replace("i", "u"))
+    if "scale_factor" in vatts or "add_offset" in vatts:
+        scale = vatts.get("scale_factor", np.ones(1))[0]
+        arr = arr * scale + vatts.get("add_offset", np.zeros(1, scale.dtype))[0]
+    return vname, arr
+
+
+def _nc_files(path):
+    """The .nc and .bz2 files of a folder, in plain sorted order (nc_reader.py:175-177)."""
+    names = [os.path.join(r, n) for r, _, fs in os.walk(path) for n in fs if n.endswith((".nc", ".bz2"))]
+    return sorted(names)
+
+
+def _nc_file_bytes(p):
+    import bz2
+    with open(p, "rb") as f:
+        buf = f.read()
+    return bz2.decompress(buf) if p.endswith(".bz2") else buf
+
+
+def _read_nc(path):
+    """A NetCDF classic file (or .nc.bz2), or a folder of them stacked along z."""
+    if os.path.isdir(path):
+        parts = [_nc_array(_nc_file_bytes(p), p, first_allowed=False)[1] for p in _nc_files(path)]
+        if not parts:
+            raise ValueError("%s: no .nc files" % path)
+        return np.concatenate(parts, axis=0)
+    return _nc_array(_nc_file_bytes(path), path)[1]
+
+
+def _nc_bytes(arr, vname="labels"):
+    """A CDF-2 file holding arr (3-D) as one variable over the dimensions
+    <vname>_zdim, _ydim, _xdim; unsigned integers stored in the signed type of
+    their size with _Unsigned = "true", as netCDF classic writers do."""
+    arr = np.ascontiguousarray(arr)
+    codes = {"i1": 1, "u1": 1, "i2": 3, "u2": 3, "i4": 4, "u4": 4, "f4": 5, "f8": 6}
+    key = arr.dtype.kind + str(arr.dtype.itemsize)
+    if arr.ndim != 3 or key not in codes:
+        raise NotImplementedError("NetCDF results hold 3-D data of 8-64 bit, not %s %s" % (arr.dtype, arr.shape))
+
+    def text(s):
+        b = s.encode()
+        return struct.pack(">I", len(b)) + b + b"\0" * (-len(b) % 4)
+    dims = b"".join(text("%s_%sdim" % (vname, a)) + struct.pack(">I", n) for a, n in zip("zyx", arr.shape))
+    vatt = struct.pack(">II", 0, 0)
+    if arr.dtype.kind == "u":
+        vatt = struct.pack(">II", 0x0C, 1) + text("_Unsigned") + struct.pack(">II", 2, 4) + b"true"
+    data = arr.astype(arr.dtype.str.replace("u", "i").replace("<", ">").replace("|", ">")).tobytes()
+    head_wo = (b"CDF\x02" + struct.pack(">I", 0) + struct.pack(">II", 0x0A, 3) + dims + struct.pack(">II", 0, 0)
+               + struct.pack(">II", 0x0B, 1) + text(vname) + struct.pack(">IIII", 3, 0, 1, 2) + vatt
+               + struct.pack(">II", codes[key], min(len(data) + (-len(data) % 4), 0xFFFFFFFF)))
+    begin = len(head_wo) + 8
+    return head_wo + struct.pack(">Q", begin) + data
+
+
+def nc_header(path):
+    """What writing NetCDF results needs from a NetCDF label file or folder:
+    the variable name and, per source file, its name and z-extent."""
+    files = _nc_files(path) if os.path.isdir(path) else [path]
+    out = []
+    for p in files:
+        buf = _nc_file_bytes(p)
+        vname, arr = _nc_array(buf, p, first_allowed=not os.path.isdir(path))
+        out.append((os.path.basename(p), arr.shape[0]))
+    return {"nc": vname, "files": out}
+
+
+def _write_nc(path, arr, header=None):
+    """One .nc file (path ends in .nc), or a folder of blocks named and sized
+    as the label files were (nc_reader.py:85-140 writes one file per source
+    block); the variable keeps the label file's name."""
+    vname = (header or {}).get("nc", "labels")
+    if path.endswith(".nc"):
+        with open(path, "wb") as f:
+            f.write(_nc_bytes(arr, vname))
+        return
+    os.makedirs(path, exist_ok=True)
+    z = 0
+    for fname, nz in header["files"]:
+        fname = fname[:-4] if fname.endswith(".bz2") else fname
+        with open(os.path.join(path, fname), "wb") as f:
+            f.write(_nc_bytes(arr[z:z + nz], vname))
+        z += nz
+    if z != arr.shape[0]:
+        raise ValueError("NetCDF blocks cover %d planes of %d" % (z, arr.shape[0]))
+
+
 _MET_TYPES = {"MET_UCHAR": "u1", "MET_CHAR": "i1", "MET_USHORT": "u2", "MET_SHORT": "i2", "MET_UINT": "u4",
               "MET_INT": "i4", "MET_ULONG_LONG": "u8", "MET_LONG_LONG": "i8", "MET_FLOAT": "f4",
               "MET_DOUBLE": "f8"}
@@ -1003,6 +1193,8 @@ def read_header(path):
         return am_header(path)
     if ext == ".zarr":
         return None
+    if ext == ".nc" or (os.path.isdir(path) and _nc_files(path)):
+        return nc_header(path)
     if os.path.isdir(path) or ext == ".zip":
         # the names of every slice and the sample type of the first
         slices = _slice_bytes(path)
@@ -1018,6 +1210,8 @@ def extension(path):
     ext = os.path.splitext(path)[1]
     if ext == ".gz":
         ext = ".nii.gz"
+    elif ext == ".bz2":
+        ext = os.path.splitext(os.path.splitext(path)[0])[1]
     return ext.lower()
 
 
@@ -1030,6 +1224,8 @@ def load_data(path):
         return None, ext
     if ext == ".zarr":
         return _read_zarr(path), ext
+    if ext == ".nc" or (os.path.isdir(path) and _nc_files(path)):
+        return _read_nc(path), "" if os.path.isdir(path) else ext
     if os.path.isdir(path):
         return _read_slices(path)[0], ""
     if ext == ".zip":
@@ -1059,7 +1255,9 @@ def save_data(path, arr, compress=True, header=None, file_type=None):
     is a folder of slices), as save_data(..., final_image_type) does;
     header: read_header of the label file, for .am and slice stacks."""
     ext = extension(path) if file_type is None else file_type
-    if ext in ("", ".zip"):
+    if ext == ".nc" or (ext == "" and isinstance(header, dict) and "nc" in header):
+        _write_nc(path, arr, header)
+    elif ext in ("", ".zip"):
         _write_slices(path, arr, compress, header)
     elif ext in (".tif", ".tiff"):
         _write_tiff(path, arr, compress)

@@ -66,7 +66,7 @@ Lattice { byte Labels } @1(HxByteRLE,0)
 # Data section follows"""
 
 
-@pytest.mark.parametrize("label_ext", [".tif", ".nrrd", ".am", ".nii.gz", ".hdr", ".mha", ".zarr", "", ".zip"])
+@pytest.mark.parametrize("label_ext", [".tif", ".nrrd", ".am", ".nii.gz", ".hdr", ".mha", ".zarr", ".nc", "", ".zip"])
 def test_cli_end_to_end(tmp_path, label_ext):
     from biomedisa_amd.__main__ import main
     from biomedisa_amd.interpolation import smart_interpolation

@@ -127,8 +127,11 @@ def test_missing_and_unsupported(tmp_path, capsys):
     assert got is None and ext == ".tif"
     assert "Error" in capsys.readouterr().out
     (tmp_path / "x.nc").write_bytes(b"CDF")
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):
         bio.load_data(str(tmp_path / "x.nc"))
+    (tmp_path / "x.raw3d").write_bytes(b"\0" * 8)
+    with pytest.raises(NotImplementedError):
+        bio.load_data(str(tmp_path / "x.raw3d"))
     (tmp_path / "x.am").write_bytes(b"# AmiraMesh")
     with pytest.raises(ValueError):
         bio.load_data(str(tmp_path / "x.am"))
@@ -782,3 +785,88 @@ def test_zarr_round_trip(tmp_path, compress):
     got, _ = bio.load_data(p)
     np.testing.assert_array_equal(got, a)
     assert bio.read_header(p) is None
+
+
+def _cdf1_tomo():
+    """A NetCDF classic (CDF-1) file laid out byte by byte after the Unidata
+    format grammar: dims z=2, x=3; no global attributes; one short variable
+    'tomo' over (z, x) with 6 big-endian values."""
+    head = (b"CDF\x01" + struct.pack(">I", 0)                            # magic, numrecs
+            + struct.pack(">II", 0x0A, 2)                                 # NC_DIMENSION, 2 dims
+            + struct.pack(">I", 1) + b"z\0\0\0" + struct.pack(">I", 2)
+            + struct.pack(">I", 1) + b"x\0\0\0" + struct.pack(">I", 3)
+            + struct.pack(">II", 0, 0)                                    # no global attributes
+            + struct.pack(">II", 0x0B, 1)                                 # NC_VARIABLE, 1 var
+            + struct.pack(">I", 4) + b"tomo" + struct.pack(">III", 2, 0, 1)
+            + struct.pack(">II", 0, 0) + struct.pack(">II", 3, 12))       # no attributes; short, vsize
+    begin = len(head) + 4
+    return head + struct.pack(">I", begin) + struct.pack(">6h", -3, 0, 7, 300, -32768, 32767)
+
+
+def test_netcdf_cdf1_known_bytes(tmp_path):
+    (tmp_path / "t.nc").write_bytes(_cdf1_tomo())
+    got, ext = bio.load_data(str(tmp_path / "t.nc"))
+    assert ext == ".nc" and got.dtype == np.int16
+    np.testing.assert_array_equal(got, [[-3, 0, 7], [300, -32768, 32767]])
+
+
+def test_netcdf_bz2_hdf5_and_missing_variable(tmp_path):
+    import bz2
+    (tmp_path / "t.nc.bz2").write_bytes(bz2.compress(_cdf1_tomo()))
+    assert bio.extension(str(tmp_path / "t.nc.bz2")) == ".nc"
+    np.testing.assert_array_equal(bio.load_data(str(tmp_path / "t.nc.bz2"))[0][1], [300, -32768, 32767])
+    (tmp_path / "h.nc").write_bytes(b"\x89HDF\r\n\x1a\n" + b"\0" * 64)
+    with pytest.raises(NotImplementedError):
+        bio.load_data(str(tmp_path / "h.nc"))
+    # a folder's blocks must hold labels, segmented or tomo
+    d = tmp_path / "blocks"
+    d.mkdir()
+    (d / "a.nc").write_bytes(_cdf1_tomo().replace(b"tomo", b"misc"))
+    with pytest.raises(ValueError):
+        bio.load_data(str(d))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint16, np.int32, np.float32])
+def test_netcdf_write_read_and_unsigned(tmp_path, dtype):
+    a = (np.random.default_rng(7).random((3, 4, 5)) * 250).astype(dtype)
+    p = str(tmp_path / "r.nc")
+    bio.save_data(p, a, header={"nc": "segmented", "files": [("r.nc", 3)]})
+    buf = open(p, "rb").read()
+    assert buf[:4] == b"CDF\x02" and b"segmented" in buf
+    assert (b"_Unsigned" in buf) == (np.dtype(dtype).kind == "u")
+    got, _ = bio.load_data(p)
+    assert got.dtype == np.dtype(dtype)
+    np.testing.assert_array_equal(got, a)
+
+
+def test_netcdf_block_folder_round_trip(tmp_path):
+    a = (np.random.default_rng(8).random((7, 4, 3)) * 9).astype(np.uint8)
+    src = tmp_path / "labels"
+    src.mkdir()
+    (src / "block_10.nc").write_bytes(bio._nc_bytes(a[3:], "labels"))      # plain sort: 10 before 2
+    (src / "block_2.nc").write_bytes(bio._nc_bytes(a[:3], "labels"))
+    got, ext = bio.load_data(str(src))
+    assert ext == ""
+    np.testing.assert_array_equal(got, np.concatenate([a[3:], a[:3]]))
+    hdr = bio.read_header(str(src))
+    assert hdr == {"nc": "labels", "files": [("block_10.nc", 4), ("block_2.nc", 3)]}
+    out = tmp_path / "final.scan"
+    bio.save_data(str(out), got, header=hdr, file_type="")
+    assert sorted(os.listdir(out)) == ["block_10.nc", "block_2.nc"]
+    np.testing.assert_array_equal(bio.load_data(str(out))[0], got)
+
+
+def test_netcdf_record_variable():
+    """A record variable (first dimension unlimited) that is the only one:
+    records are stored unpadded, back to back."""
+    vals = np.arange(2 * 3, dtype=">i2").reshape(2, 3)
+    head = (b"CDF\x01" + struct.pack(">I", 2) + struct.pack(">II", 0x0A, 2)
+            + struct.pack(">I", 1) + b"t\0\0\0" + struct.pack(">I", 0)
+            + struct.pack(">I", 1) + b"x\0\0\0" + struct.pack(">I", 3)
+            + struct.pack(">II", 0, 0) + struct.pack(">II", 0x0B, 1)
+            + struct.pack(">I", 3) + b"val\0" + struct.pack(">III", 2, 0, 1)
+            + struct.pack(">II", 0, 0) + struct.pack(">II", 3, 8))
+    buf = head + struct.pack(">I", len(head) + 4) + vals.tobytes()
+    name, got = bio._nc_array(buf)
+    assert name == "val"
+    np.testing.assert_array_equal(got, vals)
