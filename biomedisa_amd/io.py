@@ -34,7 +34,7 @@ the formats this package reads and writes itself.
                  become gray by the reference's rgb2gray, :357-364); results
                  are written as slices with the label stack's file names.
 
-Other extensions (NetCDF-4/HDF5, JPEG slices, compressed DICOM, Zarr with
+Other extensions (NetCDF-4/HDF5, JPEG slices, JPEG-family DICOM, Zarr with
 other codecs) raise
 NotImplementedError.
 """
@@ -940,7 +940,8 @@ def _natural_key(s):
 # in explicit-VR encodings (PS3.5 section 7.1.2)
 _DCM_LONG_VR = {b"OB", b"OD", b"OF", b"OL", b"OV", b"OW", b"SQ", b"UC", b"UN", b"UR", b"UT", b"SV", b"UV"}
 _DCM_TS = {"1.2.840.10008.1.2": ("<", False), "1.2.840.10008.1.2.1": ("<", True),
-           "1.2.840.10008.1.2.2": (">", True), "1.2.840.10008.1.2.1.99": ("<", True)}
+           "1.2.840.10008.1.2.2": (">", True), "1.2.840.10008.1.2.1.99": ("<", True),
+           "1.2.840.10008.1.2.5": ("<", True)}                 # RLE Lossless (encapsulated)
 _DCM_WANT = {(0x0028, 0x0002): "spp", (0x0028, 0x0006): "planar", (0x0028, 0x0008): "frames",
              (0x0028, 0x0010): "rows", (0x0028, 0x0011): "cols", (0x0028, 0x0100): "alloc",
              (0x0028, 0x0101): "stored", (0x0028, 0x0103): "signed", (0x0028, 0x1052): "inter",
@@ -973,9 +974,21 @@ def _dcm_elements(buf, pos, end, bo, explicit, out, depth=0):
         else:
             n = struct.unpack(bo + "I", buf[pos + 4:pos + 8])[0]
             pos += 8
+        if n == 0xFFFFFFFF and (g, e) == (0x7FE0, 0x0010) and depth == 0:
+            # encapsulated pixel data: the offset table item, then fragments
+            frags = []
+            while pos + 8 <= len(buf):
+                ig, ie, k = struct.unpack("<HHI", buf[pos:pos + 8])
+                pos += 8
+                if (ig, ie) == (0xFFFE, 0xE0DD):
+                    break
+                if (ig, ie) != (0xFFFE, 0xE000):
+                    raise ValueError("DICOM: bad item in encapsulated pixel data")
+                frags.append(buf[pos:pos + k])
+                pos += k
+            out["encapsulated"] = frags[1:]
+            continue
         if n == 0xFFFFFFFF:
-            if (g, e) == (0x7FE0, 0x0010):
-                raise NotImplementedError("DICOM with encapsulated (compressed) pixel data")
             pos = _dcm_elements(buf, pos, None, bo, explicit, out, depth + 1)
             continue
         key = _DCM_WANT.get((g, e)) if depth == 0 else None
@@ -986,9 +999,9 @@ def _dcm_elements(buf, pos, end, bo, explicit, out, depth=0):
 
 
 def _read_dicom(buf, name="DICOM"):
-    """One uncompressed DICOM image (PS3.10 file, or a bare implicit-VR data
-    set): implicit or explicit VR little endian, explicit big endian or
-    deflated explicit little endian; 8/16/32-bit samples with the bits above
+    """One DICOM image (PS3.10 file, or a bare implicit-VR data set):
+    implicit or explicit VR little endian, explicit big endian, deflated
+    explicit little endian or RLE Lossless; 8/16/32-bit samples with the bits above
     BitsStored masked (unsigned) or sign-extended (signed); rescale slope and
     intercept applied as GDCM does (identity: the stored type; integral:
     int32; else float64); three samples per pixel made gray by _rgb2gray."""
@@ -1027,6 +1040,13 @@ def _read_dicom(buf, name="DICOM"):
     rows, cols, alloc = num("rows"), num("cols"), num("alloc")
     spp, signed, frames = num("spp", 1), num("signed", 0), int(num("frames", 1))
     stored = num("stored", alloc)
+    if "encapsulated" in el:
+        if ts != "1.2.840.10008.1.2.5" or frames != 1 or alloc not in (8, 16, 32) or spp not in (1, 3):
+            raise NotImplementedError("%s: encapsulated DICOM pixel data in %s" % (name, ts))
+        el["pixels"] = (None, _dcm_rle(b"".join(el["encapsulated"]), rows * cols, spp, alloc // 8, name))
+        bo, planar_rle = "<", True
+    else:
+        planar_rle = False
     if "pixels" not in el:
         raise ValueError("%s: DICOM without pixel data" % name)
     if alloc not in (8, 16, 32) or spp not in (1, 3) or frames != 1 or not 1 <= stored <= alloc:
@@ -1045,7 +1065,8 @@ def _read_dicom(buf, name="DICOM"):
         else:
             px = px & px.dtype.type((1 << stored) - 1)
     if spp == 3:
-        px = px.reshape(3, rows, cols).transpose(1, 2, 0) if num("planar", 0) else px.reshape(rows, cols, 3)
+        px = (px.reshape(3, rows, cols).transpose(1, 2, 0) if num("planar", 0) or planar_rle
+              else px.reshape(rows, cols, 3))
         px = _rgb2gray(np.ascontiguousarray(px))
     else:
         px = px.reshape(rows, cols)
@@ -1055,6 +1076,41 @@ def _read_dicom(buf, name="DICOM"):
     if float(slope).is_integer() and float(inter).is_integer():
         return (px.astype(np.int64) * int(slope) + int(inter)).astype(np.int32)
     return px.astype(np.float64) * slope + inter
+
+
+def _dcm_rle(frag, npix, spp, bps, name="DICOM"):
+    """RLE Lossless frame (PS3.5 annex G): a 64-byte header (segment count,
+    15 offsets), then PackBits segments, most significant byte of each sample
+    first and, for colour, one plane after another -> little-endian sample
+    bytes (planes in sample order)."""
+    if len(frag) < 64:
+        raise ValueError("%s: RLE frame shorter than its header" % name)
+    head = struct.unpack("<16I", frag[:64])
+    nseg = head[0]
+    if nseg != spp * bps or nseg > 15:
+        raise NotImplementedError("%s: RLE with %d segments (want %d)" % (name, nseg, spp * bps))
+    offs = list(head[1:1 + nseg]) + [len(frag)]
+    segs = []
+    for k in range(nseg):
+        src, out, i = frag[offs[k]:offs[k + 1]], bytearray(), 0
+        while i < len(src) and len(out) < npix:
+            h = src[i] - 256 if src[i] > 127 else src[i]
+            if h >= 0:
+                out += src[i + 1:i + 2 + h]
+                i += 2 + h
+            elif h > -128:
+                out += src[i + 1:i + 2] * (1 - h)
+                i += 2
+            else:
+                i += 1
+        if len(out) < npix:
+            raise ValueError("%s: RLE segment %d decodes to %d of %d bytes" % (name, k, len(out), npix))
+        segs.append(np.frombuffer(bytes(out[:npix]), np.uint8))
+    # bps segments per sample, from the most significant byte down
+    planes = []
+    for s0 in range(0, nseg, bps):
+        planes.append(np.stack(segs[s0:s0 + bps][::-1], axis=1).reshape(-1))
+    return b"".join(p.tobytes() for p in planes)
 
 
 def _dicom_bytes(img):

@@ -870,3 +870,44 @@ def test_netcdf_record_variable():
     name, got = bio._nc_array(buf)
     assert name == "val"
     np.testing.assert_array_equal(got, vals)
+
+
+def _rle_dicom(rows, cols, alloc, spp, segments):
+    """An RLE Lossless DICOM file: explicit VR little endian data set whose
+    pixel data is encapsulated (empty offset table item, one fragment holding
+    the 64-byte RLE header and the given PackBits segments)."""
+    head = [len(segments)] + [0] * 15
+    pos = 64
+    for k, s in enumerate(segments):
+        head[1 + k] = pos
+        pos += len(s)
+    frag = struct.pack("<16I", *head) + b"".join(segments)
+    frag += b"\0" * (len(frag) % 2)
+    us = lambda v: struct.pack("<H", v)  # noqa: E731
+    ds = (_dcm_el(0x28, 2, us(spp), b"US") + _dcm_el(0x28, 0x10, us(rows), b"US")
+          + _dcm_el(0x28, 0x11, us(cols), b"US") + _dcm_el(0x28, 0x100, us(alloc), b"US")
+          + _dcm_el(0x28, 0x103, us(0), b"US")
+          + struct.pack("<HH2sHI", 0x7FE0, 0x10, b"OB", 0, 0xFFFFFFFF)
+          + struct.pack("<HHI", 0xFFFE, 0xE000, 0) + struct.pack("<HHI", 0xFFFE, 0xE000, len(frag)) + frag
+          + struct.pack("<HHI", 0xFFFE, 0xE0DD, 0))
+    return _dcm_file(b"1.2.840.10008.1.2.5", ds)
+
+
+def test_dicom_rle_16bit():
+    # 2 x 3 samples 0x0102, 0x0102, 0x0102, 0x0A0B, 0x0C0D, 0x0E0F:
+    # high bytes 01 01 01 0A 0C 0E = replicate 3 x 01 (header -2), literal 3 (header 2)
+    # low bytes  02 02 02 0B 0D 0F likewise; a -128 no-op byte in between
+    hi = bytes([0xFE, 0x01, 0x80, 0x02, 0x0A, 0x0C, 0x0E])
+    lo = bytes([0xFE, 0x02, 0x02, 0x0B, 0x0D, 0x0F])
+    got = bio._read_dicom(_rle_dicom(2, 3, 16, 1, [hi, lo]))
+    np.testing.assert_array_equal(got, [[0x0102, 0x0102, 0x0102], [0x0A0B, 0x0C0D, 0x0E0F]])
+
+
+def test_dicom_rle_rgb_planes():
+    # one 8-bit segment per colour plane, 1 x 6 pixels of _RGB_KNOWN
+    rgb = np.array([c for c, _ in _RGB_KNOWN], np.uint8)
+    segs = [bytes([5]) + rgb[:, k].tobytes() for k in range(3)]      # literal runs of 6
+    got = bio._read_dicom(_rle_dicom(1, 6, 8, 3, segs))
+    np.testing.assert_array_equal(got, [[g for _, g in _RGB_KNOWN]])
+    with pytest.raises(NotImplementedError):
+        bio._read_dicom(_rle_dicom(1, 6, 8, 3, segs[:2]))               # segment count mismatch
