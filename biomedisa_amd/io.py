@@ -826,12 +826,14 @@ def _rgb2gray(rgb):
 
 
 def _read_png(buf, name="PNG"):
-    """Grayscale (optionally with alpha, which is dropped) or RGB (made gray
-    by _rgb2gray), 8/16-bit, non-interlaced PNG -> (rows, cols) array."""
+    """Non-interlaced PNG -> (rows, cols) array: grayscale (1/2/4-bit samples
+    scaled to 8 bits by bit replication, as libpng's gray expansion does;
+    alpha dropped), RGB, or a palette expanded to RGB; colour is made gray by
+    _rgb2gray."""
     from . import _hip
     if buf[:8] != _PNG_SIG:
         raise ValueError("%s: not a PNG file" % name)
-    pos, idat, ihdr = 8, [], None
+    pos, idat, ihdr, plte, trns = 8, [], None, None, False
     while pos < len(buf):
         n, typ = struct.unpack(">I4s", buf[pos:pos + 8])
         body = buf[pos + 8:pos + 8 + n]
@@ -839,26 +841,42 @@ def _read_png(buf, name="PNG"):
             ihdr = struct.unpack(">IIBBBBB", body)
         elif typ == b"IDAT":
             idat.append(body)
+        elif typ == b"PLTE":
+            plte = np.frombuffer(body, np.uint8).reshape(-1, 3)
+        elif typ == b"tRNS":
+            trns = True
         elif typ == b"IEND":
             break
         pos += 12 + n
     if ihdr is None:
         raise ValueError("%s: no IHDR chunk" % name)
     W, H, depth, ctype, _, _, interlace = ihdr
-    if ctype not in (0, 2, 4) or depth not in (8, 16) or interlace:
+    ok = {0: (1, 2, 4, 8, 16), 2: (8, 16), 3: (1, 2, 4, 8), 4: (8, 16)}
+    if ctype not in ok or depth not in ok[ctype] or interlace or (ctype == 3 and (plte is None or trns)):
         raise NotImplementedError("%s: PNG colour type %d, depth %d, interlace %d" % (name, ctype, depth, interlace))
-    chans = {0: 1, 2: 3, 4: 2}[ctype]
-    bpp = chans * depth // 8
+    chans = {0: 1, 2: 3, 3: 1, 4: 2}[ctype]
+    bpp = max(1, chans * depth // 8)
     raw = np.frombuffer(zlib.decompress(b"".join(idat)), np.uint8)
-    rowbytes = W * bpp
+    rowbytes = (W * chans * depth + 7) // 8
     if raw.size < H * (rowbytes + 1):
         raise ValueError("%s: truncated image data" % name)
     out = np.empty(H * rowbytes, np.uint8)
     rc = _hip.load().bmx_png_unfilter(raw.ctypes.data, H, rowbytes, bpp, out.ctypes.data)
     if rc != 0:
         raise ValueError(_hip.load().bmx_last_error().decode(errors="replace"))
-    img = out.view(">u2" if depth == 16 else "u1").reshape(H, W, chans)
-    img = img.astype(img.dtype.newbyteorder("="))
+    if depth < 8:                              # packed samples, most significant first
+        img = np.unpackbits(out.reshape(H, rowbytes), axis=1).reshape(H, -1, depth)[:, :W]
+        img = (img * (1 << np.arange(depth - 1, -1, -1, dtype=np.uint8))).sum(axis=2, dtype=np.uint8)
+        if ctype == 0:
+            return img * np.uint8(255 // ((1 << depth) - 1))
+        img = img[:, :, None]
+    else:
+        img = out.view(">u2" if depth == 16 else "u1").reshape(H, W, chans)
+        img = img.astype(img.dtype.newbyteorder("="))
+    if ctype == 3:
+        if int(img.max()) >= len(plte):
+            raise ValueError("%s: PNG index beyond its palette" % name)
+        return _rgb2gray(plte[img[:, :, 0]])
     return _rgb2gray(img) if ctype == 2 else img[:, :, 0]
 
 

@@ -911,3 +911,26 @@ def test_dicom_rle_rgb_planes():
     np.testing.assert_array_equal(got, [[g for _, g in _RGB_KNOWN]])
     with pytest.raises(NotImplementedError):
         bio._read_dicom(_rle_dicom(1, 6, 8, 3, segs[:2]))               # segment count mismatch
+
+
+def _png_raw(W, H, depth, ctype, rows, extra=b""):
+    """A PNG with filter type 0 on every row, chunks assembled by hand."""
+    raw = b"".join(b"\0" + bytes(r) for r in rows)
+    return (b"\x89PNG\r\n\x1a\n" + _png_chunk(b"IHDR", struct.pack(">IIBBBBB", W, H, depth, ctype, 0, 0, 0))
+            + extra + _png_chunk(b"IDAT", zlib.compress(raw)) + _png_chunk(b"IEND", b""))
+
+
+def test_png_palette_and_low_bit_gray():
+    plte = _png_chunk(b"PLTE", bytes(c for rgb, _ in _RGB_KNOWN for c in rgb))
+    # 2-bit indices 0 1 2 3 | 3 2 1 0 packed most significant first; 5 pixels per row -> 2 bytes
+    img = bio._read_png(_png_raw(5, 2, 2, 3, [[0b00011011, 0b00000000], [0b11100100, 0b01000000]], plte))
+    g = [gv for _, gv in _RGB_KNOWN]
+    np.testing.assert_array_equal(img, [[g[0], g[1], g[2], g[3], g[0]], [g[3], g[2], g[1], g[0], g[1]]])
+    np.testing.assert_array_equal(bio._read_png(_png_raw(3, 1, 8, 3, [[5, 4, 3]], plte)), [[g[5], g[4], g[3]]])
+    # 1-bit and 4-bit gray scale to 8 bits: 1 -> 255, 4-bit 0x7 -> 119, 0xF -> 255
+    np.testing.assert_array_equal(bio._read_png(_png_raw(3, 1, 1, 0, [[0b10100000]])), [[255, 0, 255]])
+    np.testing.assert_array_equal(bio._read_png(_png_raw(3, 1, 4, 0, [[0x7F, 0x00]])), [[119, 255, 0]])
+    with pytest.raises(NotImplementedError):
+        bio._read_png(_png_raw(3, 1, 8, 3, [[0, 1, 2]], plte + _png_chunk(b"tRNS", b"\0")))
+    with pytest.raises(ValueError):
+        bio._read_png(_png_raw(1, 1, 8, 3, [[9]], plte))
