@@ -58,18 +58,36 @@ def measured_traffic(args):
     return float(json.loads(block)["traffic_bytes_per_launch"])
 
 
+COUNTERS_PROFILE = os.path.join(HERE, "profiles", "k_tile_walk_512_counters.md")
+
+
+def measured_valu(args):
+    """VALU issue figures of the plain walker from the committed rocprofv3 counter
+    summary of this workload (None for other sizes or when absent)."""
+    if (args.size, args.nbrw, args.sorw) != (512, 10, 4000) or not os.path.exists(COUNTERS_PROFILE):
+        return None
+    txt = open(COUNTERS_PROFILE).read()
+    i = txt.find("```json")
+    if i < 0:
+        return None
+    return json.loads(txt[i + 7:txt.index("```", i + 7)]).get("valu")
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def prepare(args):
+def prepare(args, vol=None):
+    """The benchmark volume and its host-prepared walk inputs (cropped padded
+    image, label-index volume), as the CPU baseline and the post workload use."""
     from biomedisa_amd.synth import structures_volume
     from biomedisa_amd import helpers as H
-    t = time.perf_counter()
-    n = args.size
-    vol = structures_volume(shape=(n, n, n), seed=2, n_struct=8, radius=(5, 60), every=10,
-                            allaxis=True)
-    log("volume %d^3 generated in %.1fs" % (n, time.perf_counter() - t))
+    if vol is None:
+        t = time.perf_counter()
+        n = args.size
+        vol = structures_volume(shape=(n, n, n), seed=2, n_struct=8, radius=(5, 60), every=10,
+                                allaxis=True)
+        log("volume %d^3 generated in %.1fs" % (n, time.perf_counter() - t))
     labels = vol.labels
     bm = H.Biomedisa()
     bm.data, bm.labelData, bm.only, bm.ignore, bm.success = vol.img, labels, "all", "none", True
@@ -117,7 +135,8 @@ def time_pipeline(step, eng, warmup, steps, label):
         out = step(tm)
         tms.append(tm)
         log("%s step %d: %.2fs  %s" % (label, k, time.perf_counter() - ts,
-                                       " ".join("%s=%.3g" % kv for kv in sorted(tm.items()))))
+                                       " ".join("%s=%.3g" % kv for kv in sorted(tm.items())
+                                                if isinstance(kv[1], (int, float)))))
     eng.sync()
     return out, tms, (time.perf_counter() - t0) / steps
 
@@ -149,6 +168,10 @@ def main():
     ap.add_argument("--weights", default=None)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-numpy", action="store_true",
+                    help="skip the drop-in calls with NumPy arrays in and out (H2D and D2H included)")
+    ap.add_argument("--no-reference-stream", action="store_true",
+                    help="skip the Dice of the Philox walker against the reference stream at this size")
     ap.add_argument("--workload", choices=["512", "batch", "large", "post"], default="512")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -166,97 +189,108 @@ def main():
     if args.workload == "post":
         return bench_post(args)
 
-    from biomedisa_amd import _hip
-    from biomedisa_amd.helpers import per_label_dice
-    from biomedisa_amd.interpolation import DeviceVolume, run_pipeline, run_pipeline_guided
+    from biomedisa_amd import _hip, surrogate
+    from biomedisa_amd.interpolation import (GUIDED_ROBUST, DeviceArray, smart_interpolation,
+                                             smart_interpolation_device)
 
-    vol, img_c, labidx, values, nol = prepare(args)
+    from biomedisa_amd.synth import structures_volume
+    t = time.perf_counter()
+    n = args.size
+    vol = structures_volume(shape=(n, n, n), seed=2, n_struct=8, radius=(5, 60), every=10, allaxis=True)
+    log("volume %d^3 generated in %.1fs" % (n, time.perf_counter() - t))
     eng = _hip.engine(0)
-    dv = DeviceVolume(eng, img_c, labidx)
-    log("cropped volume %s, %d labels, uploaded" % (dv.dims, nol))
-    n_vox = args.size ** 3
+    if args.weights:
+        surrogate._nets[(0, surrogate.WEIGHTS_PATH)] = surrogate.net(0, args.weights)
+    net = surrogate.net(0)
+    # the caller's arrays, resident in HBM before any timed call
+    img_d, lab_d = DeviceArray.upload(eng, vol.img), DeviceArray.upload(eng, vol.labels)
+    eng.sync()
+    n_vox = n ** 3
+    common = dict(nbrw=args.nbrw, sorw=args.sorw, allaxis=True, smooth=args.smooth)
+    guide = dict(thr=args.thr, brad=args.brad, srad=args.srad, far=args.far, far_walks=args.far_walks,
+                 **guided_kw(args))
+    robust = dict(GUIDED_ROBUST, thr=args.thr, brad=args.brad, far=args.far, repair_min=args.repair_min)
+    methods = [("plain", "plain", None), ("guided", "guided", guide)]
+    if args.method == "plain":
+        methods = methods[:1]
+    elif args.method == "guided":
+        methods = methods[1:]
+    if args.method != "plain" and not args.no_robust:
+        methods.append(("robust", "guided", robust))
+
+    def resident_step(method, g):
+        def step(tm):
+            r = smart_interpolation_device(img_d, lab_d, method=method, guide=g, timings=tm, **common)
+            eng.sync()
+            return r
+        return step
+
     res = {}
+    for name, method, g in methods:
+        out, tms, sec = time_pipeline(resident_step(method, g), eng, args.warmup, args.steps, name)
+        res[name] = dict(sec=sec, tms=tms, regular=out["regular"].download(), smooth=out["smooth"].download())
+        del out
+    # the drop-in call itself, NumPy arrays in and out (H2D and D2H included)
+    for name, method, g in ([] if args.no_numpy else methods[:2]):
+        tm = {}
+        t = time.perf_counter()
+        r = smart_interpolation(vol.img, vol.labels, method=method, guide=g, timings=tm, **common)
+        res[name]["numpy_sec"] = time.perf_counter() - t
+        res[name]["numpy_identical"] = bool(np.array_equal(r["regular"], res[name]["regular"])
+                                            and np.array_equal(r["smooth"], res[name]["smooth"]))
+        log("%s, NumPy in and out: %.2fs (identical to the resident call: %s)"
+            % (name, res[name]["numpy_sec"], res[name]["numpy_identical"]))
+        del r
+    img_d.free()
 
-    if args.method in ("both", "plain"):
-        def plain_step(tm):
-            return run_pipeline(dv, nol, values, nbrw=args.nbrw, sorw=args.sorw, allaxis=True,
-                                smooth=args.smooth, timings=tm)
-        out, tms, sec = time_pipeline(plain_step, eng, args.warmup, args.steps, "plain")
-        res["plain"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], accepted=out["accepted"],
-                            regular=out["regular"].download(dv.dims, np.uint8))
-    if args.method in ("both", "guided"):
-        from biomedisa_amd import surrogate
-        net = surrogate.Net(eng)
-        net.load(args.weights or surrogate.WEIGHTS_PATH)
-
-        def guided_step(tm):
-            return run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw,
-                                       allaxis=True, smooth=args.smooth, thr=args.thr,
-                                       brad=args.brad, srad=args.srad, far=args.far,
-                                       far_walks=args.far_walks, timings=tm, **guided_kw(args))
-        out, tms, sec = time_pipeline(guided_step, eng, args.warmup, args.steps, "guided")
-        res["guided"] = dict(sec=sec, tms=tms, nseeds=out["nseeds"], walked=out["walked_seeds"],
-                             ring=out["ring_seeds"], mid_ring=out["mid_ring_seeds"], repair=out["repair_seeds"],
-                             repair_voxels=out["repair_voxels"],
-                             repair_ring=out["repair_ring_seeds"], accepted=out["accepted"],
-                             regular=out["regular"].download(dv.dims, np.uint8))
-        if not args.no_robust:
-            from biomedisa_amd.interpolation import GUIDED_ROBUST
-
-            def robust_step(tm):
-                return run_pipeline_guided(dv, nol, values, net, nbrw=args.nbrw, sorw=args.sorw, allaxis=True,
-                                           smooth=args.smooth, thr=args.thr, brad=args.brad, far=args.far,
-                                           repair_min=args.repair_min, timings=tm, **GUIDED_ROBUST)
-            out, tms, sec = time_pipeline(robust_step, eng, args.warmup, args.steps, "guided (robust preset)")
-            res["robust"] = dict(sec=sec, regular=out["regular"].download(dv.dims, np.uint8))
-
-    if args.calibrate_copy:
-        n_cal = 64 << 20
-        src, dst = eng.alloc(n_cal * 4), eng.alloc(n_cal * 2)
-        src.zero()
-        _hip.call("bmx_f32_to_bf16", eng.ctx, src.ptr, dst.ptr, n_cal)
-        eng.sync()
-        log("calibration copy: %d floats -> bf16" % n_cal)
-
-    inner = (slice(1, -1),) * 3
     main_key = "guided" if "guided" in res else "plain"
     r = res[main_key]
-    value = n_vox / r["sec"]
     config = {"workload": "%d^3 uint8, 8 ellipsoids/tubes, labels every 10th plane allaxis, "
                           "nbrw=%d sorw=%d smooth=%d" % (args.size, args.nbrw, args.sorw, args.smooth),
-              "method": main_key}
-    if "guided" in res:
-        g = res["guided"]
-        config.update(guided_seconds=g["sec"], walked_seed_fraction=g["walked"] / g["nseeds"],
-                      ring_seed_fraction=g["ring"] / g["nseeds"], far=args.far, far_walks=args.far_walks,
-                      mid_ring_seed_fraction=g["mid_ring"] / g["nseeds"], mid=args.mid, mid_walks=args.mid_walks,
-                      repair=args.repair, repair_min=args.repair_min, repair_far=args.repair_far,
-                      repair_srad=args.repair_srad,
-                      repair_far_walks=args.repair_far_walks if args.repair_far_walks >= 0 else args.far_walks,
-                      repair_seed_fraction=g["repair"] / g["nseeds"], repair_voxels=g["repair_voxels"],
-                      temperature=net.T, thr=args.thr, brad=args.brad, srad=args.srad,
-                      guided_timings=g["tms"][-1])
-        config["min_dice_vs_ground_truth_guided"] = min(per_label_dice(g["regular"][inner], vol.gt).values())
+              "method": main_key,
+              "timed_span": "smart_interpolation_device: the caller's uint8 image and label arrays resident in "
+                            "HBM -> 'regular' and 'smooth' (Z,Y,X) volumes in HBM; label set and box, pad and "
+                            "crop, seeds, walk, merge, smoothing, result placement and the Dice check inside",
+              "seconds": r["sec"]}
+    for name in res:
+        tm = res[name]["tms"][-1]
+        st = tm.get("stats", {})
+        config["%s_seconds" % name] = res[name]["sec"]
+        config["%s_phases" % name] = {k: v for k, v in tm.items() if k != "stats"}
+        if "numpy_sec" in res[name]:
+            config["%s_numpy_in_out_seconds" % name] = res[name]["numpy_sec"]
+            config["%s_numpy_identical" % name] = res[name]["numpy_identical"]
+        if name != "plain" and st.get("nseeds"):
+            config["%s_walked_seed_fraction" % name] = st.get("walked_seeds", 0) / st["nseeds"]
+            config["%s_ring_seed_fraction" % name] = st.get("ring_seeds", 0) / st["nseeds"]
+            config["%s_repair_voxels" % name] = st.get("repair_voxels", 0)
+            config["%s_fallback" % name] = st.get("fallback", False)
+    config["guided_options"] = dict(guide, temperature=net.T)
+    inner = (slice(1, -1),) * 3
+    for name in res:
+        config["min_dice_vs_ground_truth_%s" % name] = min(dice_by_confusion(
+            np.ascontiguousarray(res[name]["regular"]), vol.gt).values())
     if "plain" in res:
         p = res["plain"]
-        config.update(plain_seconds=p["sec"], plain_voxels_per_s=n_vox / p["sec"],
-                      plain_timings=p["tms"][-1])
-        config["min_dice_vs_ground_truth_plain"] = min(per_label_dice(p["regular"][inner], vol.gt).values())
-    if "plain" in res and "guided" in res:
-        d = per_label_dice(res["guided"]["regular"], res["plain"]["regular"])
-        config.update(speedup_vs_plain=res["plain"]["sec"] / res["guided"]["sec"],
-                      min_dice_guided_vs_plain=min(d.values()),
-                      dice_guided_vs_plain={str(k): round(v, 5) for k, v in d.items()})
-        if "robust" in res:
-            from biomedisa_amd.interpolation import GUIDED_ROBUST
-            dr = per_label_dice(res["robust"]["regular"], res["plain"]["regular"])
-            config["guided_robust_preset"] = dict(GUIDED_ROBUST, seconds=res["robust"]["sec"],
-                                                  speedup_vs_plain=res["plain"]["sec"] / res["robust"]["sec"],
-                                                  min_dice_vs_plain=min(dr.values()))
+        config.update(plain_voxels_per_s=n_vox / p["sec"])
+        for name in [k for k in res if k != "plain"]:
+            g = res[name]
+            d = dice_by_confusion(g["regular"], p["regular"])
+            ds = dice_by_confusion(g["smooth"], p["smooth"])
+            pre = "" if name == "guided" else name + "_"
+            config[pre + "speedup_vs_plain"] = p["sec"] / g["sec"]
+            if "numpy_sec" in g and "numpy_sec" in p:
+                config[pre + "speedup_vs_plain_numpy_in_out"] = p["numpy_sec"] / g["numpy_sec"]
+            config[pre + "min_dice_guided_vs_plain"] = min(d.values())
+            config[pre + "min_dice_smooth_guided_vs_plain"] = min(ds.values())
+            config[pre + "min_dice_guided_vs_plain_regular_and_smooth"] = min(min(d.values()), min(ds.values()))
+            config[pre + "dice_guided_vs_plain"] = {str(k): round(v, 5) for k, v in d.items()}
+            config[pre + "dice_smooth_guided_vs_plain"] = {str(k): round(v, 5) for k, v in ds.items()}
+    del inner
 
     line = {
         "metric": "segmented voxels/s at 512^3, 8 labels; surrogate-guided vs plain at Dice>=0.99",
-        "value": value,
+        "value": n_vox / r["sec"],
         "unit": "voxels/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -269,35 +303,45 @@ def main():
         "data": "synthetic",
         "config": config,
     }
-    # roofline of the dominant kernel: the walker of the plain pipeline (full seed set)
-    rk = res["plain"] if "plain" in res else res["guided"]
+    # roofline of the dominant kernel: the walker of the plain pipeline (every seed walks)
+    rk = res["plain"] if "plain" in res else res[main_key]
     tm = rk["tms"][-1]
-    walked = rk.get("walked", rk["nseeds"])
-    rfw = args.repair_far_walks if args.repair_far_walks >= 0 else args.far_walks
-    nmid = rk.get("mid_ring") or 0
-    steps_total = ((walked + (rk.get("repair") or 0)) * args.nbrw + ((rk.get("ring") or 0) - nmid) * args.far_walks
-                   + nmid * args.mid_walks + (rk.get("repair_ring") or 0) * rfw) * args.sorw
-    walk_bytes = steps_total * 6 * 1 + rk["accepted"] * 8
+    st = tm["stats"]
+    if "plain" in res:
+        steps_total = st["nseeds"] * args.nbrw * args.sorw
+    else:
+        steps_total = None
     launches = max(int(tm["walk_launches"]), 1)
     kernel_s = tm["walk_kernel_ms"] / 1000.0
-    achieved = walk_bytes / kernel_s / 1e9
-    line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                        "traffic_source": "profiles/k_tile_walk_512_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                          "scaled by a copy-kernel calibration)",
-                        "kernel": "k_tile_walk<uint8_t, 0, true> (persistent; rounds replayed in a HIP graph, "
-                                  "kernel time = summed launch spans on the device wall clock)",
-                        "walker_steps": steps_total,
-                        "accepted_moves": rk["accepted"], "launches": launches,
-                        "launch_avg_ms": tm["walk_kernel_ms"] / launches,
-                        "algorithmic_bytes_per_launch": walk_bytes / launches}
+    if steps_total is not None:
+        walk_bytes = steps_total * 6 * 1 + st["accepted"] * 8
+        achieved = walk_bytes / kernel_s / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                            "traffic_source": "profiles/k_tile_walk_512_traffic.md (rocprofv3 FETCH_SIZE + "
+                                              "WRITE_SIZE, scaled by a copy-kernel calibration)",
+                            "kernel": "k_tile_walk<uint8_t, 0, true> (persistent; rounds replayed in a HIP graph, "
+                                      "kernel time = summed launch spans on the device wall clock)",
+                            "walker_steps": steps_total, "steps_per_s": steps_total / kernel_s,
+                            "accepted_moves": st["accepted"], "launches": launches,
+                            "launch_avg_ms": tm["walk_kernel_ms"] / launches,
+                            "algorithmic_bytes_per_launch": walk_bytes / launches}
+        valu = measured_valu(args)
+        if valu is not None:
+            line["roofline"]["valu"] = valu
+    if not args.no_reference_stream and "plain" in res:
+        dref, ref_s = reference_stream_dice(eng, vol, res["plain"]["regular"], args)
+        config["min_dice_philox_vs_reference_stream"] = min(dref.values())
+        config["dice_philox_vs_reference_stream"] = {str(k): round(v, 5) for k, v in dref.items()}
+        config["reference_stream_walk_seconds"] = ref_s
     if not args.no_cpu_baseline:
-        seeds_buf, n = eng.seeds(dv.lab, dv.img, dv.img_dtype, dv.dims,
-                                 eng.plane_flags(dv.lab, dv.dims, True))
-        seeds = seeds_buf.download((n,), _hip.SEED_DTYPE)
-        rate, nsample, dt, cores = cpu_baseline(img_c, seeds, args.nbrw, args.sorw, nol,
-                                                args.cpu_budget)
-        walk_s = n * args.nbrw * args.sorw / rate
+        _, img_c, labidx, _, nol = prepare(args, vol)
+        from biomedisa_amd.interpolation import DeviceVolume
+        dv = DeviceVolume(eng, img_c, labidx)
+        seeds_buf, ns = eng.seeds(dv.lab, dv.img, dv.img_dtype, dv.dims, eng.plane_flags(dv.lab, dv.dims, True))
+        seeds = seeds_buf.download((ns,), _hip.SEED_DTYPE)
+        rate, nsample, dt, cores = cpu_baseline(img_c, seeds, args.nbrw, args.sorw, nol, args.cpu_budget)
+        walk_s = ns * args.nbrw * args.sorw / rate
         line["cpu_baseline"] = {
             "value": n_vox / walk_s, "unit": "voxels/s", "cores": cores, "kind": "port",
             "sample": "oracle ref_walk (C/OpenMP restatement of pyopencl_small.py randomWalk, "
@@ -305,6 +349,39 @@ def main():
                       "full plain walk extrapolated to %.0fs; smoothing/merge not counted"
                       % (nsample, dt, rate, walk_s)}
     print(json.dumps(line), flush=True)
+
+
+def reference_stream_dice(eng, vol, plain_regular, args, chunk=1 << 22):
+    """Per-label Dice between the Philox walker's 'regular' result and the
+    reference stream's (the float MRG recurrence of pyopencl_small.py:213-282,
+    one thread per seed, all walks in sequence) on the same volume; the seeds
+    are walked in chunks (each seed's stream is its own, so chunks change
+    nothing) so that no launch runs long."""
+    from biomedisa_amd import _hip
+    from biomedisa_amd.interpolation import _prepare_device
+    prep = _prepare_device(eng, vol.img, vol.labels, args.nbrw, args.sorw, False, None, True, "none", "all", 0,
+                           False, False)
+    dv = prep.vol
+    flags = eng.plane_flags(dv.lab, dv.dims, True)
+    seeds, ns = eng.seeds(dv.lab, dv.img, dv.img_dtype, dv.dims, flags)
+    hits = eng.alloc(prep.bm.nol * dv.V * 4)
+    hits.zero()
+    t = time.perf_counter()
+    for c0 in range(0, ns, chunk):
+        c1 = min(ns, c0 + chunk)
+        view = _hip.BufferView(seeds, c0 * _hip.SEED_DTYPE.itemsize, (c1 - c0) * _hip.SEED_DTYPE.itemsize)
+        eng.walk(dv.img, dv.img_dtype, dv.dims, view, c1 - c0, prep.bm.nol, args.nbrw, args.sorw, _hip.RNG_REF, 0,
+                 hits)
+        eng.sync()
+        log("reference stream: %d / %d seeds walked (%.1fs)" % (c1, ns, time.perf_counter() - t))
+    walk_s = time.perf_counter() - t
+    reg = eng.alloc(dv.V)
+    eng.argmax(hits, prep.bm.nol, dv.V, prep.values, reg)
+    hits.free()
+    full = eng.alloc(int(np.prod(prep.shape)))
+    eng.place(reg, prep.shape, prep.box, full)
+    ref_regular = full.download(prep.shape, np.uint8)
+    return dice_by_confusion(ref_regular, plain_regular), walk_s
 
 
 def guided_kw(args):

@@ -13,10 +13,17 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbmx_hip.so")
 
 BMX_OK = 0
+BMX_E_INVALID, BMX_E_OOM, BMX_E_DEVICE = -1, -2, -3
 IMG_U8, IMG_F32 = 0, 1
 RNG_PHILOX, RNG_REF = 0, 1
 WALK_AUTO, WALK_DIRECT, WALK_SORTED, WALK_TILED = 0, 1, 2, 3
 LABEL_NONE = 0xFF
+LABEL_CLASSES = 258          # BMX_LABEL_CLASSES: values 0..255, negative, above 255
+CLASS_NEG, CLASS_BIG = 256, 257
+# raw dtypes of the device pre-processing (BMX_RAW_*)
+RAW_DTYPES = {np.dtype(np.uint8): 0, np.dtype(np.int8): 1, np.dtype(np.uint16): 2, np.dtype(np.int16): 3,
+              np.dtype(np.uint32): 4, np.dtype(np.int32): 5, np.dtype(np.float32): 6}
+RAW_U8, RAW_I8, RAW_F32 = 0, 1, 6
 
 
 class WalkOpts(C.Structure):
@@ -46,6 +53,12 @@ SIGNATURES = {
     "bmx_download": (_i, [_p, _p, _p, C.c_size_t]),
     "bmx_zero": (_i, [_p, _p, C.c_size_t]),
     "bmx_copy": (_i, [_p, _p, _p, C.c_size_t]),
+    "bmx_label_stats": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _p]),
+    "bmx_image_range": (_i, [_p, _p, _i, _i64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bmx_crop_image": (_i, [_p, _p, _i, _i, _i64, _i64, _i64, _p, C.c_float, C.c_float, _p]),
+    "bmx_crop_labels": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _p, _p]),
+    "bmx_place": (_i, [_p, _p, _i64, _i64, _i64, _p, _p]),
+    "bmx_dice_counts": (_i, [_p, _p, _p, _i64, _i, _p, _p, _p]),
     "bmx_plane_flags": (_i, [_p, _p, _i64, _i64, _i64, _i, _p]),
     "bmx_seeds": (_i, [_p, _p, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
     "bmx_plane_flags_ex": (_i, [_p, _p, _i64, _i64, _i64, _i, _i, _p]),
@@ -210,6 +223,49 @@ class Engine:
         arr = np.ascontiguousarray(arr)
         assert arr.nbytes <= buf.nbytes
         _check(load().bmx_upload(self.ctx, buf.ptr, arr.ctypes.data, arr.nbytes))
+
+    # pre- and post-processing ---------------------------------------------
+    def label_stats(self, lab, raw_dtype, dims):
+        """(counts[258], boxes[258, 6]) of the label classes of a raw device label volume."""
+        Z, Y, X = dims
+        counts = np.zeros(LABEL_CLASSES, np.uint64)
+        boxes = np.zeros((LABEL_CLASSES, 6), np.int32)
+        _check(load().bmx_label_stats(self.ctx, lab.ptr, int(raw_dtype), Z, Y, X, counts.ctypes.data,
+                                      boxes.ctypes.data))
+        return counts, boxes
+
+    def image_range(self, img, raw_dtype, n):
+        lo, hi = C.c_double(), C.c_double()
+        _check(load().bmx_image_range(self.ctx, img.ptr, int(raw_dtype), int(n), C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def crop_image(self, img, raw_dtype, normalize, dims, box, vmin, vrange, out):
+        Z, Y, X = dims
+        b = np.ascontiguousarray(box, dtype=np.int64)
+        _check(load().bmx_crop_image(self.ctx, img.ptr, int(raw_dtype), int(bool(normalize)), Z, Y, X,
+                                     b.ctypes.data, float(vmin), float(vrange), out.ptr))
+
+    def crop_labels(self, lab, raw_dtype, dims, box, lut, out):
+        Z, Y, X = dims
+        b = np.ascontiguousarray(box, dtype=np.int64)
+        lut = np.ascontiguousarray(lut, dtype=np.uint8)
+        assert lut.size == LABEL_CLASSES
+        _check(load().bmx_crop_labels(self.ctx, lab.ptr, int(raw_dtype), Z, Y, X, b.ctypes.data,
+                                      lut.ctypes.data, out.ptr))
+
+    def place(self, crop, dims, box, out):
+        Z, Y, X = dims
+        b = np.ascontiguousarray(box, dtype=np.int64)
+        _check(load().bmx_place(self.ctx, crop.ptr, Z, Y, X, b.ctypes.data, out.ptr))
+
+    def dice_counts(self, labidx, result, n, nlabels, values, counted):
+        values = np.ascontiguousarray(values, dtype=np.uint8)
+        counted = np.ascontiguousarray(counted, dtype=np.uint8)
+        assert counted.size == 256
+        out = np.zeros(3, np.uint64)
+        _check(load().bmx_dice_counts(self.ctx, labidx.ptr, result.ptr, int(n), int(nlabels), values.ctypes.data,
+                                      counted.ctypes.data, out.ctypes.data))
+        return [int(v) for v in out]
 
     # path -----------------------------------------------------------------
     def plane_flags(self, lab, dims, allaxis, out=None, large=False):

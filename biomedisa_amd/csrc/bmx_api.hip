@@ -44,6 +44,7 @@ int bmx_open(int device, bmx_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_counter, 16 * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&c->d_small, kSmallBytes);
     if (e != hipSuccess) {
         free(c);
         return bmx_set_error(BMX_E_DEVICE, "bmx_open: %s", hipGetErrorString(e));
@@ -69,6 +70,7 @@ int bmx_close(bmx_ctx *ctx)
         free(ctx->seg_ev);
     }
     BMX_HIP(hipFree(ctx->d_counter));
+    BMX_HIP(hipFree(ctx->d_small));
     BMX_HIP(hipEventDestroy(ctx->ev0));
     BMX_HIP(hipEventDestroy(ctx->ev1));
     BMX_HIP(hipStreamDestroy(ctx->stream));

@@ -20,6 +20,7 @@ struct bmx_ctx {
     hipEvent_t ev0, ev1;     // bracket the last walker launch (bmx_last_walk_ms)
     float last_walk_ms;
     uint64_t *d_counter;     // 16 x uint64 scratch: accepted moves, seed counts, tiled-walk round control
+    void *d_small;           // kSmallBytes of scratch: label-class counts and boxes, min/max, lookup tables
     int64_t *d_block_counts; // seed compaction scratch
     int64_t block_counts_cap;
     void *d_rec;             // packed neighbour records of the walker
@@ -38,6 +39,14 @@ struct bmx_ctx {
     float last_walk_kernel_ms;
     int last_walk_launches;
 };
+
+// Layout of bmx_ctx::d_small (bmx_ingest.hip): counts and boxes of the label
+// classes from offset 0 (BMX_LABEL_CLASSES x 8 + x 24 bytes), two 256-byte
+// tables from kSmallLutOffset.
+constexpr size_t kSmallLutOffset = 8448;
+constexpr size_t kSmallBytes = 16384;
+static_assert(BMX_LABEL_CLASSES * 32 <= kSmallLutOffset && kSmallLutOffset + 512 <= kSmallBytes,
+              "d_small layout");
 
 // Error reporting ----------------------------------------------------------
 int bmx_set_error(int code, const char *fmt, ...);

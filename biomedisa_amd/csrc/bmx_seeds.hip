@@ -207,6 +207,9 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int64_t *counts, int64_t n
     if (t == 1023) *out_total = (uint64_t)part[1023];
 }
 
+// _calc_var (pyopencl_small.py:140-185) with dev += tmp * tmp contracted to
+// fma(tmp, tmp, dev), as the reference's compilers build it (contraction on
+// by default); exact either way for uint8 images.
 template <typename T>
 __device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy, const uint8_t *fx,
                           const T *img, const Geom &g, int z, int y, int x, int64_t i0)
@@ -219,7 +222,7 @@ __device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
             const int64_t i = i0 + n * g.sy + o;
             if (mask_at(lab, fz, fy, fx, z, y + n, x + o, i) == seg) {
                 const float t = B - bmx_ld(img, i);
-                dev += t * t;
+                dev = __builtin_fmaf(t, t, dev);
                 summe += 1.0f;
             }
         }
@@ -228,7 +231,7 @@ __device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
             const int64_t i = i0 + n * g.sz + o;
             if (summe < 9.0f && mask_at(lab, fz, fy, fx, z + n, y, x + o, i) == seg) {
                 const float t = B - bmx_ld(img, i);
-                dev += t * t;
+                dev = __builtin_fmaf(t, t, dev);
                 summe += 1.0f;
             }
         }
@@ -237,7 +240,7 @@ __device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
             const int64_t i = i0 + n * g.sz + o * g.sy;
             if (summe < 9.0f && mask_at(lab, fz, fy, fx, z + n, y + o, x, i) == seg) {
                 const float t = B - bmx_ld(img, i);
-                dev += t * t;
+                dev = __builtin_fmaf(t, t, dev);
                 summe += 1.0f;
             }
         }

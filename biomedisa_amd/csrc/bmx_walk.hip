@@ -238,14 +238,16 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk_ref(
                 const float W3 = bmx_weight(B, a3, div1) + W2;
                 const float W4 = bmx_weight(B, a4, div1) + W3;
                 const float W5 = bmx_weight(B, a5, div1) + W4;
-                float p1 = a12 * s11 - a13n * s10;
+                // the recurrence as the reference's compilers contract it
+                // (FP_CONTRACT on): a*b - c*d -> fma(a, b, -(c*d)), p -= k*m -> fma(-k, m, p)
+                float p1 = __builtin_fmaf(a12, s11, -(a13n * s10));
                 int64_t kk = (int64_t)(p1 / m1);
-                p1 -= (float)kk * m1;
+                p1 = __builtin_fmaf(-(float)kk, m1, p1);
                 if (p1 < 0.0f) p1 += m1;
                 s10 = s11; s11 = s12; s12 = p1;
-                float p2 = a21 * s22 - a23n * s20;
+                float p2 = __builtin_fmaf(a21, s22, -(a23n * s20));
                 kk = (int64_t)(p2 / m2);
-                p2 -= (float)kk * m2;
+                p2 = __builtin_fmaf(-(float)kk, m2, p2);
                 if (p2 < 0.0f) p2 += m2;
                 s20 = s21; s21 = s22; s22 = p2;
                 const float rand = (p1 <= p2) ? W5 * ((p1 - p2 + m1) * norm) : W5 * ((p1 - p2) * norm);

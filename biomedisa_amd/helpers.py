@@ -97,6 +97,105 @@ def pre_processing(bm):
     return bm
 
 
+def pre_processing_counts(bm, counts):
+    """pre_processing (biomedisa_helper.py:639-724) from the label-class counts
+    of bmx_label_stats instead of the label array: counts[v] for the values
+    1..255, counts[256] negative voxels, counts[257] voxels above 255.
+
+    Sets bm.allLabels and bm.nol as pre_processing does and bm.kept, the label
+    values of 1..255 that stay non-zero in the edited label volume, and
+    bm.neg_kept (negative labels stay, as ignored regions).  Returns None when
+    an `ignore` entry names a negative value present in the volume (the counts
+    cannot tell which negative voxels it removes); the caller then works on
+    the array itself.
+    """
+    counts = np.asarray(counts)
+    present = np.flatnonzero(counts[1:256]) + 1
+    has_neg, has_big = bool(counts[256]), bool(counts[257])
+    bm.allLabels = present.astype(np.int64)
+    if has_neg:
+        print("Warning: Negative labels indicate ignored regions.")
+    if has_big:
+        print("Warning: Only label values <=255 are allowed. Label values greater than 255 will be removed.")
+    bm.allLabels = np.append(0, bm.allLabels)
+    labels_to_compute = str(bm.only).split(",")
+    if not any(x in ["all", "All", "ALL"] for x in labels_to_compute):
+        labels_to_remove = [k for k in bm.allLabels if str(k) not in labels_to_compute and k > 0]
+        for k in labels_to_remove:
+            bm.allLabels = bm.allLabels[bm.allLabels != k]
+    neg_kept = has_neg
+    labels_to_remove = str(bm.ignore).split(",")
+    if not any(x in ["none", "None", "NONE"] for x in labels_to_remove):
+        for k in labels_to_remove:
+            try:
+                k = int(k)
+            except ValueError:
+                continue
+            if k < 0 and has_neg:
+                return None
+            bm.allLabels = bm.allLabels[bm.allLabels != k]
+    bm.kept = np.asarray([v for v in bm.allLabels if v > 0], dtype=np.int64)
+    bm.neg_kept = neg_kept
+    bm.nol = len(bm.allLabels)
+    if bm.nol < 2:
+        return _error_(bm, "No labeled slices found.")
+    bm.success = True
+    return bm
+
+
+def box_from_class_boxes(boxes, kept, neg_kept, shape):
+    """predict_blocksize (biomedisa_helper.py:1105-1124) of the padded label
+    volume, from the per-class boxes of bmx_label_stats (unpadded (Z,Y,X)
+    coordinates, zmin > zmax when absent): the union of the boxes of the kept
+    values and of the negative class, +1 for the pad, +-100 clamped."""
+    sel = [int(v) for v in kept] + ([256] if neg_kept else [])
+    b = np.asarray(boxes, dtype=np.int64)[sel] if sel else np.zeros((0, 6), np.int64)
+    b = b[b[:, 0] <= b[:, 1]]
+    zsh, ysh, xsh = (int(s) + 2 for s in shape)
+    if b.shape[0] == 0:
+        amin_z, amax_z, amin_y, amax_y, amin_x, amax_x = zsh, 0, ysh, 0, xsh, 0
+    else:
+        amin_z, amax_z = int(b[:, 0].min()) + 1, int(b[:, 1].max()) + 1
+        amin_y, amax_y = int(b[:, 2].min()) + 1, int(b[:, 3].max()) + 1
+        amin_x, amax_x = int(b[:, 4].min()) + 1, int(b[:, 5].max()) + 1
+
+    def lo(v):
+        return v - 100 if v - 100 > 0 else 0
+
+    def hi(v, n):
+        return v + 100 if v + 100 < n else n
+
+    return lo(amin_z), hi(amax_z, zsh), lo(amin_y), hi(amax_y, ysh), lo(amin_x), hi(amax_x, xsh)
+
+
+def class_lut(allLabels, neg_kept):
+    """Label class -> label index (uint8[258]): the index of the value in
+    allLabels (0 for values left out: background, removed or above 255),
+    BMX_LABEL_NONE for kept negative labels; as label_index_volume."""
+    lut = np.zeros(258, np.uint8)
+    vals = np.asarray(allLabels, dtype=np.int64)
+    lut[vals] = np.arange(len(vals), dtype=np.uint8)
+    lut[257] = 0
+    lut[256] = 0xFF if neg_kept else 0
+    return lut
+
+
+def dice_counted(label_dtype):
+    """Per label value v, whether a voxel whose label and result are both v is
+    a match of Dice_score: (ground_truth + result) > 0 wraps to 0 for v = 128
+    when both are uint8 (biomedisa_helper.py:164)."""
+    counted = np.ones(256, np.uint8)
+    counted[0] = 0
+    if np.dtype(label_dtype) == np.uint8:
+        counted[128] = 0
+    return counted
+
+
+def dice_from_counts(a, b, c):
+    """The global Dice of Dice_score from bmx_dice_counts (NaN when empty)."""
+    return 2.0 * a / float(b + c) if b + c else float("nan")
+
+
 def normalize_image(data):
     """Image dtype handling of interpolation.py:135-144.
 

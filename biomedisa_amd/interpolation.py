@@ -51,6 +51,16 @@ class DeviceVolume:
         self.img = eng.upload(img)
         self.lab = eng.upload(np.ascontiguousarray(labidx, dtype=np.uint8))
 
+    @classmethod
+    def from_buffers(cls, eng, img, lab, dims, img_dtype):
+        """A volume whose image (uint8 or float32) and label-index buffers are
+        already on the device (the volume takes them over)."""
+        self = cls.__new__(cls)
+        self.eng, self.img, self.lab = eng, img, lab
+        self.dims = tuple(int(s) for s in dims)
+        self.img_dtype = img_dtype
+        return self
+
     @property
     def V(self):
         Z, Y, X = self.dims
@@ -59,6 +69,25 @@ class DeviceVolume:
 
 class PipelineError(ValueError):
     """Input rejected by the pipeline (the reference's _error_ cases)."""
+
+
+# rw_small.py:147-149 and :181-183: an allocation failure while smoothing or
+# computing the uncertainty drops that result with a warning.
+SMOOTH_OOM_WARNING = "Warning: GPU out of memory to allocate smooth array. Process continues without smoothing."
+UNCERTAINTY_OOM_WARNING = ("Warning: GPU out of memory to allocate uncertainty array. Process continues without "
+                           "uncertainty.")
+
+
+def optional_stage(stage, warning):
+    """Run an optional result stage; on a device out-of-memory error print the
+    reference's warning and return None (every other error propagates)."""
+    try:
+        return stage()
+    except _hip.BmxError as e:
+        if e.code != _hip.BMX_E_OOM:
+            raise
+        print(warning)
+        return None
 
 
 def select_path(dims, nlabels):
@@ -127,13 +156,21 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
     eng.argmax(hits, nlabels, V, values, regular)
     out["regular"] = regular
     if smooth:
-        sm = eng.alloc(V)
-        eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm, clamp_first=large)
-        out["smooth"] = sm
+        def smooth_stage():
+            sm = eng.alloc(V)
+            eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm, clamp_first=large)
+            return sm
+        sm = optional_stage(smooth_stage, SMOOTH_OOM_WARNING)
+        if sm is not None:
+            out["smooth"] = sm
     if uncertainty:
-        uq = eng.alloc(V)
-        eng.uncertainty(hits, nlabels, vol.dims, uq)
-        out["uncertainty"] = uq
+        def uncertainty_stage():
+            uq = eng.alloc(V)
+            eng.uncertainty(hits, nlabels, vol.dims, uq)
+            return uq
+        uq = optional_stage(uncertainty_stage, UNCERTAINTY_OOM_WARNING)
+        if uq is not None:
+            out["uncertainty"] = uq
     eng.sync()
     t3 = time.perf_counter()
     if keep_hits:
@@ -299,11 +336,15 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
               regular.ptr)
     out["regular"] = regular
     if smooth:
-        sm_all, sm = eng.alloc(V), eng.alloc(V)
-        eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm_all)
-        _hip.call("bmx_select_labels", eng.ctx, sm_all.ptr, pred.ptr, U.ptr, int(nlabels), values.ctypes.data,
-                  V, sm.ptr)
-        out["smooth"] = sm
+        def smooth_stage():
+            sm_all, sm = eng.alloc(V), eng.alloc(V)
+            eng.smooth_argmax(hits, nlabels, vol.dims, smooth, values, sm_all)
+            _hip.call("bmx_select_labels", eng.ctx, sm_all.ptr, pred.ptr, U.ptr, int(nlabels), values.ctypes.data,
+                      V, sm.ptr)
+            return sm
+        sm = optional_stage(smooth_stage, SMOOTH_OOM_WARNING)
+        if sm is not None:
+            out["smooth"] = sm
     eng.sync()
     t4 = time.perf_counter()
     if keep:
@@ -319,7 +360,7 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
                         ignore='none', only='all', smooth=0, compression=True, return_hits=False,
                         img_id=None, label_id=None, remote=False, queue=0, clean=None, fill=None,
                         slicer=False, rng="philox", key=DEFAULT_KEY, device=0, method="plain",
-                        guide=None, path="auto"):
+                        guide=None, path="auto", timings=None):
     """Smart interpolation of sparsely labelled slices (biomedisa/interpolation.py:43).
 
     Returns the reference's result dict ({'regular': uint8 (Z,Y,X)} plus
@@ -333,6 +374,9 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     path: 'auto' (default) takes the reference's small- or large-volume driver
     by its size rule (interpolation.py:192-196), 'small' or 'large' forces one;
     see run_pipeline.  The guided method always walks as the small driver.
+
+    timings: a dict that receives the phase times (prepare_s, pipeline_s,
+    results_s), the pipeline's own timings and counts (seeds, accepted moves).
     """
     files = None
     if data is None or labelData is None:
@@ -366,35 +410,22 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
     if method == "guided" and (rng != "philox" or uncertainty or return_hits):
         raise ValueError("guided mode runs the Philox walker and returns 'regular'/'smooth' only")
 
-    prep = _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
-                    return_hits, slicer, denoise, device)
-    if prep is None:
-        return None
-    bm = prep.bm
     eng = _hip.engine(device)
-    vol = DeviceVolume(eng, prep.img, prep.labidx)
-    try:
-        if method == "guided":
-            from . import surrogate
-            dev = run_pipeline_guided(vol, bm.nol, prep.values, surrogate.net(device), bm.nbrw, bm.sorw,
-                                      bm.allaxis, bm.smooth, key, **(guide or {}))
-        else:
-            if path == "auto":
-                path = select_path(vol.dims, bm.nol)
-            dev = run_pipeline(vol, bm.nol, prep.values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
-                               bm.uncertainty, rng, key, keep_hits=bm.return_hits, path=path)
-    except PipelineError as e:
-        _error_(bm, str(e))
+    got = _interpolate(eng, data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
+                       return_hits, slicer, denoise, rng, key, method, guide, path, device, timings)
+    if got is None:
         return None
-    results = _results(prep, dev)
+    prep, results = got
+    bm = prep.bm
     post_process(results, clean, fill, device)
     if acwe:
         # rw_small.py:244-253: the (normalised) image of the walk's box, zeros
         # elsewhere, and the regular result
         from .post import active_contour, refinement
         z0, z1, y0, y1, x0, x1 = prep.box
-        data = np.zeros((bm.zsh, bm.ysh, bm.xsh), dtype=prep.img.dtype)
-        data[z0:z1, y0:y1, x0:x1] = prep.img
+        img_c = prep.host_image()
+        data = np.zeros((bm.zsh, bm.ysh, bm.xsh), dtype=img_c.dtype)
+        data[z0:z1, y0:y1, x0:x1] = img_c
         data = data[1:-1, 1:-1, 1:-1]
         results["acwe"] = active_contour(data, results["regular"], acwe_alpha, acwe_smooth, acwe_steps, device)
         results["refined"] = refinement(data, results["regular"], device)
@@ -409,6 +440,88 @@ def smart_interpolation(data, labelData, nbrw=10, sorw=4000, acwe=False, acwe_al
                     bio.save_data(path, results[key], compression, header=label_header,
                                   file_type=final_image_type)
     return results
+
+
+class DeviceArray:
+    """A caller's array resident in device memory (shape and dtype as the
+    NumPy array it stands for, C order), as smart_interpolation_device takes
+    its inputs and returns its results."""
+
+    def __init__(self, buf, shape, dtype):
+        self.buf, self.shape, self.dtype = buf, tuple(int(s) for s in shape), np.dtype(dtype)
+
+    @classmethod
+    def upload(cls, eng, arr):
+        arr = np.ascontiguousarray(arr)
+        return cls(eng.upload(arr), arr.shape, arr.dtype)
+
+    def download(self):
+        return self.buf.download(self.shape, self.dtype)
+
+    def free(self):
+        self.buf.free()
+
+
+def smart_interpolation_device(data, labelData, nbrw=10, sorw=4000, allaxis=False, ignore='none', only='all',
+                               smooth=0, uncertainty=False, denoise=False, rng="philox", key=DEFAULT_KEY, device=0,
+                               method="plain", guide=None, path="auto", timings=None):
+    """smart_interpolation on arrays already in device memory (DeviceArray image
+    and labels in the dtypes the drop-in takes: 8/16/32-bit integers or
+    float32), with the results left there: {'regular': DeviceArray (Z,Y,X)
+    uint8, optional 'smooth', 'uncertainty'}, or None when the input is
+    rejected.  Every step after the inputs are resident runs on the device:
+    label set and box, normalisation, pad and crop, the walk and its merges,
+    result placement and the sanity check."""
+    if method == "guided" and (rng != "philox" or uncertainty):
+        raise ValueError("guided mode runs the Philox walker and returns 'regular'/'smooth' only")
+    eng = _hip.engine(device)
+    got = _interpolate(eng, data, labelData, nbrw, sorw, uncertainty, None, allaxis, ignore, only, smooth, False,
+                       False, denoise, rng, key, method, guide, path, device, timings, to_host=False)
+    if got is NotImplemented:
+        raise TypeError("smart_interpolation_device takes 8/16/32-bit integer or float32 arrays below 2^32 voxels")
+    return None if got is None else got[1]
+
+
+def _interpolate(eng, data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
+                 return_hits, slicer, denoise, rng, key, method, guide, path, device, timings, to_host=True):
+    """Pre-processing, pipeline and results of one call -> (prep, results), or
+    None when the input is rejected (after printing the error)."""
+    t = timings if timings is not None else {}
+    t0 = time.perf_counter()
+    prep = _prepare_device(eng, data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only,
+                           smooth, return_hits, slicer, denoise)
+    if prep is NotImplemented:
+        if not to_host:
+            return NotImplemented
+        # label or image types the device pre-processing does not take: on the host
+        prep = _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
+                        return_hits, slicer, denoise, device)
+        if prep is not None:
+            prep.vol = DeviceVolume(eng, prep.img, prep.labidx)
+    if prep is None:
+        return None
+    bm = prep.bm
+    vol = prep.vol
+    t1 = time.perf_counter()
+    try:
+        if method == "guided":
+            from . import surrogate
+            dev = run_pipeline_guided(vol, bm.nol, prep.values, surrogate.net(device), bm.nbrw, bm.sorw,
+                                      bm.allaxis, bm.smooth, key, timings=t, **(guide or {}))
+        else:
+            if path == "auto":
+                path = select_path(vol.dims, bm.nol)
+            dev = run_pipeline(vol, bm.nol, prep.values, bm.nbrw, bm.sorw, bm.allaxis, bm.smooth,
+                               bm.uncertainty, rng, key, keep_hits=bm.return_hits, path=path, timings=t)
+    except PipelineError as e:
+        _error_(bm, str(e))
+        return None
+    t2 = time.perf_counter()
+    results = _results(prep, dev) if to_host else _results_device(prep, dev, to_host=False)
+    t3 = time.perf_counter()
+    t.update(prepare_s=t1 - t0, pipeline_s=t2 - t1, results_s=t3 - t2,
+             stats={k: v for k, v in dev.items() if isinstance(v, (int, bool, str))})
+    return prep, results
 
 
 def post_process(results, clean=None, fill=None, device=0):
@@ -447,6 +560,134 @@ class _Prepared:
 
     def __init__(self, bm, img, lab, labidx, values, box):
         self.bm, self.img, self.lab_c, self.labidx, self.values, self.box = bm, img, lab, labidx, values, box
+        self.vol = None
+
+    def host_image(self):
+        return self.img
+
+
+class _DevicePrepared:
+    """A validated input prepared on the device: the walk's volume (cropped,
+    padded image and label-index volume), the label values and the box."""
+
+    def __init__(self, bm, vol, values, box, shape, label_dtype):
+        self.bm, self.vol, self.values, self.box = bm, vol, values, box
+        self.shape, self.label_dtype = shape, label_dtype
+
+    def host_image(self):
+        return self.vol.img.download(self.vol.dims, np.uint8 if self.vol.img_dtype == _hip.IMG_U8 else np.float32)
+
+
+def _prepare_device(eng, data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore, only, smooth,
+                    return_hits, slicer, denoise=False):
+    """Checks and pre-processing of smart_interpolation (interpolation.py:77-204)
+    with the arrays uploaded once and every step after the checks on the device:
+    label set and label box (bmx_label_stats), normalisation, denoise, pad and
+    crop (bmx_crop_image, bmx_crop_labels).  Returns a _DevicePrepared, None
+    when the input is rejected (after printing the error), or NotImplemented
+    for inputs the device steps do not take (8-byte or boolean labels, images
+    of types other than 8/16/32-bit integers and float32 are normalised on the
+    host first; volumes of 2^32 voxels and more), which the caller prepares on
+    the host instead."""
+    from .helpers import box_from_class_boxes, class_lut, pre_processing_counts
+    bm = Biomedisa()
+    bm.TIC = time.time()
+    bm.success = True
+    bm.data, bm.labelData = data, labelData
+    bm.nbrw, bm.sorw, bm.allaxis, bm.smooth = int(nbrw), int(sorw), bool(allaxis), int(smooth)
+    bm.uncertainty, bm.return_hits = bool(uncertainty), bool(return_hits)
+    bm.ignore, bm.only, bm.platform, bm.slicer = ignore, only, platform, slicer
+    # the checks of pre_processing (biomedisa_helper.py:639-661) that need no pass over the data
+    if data is None:
+        _error_(bm, "Invalid image data.")
+        return None
+    if labelData is None:
+        _error_(bm, "Invalid label data.")
+        return None
+    on_device = isinstance(data, DeviceArray), isinstance(labelData, DeviceArray)
+    if not on_device[0]:
+        data = np.asarray(data)
+    if not on_device[1]:
+        labelData = np.asarray(labelData)
+    if len(labelData.shape) != 3:
+        _error_(bm, "Label data must be three-dimensional.")
+        return None
+    if data.shape != labelData.shape:
+        _error_(bm, "Image and label data must have the same x,y,z-dimensions.")
+        return None
+    Z, Y, X = (int(s) for s in labelData.shape)
+    if Z * Y * X >= 2 ** 32 or min(Z, Y, X) < 1:
+        return NotImplemented
+    if labelData.dtype not in _hip.RAW_DTYPES or labelData.dtype == np.float32:
+        if labelData.dtype.kind != "f" or on_device[1]:
+            return NotImplemented
+        print(f"Warning: Potential label loss during conversion from {labelData.dtype} to int32.")
+        labelData = labelData.astype(np.int32)
+    if on_device[0] and data.dtype not in _hip.RAW_DTYPES:
+        return NotImplemented
+    lab_raw = _hip.RAW_DTYPES[labelData.dtype]
+    lab_dev = labelData.buf if on_device[1] else eng.upload(labelData)
+    try:
+        counts, boxes = eng.label_stats(lab_dev, lab_raw, (Z, Y, X))
+        if pre_processing_counts(bm, counts) is None:
+            return NotImplemented
+        if not bm.success:
+            return None
+        if labelData.dtype == np.uint32:
+            # interpolation.py:145-147 (the values above 255 are already 0)
+            print(f"Warning: Potential label loss during conversion from {labelData.dtype} to int32.")
+        if bm.nol > 255:
+            _error_(bm, "At most 255 labels (including background) are supported.")
+            return None
+        box = box_from_class_boxes(boxes, bm.kept, bm.neg_kept, (Z, Y, X))
+        dims_c = (box[1] - box[0], box[3] - box[2], box[5] - box[4])
+        Vc = dims_c[0] * dims_c[1] * dims_c[2]
+        if Vc >= 2 ** 32:
+            return NotImplemented
+        bm.zsh, bm.ysh, bm.xsh = Z + 2, Y + 2, X + 2
+        labidx = eng.alloc(Vc)
+        eng.crop_labels(lab_dev, lab_raw, (Z, Y, X), box, class_lut(bm.allLabels, bm.neg_kept), labidx)
+    finally:
+        if not on_device[1]:
+            lab_dev.free()
+
+    # the image: data types (interpolation.py:135-144), denoise (:149-151), pad and crop
+    owned = True
+    if on_device[0]:
+        raw, src, owned = _hip.RAW_DTYPES[data.dtype], data.buf, False
+    elif data.dtype in _hip.RAW_DTYPES:
+        raw = _hip.RAW_DTYPES[data.dtype]
+        src = eng.upload(data)
+    else:
+        raw, src = _hip.RAW_F32, eng.upload(normalize_image(data))
+    normalize, vmin, vrange = raw not in (_hip.RAW_U8, _hip.RAW_I8), 0.0, 1.0
+    if normalize and data.dtype in _hip.RAW_DTYPES:
+        lo, hi = eng.image_range(src, raw, Z * Y * X)
+        vmin = np.float32(lo)
+        vrange = np.float32(np.float32(hi) - vmin)
+    elif raw == _hip.RAW_F32:
+        normalize = False   # normalised on the host above
+    out_u8 = raw in (_hip.RAW_U8, _hip.RAW_I8)
+    img_dtype = _hip.IMG_U8 if out_u8 else _hip.IMG_F32
+    esize = 1 if out_u8 else 4
+    if denoise:
+        # the whole normalised image first, then smooth_img_3x3 on it
+        full = eng.alloc(Z * Y * X * esize)
+        eng.crop_image(src, raw, normalize, (Z, Y, X), (1, Z + 1, 1, Y + 1, 1, X + 1), vmin, vrange, full)
+        if owned:
+            src.free()
+        src, owned = eng.alloc(Z * Y * X * esize), True
+        eng.denoise(full, img_dtype, (Z, Y, X), src)
+        full.free()
+        raw, normalize = (_hip.RAW_U8 if out_u8 else _hip.RAW_F32), False
+    img_c = eng.alloc(Vc * esize)
+    eng.crop_image(src, raw, normalize, (Z, Y, X), box, vmin, vrange, img_c)
+    eng.sync()
+    if owned:
+        src.free()
+    vol = DeviceVolume.from_buffers(eng, img_c, labidx, dims_c, img_dtype)
+    values = np.asarray(bm.allLabels, dtype=np.uint8)
+    return _DevicePrepared(bm, vol, values, box, (Z, Y, X), labelData.dtype)
 
 
 def denoise_image(img, device=0):
@@ -510,6 +751,8 @@ def _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore
 def _results(prep, dev):
     """The result dictionary of smart_interpolation from the pipeline's device
     buffers (rw_small.py:195-281)."""
+    if isinstance(prep, _DevicePrepared):
+        return _results_device(prep, dev)
     bm = prep.bm
     z0, z1, y0, y1, x0, x1 = prep.box
     shape_c = prep.img.shape
@@ -534,9 +777,59 @@ def _results(prep, dev):
 
     # validate result (rw_small.py:203-211)
     mask = prep.lab_c > 0
-    dice = Dice_score(prep.lab_c, final_crop * mask)
-    if dice < 0.3:
-        print('Warning: Bad result! Use "--allaxis" if you labeled axes other than the xy-plane.')
+    bad_result_warning(Dice_score(prep.lab_c, final_crop * mask))
 
     results["regular"] = final_result
+    return results
+
+
+def bad_result_warning(dice):
+    """The sanity check of rw_small.py:203-211: a global Dice below 0.3 between
+    the labelled voxels and the result prints the reference's warning."""
+    if dice < 0.3:
+        print('Warning: Bad result! Use "--allaxis" if you labeled axes other than the xy-plane.')
+        return True
+    return False
+
+
+def _results_device(prep, dev, to_host=True):
+    """The result dictionary from the pipeline's device buffers, placed into
+    whole (Z,Y,X) volumes on the device (bmx_place) and copied out once each
+    (to_host=False: left on the device as DeviceArrays); the sanity check's
+    Dice from device counts (bmx_dice_counts)."""
+    from .helpers import dice_counted, dice_from_counts
+    bm = prep.bm
+    eng = prep.vol.eng
+    z0, z1, y0, y1, x0, x1 = prep.box
+    shape_c = prep.vol.dims
+    shape = prep.shape
+    results = {}
+    full = eng.alloc(int(np.prod(shape))) if to_host else None
+
+    def place(buf):
+        if to_host:
+            eng.place(buf, shape, prep.box, full)
+            return full.download(shape, np.uint8)
+        out = eng.alloc(int(np.prod(shape)))
+        eng.place(buf, shape, prep.box, out)
+        return DeviceArray(out, shape, np.uint8)
+
+    for key in ("regular", "smooth", "uncertainty"):
+        if key in dev:
+            results[key] = place(dev[key])
+    if full is not None:
+        full.free()
+    if bm.return_hits:
+        h = dev["hits"].download((bm.nol,) + shape_c, np.uint32)
+        hits = np.zeros((bm.nol, bm.zsh, bm.ysh, bm.xsh), dtype=np.float32)
+        hits[:, z0:z1, y0:y1, x0:x1] = h
+        results["hits"] = hits[:, 1:-1, 1:-1, 1:-1]
+    a, b, c = eng.dice_counts(prep.vol.lab, dev["regular"], prep.vol.V, bm.nol, prep.values,
+                              dice_counted(prep.label_dtype))
+    bad_result_warning(dice_from_counts(a, b, c))
+    if not to_host:
+        eng.sync()
+    for key in ("regular", "smooth", "uncertainty", "hits"):
+        if key in dev:
+            dev[key].free()
     return results

@@ -79,6 +79,74 @@ int bmx_zero(bmx_ctx *ctx, void *dst_dev, size_t bytes);
 /* Device-to-device copy on the context stream. */
 int bmx_copy(bmx_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 
+/* ------------------------------------------------------------------------ */
+/* Pre- and post-processing of one call on the device.  The caller's arrays  */
+/* are uploaded as they are (raw dtypes below); the cropped, padded image    */
+/* and label-index volume of the walk are written from them.                 */
+/* ------------------------------------------------------------------------ */
+#define BMX_RAW_U8 0
+#define BMX_RAW_I8 1
+#define BMX_RAW_U16 2
+#define BMX_RAW_I16 3
+#define BMX_RAW_U32 4
+#define BMX_RAW_I32 5
+#define BMX_RAW_F32 6
+
+/* Label classes: the values 0..255, then "negative", then "above 255". */
+#define BMX_LABEL_CLASSES 258
+
+/* Label set and label boxes of a raw label volume (integer dtype, device,
+ * Z*Y*X < 2^32): counts[c] = voxels of class c (counts[0] is left 0: zero
+ * voxels are not counted) and boxes[6c..6c+5] = zmin, zmax, ymin, ymax, xmin,
+ * xmax of its voxels (zmin > zmax when absent), host arrays of
+ * BMX_LABEL_CLASSES and 6*BMX_LABEL_CLASSES entries.  Replaces the unique()
+ * of pre_processing (biomedisa/features/biomedisa_helper.py:82-103, 668-717)
+ * and the non-zero scan of predict_blocksize (:1105-1124), whose box is the
+ * union of the boxes of the classes that stay non-zero. */
+int bmx_label_stats(bmx_ctx *ctx, const void *lab, int lab_dtype, int64_t Z, int64_t Y, int64_t X,
+                    uint64_t *counts, int32_t *boxes);
+
+/* Smallest and largest value of a raw image (device, n < 2^32 elements), for
+ * the float32 normalisation of interpolation.py:140-144. */
+int bmx_image_range(bmx_ctx *ctx, const void *img, int img_raw_dtype, int64_t n, double *vmin, double *vmax);
+
+/* The walk's image: the 1-voxel zero pad of interpolation.py:157-158 and the
+ * crop to the label box of :203, in one pass from the raw (Z,Y,X) image.
+ * box = [z0, z1, y0, y1, x0, x1) in padded coordinates; out has
+ * (z1-z0)(y1-y0)(x1-x0) voxels.  normalize = 0: uint8 copied, int8 + 128 as
+ * uint8 (:136-139), float32 copied; normalize = 1: float32
+ * ((float)v - vmin) / vrange * 255 in that order of IEEE operations, with
+ * vmin = amin and vrange = amax of (image - vmin) as float32 (:140-144); pad
+ * voxels are 0 either way. */
+int bmx_crop_image(bmx_ctx *ctx, const void *img, int img_raw_dtype, int normalize, int64_t Z, int64_t Y,
+                   int64_t X, const int64_t *box, float vmin, float vrange, void *out);
+
+/* The walk's label-index volume: pad and crop as bmx_crop_image, every voxel
+ * mapped through lut[BMX_LABEL_CLASSES] (host; class as for bmx_label_stats,
+ * pad voxels as value 0).  Replaces the label edits of pre_processing
+ * (values > 255, `only` and `ignore` set to 0: biomedisa_helper.py:682-717)
+ * and the label volume rebuild of pyopencl_small.py:89-107 (label index per
+ * voxel, negative = ignored = BMX_LABEL_NONE). */
+int bmx_crop_labels(bmx_ctx *ctx, const void *lab, int lab_dtype, int64_t Z, int64_t Y, int64_t X,
+                    const int64_t *box, const uint8_t *lut, uint8_t *out);
+
+/* A result of the cropped walk volume placed into the caller's (Z,Y,X)
+ * volume (device): out = crop where the box covers the padded voxel, else 0.
+ * Replaces the result placement of rw_small.py:212-216 (zeros, box slice,
+ * [1:-1] on every axis). */
+int bmx_place(bmx_ctx *ctx, const uint8_t *crop, int64_t Z, int64_t Y, int64_t X, const int64_t *box,
+              uint8_t *out);
+
+/* Counts of the sanity check of rw_small.py:203-211, Dice_score(labels,
+ * result * (labels > 0)) (biomedisa_helper.py:155-165), over n voxels of the
+ * label-index volume and the regular result (label values): counts[0] =
+ * labelled voxels (index 1..nlabels-1) whose result equals their value v and
+ * counted[v] != 0 (the caller clears counted[v] where (labels + result) wraps
+ * to 0 in the label dtype), counts[1] = labelled voxels, counts[2] = labelled
+ * voxels with a non-zero result; Dice = 2 c0 / (c1 + c2). */
+int bmx_dice_counts(bmx_ctx *ctx, const uint8_t *labidx, const uint8_t *result, int64_t n, int nlabels,
+                    const uint8_t *values, const uint8_t *counted, uint64_t *counts);
+
 /* Labelled planes: replaces read_labeled_slices / read_labeled_slices_allx /
  * read_indices_allx (biomedisa_helper.py:1015-1066).  flags (device, Z+Y+X
  * bytes) receives 1 for every labelled z-plane, then y-plane, then x-plane. */

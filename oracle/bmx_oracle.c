@@ -181,7 +181,7 @@ static float orc_calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
             int64_t i = i0 + n * sy + o;
             if (orc_mask(lab, fz, fy, fx, z, y + n, x + o, i) == seg) {
                 float t = B - orc_img(img, dtype, i);
-                dev += t * t; summe += 1.0f;
+                dev = fmaf(t, t, dev); summe += 1.0f;
             }
         }
     for (int n = -1; n <= 1; n += 2)        /* "XZ": (z+n, y, x+o) */
@@ -189,7 +189,7 @@ static float orc_calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
             int64_t i = i0 + n * sz + o;
             if (summe < 9.0f && orc_mask(lab, fz, fy, fx, z + n, y, x + o, i) == seg) {
                 float t = B - orc_img(img, dtype, i);
-                dev += t * t; summe += 1.0f;
+                dev = fmaf(t, t, dev); summe += 1.0f;
             }
         }
     for (int n = -1; n <= 1; n += 2)        /* "YZ": (z+n, y+o, x) */
@@ -197,7 +197,7 @@ static float orc_calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
             int64_t i = i0 + n * sz + o * sy;
             if (summe < 9.0f && orc_mask(lab, fz, fy, fx, z + n, y + o, x, i) == seg) {
                 float t = B - orc_img(img, dtype, i);
-                dev += t * t; summe += 1.0f;
+                dev = fmaf(t, t, dev); summe += 1.0f;
             }
         }
     float var = dev / summe;
@@ -325,6 +325,41 @@ int64_t orc_philox_walk(const void *img, int dtype, int Z, int Y, int X,
 /* The direction registers start at (0,0,0) (uninitialised in the kernel)    */
 /* and keep their value when no branch of the chain matches.                 */
 /* ------------------------------------------------------------------------ */
+/* One step of the reference's float MRG32k3a-like recurrence
+ * (pyopencl_small.py:213-223, :256-282) as its GPU compilers build it: both
+ * backends compile with floating-point contraction on (OpenCL C's FP_CONTRACT
+ * defaults to ON, nvcc's --fmad to true), so a*b - c*d becomes
+ * fma(a, b, -(c*d)) and p -= k*m becomes fma(-k, m, p), the forms clang's
+ * contraction gives (the left product fused).  Written without contraction
+ * the recurrence loses most of its state bits: on a uniform image its walks
+ * drift by tens of voxels per 4000 steps.  s = {s10, s11, s12, s20, s21, s22};
+ * returns the uniform (p1 - p2 [+ m1]) * norm, rand = W5 * that. */
+static float orc_mrg_next(float s[6])
+{
+    const float norm = 2.328306549295728e-10f;
+    const float m1 = 4294967087.0f, m2 = 4294944443.0f;
+    const float a12 = 1403580.0f, a13n = 810728.0f, a21 = 527612.0f, a23n = 1370589.0f;
+    float p1 = fmaf(a12, s[1], -(a13n * s[0]));
+    int64_t k1 = (int64_t)(p1 / m1);
+    p1 = fmaf(-(float)k1, m1, p1);
+    if (p1 < 0.0f) p1 += m1;
+    s[0] = s[1]; s[1] = s[2]; s[2] = p1;
+    float p2 = fmaf(a21, s[5], -(a23n * s[3]));
+    k1 = (int64_t)(p2 / m2);
+    p2 = fmaf(-(float)k1, m2, p2);
+    if (p2 < 0.0f) p2 += m2;
+    s[3] = s[4]; s[4] = s[5]; s[5] = p2;
+    return (p1 <= p2) ? ((p1 - p2 + m1) * norm) : ((p1 - p2) * norm);
+}
+
+/* The first n uniforms of the stream seeded with voxel index `index`. */
+void orc_mrg_stream(int64_t index, int n, float *u)
+{
+    const float fi = (float)(int32_t)index;
+    float s[6] = {fi, fi, fi, fi, fi, fi};
+    for (int t = 0; t < n; ++t) u[t] = orc_mrg_next(s);
+}
+
 int64_t orc_ref_walk(const void *img, int dtype, int Z, int Y, int X,
                      const int64_t *pos, const uint8_t *label, const float *var,
                      int64_t nseeds, int nbrw, int sorw, uint32_t *hits)
@@ -336,11 +371,8 @@ int64_t orc_ref_walk(const void *img, int dtype, int Z, int Y, int X,
         const int64_t p0 = pos[s];
         const int z0 = (int)(p0 / sz), y0 = (int)((p0 / sy) % Y), x0 = (int)(p0 % X);
         uint32_t *h = hits + (int64_t)label[s] * V;
-        const float norm = 2.328306549295728e-10f;
-        const float m1 = 4294967087.0f, m2 = 4294944443.0f;
-        const float a12 = 1403580.0f, a13n = 810728.0f, a21 = 527612.0f, a23n = 1370589.0f;
         const float fi = (float)(int32_t)p0;
-        float s10 = fi, s11 = fi, s12 = fi, s20 = fi, s21 = fi, s22 = fi;
+        float st[6] = {fi, fi, fi, fi, fi, fi};
         const float B = orc_img(img, dtype, p0);
         const float div1 = 1.0f / (2.0f * var[s]);
         int k = z0, l = y0, m = x0;
@@ -350,19 +382,7 @@ int64_t orc_ref_walk(const void *img, int dtype, int Z, int Y, int X,
         while (n_rw < nbrw) {
             float W[6];
             orc_weights(img, dtype, i, sy, sz, B, div1, W);
-            float p1 = a12 * s11 - a13n * s10;
-            int64_t k1 = (int64_t)(p1 / m1);
-            p1 -= (float)k1 * m1;
-            if (p1 < 0.0f) p1 += m1;
-            s10 = s11; s11 = s12; s12 = p1;
-            float p2 = a21 * s22 - a23n * s20;
-            k1 = (int64_t)(p2 / m2);
-            p2 -= (float)k1 * m2;
-            if (p2 < 0.0f) p2 += m2;
-            s20 = s21; s21 = s22; s22 = p2;
-            float rand;
-            if (p1 <= p2) rand = W[5] * ((p1 - p2 + m1) * norm);
-            else          rand = W[5] * ((p1 - p2) * norm);
+            const float rand = W[5] * orc_mrg_next(st);
             int d = orc_direction(rand, W);
             if (d >= 0) { n = ORC_DZ[d]; o = ORC_DY[d]; p = ORC_DX[d]; }
             if (k + n > 0 && k + n < Z - 1 && l + o > 0 && l + o < Y - 1 && m + p > 0 && m + p < X - 1) {

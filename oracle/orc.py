@@ -42,6 +42,7 @@ def lib():
         L.orc_philox_walk.argtypes = [_p, C.c_int, C.c_int, C.c_int, C.c_int, _p, _p, _p, _i64,
                                       C.c_int, C.c_int, C.c_uint64, _p]
         L.orc_ref_walk.restype = _i64
+        L.orc_mrg_stream.argtypes = [_i64, C.c_int, _p]
         L.orc_ref_walk.argtypes = [_p, C.c_int, C.c_int, C.c_int, C.c_int, _p, _p, _p, _i64,
                                    C.c_int, C.c_int, _p]
         L.orc_argmax_u32.argtypes = [_p, C.c_int, _i64, _p, _p]
@@ -187,6 +188,13 @@ def philox_walk(img, pos, lab, var, nlabels, nbrw, sorw, key):
                                 _ptr(var), pos.size, int(nbrw), int(sorw),
                                 C.c_uint64(int(key) & 0xFFFFFFFFFFFFFFFF), _ptr(hits))
     return hits, acc
+
+
+def mrg_stream(index, n):
+    """The first n uniforms of the reference stream seeded with voxel `index`."""
+    u = np.empty(int(n), np.float32)
+    lib().orc_mrg_stream(C.c_int64(int(index)), int(n), _ptr(u))
+    return u
 
 
 def ref_walk(img, pos, lab, var, nlabels, nbrw, sorw):

@@ -312,6 +312,22 @@ def test_config1_shape_dice_vs_reference_stream():
     assert min(d.values()) >= 0.98, d
 
 
+def test_config1_dice_vs_reference_stream_real_parameters():
+    """The CPU-reference workload at its own parameters (128^3, 3 ellipsoids,
+    labels every 8th XY plane, nbrw=10, sorw=4000, about 1e10 steps): the
+    Philox walker against the reference's float MRG stream (as its compilers
+    contract it), per-label Dice >= 0.98."""
+    from biomedisa_amd import smart_interpolation
+    from biomedisa_amd.helpers import per_label_dice
+    vol = ellipsoids_volume(seed=1)
+    _say("config1, nbrw=10 sorw=4000, philox vs reference stream")
+    a = smart_interpolation(vol.img, vol.labels, nbrw=10, sorw=4000)["regular"]
+    b = smart_interpolation(vol.img, vol.labels, nbrw=10, sorw=4000, rng="reference")["regular"]
+    d = per_label_dice(a, b)
+    print("per-label Dice philox vs reference stream:", d, flush=True)
+    assert min(d.values()) >= 0.98, d
+
+
 def test_hit_weight_scales_hits(eng):
     """hit_weight k adds k per accepted move: walk(nbrw=1, k) == k * walk(nbrw=1, 1),
     and only the tiled Philox walker accepts it."""

@@ -132,7 +132,10 @@ def test_plane_flags_thin_line_rule_hand_cases():
 # --- seeds and variance ------------------------------------------------------------
 def _bruteforce_seeds(labidx, img, flags):
     """Rebuild the reference's slices volume (pyopencl_small.py:97-107) and scan it
-    with _calc_var (:140-185) written out in float32 scalars."""
+    with _calc_var (:140-185) written out in float32 scalars, dev += tmp * tmp
+    contracted to one fused multiply-add (exact rationals, one rounding) as
+    the reference's compilers build it."""
+    from tests.exact_f32 import fma
     Z, Y, X = labidx.shape
     fz, fy, fx = flags
     sl = np.full(labidx.shape, 0xFF, np.int32)
@@ -157,15 +160,15 @@ def _bruteforce_seeds(labidx, img, flags):
                 for n in (-1, 0, 1):
                     for o in (-1, 0, 1):
                         if sl[z, y + n, x + o] == seg:
-                            t = f(B - raw[z, y + n, x + o]); dev = f(dev + f(t * t)); summe = f(summe + 1)
+                            t = f(B - raw[z, y + n, x + o]); dev = fma(t, t, dev); summe = f(summe + 1)
                 for n in (-1, 1):
                     for o in (-1, 0, 1):
                         if summe < 9 and sl[z + n, y, x + o] == seg:
-                            t = f(B - raw[z + n, y, x + o]); dev = f(dev + f(t * t)); summe = f(summe + 1)
+                            t = f(B - raw[z + n, y, x + o]); dev = fma(t, t, dev); summe = f(summe + 1)
                 for n in (-1, 1):
                     for o in (-1, 1):
                         if summe < 9 and sl[z + n, y + o, x] == seg:
-                            t = f(B - raw[z + n, y + o, x]); dev = f(dev + f(t * t)); summe = f(summe + 1)
+                            t = f(B - raw[z + n, y + o, x]); dev = fma(t, t, dev); summe = f(summe + 1)
                 var = f(dev / summe)
                 if var < 1:
                     var = f(1)
@@ -261,10 +264,9 @@ def test_uniform_image_isotropic(ref):
     assert acc == N
     nbrs = [(3, 2, 2), (1, 2, 2), (2, 3, 2), (2, 1, 2), (2, 2, 3), (2, 2, 1)]
     frac = np.array([hits[0][k] for k in nbrs]) / N
-    # Philox: within 7 sigma of 1/6.  The reference's float-valued MRG recurrence
-    # is visibly biased (about +-0.008 on 3e5 draws), hence the looser bound.
-    tol = 0.02 if ref else 0.0048
-    assert np.all(np.abs(frac - 1 / 6) < tol), frac
+    # within 7 sigma of 1/6, for the Philox stream and for the reference stream
+    # as its compilers contract it
+    assert np.all(np.abs(frac - 1 / 6) < 0.0048), frac
 
 
 @pytest.mark.parametrize("ref", [False, True])
@@ -340,6 +342,86 @@ def test_philox_stream_layout():
                 z, y, x = z + dz, y + dy, x + dx
                 expect[z, y, x] += 1
     np.testing.assert_array_equal(hits[0], expect)
+
+
+# --- the reference stream -------------------------------------------------------------
+F32 = np.float32
+MRG = dict(norm=F32(2.328306549295728e-10), m1=F32(4294967087.0), m2=F32(4294944443.0), a12=F32(1403580.0),
+           a13n=F32(810728.0), a21=F32(527612.0), a23n=F32(1370589.0))
+
+
+def _mrg_model(index, n, contract=True):
+    """pyopencl_small.py:213-223, 256-282 in exact rationals rounded once per
+    operation; contract=True fuses a*b - c*d into fma(a, b, -(c*d)) and p -= k*m
+    into fma(-k, m, p), as FP_CONTRACT (on by default) builds it."""
+    from tests.exact_f32 import add, div, fma, mul, sub
+    c = MRG
+    s1 = [F32(index)] * 3
+    s2 = [F32(index)] * 3
+    out = []
+    for _ in range(n):
+        comps = []
+        for st, a, b, m in ((s1, c["a12"], c["a13n"], c["m1"]), (s2, c["a21"], c["a23n"], c["m2"])):
+            lead = st[1] if a is c["a12"] else st[2]    # a12 * s11, a21 * s22
+            p = fma(a, lead, -mul(b, st[0])) if contract else sub(mul(a, lead), mul(b, st[0]))
+            k = int(div(p, m))                          # float quotient truncated to long
+            p = fma(-F32(k), m, p) if contract else sub(p, mul(F32(k), m))
+            if p < 0:
+                p = add(p, m)
+            st[0], st[1], st[2] = st[1], st[2], p
+            comps.append(p)
+        p1, p2 = comps
+        out.append(mul(add(sub(p1, p2), c["m1"]), c["norm"]) if p1 <= p2 else mul(sub(p1, p2), c["norm"]))
+    return np.array(out, np.float32)
+
+
+def test_reference_stream_known_answer():
+    """First uniform of the stream seeded with voxel 1, derived by hand:
+    component 1: fma(1403580, 1, -810728) = 592852 (k1 = 0), component 2:
+    fma(527612, 1, -1370589) = -842977 (k1 = 0), + m2 (4294944443 -> 4294944512
+    as float32) = 4294101535 -> 4294101504 (multiple of 256); p1 <= p2, so
+    (592852 - 4294101504) = -4293508652 -> -4293508608, + m1 (4294967040) =
+    1458432, times norm (2.3283064365386963e-10) -> 3.3956766e-04.  The next
+    values involve rounded products and come from the exact model."""
+    assert float(MRG["m1"]) == 4294967040.0 and float(MRG["m2"]) == 4294944512.0
+    u = orc.mrg_stream(1, 64)
+    assert u[0] == np.float32(1458432 * np.float64(MRG["norm"]))
+    assert u[0] == np.float32(3.3956766e-04)
+    for index in (1, 7, 1234567, 98765432, 2 ** 26 + 3):
+        np.testing.assert_array_equal(orc.mrg_stream(index, 64), _mrg_model(index, 64))
+
+
+def test_reference_stream_statistics_and_uncontracted_planting():
+    """Across seeds the stream's directions on a uniform image are 1/6 each and
+    4000-step walks do not drift; the same recurrence without contraction
+    (the planted mistake, in the exact model) drifts by tens of voxels."""
+    rng = np.random.default_rng(0)
+    idx = rng.integers(1, 512 ** 3, size=1500)
+    d = np.stack([np.minimum((orc.mrg_stream(int(i), 4000) * 6).astype(np.int64), 5) for i in idx])
+    freq = np.bincount(d.ravel(), minlength=6) / d.size
+    assert np.all(np.abs(freq - 1 / 6) < 0.002), freq
+    drift = [((d == 2 * a).sum(1) - (d == 2 * a + 1).sum(1)).mean() for a in range(3)]
+    assert np.all(np.abs(drift) < 5.0), drift      # sigma of the mean ~ 36.5 / sqrt(1500) ~ 0.9
+    bad = np.stack([np.minimum((_mrg_model(int(i), 1200, contract=False) * 6).astype(np.int64), 5)
+                    for i in idx[:40]])
+    fb = np.bincount(bad.ravel(), minlength=6) / bad.size
+    assert np.max(np.abs(fb - 1 / 6)) > 0.004, fb
+
+
+def test_reference_walk_consumes_the_stream_in_order():
+    """One seed on a uniform image, N walks of one step: walk n moves by the
+    direction of its uniform u_n (rand = 6 u_n against W_d = d + 1), so the
+    hit counts of the six neighbours follow from the pinned stream."""
+    img = np.full((5, 5, 5), 77, np.uint8)
+    N = 5000
+    hits, acc = _single_seed_walk(img, 1.0, N, 1, ref=True)
+    p0 = 2 * 25 + 2 * 5 + 2
+    u = orc.mrg_stream(p0, N)
+    rand = (np.float32(6.0) * u).astype(np.float32)
+    d = np.where(rand == 0, 0, (rand[:, None] >= np.arange(1, 6, dtype=np.float32)[None, :]).sum(1))
+    want = np.bincount(d, minlength=6)
+    nbrs = [(3, 2, 2), (1, 2, 2), (2, 3, 2), (2, 1, 2), (2, 2, 3), (2, 2, 1)]
+    np.testing.assert_array_equal([hits[0][k] for k in nbrs], want)
 
 
 def test_reference_stream_and_philox_agree_statistically():
