@@ -144,9 +144,11 @@ __global__ __launch_bounds__(256) void k_seed_walks(bmx_seed *__restrict__ seeds
 // current state array, nb = non-empty buckets, bnext = next bucket to walk.
 // The bucket list holds the nheavy buckets of at least one walker per thread
 // at its front and the nlight others at its back (nb = nheavy + nlight), so a
-// round starts its longest buckets first and ends on short ones.
+// round starts its longest buckets first and ends on short ones.  With a tiny
+// threshold, buckets below it go to a list of their own (ntiny of them, taken
+// from tnext), walked without the LDS tile.
 struct RoundCtl {
-    uint32_t nb, bnext, n, done, nheavy, nlight;
+    uint32_t nb, bnext, n, done, nheavy, nlight, ntiny, tnext;
     // persistent walk kernels: first start and last end of the current round
     // (device wall clock), and the summed spans of the rounds so far
     unsigned long long t0, t1, ticks;
@@ -162,6 +164,8 @@ __global__ __launch_bounds__(256) void k_round_begin(uint32_t *__restrict__ coun
         ctl->bnext = 0u;
         ctl->nheavy = 0u;
         ctl->nlight = 0u;
+        ctl->ntiny = 0u;
+        ctl->tnext = 0u;
     }
 }
 
@@ -236,10 +240,13 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const uint4 *__restrict__ 
 
 // Non-empty buckets of the exclusive offsets off[0..nkeys] -> (key, offset,
 // count): buckets of at least `heavy` walkers from the front of the arrays,
-// the others from the back (nheavy + nlight <= nkeys, so the two never meet).
+// the others from the back (nheavy + nlight <= nkeys, so the two never meet);
+// buckets of fewer than `tiny` walkers (tiny = 0: none) into the tiny arrays
+// tkey/toff/tcnt instead.
 __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict__ off, int64_t nkeys,
                                                      uint32_t *bkey, uint32_t *boff, uint32_t *bcnt,
-                                                     RoundCtl *ctl, uint32_t heavy)
+                                                     uint32_t *tkey, uint32_t *toff, uint32_t *tcnt,
+                                                     RoundCtl *ctl, uint32_t heavy, uint32_t tiny)
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0, o = 0;
@@ -247,16 +254,28 @@ __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict_
         o = off[k];
         c = off[k + 1] - o;
     }
+    const bool is_tiny = c < tiny;
     const bool is_heavy = c >= heavy;
     const uint32_t ih = agg_atomic_inc(&ctl->nheavy, 0u, c > 0 && is_heavy);
-    const uint32_t il = agg_atomic_inc(&ctl->nlight, 0u, c > 0 && !is_heavy);
-    if (c > 0) {
+    const uint32_t il = agg_atomic_inc(&ctl->nlight, 0u, c > 0 && !is_heavy && !is_tiny);
+    const uint32_t it = agg_atomic_inc(&ctl->ntiny, 0u, c > 0 && is_tiny);
+    if (c > 0 && is_tiny) {
+        tkey[it] = (uint32_t)k;
+        toff[it] = o;
+        tcnt[it] = c;
+    } else if (c > 0) {
         const uint32_t idx = is_heavy ? ih : (uint32_t)nkeys - 1u - il;
         bkey[idx] = (uint32_t)k;
         boff[idx] = o;
         bcnt[idx] = c;
     }
 }
+
+// Buckets of fewer walkers than this are walked without the LDS tile in the
+// graph-driven rounds: a wave per bucket, a lane per walker, image reads and
+// hits in global memory.  Staging a whole tile for a few dozen walkers leaves
+// the workgroup waiting on one wave's dependent steps.
+constexpr uint32_t kTinyWalkers = 128;
 
 // Weight exp(-(B-A)^2 * div1) of the reference (pyopencl_small.py:187-190).
 __device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
@@ -336,6 +355,65 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
     }
 }
 
+// One walker of a tiny bucket, walked from global memory for up to `cap` steps
+// (or to the end of its walk): the transition law, Philox stream, interior
+// rule and hit weights of tile_block, so the hits it adds are the ones the tile
+// walker would add.  w4: its state, written back by the caller.
+template <typename T>
+__device__ __forceinline__ void tiny_walker(const T *__restrict__ img, const WalkGeom &g, uint32_t gbase,
+                                            const bmx_seed *__restrict__ seeds, uint4 &w4, int sorw, int cap,
+                                            uint32_t weight0, int overrides, uint32_t k0, uint32_t k1,
+                                            uint32_t *__restrict__ hg, uint32_t &acc)
+{
+    uint32_t pos = w4.x - gbase;
+    const uint32_t seed = w4.y, wl = w4.z;
+    int step = (int)w4.w;
+    const bmx_seed sd = seeds[seed];
+    const uint32_t p0 = sd.pos;
+    const float B = bmx_ld(img, (int64_t)p0);
+    const float div1 = 1.0f / (2.0f * sd.var);
+    uint32_t weight = weight0;
+    if (overrides && (sd.reserved >> 16) != 0u) weight = weight0 * (sd.reserved >> 16);
+    int z = (int)(pos / g.sz);
+    const uint32_t rr0 = pos - (uint32_t)z * g.sz;
+    int y = (int)(rr0 / g.sy);
+    int x = (int)(rr0 - (uint32_t)y * g.sy);
+    // interior bounds and index strides as scalars (a select over struct fields
+    // would be lowered to an indexed scratch load)
+    const int zmax = g.Z - 2, ymax = g.Y - 2, xmax = g.X - 2;
+    const uint32_t sz = g.sz, sy = g.sy;
+    for (int taken = 0; taken < cap && step < sorw; taken += 4) {
+        const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (step < sorw) {
+                const uint32_t word = q == 0 ? r4.x : (q == 1 ? r4.y : (q == 2 ? r4.z : r4.w));
+                const int64_t i = pos;
+                const int d = choose(weight_nonpos(B, bmx_ld(img, i + sz), div1),
+                                     weight_nonpos(B, bmx_ld(img, i - sz), div1),
+                                     weight_nonpos(B, bmx_ld(img, i + sy), div1),
+                                     weight_nonpos(B, bmx_ld(img, i - sy), div1),
+                                     weight_nonpos(B, bmx_ld(img, i + 1), div1),
+                                     weight_nonpos(B, bmx_ld(img, i - 1), div1), u01(word));
+                const int axis = d >> 1;
+                const int sgn = 1 - ((d & 1) << 1);
+                const int c = (axis == 0 ? z : (axis == 1 ? y : x)) + sgn;
+                const int hi = axis == 0 ? zmax : (axis == 1 ? ymax : xmax);
+                if (c >= 1 && c <= hi) {
+                    z += axis == 0 ? sgn : 0;
+                    y += axis == 1 ? sgn : 0;
+                    x += axis == 2 ? sgn : 0;
+                    pos += (uint32_t)sgn * (axis == 0 ? sz : (axis == 1 ? sy : 1u));
+                    atomicAdd(&hg[pos], weight);
+                    ++acc;
+                }
+                ++step;
+            }
+        }
+    }
+    w4 = make_uint4(gbase + pos, seed, wl, (uint32_t)step);
+}
+
 // Walks the round's buckets as described at the top of this file.  PERSIST =
 // false: workgroup b walks bucket b (the grid is ctl->nb).  PERSIST = true: a
 // grid of resident workgroups takes buckets from ctl->bnext until all ctl->nb
@@ -346,7 +424,8 @@ template <typename T, int S, bool PERSIST>
 __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
     const T *__restrict__ img_all, TileGeom tg, const bmx_seed *__restrict__ seeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
-    const uint32_t *__restrict__ bcnt, RoundCtl *__restrict__ ctl, uint32_t nkeys, int sorw, int cap,
+    const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ tkey, const uint32_t *__restrict__ toff,
+    const uint32_t *__restrict__ tcnt, RoundCtl *__restrict__ ctl, uint32_t nkeys, int sorw, int cap,
     uint32_t weight0, int overrides, uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits,
     unsigned long long *accepted)
 {
@@ -491,6 +570,32 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
         if (!PERSIST) break;
         __syncthreads();   // the LDS tile and `bucket` are reused by the next bucket
     }
+    // Tiny buckets (graph-driven rounds): each wave takes one, each lane walks one
+    // of its walkers from global memory.  Workgroups come here as they run out of
+    // tile buckets, so this work fills the end of the round.
+    if (PERSIST) {
+        const uint32_t ntiny = ctl->ntiny;
+        const int lane = (int)(threadIdx.x & 63u);
+        uint32_t tacc = 0;
+        while (true) {
+            uint32_t tb = 0;
+            if (lane == 0) tb = atomicAdd(&ctl->tnext, 1u);
+            tb = __builtin_amdgcn_readfirstlane(__shfl(tb, 0, 64));
+            if (tb >= ntiny) break;
+            const uint32_t key = tkey[tb], off = toff[tb], cnt = tcnt[tb];
+            const uint32_t label = key % tg.nlabels, tile = key / tg.nlabels;
+            const VolGeom gv = tg.vols[vol_of_tile(tg.vols, tg.nvol, tile)];
+            const WalkGeom g{gv.Z, gv.Y, gv.X, gv.sy, gv.sz, (int64_t)gv.V};
+            uint32_t *__restrict__ hg = hits + gv.hits_base + (int64_t)label * gv.V;
+            for (uint32_t w = (uint32_t)lane; w < cnt; w += 64u) {
+                uint4 w4 = st[off + w];
+                tiny_walker<T>(img_all + gv.base, g, gv.base, seeds, w4, sorw, cap, weight0, overrides, k0, k1, hg,
+                               tacc);
+                st[off + w] = w4;
+            }
+        }
+        wave_add_u64(accepted, (uint64_t)tacc);
+    }
     // The span of a persistent launch, first workgroup start to last workgroup
     // end, is added to ctl->ticks by the workgroup that ends last, which then
     // clears the round's fields for the next launch.
@@ -536,7 +641,7 @@ struct RoundArgs {
     const T *img;
     TileGeom tg;
     const bmx_seed *seeds;
-    uint32_t *counts, *bkey, *boff, *bcnt;
+    uint32_t *counts, *bkey, *boff, *bcnt, *tkey, *toff, *tcnt;
     RoundCtl *ctl;
     int64_t nkeys;
     int sorw, cap, overrides;
@@ -549,7 +654,7 @@ struct RoundArgs {
 // (tile, label) key, listed as buckets and scattered live into `out`; ctl->n
 // becomes the live count.  `grid` blocks run the per-walker kernels.
 template <typename T, int S>
-int round_bucket(bmx_ctx *ctx, const RoundArgs<T> &a, const uint4 *in, uint4 *out, unsigned grid)
+int round_bucket(bmx_ctx *ctx, const RoundArgs<T> &a, const uint4 *in, uint4 *out, unsigned grid, uint32_t tiny)
 {
     constexpr int C = TileShape<T, S>::C;
     k_round_begin<<<bmx_stride_blocks(a.nkeys + 1, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys + 1, a.ctl);
@@ -559,7 +664,8 @@ int round_bucket(bmx_ctx *ctx, const RoundArgs<T> &a, const uint4 *in, uint4 *ou
     int rc = scan_u32(ctx, a.counts, a.nkeys + 1);
     if (rc) return rc;
     k_bucket_list<<<bmx_blocks(a.nkeys, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys, a.bkey, a.boff, a.bcnt,
-                                                                     a.ctl, (uint32_t)TileShape<T, S>::NTH);
+                                                                     a.tkey, a.toff, a.tcnt, a.ctl,
+                                                                     (uint32_t)TileShape<T, S>::NTH, tiny);
     BMX_LAUNCH_CHECK();
     k_tile_scatter<C><<<grid, 256, 0, ctx->stream>>>(in, a.ctl, out, a.counts, a.tg, a.sorw);
     BMX_LAUNCH_CHECK();
@@ -572,7 +678,8 @@ template <typename T, int S, bool PERSIST>
 int round_walk(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *st, unsigned grid)
 {
     k_tile_walk<T, S, PERSIST><<<grid, TileShape<T, S>::NTH, tile_lds_bytes<T, S>(), ctx->stream>>>(
-        a.img, a.tg, a.seeds, st, a.bkey, a.boff, a.bcnt, a.ctl, (uint32_t)a.nkeys, a.sorw, a.cap, a.weight,
+        a.img, a.tg, a.seeds, st, a.bkey, a.boff, a.bcnt, a.tkey, a.toff, a.tcnt, a.ctl, (uint32_t)a.nkeys, a.sorw,
+        a.cap, a.weight,
         a.overrides, a.k0, a.k1, a.hits, a.acc);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
@@ -588,7 +695,7 @@ int capture_rounds(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *A, uint4 *Bst, un
     int rc = BMX_OK;
     for (int r = 0; r < 2 * pairs && rc == BMX_OK; ++r) {
         uint4 *in = (r & 1) ? Bst : A, *out = (r & 1) ? A : Bst;
-        rc = round_bucket<T, S>(ctx, a, in, out, grid);
+        rc = round_bucket<T, S>(ctx, a, in, out, grid, kTinyWalkers);
         if (rc == BMX_OK) rc = round_walk<T, S, true>(ctx, a, out, wgrid);
     }
     const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
@@ -679,7 +786,7 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     if (rc) return rc;
     rc = grow(ctx, &ctx->d_keys, &ctx->keys_cap, (size_t)(nkeys + 1) * sizeof(uint32_t));
     if (rc) return rc;
-    rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap, (size_t)nkeys * 3 * sizeof(uint32_t));
+    rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap, (size_t)nkeys * 6 * sizeof(uint32_t));
     if (rc) return rc;
     // the scan's block partials, allocated before any capture
     rc = grow(ctx, &ctx->d_scan, &ctx->scan_cap,
@@ -697,6 +804,9 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     a.bkey = (uint32_t *)ctx->d_buckets;
     a.boff = a.bkey + nkeys;
     a.bcnt = a.boff + nkeys;
+    a.tkey = a.bcnt + nkeys;
+    a.toff = a.tkey + nkeys;
+    a.tcnt = a.toff + nkeys;
     a.ctl = (RoundCtl *)(ctx->d_counter + 8);
     static_assert(sizeof(RoundCtl) <= 8 * sizeof(uint64_t), "RoundCtl fits the context's scratch counters");
     a.nkeys = nkeys;
@@ -734,7 +844,7 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     // BMX_WALK_TRACE=1 prints one line per round (live walkers, buckets, kernel ms) to stderr.
     static const bool trace = getenv("BMX_WALK_TRACE") && getenv("BMX_WALK_TRACE")[0] == '1';
     while (n > 0) {
-        rc = round_bucket<T, S>(ctx, a, A, Bst, walker_grid(n));
+        rc = round_bucket<T, S>(ctx, a, A, Bst, walker_grid(n), 0u);
         if (rc) return rc;
         RoundCtl host{};
         BMX_HIP(hipMemcpyAsync(&host, a.ctl, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));

@@ -64,31 +64,38 @@ void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
 
 /* ------------------------------------------------------------------------ */
 /* exp for binary32.  Cody & Waite argument reduction x = n*ln2 + r with a   */
-/* two-part ln2 (hi part exact in 16 bits), fused with fmaf; e^r by a degree-6 */
-/* polynomial 1 + r + c2 r^2 + ... + c6 r^6 whose coefficients are a near-     */
-/* minimax (relative error) fit on |r| <= ln2/2 (fit error 1.7e-8 with the     */
+/* two-part ln2 (hi part exact in 16 bits), fused with fmaf; e^r by a degree-5 */
+/* polynomial 1 + r + c2 r^2 + ... + c5 r^5 whose coefficients are a near-     */
+/* minimax (relative error) fit on |r| <= ln2/2 (fit error 1.06e-7 with the    */
 /* float32 coefficients); 2^n applied with ldexpf, so results below FLT_MIN    */
-/* underflow gradually as IEEE scaling does.                                   */
+/* underflow gradually as IEEE scaling does.  n = round(x log2 e) is read off  */
+/* one fmaf onto 1.5 * 2^23 (its low mantissa bits hold n).  Max error 2.2 ulp */
+/* on [-104, 0] against a long double exp.                                     */
 /* ------------------------------------------------------------------------ */
 float orc_expf(float x)
 {
     const float LOG2E = 1.44269502925872802734375f;     /* 0x3fb8aa3b */
     const float LN2_HI = 0.693145751953125f;            /* 0x3f317200 */
     const float LN2_LO = 1.428606765330187045e-06f;     /* 0x35bfbe8e */
+    const float SHIFT = 12582912.0f;                    /* 0x4b400000 */
     if (x > 88.0f) return INFINITY;
     if (x < -1000.0f) return 0.0f;
-    float t = x * LOG2E;
-    float n = rintf(t);
+    const float s = fmaf(x, LOG2E, SHIFT);
+    const float n = s - SHIFT;
+    uint32_t su, shu;
+    const float sh = SHIFT;
+    memcpy(&su, &s, 4);
+    memcpy(&shu, &sh, 4);
+    const int ni = (int)(su - shu);
     float r = fmaf(-n, LN2_HI, x);
     r = fmaf(-n, LN2_LO, r);
-    float p = 0.00138436537f;                   /* 0x3ab57398 */
-    p = fmaf(p, r, 0.0083741555f);              /* 0x3c0933c1 */
-    p = fmaf(p, r, 0.0416680016f);              /* 0x3d2aac11 */
-    p = fmaf(p, r, 0.166664317f);               /* 0x3e2aaa0d */
-    p = fmaf(p, r, 0.49999994f);                /* 0x3efffffe */
+    float p = 0.00831252523f;
+    p = fmaf(p, r, 0.0418901145f);
+    p = fmaf(p, r, 0.166671142f);
+    p = fmaf(p, r, 0.499992311f);
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
-    return ldexpf(p, (int)n);
+    return ldexpf(p, ni);
 }
 
 /* weight(B, A, div1) of pyopencl_small.py:187-190: exp(-tmp*tmp*div1). */

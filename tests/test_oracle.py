@@ -41,7 +41,10 @@ def test_philox_planted_mistake_fails():
 
 
 # --- exp -----------------------------------------------------------------------
-def test_expf_within_1_1_ulp():
+def test_expf_within_2_2_ulp():
+    """Degree-5 Cody-Waite exp: within 2.2 ulp of float64 exp over the weights'
+    range, including every -t^2/2 of uint8 differences; a planted coefficient
+    error of 1e-5 relative in c2 must exceed the bound."""
     rng = np.random.default_rng(0)
     x = np.concatenate([rng.uniform(-86.5, 0, 2_000_000), np.linspace(-86.5, 0, 100_001),
                         -np.arange(0, 65026, dtype=np.float64) / 2.0]).astype(np.float32)
@@ -49,7 +52,17 @@ def test_expf_within_1_1_ulp():
     y = orc.expf(x).astype(np.float64)
     r = np.exp(x.astype(np.float64))
     ulp = np.spacing(r.astype(np.float32)).astype(np.float64)
-    assert np.max(np.abs(y - r) / ulp) <= 1.1
+    err = np.abs(y - r) / ulp
+    assert np.max(err) <= 2.2
+    assert np.mean(err) < 0.75
+    # the same method with c2 perturbed (float64 evaluation of the reduced polynomial)
+    t = x.astype(np.float64) * 1.4426950408889634
+    n = np.rint(t)
+    rr = x - n * np.log(2)
+    c = [1.0, 1.0, 0.499992311 * (1 + 1e-5), 0.166671142, 0.0418901145, 0.00831252523]
+    p = sum(ck * rr ** k for k, ck in enumerate(c))
+    bad = np.abs(p * 2.0 ** n - r) / ulp
+    assert np.max(bad) > 2.2
 
 
 def test_expf_special_values():
