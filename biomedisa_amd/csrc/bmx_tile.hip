@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict_
 // graph-driven rounds: a wave per bucket, a lane per walker, image reads and
 // hits in global memory.  Staging a whole tile for a few dozen walkers leaves
 // the workgroup waiting on one wave's dependent steps.
-constexpr uint32_t kTinyWalkers = 128;
+constexpr uint32_t kTinyWalkers = 64;
 
 // Weight exp(-(B-A)^2 * div1) of the reference (pyopencl_small.py:187-190).
 __device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
