@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbmx_hip.so")
 
 BMX_OK = 0
+NOALLOC_SCRATCH_BYTES = 64 << 20   # BMX_NOALLOC_SCRATCH_BYTES
 BMX_E_INVALID, BMX_E_OOM, BMX_E_DEVICE = -1, -2, -3
 IMG_U8, IMG_F32 = 0, 1
 RNG_PHILOX, RNG_REF = 0, 1
@@ -84,6 +85,8 @@ SIGNATURES = {
                       C.POINTER(C.c_uint64)]),
     "bmx_walk_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, _i, C.c_uint64, _p,
                          C.POINTER(C.c_uint64), C.POINTER(WalkOpts)]),
+    "bmx_walk_noalloc": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _i64, _i, _i, _i, C.c_uint64, _p,
+                              C.POINTER(C.c_uint64)]),
     "bmx_walk_batch": (_i, [_p, _p, _i, _p, _i, _p, _i, _i, C.c_uint64, _p, C.POINTER(C.c_uint64),
                             C.POINTER(WalkOpts)]),
     "bmx_last_walk_ms": (C.c_float, [_p]),
@@ -329,6 +332,17 @@ class Engine:
                                   int(nlabels), int(nbrw), int(sorw), int(rng),
                                   C.c_uint64(int(key) & (2**64 - 1)), hits.ptr, C.byref(acc),
                                   C.byref(opts)))
+        return acc.value
+
+    def walk_noalloc(self, img, img_dtype, dims, seeds, nseeds, nlabels, nbrw, sorw, key, hits):
+        """bmx_walk_noalloc: the direct Philox walk with its scratch in the library's
+        static device buffer (no allocation); returns the accepted moves."""
+        Z, Y, X = dims
+        acc = C.c_uint64()
+        _check(load().bmx_walk_noalloc(self.ctx, img.ptr, img_dtype, Z, Y, X,
+                                       seeds.ptr if seeds is not None else None, int(nseeds), int(nlabels),
+                                       int(nbrw), int(sorw), C.c_uint64(int(key) & (2**64 - 1)), hits.ptr,
+                                       C.byref(acc)))
         return acc.value
 
     def walk_batch(self, img, img_dtype, volumes, seeds, nbrw, sorw, key, hits, segment_steps=0,

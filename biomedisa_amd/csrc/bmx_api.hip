@@ -5,6 +5,41 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <mutex>
+
+// Scratch of bmx_walk_noalloc: one static buffer per GPU, held until the
+// process ends.  In host code `scratch` only names the symbol; kernels get its
+// device address, looked up once per device in bmx_open.
+static __device__ __align__(256) unsigned char scratch[BMX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+}  // namespace
+
+void *bmx_noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+// With `device` current: its address of the scratch buffer, once.
+static int resolve_scratch(int device)
+{
+    BMX_CHECK(device < kMaxDevices, "bmx_open: device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return BMX_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        return bmx_set_error(BMX_E_DEVICE, "bmx_open: hipGetSymbolAddress(scratch) failed: %s",
+                             hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return BMX_OK;
+}
+
 static thread_local char g_err[512] = "";
 
 int bmx_set_error(int code, const char *fmt, ...)
@@ -37,6 +72,8 @@ int bmx_open(int device, bmx_ctx **out)
     BMX_HIP(hipGetDeviceCount(&n));
     BMX_CHECK(device >= 0 && device < n, "bmx_open: device %d not in [0,%d)", device, n);
     BMX_HIP(hipSetDevice(device));
+    const int rs = resolve_scratch(device);
+    if (rs) return rs;
     bmx_ctx *c = (bmx_ctx *)calloc(1, sizeof(bmx_ctx));
     if (!c) return bmx_set_error(BMX_E_OOM, "bmx_open: host allocation failed");
     c->device = device;

@@ -69,6 +69,13 @@ int bmx_set_error(int code, const char *fmt, ...);
 
 int bmx_set_device(bmx_ctx *ctx);
 
+// Device address of the static scratch buffer (BMX_NOALLOC_SCRATCH_BYTES) on
+// `device`, resolved once per device by bmx_open; nullptr before that.  Used
+// by bmx_walk_noalloc: the move counter at offset 0, records from
+// kNoAllocRecOffset.
+void *bmx_noalloc_scratch(int device);
+constexpr size_t kNoAllocRecOffset = 256;
+
 // Tile-owner walker (bmx_tile.hip), called by bmx_walk_ex (one volume) and
 // bmx_walk_batch.  vols is a host array of nvol validated volumes.  On return
 // *launches = walk rounds; *kernel_ms = -1 for host-driven rounds (walk

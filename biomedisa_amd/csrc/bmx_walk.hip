@@ -524,6 +524,50 @@ static int finish_walk(bmx_ctx *ctx, bool run, int launches, uint64_t *accepted,
 }
 
 
+// Packed neighbour records of the direct Philox walkers (V * 8 bytes for
+// uint8, V * sizeof(RecF32) for float32) into rec.
+static int pack_records(bmx_ctx *ctx, const void *img, int img_dtype, const WalkGeom &g, void *rec)
+{
+    if (img_dtype == BMX_IMG_U8)
+        k_pack_u8<<<bmx_blocks(g.V, 256), 256, 0, ctx->stream>>>((const uint8_t *)img, g, (uint64_t *)rec);
+    else
+        k_pack_f32<<<bmx_blocks(g.V, 256), 256, 0, ctx->stream>>>((const float *)img, g, (RecF32 *)rec);
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+// The direct Philox walk over packed records: per-seed weight tables in LDS for
+// uint8 images with 8..256 walks per seed, exact weights otherwise.
+static int walk_direct_philox(bmx_ctx *ctx, const void *img, int img_dtype, const WalkGeom &g,
+                              const bmx_seed *seeds, int64_t nseeds, int nbrw, int sorw, uint32_t k0, uint32_t k1,
+                              uint32_t *hits, const void *rec, unsigned long long *acc)
+{
+    const int64_t nwalkers = nseeds * (int64_t)nbrw;
+    if (img_dtype == BMX_IMG_U8 && nbrw >= kLutMinWalks && nbrw <= kWalkThreads) {
+        const int spb = kWalkThreads / nbrw;
+        const int64_t blocks = (nseeds + spb - 1) / spb;
+        BMX_CHECK(blocks < ((int64_t)1 << 31), "bmx_walk: too many walkers");
+        k_walk_philox_u8_lut<<<(unsigned)blocks, kWalkThreads, (size_t)spb * 256 * sizeof(float), ctx->stream>>>(
+            (const uint64_t *)rec, (const uint8_t *)img, g, seeds, nseeds, nbrw, sorw, spb, k0, k1, hits, acc);
+    } else {
+        const int64_t blocks = (nwalkers + kWalkThreads - 1) / kWalkThreads;
+        BMX_CHECK(blocks < ((int64_t)1 << 31), "bmx_walk: too many walkers");
+        if (img_dtype == BMX_IMG_U8)
+            k_walk_philox_exp<uint8_t><<<(unsigned)blocks, kWalkThreads, 0, ctx->stream>>>(
+                rec, (const uint8_t *)img, g, seeds, nseeds, nbrw, sorw, k0, k1, hits, acc);
+        else
+            k_walk_philox_exp<float><<<(unsigned)blocks, kWalkThreads, 0, ctx->stream>>>(
+                rec, (const float *)img, g, seeds, nseeds, nbrw, sorw, k0, k1, hits, acc);
+    }
+    BMX_LAUNCH_CHECK();
+    return BMX_OK;
+}
+
+__global__ void k_zero_counter(unsigned long long *c)
+{
+    if (threadIdx.x == 0) *c = 0ull;
+}
+
 extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y,
                            int64_t X, const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw,
                            int sorw, int rng, uint64_t key, uint32_t *hits, uint64_t *accepted,
@@ -573,13 +617,8 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
         const size_t rec_bytes = (size_t)g.V * (img_dtype == BMX_IMG_U8 ? 8 : sizeof(RecF32));
         rc = ensure_records(ctx, rec_bytes);
         if (rc) return rc;
-        if (img_dtype == BMX_IMG_U8)
-            k_pack_u8<<<bmx_blocks(g.V, 256), 256, 0, ctx->stream>>>((const uint8_t *)img, g,
-                                                                    (uint64_t *)ctx->d_rec);
-        else
-            k_pack_f32<<<bmx_blocks(g.V, 256), 256, 0, ctx->stream>>>((const float *)img, g,
-                                                                     (RecF32 *)ctx->d_rec);
-        BMX_LAUNCH_CHECK();
+        rc = pack_records(ctx, img, img_dtype, g, ctx->d_rec);
+        if (rc) return rc;
     }
     int launches = 0;
     float graph_ms = -1.0f;
@@ -599,27 +638,8 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
                              blog, &launches);
             if (rc) return rc;
         } else if (rng == BMX_RNG_PHILOX) {
-            if (img_dtype == BMX_IMG_U8 && nbrw >= kLutMinWalks && nbrw <= kWalkThreads) {
-                const int spb = kWalkThreads / nbrw;
-                const int64_t blocks = (nseeds + spb - 1) / spb;
-                BMX_CHECK(blocks < ((int64_t)1 << 31), "bmx_walk: too many walkers");
-                k_walk_philox_u8_lut<<<(unsigned)blocks, kWalkThreads, (size_t)spb * 256 * sizeof(float),
-                                       ctx->stream>>>((const uint64_t *)ctx->d_rec,
-                                                      (const uint8_t *)img, g, seeds, nseeds, nbrw,
-                                                      sorw, spb, k0, k1, hits, acc);
-            } else {
-                const int64_t blocks = (nwalkers + kWalkThreads - 1) / kWalkThreads;
-                BMX_CHECK(blocks < ((int64_t)1 << 31), "bmx_walk: too many walkers");
-                if (img_dtype == BMX_IMG_U8)
-                    k_walk_philox_exp<uint8_t><<<(unsigned)blocks, kWalkThreads, 0, ctx->stream>>>(
-                        ctx->d_rec, (const uint8_t *)img, g, seeds, nseeds, nbrw, sorw, k0, k1, hits,
-                        acc);
-                else
-                    k_walk_philox_exp<float><<<(unsigned)blocks, kWalkThreads, 0, ctx->stream>>>(
-                        ctx->d_rec, (const float *)img, g, seeds, nseeds, nbrw, sorw, k0, k1, hits,
-                        acc);
-            }
-            BMX_LAUNCH_CHECK();
+            rc = walk_direct_philox(ctx, img, img_dtype, g, seeds, nseeds, nbrw, sorw, k0, k1, hits, ctx->d_rec, acc);
+            if (rc) return rc;
         } else {
             const int64_t blocks = (nseeds + kWalkThreads - 1) / kWalkThreads;
             if (img_dtype == BMX_IMG_U8)
@@ -633,6 +653,65 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
     }
     BMX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     return finish_walk(ctx, run, launches, accepted, graph_ms);
+}
+
+// The direct Philox walk with its packed records and move counter in the
+// library's static scratch buffer (bmx_noalloc_scratch): no device memory is
+// allocated.  Layout of the buffer: the move counter at offset 0, the records
+// from kNoAllocRecOffset.
+extern "C" int bmx_walk_noalloc(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,
+                                const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw,
+                                uint64_t key, uint32_t *hits, uint64_t *accepted)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(img && hits, "bmx_walk_noalloc: null pointer");
+    BMX_CHECK(nseeds == 0 || seeds, "bmx_walk_noalloc: null seeds");
+    BMX_CHECK(img_dtype == BMX_IMG_U8 || img_dtype == BMX_IMG_F32, "bmx_walk_noalloc: bad image dtype %d",
+              img_dtype);
+    BMX_CHECK(Z >= 3 && Y >= 3 && X >= 3 && Z * Y * X < (int64_t)1 << 32,
+              "bmx_walk_noalloc: bad dims %lld x %lld x %lld", (long long)Z, (long long)Y, (long long)X);
+    BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_walk_noalloc: nlabels %d not in [1,255]", nlabels);
+    BMX_CHECK(nbrw >= 0 && sorw >= 0 && nbrw <= 65535, "bmx_walk_noalloc: nbrw/sorw out of range");
+    BMX_CHECK(nseeds >= 0, "bmx_walk_noalloc: negative nseeds");
+    WalkGeom g{(int)Z, (int)Y, (int)X, (uint32_t)X, (uint32_t)(X * Y), Z * Y * X};
+    const size_t rec_bytes = (size_t)g.V * (img_dtype == BMX_IMG_U8 ? 8 : sizeof(RecF32));
+    BMX_CHECK(rec_bytes <= (size_t)BMX_NOALLOC_SCRATCH_BYTES - kNoAllocRecOffset,
+              "bmx_walk_noalloc: a %lld-voxel volume needs %zu bytes of scratch, above the %u-byte maximum "
+              "(%llu voxels for uint8 images, %llu for float32)", (long long)g.V, rec_bytes + kNoAllocRecOffset,
+              (unsigned)BMX_NOALLOC_SCRATCH_BYTES,
+              (unsigned long long)((BMX_NOALLOC_SCRATCH_BYTES - kNoAllocRecOffset) / 8),
+              (unsigned long long)((BMX_NOALLOC_SCRATCH_BYTES - kNoAllocRecOffset) / sizeof(RecF32)));
+    unsigned char *scr = (unsigned char *)bmx_noalloc_scratch(ctx->device);
+    BMX_CHECK(scr != nullptr, "bmx_walk_noalloc: no scratch buffer on device %d", ctx->device);
+    unsigned long long *acc = (unsigned long long *)scr;
+    void *rec = scr + kNoAllocRecOffset;
+    k_zero_counter<<<1, 64, 0, ctx->stream>>>(acc);
+    BMX_LAUNCH_CHECK();
+    const bool run = nseeds > 0 && nbrw > 0 && sorw > 0;
+    if (run) {
+        rc = pack_records(ctx, img, img_dtype, g, rec);
+        if (rc) return rc;
+    }
+    BMX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (run) {
+        rc = walk_direct_philox(ctx, img, img_dtype, g, seeds, nseeds, nbrw, sorw, (uint32_t)key,
+                                (uint32_t)(key >> 32), hits, rec, acc);
+        if (rc) return rc;
+    }
+    BMX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    if (accepted) {
+        uint64_t a = 0;
+        BMX_HIP(hipMemcpyAsync(&a, acc, sizeof(a), hipMemcpyDeviceToHost, ctx->stream));
+        BMX_HIP(hipStreamSynchronize(ctx->stream));
+        *accepted = a;
+        float ms = 0.0f;
+        BMX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_walk_ms = ms;
+        ctx->last_walk_kernel_ms = ms;
+        ctx->last_walk_launches = run ? 1 : 0;
+    }
+    return BMX_OK;
 }
 
 extern "C" int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,

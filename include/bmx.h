@@ -194,6 +194,22 @@ int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y,
              const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw,
              int rng, uint64_t key, uint32_t *hits, uint64_t *accepted);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the Philox walk of bmx_walk_ex with algorithm BMX_WALK_DIRECT
+ * (identical hits and accepted moves), whose packed neighbour records and
+ * move counter live in a static device buffer of BMX_NOALLOC_SCRATCH_BYTES per
+ * GPU instead of context memory: 256 bytes plus V * 8 bytes for uint8 images
+ * or V * 32 bytes for float32, so volumes up to 8 Mi voxels (uint8) or 2 Mi
+ * voxels (float32); larger ones are rejected with BMX_E_INVALID.  Calls no
+ * hipMalloc, hipFree or allocating routine.  Every call on one GPU uses that
+ * GPU's buffer: calls that overlap on one GPU (from several contexts or
+ * threads) race.  hits is accumulated into, as for bmx_walk. */
+#define BMX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int bmx_walk_noalloc(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,
+                     const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw, uint64_t key,
+                     uint32_t *hits, uint64_t *accepted);
+
 /* Walker algorithms (identical hit counts; they differ only in speed). */
 #define BMX_WALK_AUTO 0   /* tiled for >= 2^16 walkers, hit weights or per-seed overrides
                              (Philox), direct otherwise */
