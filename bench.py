@@ -174,6 +174,9 @@ def main():
                     help="skip the Dice of the Philox walker against the reference stream at this size")
     ap.add_argument("--workload", choices=["512", "batch", "large", "post"], default="512")
     ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--calibrate-valu", action="store_true",
+                    help="also launch bmx_valu_calibrate once (256 Ki threads x 2000 x 128 float32 FMAs), the "
+                         "known-count reference for VALU counters")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch bmx_f32_to_bf16 over 64 Mi floats (reads 256 MiB, writes "
                          "128 MiB) once, as the known-traffic reference for HBM counters")
@@ -230,6 +233,8 @@ def main():
         out, tms, sec = time_pipeline(resident_step(method, g), eng, args.warmup, args.steps, name)
         res[name] = dict(sec=sec, tms=tms, regular=out["regular"].download(), smooth=out["smooth"].download())
         del out
+    if args.calibrate_copy or args.calibrate_valu:
+        calibrate(eng, args)
     # the drop-in call itself, NumPy arrays in and out (H2D and D2H included)
     for name, method, g in ([] if args.no_numpy else methods[:2]):
         tm = {}
@@ -349,6 +354,22 @@ def main():
                       "full plain walk extrapolated to %.0fs; smoothing/merge not counted"
                       % (nsample, dt, rate, walk_s)}
     print(json.dumps(line), flush=True)
+
+
+def calibrate(eng, args):
+    """Known-work kernels for scaling counters read in the same run: a copy of
+    64 Mi floats to bf16 (256 MiB read, 128 MiB written) and an FMA-chain
+    kernel of 262144 threads x 2000 x 128 float32 FMAs."""
+    import ctypes
+    from biomedisa_amd import _hip
+    if args.calibrate_valu:
+        threads, iters = 1 << 18, 2000
+        out = eng.alloc(threads * 4)
+        ms = ctypes.c_float()
+        _hip.call("bmx_valu_calibrate", eng.ctx, out.ptr, threads, iters, ctypes.byref(ms))
+        fma = threads * iters * 128
+        log("calibration FMA chains: %.4g FMAs in %.3f ms = %.2f TFLOP/s, %.4g wave-level instructions"
+            % (fma, ms.value, 2 * fma / (ms.value * 1e-3) / 1e12, fma / 64))
 
 
 def reference_stream_dice(eng, vol, plain_regular, args, chunk=1 << 22):

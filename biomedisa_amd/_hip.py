@@ -72,6 +72,7 @@ SIGNATURES = {
     "bmx_refine_stats": (_i, [_p, _p, _p, _p, _i64, _i64, _i64, _p]),
     "bmx_refine_apply": (_i, [_p, _p, _p, _i64, _i64, _i64, _p, _p, _p]),
     "bmx_histogram": (_i, [_p, _p, _i64, _p]),
+    "bmx_valu_calibrate": (_i, [_p, _p, _i64, _i, C.POINTER(C.c_float)]),
     "bmx_denoise": (_i, [_p, _p, _i, _i64, _i64, _i64, _p]),
     "bmx_hxbyterle_decode": (_i64, [_p, _i64, _p, _i64]),
     "bmx_hxbyterle_encode": (_i64, [_p, _i64, _p, _i64]),

@@ -194,3 +194,39 @@ int bmx_zero(bmx_ctx *ctx, void *dst_dev, size_t bytes)
 float bmx_last_walk_ms(bmx_ctx *ctx) { return ctx ? ctx->last_walk_ms : -1.0f; }
 
 }  // extern "C"
+
+// VALU calibration: `threads` threads (256 per block) each run `iters` x 16
+// steps of 8 independent fused multiply-add chains, i.e. threads*iters*128/64
+// wave-level v_fma_f32 instructions with every lane active, so that a VALU
+// instruction counter read over this kernel gives the rate of a full-rate op.
+__global__ __launch_bounds__(256) void k_valu_cal(float *__restrict__ out, int iters, float b, float c)
+{
+    float a0 = (float)threadIdx.x, a1 = a0 + 1.f, a2 = a0 + 2.f, a3 = a0 + 3.f;
+    float a4 = a0 + 4.f, a5 = a0 + 5.f, a6 = a0 + 6.f, a7 = a0 + 7.f;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            a0 = __builtin_fmaf(a0, b, c); a1 = __builtin_fmaf(a1, b, c);
+            a2 = __builtin_fmaf(a2, b, c); a3 = __builtin_fmaf(a3, b, c);
+            a4 = __builtin_fmaf(a4, b, c); a5 = __builtin_fmaf(a5, b, c);
+            a6 = __builtin_fmaf(a6, b, c); a7 = __builtin_fmaf(a7, b, c);
+        }
+    }
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+}
+
+extern "C" int bmx_valu_calibrate(bmx_ctx *ctx, float *out, int64_t threads, int iters, float *ms)
+{
+    int rc = bmx_set_device(ctx);
+    if (rc) return rc;
+    BMX_CHECK(out && ms, "bmx_valu_calibrate: null pointer");
+    BMX_CHECK(threads >= 256 && threads % 256 == 0 && threads / 256 < ((int64_t)1 << 31) && iters >= 1,
+              "bmx_valu_calibrate: threads must be a positive multiple of 256, iters >= 1");
+    BMX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    k_valu_cal<<<(unsigned)(threads / 256), 256, 0, ctx->stream>>>(out, iters, 0.999f, 0.001f);
+    BMX_LAUNCH_CHECK();
+    BMX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    BMX_HIP(hipEventSynchronize(ctx->ev1));
+    BMX_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return BMX_OK;
+}

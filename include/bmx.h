@@ -335,6 +335,13 @@ int bmx_png_unfilter(const uint8_t *in, int64_t rows, int64_t rowbytes, int bpp,
  * double, stored in the image's type; uint8 truncates).  out != img. */
 int bmx_denoise(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X, void *out);
 
+/* Measurement aid, no counterpart in the reference: `threads` threads (a
+ * multiple of 256) run iters x 128 independent float32 fused multiply-adds
+ * each into out[threads]; *ms receives the kernel time.  Its known count of
+ * full-rate VALU instructions (threads * iters * 2 wave-level instructions)
+ * scales VALU counters read in the same profiling run. */
+int bmx_valu_calibrate(bmx_ctx *ctx, float *out, int64_t threads, int iters, float *ms);
+
 /* Counts of the 256 values of a uint8 device array (host uint64[256] out);
  * below 2^32 per value.  Used for the label sets of the post-processing. */
 int bmx_histogram(bmx_ctx *ctx, const uint8_t *vol, int64_t n, uint64_t *counts);
