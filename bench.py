@@ -362,6 +362,13 @@ def calibrate(eng, args):
     kernel of 262144 threads x 2000 x 128 float32 FMAs."""
     import ctypes
     from biomedisa_amd import _hip
+    if args.calibrate_copy:
+        n_cal = 64 << 20
+        src, dst = eng.alloc(n_cal * 4), eng.alloc(n_cal * 2)
+        src.zero()
+        _hip.call("bmx_f32_to_bf16", eng.ctx, src.ptr, dst.ptr, n_cal)
+        eng.sync()
+        log("calibration copy: %d floats -> bf16" % n_cal)
     if args.calibrate_valu:
         threads, iters = 1 << 18, 2000
         out = eng.alloc(threads * 4)

@@ -9,16 +9,15 @@
 // per step caps the walker at the fabric's atomic rate whatever the locality.
 // Here a workgroup owns one (core tile, label) bucket: a C^3 core tile plus a
 // halo of H voxels, i.e. an R^3 region (R = C + 2H).  It stages the region's
-// image (with one more voxel of halo for the neighbour reads) and R^3 hit
-// counters in LDS and runs every walker of the bucket - walkers whose current
-// voxel lies in the core - until the walker leaves the region, finishes its
-// walk, or has made `cap` steps in this round.  Steps taken outside the region
-// (inside the Philox block during which the walker left) read the image from
-// global memory and add their hit with a global atomic.  At the end the
-// workgroup adds each non-zero LDS counter to the hit volume with one global
-// atomic.  Between rounds the walkers are counting-sorted by (core tile,
-// label) again.  A walker pauses only at a multiple of 4 steps, so every
-// Philox block is used whole within one round.
+// image and hit counters, both with one more voxel of halo, in LDS and runs
+// every walker of the bucket - walkers whose current voxel lies in the core -
+// until the walker steps out of the region, finishes its walk, or has made
+// `cap` steps in this round.  Every step starts in the region, so its
+// neighbour reads and its hit stay in LDS.  At the end the workgroup adds
+// each non-zero LDS counter to the hit volume with one global atomic.
+// Between rounds the walkers are counting-sorted by (core tile, label) again.
+// A walker may pause inside a Philox block; step s always uses word s & 3 of
+// block s >> 2, wherever the step is taken.
 #include "bmx_walk_common.h"
 
 #include <algorithm>
@@ -283,46 +282,35 @@ __device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
     return bmx_weight(B, A, div1);
 }
 
-// One Philox block (4 steps) of one walker.  Local coordinates (lz,ly,lx)
-// address the LDS tile, whose voxel (1,1,1) is the region origin; the region
-// proper is [1,R]^3 and the tile adds one voxel on each side for neighbour
-// reads.  li is the walker's index in the tile; the global voxel index is
-// rebuilt from the coordinates only on the rare steps that leave the tile.
-// EDGE: the tile may reach the volume boundary, so moves are checked against
-// [lmin, lmax] per axis (the local range of interior voxels); otherwise every
-// move is interior.  A step that starts inside the region lands inside the
-// tile, so its hit always goes to the LDS counters.
+// The steps of one Philox block (4 steps) that one walker takes in this
+// tile.  Local coordinates (lz,ly,lx) address the LDS tile, whose voxel
+// (1,1,1) is the region origin; the region proper is [1,R]^3 and the tile adds
+// one voxel on each side for neighbour reads.  li is the walker's index in the
+// tile.  Step s uses word s & 3 of block s >> 2, so a walker whose visit began
+// or ended inside a block resumes at the right word.  A step is taken only
+// from a voxel of the region: its neighbours and its landing voxel are in the
+// tile, so every read and hit stays in LDS; a walker that steps out of the
+// region stops, and the next round continues it from the tile that owns its
+// voxel.  EDGE: the tile may reach the volume boundary, so moves are checked
+// against [lmin, lmax] per axis (the local range of interior voxels);
+// otherwise every move is interior.
 template <typename T, int S, bool EDGE>
-__device__ __forceinline__ void tile_block(const T *__restrict__ img, const WalkGeom &g, const T *simg,
-                                           uint32_t *shits, uint32_t *__restrict__ hg, int sorw,
-                                           const uint32_t rr[4], float B, float div1, int &step, int &li,
-                                           int &lz, int &ly, int &lx, int bz, int by, int bx, int lmin0,
-                                           int lmin1, int lmin2, int lmax0, int lmax1, int lmax2,
+__device__ __forceinline__ void tile_block(const T *simg, uint32_t *shits, int sorw, const uint32_t rr[4],
+                                           float B, float div1, int &step, int &li, int &lz, int &ly, int &lx,
+                                           int lmin0, int lmin1, int lmin2, int lmax0, int lmax1, int lmax2,
                                            uint32_t weight, uint32_t &acc)
 {
     constexpr int R = TileShape<T, S>::C + 2 * TileShape<T, S>::H, RI = R + 2;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (step < sorw) {
-            const bool in = max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) <
-                            (unsigned)R;
-            float a0, a1, a2, a3, a4, a5;
-            if (in) {
-                a0 = (float)simg[li + RI * RI];
-                a1 = (float)simg[li - RI * RI];
-                a2 = (float)simg[li + RI];
-                a3 = (float)simg[li - RI];
-                a4 = (float)simg[li + 1];
-                a5 = (float)simg[li - 1];
-            } else {
-                const int64_t pos = ((int64_t)(bz + lz) * g.sz + (int64_t)(by + ly) * g.sy) + (bx + lx);
-                a0 = bmx_ld(img, pos + g.sz);
-                a1 = bmx_ld(img, pos - g.sz);
-                a2 = bmx_ld(img, pos + g.sy);
-                a3 = bmx_ld(img, pos - g.sy);
-                a4 = bmx_ld(img, pos + 1);
-                a5 = bmx_ld(img, pos - 1);
-            }
+        const bool in = max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
+        if (in && step < sorw && (step & 3) == q) {
+            const float a0 = (float)simg[li + RI * RI];
+            const float a1 = (float)simg[li - RI * RI];
+            const float a2 = (float)simg[li + RI];
+            const float a3 = (float)simg[li - RI];
+            const float a4 = (float)simg[li + 1];
+            const float a5 = (float)simg[li - 1];
             const int d = choose(weight_nonpos(B, a0, div1), weight_nonpos(B, a1, div1),
                                  weight_nonpos(B, a2, div1), weight_nonpos(B, a3, div1),
                                  weight_nonpos(B, a4, div1), weight_nonpos(B, a5, div1), u01(rr[q]));
@@ -342,13 +330,7 @@ __device__ __forceinline__ void tile_block(const T *__restrict__ img, const Walk
                 const int lmag = axis == 0 ? RI * RI : (axis == 1 ? RI : 1);
                 li += sgn * lmag;
                 ++acc;
-                if (in || max(max((unsigned)lz, (unsigned)ly), (unsigned)lx) < (unsigned)RI) {
-                    atomicAdd(&shits[li], weight);
-                } else {
-                    const int64_t pos =
-                        ((int64_t)(bz + lz) * g.sz + (int64_t)(by + ly) * g.sy) + (bx + lx);
-                    atomicAdd(&hg[pos], weight);
-                }
+                atomicAdd(&shits[li], weight);
             }
             ++step;
         }
@@ -382,11 +364,12 @@ __device__ __forceinline__ void tiny_walker(const T *__restrict__ img, const Wal
     // would be lowered to an indexed scratch load)
     const int zmax = g.Z - 2, ymax = g.Y - 2, xmax = g.X - 2;
     const uint32_t sz = g.sz, sy = g.sy;
-    for (int taken = 0; taken < cap && step < sorw; taken += 4) {
+    const int step0 = step;
+    while (step - step0 < cap && step < sorw) {
         const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            if (step < sorw) {
+            if (step < sorw && (step & 3) == q) {
                 const uint32_t word = q == 0 ? r4.x : (q == 1 ? r4.y : (q == 2 ? r4.z : r4.w));
                 const int64_t i = pos;
                 const int d = choose(weight_nonpos(B, bmx_ld(img, i + sz), div1),
@@ -511,7 +494,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
         uint32_t acc = 0;  // accepted moves of this thread in this bucket (< cnt * cap < 2^32)
         bool have = false;
         uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = weight0;
-        int step = 0, lz = 0, ly = 0, lx = 0, li = 0, taken = 0;
+        int step = 0, step0 = 0, lz = 0, ly = 0, lx = 0, li = 0;
         float B = 0.0f, div1 = 0.0f;
         while (true) {
             if (!have) {
@@ -536,21 +519,20 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
                 ly = y - by;
                 lx = x - bx;
                 li = (lz * RI + ly) * RI + lx;
-                taken = 0;
+                step0 = step;
                 have = true;
             }
             const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
             const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
             if (edge)
-                tile_block<T, S, true>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by,
-                                       bx, lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
+                tile_block<T, S, true>(simg, shits, sorw, rr, B, div1, step, li, lz, ly, lx, lmin0, lmin1, lmin2,
+                                       lmax0, lmax1, lmax2, weight, acc);
             else
-                tile_block<T, S, false>(img, g, simg, shits, hg, sorw, rr, B, div1, step, li, lz, ly, lx, bz, by,
-                                        bx, lmin0, lmin1, lmin2, lmax0, lmax1, lmax2, weight, acc);
-            taken += 4;
+                tile_block<T, S, false>(simg, shits, sorw, rr, B, div1, step, li, lz, ly, lx, lmin0, lmin1, lmin2,
+                                        lmax0, lmax1, lmax2, weight, acc);
             const bool inside =
                 max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
-            if (step >= sorw || !inside || taken >= cap) {
+            if (step >= sorw || !inside || step - step0 >= cap) {
                 const uint32_t pos =
                     (uint32_t)(bz + lz) * g.sz + (uint32_t)(by + ly) * g.sy + (uint32_t)(bx + lx);
                 st[off + j] = make_uint4(gv.base + pos, seed, wl, (uint32_t)step);
