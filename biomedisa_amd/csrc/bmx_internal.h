@@ -35,6 +35,8 @@ struct bmx_ctx {
     size_t scan_cap;
     void *d_vols;            // tiled walker: volume table of the walk
     size_t vols_cap;
+    void *d_wseeds;          // tiled walker: per-seed constants of the walk
+    size_t wseeds_cap;
     hipEvent_t *seg_ev;      // sorted walker: event pairs around each walk launch
     float last_walk_kernel_ms;
     int last_walk_launches;

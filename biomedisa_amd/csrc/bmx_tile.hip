@@ -64,6 +64,34 @@ struct VolGeom {
     uint32_t tile_base, nty, ntx;
     uint32_t seed_base, pad;
     uint64_t hits_base;
+    uint32_t mz, my;     // multipliers of the exact divisions by sz and sy (div_u32)
+    uint32_t shz, shy;   // their shifts
+};
+
+// Exact quotient n / d for every 32-bit n, d >= 2, from the multiplier m and
+// shift sh of magic_u32 (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994, Figure 4.1: the multiplier is
+// 2^32 + m, applied as hi + ((n - hi) >> 1) so that no sum passes 32 bits).
+__device__ __forceinline__ uint32_t div_u32(uint32_t n, uint32_t m, uint32_t sh)
+{
+    const uint32_t hi = __umulhi(n, m);
+    return (hi + ((n - hi) >> 1)) >> sh;
+}
+static void magic_u32(uint32_t d, uint32_t *m, uint32_t *sh)
+{
+    int l = 0;
+    while (((uint64_t)1 << l) < d) ++l;   // l = ceil(log2 d), 1 <= l <= 32 for d >= 2
+    *m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+    *sh = (uint32_t)(l - 1);
+}
+
+// Per-seed constants of a walk, built once per walk (k_walk_seeds): the seed's
+// volume-local position (the key of its Philox streams), its image value B,
+// div1 = 1 / (2 var) and the weight its walkers add per hit.
+struct WalkSeed {
+    uint32_t pos;
+    float B, div1;
+    uint32_t weight;
 };
 
 struct TileGeom {
@@ -106,9 +134,9 @@ __device__ __forceinline__ uint32_t tile_key(uint32_t pos, uint32_t label, const
 {
     const VolGeom &g = tg.vols[vol_of_pos(tg.vols, tg.nvol, pos)];
     const uint32_t r0 = pos - g.base;
-    const uint32_t z = r0 / g.sz;
+    const uint32_t z = div_u32(r0, g.mz, g.shz);
     const uint32_t r = r0 - z * g.sz;
-    const uint32_t y = r / g.sy;
+    const uint32_t y = div_u32(r, g.my, g.shy);
     const uint32_t x = r - y * g.sy;
     return (g.tile_base + ((z / C) * g.nty + y / C) * g.ntx + x / C) * tg.nlabels + label;
 }
@@ -130,6 +158,20 @@ __global__ __launch_bounds__(256) void k_tile_init(const bmx_seed *__restrict__ 
     const uint32_t walks = overrides ? (sd.reserved & 0xFFFFu) : 0u;
     const uint32_t step0 = (walks != 0u && w >= walks) ? (uint32_t)sorw : 0u;
     st[i] = make_uint4(base + sd.pos, (uint32_t)s, w | (sd.label << 16), step0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_walk_seeds(const T *__restrict__ img_all, TileGeom tg,
+                                                    const bmx_seed *__restrict__ seeds, int64_t nseeds,
+                                                    uint32_t weight0, int overrides, WalkSeed *__restrict__ ws)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseeds) return;
+    const bmx_seed sd = seeds[s];
+    const uint32_t base = tg.vols[vol_of_seed(tg.vols, tg.nvol, (uint32_t)s)].base;
+    uint32_t weight = weight0;
+    if (overrides && (sd.reserved >> 16) != 0u) weight = weight0 * (sd.reserved >> 16);
+    ws[s] = WalkSeed{sd.pos, bmx_ld(img_all + base, (int64_t)sd.pos), 1.0f / (2.0f * sd.var), weight};
 }
 
 __global__ __launch_bounds__(256) void k_seed_walks(bmx_seed *__restrict__ seeds, int64_t n, uint32_t value)
@@ -342,23 +384,20 @@ __device__ __forceinline__ void tile_block(const T *simg, uint32_t *shits, int s
 // rule and hit weights of tile_block, so the hits it adds are the ones the tile
 // walker would add.  w4: its state, written back by the caller.
 template <typename T>
-__device__ __forceinline__ void tiny_walker(const T *__restrict__ img, const WalkGeom &g, uint32_t gbase,
-                                            const bmx_seed *__restrict__ seeds, uint4 &w4, int sorw, int cap,
-                                            uint32_t weight0, int overrides, uint32_t k0, uint32_t k1,
-                                            uint32_t *__restrict__ hg, uint32_t &acc)
+__device__ __forceinline__ void tiny_walker(const T *__restrict__ img, const WalkGeom &g, const VolGeom &gv,
+                                            const WalkSeed *__restrict__ wseeds, uint4 &w4, int sorw, int cap,
+                                            uint32_t k0, uint32_t k1, uint32_t *__restrict__ hg, uint32_t &acc)
 {
+    const uint32_t gbase = gv.base;
     uint32_t pos = w4.x - gbase;
     const uint32_t seed = w4.y, wl = w4.z;
     int step = (int)w4.w;
-    const bmx_seed sd = seeds[seed];
-    const uint32_t p0 = sd.pos;
-    const float B = bmx_ld(img, (int64_t)p0);
-    const float div1 = 1.0f / (2.0f * sd.var);
-    uint32_t weight = weight0;
-    if (overrides && (sd.reserved >> 16) != 0u) weight = weight0 * (sd.reserved >> 16);
-    int z = (int)(pos / g.sz);
+    const WalkSeed ws = wseeds[seed];
+    const uint32_t p0 = ws.pos, weight = ws.weight;
+    const float B = ws.B, div1 = ws.div1;
+    int z = (int)div_u32(pos, gv.mz, gv.shz);
     const uint32_t rr0 = pos - (uint32_t)z * g.sz;
-    int y = (int)(rr0 / g.sy);
+    int y = (int)div_u32(rr0, gv.my, gv.shy);
     int x = (int)(rr0 - (uint32_t)y * g.sy);
     // interior bounds and index strides as scalars (a select over struct fields
     // would be lowered to an indexed scratch load)
@@ -405,12 +444,11 @@ __device__ __forceinline__ void tiny_walker(const T *__restrict__ img, const Wal
 // where LDS already limits a CU to one workgroup.
 template <typename T, int S, bool PERSIST>
 __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
-    const T *__restrict__ img_all, TileGeom tg, const bmx_seed *__restrict__ seeds,
+    const T *__restrict__ img_all, TileGeom tg, const WalkSeed *__restrict__ wseeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
     const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ tkey, const uint32_t *__restrict__ toff,
     const uint32_t *__restrict__ tcnt, RoundCtl *__restrict__ ctl, uint32_t nkeys, int sorw, int cap,
-    uint32_t weight0, int overrides, uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits,
-    unsigned long long *accepted)
+    uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted)
 {
     using Sh = TileShape<T, S>;
     constexpr int C = Sh::C, H = Sh::H, R = C + 2 * H, RI = R + 2, NTH = Sh::NTH;
@@ -493,7 +531,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
         uint32_t *__restrict__ hg = hits + gv.hits_base + (int64_t)label * gv.V;
         uint32_t acc = 0;  // accepted moves of this thread in this bucket (< cnt * cap < 2^32)
         bool have = false;
-        uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = weight0;
+        uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = 0;
         int step = 0, step0 = 0, lz = 0, ly = 0, lx = 0, li = 0;
         float B = 0.0f, div1 = 0.0f;
         while (true) {
@@ -505,15 +543,14 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
                 seed = w4.y;
                 wl = w4.z;
                 step = (int)w4.w;
-                const bmx_seed sd = seeds[seed];
-                p0 = sd.pos;   // volume-local: a seed's Philox stream does not depend on the batch
-                B = bmx_ld(img, (int64_t)p0);
-                div1 = 1.0f / (2.0f * sd.var);
-                weight = weight0;
-                if (overrides && (sd.reserved >> 16) != 0u) weight = weight0 * (sd.reserved >> 16);
-                const int z = (int)(pos / g.sz);
+                const WalkSeed ws = wseeds[seed];
+                p0 = ws.pos;   // volume-local: a seed's Philox stream does not depend on the batch
+                B = ws.B;
+                div1 = ws.div1;
+                weight = ws.weight;
+                const int z = (int)div_u32(pos, gv.mz, gv.shz);
                 const uint32_t r = pos - (uint32_t)z * g.sz;
-                const int y = (int)(r / g.sy);
+                const int y = (int)div_u32(r, gv.my, gv.shy);
                 const int x = (int)(r - (uint32_t)y * g.sy);
                 lz = z - bz;
                 ly = y - by;
@@ -571,8 +608,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
             uint32_t *__restrict__ hg = hits + gv.hits_base + (int64_t)label * gv.V;
             for (uint32_t w = (uint32_t)lane; w < cnt; w += 64u) {
                 uint4 w4 = st[off + w];
-                tiny_walker<T>(img_all + gv.base, g, gv.base, seeds, w4, sorw, cap, weight0, overrides, k0, k1, hg,
-                               tacc);
+                tiny_walker<T>(img_all + gv.base, g, gv, wseeds, w4, sorw, cap, k0, k1, hg, tacc);
                 st[off + w] = w4;
             }
         }
@@ -622,12 +658,12 @@ template <typename T>
 struct RoundArgs {
     const T *img;
     TileGeom tg;
-    const bmx_seed *seeds;
+    const WalkSeed *wseeds;
     uint32_t *counts, *bkey, *boff, *bcnt, *tkey, *toff, *tcnt;
     RoundCtl *ctl;
     int64_t nkeys;
-    int sorw, cap, overrides;
-    uint32_t weight, k0, k1;
+    int sorw, cap;
+    uint32_t k0, k1;
     uint32_t *hits;
     unsigned long long *acc;
 };
@@ -660,9 +696,8 @@ template <typename T, int S, bool PERSIST>
 int round_walk(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *st, unsigned grid)
 {
     k_tile_walk<T, S, PERSIST><<<grid, TileShape<T, S>::NTH, tile_lds_bytes<T, S>(), ctx->stream>>>(
-        a.img, a.tg, a.seeds, st, a.bkey, a.boff, a.bcnt, a.tkey, a.toff, a.tcnt, a.ctl, (uint32_t)a.nkeys, a.sorw,
-        a.cap, a.weight,
-        a.overrides, a.k0, a.k1, a.hits, a.acc);
+        a.img, a.tg, a.wseeds, st, a.bkey, a.boff, a.bcnt, a.tkey, a.toff, a.tcnt, a.ctl, (uint32_t)a.nkeys,
+        a.sorw, a.cap, a.k0, a.k1, a.hits, a.acc);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -756,6 +791,9 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
         g.hits_base = b.hits_offset;
         g.seed_base = (uint32_t)b.seed_offset;
         g.pad = 0;
+        BMX_CHECK(b.X >= 2, "bmx_walk: volume %d has X = %d < 2", v, b.X);
+        magic_u32(g.sz, &g.mz, &g.shz);
+        magic_u32(g.sy, &g.my, &g.shy);
         ntiles += (int64_t)((b.Z + C - 1) / C) * g.nty * g.ntx;
     }
     const int64_t nkeys = ntiles * nlabels;
@@ -765,6 +803,8 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     BMX_HIP(hipMemcpyAsync(ctx->d_vols, vg.data(), (size_t)nvol * sizeof(VolGeom), hipMemcpyHostToDevice,
                            ctx->stream));
     rc = grow(ctx, &ctx->d_state, &ctx->state_cap, (size_t)n0 * 2 * sizeof(uint4));
+    if (rc) return rc;
+    rc = grow(ctx, &ctx->d_wseeds, &ctx->wseeds_cap, (size_t)(nseeds > 0 ? nseeds : 1) * sizeof(WalkSeed));
     if (rc) return rc;
     rc = grow(ctx, &ctx->d_keys, &ctx->keys_cap, (size_t)(nkeys + 1) * sizeof(uint32_t));
     if (rc) return rc;
@@ -781,7 +821,7 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     RoundArgs<T> a;
     a.img = img;
     a.tg = TileGeom{(const VolGeom *)ctx->d_vols, nvol, (uint32_t)nlabels};
-    a.seeds = seeds;
+    a.wseeds = (const WalkSeed *)ctx->d_wseeds;
     a.counts = (uint32_t *)ctx->d_keys;
     a.bkey = (uint32_t *)ctx->d_buckets;
     a.boff = a.bkey + nkeys;
@@ -794,13 +834,16 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     a.nkeys = nkeys;
     a.sorw = sorw;
     a.cap = cap;
-    a.overrides = overrides;
-    a.weight = weight;
     a.k0 = k0;
     a.k1 = k1;
     a.hits = hits;
     a.acc = acc;
     uint4 *A = (uint4 *)ctx->d_state, *Bst = A + n0;
+    if (nseeds > 0) {
+        k_walk_seeds<T><<<bmx_blocks(nseeds, 256), 256, 0, ctx->stream>>>(img, a.tg, seeds, nseeds, weight, overrides,
+                                                                         (WalkSeed *)ctx->d_wseeds);
+        BMX_LAUNCH_CHECK();
+    }
     k_tile_init<<<bmx_blocks(n0, 256), 256, 0, ctx->stream>>>(seeds, nseeds, nbrw, sorw, overrides, a.tg, A);
     BMX_LAUNCH_CHECK();
     k_set_count<<<1, 64, 0, ctx->stream>>>(a.ctl, (uint32_t)n0);
