@@ -172,7 +172,7 @@ def main():
                     help="skip the drop-in calls with NumPy arrays in and out (H2D and D2H included)")
     ap.add_argument("--no-reference-stream", action="store_true",
                     help="skip the Dice of the Philox walker against the reference stream at this size")
-    ap.add_argument("--workload", choices=["512", "batch", "large", "post"], default="512")
+    ap.add_argument("--workload", choices=["512", "batch", "large", "post", "net"], default="512")
     ap.add_argument("--batch-n", type=int, default=64)
     ap.add_argument("--calibrate-valu", action="store_true",
                     help="also launch bmx_valu_calibrate once (256 Ki threads x 2000 x 128 float32 FMAs), the "
@@ -191,6 +191,8 @@ def main():
         return bench_large(args)
     if args.workload == "post":
         return bench_post(args)
+    if args.workload == "net":
+        return bench_net(args)
 
     from biomedisa_amd import _hip, surrogate
     from biomedisa_amd.interpolation import (GUIDED_ROBUST, DeviceArray, smart_interpolation,
@@ -554,6 +556,81 @@ def bench_post(args):
                             "cores": int(orc.lib().orc_num_threads()), "kind": "port",
                             "sample": "the oracle's C/NumPy restatements of clean, fill, acwe and refined "
                                       "on the 256^3 corner of the same volumes"}
+    print(json.dumps(line), flush=True)
+
+
+def bench_net(args):
+    """The learned proposal's network on its own: the forward over the 512^3
+    benchmark's proposal grid (the cropped 514^3 volume padded to 520^3, one
+    slab, 32 input channels of random bf16 values), and training steps (forward,
+    softmax cross-entropy, backward, Adam) at batch 8 x 72^3, each timed after a
+    warm-up.  FLOPs count the 3x3x3 convolutions and the head (Net.flops_forward);
+    a training step does the forward, the weight gradient of every convolution
+    and the input gradient of every convolution but the first, i.e. about 3x."""
+    from biomedisa_amd import _hip, surrogate
+    eng = _hip.engine(0)
+    net = surrogate.net(0)
+    rng = np.random.default_rng(0)
+    peak = 2500.0   # TFLOP/s, dense bf16 MFMA of one MI355X
+
+    def rand_bf16(n):
+        return (rng.standard_normal(n, dtype=np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+    g = (520, 520, 520)
+    x = eng.upload(rand_bf16(int(np.prod(g)) * surrogate.CIN))
+    net.forward(x, 1, *g)
+    eng.sync()
+    fwd = []
+    for _ in range(max(args.steps, 3)):
+        t = time.perf_counter()
+        net.forward(x, 1, *g)
+        eng.sync()
+        fwd.append(time.perf_counter() - t)
+    fwd_s = min(fwd)
+    f_fwd = net.flops_forward(1, *g)
+    log("forward %s: %.2f ms, %.1f TFLOP/s" % (g, 1e3 * fwd_s, f_fwd / fwd_s / 1e12))
+    del x
+    B, tg = 8, (72, 72, 72)
+    ng = int(np.prod(tg))
+    tnet = surrogate.Net(eng, seed=1)
+    xb = eng.upload(rand_bf16(B * ng * surrogate.CIN))
+    ib = eng.upload(rng.integers(0, surrogate.L_MAX, size=B * ng).astype(np.uint8))
+    mb = eng.upload(np.ones(B * ng, np.uint8))
+    lossb = eng.alloc(4)
+
+    def train_step():
+        tnet.zero_grad()
+        lossb.zero()
+        tnet.loss_and_grad(xb, None, mb, B, *tg, lossb, target_idx=ib)
+        tnet.adam(1e-3)
+
+    for _ in range(2):
+        train_step()
+    eng.sync()
+    it = max(args.steps, 10)
+    t = time.perf_counter()
+    for _ in range(it):
+        train_step()
+    eng.sync()
+    tr_s = (time.perf_counter() - t) / it
+    v0 = B * ng
+    conv = sum(2.0 * 27 * ci * co * v0 / {"enc0": 1, "enc1": 8, "enc2": 64, "bott": 512, "dec2": 64,
+                                           "dec1": 8, "dec0": 1}[nm] for nm, ci, co in surrogate.LAYERS)
+    dgrad = conv - 2.0 * 27 * 32 * 16 * v0
+    f_tr = tnet.flops_forward(B, *tg) + conv + dgrad + 2 * 2.0 * 16 * surrogate.L_MAX * v0
+    log("training step %d x %s: %.2f ms, %.1f TFLOP/s" % (B, tg, 1e3 * tr_s, f_tr / tr_s / 1e12))
+    tf = f_fwd / fwd_s / 1e12
+    line = {"metric": "surrogate network forward TFLOP/s on the 512^3 proposal grid",
+            "value": tf, "unit": "TFLOP/s", "n_gpus": 1, "steps": len(fwd), "warmup": 1,
+            "ms_per_step": 1e3 * fwd_s, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16", "data": "synthetic",
+            "config": {"workload": "3-D conv encoder-decoder (16/32/64/128 channels, 32 inputs, 30 label "
+                                   "slots), forward on 1 x 520^3; training at batch 8 x 72^3",
+                       "forward_flop": f_fwd, "forward_seconds": fwd_s, "forward_frac_of_bf16_peak": tf / peak,
+                       "train_step_flop": f_tr, "train_step_seconds": tr_s,
+                       "train_tflops": f_tr / tr_s / 1e12, "train_frac_of_bf16_peak": f_tr / tr_s / 1e12 / peak},
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
+                         "traffic": None}}
     print(json.dumps(line), flush=True)
 
 

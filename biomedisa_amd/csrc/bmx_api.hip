@@ -103,6 +103,7 @@ int bmx_close(bmx_ctx *ctx)
     if (ctx->d_scan) BMX_HIP(hipFree(ctx->d_scan));
     if (ctx->d_vols) BMX_HIP(hipFree(ctx->d_vols));
     if (ctx->d_wseeds) BMX_HIP(hipFree(ctx->d_wseeds));
+    if (ctx->d_netpart) BMX_HIP(hipFree(ctx->d_netpart));
     if (ctx->seg_ev) {
         for (int k = 0; k < 2 * 1024; ++k) BMX_HIP(hipEventDestroy(ctx->seg_ev[k]));
         free(ctx->seg_ev);

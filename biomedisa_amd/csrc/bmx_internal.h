@@ -37,6 +37,8 @@ struct bmx_ctx {
     size_t vols_cap;
     void *d_wseeds;          // tiled walker: per-seed constants of the walk
     size_t wseeds_cap;
+    void *d_netpart;         // conv3d: float32 partial sums of a two-launch convolution
+    size_t netpart_cap;
     hipEvent_t *seg_ev;      // sorted walker: event pairs around each walk launch
     float last_walk_kernel_ms;
     int last_walk_launches;
@@ -70,6 +72,22 @@ int bmx_set_error(int code, const char *fmt, ...);
 #define BMX_LAUNCH_CHECK() BMX_HIP(hipGetLastError())
 
 int bmx_set_device(bmx_ctx *ctx);
+
+// Grows a context buffer to at least `bytes` (contents not kept).
+template <typename P>
+int grow(bmx_ctx *ctx, P **ptr, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return BMX_OK;
+    if (*ptr) {
+        BMX_HIP(hipStreamSynchronize(ctx->stream));
+        BMX_HIP(hipFree(*ptr));
+    }
+    *ptr = nullptr;
+    *cap = 0;
+    BMX_HIP(hipMalloc((void **)ptr, bytes));
+    *cap = bytes;
+    return BMX_OK;
+}
 
 // Device address of the static scratch buffer (BMX_NOALLOC_SCRATCH_BYTES) on
 // `device`, resolved once per device by bmx_open; nullptr before that.  Used

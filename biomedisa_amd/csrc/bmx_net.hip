@@ -9,12 +9,14 @@
 // are bf16 [Co][27][Ci] (tap = (kz*3 + ky)*3 + kx, offsets k-1), with an FP32
 // master copy kept by the optimiser; gradients of weights are FP32.
 //
-// MFMA tiling (v_mfma_f32_16x16x16bf16_1k): lane l supplies A[l%16][4(l/16)..+3]
-// and B[4(l/16)..+3][l%16] and receives D[4(l/16)+i][l%16], i = 0..3.  A wave
-// computes 16*RB voxels (RB 16-voxel row blocks along x; RB = 2 in the
-// gradient kernels, 4 or 8 in the forward) by 16*NB channels; the K loop runs
-// over the 27 taps and 16-channel chunks of the input.
+// The forward and the input gradient run on k_conv_mm (gfx950's
+// v_mfma_f32_16x16x32_bf16, planes streamed through LDS, weights in
+// registers; described at the kernel).  The weight gradient runs on
+// v_mfma_f32_16x16x16bf16_1k: lane l supplies A[l%16][4(l/16)..+3] and
+// B[4(l/16)..+3][l%16] and receives D[4(l/16)+i][l%16], i = 0..3.
 #include "bmx_internal.h"
+
+#include <algorithm>
 
 namespace {
 
@@ -30,17 +32,6 @@ __device__ __forceinline__ uint16_t f2bf(float f)
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
-__device__ __forceinline__ short4_t ld4(const uint16_t *p)
-{
-    const uint2 v = *(const uint2 *)p;
-    short4_t r;
-    r[0] = (short)(v.x & 0xFFFF);
-    r[1] = (short)(v.x >> 16);
-    r[2] = (short)(v.y & 0xFFFF);
-    r[3] = (short)(v.y >> 16);
-    return r;
-}
-
 __device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -50,253 +41,233 @@ struct Shape {
     int N, D, H, W;
 };
 
-// Forward: y = act(conv(x, w) + b).  grid.x: voxel groups of 64*RB (4 waves x
-// 16*RB), grid.y: output-channel groups of 16*NB.  Each lane decodes its own
-// voxel's coordinates, so any grid shape works; requires Ci % 16 == 0 and
-// Co % (16*NB) == 0.  With a second source xb (Cb channels on the half-size
-// grid), the input is concat(x, up2(xb)) with Ci = Ca + Cb, read in place: a
-// 16-channel chunk comes from x below Ca and from xb at the voxel's parent
-// (z/2, y/2, x/2) above, so neither the upsampled tensor nor the concatenation
-// is materialised.
-template <int NB, bool CAT, int RB = 2>
-__global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ x, Shape s, int Ci,
-                                                  const uint16_t *__restrict__ w,
-                                                  const float *__restrict__ bias,
-                                                  uint16_t *__restrict__ y, int Co, int relu,
-                                                  const uint16_t *__restrict__ xb, int Ca)
+// ---------------------------------------------------------------------------
+// The forward on the gfx950 MFMA v_mfma_f32_16x16x32_bf16 (K = 32 per
+// instruction): lane l supplies A[l%16][8(l/16)..+7] and B[8(l/16)..+7][l%16]
+// and receives D[4(l/16)+i][l%16], i = 0..3.
+//
+// A block (4 waves) computes 16 output channels of a 32 x 8 (x, y) tile of
+// output rows over a chunk of kMmZC output planes of one volume, and streams
+// the input planes z' = zb-1 .. ze through two LDS buffers: the next plane is
+// loaded straight from global memory into one while the other is used.  A
+// plane holds the tile plus a 1-voxel halo, channels padded with zeros to CP
+// = 32 CH, in 16-byte slots: CP/8 per voxel and one unused slot, so a voxel's
+// pitch is CP + 8 bf16 and 16 consecutive voxels at one channel offset fall in
+// distinct banks.  Each wave keeps the 27 x CH B fragments of the block's
+// output channels in registers, so no other vector-memory load runs during
+// the compute and nothing waits on the plane in flight.  Wave w owns output
+// rows y0 + 2w, y0 + 2w + 1: four row blocks of 16 voxels.  For every (dy, dx)
+// tap and 32-channel chunk of plane z' it reads each row block's A fragment
+// once and applies it to the three output planes z'+1, z', z'-1 (taps dz =
+// -1, 0, +1), whose accumulators it keeps: plane z'-1 is complete after z' and
+// is written out.  K runs in the order (dz, dy, dx, chunk) for every output
+// channel, whatever the channel count, the tile or a concatenated input, so
+// results do not depend on them.  CAT: the input is concat(x, up2(xb)) with Ca
+// channels from x, read in place.  MASK: the epilogue zeroes outputs where
+// xact <= 0 (the input gradient through a ReLU) instead of adding a bias.
+// SPLIT: a launch convolves input channels coff .. coff+Ci-1 of Cf, and wider
+// inputs take several launches of at most 64 channels: SPLIT = 1 writes its
+// float32 sums to `part`, 2 adds them to `part`, 3 adds `part` before the
+// epilogue (0: a single launch).
+constexpr int kMmTX = 32, kMmTY = 8, kMmZC = 32, kMmRB = 4;
+constexpr int kMmPX = kMmTX + 2, kMmPlane = kMmPX * (kMmTY + 2);
+
+typedef short short8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ __forceinline__ float4_t mfma32(short8_t a, short8_t b, float4_t c)
 {
-    const int Cb = Ci - Ca;
-    const int Dh = s.D / 2, Hh = s.H / 2, Wh = s.W / 2;
-    const int lane = threadIdx.x & 63;
-    const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
-    const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * RB);
-    if (v0 >= nvox) return;
-    const int co0 = blockIdx.y * 16 * NB;
-    const int r = lane & 15, kq = (lane >> 4) * 4;
-    int cz[RB], cy[RB], cx[RB], cn[RB];
-    bool vin[RB];
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-        int64_t v = v0 + rb * 16 + r;
-        vin[rb] = v < nvox;
-        cx[rb] = (int)(v % s.W); v /= s.W;
-        cy[rb] = (int)(v % s.H); v /= s.H;
-        cz[rb] = (int)(v % s.D);
-        cn[rb] = (int)(v / s.D);
-    }
-    float4_t acc[RB][NB];
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[rb][nb] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 27; ++tap) {
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const uint16_t *src[RB], *srcb[RB];
-        bool ok[RB];
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-            const int z = cz[rb] + dz, yy = cy[rb] + dy, xx = cx[rb] + dx;
-            ok[rb] = vin[rb] && (unsigned)z < (unsigned)s.D && (unsigned)yy < (unsigned)s.H &&
-                     (unsigned)xx < (unsigned)s.W;
-            const int zc = ok[rb] ? z : 0, yc = ok[rb] ? yy : 0, xc = ok[rb] ? xx : 0;
-            const int64_t nv = (((int64_t)cn[rb] * s.D + zc) * s.H + yc) * s.W + xc;
-            src[rb] = x + nv * (CAT ? Ca : Ci) + kq;
-            if (CAT) {
-                const int64_t nh = (((int64_t)cn[rb] * Dh + zc / 2) * Hh + yc / 2) * Wh + xc / 2;
-                srcb[rb] = xb + nh * Cb + kq;
-            }
-        }
-        const uint16_t *wt = w + ((int64_t)(co0 + r) * 27 + tap) * Ci + kq;
-        for (int c0 = 0; c0 < Ci; c0 += 16) {
-            short4_t a[RB];
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                a[rb] = (CAT && c0 >= Ca) ? ld4(srcb[rb] + (c0 - Ca)) : ld4(src[rb] + c0);
-                if (!ok[rb]) a[rb] = short4_t{0, 0, 0, 0};
-            }
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const short4_t b = ld4(wt + (int64_t)nb * 16 * 27 * Ci + c0);
-#pragma unroll
-                for (int rb = 0; rb < RB; ++rb) acc[rb][nb] = mfma16(a[rb], b, acc[rb][nb]);
-            }
-        }
-    }
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int co = co0 + nb * 16 + r;
-            const float bb = bias ? bias[co] : 0.0f;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t v = v0 + rb * 16 + kq + i;
-                if (v >= nvox) continue;
-                float val = acc[rb][nb][i] + bb;
-                if (relu) val = fmaxf(val, 0.0f);
-                y[v * Co + co] = f2bf(val);
-            }
-        }
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
 }
 
-// Forward for 16 output channels from a plain input of at most 48 channels
-// (the first encoder layer) with the input staged in LDS: a block owns a 32 (x)
-// by 4 (y) tile of output rows and marches along ZC planes of z; the input
-// planes z-1, z, z+1 of the tile plus a 1-voxel halo rotate through three LDS
-// slots, so each input voxel is read from memory about 1.6 times instead of
-// once per tap.  Wave w computes the 32 voxels of row y0 + w with the same MFMA
-// sequence as k_conv_fwd<1, false> (taps outer, 16-channel chunks inner), so
-// the results are bit-identical to it.
-constexpr int kZmTX = 32, kZmTY = 4, kZmZC = 16;
-constexpr int kZmPX = kZmTX + 2, kZmPY = kZmTY + 2, kZmPlane = kZmPX * kZmPY;
-constexpr int kZmMaxCi = 48;
-
-__host__ __device__ constexpr int zm_pitch(int Ci) { return Ci + 4; }   // bf16 per LDS voxel (bank spread)
-
-__global__ __launch_bounds__(256) void k_conv_fwd_zm(const uint16_t *__restrict__ x, Shape s, int Ci,
-                                                     const uint16_t *__restrict__ w,
-                                                     const float *__restrict__ bias,
-                                                     uint16_t *__restrict__ y, int relu)
+__host__ __device__ constexpr int mm_pitch(int ch) { return 32 * ch + 8; }
+__host__ __device__ constexpr size_t mm_lds_bytes(int ch)
 {
-    extern __shared__ __align__(16) uint16_t zm_lds[];
-    const int P = zm_pitch(Ci);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int r = lane & 15, kq = (lane >> 4) * 4;
-    const int x0 = blockIdx.x * kZmTX, y0 = blockIdx.y * kZmTY;
-    const int nzc = (s.D + kZmZC - 1) / kZmZC;
-    const int n = blockIdx.z / nzc;
-    const int zb = (blockIdx.z % nzc) * kZmZC, ze = min(s.D, zb + kZmZC);
-    const int groups = Ci / 4;                       // 8-byte channel groups per voxel
+    return (size_t)2 * kMmPlane * mm_pitch(ch) * sizeof(uint16_t);
+}
 
-    // input plane z into LDS slot (z + 3) % 3: zeros outside the grid
-    auto load_plane = [&](int z) {
-        uint16_t *dst = zm_lds + ((z + 3) % 3) * kZmPlane * P;
-        for (int u = tid; u < kZmPlane * groups; u += 256) {
-            const int p = u / groups, c = (u - p * groups) * 4;
-            const int ly = p / kZmPX, lx = p - ly * kZmPX;
-            const int yy = y0 - 1 + ly, xx = x0 - 1 + lx;
-            uint2 v = make_uint2(0u, 0u);
-            if ((unsigned)z < (unsigned)s.D && (unsigned)yy < (unsigned)s.H && (unsigned)xx < (unsigned)s.W) {
-                const int64_t vox = (((int64_t)n * s.D + z) * s.H + yy) * s.W + xx;
-                v = *(const uint2 *)(x + vox * Ci + c);
+template <int CH, bool CAT, bool MASK, int SPLIT>
+__global__ __launch_bounds__(256) void k_conv_mm(const uint16_t *__restrict__ x, Shape s, int Ci,
+                                                 const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                                 uint16_t *__restrict__ y, int Co, int relu,
+                                                 const uint16_t *__restrict__ xb, int Ca,
+                                                 const uint16_t *__restrict__ xact, int Cf, int coff,
+                                                 float *__restrict__ part)
+{
+    constexpr int CP = 32 * CH, CPG = CP / 8, SPV = CPG + 1, P = mm_pitch(CH);
+    constexpr int SLOTS = kMmPlane * SPV, NIT = (SLOTS + 255) / 256;
+    constexpr int NSTEP = 9 * CH;   // (dy, dx, chunk) steps of a plane
+    extern __shared__ __align__(16) uint16_t mm_lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = lane & 15, kg = lane >> 4;
+    const int x0 = blockIdx.x * kMmTX, y0 = blockIdx.y * kMmTY;
+    const int nzc = (s.D + kMmZC - 1) / kMmZC;
+    const int ncob = Co / 16;
+    const int co0 = (int)(blockIdx.z % (unsigned)ncob) * 16, zn = (int)(blockIdx.z / (unsigned)ncob);
+    const int n = zn / nzc;
+    const int zb = (zn % nzc) * kMmZC, ze = min(s.D, zb + kMmZC);
+    // this launch's Ci input channels are channels coff .. coff + Ci - 1 of the Cf of the input
+    const int Cx = CAT ? Ca : Cf, Cb = Cf - Ca;
+    const int Dh = s.D / 2, Hh = s.H / 2, Wh = s.W / 2;
+
+    // The slots this lane loads (slot wv*64 + it*256 + lane of every plane, so
+    // that a wave instruction fills 64 consecutive slots) as element offsets
+    // within a plane of x (>= 0), of xb (encoded -2 - offset) or -1 for a slot
+    // no load writes (padded channels, voxels off the grid in y or x, the
+    // unused slot of a voxel).
+    int soff[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int u = wv * 64 + it * 256 + lane;
+        const int p = u / SPV, q = u - p * SPV, c = 8 * q, cf = coff + c;
+        const int ly = p / kMmPX, lx = p - ly * kMmPX;
+        const int yy = y0 - 1 + ly, xx = x0 - 1 + lx;
+        int o = -1;
+        if (u < SLOTS && q < CPG && c < Ci && (unsigned)yy < (unsigned)s.H && (unsigned)xx < (unsigned)s.W) {
+            if (!CAT || cf < Ca) o = (yy * s.W + xx) * Cx + cf;
+            else o = -2 - (((yy / 2) * Wh + xx / 2) * Cb + (cf - Ca));
+        }
+        soff[it] = o;
+    }
+    // output element offsets within a plane of y (-1: off the grid)
+    int yoff[kMmRB][4];
+#pragma unroll
+    for (int rb = 0; rb < kMmRB; ++rb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int yrow = y0 + 2 * wv + rb / 2, xx = x0 + 16 * (rb % 2) + 4 * kg + i;
+            yoff[rb][i] = (yrow < s.H && xx < s.W) ? (yrow * s.W + xx) * Co + co0 + r : -1;
+        }
+
+    short8_t bw[27][CH];
+#pragma unroll
+    for (int t = 0; t < 27; ++t)
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int k0 = 32 * c + 8 * kg;
+            bw[t][c] = k0 < Ci ? *(const short8_t *)(w + ((int64_t)(co0 + r) * 27 + t) * Cf + coff + k0)
+                               : short8_t{0, 0, 0, 0, 0, 0, 0, 0};
+        }
+
+    // plane zp into LDS buffer `buf`; a plane off the grid is cleared
+    auto load_plane = [&](int zp, int buf) {
+        uint16_t *dst = mm_lds + buf * kMmPlane * P;
+        if ((unsigned)zp >= (unsigned)s.D) {
+            for (int u = tid; u < SLOTS; u += 256) *(uint4 *)(dst + u * 8) = make_uint4(0u, 0u, 0u, 0u);
+            return;
+        }
+        const uint16_t *xp = x + (((int64_t)n * s.D + zp) * s.H) * s.W * Cx;
+        const uint16_t *bp = CAT ? xb + (((int64_t)n * Dh + zp / 2) * Hh) * Wh * Cb : nullptr;
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int o = soff[it];
+            const int s0 = wv * 64 + it * 256;
+            if (o != -1)
+                __builtin_amdgcn_global_load_lds((CAT && o < -1) ? bp + (-2 - o) : xp + o,
+                                                 (lds_void_t *)(dst + s0 * 8), 16, 0, 0);
+        }
+    };
+    // slots no load writes are zero in both buffers
+    for (int u = tid; u < 2 * SLOTS; u += 256) {
+        const int uu = u % SLOTS;
+        const int p = uu / SPV, q = uu - p * SPV;
+        const int ly = p / kMmPX, lx = p - ly * kMmPX;
+        const int yy = y0 - 1 + ly, xx = x0 - 1 + lx;
+        if (q < CPG && (8 * q >= Ci || (unsigned)yy >= (unsigned)s.H || (unsigned)xx >= (unsigned)s.W))
+            *(uint4 *)(mm_lds + u * 8) = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+
+    float4_t acc[3][kMmRB];   // output planes z'-1, z', z'+1
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int rb = 0; rb < kMmRB; ++rb) acc[q][rb] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+    const float bb = (!MASK && (SPLIT == 0 || SPLIT == 3) && bias) ? bias[co0 + r] : 0.0f;
+    // the partial sums and ReLU outputs the epilogue of output plane z reads,
+    // fetched before the plane's MFMAs so that their latency is hidden
+    float pv[kMmRB][4];
+    uint16_t mv[kMmRB][4];
+    auto prefetch = [&](int z) {
+        const int64_t ybase = (((int64_t)n * s.D + z) * s.H) * s.W * Co;
+#pragma unroll
+        for (int rb = 0; rb < kMmRB; ++rb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t v = ybase + (yoff[rb][i] < 0 ? 0 : yoff[rb][i]);
+                if (SPLIT >= 2) pv[rb][i] = yoff[rb][i] < 0 ? 0.0f : part[v];
+                if (MASK) mv[rb][i] = (yoff[rb][i] < 0 || !xact) ? (uint16_t)0x3F80u : xact[v];
             }
-            *(uint2 *)(dst + p * P + c) = v;
+    };
+    auto store = [&](int z) {
+        const int64_t ybase = (((int64_t)n * s.D + z) * s.H) * s.W * Co;
+#pragma unroll
+        for (int rb = 0; rb < kMmRB; ++rb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (yoff[rb][i] < 0) continue;
+                const int64_t v = ybase + yoff[rb][i];
+                float val = acc[0][rb][i];
+                if (SPLIT == 1 || SPLIT == 2) {
+                    part[v] = SPLIT == 1 ? val : pv[rb][i] + val;
+                    continue;
+                }
+                if (SPLIT == 3) val += pv[rb][i];
+                if (MASK) {
+                    if (!(bf2f(mv[rb][i]) > 0.0f)) val = 0.0f;
+                } else {
+                    val += bb;
+                    if (relu) val = fmaxf(val, 0.0f);
+                }
+                y[v] = f2bf(val);
+            }
+    };
+
+    // A fragments of step k (dy, dx, chunk) of the current plane
+    auto read_a = [&](const uint16_t *pl, int k, short8_t *a) {
+        const int dy = k / (3 * CH), dx = (k / CH) % 3, c = k % CH;
+#pragma unroll
+        for (int rb = 0; rb < kMmRB; ++rb) {
+            const int row = 2 * wv + rb / 2, col = 16 * (rb % 2) + r;
+            a[rb] = *(const short8_t *)(pl + ((row + dy) * kMmPX + col + dx) * P + 32 * c + 8 * kg);
         }
     };
 
-    const int yrow = y0 + wv;
-    const float bb = bias ? bias[r] : 0.0f;
-    load_plane(zb - 1);
-    load_plane(zb);
-    for (int z = zb; z < ze; ++z) {
-        load_plane(z + 1);
-        __syncthreads();
-        float4_t acc[2] = {float4_t{0.f, 0.f, 0.f, 0.f}, float4_t{0.f, 0.f, 0.f, 0.f}};
-        for (int tap = 0; tap < 27; ++tap) {
-            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            const uint16_t *pl = zm_lds + ((z + dz + 3) % 3) * kZmPlane * P;
-            const uint16_t *a0 = pl + ((wv + 1 + dy) * kZmPX + (1 + r + dx)) * P + kq;
-            const uint16_t *a1 = a0 + 16 * P;
-            const uint16_t *wt = w + ((int64_t)r * 27 + tap) * Ci + kq;
-            for (int c0 = 0; c0 < Ci; c0 += 16) {
-                const short4_t av0 = ld4(a0 + c0), av1 = ld4(a1 + c0);
-                const short4_t b = ld4(wt + c0);
-                acc[0] = mfma16(av0, b, acc[0]);
-                acc[1] = mfma16(av1, b, acc[1]);
+    load_plane(zb - 1, 0);
+    __syncthreads();
+    int buf = 0;
+    for (int zp = zb - 1; zp <= ze; ++zp) {
+        if (zp + 1 <= ze) load_plane(zp + 1, buf ^ 1);
+        if ((SPLIT >= 2 || MASK) && zp - 1 >= zb) prefetch(zp - 1);
+        const uint16_t *pl = mm_lds + buf * kMmPlane * P;
+        short8_t a[2][kMmRB];
+        read_a(pl, 0, a[0]);
+#pragma unroll
+        for (int k = 0; k < NSTEP; ++k) {
+            if (k + 1 < NSTEP) read_a(pl, k + 1, a[(k + 1) & 1]);
+            const int dy = k / (3 * CH), dx = (k / CH) % 3, c = k % CH;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                // slot q holds output plane z' - 1 + q, reached through tap dz = 1 - q (the
+                // first and last planes of a chunk also feed planes outside it, which are dropped)
+                const int t = (2 - q) * 9 + dy * 3 + dx;
+#pragma unroll
+                for (int rb = 0; rb < kMmRB; ++rb) acc[q][rb] = mfma32(a[k & 1][rb], bw[t][c], acc[q][rb]);
             }
         }
-        if (yrow < s.H) {
+        if (zp - 1 >= zb) store(zp - 1);
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int xx = x0 + rb * 16 + kq + i;
-                    if (xx >= s.W) continue;
-                    float val = acc[rb][i] + bb;
-                    if (relu) val = fmaxf(val, 0.0f);
-                    const int64_t v = (((int64_t)n * s.D + z) * s.H + yrow) * s.W + xx;
-                    y[v * 16 + r] = f2bf(val);
-                }
+        for (int rb = 0; rb < kMmRB; ++rb) {
+            acc[0][rb] = acc[1][rb];
+            acc[1][rb] = acc[2][rb];
+            acc[2][rb] = float4_t{0.f, 0.f, 0.f, 0.f};
         }
-        __syncthreads();
+        __syncthreads();   // the next plane is in, and every wave is done with this one
+        buf ^= 1;
     }
-}
-
-// Input gradient: dx[v][ci] = sum_tap sum_co dy[v - off(tap)][co] * w[co][tap][ci],
-// times (xact[v][ci] > 0) when xact is given (ReLU of the layer below).
-// wt: weights re-laid out as [Ci][27][Co].
-template <int NB>
-__global__ __launch_bounds__(256) void k_conv_dgrad(const uint16_t *__restrict__ dy, Shape s, int Co,
-                                                    const uint16_t *__restrict__ wt,
-                                                    const uint16_t *__restrict__ xact,
-                                                    uint16_t *__restrict__ dx, int Ci)
-{
-    const int lane = threadIdx.x & 63;
-    const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
-    const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
-    if (v0 >= nvox) return;
-    const int ci0 = blockIdx.y * 16 * NB;
-    const int r = lane & 15, kq = (lane >> 4) * 4;
-    int cz[2], cy[2], cx[2], cn[2];
-    bool vin[2];
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
-        int64_t v = v0 + rb * 16 + r;
-        vin[rb] = v < nvox;
-        cx[rb] = (int)(v % s.W); v /= s.W;
-        cy[rb] = (int)(v % s.H); v /= s.H;
-        cz[rb] = (int)(v % s.D);
-        cn[rb] = (int)(v / s.D);
-    }
-    float4_t acc[2][NB];
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[rb][nb] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 27; ++tap) {
-        // output voxel u = v - off contributes through tap
-        const int dz = 1 - tap / 9, dyo = 1 - (tap / 3) % 3, dxo = 1 - tap % 3;
-        const uint16_t *src[2];
-        bool ok[2];
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-            const int z = cz[rb] + dz, yy = cy[rb] + dyo, xx = cx[rb] + dxo;
-            ok[rb] = vin[rb] && (unsigned)z < (unsigned)s.D && (unsigned)yy < (unsigned)s.H &&
-                     (unsigned)xx < (unsigned)s.W;
-            const int64_t nv = (((int64_t)cn[rb] * s.D + (ok[rb] ? z : 0)) * s.H + (ok[rb] ? yy : 0)) * s.W +
-                               (ok[rb] ? xx : 0);
-            src[rb] = dy + (ok[rb] ? nv : 0) * Co + kq;
-        }
-        const uint16_t *wr = wt + ((int64_t)(ci0 + r) * 27 + tap) * Co + kq;
-        for (int c0 = 0; c0 < Co; c0 += 16) {
-            short4_t a[2];
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-                a[rb] = ld4(src[rb] + c0);
-                if (!ok[rb]) a[rb] = short4_t{0, 0, 0, 0};
-            }
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const short4_t b = ld4(wr + (int64_t)nb * 16 * 27 * Co + c0);
-#pragma unroll
-                for (int rb = 0; rb < 2; ++rb) acc[rb][nb] = mfma16(a[rb], b, acc[rb][nb]);
-            }
-        }
-    }
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int ci = ci0 + nb * 16 + r;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t v = v0 + rb * 16 + kq + i;
-                if (v >= nvox) continue;
-                float val = acc[rb][nb][i];
-                if (xact && !(bf2f(xact[v * Ci + ci]) > 0.0f)) val = 0.0f;
-                dx[v * Ci + ci] = f2bf(val);
-            }
-        }
 }
 
 // Weight gradient: dw[co][tap][ci] += sum_v dy[v][co] * x[v + off(tap)][ci];
@@ -362,6 +333,9 @@ __global__ void k_relu_mask(uint16_t *__restrict__ g, const uint16_t *__restrict
 }
 
 // [Co][27][Ci] -> [Ci][27][Co]
+// wt[ci][26 - tap][co] = w[co][tap][ci]: the input gradient dx[v][ci] =
+// sum_tap sum_co dy[v - off(tap)][co] w[co][tap][ci] is the forward
+// convolution of dy with wt (off(26 - tap) = -off(tap)).
 __global__ void k_transpose_w(const uint16_t *__restrict__ w, int Co, int Ci, uint16_t *__restrict__ wt)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -369,27 +343,40 @@ __global__ void k_transpose_w(const uint16_t *__restrict__ w, int Co, int Ci, ui
     const int ci = (int)(i % Ci);
     const int tap = (int)((i / Ci) % 27);
     const int co = (int)(i / ((int64_t)Ci * 27));
-    wt[((int64_t)ci * 27 + tap) * Co + co] = w[i];
+    wt[((int64_t)ci * 27 + (26 - tap)) * Co + co] = w[i];
 }
 
 // 2x2x2 average pooling (channels-last), and its backward.
+// One work-item per output voxel and 8-channel group (16-byte loads and stores;
+// C is a multiple of 16); each channel sums its 8 inputs in (z, y, x) order.
 __global__ void k_pool2(const uint16_t *__restrict__ x, Shape s, int C, uint16_t *__restrict__ y)
 {
-    const int Do = s.D / 2, Ho = s.H / 2, Wo = s.W / 2;
-    const int64_t n_out = (int64_t)s.N * Do * Ho * Wo * C;
+    const int Do = s.D / 2, Ho = s.H / 2, Wo = s.W / 2, G = C / 8;
+    const int64_t n_out = (int64_t)s.N * Do * Ho * Wo * G;
     for (int64_t i = bmx_gtid(); i < n_out; i += bmx_gstride()) {
-        const int c = (int)(i % C);
-        int64_t t = i / C;
+        const int g = (int)(i % G);
+        int64_t t = i / G;
         const int xo = (int)(t % Wo); t /= Wo;
         const int yo = (int)(t % Ho); t /= Ho;
         const int zo = (int)(t % Do);
         const int n = (int)(t / Do);
-        float acc = 0.0f;
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < 8; ++k) {
             const int z = 2 * zo + (k >> 2), yy = 2 * yo + ((k >> 1) & 1), xx = 2 * xo + (k & 1);
-            acc += bf2f(x[((((int64_t)n * s.D + z) * s.H + yy) * s.W + xx) * C + c]);
+            const uint4 v = *(const uint4 *)(x + ((((int64_t)n * s.D + z) * s.H + yy) * s.W + xx) * C + 8 * g);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[2 * j] += bf2f((uint16_t)(w4[j] & 0xFFFFu));
+                acc[2 * j + 1] += bf2f((uint16_t)(w4[j] >> 16));
+            }
         }
-        y[i] = f2bf(acc * 0.125f);
+        uint4 o;
+        o.x = (uint32_t)f2bf(acc[0] * 0.125f) | ((uint32_t)f2bf(acc[1] * 0.125f) << 16);
+        o.y = (uint32_t)f2bf(acc[2] * 0.125f) | ((uint32_t)f2bf(acc[3] * 0.125f) << 16);
+        o.z = (uint32_t)f2bf(acc[4] * 0.125f) | ((uint32_t)f2bf(acc[5] * 0.125f) << 16);
+        o.w = (uint32_t)f2bf(acc[6] * 0.125f) | ((uint32_t)f2bf(acc[7] * 0.125f) << 16);
+        *(uint4 *)(y + i * 8) = o;   // element i * 8 = output voxel * C + 8 g
     }
 }
 
@@ -475,9 +462,11 @@ __global__ void k_split(const uint16_t *__restrict__ y, int64_t nvox, uint16_t *
 }
 
 // 1x1x1 head: logits = W x + b over C=16 input channels, L outputs;
-// prob = softmax(logits).  With a target (soft labels, float [v][L]) it also
-// writes dx (bf16, scale/nvox * W^T (prob - target)), accumulates dW, db
-// (float atomics per wave) and the summed cross-entropy into *loss.
+// prob = softmax(logits).  With a target (soft labels, float [v][L], or label
+// indices) it also writes dx (bf16, scale/nvox * W^T (prob - target)) and
+// accumulates dW, db and the summed cross-entropy into *loss: per wave by
+// shuffles, per block in LDS over the voxels the block strides over, then one
+// float atomic per entry and block.
 __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, int64_t nvox, int L,
                                               const float *__restrict__ W, const float *__restrict__ b,
                                               float *__restrict__ prob, const float *__restrict__ target,
@@ -487,64 +476,79 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, in
                                               float *__restrict__ db, float *__restrict__ loss)
 {
     constexpr int C = 16, LM = 32;
+    __shared__ float s_dw[LM * C], s_db[LM], s_loss;
     const bool train = target || tidx;
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = v < nvox;
-    float xin[C];
-    float lg[LM];
-    float gl[LM];
-    float ls = 0.0f;
-    for (int c = 0; c < C; ++c) xin[c] = act ? bf2f(x[v * C + c]) : 0.0f;
-    float mx = -1e30f;
-    for (int l = 0; l < LM; ++l) {
-        float s = 0.0f;
-        if (l < L) {
-            s = b[l];
-            for (int c = 0; c < C; ++c) s += W[l * C + c] * xin[c];
-            mx = fmaxf(mx, s);
-        }
-        lg[l] = s;
+    if (train) {
+        for (int i = threadIdx.x; i < LM * C; i += blockDim.x) s_dw[i] = 0.0f;
+        if (threadIdx.x < LM) s_db[threadIdx.x] = 0.0f;
+        if (threadIdx.x == 0) s_loss = 0.0f;
+        __syncthreads();
     }
-    float den = 0.0f;
-    for (int l = 0; l < LM; ++l)
-        if (l < L) {
-            lg[l] = __expf(lg[l] - mx);
-            den += lg[l];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // the loop bound is uniform per block (the wave shuffles need every lane)
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < nvox; v0 += stride) {
+        const int64_t v = v0 + threadIdx.x;
+        const bool act = v < nvox;
+        float xin[C];
+        float lg[LM];
+        float gl[LM];
+        float ls = 0.0f;
+        for (int c = 0; c < C; ++c) xin[c] = act ? bf2f(x[v * C + c]) : 0.0f;
+        float mx = -1e30f;
+        for (int l = 0; l < LM; ++l) {
+            float s = 0.0f;
+            if (l < L) {
+                s = b[l];
+                for (int c = 0; c < C; ++c) s += W[l * C + c] * xin[c];
+                mx = fmaxf(mx, s);
+            }
+            lg[l] = s;
         }
-    for (int l = 0; l < LM; ++l) {
-        gl[l] = 0.0f;
-        if (l < L) {
-            const float p = lg[l] / den;
-            if (act && prob) prob[v * L + l] = p;
-            if (train) {
-                const float w = (act && (!vmask || vmask[v])) ? 1.0f : 0.0f;
-                const float t = !act ? 0.0f : (target ? target[v * L + l] : (tidx[v] == l ? 1.0f : 0.0f));
-                gl[l] = w * scale * (p - t);
-                if (w > 0.0f && t > 0.0f) ls -= t * __logf(fmaxf(p, 1e-30f));
+        float den = 0.0f;
+        for (int l = 0; l < LM; ++l)
+            if (l < L) {
+                lg[l] = __expf(lg[l] - mx);
+                den += lg[l];
+            }
+        for (int l = 0; l < LM; ++l) {
+            gl[l] = 0.0f;
+            if (l < L) {
+                const float p = lg[l] / den;
+                if (act && prob) prob[v * L + l] = p;
+                if (train) {
+                    const float w = (act && (!vmask || vmask[v])) ? 1.0f : 0.0f;
+                    const float t = !act ? 0.0f : (target ? target[v * L + l] : (tidx[v] == l ? 1.0f : 0.0f));
+                    gl[l] = w * scale * (p - t);
+                    if (w > 0.0f && t > 0.0f) ls -= t * __logf(fmaxf(p, 1e-30f));
+                }
             }
         }
+        if (!train) continue;
+        if (act) {
+            for (int c = 0; c < C; ++c) {
+                float s = 0.0f;
+                for (int l = 0; l < L; ++l) s += W[l * C + c] * gl[l];
+                dx[v * C + c] = f2bf(s);
+            }
+        }
+        for (int l = 0; l < L; ++l) {
+            float g = gl[l];
+            for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&s_db[l], g);
+            for (int c = 0; c < C; ++c) {
+                float gw = gl[l] * xin[c];
+                for (int off = 32; off > 0; off >>= 1) gw += __shfl_xor(gw, off, 64);
+                if ((threadIdx.x & 63) == 0) atomicAdd(&s_dw[l * C + c], gw);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&s_loss, ls);
     }
     if (!train) return;
-    if (act) {
-        for (int c = 0; c < C; ++c) {
-            float s = 0.0f;
-            for (int l = 0; l < L; ++l) s += W[l * C + c] * gl[l];
-            dx[v * C + c] = f2bf(s);
-        }
-    }
-    // wave reductions of dW, db, loss
-    for (int l = 0; l < L; ++l) {
-        float g = gl[l];
-        for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&db[l], g);
-        for (int c = 0; c < C; ++c) {
-            float gw = gl[l] * xin[c];
-            for (int off = 32; off > 0; off >>= 1) gw += __shfl_xor(gw, off, 64);
-            if ((threadIdx.x & 63) == 0) atomicAdd(&dW[l * C + c], gw);
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, ls);
+    __syncthreads();
+    for (int i = threadIdx.x; i < L * C; i += blockDim.x) atomicAdd(&dW[i], s_dw[i]);
+    if (threadIdx.x < L) atomicAdd(&db[threadIdx.x], s_db[threadIdx.x]);
+    if (threadIdx.x == 0) atomicAdd(loss, s_loss);
 }
 
 // Adam step on FP32 master weights; refreshes the bf16 compute copy.
@@ -574,51 +578,58 @@ __global__ void k_f2bf(const float *__restrict__ a, uint16_t *__restrict__ b, in
     for (int64_t i = bmx_gtid(); i < n; i += bmx_gstride()) b[i] = f2bf(a[i]);
 }
 
-template <int NB>
-int launch_fwd(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y,
-               int Co, int relu, const uint16_t *xb = nullptr, int Ca = 0)
+template <int CH, bool CAT, bool MASK, int SPLIT>
+int launch_mm(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y,
+              int Co, int relu, const uint16_t *xb, int Ca, const uint16_t *xact, int Cf, int coff, float *part)
 {
-    // 16-voxel row blocks per wave: RB * NB independent accumulator chains in
-    // flight (the MFMA latency, not its throughput, bounds these layers); the
-    // counts are the fastest measured at 512^3 (two row blocks: 44 / 21 / 13 ms
-    // for NB = 1 / 2 / 4 in the decoder; these: 38 / 15.5 / 8.4 ms)
-    constexpr int RB = NB == 2 ? 8 : 4;
-    const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
-    dim3 grid((unsigned)((nvox + 64 * RB - 1) / (64 * RB)), (unsigned)(Co / (16 * NB)));
-    if (xb)
-        k_conv_fwd<NB, true, RB><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, xb, Ca);
-    else
-        k_conv_fwd<NB, false, RB><<<grid, 256, 0, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, nullptr, Ci);
+    const size_t lds = mm_lds_bytes(CH);
+    const void *kern = (const void *)k_conv_mm<CH, CAT, MASK, SPLIT>;
+    // an error here is left to the launch itself to report
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        (void)hipGetLastError();
+    const int nzc = (s.D + kMmZC - 1) / kMmZC;
+    const int64_t gz = (int64_t)s.N * nzc * (Co / 16);
+    BMX_CHECK(gz < 65536, "conv3d: too many z chunks for one dispatch");
+    BMX_CHECK((int64_t)s.H * s.W * (Cf > Co ? Cf : Co) < ((int64_t)1 << 31) - 2, "conv3d: plane too large");
+    dim3 grid((unsigned)((s.W + kMmTX - 1) / kMmTX), (unsigned)((s.H + kMmTY - 1) / kMmTY), (unsigned)gz);
+    k_conv_mm<CH, CAT, MASK, SPLIT><<<grid, 256, lds, ctx->stream>>>(x, s, Ci, w, b, y, Co, relu, xb, Ca, xact, Cf,
+                                                                     coff, part);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
 
-// The 16-output-channel forward: staged in LDS (k_conv_fwd_zm) for a plain
-// input of at most 48 channels, where it measured 1.4x faster at 512^3 (first
-// encoder layer: 27.9 against 39.6 ms); k_conv_fwd otherwise (staging the
-// decoder's concatenated input measured slower: 50.1 against 44.4 ms).
-int launch_fwd16(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y,
-                 int relu, const uint16_t *xb = nullptr, int Ca = 0)
+template <bool CAT, bool MASK, int SPLIT>
+int launch_mm_ch(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y,
+                 int Co, int relu, const uint16_t *xb, int Ca, const uint16_t *xact, int Cf, int coff, float *part)
 {
-    if (Ci > kZmMaxCi || xb) return launch_fwd<1>(ctx, x, s, Ci, w, b, y, 16, relu, xb, Ca);
-    const int nzc = (s.D + kZmZC - 1) / kZmZC;
-    BMX_CHECK((int64_t)s.N * nzc < 65536, "conv3d: too many z chunks for one dispatch");
-    dim3 grid((unsigned)((s.W + kZmTX - 1) / kZmTX), (unsigned)((s.H + kZmTY - 1) / kZmTY), (unsigned)(s.N * nzc));
-    const size_t lds = (size_t)3 * kZmPlane * zm_pitch(Ci) * sizeof(uint16_t);
-    k_conv_fwd_zm<<<grid, 256, lds, ctx->stream>>>(x, s, Ci, w, b, y, relu);
-    BMX_LAUNCH_CHECK();
-    return BMX_OK;
+    if (Ci <= 32) return launch_mm<1, CAT, MASK, SPLIT>(ctx, x, s, Ci, w, b, y, Co, relu, xb, Ca, xact, Cf, coff, part);
+    return launch_mm<2, CAT, MASK, SPLIT>(ctx, x, s, Ci, w, b, y, Co, relu, xb, Ca, xact, Cf, coff, part);
 }
 
-template <int NB>
-int launch_dgrad(bmx_ctx *ctx, const uint16_t *dy, Shape s, int Co, const uint16_t *wt, const uint16_t *xact,
-                 uint16_t *dx, int Ci)
+// The MFMA (16x16x32) forward: up to 64 input channels in one launch (CH =
+// ceil(Ci / 32) <= 2 chunks, the weights of a block's 16 output channels in
+// registers), wider inputs in launches over 64-channel parts with float32
+// partial sums in a context buffer.
+template <bool CAT, bool MASK>
+int conv_mm(bmx_ctx *ctx, const uint16_t *x, Shape s, int Ci, const uint16_t *w, const float *b, uint16_t *y, int Co,
+            int relu, const uint16_t *xb, int Ca, const uint16_t *xact)
 {
-    const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
-    dim3 grid((unsigned)((nvox + 127) / 128), (unsigned)(Ci / (16 * NB)));
-    k_conv_dgrad<NB><<<grid, 256, 0, ctx->stream>>>(dy, s, Co, wt, xact, dx, Ci);
-    BMX_LAUNCH_CHECK();
-    return BMX_OK;
+    if (Ci <= 64)
+        return launch_mm_ch<CAT, MASK, 0>(ctx, x, s, Ci, w, b, y, Co, relu, xb, Ca, xact, Ci, 0, nullptr);
+    const size_t bytes = (size_t)s.N * s.D * s.H * s.W * Co * sizeof(float);
+    int rc = grow(ctx, &ctx->d_netpart, &ctx->netpart_cap, bytes);
+    if (rc) return rc;
+    float *part = (float *)ctx->d_netpart;
+    for (int c0 = 0; c0 < Ci && rc == BMX_OK; c0 += 64) {
+        const int cw = Ci - c0 < 64 ? Ci - c0 : 64;
+        if (c0 == 0)
+            rc = launch_mm_ch<CAT, MASK, 1>(ctx, x, s, cw, w, b, y, Co, relu, xb, Ca, xact, Ci, c0, part);
+        else if (c0 + cw < Ci)
+            rc = launch_mm_ch<CAT, MASK, 2>(ctx, x, s, cw, w, b, y, Co, relu, xb, Ca, xact, Ci, c0, part);
+        else
+            rc = launch_mm_ch<CAT, MASK, 3>(ctx, x, s, cw, w, b, y, Co, relu, xb, Ca, xact, Ci, c0, part);
+    }
+    return rc;
 }
 
 int check_conv_shape(int N, int D, int H, int W, int Ci, int Co)
@@ -644,9 +655,7 @@ int bmx_conv3d_fwd(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, 
     if (rc) return rc;
     BMX_CHECK(x && w && y, "bmx_conv3d_fwd: null pointer");
     Shape s{N, D, H, W};
-    if (Co % 64 == 0) return launch_fwd<4>(ctx, x, s, Ci, w, b, y, Co, relu);
-    if (Co % 32 == 0) return launch_fwd<2>(ctx, x, s, Ci, w, b, y, Co, relu);
-    return launch_fwd16(ctx, x, s, Ci, w, b, y, relu);
+    return conv_mm<false, false>(ctx, x, s, Ci, w, b, y, Co, relu, nullptr, Ci, nullptr);
 }
 
 int bmx_conv3d_fwd_cat(bmx_ctx *ctx, const uint16_t *a, int N, int D, int H, int W, int Ca, const uint16_t *b,
@@ -661,9 +670,7 @@ int bmx_conv3d_fwd_cat(bmx_ctx *ctx, const uint16_t *a, int N, int D, int H, int
     BMX_CHECK(D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "bmx_conv3d_fwd_cat: grid must be even");
     Shape s{N, D, H, W};
     const int Ci = Ca + Cb;
-    if (Co % 64 == 0) return launch_fwd<4>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
-    if (Co % 32 == 0) return launch_fwd<2>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca);
-    return launch_fwd16(ctx, a, s, Ci, w, bias, y, relu, b, Ca);
+    return conv_mm<true, false>(ctx, a, s, Ci, w, bias, y, Co, relu, b, Ca, nullptr);
 }
 
 int bmx_conv3d_dgrad(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, int Co, const uint16_t *w,
@@ -678,9 +685,7 @@ int bmx_conv3d_dgrad(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int 
     k_transpose_w<<<bmx_blocks(nw, 256), 256, 0, ctx->stream>>>(w, Co, Ci, wt_scratch);
     BMX_LAUNCH_CHECK();
     Shape s{N, D, H, W};
-    if (Ci % 64 == 0) return launch_dgrad<4>(ctx, dy, s, Co, wt_scratch, xact, dx, Ci);
-    if (Ci % 32 == 0) return launch_dgrad<2>(ctx, dy, s, Co, wt_scratch, xact, dx, Ci);
-    return launch_dgrad<1>(ctx, dy, s, Co, wt_scratch, xact, dx, Ci);
+    return conv_mm<false, true>(ctx, dy, s, Co, wt_scratch, nullptr, dx, Ci, 0, nullptr, Co, xact);
 }
 
 int bmx_conv3d_wgrad(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int Ci, const uint16_t *dy,
@@ -704,8 +709,9 @@ int bmx_pool2(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int C
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
-    BMX_CHECK(x && y && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "bmx_pool2: bad arguments");
-    const int64_t n = (int64_t)N * (D / 2) * (H / 2) * (W / 2) * C;
+    BMX_CHECK(x && y && D % 2 == 0 && H % 2 == 0 && W % 2 == 0 && C >= 16 && C % 16 == 0,
+              "bmx_pool2: bad arguments");
+    const int64_t n = (int64_t)N * (D / 2) * (H / 2) * (W / 2) * (C / 8);
     k_pool2<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
@@ -776,8 +782,12 @@ int bmx_head(bmx_ctx *ctx, const uint16_t *x, int64_t nvox, int L, const float *
     BMX_CHECK(!(target || target_idx) || (dx && dW && db && loss), "bmx_head: training needs dx, dW, db, loss");
     BMX_CHECK(nvox >= 0 && nvox < ((int64_t)1 << 32) - 256, "bmx_head: nvox %lld exceeds one dispatch",
               (long long)nvox);
-    k_head<<<bmx_blocks(nvox, 256), 256, 0, ctx->stream>>>(x, nvox, L, W, b, prob, target, target_idx, vmask,
-                                                          scale, dx, dW, db, loss);
+    // training: a bounded grid, each block striding over voxels, so that the
+    // parameter gradients take few global atomics
+    const bool train = target || target_idx;
+    const unsigned blocks = train ? (unsigned)std::min<int64_t>(bmx_blocks(nvox, 256), 2048) : bmx_blocks(nvox, 256);
+    k_head<<<blocks > 0 ? blocks : 1, 256, 0, ctx->stream>>>(x, nvox, L, W, b, prob, target, target_idx, vmask,
+                                                             scale, dx, dW, db, loss);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }

@@ -169,20 +169,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t *__restric
         }
 }
 
-template <typename P>
-int grow(bmx_ctx *ctx, P **ptr, size_t *cap, size_t bytes)
-{
-    if (*cap >= bytes) return BMX_OK;
-    if (*ptr) {
-        BMX_HIP(hipStreamSynchronize(ctx->stream));
-        BMX_HIP(hipFree(*ptr));
-    }
-    *ptr = nullptr;
-    *cap = 0;
-    BMX_HIP(hipMalloc((void **)ptr, bytes));
-    *cap = bytes;
-    return BMX_OK;
-}
 
 // Exclusive scan of counts[0..n) on the context stream (block partials in ctx->d_scan).
 inline int scan_u32(bmx_ctx *ctx, uint32_t *counts, int64_t n)
