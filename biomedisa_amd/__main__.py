@@ -15,36 +15,35 @@ import sys
 def parser():
     p = argparse.ArgumentParser(prog="python -m biomedisa_amd", description="Biomedisa interpolation on one MI355X.",
                                 formatter_class=argparse.ArgumentDefaultsHelpFormatter)
-    p.add_argument("path_to_data", type=str, metavar="PATH_TO_IMAGE", help="Location of image data")
-    p.add_argument("path_to_labels", type=str, metavar="PATH_TO_LABELS", help="Location of label data")
-    p.add_argument("--nbrw", type=int, default=10, help="Number of random walks starting at each pre-segmented pixel")
-    p.add_argument("--sorw", type=int, default=4000, help="Steps of a random walk")
-    p.add_argument("--acwe", action="store_true", default=False, help="Post-processing with active contour")
-    p.add_argument("--acwe_alpha", metavar="ALPHA", type=float, default=1.0, help="Pushing force of active contour")
+    p.add_argument("path_to_data", type=str, metavar="PATH_TO_IMAGE", help="image volume file (or folder of slices)")
+    p.add_argument("path_to_labels", type=str, metavar="PATH_TO_LABELS", help="label volume with the pre-segmented slices")
+    p.add_argument("--nbrw", type=int, default=10, help="walks started from every seed voxel")
+    p.add_argument("--sorw", type=int, default=4000, help="steps per walk")
+    p.add_argument("--acwe", action="store_true", default=False, help="also refine the result with an active-contour (Chan-Vese) pass")
+    p.add_argument("--acwe_alpha", metavar="ALPHA", type=float, default=1.0, help="active-contour balloon force")
     p.add_argument("--acwe_smooth", metavar="SMOOTH", type=int, default=1,
-                   help="Smoothing steps of active contour")
-    p.add_argument("--acwe_steps", metavar="STEPS", type=int, default=3, help="Iterations of active contour")
+                   help="curvature steps per active-contour iteration")
+    p.add_argument("--acwe_steps", metavar="STEPS", type=int, default=3, help="active-contour iterations")
     p.add_argument("-nc", "--no-compression", dest="compression", action="store_false",
-                   help="Disable compression of segmentation results")
+                   help="write result files uncompressed")
     p.add_argument("-allx", "--allaxis", action="store_true", default=False,
-                   help="If pre-segmentation is not exlusively in xy-plane")
+                   help="seeds come from labelled planes of all three axes, not only z-planes")
     p.add_argument("-d", "--denoise", action="store_true", default=False,
-                   help="Smooth/denoise image data before processing")
+                   help="apply the 3x3x3 denoising filter to the image first")
     p.add_argument("-u", "--uncertainty", action="store_true", default=False,
-                   help="Return uncertainty of segmentation result")
-    p.add_argument("-i", "--ignore", type=str, default="none", help="Ignore specific label(s), e.g. 2,5,6")
-    p.add_argument("-o", "--only", type=str, default="all", help="Segment only specific label(s), e.g. 1,3,5")
+                   help="also write the per-voxel uncertainty volume")
+    p.add_argument("-i", "--ignore", type=str, default="none", help="comma-separated label values to leave out, or 'none'")
+    p.add_argument("-o", "--only", type=str, default="all", help="comma-separated label values to keep, or 'all'")
     p.add_argument("-s", "--smooth", nargs="?", type=int, const=100, default=0,
-                   help="Number of smoothing iterations for segmentation result")
+                   help="curvature-smoothing iterations of the result (flag alone: 100)")
     p.add_argument("-c", "--clean", nargs="?", type=float, const=0.1, default=None,
-                   help="Remove outliers, e.g. 0.5 means that objects smaller than 50 percent of the size of "
-                        "the largest object will be removed")
+                   help="drop connected pieces of a label below this fraction of its largest piece "
+                        "(flag alone: 0.1)")
     p.add_argument("-f", "--fill", nargs="?", type=float, const=0.9, default=None,
-                   help="Fill holes, e.g. 0.5 means that all holes smaller than 50 percent of the entire label "
-                        "will be filled")
+                   help="fill enclosed holes smaller than this fraction of the label's size (flag alone: 0.9)")
     p.add_argument("-p", "--platform", default=None, help="Accepted for compatibility; the engine runs on HIP")
     p.add_argument("-rh", "--return_hits", action="store_true", default=False,
-                   help="Return hits from each label")
+                   help="also return the per-label hit volumes")
     p.add_argument("-iid", "--img_id", type=str, default=None, help="Accepted for compatibility")
     p.add_argument("-lid", "--label_id", type=str, default=None, help="Accepted for compatibility")
     p.add_argument("-r", "--remote", action="store_true", default=False, help="Accepted for compatibility")

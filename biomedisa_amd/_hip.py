@@ -19,6 +19,7 @@ IMG_U8, IMG_F32 = 0, 1
 RNG_PHILOX, RNG_REF = 0, 1
 WALK_AUTO, WALK_DIRECT, WALK_SORTED, WALK_TILED = 0, 1, 2, 3
 LABEL_NONE = 0xFF
+LABEL_ALL = 256     # label_none of a volume whose 256 indices are all labels
 LABEL_CLASSES = 258          # BMX_LABEL_CLASSES: values 0..255, negative, above 255
 CLASS_NEG, CLASS_BIG = 256, 257
 # raw dtypes of the device pre-processing (BMX_RAW_*)
@@ -64,6 +65,9 @@ SIGNATURES = {
     "bmx_seeds": (_i, [_p, _p, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
     "bmx_plane_flags_ex": (_i, [_p, _p, _i64, _i64, _i64, _i, _i, _p]),
     "bmx_adaptive_mask": (_i, [_p, _p, _i64, _i64, _i64, _p, _p, _p]),
+    "bmx_plane_flags_lab": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _i, _p]),
+    "bmx_adaptive_mask_lab": (_i, [_p, _p, _i, _i64, _i64, _i64, _p, _p, _p]),
+    "bmx_seeds_lab": (_i, [_p, _p, _i, _p, _i, _i64, _i64, _i64, _p, _p, _i64, C.POINTER(_i64)]),
     "bmx_smooth_argmax_ex": (_i, [_p, _p, _i, _i64, _i64, _i64, _i, _p, _p, _i]),
     "bmx_clean": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
     "bmx_fill": (_i, [_p, _p, _i64, _i64, _i64, C.c_double, _p]),
@@ -272,42 +276,45 @@ class Engine:
         return [int(v) for v in out]
 
     # path -----------------------------------------------------------------
-    def plane_flags(self, lab, dims, allaxis, out=None, large=False):
+    # label_none: the label volume's index for ignored voxels (LABEL_NONE, or
+    # LABEL_ALL when all 256 indices are labels)
+    def plane_flags(self, lab, dims, allaxis, out=None, large=False, label_none=LABEL_NONE):
         Z, Y, X = dims
         flags = out if out is not None else DeviceBuffer(self, Z + Y + X)
-        _check(load().bmx_plane_flags_ex(self.ctx, lab.ptr, Z, Y, X, int(bool(allaxis)), int(bool(large)),
-                                         flags.ptr))
+        _check(load().bmx_plane_flags_lab(self.ctx, lab.ptr, int(label_none), Z, Y, X, int(bool(allaxis)),
+                                          int(bool(large)), flags.ptr))
         return flags
 
-    def adaptive_mask(self, lab, dims, flags, out=None):
+    def adaptive_mask(self, lab, dims, flags, out=None, label_none=LABEL_NONE):
         """uint8 mask of the voxels whose seeds walk in the large-volume path."""
         Z, Y, X = dims
         V = Z * Y * X
         mask = out if out is not None else DeviceBuffer(self, V)
         tmp = DeviceBuffer(self, 4 * V)
         try:
-            _check(load().bmx_adaptive_mask(self.ctx, lab.ptr, Z, Y, X, flags.ptr, tmp.ptr, mask.ptr))
+            _check(load().bmx_adaptive_mask_lab(self.ctx, lab.ptr, int(label_none), Z, Y, X, flags.ptr, tmp.ptr,
+                                                mask.ptr))
         finally:
             self.sync()
             tmp.free()
         return mask
 
-    def count_seeds(self, lab, dims, flags):
+    def count_seeds(self, lab, dims, flags, label_none=LABEL_NONE):
         Z, Y, X = dims
         n = C.c_int64()
-        _check(load().bmx_seeds(self.ctx, lab.ptr, None, IMG_U8, Z, Y, X, flags.ptr, None, 0,
-                                C.byref(n)))
+        _check(load().bmx_seeds_lab(self.ctx, lab.ptr, int(label_none), None, IMG_U8, Z, Y, X, flags.ptr, None, 0,
+                                    C.byref(n)))
         return n.value
 
-    def seeds(self, lab, img, img_dtype, dims, flags, capacity=None, out=None):
+    def seeds(self, lab, img, img_dtype, dims, flags, capacity=None, out=None, label_none=LABEL_NONE):
         """Seed list on the device -> (buffer, count)."""
         Z, Y, X = dims
         if capacity is None:
-            capacity = self.count_seeds(lab, dims, flags)
+            capacity = self.count_seeds(lab, dims, flags, label_none)
         buf = out if out is not None else DeviceBuffer(self, max(1, capacity) * SEED_DTYPE.itemsize)
         n = C.c_int64()
-        _check(load().bmx_seeds(self.ctx, lab.ptr, img.ptr, img_dtype, Z, Y, X, flags.ptr, buf.ptr,
-                                capacity, C.byref(n)))
+        _check(load().bmx_seeds_lab(self.ctx, lab.ptr, int(label_none), img.ptr, img_dtype, Z, Y, X, flags.ptr,
+                                    buf.ptr, capacity, C.byref(n)))
         return buf, n.value
 
     def select_seeds(self, seeds, nseeds, mask, out=None):

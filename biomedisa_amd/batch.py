@@ -72,13 +72,15 @@ class DeviceBatch:
         self.flags = eng.alloc(max(1, nflags))
         counts, fo = [], 0
         self._flag_views = []
-        for (Z, Y, X), (_, _, _, allaxis), o in zip(self.dims, items, self.offsets):
+        # a volume of all 256 label values uses every index as a label (no ignored index)
+        self.label_none = [_hip.LABEL_ALL if nl == 256 else _hip.LABEL_NONE for nl in self.nlabels]
+        for (Z, Y, X), (_, _, _, allaxis), o, ln in zip(self.dims, items, self.offsets, self.label_none):
             fv = _hip.BufferView(self.flags, fo, Z + Y + X)
-            eng.plane_flags(self.view_lab(o, Z * Y * X), (Z, Y, X), allaxis, out=fv)
+            eng.plane_flags(self.view_lab(o, Z * Y * X), (Z, Y, X), allaxis, out=fv, label_none=ln)
             flags = fv.download((Z + Y + X,), np.uint8)
             if allaxis and adjacent_planes((flags[:Z], flags[Z:Z + Y], flags[Z + Y:])):
                 raise PipelineError("At least one empty slice between labels required.")
-            counts.append(eng.count_seeds(self.view_lab(o, Z * Y * X), (Z, Y, X), fv))
+            counts.append(eng.count_seeds(self.view_lab(o, Z * Y * X), (Z, Y, X), fv, label_none=ln))
             self._flag_views.append(fv)
             fo += Z + Y + X
         self.nseeds = counts
@@ -91,7 +93,7 @@ class DeviceBatch:
                 continue
             out = _hip.BufferView(self.seeds, int(self.seed_offsets[k]) * isz, counts[k] * isz)
             _, n = eng.seeds(self.view_lab(o, Z * Y * X), self.view_img(o, Z * Y * X), self.img_dtype, (Z, Y, X),
-                             self._flag_views[k], capacity=counts[k], out=out)
+                             self._flag_views[k], capacity=counts[k], out=out, label_none=self.label_none[k])
             assert n == counts[k]
         self.table = np.zeros(len(items), dtype=_hip.VOLUME_DTYPE)
         for k, (Z, Y, X) in enumerate(self.dims):

@@ -98,7 +98,7 @@ int bmx_argmax(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t V, const
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(hits && values && out, "bmx_argmax: null pointer");
-    BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_argmax: nlabels %d not in [1,255]", nlabels);
+    BMX_CHECK(nlabels >= 1 && nlabels <= 256, "bmx_argmax: nlabels %d not in [1,256]", nlabels);
     BMX_CHECK(V >= 0 && V < ((int64_t)1 << 32) - 256, "bmx_argmax: V %lld not in [0, 2^32)", (long long)V);
     if (V == 0) return BMX_OK;
     LabelValues lv;
@@ -120,7 +120,7 @@ int bmx_uncertainty(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_t Z, 
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(hits && out, "bmx_uncertainty: null pointer");
-    BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_uncertainty: nlabels %d not in [1,255]", nlabels);
+    BMX_CHECK(nlabels >= 1 && nlabels <= 256, "bmx_uncertainty: nlabels %d not in [1,256]", nlabels);
     BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_uncertainty: bad dims");
     const int64_t V = Z * Y * X;
     k_uncertainty<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(hits, nlabels, (int)Z, (int)Y, (int)X,

@@ -26,43 +26,50 @@ constexpr int kSeedThreads = 256;
 constexpr int kSeedPerThread = 16;
 constexpr int kSeedChunk = kSeedThreads * kSeedPerThread;
 
-// Voxel counts for plane detection: any non-zero label index (negative labels,
-// BMX_LABEL_NONE, included), or with POS only positive labels (index 1..254).
+// Label volumes: label index per voxel, 0 = background; the index `none`
+// (BMX_LABEL_NONE, or 256 when all 256 indices are labels) marks ignored
+// voxels (negative labels).  Inside the kernels kNone16 stands for "no label"
+// (ignored, or off the labelled planes), which no index equals.
+constexpr int kNone16 = 0x100;
+
+// Voxel counts for plane detection: any non-zero label index (ignored voxels
+// included), or with POS only positive labels.
 template <bool POS>
-__device__ __forceinline__ bool nz(const uint8_t *lab, int64_t i)
+__device__ __forceinline__ bool nz(const uint8_t *lab, int64_t i, int none)
 {
-    const uint8_t v = lab[i];
-    return POS ? (v != 0 && v != BMX_LABEL_NONE) : v != 0;
+    const int v = lab[i];
+    return POS ? (v != 0 && v != none) : v != 0;
 }
 
 // Voxel survives the first thin-line pass along an axis of extent na, stride sa.
 template <bool POS>
-__device__ __forceinline__ bool pass1(const uint8_t *lab, int64_t i, int a, int na, int64_t sa)
+__device__ __forceinline__ bool pass1(const uint8_t *lab, int64_t i, int a, int na, int64_t sa, int none)
 {
-    if (!nz<POS>(lab, i)) return false;
-    if (a > 0 && a < na - 1 && !nz<POS>(lab, i - sa) && !nz<POS>(lab, i + sa)) return false;
+    if (!nz<POS>(lab, i, none)) return false;
+    if (a > 0 && a < na - 1 && !nz<POS>(lab, i - sa, none) && !nz<POS>(lab, i + sa, none)) return false;
     return true;
 }
 
 template <bool POS>
 __device__ __forceinline__ bool pass2(const uint8_t *lab, int64_t i, int a, int na, int64_t sa,
-                                      int b, int nb, int64_t sb)
+                                      int b, int nb, int64_t sb, int none)
 {
-    if (!pass1<POS>(lab, i, a, na, sa)) return false;
-    if (b > 0 && b < nb - 1 && !pass1<POS>(lab, i - sb, a, na, sa) && !pass1<POS>(lab, i + sb, a, na, sa))
+    if (!pass1<POS>(lab, i, a, na, sa, none)) return false;
+    if (b > 0 && b < nb - 1 && !pass1<POS>(lab, i - sb, a, na, sa, none) &&
+        !pass1<POS>(lab, i + sb, a, na, sa, none))
         return false;
     return true;
 }
 
 template <bool POS>
-__global__ __launch_bounds__(256) void k_plane_flags(const uint8_t *__restrict__ lab, int Z, int Y,
+__global__ __launch_bounds__(256) void k_plane_flags(const uint8_t *__restrict__ lab, int none, int Z, int Y,
                                                      int X, int allaxis, uint8_t *fz, uint8_t *fy,
                                                      uint8_t *fx)
 {
     const int64_t sy = X, sz = (int64_t)X * Y, V = sz * Z;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
-        if (!nz<false>(lab, i)) continue;
+        if (!nz<false>(lab, i, none)) continue;
         const int z = (int)(i / sz);
         const int64_t r = i - z * sz;
         const int y = (int)(r / sy);
@@ -71,9 +78,9 @@ __global__ __launch_bounds__(256) void k_plane_flags(const uint8_t *__restrict__
             if (!fz[z]) fz[z] = 1;
             continue;
         }
-        if (!fz[z] && pass2<POS>(lab, i, y, Y, sy, x, X, 1)) fz[z] = 1;
-        if (!fy[y] && pass2<POS>(lab, i, z, Z, sz, x, X, 1)) fy[y] = 1;
-        if (!fx[x] && pass2<POS>(lab, i, z, Z, sz, y, Y, sy)) fx[x] = 1;
+        if (!fz[z] && pass2<POS>(lab, i, y, Y, sy, x, X, 1, none)) fz[z] = 1;
+        if (!fy[y] && pass2<POS>(lab, i, z, Z, sz, x, X, 1, none)) fy[y] = 1;
+        if (!fx[x] && pass2<POS>(lab, i, z, Z, sz, y, Y, sy, none)) fx[x] = 1;
     }
 }
 
@@ -96,12 +103,17 @@ __global__ void k_skip_planes(uint8_t *fz, int Z)
 struct Geom {
     int Z, Y, X;
     int64_t sy, sz, V;
+    int none;   // the label volume's index for ignored voxels (256: none)
 };
 
-__device__ __forceinline__ uint8_t mask_at(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
-                                           const uint8_t *fx, int z, int y, int x, int64_t i)
+// The label the reference's rebuilt slice volume holds at a voxel
+// (pyopencl_small.py:89-107): its label on a labelled plane, else kNone16.
+__device__ __forceinline__ int mask_at(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
+                                       const uint8_t *fx, const Geom &g, int z, int y, int x, int64_t i)
 {
-    return (fz[z] | fy[y] | fx[x]) ? lab[i] : (uint8_t)BMX_LABEL_NONE;
+    if (!(fz[z] | fy[y] | fx[x])) return kNone16;
+    const int v = lab[i];
+    return v == g.none ? kNone16 : v;
 }
 
 __device__ __forceinline__ bool is_seed(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
@@ -109,7 +121,7 @@ __device__ __forceinline__ bool is_seed(const uint8_t *lab, const uint8_t *fz, c
                                         int64_t i)
 {
     if (z <= 0 || z >= g.Z - 1 || y <= 0 || y >= g.Y - 1 || x <= 0 || x >= g.X - 1) return false;
-    return mask_at(lab, fz, fy, fx, z, y, x, i) != BMX_LABEL_NONE;
+    return mask_at(lab, fz, fy, fx, g, z, y, x, i) != kNone16;
 }
 
 __device__ __forceinline__ void coords(const Geom &g, int64_t i, int &z, int &y, int &x)
@@ -214,13 +226,13 @@ template <typename T>
 __device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy, const uint8_t *fx,
                           const T *img, const Geom &g, int z, int y, int x, int64_t i0)
 {
-    const uint8_t seg = mask_at(lab, fz, fy, fx, z, y, x, i0);
+    const int seg = mask_at(lab, fz, fy, fx, g, z, y, x, i0);
     const float B = bmx_ld(img, i0);
     float dev = 0.0f, summe = 0.0f;
     for (int n = -1; n < 2; ++n)
         for (int o = -1; o < 2; ++o) {
             const int64_t i = i0 + n * g.sy + o;
-            if (mask_at(lab, fz, fy, fx, z, y + n, x + o, i) == seg) {
+            if (mask_at(lab, fz, fy, fx, g, z, y + n, x + o, i) == seg) {
                 const float t = B - bmx_ld(img, i);
                 dev = __builtin_fmaf(t, t, dev);
                 summe += 1.0f;
@@ -229,7 +241,7 @@ __device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
     for (int n = -1; n <= 1; n += 2)
         for (int o = -1; o < 2; ++o) {
             const int64_t i = i0 + n * g.sz + o;
-            if (summe < 9.0f && mask_at(lab, fz, fy, fx, z + n, y, x + o, i) == seg) {
+            if (summe < 9.0f && mask_at(lab, fz, fy, fx, g, z + n, y, x + o, i) == seg) {
                 const float t = B - bmx_ld(img, i);
                 dev = __builtin_fmaf(t, t, dev);
                 summe += 1.0f;
@@ -238,7 +250,7 @@ __device__ float calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
     for (int n = -1; n <= 1; n += 2)
         for (int o = -1; o <= 1; o += 2) {
             const int64_t i = i0 + n * g.sz + o * g.sy;
-            if (summe < 9.0f && mask_at(lab, fz, fy, fx, z + n, y + o, x, i) == seg) {
+            if (summe < 9.0f && mask_at(lab, fz, fy, fx, g, z + n, y + o, x, i) == seg) {
                 const float t = B - bmx_ld(img, i);
                 dev = __builtin_fmaf(t, t, dev);
                 summe += 1.0f;
@@ -287,12 +299,13 @@ __global__ __launch_bounds__(kSeedThreads) void k_seed_write(const uint8_t *__re
 
 extern "C" {
 
-int bmx_plane_flags_ex(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, int allaxis,
-                       int large, uint8_t *flags)
+int bmx_plane_flags_lab(bmx_ctx *ctx, const uint8_t *lab, int label_none, int64_t Z, int64_t Y, int64_t X,
+                        int allaxis, int large, uint8_t *flags)
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(lab && flags, "bmx_plane_flags: null pointer");
+    BMX_CHECK(label_none >= 1 && label_none <= 256, "bmx_plane_flags: label_none %d not in [1,256]", label_none);
     BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32,
               "bmx_plane_flags: bad dims %lld x %lld x %lld", (long long)Z, (long long)Y,
               (long long)X);
@@ -301,11 +314,11 @@ int bmx_plane_flags_ex(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, i
     int64_t blocks = (V + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (large)
-        k_plane_flags<true><<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, (int)Z, (int)Y, (int)X, allaxis,
-                                                                      flags, flags + Z, flags + Z + Y);
+        k_plane_flags<true><<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, label_none, (int)Z, (int)Y, (int)X,
+                                                                      allaxis, flags, flags + Z, flags + Z + Y);
     else
-        k_plane_flags<false><<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, (int)Z, (int)Y, (int)X, allaxis,
-                                                                       flags, flags + Z, flags + Z + Y);
+        k_plane_flags<false><<<(unsigned)blocks, 256, 0, ctx->stream>>>(lab, label_none, (int)Z, (int)Y, (int)X,
+                                                                       allaxis, flags, flags + Z, flags + Z + Y);
     BMX_LAUNCH_CHECK();
     if (large && !allaxis) {
         k_skip_planes<<<1, 64, 0, ctx->stream>>>(flags, (int)Z);
@@ -314,24 +327,31 @@ int bmx_plane_flags_ex(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, i
     return BMX_OK;
 }
 
+int bmx_plane_flags_ex(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, int allaxis,
+                       int large, uint8_t *flags)
+{
+    return bmx_plane_flags_lab(ctx, lab, BMX_LABEL_NONE, Z, Y, X, allaxis, large, flags);
+}
+
 int bmx_plane_flags(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X, int allaxis,
                     uint8_t *flags)
 {
-    return bmx_plane_flags_ex(ctx, lab, Z, Y, X, allaxis, 0, flags);
+    return bmx_plane_flags_lab(ctx, lab, BMX_LABEL_NONE, Z, Y, X, allaxis, 0, flags);
 }
 
-int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype, int64_t Z, int64_t Y,
-              int64_t X, const uint8_t *flags, bmx_seed *seeds, int64_t capacity, int64_t *nseeds)
+int bmx_seeds_lab(bmx_ctx *ctx, const uint8_t *lab, int label_none, const void *img, int img_dtype, int64_t Z,
+                  int64_t Y, int64_t X, const uint8_t *flags, bmx_seed *seeds, int64_t capacity, int64_t *nseeds)
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(lab && flags && nseeds, "bmx_seeds: null pointer");
+    BMX_CHECK(label_none >= 1 && label_none <= 256, "bmx_seeds: label_none %d not in [1,256]", label_none);
     BMX_CHECK(img_dtype == BMX_IMG_U8 || img_dtype == BMX_IMG_F32, "bmx_seeds: bad image dtype %d",
               img_dtype);
     BMX_CHECK(seeds == nullptr || img != nullptr, "bmx_seeds: image needed to write seeds");
     BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32,
               "bmx_seeds: bad dims %lld x %lld x %lld", (long long)Z, (long long)Y, (long long)X);
-    Geom g{(int)Z, (int)Y, (int)X, X, X * Y, X * Y * Z};
+    Geom g{(int)Z, (int)Y, (int)X, X, X * Y, X * Y * Z, label_none};
     const int64_t nblk = (g.V + kSeedChunk - 1) / kSeedChunk;
     if (ctx->block_counts_cap < nblk) {
         if (ctx->d_block_counts) BMX_HIP(hipFree(ctx->d_block_counts));
@@ -361,6 +381,12 @@ int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype, 
         return bmx_set_error(BMX_E_INVALID, "bmx_seeds: %lld seeds exceed capacity %lld",
                              (long long)total, (long long)capacity);
     return BMX_OK;
+}
+
+int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype, int64_t Z, int64_t Y,
+              int64_t X, const uint8_t *flags, bmx_seed *seeds, int64_t capacity, int64_t *nseeds)
+{
+    return bmx_seeds_lab(ctx, lab, BMX_LABEL_NONE, img, img_dtype, Z, Y, X, flags, seeds, capacity, nseeds);
 }
 
 }  // extern "C"
@@ -398,8 +424,10 @@ __global__ __launch_bounds__(256) void k_select_seeds(const bmx_seed *__restrict
 // labelled plane.  p itself is in that window with label L, so "another label
 // is seen" is "the window's smallest and largest label differ".  The window is
 // a product of one stride-5 set per axis (the interior test is per axis too),
-// so min and max are taken in three separable passes; BMX_LABEL_NONE (0xFF,
-// off-plane or ignored) is the identity of min and is mapped to 0 for max.
+// so min and max are taken in three separable passes: off-plane voxels are
+// skipped and ignored ones (index `none`) count as no label: 0xFF is the
+// identity of min (the seed's own label is always in its window) and such a
+// voxel counts as 0 for max.
 constexpr int kAdaptStep = 5, kAdaptTaps = 20;   // offsets -100..100 in steps of 5
 
 struct MinMax {
@@ -408,9 +436,10 @@ struct MinMax {
 
 // Pass along x: the label seen at q (labelled plane, else none), min/max over
 // the stride-5 window of x.
-__global__ __launch_bounds__(256) void k_adapt_x(const uint8_t *__restrict__ lab, const uint8_t *__restrict__ fz,
-                                                 const uint8_t *__restrict__ fy, const uint8_t *__restrict__ fx,
-                                                 int Z, int Y, int X, MinMax *__restrict__ out)
+__global__ __launch_bounds__(256) void k_adapt_x(const uint8_t *__restrict__ lab, int none,
+                                                 const uint8_t *__restrict__ fz, const uint8_t *__restrict__ fy,
+                                                 const uint8_t *__restrict__ fx, int Z, int Y, int X,
+                                                 MinMax *__restrict__ out)
 {
     const int64_t sz = (int64_t)X * Y, V = sz * Z;
     for (int64_t i = bmx_gtid(); i < V; i += bmx_gstride()) {
@@ -425,8 +454,8 @@ __global__ __launch_bounds__(256) void k_adapt_x(const uint8_t *__restrict__ lab
                 const int qx = x + kAdaptStep * k;
                 if (qx <= 0 || qx >= X - 1) continue;
                 const uint8_t v = lab[row + qx];
-                lo = min(lo, v);
-                hi = max(hi, v == BMX_LABEL_NONE ? (uint8_t)0 : v);
+                lo = (int)v == none ? lo : min(lo, v);
+                hi = max(hi, (int)v == none ? (uint8_t)0 : v);
             }
         } else {
             // only x-planes can be labelled on this row
@@ -434,8 +463,8 @@ __global__ __launch_bounds__(256) void k_adapt_x(const uint8_t *__restrict__ lab
                 const int qx = x + kAdaptStep * k;
                 if (qx <= 0 || qx >= X - 1 || !fx[qx]) continue;
                 const uint8_t v = lab[i - x + qx];
-                lo = min(lo, v);
-                hi = max(hi, v == BMX_LABEL_NONE ? (uint8_t)0 : v);
+                lo = (int)v == none ? lo : min(lo, v);
+                hi = max(hi, (int)v == none ? (uint8_t)0 : v);
             }
         }
         out[i] = MinMax{lo, hi};
@@ -476,23 +505,31 @@ __global__ __launch_bounds__(256) void k_adapt_axis(const MinMax *__restrict__ i
 
 }  // namespace
 
-extern "C" int bmx_adaptive_mask(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
-                                 const uint8_t *flags, void *tmp, uint8_t *mask)
+extern "C" int bmx_adaptive_mask_lab(bmx_ctx *ctx, const uint8_t *lab, int label_none, int64_t Z, int64_t Y,
+                                     int64_t X, const uint8_t *flags, void *tmp, uint8_t *mask)
 {
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(lab && flags && tmp && mask, "bmx_adaptive_mask: null pointer");
+    BMX_CHECK(label_none >= 1 && label_none <= 256, "bmx_adaptive_mask: label_none %d not in [1,256]", label_none);
     BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_adaptive_mask: bad dims");
     const int64_t V = Z * Y * X;
     MinMax *a = (MinMax *)tmp, *b = a + V;
     const unsigned grid = bmx_stride_blocks(V, 256);
-    k_adapt_x<<<grid, 256, 0, ctx->stream>>>(lab, flags, flags + Z, flags + Z + Y, (int)Z, (int)Y, (int)X, a);
+    k_adapt_x<<<grid, 256, 0, ctx->stream>>>(lab, label_none, flags, flags + Z, flags + Z + Y, (int)Z, (int)Y, (int)X,
+                                             a);
     BMX_LAUNCH_CHECK();
     k_adapt_axis<false><<<grid, 256, 0, ctx->stream>>>(a, (int)Z, (int)Y, (int)X, 1, b, nullptr);
     BMX_LAUNCH_CHECK();
     k_adapt_axis<true><<<grid, 256, 0, ctx->stream>>>(b, (int)Z, (int)Y, (int)X, 2, nullptr, mask);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
+}
+
+extern "C" int bmx_adaptive_mask(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
+                                 const uint8_t *flags, void *tmp, uint8_t *mask)
+{
+    return bmx_adaptive_mask_lab(ctx, lab, BMX_LABEL_NONE, Z, Y, X, flags, tmp, mask);
 }
 
 extern "C" int bmx_select_seeds(bmx_ctx *ctx, const bmx_seed *seeds, int64_t nseeds, const uint8_t *mask,

@@ -445,7 +445,7 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(hits && values && out, "bmx_smooth_argmax: null pointer");
-    BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_smooth_argmax: nlabels %d not in [1,255]", nlabels);
+    BMX_CHECK(nlabels >= 1 && nlabels <= 256, "bmx_smooth_argmax: nlabels %d not in [1,256]", nlabels);
     BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_smooth_argmax: bad dims");
     BMX_CHECK(iters >= 1, "bmx_smooth_argmax: iters must be >= 1");
     const int64_t V = Z * Y * X;

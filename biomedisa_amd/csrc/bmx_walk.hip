@@ -581,7 +581,7 @@ extern "C" int bmx_walk_ex(bmx_ctx *ctx, const void *img, int img_dtype, int64_t
               img_dtype);
     BMX_CHECK(Z >= 3 && Y >= 3 && X >= 3 && Z * Y * X < (int64_t)1 << 32,
               "bmx_walk: bad dims %lld x %lld x %lld", (long long)Z, (long long)Y, (long long)X);
-    BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_walk: nlabels %d not in [1,255]", nlabels);
+    BMX_CHECK(nlabels >= 1 && nlabels <= 256, "bmx_walk: nlabels %d not in [1,256]", nlabels);
     BMX_CHECK(nbrw >= 0 && sorw >= 0, "bmx_walk: negative nbrw/sorw");
     BMX_CHECK(nseeds >= 0, "bmx_walk: negative nseeds");
     BMX_CHECK(rng == BMX_RNG_PHILOX || rng == BMX_RNG_REF, "bmx_walk: bad rng %d", rng);
@@ -671,7 +671,7 @@ extern "C" int bmx_walk_noalloc(bmx_ctx *ctx, const void *img, int img_dtype, in
               img_dtype);
     BMX_CHECK(Z >= 3 && Y >= 3 && X >= 3 && Z * Y * X < (int64_t)1 << 32,
               "bmx_walk_noalloc: bad dims %lld x %lld x %lld", (long long)Z, (long long)Y, (long long)X);
-    BMX_CHECK(nlabels >= 1 && nlabels <= 255, "bmx_walk_noalloc: nlabels %d not in [1,255]", nlabels);
+    BMX_CHECK(nlabels >= 1 && nlabels <= 256, "bmx_walk_noalloc: nlabels %d not in [1,256]", nlabels);
     BMX_CHECK(nbrw >= 0 && sorw >= 0 && nbrw <= 65535, "bmx_walk_noalloc: nbrw/sorw out of range");
     BMX_CHECK(nseeds >= 0, "bmx_walk_noalloc: negative nseeds");
     WalkGeom g{(int)Z, (int)Y, (int)X, (uint32_t)X, (uint32_t)(X * Y), Z * Y * X};
@@ -750,7 +750,7 @@ extern "C" int bmx_walk_batch(bmx_ctx *ctx, const void *img, int img_dtype, cons
         const bmx_volume &b = vols[v];
         BMX_CHECK(b.Z >= 3 && b.Y >= 3 && b.X >= 3, "bmx_walk_batch: volume %d: bad dims %d x %d x %d", v, b.Z,
                   b.Y, b.X);
-        BMX_CHECK(b.nlabels >= 1 && b.nlabels <= 255, "bmx_walk_batch: volume %d: nlabels %d not in [1,255]", v,
+        BMX_CHECK(b.nlabels >= 1 && b.nlabels <= 256, "bmx_walk_batch: volume %d: nlabels %d not in [1,256]", v,
                   b.nlabels);
         BMX_CHECK(b.offset >= img_end, "bmx_walk_batch: volume %d overlaps or precedes volume %d", v, v - 1);
         BMX_CHECK(b.seed_offset == seed_end, "bmx_walk_batch: volume %d: seeds must follow volume %d's", v, v - 1);

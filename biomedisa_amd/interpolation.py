@@ -73,6 +73,10 @@ class PipelineError(ValueError):
 
 # rw_small.py:147-149 and :181-183: an allocation failure while smoothing or
 # computing the uncertainty drops that result with a warning.
+# A label volume with all 256 values 0..255 holds no index for ignored
+# (negative) labels: the reference's short-label path (pyopencl_small.py:90-95)
+# needs one more value than a uint8 label index has.
+NEG_WITH_256_LABELS = "Negative (ignored) labels together with all 256 label values are not supported."
 SMOOTH_OOM_WARNING = "Warning: GPU out of memory to allocate smooth array. Process continues without smoothing."
 UNCERTAINTY_OOM_WARNING = ("Warning: GPU out of memory to allocate uncertainty array. Process continues without "
                            "uncertainty.")
@@ -122,24 +126,26 @@ def run_pipeline(vol, nlabels, values, nbrw=10, sorw=4000, allaxis=False, smooth
     eng = vol.eng
     Z, Y, X = vol.dims
     V = vol.V
-    if nlabels > 255:
-        raise PipelineError("at most 255 labels (including background) are supported")
+    if nlabels > 256:
+        raise PipelineError("at most 256 labels (including background) are supported")
+    # a volume of all 256 label values uses every index as a label (no ignored index)
+    lnone = _hip.LABEL_ALL if nlabels == 256 else _hip.LABEL_NONE
     t = timings if timings is not None else {}
     t0 = time.perf_counter()
     if path not in ("small", "large"):
         raise ValueError("path must be 'small' or 'large'")
     large = path == "large"
-    flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis)
+    flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis, label_none=lnone)
     flags = flags_buf.download((Z + Y + X,), np.uint8)
     if allaxis and adjacent_planes((flags[:Z], flags[Z:Z + Y], flags[Z + Y:])):
         raise PipelineError("At least one empty slice between labels required.")
     if large:
-        eng.plane_flags(vol.lab, vol.dims, allaxis, out=flags_buf, large=True)
+        eng.plane_flags(vol.lab, vol.dims, allaxis, out=flags_buf, large=True, label_none=lnone)
         flags = flags_buf.download((Z + Y + X,), np.uint8)
-    seeds, nseeds = eng.seeds(vol.lab, vol.img, vol.img_dtype, vol.dims, flags_buf)
+    seeds, nseeds = eng.seeds(vol.lab, vol.img, vol.img_dtype, vol.dims, flags_buf, label_none=lnone)
     nwalk = nseeds
     if large and nseeds:
-        mask = eng.adaptive_mask(vol.lab, vol.dims, flags_buf)
+        mask = eng.adaptive_mask(vol.lab, vol.dims, flags_buf, label_none=lnone)
         walkers = eng.alloc(max(1, nseeds) * _hip.SEED_DTYPE.itemsize)
         _, nwalk = eng.select_seeds(seeds, nseeds, mask, out=walkers)
         mask.free()
@@ -636,8 +642,8 @@ def _prepare_device(eng, data, labelData, nbrw, sorw, uncertainty, platform, all
         if labelData.dtype == np.uint32:
             # interpolation.py:145-147 (the values above 255 are already 0)
             print(f"Warning: Potential label loss during conversion from {labelData.dtype} to int32.")
-        if bm.nol > 255:
-            _error_(bm, "At most 255 labels (including background) are supported.")
+        if bm.nol > 256 or (bm.nol == 256 and bm.neg_kept):
+            _error_(bm, NEG_WITH_256_LABELS)
             return None
         box = box_from_class_boxes(boxes, bm.kept, bm.neg_kept, (Z, Y, X))
         dims_c = (box[1] - box[0], box[3] - box[2], box[5] - box[4])
@@ -742,8 +748,8 @@ def _prepare(data, labelData, nbrw, sorw, uncertainty, platform, allaxis, ignore
     lab_c = np.ascontiguousarray(lab[z0:z1, y0:y1, x0:x1])
     labidx = label_index_volume(lab_c, bm.allLabels)
     values = np.asarray(bm.allLabels, dtype=np.uint8)
-    if bm.nol > 255:
-        _error_(bm, "At most 255 labels (including background) are supported.")
+    if bm.nol > 256 or (bm.nol == 256 and np.any(lab_c < 0)):
+        _error_(bm, NEG_WITH_256_LABELS)
         return None
     return _Prepared(bm, img_c, lab_c, labidx, values, box)
 

@@ -12,7 +12,10 @@
  *     with bmx_malloc, fill with bmx_upload).  V = Z*Y*X must be < 2^32.
  *   - Label volumes hold a label INDEX per voxel (uint8): 0 = background,
  *     1..nlabels-1 = the sorted non-zero label values, BMX_LABEL_NONE (0xFF)
- *     = negative (ignored) label.  At most 255 label indices (0..254).
+ *     = negative (ignored) label: at most 255 label indices (0..254).  A
+ *     volume with all 256 label values 0..255 (and no negative label) uses
+ *     every index as a label, with the *_lab entry points' label_none = 256;
+ *     the walk, merge, smoothing and uncertainty take nlabels up to 256.
  *   - Images are uint8 (BMX_IMG_U8) or float32 in [0,255] (BMX_IMG_F32);
  *     the reference normalises every other dtype to the latter
  *     (biomedisa/interpolation.py:135-144).
@@ -168,6 +171,20 @@ int bmx_plane_flags_ex(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, i
  * Select the walking seeds with bmx_select_seeds. */
 int bmx_adaptive_mask(bmx_ctx *ctx, const uint8_t *lab, int64_t Z, int64_t Y, int64_t X,
                       const uint8_t *flags, void *tmp, uint8_t *mask);
+
+/* The three entry points above and bmx_seeds with the label volume's index
+ * for ignored voxels given: label_none = BMX_LABEL_NONE (what the others
+ * assume) or 256 when all 256 label indices are labels (a volume using every
+ * label value 0..255 and no negative label).  Replace the same reference code;
+ * the 256-label case is the reference's switch to short labels with -1 as
+ * ignore value (pyopencl_small.py:90-95). */
+int bmx_plane_flags_lab(bmx_ctx *ctx, const uint8_t *lab, int label_none, int64_t Z, int64_t Y, int64_t X,
+                        int allaxis, int large, uint8_t *flags);
+int bmx_adaptive_mask_lab(bmx_ctx *ctx, const uint8_t *lab, int label_none, int64_t Z, int64_t Y, int64_t X,
+                          const uint8_t *flags, void *tmp, uint8_t *mask);
+int bmx_seeds_lab(bmx_ctx *ctx, const uint8_t *lab, int label_none, const void *img, int img_dtype, int64_t Z,
+                  int64_t Y, int64_t X, const uint8_t *flags, bmx_seed *seeds, int64_t capacity,
+                  int64_t *nseeds);
 
 /* Seed extraction: replaces the label-volume rebuild of pyopencl_small.py:
  * 89-107 and the per-work-item seed test and _calc_var of :140-185, :205-228.

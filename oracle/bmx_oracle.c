@@ -37,7 +37,7 @@
 #include <omp.h>
 #endif
 
-#define ORC_IGN 0xFFu
+#define ORC_IGN 0xFFFFu   /* label index of ignored voxels (uint16 label volumes) */
 
 /* ------------------------------------------------------------------------ */
 /* Philox4x32-10: Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as   */
@@ -124,21 +124,22 @@ static inline float orc_img(const void *img, int dtype, int64_t i)
 /*   1-voxel-thin lines are removed first along the first in-plane axis, then */
 /*   along the second; a plane is labelled iff anything survives.  In-plane  */
 /*   axis order: plane z -> (y, x); plane y -> (z, x); plane x -> (z, y).    */
-/* labidx: label index per voxel, 0 = background, ORC_IGN = negative label;  */
-/* "non-zero label value" <=> labidx != 0.                                   */
+/* labidx: uint16 label index per voxel (up to 256 labels, the reference's  */
+/* short labels of pyopencl_small.py:90-95), 0 = background, ORC_IGN =      */
+/* negative label; "non-zero label value" <=> labidx != 0.                   */
 /* ------------------------------------------------------------------------ */
-static inline int orc_nz(const uint8_t *lab, int64_t i) { return lab[i] != 0; }
+static inline int orc_nz(const uint16_t *lab, int64_t i) { return lab[i] != 0; }
 
 /* ones after the first pass, at voxel (a,b) of a plane whose first in-plane  */
 /* axis has extent na and stride sa. */
-static inline int orc_pass1(const uint8_t *lab, int64_t base, int a, int na, int64_t sa)
+static inline int orc_pass1(const uint16_t *lab, int64_t base, int a, int na, int64_t sa)
 {
     if (!orc_nz(lab, base)) return 0;
     if (a > 0 && a < na - 1 && !orc_nz(lab, base - sa) && !orc_nz(lab, base + sa)) return 0;
     return 1;
 }
 
-static inline int orc_pass2(const uint8_t *lab, int64_t base, int a, int na, int64_t sa,
+static inline int orc_pass2(const uint16_t *lab, int64_t base, int a, int na, int64_t sa,
                             int b, int nb, int64_t sb)
 {
     if (!orc_pass1(lab, base, a, na, sa)) return 0;
@@ -147,7 +148,7 @@ static inline int orc_pass2(const uint8_t *lab, int64_t base, int a, int na, int
     return 1;
 }
 
-void orc_plane_flags(const uint8_t *lab, int Z, int Y, int X, int allaxis,
+void orc_plane_flags(const uint16_t *lab, int Z, int Y, int X, int allaxis,
                      uint8_t *fz, uint8_t *fy, uint8_t *fx)
 {
     const int64_t sy = X, sz = (int64_t)X * Y;
@@ -167,20 +168,20 @@ void orc_plane_flags(const uint8_t *lab, int Z, int Y, int X, int allaxis,
 /* Mask value of the volume the reference rebuilds from the labelled slices  */
 /* (pyopencl_small.py:97-107): label index inside a labelled plane, ORC_IGN  */
 /* elsewhere (negative labels are ORC_IGN already). */
-static inline uint8_t orc_mask(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
-                               const uint8_t *fx, int z, int y, int x, int64_t i)
+static inline uint16_t orc_mask(const uint16_t *lab, const uint8_t *fz, const uint8_t *fy,
+                                const uint8_t *fx, int z, int y, int x, int64_t i)
 {
-    return (fz[z] || fy[y] || fx[x]) ? lab[i] : (uint8_t)ORC_IGN;
+    return (fz[z] || fy[y] || fx[x]) ? lab[i] : (uint16_t)ORC_IGN;
 }
 
 /* _calc_var of pyopencl_small.py:140-185 for the seed at (z,y,x). */
-static float orc_calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *fy,
+static float orc_calc_var(const uint16_t *lab, const uint8_t *fz, const uint8_t *fy,
                           const uint8_t *fx, const void *img, int dtype,
                           int Y, int X, int z, int y, int x)
 {
     const int64_t sy = X, sz = (int64_t)X * Y;
     const int64_t i0 = z * sz + y * sy + x;
-    const uint8_t seg = orc_mask(lab, fz, fy, fx, z, y, x, i0);
+    const uint16_t seg = orc_mask(lab, fz, fy, fx, z, y, x, i0);
     const float B = orc_img(img, dtype, i0);
     float dev = 0.0f, summe = 0.0f;
     for (int n = -1; n < 2; ++n)            /* XY plane */
@@ -215,7 +216,7 @@ static float orc_calc_var(const uint8_t *lab, const uint8_t *fz, const uint8_t *
 /* Seeds: every interior voxel of a labelled plane whose label is not        */
 /* negative (pyopencl_small.py:205-207), in ascending voxel order.           */
 /* Returns the seed count; with pos == NULL only counts.                     */
-int64_t orc_seeds(const uint8_t *lab, const void *img, int dtype, int Z, int Y, int X,
+int64_t orc_seeds(const uint16_t *lab, const void *img, int dtype, int Z, int Y, int X,
                   const uint8_t *fz, const uint8_t *fy, const uint8_t *fx,
                   int64_t *pos, uint8_t *label, float *var)
 {
@@ -225,11 +226,11 @@ int64_t orc_seeds(const uint8_t *lab, const void *img, int dtype, int Z, int Y, 
         for (int y = 1; y < Y - 1; ++y)
             for (int x = 1; x < X - 1; ++x) {
                 int64_t i = z * sz + y * sy + x;
-                uint8_t m = orc_mask(lab, fz, fy, fx, z, y, x, i);
+                uint16_t m = orc_mask(lab, fz, fy, fx, z, y, x, i);
                 if (m == ORC_IGN) continue;
                 if (pos) {
                     pos[n] = i;
-                    label[n] = m;
+                    label[n] = (uint8_t)m;
                     var[n] = orc_calc_var(lab, fz, fy, fx, img, dtype, Y, X, z, y, x);
                 }
                 ++n;
@@ -529,7 +530,7 @@ void orc_uncertainty(const float *hits, int nlabels, int Z, int Y, int X, float 
  * -100..100 in steps of 5 along each axis and walks if an interior voxel there
  * holds a label of the seeds volume (labelled planes; ignore elsewhere) that is
  * neither its own nor the ignore value.  out[k] = 1 if seed k walks. */
-void orc_adaptive(const uint8_t *lab, int Z, int Y, int X, const uint8_t *fz, const uint8_t *fy,
+void orc_adaptive(const uint16_t *lab, int Z, int Y, int X, const uint8_t *fz, const uint8_t *fy,
                   const uint8_t *fx, const int64_t *pos, int64_t n, uint8_t *out)
 {
     const int64_t sy = X, sz = (int64_t)X * Y;
@@ -537,7 +538,7 @@ void orc_adaptive(const uint8_t *lab, int Z, int Y, int X, const uint8_t *fz, co
     for (int64_t k = 0; k < n; ++k) {
         const int64_t p = pos[k];
         const int plane = (int)(p / sz), row = (int)((p % sz) / sy), column = (int)(p % sy);
-        const uint8_t segment = orc_mask(lab, fz, fy, fx, plane, row, column, p);
+        const uint16_t segment = orc_mask(lab, fz, fy, fx, plane, row, column, p);
         int found = 0;
         if ((column + plane) % 2 == 0 && (row + plane) % 2 == 0) {
             found = 1;
@@ -548,7 +549,7 @@ void orc_adaptive(const uint8_t *lab, int Z, int Y, int X, const uint8_t *fz, co
                         if (plane + z > 0 && row + y > 0 && column + x > 0 && plane + z < Z - 1 &&
                             row + y < Y - 1 && column + x < X - 1) {
                             const int64_t q = (int64_t)(plane + z) * sz + (int64_t)(row + y) * sy + column + x;
-                            const uint8_t v = orc_mask(lab, fz, fy, fx, plane + z, row + y, column + x, q);
+                            const uint16_t v = orc_mask(lab, fz, fy, fx, plane + z, row + y, column + x, q);
                             if (v != segment && v != ORC_IGN) found = 1;
                         }
         }

@@ -95,9 +95,22 @@ def philox4x32_10(ctr, key):
     return out
 
 
+def _lab16(labidx):
+    """The oracle's uint16 label-index volume: uint16 as given (0xFFFF =
+    ignored), uint8 with 0xFF as its ignored index."""
+    labidx = np.asarray(labidx)
+    if labidx.dtype == np.uint16:
+        return np.ascontiguousarray(labidx)
+    lab8 = np.ascontiguousarray(labidx, dtype=np.uint8)
+    out = lab8.astype(np.uint16)
+    out[lab8 == 0xFF] = 0xFFFF
+    return out
+
+
 def plane_flags(labidx, allaxis):
-    """Labelled planes of a label-index volume (0 = background, IGN = negative)."""
-    labidx = np.ascontiguousarray(labidx, dtype=np.uint8)
+    """Labelled planes of a label-index volume (0 = background, ignored index:
+    0xFF for uint8, 0xFFFF for uint16 volumes)."""
+    labidx = _lab16(labidx)
     Z, Y, X = labidx.shape
     fz, fy, fx = np.zeros(Z, np.uint8), np.zeros(Y, np.uint8), np.zeros(X, np.uint8)
     lib().orc_plane_flags(_ptr(labidx), Z, Y, X, int(bool(allaxis)), _ptr(fz), _ptr(fy), _ptr(fx))
@@ -106,7 +119,7 @@ def plane_flags(labidx, allaxis):
 
 def seeds(labidx, img, flags):
     """Seed list (pos int64, label index uint8, var float32), ascending pos."""
-    labidx = np.ascontiguousarray(labidx, dtype=np.uint8)
+    labidx = _lab16(labidx)
     img = np.ascontiguousarray(img)
     Z, Y, X = labidx.shape
     fz, fy, fx = [np.ascontiguousarray(f, dtype=np.uint8) for f in flags]
@@ -167,7 +180,7 @@ def smooth_img(img):
 
 def adaptive(labidx, flags, pos):
     """The large-volume kernel's `found` test per seed (bool array)."""
-    labidx = np.ascontiguousarray(labidx, dtype=np.uint8)
+    labidx = _lab16(labidx)
     Z, Y, X = labidx.shape
     fz, fy, fx = [np.ascontiguousarray(f, dtype=np.uint8) for f in flags]
     pos = np.ascontiguousarray(pos, np.int64)

@@ -67,7 +67,9 @@ def prepare(data, labelData, denoise=False):
     z0, z1, y0, y1, x0, x1 = _bbox(labels)
     img_c = np.ascontiguousarray(img[z0:z1, y0:y1, x0:x1])
     lab_c = np.ascontiguousarray(labels[z0:z1, y0:y1, x0:x1])
-    labidx = np.full(lab_c.shape, 0xFF, np.uint8)
+    # uint16 label indices (0xFFFF: negative, ignored): the reference switches to
+    # short labels with -1 as ignore value for 256 labels (pyopencl_small.py:90-95)
+    labidx = np.full(lab_c.shape, 0xFFFF, np.uint16)
     for i, v in enumerate(vals):
         labidx[lab_c == v] = i
     return dict(img=img_c, labidx=labidx, lab=lab_c, values=vals.astype(np.uint8),

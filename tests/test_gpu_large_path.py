@@ -49,7 +49,7 @@ def test_plane_flags_large_bitexact(eng, allaxis):
 
 
 def _mask_vs_oracle(eng, labidx, flags, dims):
-    lb = eng.upload(labidx)
+    lb = eng.upload(np.asarray(labidx).astype(np.uint8))   # the device's uint8 indices (0xFF: ignored)
     fb = eng.upload(np.concatenate(flags).astype(np.uint8))
     mask = eng.adaptive_mask(lb, dims, fb).download(dims, np.uint8)
     pos, _, _ = orc.seeds(labidx, np.zeros(dims, np.uint8), flags)

@@ -151,14 +151,16 @@ def _bruteforce_seeds(labidx, img, flags):
     from tests.exact_f32 import fma
     Z, Y, X = labidx.shape
     fz, fy, fx = flags
-    sl = np.full(labidx.shape, 0xFF, np.int32)
+    # -1: no label (off the labelled planes, or ignored), as the short-label path's ignore value
+    ign = 0xFFFF if labidx.dtype == np.uint16 else oracle.IGN
+    lab = np.where(labidx == ign, -1, labidx.astype(np.int32))
+    sl = np.full(labidx.shape, -1, np.int32)
     for z in np.flatnonzero(fz):
-        sl[z] = labidx[z]
+        sl[z] = lab[z]
     for y in np.flatnonzero(fy):
-        sl[:, y] = labidx[:, y]
+        sl[:, y] = lab[:, y]
     for x in np.flatnonzero(fx):
-        sl[:, :, x] = labidx[:, :, x]
-    sl[sl == oracle.IGN] = 0xFF
+        sl[:, :, x] = lab[:, :, x]
     f = np.float32
     raw = img.astype(np.float32)
     out = []
@@ -166,7 +168,7 @@ def _bruteforce_seeds(labidx, img, flags):
         for y in range(1, Y - 1):
             for x in range(1, X - 1):
                 seg = sl[z, y, x]
-                if seg == 0xFF:
+                if seg == -1:
                     continue
                 B = raw[z, y, x]
                 dev, summe = f(0), f(0)
@@ -208,6 +210,33 @@ def test_seeds_vs_bruteforce(allaxis, dtype):
     ref = _bruteforce_seeds(lab, img, flags)
     for g, r in zip(got, ref):
         np.testing.assert_array_equal(g, r)
+
+
+@pytest.mark.parametrize("allaxis", [False, True])
+def test_seeds_256_labels_vs_bruteforce(allaxis):
+    """All 256 label values 0..255 (uint16 indices, no ignored index), as the
+    reference's short-label path holds them (pyopencl_small.py:90-95): label
+    255 seeds like any other and its variance counts only its own voxels."""
+    rng = np.random.default_rng(11 + allaxis)
+    shape = (20, 22, 24)
+    lab = np.zeros(shape, np.uint16)
+    planes = [(2, None, None), (6, None, None)] + ([(None, 11, None)] if allaxis else [])
+    for z, y, x in planes:
+        # every value on interior voxels of the plane
+        idx = (z if z is not None else slice(1, -1), y if y is not None else slice(1, -1), slice(1, -1))
+        n = lab[idx].size
+        vals = np.concatenate([rng.permutation(256), rng.integers(0, 256, n - 256)])
+        lab[idx] = vals.reshape(lab[idx].shape)
+    lab[2, 5:8, 5:9] = 255                  # a patch of label 255 with same-label neighbours
+    img = rng.integers(0, 256, shape).astype(np.uint8)
+    flags = orc.plane_flags(lab, allaxis)
+    got = orc.seeds(lab, img, flags)
+    ref = _bruteforce_seeds(lab, img, flags)
+    for g, r in zip(got, ref):
+        np.testing.assert_array_equal(g, r)
+    assert set(np.unique(got[1])) == set(range(256))
+    sel = got[1] == 255
+    assert sel.sum() >= 12 and np.any(got[2][sel] > 1.0)
 
 
 def test_variance_closed_forms():
