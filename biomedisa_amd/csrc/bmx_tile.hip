@@ -897,6 +897,58 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     return BMX_OK;
 }
 
+// Walker indices, the live count and the round buffers of one walk are 32-bit.
+// A single volume with 2^32 walkers or more is walked in balanced chunks of
+// whole seeds, one after the other, into the same hits and move counter.  A
+// walker's stream is keyed by its seed's voxel, walk and step, never by its
+// index, so the chunks' hits add up to those of one walk over every seed.
+// The chunks also bound the round buffers (32 bytes per walker) at 2^32
+// walkers.  BMX_WALK_CHUNK=n lowers the chunk size to n walkers, so that
+// chunked and unchunked walks can be compared on small volumes.
+template <typename T, int S>
+int walk_tiled_chunked(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, const bmx_seed *seeds,
+                       int64_t nseeds, int nlabels, int nbrw, int sorw, int cap, uint32_t weight, int overrides,
+                       uint32_t k0, uint32_t k1, uint32_t *hits, unsigned long long *acc, int graph_pairs,
+                       hipEvent_t *ev, int max_timed, int *launches, float *kernel_ms)
+{
+    int64_t lim = ((int64_t)1 << 32) - 1;
+    const char *env = getenv("BMX_WALK_CHUNK");
+    if (env && *env) {
+        const long long v = atoll(env);
+        if (v > 0 && v < lim) lim = v;
+    }
+    const int64_t n0 = nseeds * (int64_t)nbrw;
+    if (nvol != 1 || nbrw <= 0 || n0 <= lim)
+        return walk_tiled_t<T, S>(ctx, img, vols, nvol, seeds, nseeds, nlabels, nbrw, sorw, cap, weight, overrides,
+                                  k0, k1, hits, acc, graph_pairs, ev, max_timed, launches, kernel_ms);
+    const int64_t per = std::max<int64_t>(1, lim / nbrw);
+    const int64_t nchunks = (nseeds + per - 1) / per;
+    const int64_t chunk = (nseeds + nchunks - 1) / nchunks;
+    int total = 0;
+    bool unknown = false, host = false;
+    float ms_sum = 0.0f;
+    for (int64_t s0 = 0; s0 < nseeds; s0 += chunk) {
+        const int64_t cnt = std::min(chunk, nseeds - s0);
+        bmx_volume v = vols[0];
+        v.seed_offset = 0;
+        v.nseeds = (uint64_t)cnt;
+        const int used = std::min(total, max_timed);
+        int l = 0;
+        float ms = -1.0f;
+        const int rc = walk_tiled_t<T, S>(ctx, img, &v, 1, seeds + s0, cnt, nlabels, nbrw, sorw, cap, weight,
+                                          overrides, k0, k1, hits, acc, graph_pairs, ev + 2 * used,
+                                          max_timed - used, &l, &ms);
+        if (rc) return rc;
+        total += l;
+        if (ms == -2.0f) unknown = true;
+        else if (ms >= 0.0f) ms_sum += ms;
+        else host = true;
+    }
+    *launches = total;
+    *kernel_ms = unknown ? -2.0f : (host ? -1.0f : ms_sum);
+    return BMX_OK;
+}
+
 }  // namespace
 
 extern "C" int bmx_seed_walks(bmx_ctx *ctx, bmx_seed *seeds, int64_t n, int walks, int weight)
@@ -927,9 +979,9 @@ int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volum
               BMX_TILE_SHAPES);
 #define BMX_TILED_CASE(TT, SS)                                                                              \
     case SS:                                                                                                \
-        return walk_tiled_t<TT, SS>(ctx, (const TT *)img, vols, nvol, seeds, nseeds, nlabels, nbrw, sorw, cap, \
-                                    weight, overrides, k0, k1, hits, acc, graph_pairs, ev, max_timed, launches, \
-                                    kernel_ms);
+        return walk_tiled_chunked<TT, SS>(ctx, (const TT *)img, vols, nvol, seeds, nseeds, nlabels, nbrw, sorw, \
+                                          cap, weight, overrides, k0, k1, hits, acc, graph_pairs, ev, max_timed, \
+                                          launches, kernel_ms);
     if (img_dtype == BMX_IMG_U8) {
         switch (shape) {
             BMX_TILED_CASE(uint8_t, 0)

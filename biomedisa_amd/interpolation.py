@@ -221,7 +221,8 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     plain_above: when more than this fraction of the seeds falls in the near
     set (the proposal is uncertain almost everywhere, as on volumes unlike its
     training data), the plain pipeline runs instead: its result is exact and
-    costs little more (None: never).
+    costs little more (None: never).  So does a volume of more labels than the
+    network predicts (surrogate.L_MAX).
     walk_opts: as for run_pipeline.  The result takes the walker's argmax on U and
     the network's label elsewhere ('smooth': the smoothed walker argmax on U).
     Returns the dict of run_pipeline plus 'walked_seeds', 'ring_seeds' (of
@@ -232,8 +233,6 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     eng = vol.eng
     Z, Y, X = vol.dims
     V = vol.V
-    if nlabels > S.L_MAX:
-        raise PipelineError("guided mode supports at most %d labels" % S.L_MAX)
     if far > srad and (far_walks < 1 or nbrw % far_walks):
         raise PipelineError("far_walks must divide nbrw")
     if mid_walks is None:
@@ -243,6 +242,14 @@ def run_pipeline_guided(vol, nlabels, values, net, nbrw=10, sorw=4000, allaxis=F
     if mid is not None and (mid_walks < 1 or nbrw % mid_walks):
         raise PipelineError("mid_walks must divide nbrw")
     t = timings if timings is not None else {}
+    if nlabels > S.L_MAX:
+        # the network's head predicts at most L_MAX labels: the plain walk
+        # labels the volume instead (its result is exact)
+        out = run_pipeline(vol, nlabels, values, nbrw, sorw, allaxis, smooth, key=key, keep_hits=keep,
+                           walk_opts=walk_opts, timings=t)
+        out.update(ring_seeds=0, mid_ring_seeds=0, repair_seeds=0, repair_ring_seeds=0, repair_voxels=0,
+                   fallback=True)
+        return out
     t0 = time.perf_counter()
     flags_buf = eng.plane_flags(vol.lab, vol.dims, allaxis)
     flags = flags_buf.download((Z + Y + X,), np.uint8)

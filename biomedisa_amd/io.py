@@ -678,6 +678,12 @@ def _nc_files(path):
     return sorted(names)
 
 
+def _is_nc_folder(path):
+    """A folder is read as NetCDF blocks when it holds a .nc or .nc.bz2 file
+    (biomedisa_helper.py:527-536); a stray .bz2 file beside slices is not one."""
+    return os.path.isdir(path) and any(n.endswith((".nc", ".nc.bz2")) for _, _, fs in os.walk(path) for n in fs)
+
+
 def _nc_file_bytes(p):
     import bz2
     with open(p, "rb") as f:
@@ -1267,7 +1273,7 @@ def read_header(path):
         return am_header(path)
     if ext == ".zarr":
         return None
-    if ext == ".nc" or (os.path.isdir(path) and _nc_files(path)):
+    if ext == ".nc" or _is_nc_folder(path):
         return nc_header(path)
     if os.path.isdir(path) or ext == ".zip":
         # the names of every slice and the sample type of the first
@@ -1298,7 +1304,7 @@ def load_data(path):
         return None, ext
     if ext == ".zarr":
         return _read_zarr(path), ext
-    if ext == ".nc" or (os.path.isdir(path) and _nc_files(path)):
+    if ext == ".nc" or _is_nc_folder(path):
         return _read_nc(path), "" if os.path.isdir(path) else ext
     if os.path.isdir(path):
         return _read_slices(path)[0], ""

@@ -206,7 +206,10 @@ int bmx_select_seeds(bmx_ctx *ctx, const bmx_seed *seeds, int64_t nseeds, const 
  * the randomWalk kernel :192-314.  hits (device, uint32 [nlabels][V]) is
  * ACCUMULATED into (zero it first with bmx_zero).  rng selects the stream;
  * key is the Philox key (ignored for BMX_RNG_REF).  *accepted (may be NULL)
- * receives the number of accepted moves = the hits added. */
+ * receives the number of accepted moves = the hits added.  The tiled walker
+ * (chosen automatically from 2^16 walkers up) walks 2^32 walkers (nseeds *
+ * nbrw) or more in chunks of whole seeds, with the same hits; the sorted
+ * walker refuses them. */
 int bmx_walk(bmx_ctx *ctx, const void *img, int img_dtype, int64_t Z, int64_t Y, int64_t X,
              const bmx_seed *seeds, int64_t nseeds, int nlabels, int nbrw, int sorw,
              int rng, uint64_t key, uint32_t *hits, uint64_t *accepted);

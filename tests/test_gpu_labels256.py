@@ -53,3 +53,16 @@ def test_256_labels_with_negative_rejected(capsys):
     got = I.smart_interpolation(img, lab, nbrw=2, sorw=50)
     assert got is None
     assert I.NEG_WITH_256_LABELS in capsys.readouterr().out
+
+
+def test_guided_method_with_256_labels_is_plain():
+    """More labels than the proposal network predicts: method='guided' returns
+    the plain result, bit for bit."""
+    img, lab = volume_256(seed=2)
+    kw = dict(nbrw=2, sorw=100, smooth=3)
+    print("256 labels, guided method", flush=True)
+    plain = I.smart_interpolation(img, lab, **kw)
+    guided = I.smart_interpolation(img, lab, method="guided", **kw)
+    assert plain is not None and guided is not None
+    for key in ("regular", "smooth"):
+        np.testing.assert_array_equal(guided[key], plain[key], err_msg=key)

@@ -112,12 +112,24 @@ def test_guided_parameter_checks(kw):
         run_pipeline_guided(NoDevice(), 2, np.arange(2, dtype=np.uint8), None, **args)
 
 
-def test_guided_label_limit():
+def test_guided_label_limit_falls_back_to_plain(monkeypatch):
+    """More labels than the network's head predicts: the guided method runs the
+    plain pipeline with the same walk arguments and marks the result as a
+    fallback, before any device work of its own."""
+    from biomedisa_amd import interpolation as I
     from biomedisa_amd import surrogate as S
-    from biomedisa_amd.interpolation import PipelineError, run_pipeline_guided
 
     class NoDevice:
         eng, dims, V = None, (4, 4, 4), 64
 
-    with pytest.raises(PipelineError, match="at most"):
-        run_pipeline_guided(NoDevice(), S.L_MAX + 1, np.arange(S.L_MAX + 1, dtype=np.uint8), None)
+    calls = []
+
+    def plain(vol, nlabels, values, nbrw, sorw, allaxis, smooth, **kw):
+        calls.append((nlabels, nbrw, sorw, allaxis, smooth, kw["key"]))
+        return {"nseeds": 7, "walked_seeds": 7}
+
+    monkeypatch.setattr(I, "run_pipeline", plain)
+    out = I.run_pipeline_guided(NoDevice(), S.L_MAX + 1, np.arange(S.L_MAX + 1, dtype=np.uint8), None,
+                                nbrw=10, sorw=50, allaxis=True, smooth=3, key=9)
+    assert calls == [(S.L_MAX + 1, 10, 50, True, 3, 9)]
+    assert out["fallback"] is True and out["walked_seeds"] == 7 and out["ring_seeds"] == 0

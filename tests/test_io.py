@@ -5,6 +5,7 @@ of line, BigTIFF; NRRD with gzip and comments), the result file names of
 interpolation.py:115-133, and the reference's command-line defaults.  AmiraMesh
 (HxByteRLE checked against a decoder written from its definition), NIfTI-1 and
 MetaImage files likewise."""
+import bz2
 import gzip
 import os
 import struct
@@ -854,6 +855,22 @@ def test_netcdf_block_folder_round_trip(tmp_path):
     bio.save_data(str(out), got, header=hdr, file_type="")
     assert sorted(os.listdir(out)) == ["block_10.nc", "block_2.nc"]
     np.testing.assert_array_equal(bio.load_data(str(out))[0], got)
+
+
+def test_slice_folder_with_stray_bz2_is_not_netcdf(tmp_path):
+    """A folder of slices that also holds an unrelated .bz2 file is read as
+    slices: only .nc and .nc.bz2 names make a NetCDF block folder."""
+    arr = (np.random.default_rng(9).random((3, 5, 6)) * 255).astype(np.uint8)
+    names = ["s_%d.png" % k for k in range(3)]
+    d = tmp_path / "stack"
+    bio.save_data(str(d), arr, header={"slices": names, "dtype": np.uint8}, file_type="")
+    (d / "notes.txt.bz2").write_bytes(bz2.compress(b"not a volume"))
+    got, ext = bio.load_data(str(d))
+    assert ext == ""
+    np.testing.assert_array_equal(got, arr)
+    assert bio.read_header(str(d))["slices"] == names
+    (d / "block.nc.bz2").write_bytes(bz2.compress(bio._nc_bytes(arr, "labels")))
+    assert bio._is_nc_folder(str(d))
 
 
 def test_netcdf_record_variable():
