@@ -33,7 +33,7 @@ class WalkOpts(C.Structure):
                 ("hit_weight", C.c_int), ("seed_overrides", C.c_int), ("tile_shape", C.c_int),
                 ("graph_rounds", C.c_int)]
 
-TILE_SHAPES = 4   # BMX_TILE_SHAPES
+TILE_SHAPES = 6   # BMX_TILE_SHAPES
 
 # bmx_volume: one volume of a batched walk.
 VOLUME_DTYPE = np.dtype([("offset", "<u8"), ("Z", "<i4"), ("Y", "<i4"), ("X", "<i4"), ("nlabels", "<i4"),
@@ -45,6 +45,7 @@ SEED_DTYPE = np.dtype([("pos", "<u4"), ("label", "<u4"), ("var", "<f4"), ("reser
 _p, _i64, _i = C.c_void_p, C.c_int64, C.c_int
 SIGNATURES = {
     "bmx_last_error": (C.c_char_p, []),
+    "bmx_tile_shape_info": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "bmx_version": (_i, []),
     "bmx_open": (_i, [_i, C.POINTER(_p)]),
     "bmx_close": (_i, [_p]),
@@ -448,6 +449,14 @@ def ptr(buf):
 def call(name, *args):
     """Call a bmx_* entry point with a context as first argument; raise on error."""
     _check(getattr(load(), name)(*args))
+
+
+def tile_shape_info(img_dtype, shape):
+    """Geometry of a tiled-walker shape (bmx_tile_shape_info) as a dict: core,
+    halo, threads, hit_bytes (2 = 16-bit LDS counters), lds_bytes."""
+    vals = [C.c_int() for _ in range(5)]
+    _check(load().bmx_tile_shape_info(int(img_dtype), int(shape), *[C.byref(v) for v in vals]))
+    return dict(zip(("core", "halo", "threads", "hit_bytes", "lds_bytes"), (v.value for v in vals)))
 
 
 _engines = {}

@@ -25,31 +25,60 @@
 
 namespace {
 
-// Tile shapes: core side C, halo H and workgroup size NTH.  Shape 0 is the
-// default (one 135 KB / 140 KB workgroup per CU); the others trade fewer
-// voxels per round for more resident workgroups, which can pay on small
-// volumes.  BMX_TILE_SHAPES of them are compiled; bmx_walk_opts.tile_shape
-// picks one (the walk autotuner times them).
+// Tile shapes: core side C, halo H, workgroup size NTH, bytes per LDS hit
+// counter HB and the waves per SIMD the kernel is built for (WPE).
+// BMX_TILE_SHAPES of them are compiled; bmx_walk_opts.tile_shape picks one
+// (the walk autotuner times them).
+//  - HB = 4: one uint32 counter per tile voxel.  A u8 16/6 tile of this kind
+//    takes 135 KB, so one workgroup per CU.
+//  - HB = 2: two 16-bit counters per 32-bit LDS word (voxels 2k and 2k+1 of
+//    the tile).  The u8 16/6 tile then takes 81 KB and two workgroups share a
+//    CU: one bucket's last walkers, its tile load and its flush overlap the
+//    other workgroup's walk.  Counter wraps are accounted for exactly (see
+//    hit16_wrap); a walk whose per-hit weight can reach 2^16 takes an HB = 4
+//    shape instead (Hit32Sibling, bmx_walk_tiled).
+// Shape 0 is the default.  u8: 16/6 x 768 threads with 16-bit counters (the
+// 512^3 plain walk kernel 4.49 s -> 4.08 s against shape 1, the guided walk
+// 1.10 s -> 0.94 s).  float32 (4-byte image voxels): 16/4 x 1024 with 32-bit
+// counters; the two-per-CU float tiles need a halo of 2-3 and were not faster
+// (384^3, 12-bit image).  The smaller cores trade fewer voxels per round for
+// more resident workgroups, which can pay on small volumes.
 template <typename T, int S> struct TileShape;
-template <> struct TileShape<uint8_t, 0> { static constexpr int C = 16, H = 6, NTH = 1024; };
-template <> struct TileShape<uint8_t, 1> { static constexpr int C = 12, H = 5, NTH = 1024; };
-template <> struct TileShape<uint8_t, 2> { static constexpr int C = 12, H = 5, NTH = 512; };
-template <> struct TileShape<uint8_t, 3> { static constexpr int C = 8, H = 4, NTH = 512; };
-template <> struct TileShape<float, 0> { static constexpr int C = 16, H = 4, NTH = 1024; };
-template <> struct TileShape<float, 1> { static constexpr int C = 12, H = 3, NTH = 1024; };
-template <> struct TileShape<float, 2> { static constexpr int C = 12, H = 3, NTH = 512; };
-template <> struct TileShape<float, 3> { static constexpr int C = 8, H = 3, NTH = 512; };
-static_assert(BMX_TILE_SHAPES == 4, "one TileShape per shape index");
+template <> struct TileShape<uint8_t, 0> { static constexpr int C = 16, H = 6, NTH = 768, HB = 2, WPE = 6; };
+template <> struct TileShape<uint8_t, 1> { static constexpr int C = 16, H = 6, NTH = 1024, HB = 4, WPE = 4; };
+template <> struct TileShape<uint8_t, 2> { static constexpr int C = 16, H = 4, NTH = 512, HB = 2, WPE = 6; };
+template <> struct TileShape<uint8_t, 3> { static constexpr int C = 12, H = 5, NTH = 1024, HB = 4, WPE = 4; };
+template <> struct TileShape<uint8_t, 4> { static constexpr int C = 8, H = 4, NTH = 512, HB = 4, WPE = 2; };
+template <> struct TileShape<uint8_t, 5> { static constexpr int C = 16, H = 6, NTH = 1024, HB = 2, WPE = 8; };
+template <> struct TileShape<float, 0> { static constexpr int C = 16, H = 4, NTH = 1024, HB = 4, WPE = 4; };
+template <> struct TileShape<float, 1> { static constexpr int C = 14, H = 3, NTH = 768, HB = 2, WPE = 6; };
+template <> struct TileShape<float, 2> { static constexpr int C = 16, H = 2, NTH = 768, HB = 2, WPE = 6; };
+template <> struct TileShape<float, 3> { static constexpr int C = 12, H = 3, NTH = 1024, HB = 4, WPE = 4; };
+template <> struct TileShape<float, 4> { static constexpr int C = 8, H = 3, NTH = 512, HB = 4, WPE = 2; };
+template <> struct TileShape<float, 5> { static constexpr int C = 12, H = 3, NTH = 512, HB = 4, WPE = 2; };
+static_assert(BMX_TILE_SHAPES == 6, "one TileShape per shape index");
 
 template <typename T, int S> constexpr int tile_RI() { return TileShape<T, S>::C + 2 * TileShape<T, S>::H + 2; }
 template <typename T, int S> constexpr size_t tile_img_bytes()
 {
     return ((size_t)tile_RI<T, S>() * tile_RI<T, S>() * tile_RI<T, S>() * sizeof(T) + 15) & ~(size_t)15;
 }
-template <typename T, int S> constexpr size_t tile_lds_bytes()
+template <typename T, int S> constexpr size_t tile_hit_bytes()
 {
-    return tile_img_bytes<T, S>() + (size_t)tile_RI<T, S>() * tile_RI<T, S>() * tile_RI<T, S>() * sizeof(uint32_t);
+    return (size_t)tile_RI<T, S>() * tile_RI<T, S>() * tile_RI<T, S>() * TileShape<T, S>::HB;
 }
+template <typename T, int S> constexpr size_t tile_lds_bytes() { return tile_img_bytes<T, S>() + tile_hit_bytes<T, S>(); }
+
+// A walk whose hits add at most this much per accepted move may use 16-bit
+// LDS counters: one add then wraps a counter at most once.
+constexpr uint32_t kHit16MaxWeight = 0xFFFFu;
+
+// An HB = 4 shape to take instead of shape S when the walk's hit weight may
+// reach 2^16 (S itself when S has HB = 4).
+template <typename T, int S> struct Hit32Sibling {
+    static constexpr int value = TileShape<T, S>::HB == 4 ? S : (sizeof(T) == 1 ? 1 : 0);
+};
+static_assert(TileShape<uint8_t, 1>::HB == 4 && TileShape<float, 0>::HB == 4, "the 32-bit counter fallbacks");
 
 // One volume of a walk (a batch walks many at once): voxels [base, base + V)
 // of the concatenated image, z-major with x fastest; tiles [tile_base,
@@ -324,6 +353,30 @@ __device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
     return bmx_weight(B, A, div1);
 }
 
+// A 16-bit LDS hit counter wrapped.  LDS word k holds the counters of tile
+// voxels 2k (bits 0-15, "low") and 2k+1 (bits 16-31, "high"), and every
+// change to a word is an atomic add of w (low) or w << 16 (high), w < 2^16,
+// so the word always equals low_total + 2^16 * high_total mod 2^32.  The
+// add that wraps a field is known from the value the atomic returns:
+//  - a low add wraps when old_low + w >= 2^16: the low total has gained
+//    2^16, and its carry has added 1 to the high field, which itself wraps
+//    out of the word when old_high == 0xFFFF;
+//  - a high add wraps when old_high + w >= 2^16 (the high total gains 2^16).
+// Each wrap is seen by exactly one add, whatever the order of the atomics, so
+// the lane that sees it settles it in the global hits at once: +2^16 for the
+// field that wrapped, and for a low wrap -1 on the high voxel (its carry) plus
+// 2^16 when that carry wrapped the high field.  The flush then adds the two
+// fields as they stand.  Global counts are sums mod 2^32, so the order of
+// these adds does not matter.  A word changes only when a walker lands on
+// one of its voxels, an interior voxel, and voxels 2k and 2k+1 lie in the
+// same row of the tile (its side is even): both are voxels of the volume.
+// gi: global index of the voxel the walker landed on.
+__device__ __noinline__ void hit16_wrap(uint32_t *hg, int64_t gi, bool high, uint32_t old)
+{
+    atomicAdd(&hg[gi], 0x10000u);
+    if (!high) atomicAdd(&hg[gi + 1], (old >> 16) == 0xFFFFu ? 0xFFFFu : 0xFFFFFFFFu);
+}
+
 // The steps of one Philox block (4 steps) that one walker takes in this
 // tile.  Local coordinates (lz,ly,lx) address the LDS tile, whose voxel
 // (1,1,1) is the region origin; the region proper is [1,R]^3 and the tile adds
@@ -340,9 +393,11 @@ template <typename T, int S, bool EDGE>
 __device__ __forceinline__ void tile_block(const T *simg, uint32_t *shits, int sorw, const uint32_t rr[4],
                                            float B, float div1, int &step, int &li, int &lz, int &ly, int &lx,
                                            int lmin0, int lmin1, int lmin2, int lmax0, int lmax1, int lmax2,
-                                           uint32_t weight, uint32_t &acc)
+                                           uint32_t weight, uint32_t &acc, uint32_t *hg, int64_t g0,
+                                           uint32_t gsz, uint32_t gsy)
 {
     constexpr int R = TileShape<T, S>::C + 2 * TileShape<T, S>::H, RI = R + 2;
+    static_assert(TileShape<T, S>::HB == 4 || RI % 2 == 0, "16-bit counters pair voxels of one tile row");
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const bool in = max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
@@ -372,7 +427,14 @@ __device__ __forceinline__ void tile_block(const T *simg, uint32_t *shits, int s
                 const int lmag = axis == 0 ? RI * RI : (axis == 1 ? RI : 1);
                 li += sgn * lmag;
                 ++acc;
-                atomicAdd(&shits[li], weight);
+                if (TileShape<T, S>::HB == 4) {
+                    atomicAdd(&shits[li], weight);
+                } else {
+                    const uint32_t sh = ((uint32_t)li & 1u) << 4;
+                    const uint32_t old = atomicAdd(&shits[li >> 1], weight << sh);
+                    if (((old >> sh) & 0xFFFFu) + weight > 0xFFFFu)
+                        hit16_wrap(hg, g0 + (int64_t)lz * gsz + (int64_t)ly * gsy + lx, sh != 0u, old);
+                }
             }
             ++step;
         }
@@ -443,7 +505,7 @@ __device__ __forceinline__ void tiny_walker(const T *__restrict__ img, const Wal
 // values live across buckets (about 80 VGPRs against 44), which costs nothing
 // where LDS already limits a CU to one workgroup.
 template <typename T, int S, bool PERSIST>
-__global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
+__global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) void k_tile_walk(
     const T *__restrict__ img_all, TileGeom tg, const WalkSeed *__restrict__ wseeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
     const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ tkey, const uint32_t *__restrict__ toff,
@@ -514,9 +576,10 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
                     if (k < NT) simg[k] = v[q];
                 }
             }
+            constexpr int NW = (int)(tile_hit_bytes<T, S>() / 4);   // 32-bit counter words
             uint4 *sh4 = (uint4 *)shits;
-            for (int k = tid; k < NT / 4; k += NTH) sh4[k] = make_uint4(0u, 0u, 0u, 0u);
-            for (int k = (NT / 4) * 4 + tid; k < NT; k += NTH) shits[k] = 0u;
+            for (int k = tid; k < NW / 4; k += NTH) sh4[k] = make_uint4(0u, 0u, 0u, 0u);
+            for (int k = (NW / 4) * 4 + tid; k < NW; k += NTH) shits[k] = 0u;
         }
         __syncthreads();
 
@@ -529,6 +592,8 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
                           (lmax1 < R + 4) | (lmax2 < R + 4);
 
         uint32_t *__restrict__ hg = hits + gv.hits_base + (int64_t)label * gv.V;
+        // global index of tile voxel (0,0,0) (negative at the volume's low faces)
+        const int64_t g0 = (int64_t)bz * g.sz + (int64_t)by * g.sy + bx;
         uint32_t acc = 0;  // accepted moves of this thread in this bucket (< cnt * cap < 2^32)
         bool have = false;
         uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = 0;
@@ -563,10 +628,10 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
             const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
             if (edge)
                 tile_block<T, S, true>(simg, shits, sorw, rr, B, div1, step, li, lz, ly, lx, lmin0, lmin1, lmin2,
-                                       lmax0, lmax1, lmax2, weight, acc);
+                                       lmax0, lmax1, lmax2, weight, acc, hg, g0, g.sz, g.sy);
             else
                 tile_block<T, S, false>(simg, shits, sorw, rr, B, div1, step, li, lz, ly, lx, lmin0, lmin1, lmin2,
-                                        lmax0, lmax1, lmax2, weight, acc);
+                                        lmax0, lmax1, lmax2, weight, acc, hg, g0, g.sz, g.sy);
             const bool inside =
                 max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
             if (step >= sorw || !inside || step - step0 >= cap) {
@@ -578,12 +643,26 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH)) void k_tile_walk(
         }
         wave_add_u64(accepted, (uint64_t)acc);
         __syncthreads();
-        for (int k = tid; k < NT; k += NTH) {
-            const uint32_t c = shits[k];
-            if (c) {
-                const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
-                const int64_t gi = (int64_t)(bz + lz2) * g.sz + (int64_t)(by + ly2) * g.sy + (bx + lx2);
-                atomicAdd(&hg[gi], c);
+        if (Sh::HB == 4) {
+            for (int k = tid; k < NT; k += NTH) {
+                const uint32_t c = shits[k];
+                if (c) {
+                    const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
+                    atomicAdd(&hg[g0 + (int64_t)lz2 * g.sz + (int64_t)ly2 * g.sy + lx2], c);
+                }
+            }
+        } else {
+            // word k: voxels 2k (low half) and 2k+1 (high half) of one tile row,
+            // both in the volume when the word is not 0 (hit16_wrap)
+            for (int k = tid; k < NT / 2; k += NTH) {
+                const uint32_t c = shits[k];
+                if (c) {
+                    const int v = 2 * k;
+                    const int lz2 = v / (RI * RI), ly2 = (v / RI) % RI, lx2 = v % RI;
+                    const int64_t gi = g0 + (int64_t)lz2 * g.sz + (int64_t)ly2 * g.sy + lx2;
+                    if (c & 0xFFFFu) atomicAdd(&hg[gi], c & 0xFFFFu);
+                    if (c >> 16) atomicAdd(&hg[gi + 1], c >> 16);
+                }
             }
         }
         if (!PERSIST) break;
@@ -967,6 +1046,41 @@ extern "C" int bmx_seed_walks(bmx_ctx *ctx, bmx_seed *seeds, int64_t n, int walk
     return BMX_OK;
 }
 
+template <typename T, int S>
+static void shape_info(int *core, int *halo, int *threads, int *hit_bytes, int *lds_bytes)
+{
+    *core = TileShape<T, S>::C;
+    *halo = TileShape<T, S>::H;
+    *threads = TileShape<T, S>::NTH;
+    *hit_bytes = TileShape<T, S>::HB;
+    *lds_bytes = (int)tile_lds_bytes<T, S>();
+}
+
+extern "C" int bmx_tile_shape_info(int img_dtype, int shape, int *core, int *halo, int *threads, int *hit_bytes,
+                                   int *lds_bytes)
+{
+    BMX_CHECK(core && halo && threads && hit_bytes && lds_bytes, "bmx_tile_shape_info: null pointer");
+    BMX_CHECK(img_dtype == BMX_IMG_U8 || img_dtype == BMX_IMG_F32, "bmx_tile_shape_info: bad image dtype %d",
+              img_dtype);
+    BMX_CHECK(shape >= 0 && shape < BMX_TILE_SHAPES, "bmx_tile_shape_info: tile_shape %d not in [0,%d)", shape,
+              BMX_TILE_SHAPES);
+#define BMX_INFO_CASE(TT, SS) \
+    case SS: shape_info<TT, SS>(core, halo, threads, hit_bytes, lds_bytes); return BMX_OK;
+    if (img_dtype == BMX_IMG_U8) {
+        switch (shape) {
+            BMX_INFO_CASE(uint8_t, 0) BMX_INFO_CASE(uint8_t, 1) BMX_INFO_CASE(uint8_t, 2)
+            BMX_INFO_CASE(uint8_t, 3) BMX_INFO_CASE(uint8_t, 4) BMX_INFO_CASE(uint8_t, 5)
+        }
+    } else {
+        switch (shape) {
+            BMX_INFO_CASE(float, 0) BMX_INFO_CASE(float, 1) BMX_INFO_CASE(float, 2)
+            BMX_INFO_CASE(float, 3) BMX_INFO_CASE(float, 4) BMX_INFO_CASE(float, 5)
+        }
+    }
+#undef BMX_INFO_CASE
+    return bmx_set_error(BMX_E_INVALID, "bmx_tile_shape_info: bad tile shape");
+}
+
 // Entry from bmx_walk_ex / bmx_walk_batch (bmx_walk.hip).  A single volume is a
 // one-entry table.
 int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volume *vols, int nvol,
@@ -977,17 +1091,27 @@ int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volum
 {
     BMX_CHECK(shape >= 0 && shape < BMX_TILE_SHAPES, "bmx_walk: tile_shape %d not in [0,%d)", shape,
               BMX_TILE_SHAPES);
-#define BMX_TILED_CASE(TT, SS)                                                                              \
-    case SS:                                                                                                \
+    // per-hit weight: hit_weight, times a seed's own weight (< 2^16) with overrides
+    const bool hit16 = overrides ? weight <= 1u : weight <= kHit16MaxWeight;
+#define BMX_TILED_CASE(TT, SS)                                                                                \
+    case SS: {                                                                                                \
+        constexpr int S2 = TileShape<TT, SS>::HB == 2 ? Hit32Sibling<TT, SS>::value : SS;                     \
+        if (!hit16 && S2 != SS)                                                                               \
+            return walk_tiled_chunked<TT, S2>(ctx, (const TT *)img, vols, nvol, seeds, nseeds, nlabels, nbrw, \
+                                              sorw, cap, weight, overrides, k0, k1, hits, acc, graph_pairs, ev, \
+                                              max_timed, launches, kernel_ms);                                \
         return walk_tiled_chunked<TT, SS>(ctx, (const TT *)img, vols, nvol, seeds, nseeds, nlabels, nbrw, sorw, \
                                           cap, weight, overrides, k0, k1, hits, acc, graph_pairs, ev, max_timed, \
-                                          launches, kernel_ms);
+                                          launches, kernel_ms);                                               \
+    }
     if (img_dtype == BMX_IMG_U8) {
         switch (shape) {
             BMX_TILED_CASE(uint8_t, 0)
             BMX_TILED_CASE(uint8_t, 1)
             BMX_TILED_CASE(uint8_t, 2)
             BMX_TILED_CASE(uint8_t, 3)
+            BMX_TILED_CASE(uint8_t, 4)
+            BMX_TILED_CASE(uint8_t, 5)
         }
     } else {
         switch (shape) {
@@ -995,6 +1119,8 @@ int bmx_walk_tiled(bmx_ctx *ctx, const void *img, int img_dtype, const bmx_volum
             BMX_TILED_CASE(float, 1)
             BMX_TILED_CASE(float, 2)
             BMX_TILED_CASE(float, 3)
+            BMX_TILED_CASE(float, 4)
+            BMX_TILED_CASE(float, 5)
         }
     }
 #undef BMX_TILED_CASE

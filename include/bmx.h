@@ -257,7 +257,15 @@ typedef struct bmx_walk_opts {
 } bmx_walk_opts;
 
 /* Tile shapes compiled into the tiled walker. */
-#define BMX_TILE_SHAPES 4
+#define BMX_TILE_SHAPES 6
+
+/* Geometry of tile shape `shape` for images of type img_dtype (BMX_IMG_*):
+ * core side and halo in voxels, threads per workgroup, bytes per LDS hit
+ * counter (2: 16-bit counters, two workgroups per CU for u8; 4: 32-bit) and
+ * LDS bytes per workgroup.  No counterpart in the reference, whose kernel has
+ * no tiles; for the autotuner and for callers choosing tile_shape. */
+int bmx_tile_shape_info(int img_dtype, int shape, int *core, int *halo, int *threads, int *hit_bytes,
+                        int *lds_bytes);
 
 /* bmx_walk with a choice of algorithm (opts may be NULL = automatic).  No
  * counterpart in the reference, which has one kernel per backend. */

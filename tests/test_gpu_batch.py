@@ -4,6 +4,8 @@ Every variant must give the oracle's hit counts exactly: the tile shape, the
 round cap and the round driver change only the order of work, and a volume
 walked inside a batch follows the same Philox streams as walked alone.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -59,6 +61,78 @@ def test_tile_shapes_and_graph_rounds_bitexact(eng, dtype, shape_idx, graph_roun
                                seeds["var"], nl, nbrw, sorw, key)
     assert acc == oacc
     np.testing.assert_array_equal(h, oh)
+
+
+def _hit16_shapes(dtype):
+    """Tile shapes with 16-bit LDS hit counters (read from the library; none
+    while it is not built, so that collection never fails)."""
+    if not os.path.exists(_hip.LIB_PATH):
+        return []
+    dt = _hip.IMG_U8 if dtype == np.uint8 else _hip.IMG_F32
+    return [s for s in range(_hip.TILE_SHAPES) if _hip.tile_shape_info(dt, s)["hit_bytes"] == 2]
+
+
+@pytest.mark.parametrize("weight", [40000, 65535, 65536])
+@pytest.mark.parametrize("dtype,shape_idx", [(d, s) for d in (np.uint8, np.float32) for s in _hit16_shapes(d)])
+def test_hit16_counter_wraps_exact(eng, dtype, shape_idx, weight):
+    """Tile shapes with 16-bit LDS hit counters, under hit weights that wrap a
+    counter on most adds (65535: the high half of a word sits at 0xFFFF when a
+    carry from its low half arrives): every hit equals weight x the oracle's
+    count.  A weight of 2^16 takes the 32-bit sibling shape; same bar."""
+    _say("16-bit counters: shape %d weight %d dtype %s" % (shape_idx, weight, np.dtype(dtype).name))
+    img, lab, nl = _volume((33, 40, 36), 65, 3, 6, True, dtype)
+    nbrw, sorw, key = 4, 301, 4242
+    for graph_rounds in (0, -1):
+        h, acc, seeds = _single(eng, img, lab, nl, True, nbrw, sorw, key, segment_steps=32, tile_shape=shape_idx,
+                                graph_rounds=graph_rounds, hit_weight=weight)
+        oh, oacc = orc.philox_walk(img, seeds["pos"].astype(np.int64), seeds["label"].astype(np.uint8),
+                                   seeds["var"], nl, nbrw, sorw, key)
+        assert acc == oacc
+        assert int(oh.max()) * weight < 2 ** 32
+        np.testing.assert_array_equal(h, (oh.astype(np.uint64) * weight).astype(np.uint32))
+
+
+@pytest.mark.parametrize("shape_idx", _hit16_shapes(np.uint8))
+def test_hit16_seed_weights_exact(eng, shape_idx):
+    """Per-seed walk counts and hit weights (seed overrides) on the 16-bit
+    shapes: weights up to 65535 per seed, then hit_weight 2 on top (the 32-bit
+    sibling shape): weight x the oracle's per-seed hits."""
+    img, lab, nl = _volume((30, 34, 38), 66, 3, 5, True, np.uint8)
+    nbrw, sorw, key = 4, 257, 77
+    dims = img.shape
+    ib, lb = eng.upload(img), eng.upload(lab)
+    fb = eng.plane_flags(lb, dims, True)
+    sb, n = eng.seeds(lb, ib, _hip.IMG_U8, dims, fb)
+    seeds = sb.download((n,), _hip.SEED_DTYPE)
+    rng = np.random.default_rng(5)
+    walks = rng.integers(1, nbrw + 1, size=n).astype(np.uint32)
+    wts = rng.choice(np.array([1, 3, 40000, 65535], np.uint32), size=n)
+    seeds["reserved"] = walks | (wts << 16)
+    sb2 = eng.upload(seeds)
+    for hit_weight in (1, 2):
+        _say("16-bit counters, seed overrides: shape %d hit_weight %d" % (shape_idx, hit_weight))
+        h = eng.alloc(nl * img.size * 4)
+        h.zero()
+        acc = eng.walk(ib, _hip.IMG_U8, dims, sb2, n, nl, nbrw, sorw, _hip.RNG_PHILOX, key, h,
+                       algorithm=_hip.WALK_TILED, tile_shape=shape_idx, seed_overrides=True, hit_weight=hit_weight,
+                       segment_steps=32)
+        hits = h.download((nl,) + dims, np.uint32)
+        expect = np.zeros((nl,) + dims, np.uint64)
+        oacc = 0
+        for w in np.unique(wts):
+            for k in range(1, nbrw + 1):
+                sel = (wts == w) & (walks == k)
+                if not sel.any():
+                    continue
+                s = seeds[sel]
+                oh, a = orc.philox_walk(img, s["pos"].astype(np.int64), s["label"].astype(np.uint8), s["var"], nl,
+                                        k, sorw, key)
+                expect += oh.astype(np.uint64) * int(w) * hit_weight
+                oacc += a
+        assert acc == oacc
+        assert int(expect.max()) < 2 ** 32
+        np.testing.assert_array_equal(hits, expect.astype(np.uint32))
+        h.free()
 
 
 def _batch_items(dtype):
