@@ -182,6 +182,17 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__res
 // about 1.5 times per two rounds.  Every voxel follows the single-round rules:
 // 0 outside the box, copied on the volume boundary, the stencil elsewhere, so
 // the results are those of two k_smooth_step passes bit for bit.
+// Per-block boxes of non-zero outputs of the pair passes of one smoothing
+// (6 ints per block of the launch grid: min z, max z, min y, max y, min x, max
+// x; max < 0 = none): out is written by this pass, prev was written by the
+// pass before (the input's), prev2 by the pass before that (the output
+// buffer's current contents).  prev = null: no block is skipped; prev2 = null:
+// the output buffer's contents are unknown; out = null: no boxes are kept.
+struct PassRecords {
+    const int *prev, *prev2;
+    int *out;
+};
+
 constexpr int PW = TX + 4, PH = TY + 4;        // input plane with a 2-voxel halo
 constexpr int kPairLoads = (PW * PH + kSmoothThreads - 1) / kSmoothThreads;
 constexpr int kPairZChunk = 32;               // output planes per block
@@ -191,15 +202,53 @@ __device__ __forceinline__ int slot3(int z) { return (z + 3) % 3; }
 template <typename Tin>
 __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__restrict__ in,
                                                                float *__restrict__ out, int Z, int Y,
-                                                               int X, Box bx)
+                                                               int X, Box bx, PassRecords rec)
 {
     __shared__ float L0[3][PH][PW];   // round t, planes z-1..z+1 (frame: y0-2.., x0-2..)
     __shared__ float L1[3][PH][PW];   // round t+1, rows 1..PH-2, columns 1..PW-2
+    __shared__ int red[kSmoothThreads / 64][6];
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const int x0 = bx.x0 + blockIdx.x * TX, y0 = bx.y0 + blockIdx.y * TY;
     const int zb = bx.z0 + blockIdx.z * kPairZChunk;
     const int ze = min(bx.z1, zb + kPairZChunk);
     const int64_t sy = X, sz = (int64_t)X * Y;
+    const int blk = ((int)blockIdx.z * (int)gridDim.y + (int)blockIdx.y) * (int)gridDim.x + (int)blockIdx.x;
+
+    // Skip a block whose inputs within reach are all +0 (rec.prev: the boxes of
+    // the non-zero outputs of the pass before, per block).  Its outputs are then
+    // +0: the stencil of +0 inputs is +0, as are copied boundary voxels.  They
+    // are written only where the buffer may hold something else (rec.prev2:
+    // this buffer's boxes as the pass before last left them).
+    if (rec.prev) {
+        bool near = false;
+        if (tid < 27) {
+            const int nx = (int)blockIdx.x + tid % 3 - 1, ny = (int)blockIdx.y + (tid / 3) % 3 - 1,
+                      nz = (int)blockIdx.z + tid / 9 - 1;
+            if (nx >= 0 && nx < (int)gridDim.x && ny >= 0 && ny < (int)gridDim.y && nz >= 0 && nz < (int)gridDim.z) {
+                const int *q = rec.prev + 6 * ((nz * (int)gridDim.y + ny) * (int)gridDim.x + nx);
+                const int xe = min(x0 + TX, bx.x1) - 1, ye = min(y0 + TY, bx.y1) - 1;
+                near = q[1] >= 0 && q[0] <= ze - 1 + 2 && q[1] >= zb - 2 && q[2] <= ye + 2 && q[3] >= y0 - 2 &&
+                       q[4] <= xe + 2 && q[5] >= x0 - 2;
+            }
+        }
+        if (!__syncthreads_or(near)) {
+            if (!rec.prev2 || rec.prev2[6 * blk + 1] >= 0) {
+                for (int z = zb; z < ze; ++z)
+#pragma unroll
+                    for (int k = 0; k < RY; ++k) {
+                        const int y = y0 + ty * RY + k, x = x0 + tx;
+                        if (x < bx.x1 && y < bx.y1) out[(int64_t)z * sz + (int64_t)y * sy + x] = 0.0f;
+                    }
+            }
+            if (tid == 0) {
+                int *o = rec.out + 6 * blk;
+                o[0] = INT_MAX; o[1] = -1; o[2] = INT_MAX; o[3] = -1; o[4] = INT_MAX; o[5] = -1;
+            }
+            return;
+        }
+    }
+    // box of this thread's non-zero outputs (float bits, so -0 counts as non-zero)
+    int bb[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
 
     // Per-thread constants of the z-independent parts: the in-plane offset and
     // in-plane validity (inside the volume and the box) of the thread's input
@@ -293,10 +342,34 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
             for (int k = 0; k < RY; ++k) {
                 const int ly = 2 + ty * RY + k, lx = 2 + tx;
                 const int y = y0 + ty * RY + k;
-                if (core_in[k])
-                    out[(int64_t)r2 * sz + (int64_t)y * sy + xc] = round_at(m, c, p, true, zint, true, core_int[k],
-                                                                            ly, lx);
+                if (core_in[k]) {
+                    const float v = round_at(m, c, p, true, zint, true, core_int[k], ly, lx);
+                    out[(int64_t)r2 * sz + (int64_t)y * sy + xc] = v;
+                    if (__float_as_uint(v) != 0u) {
+                        bb[0] = min(bb[0], r2); bb[1] = max(bb[1], r2);
+                        bb[2] = min(bb[2], y); bb[3] = max(bb[3], y);
+                        bb[4] = min(bb[4], xc); bb[5] = max(bb[5], xc);
+                    }
+                }
             }
+        }
+    }
+    if (rec.out) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 6; k += 2) {
+                bb[k] = min(bb[k], __shfl_xor(bb[k], off, 64));
+                bb[k + 1] = max(bb[k + 1], __shfl_xor(bb[k + 1], off, 64));
+            }
+        }
+        if ((tid & 63) == 0)
+            for (int k = 0; k < 6; ++k) red[tid >> 6][k] = bb[k];
+        __syncthreads();
+        if (tid < 6) {
+            int v = red[0][tid];
+            for (int w = 1; w < kSmoothThreads / 64; ++w) v = (tid & 1) ? max(v, red[w][tid]) : min(v, red[w][tid]);
+            rec.out[6 * blk + tid] = v;
         }
     }
 }
@@ -380,26 +453,44 @@ int launch_step(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, co
 }
 
 template <typename Tin>
-int launch_pair(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, const Box &bx)
+int launch_pair(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, const Box &bx,
+                PassRecords rec = PassRecords{nullptr, nullptr, nullptr})
 {
     dim3 grid((bx.x1 - bx.x0 + TX - 1) / TX, (bx.y1 - bx.y0 + TY - 1) / TY,
               (bx.z1 - bx.z0 + kPairZChunk - 1) / kPairZChunk);
-    k_smooth_pair<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X, bx);
+    k_smooth_pair<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X, bx, rec);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
 
+// Ints of the pass records of one pair pass over the whole volume (an upper
+// bound for any box): 6 per block of the launch grid.
+static int64_t pass_record_ints(int64_t Z, int64_t Y, int64_t X)
+{
+    return 6 * ((X + TX - 1) / TX) * ((Y + TY - 1) / TY) * ((Z + kPairZChunk - 1) / kPairZChunk);
+}
+
 // iters >= 1 rounds starting from `in`, two per pass and a single pass for an
-// odd count; the result lands in *result (one of a / b).
+// odd count; the result lands in *result (one of a / b).  With recs (three
+// arrays of pass_record_ints each), pair passes after the first skip the
+// blocks that only see +0 (PassRecords).
 template <typename Tin>
 int smooth_rounds(bmx_ctx *ctx, const Tin *in, float *a, float *b, int Z, int Y, int X, int iters,
-                  const Box &bx, float **result)
+                  const Box &bx, float **result, int *recs = nullptr, int64_t rec_ints = 0)
 {
-    int rc = iters >= 2 ? launch_pair<Tin>(ctx, in, a, Z, Y, X, bx) : launch_step<Tin>(ctx, in, a, Z, Y, X, bx);
+    auto rec_of = [&](int pass) -> PassRecords {
+        if (!recs) return PassRecords{nullptr, nullptr, nullptr};
+        int *r[3] = {recs, recs + rec_ints, recs + 2 * rec_ints};
+        return PassRecords{pass >= 1 ? r[(pass - 1) % 3] : nullptr, pass >= 2 ? r[(pass - 2) % 3] : nullptr,
+                           r[pass % 3]};
+    };
+    int rc = iters >= 2 ? launch_pair<Tin>(ctx, in, a, Z, Y, X, bx, rec_of(0))
+                        : launch_step<Tin>(ctx, in, a, Z, Y, X, bx);
     if (rc) return rc;
     float *cur = a, *nxt = b;
-    for (int it = iters >= 2 ? 2 : 1; it < iters; it += 2) {
-        rc = iters - it >= 2 ? launch_pair<float>(ctx, cur, nxt, Z, Y, X, bx)
+    int pass = 1;
+    for (int it = iters >= 2 ? 2 : 1; it < iters; it += 2, ++pass) {
+        rc = iters - it >= 2 ? launch_pair<float>(ctx, cur, nxt, Z, Y, X, bx, rec_of(pass))
                              : launch_step<float>(ctx, cur, nxt, Z, Y, X, bx);
         if (rc) return rc;
         float *t = cur; cur = nxt; nxt = t;
@@ -457,6 +548,9 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
     if (e == hipSuccess) e = hipMalloc((void **)&best, (size_t)V * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&bidx, (size_t)V);
     if (e == hipSuccess) e = hipMalloc((void **)&dbox, (size_t)nlabels * 6 * sizeof(int));
+    const int64_t rec_ints = pass_record_ints(Z, Y, X);
+    int *recs = nullptr;
+    if (e == hipSuccess) e = hipMalloc((void **)&recs, (size_t)(3 * rec_ints) * sizeof(int));
     if (e == hipSuccess) {
         // Each label is smoothed only inside the box of its non-zero hits grown by
         // `iters`: where all 19 stencil inputs are 0 the update gives exactly 0,
@@ -479,7 +573,7 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
                 bx = Box{max(h[0] - iters, 0), min(h[1] + 1 + iters, (int)Z), max(h[2] - iters, 0),
                          min(h[3] + 1 + iters, (int)Y), max(h[4] - iters, 0), min(h[5] + 1 + iters, (int)X)};
                 rc = smooth_rounds<uint32_t>(ctx, hits + (int64_t)l * V, a, b, (int)Z, (int)Y, (int)X, iters,
-                                             bx, &res);
+                                             bx, &res, recs, rec_ints);
             }
             if (rc == BMX_OK) {
                 k_argmax_update<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(res, best, bidx, l, (int)Z, (int)Y,
@@ -508,6 +602,7 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
     if (best) (void)hipFree(best);
     if (bidx) (void)hipFree(bidx);
     if (dbox) (void)hipFree(dbox);
+    if (recs) (void)hipFree(recs);
     return rc;
 }
 

@@ -226,6 +226,28 @@ def test_smooth_argmax_small_boxes_bitexact(eng, iters):
     np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
 
 
+@pytest.mark.parametrize("iters", [20, 21])
+def test_smooth_argmax_skipped_blocks_bitexact(eng, iters):
+    """Boxes spanning many pair-pass blocks (64 x 16 x 32 voxels), most of them
+    still +0 for the first passes: those blocks are skipped, and write zeros
+    where the reused buffers hold the dense first label's values.  Labels
+    against the oracle's whole-volume rounds, even and odd round counts."""
+    rng = np.random.default_rng(60 + iters)
+    shape = (100, 70, 300)
+    nl = 4
+    h = np.zeros((nl,) + shape, np.uint32)
+    h[0] = rng.integers(0, 25, shape)
+    for l, (z, y, x) in enumerate([(50, 35, 150), (5, 60, 20), (90, 10, 280)], start=1):
+        sl = tuple(slice(max(0, c - 3), c + 4) for c in (z, y, x))
+        h[l][sl] = rng.integers(20, 400, h[l][sl].shape)
+    values = np.array([0, 3, 5, 7], np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(h[0].size)
+    eng.smooth_argmax(hb, nl, shape, iters, values, out)
+    phi = np.stack([orc.smooth(x.astype(np.float32), iters) for x in h])
+    np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
+
+
 @pytest.mark.parametrize("iters", [2, 3, 4])
 def test_smooth_pairs_bitexact_wide(eng, iters):
     """A volume wider than one tile in x and y and taller than one z chunk."""
