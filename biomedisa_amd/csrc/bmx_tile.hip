@@ -211,43 +211,31 @@ __global__ __launch_bounds__(256) void k_seed_walks(bmx_seed *__restrict__ seeds
 
 // Round bookkeeping in device memory, so that a round needs no host values
 // and a sequence of rounds can be captured in one graph: n = walkers in the
-// current state array, nb = non-empty buckets, bnext = next bucket to walk.
-// The bucket list holds the nheavy buckets of at least one walker per thread
-// at its front and the nlight others at its back (nb = nheavy + nlight), so a
-// round starts its longest buckets first and ends on short ones.  With a tiny
-// threshold, buckets below it go to a list of their own (ntiny of them, taken
-// from tnext), walked without the LDS tile.
+// current state array, nnext = live walkers among them (the key-count total),
+// nb = non-empty buckets, bnext = next bucket to walk.  The bucket list holds
+// the nheavy buckets of at least one walker per thread at its front and the
+// nlight others at its back (nb = nheavy + nlight), so a round starts its
+// longest buckets first and ends on short ones.  With a tiny threshold,
+// buckets below it go to a list of their own (ntiny of them, taken from
+// tnext), walked without the LDS tile.
 struct RoundCtl {
-    uint32_t nb, bnext, n, done, nheavy, nlight, ntiny, tnext;
+    uint32_t nb, bnext, n, done, nheavy, nlight, ntiny, tnext, nnext;
     // persistent walk kernels: first start and last end of the current round
     // (device wall clock), and the summed spans of the rounds so far
     unsigned long long t0, t1, ticks;
 };
 
-// Start of a round: clear the key counts and the bucket counters.
-__global__ __launch_bounds__(256) void k_round_begin(uint32_t *__restrict__ counts, int64_t ncounts,
-                                                     RoundCtl *__restrict__ ctl)
-{
-    for (int64_t i = bmx_gtid(); i < ncounts; i += bmx_gstride()) counts[i] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->nb = 0u;
-        ctl->bnext = 0u;
-        ctl->nheavy = 0u;
-        ctl->nlight = 0u;
-        ctl->ntiny = 0u;
-        ctl->tnext = 0u;
-    }
-}
-
-// End of a round's re-bucketing: the live walkers (the key-count total) become
-// the next round's n.
-__global__ void k_round_end(const uint32_t *__restrict__ counts, int64_t nkeys, RoundCtl *__restrict__ ctl)
-{
-    if (threadIdx.x == 0) {
-        ctl->n = counts[nkeys];
-        ctl->nb = ctl->nheavy + ctl->nlight;
-    }
-}
+// Where a walker goes in the next round's state array, written by whatever
+// moved it last (k_keys_first before the first round, the walk kernel after):
+//  {kDestDead, 0}: its walk has ended, it is dropped;
+//  {kDestDirect | key, slot}: entry `slot` of key's range;
+//  {bi, rank}: the rank-th walker of round bucket bi to stop in the bucket's
+//    neighbour tile nb (0..26, kept in bits 24-28 of the walker's walk word);
+//    the bucket reserved that neighbour's entries at bres[27 bi + nb] =
+//    {key, first entry within the key's range}.
+// The key counts are added by the same atomics, so no pass over the walkers
+// counts keys and the placement needs no atomics.
+constexpr uint32_t kDestDead = 0xFFFFFFFFu, kDestDirect = 0x80000000u;
 
 // Slot in the bucket arrays of the round's bucket i (heavy ones from the front,
 // light ones from the back).
@@ -256,10 +244,18 @@ __device__ __forceinline__ uint32_t bucket_slot(uint32_t i, uint32_t nheavy, uin
     return i < nheavy ? i : nkeys - 1u - (i - nheavy);
 }
 
+// Start of a walk: n walkers in the state array, nothing listed yet.
 __global__ void k_set_count(RoundCtl *__restrict__ ctl, uint32_t n)
 {
     if (threadIdx.x == 0) {
         ctl->n = n;
+        ctl->nnext = n;
+        ctl->nb = 0u;
+        ctl->bnext = 0u;
+        ctl->nheavy = 0u;
+        ctl->nlight = 0u;
+        ctl->ntiny = 0u;
+        ctl->tnext = 0u;
         ctl->done = 0u;
         ctl->t0 = ~0ull;
         ctl->t1 = 0ull;
@@ -267,11 +263,18 @@ __global__ void k_set_count(RoundCtl *__restrict__ ctl, uint32_t n)
     }
 }
 
-// Walker counts per key.  The loop bound is uniform per block, as
-// agg_atomic_inc needs every lane of a wave.
+__global__ __launch_bounds__(256) void k_clear_counts(uint32_t *__restrict__ counts, int64_t ncounts)
+{
+    for (int64_t i = bmx_gtid(); i < ncounts; i += bmx_gstride()) counts[i] = 0u;
+}
+
+// Keys and destinations of the walkers as created (before the first round).
+// The loop bound is uniform per block, as agg_atomic_inc needs every lane of a
+// wave.
 template <int C>
-__global__ __launch_bounds__(256) void k_tile_hist(const uint4 *__restrict__ st, const RoundCtl *__restrict__ ctl,
-                                                   TileGeom tg, int sorw, uint32_t *counts)
+__global__ __launch_bounds__(256) void k_keys_first(const uint4 *__restrict__ st, const RoundCtl *__restrict__ ctl,
+                                                    TileGeom tg, int sorw, uint32_t *counts,
+                                                    uint2 *__restrict__ dest)
 {
     const int64_t n = ctl->n;
     for (int64_t b0 = (int64_t)blockIdx.x * 256; b0 < n; b0 += (int64_t)gridDim.x * 256) {
@@ -281,30 +284,59 @@ __global__ __launch_bounds__(256) void k_tile_hist(const uint4 *__restrict__ st,
         if (i < n) {
             const uint4 w4 = st[i];
             live = (int)w4.w < sorw;
-            key = tile_key<C>(w4.x, w4.z >> 16, tg);
+            key = tile_key<C>(w4.x, (w4.z >> 16) & 0xFFu, tg);
         }
-        (void)agg_atomic_inc(counts, key, live);
+        const uint32_t slot = agg_atomic_inc(counts, key, live);
+        if (i < n) dest[i] = live ? make_uint2(kDestDirect | key, slot) : make_uint2(kDestDead, 0u);
     }
 }
 
-template <int C>
-__global__ __launch_bounds__(256) void k_tile_scatter(const uint4 *__restrict__ in, const RoundCtl *__restrict__ ctl,
-                                                      uint4 *__restrict__ out, uint32_t *cursor, TileGeom tg,
-                                                      int sorw)
+// After the scan of the key counts: the live total, and the bucket counters
+// cleared for k_bucket_list.
+__global__ void k_round_mid(const uint32_t *__restrict__ off, int64_t nkeys, RoundCtl *__restrict__ ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->nnext = off[nkeys];
+        ctl->nheavy = 0u;
+        ctl->nlight = 0u;
+        ctl->ntiny = 0u;
+    }
+}
+
+// Each live walker of `in` (ctl->n entries) to its entry of `out`: the
+// exclusive offset of its key plus its place in the key's range (dest).
+__global__ __launch_bounds__(256) void k_tile_place(const uint4 *__restrict__ in, const uint2 *__restrict__ dest,
+                                                    const RoundCtl *__restrict__ ctl, const uint32_t *__restrict__ off,
+                                                    const uint2 *__restrict__ bres, uint4 *__restrict__ out)
 {
     const int64_t n = ctl->n;
-    for (int64_t b0 = (int64_t)blockIdx.x * 256; b0 < n; b0 += (int64_t)gridDim.x * 256) {
-        const int64_t i = b0 + threadIdx.x;
-        uint4 w4 = make_uint4(0, 0, 0, 0);
-        bool live = false;
-        uint32_t key = 0;
-        if (i < n) {
-            w4 = in[i];
-            live = (int)w4.w < sorw;
-            key = tile_key<C>(w4.x, w4.z >> 16, tg);
+    for (int64_t i = bmx_gtid(); i < n; i += bmx_gstride()) {
+        const uint2 d = dest[i];
+        if (d.x == kDestDead) continue;
+        uint4 w4 = in[i];
+        uint32_t at;
+        if (d.x & kDestDirect) {
+            at = off[d.x & ~kDestDirect] + d.y;
+        } else {
+            const uint2 r = bres[(int64_t)d.x * 27 + ((w4.z >> 24) & 31u)];
+            at = off[r.x] + r.y + d.y;
+            w4.z &= 0x00FFFFFFu;
         }
-        const uint32_t dst = agg_atomic_inc(cursor, key, live);
-        if (live) out[dst] = w4;
+        out[at] = w4;
+    }
+}
+
+// Start of a round's walk: the placed walkers are the state array, the counts
+// are cleared for the walk to fill, and the bucket counters reset.
+__global__ __launch_bounds__(256) void k_round_start(uint32_t *__restrict__ counts, int64_t ncounts,
+                                                     RoundCtl *__restrict__ ctl)
+{
+    for (int64_t i = bmx_gtid(); i < ncounts; i += bmx_gstride()) counts[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->n = ctl->nnext;
+        ctl->nb = ctl->nheavy + ctl->nlight;
+        ctl->bnext = 0u;
+        ctl->tnext = 0u;
     }
 }
 
@@ -510,15 +542,18 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
     const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ tkey, const uint32_t *__restrict__ toff,
     const uint32_t *__restrict__ tcnt, RoundCtl *__restrict__ ctl, uint32_t nkeys, int sorw, int cap,
-    uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted)
+    uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted,
+    uint2 *__restrict__ dest, uint2 *__restrict__ bres, uint32_t *__restrict__ counts)
 {
     using Sh = TileShape<T, S>;
     constexpr int C = Sh::C, H = Sh::H, R = C + 2 * H, RI = R + 2, NTH = Sh::NTH;
     constexpr int NT = RI * RI * RI;
+    static_assert(H + 1 <= C, "a walker stops in its tile or a neighbouring one");
     extern __shared__ __align__(16) unsigned char smem[];
     T *simg = (T *)smem;
     uint32_t *shits = (uint32_t *)(smem + tile_img_bytes<T, S>());
     __shared__ uint32_t next, bucket;
+    __shared__ uint32_t snb[27];   // walkers of the bucket stopping in each neighbour tile
 
     const uint32_t nb = ctl->nb, nheavy = ctl->nheavy;
     if (PERSIST && threadIdx.x == 0) atomicMin(&ctl->t0, (unsigned long long)__builtin_amdgcn_s_memrealtime());
@@ -577,6 +612,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
                 }
             }
             constexpr int NW = (int)(tile_hit_bytes<T, S>() / 4);   // 32-bit counter words
+            if (tid < 27) snb[tid] = 0u;
             uint4 *sh4 = (uint4 *)shits;
             for (int k = tid; k < NW / 4; k += NTH) sh4[k] = make_uint4(0u, 0u, 0u, 0u);
             for (int k = (NW / 4) * 4 + tid; k < NW; k += NTH) shits[k] = 0u;
@@ -637,12 +673,34 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
             if (step >= sorw || !inside || step - step0 >= cap) {
                 const uint32_t pos =
                     (uint32_t)(bz + lz) * g.sz + (uint32_t)(by + ly) * g.sy + (uint32_t)(bx + lx);
-                st[off + j] = make_uint4(gv.base + pos, seed, wl, (uint32_t)step);
+                uint2 d = make_uint2(kDestDead, 0u);
+                uint32_t wl2 = wl;
+                if (step < sorw) {
+                    // the tile it stopped in, relative to this one: local core [1+H, 1+H+C)
+                    const auto rel = [](int l) { return l < 1 + H ? 0 : (l < 1 + H + C ? 1 : 2); };
+                    const uint32_t nbr = (uint32_t)(9 * rel(lz) + 3 * rel(ly) + rel(lx));
+                    d = make_uint2(bi, atomicAdd(&snb[nbr], 1u));
+                    wl2 = (wl & 0x00FFFFFFu) | (nbr << 24);
+                }
+                st[off + j] = make_uint4(gv.base + pos, seed, wl2, (uint32_t)step);
+                dest[off + j] = d;
                 have = false;
             }
         }
         wave_add_u64(accepted, (uint64_t)acc);
         __syncthreads();
+        // reserve the next round's entries of the neighbour keys this bucket's
+        // walkers stopped in
+        if (tid < 27) {
+            const uint32_t c = snb[tid];
+            if (c) {
+                const int nz = tz + tid / 9 - 1, ny = ty + (tid / 3) % 3 - 1, nx = tx + tid % 3 - 1;
+                const uint32_t key_nb =
+                    (gv.tile_base + ((uint32_t)nz * gv.nty + (uint32_t)ny) * gv.ntx + (uint32_t)nx) * tg.nlabels +
+                    label;
+                bres[(int64_t)bi * 27 + tid] = make_uint2(key_nb, atomicAdd(&counts[key_nb], c));
+            }
+        }
         if (Sh::HB == 4) {
             for (int k = tid; k < NT; k += NTH) {
                 const uint32_t c = shits[k];
@@ -688,7 +746,13 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
             for (uint32_t w = (uint32_t)lane; w < cnt; w += 64u) {
                 uint4 w4 = st[off + w];
                 tiny_walker<T>(img_all + gv.base, g, gv, wseeds, w4, sorw, cap, k0, k1, hg, tacc);
+                uint2 d = make_uint2(kDestDead, 0u);
+                if ((int)w4.w < sorw) {
+                    const uint32_t key2 = tile_key<C>(w4.x, label, tg);
+                    d = make_uint2(kDestDirect | key2, atomicAdd(&counts[key2], 1u));
+                }
                 st[off + w] = w4;
+                dest[off + w] = d;
             }
         }
         wave_add_u64(accepted, (uint64_t)tacc);
@@ -745,28 +809,29 @@ struct RoundArgs {
     uint32_t k0, k1;
     uint32_t *hits;
     unsigned long long *acc;
+    uint2 *dest;   // per entry of the walked state array (see kDestDead)
+    uint2 *bres;   // 27 per round bucket
 };
 
-// Re-bucketing of one round: walkers of `in` (ctl->n of them) counted per
-// (tile, label) key, listed as buckets and scattered live into `out`; ctl->n
-// becomes the live count.  `grid` blocks run the per-walker kernels.
+// Re-bucketing of one round: the key counts left by the walk before (or by
+// k_keys_first) scanned into offsets, listed as buckets, and the live walkers
+// of `in` (ctl->n entries) placed into `out` by their destination records;
+// ctl->n becomes the live count and the counts are cleared for the walk.
+// `grid` blocks run the per-walker kernel.
 template <typename T, int S>
 int round_bucket(bmx_ctx *ctx, const RoundArgs<T> &a, const uint4 *in, uint4 *out, unsigned grid, uint32_t tiny)
 {
-    constexpr int C = TileShape<T, S>::C;
-    k_round_begin<<<bmx_stride_blocks(a.nkeys + 1, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys + 1, a.ctl);
-    BMX_LAUNCH_CHECK();
-    k_tile_hist<C><<<grid, 256, 0, ctx->stream>>>(in, a.ctl, a.tg, a.sorw, a.counts);
-    BMX_LAUNCH_CHECK();
     int rc = scan_u32(ctx, a.counts, a.nkeys + 1);
     if (rc) return rc;
+    k_round_mid<<<1, 64, 0, ctx->stream>>>(a.counts, a.nkeys, a.ctl);
+    BMX_LAUNCH_CHECK();
     k_bucket_list<<<bmx_blocks(a.nkeys, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys, a.bkey, a.boff, a.bcnt,
                                                                      a.tkey, a.toff, a.tcnt, a.ctl,
                                                                      (uint32_t)TileShape<T, S>::NTH, tiny);
     BMX_LAUNCH_CHECK();
-    k_tile_scatter<C><<<grid, 256, 0, ctx->stream>>>(in, a.ctl, out, a.counts, a.tg, a.sorw);
+    k_tile_place<<<grid, 256, 0, ctx->stream>>>(in, a.dest, a.ctl, a.counts, a.bres, out);
     BMX_LAUNCH_CHECK();
-    k_round_end<<<1, 64, 0, ctx->stream>>>(a.counts, a.nkeys, a.ctl);
+    k_round_start<<<bmx_stride_blocks(a.nkeys + 1, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys + 1, a.ctl);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -776,7 +841,7 @@ int round_walk(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *st, unsigned grid)
 {
     k_tile_walk<T, S, PERSIST><<<grid, TileShape<T, S>::NTH, tile_lds_bytes<T, S>(), ctx->stream>>>(
         a.img, a.tg, a.wseeds, st, a.bkey, a.boff, a.bcnt, a.tkey, a.toff, a.tcnt, a.ctl, (uint32_t)a.nkeys,
-        a.sorw, a.cap, a.k0, a.k1, a.hits, a.acc);
+        a.sorw, a.cap, a.k0, a.k1, a.hits, a.acc, a.dest, a.bres, a.counts);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -881,13 +946,15 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     if (rc) return rc;
     BMX_HIP(hipMemcpyAsync(ctx->d_vols, vg.data(), (size_t)nvol * sizeof(VolGeom), hipMemcpyHostToDevice,
                            ctx->stream));
-    rc = grow(ctx, &ctx->d_state, &ctx->state_cap, (size_t)n0 * 2 * sizeof(uint4));
+    // two walker-state arrays and the destination records of one of them
+    rc = grow(ctx, &ctx->d_state, &ctx->state_cap, (size_t)n0 * (2 * sizeof(uint4) + sizeof(uint2)));
     if (rc) return rc;
     rc = grow(ctx, &ctx->d_wseeds, &ctx->wseeds_cap, (size_t)(nseeds > 0 ? nseeds : 1) * sizeof(WalkSeed));
     if (rc) return rc;
     rc = grow(ctx, &ctx->d_keys, &ctx->keys_cap, (size_t)(nkeys + 1) * sizeof(uint32_t));
     if (rc) return rc;
-    rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap, (size_t)nkeys * 6 * sizeof(uint32_t));
+    // six uint32 bucket lists, then 27 neighbour reservations per bucket
+    rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap, (size_t)nkeys * (6 * sizeof(uint32_t) + 27 * sizeof(uint2)));
     if (rc) return rc;
     // the scan's block partials, allocated before any capture
     rc = grow(ctx, &ctx->d_scan, &ctx->scan_cap,
@@ -918,6 +985,8 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     a.hits = hits;
     a.acc = acc;
     uint4 *A = (uint4 *)ctx->d_state, *Bst = A + n0;
+    a.dest = (uint2 *)(Bst + n0);
+    a.bres = (uint2 *)(a.tcnt + nkeys);   // 8-byte aligned: nkeys * 24 bytes in
     if (nseeds > 0) {
         k_walk_seeds<T><<<bmx_blocks(nseeds, 256), 256, 0, ctx->stream>>>(img, a.tg, seeds, nseeds, weight, overrides,
                                                                          (WalkSeed *)ctx->d_wseeds);
@@ -932,6 +1001,11 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
         const int64_t b = (n + 255) / 256;
         return (unsigned)(b < 1 ? 1 : (b < kMaxWalkerBlocks ? b : kMaxWalkerBlocks));
     };
+    // the first round's key counts and destinations, from the walkers as created
+    k_clear_counts<<<bmx_stride_blocks(nkeys + 1, 256), 256, 0, ctx->stream>>>(a.counts, nkeys + 1);
+    BMX_LAUNCH_CHECK();
+    k_keys_first<C><<<walker_grid(n0), 256, 0, ctx->stream>>>(A, a.ctl, a.tg, sorw, a.counts, a.dest);
+    BMX_LAUNCH_CHECK();
     int round = 0;
     *launches = 0;
     *kernel_ms = -1.0f;
@@ -981,7 +1055,7 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
 // whole seeds, one after the other, into the same hits and move counter.  A
 // walker's stream is keyed by its seed's voxel, walk and step, never by its
 // index, so the chunks' hits add up to those of one walk over every seed.
-// The chunks also bound the round buffers (32 bytes per walker) at 2^32
+// The chunks also bound the round buffers (40 bytes per walker) at 2^32
 // walkers.  BMX_WALK_CHUNK=n lowers the chunk size to n walkers, so that
 // chunked and unchunked walks can be compared on small volumes.
 template <typename T, int S>
