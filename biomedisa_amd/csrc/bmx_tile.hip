@@ -701,26 +701,15 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
                 bres[(int64_t)bi * 27 + tid] = make_uint2(key_nb, atomicAdd(&counts[key_nb], c));
             }
         }
-        if (Sh::HB == 4) {
-            for (int k = tid; k < NT; k += NTH) {
-                const uint32_t c = shits[k];
-                if (c) {
-                    const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
-                    atomicAdd(&hg[g0 + (int64_t)lz2 * g.sz + (int64_t)ly2 * g.sy + lx2], c);
-                }
-            }
-        } else {
-            // word k: voxels 2k (low half) and 2k+1 (high half) of one tile row,
-            // both in the volume when the word is not 0 (hit16_wrap)
-            for (int k = tid; k < NT / 2; k += NTH) {
-                const uint32_t c = shits[k];
-                if (c) {
-                    const int v = 2 * k;
-                    const int lz2 = v / (RI * RI), ly2 = (v / RI) % RI, lx2 = v % RI;
-                    const int64_t gi = g0 + (int64_t)lz2 * g.sz + (int64_t)ly2 * g.sy + lx2;
-                    if (c & 0xFFFFu) atomicAdd(&hg[gi], c & 0xFFFFu);
-                    if (c >> 16) atomicAdd(&hg[gi + 1], c >> 16);
-                }
+        // one counter per lane, consecutive lanes on consecutive voxels; a 16-bit
+        // counter k is half k of the LDS words (little-endian: voxel 2w is the
+        // low half of word w), and it is not 0 only for a voxel of the volume
+        // (hit16_wrap)
+        for (int k = tid; k < NT; k += NTH) {
+            const uint32_t c = Sh::HB == 4 ? shits[k] : (uint32_t)((const uint16_t *)shits)[k];
+            if (c) {
+                const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
+                atomicAdd(&hg[g0 + (int64_t)lz2 * g.sz + (int64_t)ly2 * g.sy + lx2], c);
             }
         }
         if (!PERSIST) break;
