@@ -11,16 +11,14 @@
 //
 // The forward and the input gradient run on k_conv_mm (gfx950's
 // v_mfma_f32_16x16x32_bf16, planes streamed through LDS, weights in
-// registers; described at the kernel).  The weight gradient runs on
-// v_mfma_f32_16x16x16bf16_1k: lane l supplies A[l%16][4(l/16)..+3] and
-// B[4(l/16)..+3][l%16] and receives D[4(l/16)+i][l%16], i = 0..3.
+// registers; described at the kernel), the weight gradient on
+// k_conv_wgrad_mm (the same instruction, with voxels as the reduction).
 #include "bmx_internal.h"
 
 #include <algorithm>
 
 namespace {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -32,10 +30,6 @@ __device__ __forceinline__ uint16_t f2bf(float f)
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
-__device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
 
 struct Shape {
     int N, D, H, W;
@@ -270,59 +264,182 @@ __global__ __launch_bounds__(256) void k_conv_mm(const uint16_t *__restrict__ x,
     }
 }
 
-// Weight gradient: dw[co][tap][ci] += sum_v dy[v][co] * x[v + off(tap)][ci];
-// db[co] += sum_v dy[v][co].  One wave: 16 output channels x 16 input
-// channels of one tap, reducing over a chunk of `chunk` voxels (multiple of
-// 16); partial sums are added to dw with float atomics.
-// grid: (ceil(nvox / chunk), 27 * (Ci/16), Co/16), block 64.
-__global__ __launch_bounds__(64) void k_conv_wgrad(const uint16_t *__restrict__ x, Shape s, int Ci,
-                                                   const uint16_t *__restrict__ dy, int Co, int64_t chunk,
-                                                   float *__restrict__ dw, float *__restrict__ db)
+// Weight gradient on v_mfma_f32_16x16x32_bf16:
+//   dw[co][tap][ci] += sum_v dy[v][co] * x[v + off(tap)][ci],
+// a product with M = output channels, N = input channels and K = voxels.
+// K runs along x: one K step is the 32 voxels x0 .. x0+31 of one row, and
+// lane group g (lanes 16g .. 16g+15) supplies voxels x0+8g .. x0+8g+7, so a
+// lane's 8 K values are 8 consecutive voxels of one channel.  A block stages
+// its planes channel-major in LDS (the channels-last tensors are transposed
+// on the way in), and a fragment is one 16-byte LDS read:
+//  - dy plane z: [16*CO_T channels][kWgTY rows][32 voxels], channel pitch
+//    kWgTY*32 + 8 (16 channels at one voxel offset fall in distinct banks);
+//  - x planes z-1, z, z+1 (a ring of three): [16 channels][kWgTY+2 rows][48],
+//    element i of a row holding x = x0 - 8 + i (the halo x0-1 .. x0+32 at
+//    7 .. 40).  Tap offset dx = 0 reads at 8g + 8 (aligned); dx = -1 and +1
+//    start one element before or after, and their fragment is rebuilt from two
+//    aligned reads with v_alignbit (one 16-bit shift per dword).
+// A block has three waves, wave w the taps of dz = w - 1: 9 taps x CO_T
+// output-channel tiles of 16 x 16 accumulators, over the block's voxels (a
+// 32 x kWgTY (x, y) tile and a chunk of zc planes of one volume), then
+// added to dw with float atomics.  Each A fragment (dy) serves 9 taps and
+// each B fragment (x) CO_T tiles.  lane l supplies A[l%16][8(l/16)..+7] =
+// dy[voxel 8(l/16)+j][co l%16] and B[8(l/16)..+7][l%16] = x[that voxel
+// shifted by the tap][ci l%16], and receives D[4(l/16)+i][l%16] = dw[co
+// 4(l/16)+i][ci l%16].  Positions off the grid read as zero (the convolution's
+// zero padding; dy is zero beyond the grid).
+constexpr int kWgTX = 32, kWgTY = 4;
+constexpr int kWgXR = 48;                                   // x row length (elements)
+constexpr int kWgXC = (kWgTY + 2) * kWgXR + 8;              // x channel pitch
+constexpr int kWgXS = 16 * kWgXC;                           // x plane (slot) size
+constexpr int kWgDC = kWgTY * kWgTX + 8;                    // dy channel pitch
+
+__host__ __device__ constexpr size_t wg_lds_bytes(int cot)
 {
-    const int lane = threadIdx.x;
-    const int64_t nvox = (int64_t)s.N * s.D * s.H * s.W;
-    const int64_t vb = (int64_t)blockIdx.x * chunk;
-    const int tap = blockIdx.y % 27, cib = blockIdx.y / 27;
-    const int co0 = blockIdx.z * 16, ci0 = cib * 16;
-    const int dz = tap / 9 - 1, dyo = (tap / 3) % 3 - 1, dxo = tap % 3 - 1;
-    const int r = lane & 15, kq = (lane >> 4) * 4;
-    float4_t acc = float4_t{0.f, 0.f, 0.f, 0.f};
-    float bacc = 0.0f;
-    const int64_t vend = min(vb + chunk, nvox);
-    for (int64_t k0 = vb; k0 < vend; k0 += 16) {
-        // A[m=co][k=v] = dy[v][co]; B[k=v][n=ci] = x[v+off][ci]
-        short4_t a, b;
+    return ((size_t)3 * kWgXS + (size_t)16 * cot * kWgDC) * sizeof(uint16_t);
+}
+
+template <int CO_T>
+__global__ __launch_bounds__(192) void k_conv_wgrad_mm(const uint16_t *__restrict__ x, Shape s, int Ci,
+                                                       const uint16_t *__restrict__ dy, int Co, int zc,
+                                                       float *__restrict__ dw)
+{
+    extern __shared__ __align__(16) uint16_t wg_lds[];
+    uint16_t *xs = wg_lds;                 // [3][16][kWgTY+2][kWgXR] (+ pads)
+    uint16_t *ds = wg_lds + 3 * kWgXS;     // [16*CO_T][kWgTY][32] (+ pads)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, kg = lane >> 4;
+    const int x0 = blockIdx.x * kWgTX, y0 = blockIdx.y * kWgTY;
+    const int nzc = (s.D + zc - 1) / zc;
+    const int ncib = Ci / 16, ncob = Co / (16 * CO_T);
+    int q = (int)blockIdx.z;
+    const int cib = q % ncib; q /= ncib;
+    const int cob = q % ncob; q /= ncob;
+    const int n = q / nzc, zb = (q % nzc) * zc, ze = min(s.D, zb + zc);
+    const int ci0 = cib * 16, co0 = cob * 16 * CO_T;
+
+    // x plane zz (channels ci0..ci0+15) into ring slot `slot`, zero off the grid
+    auto load_x = [&](int zz, int slot) {
+        uint16_t *dst = xs + slot * kWgXS;
+        const bool zin = (unsigned)zz < (unsigned)s.D;
+        for (int it = tid; it < (kWgTY + 2) * (kWgTX + 2) * 2; it += 192) {
+            const int g = it & 1, p = it >> 1, ly = p / (kWgTX + 2), lx = p - ly * (kWgTX + 2);
+            const int yy = y0 - 1 + ly, xx = x0 - 1 + lx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (zin && (unsigned)yy < (unsigned)s.H && (unsigned)xx < (unsigned)s.W)
+                v = *(const uint4 *)(x + ((((int64_t)n * s.D + zz) * s.H + yy) * s.W + xx) * Ci + ci0 + 8 * g);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+            uint16_t *o = dst + (8 * g) * kWgXC + ly * kWgXR + lx + 7;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int64_t v = k0 + kq + i;
-            uint16_t av = 0, bv = 0;
-            if (v < vend) {
-                av = dy[v * Co + co0 + r];
-                int64_t t = v;
-                const int xx = (int)(t % s.W) + dxo; t /= s.W;
-                const int yy = (int)(t % s.H) + dyo; t /= s.H;
-                const int z = (int)(t % s.D) + dz;
-                const int n = (int)(t / s.D);
-                if ((unsigned)z < (unsigned)s.D && (unsigned)yy < (unsigned)s.H && (unsigned)xx < (unsigned)s.W)
-                    bv = x[((((int64_t)n * s.D + z) * s.H + yy) * s.W + xx) * Ci + ci0 + r];
-            }
-            a[i] = (short)av;
-            b[i] = (short)bv;
-        }
-        acc = mfma16(a, b, acc);
-        // the bias gradient rides on one (tap, ci-block) of each co-block
-        if (tap == 13 && cib == 0 && lane < 16) {
-            for (int j = 0; j < 16; ++j) {
-                const int64_t v = k0 + j;
-                if (v < vend) bacc += bf2f(dy[v * Co + co0 + lane]);
+            for (int j = 0; j < 4; ++j) {
+                o[(2 * j) * kWgXC] = (uint16_t)(w4[j] & 0xFFFFu);
+                o[(2 * j + 1) * kWgXC] = (uint16_t)(w4[j] >> 16);
             }
         }
+    };
+    // dy plane zz (channels co0 .. co0 + 16*CO_T - 1), zero off the grid
+    auto load_dy = [&](int zz) {
+        constexpr int G = 2 * CO_T;   // 8-channel groups
+        for (int it = tid; it < kWgTY * kWgTX * G; it += 192) {
+            const int g = it % G, p = it / G, ly = p / kWgTX, lx = p - ly * kWgTX;
+            const int yy = y0 + ly, xx = x0 + lx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if ((unsigned)yy < (unsigned)s.H && (unsigned)xx < (unsigned)s.W)
+                v = *(const uint4 *)(dy + ((((int64_t)n * s.D + zz) * s.H + yy) * s.W + xx) * Co + co0 + 8 * g);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+            uint16_t *o = ds + (8 * g) * kWgDC + ly * kWgTX + lx;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                o[(2 * j) * kWgDC] = (uint16_t)(w4[j] & 0xFFFFu);
+                o[(2 * j + 1) * kWgDC] = (uint16_t)(w4[j] >> 16);
+            }
+        }
+    };
+
+    float4_t acc[9][CO_T];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int c = 0; c < CO_T; ++c) acc[t][c] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+    load_x(zb - 1, (zb + 2) % 3);
+    load_x(zb, zb % 3);
+    for (int z = zb; z < ze; ++z) {
+        load_x(z + 1, (z + 1) % 3);
+        load_dy(z);
+        __syncthreads();
+        // this wave's x plane z + dz, dz = wv - 1
+        const uint16_t *xp = xs + ((z + wv + 2) % 3) * kWgXS + l16 * kWgXC;
+        const uint16_t *dp = ds + l16 * kWgDC + 8 * kg;
+#pragma unroll
+        for (int r = 0; r < kWgTY; ++r) {
+            short8_t a[CO_T];
+#pragma unroll
+            for (int c = 0; c < CO_T; ++c) a[c] = *(const short8_t *)(dp + 16 * c * kWgDC + r * kWgTX);
+#pragma unroll
+            for (int ty = 0; ty < 3; ++ty) {
+                const uint16_t *row = xp + (r + ty) * kWgXR + 8 * kg + 8;   // element 8g + 8: x0 + 8g
+                const uint4 mid = *(const uint4 *)row;
+                const uint32_t lo = *(const uint32_t *)(row - 2);          // elements 8g + 6, 8g + 7
+                const uint32_t hi = *(const uint32_t *)(row + 8);          // elements 8g + 16, 8g + 17
+                short8_t b[3];
+                // dx = -1: elements 8g+7 .. 8g+14
+                b[0] = __builtin_bit_cast(short8_t, make_uint4(__builtin_amdgcn_alignbit(mid.x, lo, 16),
+                                                               __builtin_amdgcn_alignbit(mid.y, mid.x, 16),
+                                                               __builtin_amdgcn_alignbit(mid.z, mid.y, 16),
+                                                               __builtin_amdgcn_alignbit(mid.w, mid.z, 16)));
+                b[1] = __builtin_bit_cast(short8_t, mid);
+                // dx = +1: elements 8g+9 .. 8g+16
+                b[2] = __builtin_bit_cast(short8_t, make_uint4(__builtin_amdgcn_alignbit(mid.y, mid.x, 16),
+                                                               __builtin_amdgcn_alignbit(mid.z, mid.y, 16),
+                                                               __builtin_amdgcn_alignbit(mid.w, mid.z, 16),
+                                                               __builtin_amdgcn_alignbit(hi, mid.w, 16)));
+#pragma unroll
+                for (int tx = 0; tx < 3; ++tx)
+#pragma unroll
+                    for (int c = 0; c < CO_T; ++c) acc[ty * 3 + tx][c] = mfma32(a[c], b[tx], acc[ty * 3 + tx][c]);
+            }
+        }
+        __syncthreads();   // the ring slot of plane z - 1 and the dy plane are reused
     }
-    // D[m=co][n=ci]: lane holds rows kq..kq+3 (co), column r (ci)
+    // D[4kg + i][l16]: dw[co0 + 16c + 4kg + i][tap][ci0 + l16], tap = (dz+1)*9 + (dy+1)*3 + (dx+1)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        atomicAdd(&dw[((int64_t)(co0 + kq + i) * 27 + tap) * Ci + ci0 + r], acc[i]);
-    if (tap == 13 && cib == 0 && lane < 16 && db) atomicAdd(&db[co0 + lane], bacc);
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int c = 0; c < CO_T; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                atomicAdd(&dw[((int64_t)(co0 + 16 * c + 4 * kg + i) * 27 + wv * 9 + t) * Ci + ci0 + l16], acc[t][c][i]);
+}
+
+// Bias gradient db[co] += sum_v dy[v][co] (channels-last dy, Co <= 256 and a
+// multiple of 16): each block sums a stride of voxels per channel in
+// registers, then in LDS, and adds once per channel.
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ dy, int64_t nvox, int Co,
+                                                   float *__restrict__ db)
+{
+    __shared__ float part[256];
+    const int G = Co / 8;                       // 16-byte groups per voxel
+    const int per = 256 / G;                    // voxels a block reads at once
+    const int g = threadIdx.x % G, vo = threadIdx.x / G;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (vo < per)
+        for (int64_t v = (int64_t)blockIdx.x * per + vo; v < nvox; v += (int64_t)gridDim.x * per) {
+            const uint4 q = *(const uint4 *)(dy + v * Co + 8 * g);
+            const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[2 * j] += bf2f((uint16_t)(w4[j] & 0xFFFFu));
+                acc[2 * j + 1] += bf2f((uint16_t)(w4[j] >> 16));
+            }
+        }
+    for (int c = threadIdx.x; c < Co; c += 256) part[c] = 0.0f;
+    __syncthreads();
+    if (vo < per)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) atomicAdd(&part[8 * g + j], acc[j]);
+    __syncthreads();
+    for (int c = threadIdx.x; c < Co; c += 256) atomicAdd(&db[c], part[c]);
 }
 
 // g = (a > 0) ? g : 0  (backward of a ReLU whose output is a).
@@ -696,12 +813,31 @@ int bmx_conv3d_wgrad(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W
     rc = check_conv_shape(N, D, H, W, Ci, Co);
     if (rc) return rc;
     BMX_CHECK(x && dy && dw, "bmx_conv3d_wgrad: null pointer");
+    BMX_CHECK(Co <= 256, "bmx_conv3d_wgrad: at most 256 output channels");
     Shape s{N, D, H, W};
     const int64_t nvox = (int64_t)N * D * H * W;
-    int64_t chunk = 4096;
-    dim3 grid((unsigned)((nvox + chunk - 1) / chunk), (unsigned)(27 * (Ci / 16)), (unsigned)(Co / 16));
-    k_conv_wgrad<<<grid, 64, 0, ctx->stream>>>(x, s, Ci, dy, Co, chunk, dw, db);
+    const int cot = Co % 64 == 0 ? 4 : (Co % 32 == 0 ? 2 : 1);
+    // z chunk per block: about 1024 blocks in all, each adding its sums once
+    const int64_t base = (int64_t)((W + kWgTX - 1) / kWgTX) * ((H + kWgTY - 1) / kWgTY) * N * (Co / (16 * cot)) *
+                         (Ci / 16);
+    const int64_t want = std::min<int64_t>(std::max<int64_t>((1024 + base - 1) / base, 1), D);
+    const int zc = (int)((D + want - 1) / want);
+    const int64_t nz = (int64_t)N * ((D + zc - 1) / zc) * (Co / (16 * cot)) * (Ci / 16);
+    BMX_CHECK(nz < ((int64_t)1 << 31), "bmx_conv3d_wgrad: grid too large");
+    dim3 grid((unsigned)((W + kWgTX - 1) / kWgTX), (unsigned)((H + kWgTY - 1) / kWgTY), (unsigned)nz);
+    if (cot == 4)
+        k_conv_wgrad_mm<4><<<grid, 192, wg_lds_bytes(4), ctx->stream>>>(x, s, Ci, dy, Co, zc, dw);
+    else if (cot == 2)
+        k_conv_wgrad_mm<2><<<grid, 192, wg_lds_bytes(2), ctx->stream>>>(x, s, Ci, dy, Co, zc, dw);
+    else
+        k_conv_wgrad_mm<1><<<grid, 192, wg_lds_bytes(1), ctx->stream>>>(x, s, Ci, dy, Co, zc, dw);
     BMX_LAUNCH_CHECK();
+    if (db) {
+        const int64_t per = 256 / (Co / 8);
+        const int64_t blocks = std::min<int64_t>((nvox + per - 1) / per, 2048);
+        k_bias_grad<<<(unsigned)blocks, 256, 0, ctx->stream>>>(dy, nvox, Co, db);
+        BMX_LAUNCH_CHECK();
+    }
     return BMX_OK;
 }
 

@@ -84,6 +84,28 @@ def test_conv_wgrad(eng, shape, ci, co):
     np.testing.assert_allclose(dbb.download((co,), np.float32), db, rtol=1e-4, atol=1e-3)
 
 
+@pytest.mark.parametrize("shape", [(1, 9, 13, 70), (2, 40, 6, 33)])
+@pytest.mark.parametrize("ci,co", [(16, 32), (64, 128), (192, 64)])
+def test_conv_wgrad_tiles(eng, shape, ci, co):
+    """Weight gradient over several (x, y) tiles and z chunks of a block (x
+    past 32 and 64, y past 4, depth 40 split into chunks), every output-channel
+    tiling (16 x 2 and 16 x 4 per block, two output-channel blocks) and twelve
+    input-channel blocks."""
+    rng = np.random.default_rng(ci * 5 + co + shape[3])
+    x = O.bf16_round(rng.standard_normal(shape + (ci,)))
+    dy = O.bf16_round(rng.standard_normal(shape + (co,)))
+    xb, dyb = up_bf16(eng, x), up_bf16(eng, dy)
+    dwb = eng.alloc(co * 27 * ci * 4)
+    dbb = eng.alloc(co * 4)
+    dwb.zero()
+    dbb.zero()
+    _hip.call("bmx_conv3d_wgrad", eng.ctx, xb.ptr, *shape, ci, dyb.ptr, co, dwb.ptr, dbb.ptr)
+    dw, db = O.conv3d_wgrad(x, dy)
+    got = dwb.download((co, 27, ci), np.float32)
+    np.testing.assert_allclose(got, dw, rtol=1e-4, atol=1e-4 * np.abs(dw).max())
+    np.testing.assert_allclose(dbb.download((co,), np.float32), db, rtol=1e-4, atol=1e-3)
+
+
 def test_pool_up_concat_split(eng):
     rng = np.random.default_rng(9)
     shape = (2, 4, 6, 8)
