@@ -593,23 +593,32 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, in
                                               float *__restrict__ db, float *__restrict__ loss)
 {
     constexpr int C = 16, LM = 32;
+    // padded rows of the wave's staged gradients (odd: lanes writing their rows hit distinct banks) and
+    // inputs (16-byte aligned: a lane reads its 8 channels with two 16-byte reads)
+    constexpr int GP = LM + 1, XP = C + 4;
+    // per wave: the logit gradients [64][GP] and inputs [64][XP] of its 64 voxels
+    __shared__ float s_g[4][64 * GP], s_x[4][64 * XP];
     __shared__ float s_dw[LM * C], s_db[LM], s_loss;
     const bool train = target || tidx;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (train) {
         for (int i = threadIdx.x; i < LM * C; i += blockDim.x) s_dw[i] = 0.0f;
         if (threadIdx.x < LM) s_db[threadIdx.x] = 0.0f;
         if (threadIdx.x == 0) s_loss = 0.0f;
         __syncthreads();
     }
+    // this lane's share of the weight gradient: label lane % 32, channels
+    // 8 (lane / 32) .. +7, summed over every voxel the wave visits
+    const int ml = lane & 31, mh = lane >> 5;
+    float wacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.0f, ls = 0.0f;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    // the loop bound is uniform per block (the wave shuffles need every lane)
+    // the loop bound is uniform per block
     for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < nvox; v0 += stride) {
         const int64_t v = v0 + threadIdx.x;
         const bool act = v < nvox;
         float xin[C];
         float lg[LM];
-        float gl[LM];
-        float ls = 0.0f;
         for (int c = 0; c < C; ++c) xin[c] = act ? bf2f(x[v * C + c]) : 0.0f;
         float mx = -1e30f;
         for (int l = 0; l < LM; ++l) {
@@ -628,40 +637,51 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, in
                 den += lg[l];
             }
         for (int l = 0; l < LM; ++l) {
-            gl[l] = 0.0f;
+            float gl = 0.0f;
             if (l < L) {
                 const float p = lg[l] / den;
                 if (act && prob) prob[v * L + l] = p;
                 if (train) {
                     const float w = (act && (!vmask || vmask[v])) ? 1.0f : 0.0f;
                     const float t = !act ? 0.0f : (target ? target[v * L + l] : (tidx[v] == l ? 1.0f : 0.0f));
-                    gl[l] = w * scale * (p - t);
+                    gl = w * scale * (p - t);
                     if (w > 0.0f && t > 0.0f) ls -= t * __logf(fmaxf(p, 1e-30f));
                 }
             }
+            lg[l] = gl;   // from here on: the logit gradient
         }
         if (!train) continue;
         if (act) {
             for (int c = 0; c < C; ++c) {
                 float s = 0.0f;
-                for (int l = 0; l < L; ++l) s += W[l * C + c] * gl[l];
+                for (int l = 0; l < L; ++l) s += W[l * C + c] * lg[l];
                 dx[v * C + c] = f2bf(s);
             }
         }
-        for (int l = 0; l < L; ++l) {
-            float g = gl[l];
-            for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
-            if ((threadIdx.x & 63) == 0) atomicAdd(&s_db[l], g);
-            for (int c = 0; c < C; ++c) {
-                float gw = gl[l] * xin[c];
-                for (int off = 32; off > 0; off >>= 1) gw += __shfl_xor(gw, off, 64);
-                if ((threadIdx.x & 63) == 0) atomicAdd(&s_dw[l * C + c], gw);
-            }
+        // dW[l][c] += sum over the wave's voxels of g[v][l] x[v][c], db[l] += g[v][l]:
+        // each lane stages its voxel, then sums its entries over the 64 staged voxels
+        float *sg = s_g[wv], *sx = s_x[wv];
+        for (int l = 0; l < LM; ++l) sg[lane * GP + l] = lg[l];
+        for (int c = 0; c < C; ++c) sx[lane * XP + c] = xin[c];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int u = 0; u < 64; ++u) {
+            const float g = sg[u * GP + ml];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wacc[j] += g * sx[u * XP + 8 * mh + j];
+            bacc += g;
         }
-        for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&s_loss, ls);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     if (!train) return;
+    for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off, 64);
+    if (ml < L) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) atomicAdd(&s_dw[ml * C + 8 * mh + j], wacc[j]);
+        if (mh == 0) atomicAdd(&s_db[ml], bacc);
+    }
+    if (lane == 0) atomicAdd(&s_loss, ls);
     __syncthreads();
     for (int i = threadIdx.x; i < L * C; i += blockDim.x) atomicAdd(&dW[i], s_dw[i]);
     if (threadIdx.x < L) atomicAdd(&db[threadIdx.x], s_db[threadIdx.x]);
