@@ -581,9 +581,9 @@ __global__ void k_split(const uint16_t *__restrict__ y, int64_t nvox, uint16_t *
 // 1x1x1 head: logits = W x + b over C=16 input channels, L outputs;
 // prob = softmax(logits).  With a target (soft labels, float [v][L], or label
 // indices) it also writes dx (bf16, scale/nvox * W^T (prob - target)) and
-// accumulates dW, db and the summed cross-entropy into *loss: per wave by
-// shuffles, per block in LDS over the voxels the block strides over, then one
-// float atomic per entry and block.
+// accumulates dW, db and the summed cross-entropy into *loss: each lane sums
+// its share of dW/db over the voxels its wave visits (staged in LDS), then per
+// block in LDS, then one float atomic per entry and block.
 __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, int64_t nvox, int L,
                                               const float *__restrict__ W, const float *__restrict__ b,
                                               float *__restrict__ prob, const float *__restrict__ target,
@@ -709,6 +709,150 @@ __global__ void k_add_bf16(uint16_t *__restrict__ y, const uint16_t *__restrict_
 {
     for (int64_t i = bmx_gtid(); i < n; i += bmx_gstride()) y[i] = f2bf(bf2f(y[i]) + bf2f(x[i]));
 }
+
+// The same operations on 8 channels (16 bytes) per work-item, for tensors
+// whose channel counts are multiples of 8 (the network's are multiples of 16)
+// and whose pointers are 16-byte aligned; element for element the results are
+// those of the kernels above.
+__device__ __forceinline__ void unpack8(uint4 q, float *f)
+{
+    const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        f[2 * j] = bf2f((uint16_t)(w4[j] & 0xFFFFu));
+        f[2 * j + 1] = bf2f((uint16_t)(w4[j] >> 16));
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float *f)
+{
+    return make_uint4((uint32_t)f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16), (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16),
+                      (uint32_t)f2bf(f[4]) | ((uint32_t)f2bf(f[5]) << 16), (uint32_t)f2bf(f[6]) | ((uint32_t)f2bf(f[7]) << 16));
+}
+
+__global__ void k_up2_v8(const uint16_t *__restrict__ x, Shape s, int C, uint16_t *__restrict__ y)
+{
+    const int D2 = s.D * 2, H2 = s.H * 2, W2 = s.W * 2, G = C / 8;
+    const int64_t n_out = (int64_t)s.N * D2 * H2 * W2 * G;
+    for (int64_t i = bmx_gtid(); i < n_out; i += bmx_gstride()) {
+        const int g = (int)(i % G);
+        int64_t t = i / G;
+        const int xx = (int)(t % W2); t /= W2;
+        const int yy = (int)(t % H2); t /= H2;
+        const int z = (int)(t % D2);
+        const int n = (int)(t / D2);
+        *(uint4 *)(y + i * 8) = *(const uint4 *)(x + ((((int64_t)n * s.D + z / 2) * s.H + yy / 2) * s.W + xx / 2) * C + 8 * g);
+    }
+}
+
+__global__ void k_up2_bwd_v8(const uint16_t *__restrict__ dy, Shape s, int C, uint16_t *__restrict__ dx)
+{
+    const int G = C / 8;
+    const int64_t n_small = (int64_t)s.N * s.D * s.H * s.W * G;
+    for (int64_t i = bmx_gtid(); i < n_small; i += bmx_gstride()) {
+        const int g = (int)(i % G);
+        int64_t t = i / G;
+        const int xo = (int)(t % s.W); t /= s.W;
+        const int yo = (int)(t % s.H); t /= s.H;
+        const int zo = (int)(t % s.D);
+        const int n = (int)(t / s.D);
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < 8; ++k) {
+            const int z = 2 * zo + (k >> 2), yy = 2 * yo + ((k >> 1) & 1), xx = 2 * xo + (k & 1);
+            float f[8];
+            unpack8(*(const uint4 *)(dy + ((((int64_t)n * 2 * s.D + z) * 2 * s.H + yy) * 2 * s.W + xx) * C + 8 * g), f);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += f[j];
+        }
+        *(uint4 *)(dx + i * 8) = pack8(acc);
+    }
+}
+
+__global__ void k_pool2_bwd_v8(const uint16_t *__restrict__ dy, Shape s, int C, uint16_t *__restrict__ dx)
+{
+    const int G = C / 8;
+    const int64_t n_in = (int64_t)s.N * s.D * s.H * s.W * G;
+    for (int64_t i = bmx_gtid(); i < n_in; i += bmx_gstride()) {
+        const int g = (int)(i % G);
+        int64_t t = i / G;
+        const int xx = (int)(t % s.W); t /= s.W;
+        const int yy = (int)(t % s.H); t /= s.H;
+        const int z = (int)(t % s.D);
+        const int n = (int)(t / s.D);
+        float f[8];
+        unpack8(*(const uint4 *)(dy + ((((int64_t)n * (s.D / 2) + z / 2) * (s.H / 2) + yy / 2) * (s.W / 2) + xx / 2) * C +
+                                 8 * g), f);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) f[j] *= 0.125f;
+        *(uint4 *)(dx + i * 8) = pack8(f);
+    }
+}
+
+__global__ void k_concat_v8(const uint16_t *__restrict__ a, int Ca, const uint16_t *__restrict__ b, int Cb,
+                            int64_t nvox, uint16_t *__restrict__ y)
+{
+    const int G = (Ca + Cb) / 8, Ga = Ca / 8;
+    for (int64_t i = bmx_gtid(); i < nvox * G; i += bmx_gstride()) {
+        const int64_t v = i / G;
+        const int g = (int)(i - v * G);
+        *(uint4 *)(y + i * 8) = g < Ga ? *(const uint4 *)(a + v * Ca + 8 * g) : *(const uint4 *)(b + v * Cb + 8 * (g - Ga));
+    }
+}
+
+__global__ void k_split_v8(const uint16_t *__restrict__ y, int64_t nvox, uint16_t *__restrict__ a, int Ca,
+                           uint16_t *__restrict__ b, int Cb, int accumulate_a)
+{
+    const int G = (Ca + Cb) / 8, Ga = Ca / 8;
+    for (int64_t i = bmx_gtid(); i < nvox * G; i += bmx_gstride()) {
+        const int64_t v = i / G;
+        const int g = (int)(i - v * G);
+        const uint4 q = *(const uint4 *)(y + i * 8);
+        if (g < Ga) {
+            uint16_t *pa = a + v * Ca + 8 * g;
+            if (accumulate_a) {
+                float f[8], e[8];
+                unpack8(q, f);
+                unpack8(*(const uint4 *)pa, e);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) f[j] += e[j];
+                *(uint4 *)pa = pack8(f);
+            } else {
+                *(uint4 *)pa = q;
+            }
+        } else {
+            *(uint4 *)(b + v * Cb + 8 * (g - Ga)) = q;
+        }
+    }
+}
+
+__global__ void k_relu_mask_v8(uint16_t *__restrict__ g, const uint16_t *__restrict__ a, int64_t n8)
+{
+    for (int64_t i = bmx_gtid(); i < n8; i += bmx_gstride()) {
+        float f[8];
+        unpack8(*(const uint4 *)(a + i * 8), f);
+        uint4 q = *(const uint4 *)(g + i * 8);
+        uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!(f[2 * j] > 0.0f)) w4[j] &= 0xFFFF0000u;
+            if (!(f[2 * j + 1] > 0.0f)) w4[j] &= 0x0000FFFFu;
+        }
+        *(uint4 *)(g + i * 8) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    }
+}
+
+__global__ void k_add_bf16_v8(uint16_t *__restrict__ y, const uint16_t *__restrict__ x, int64_t n8)
+{
+    for (int64_t i = bmx_gtid(); i < n8; i += bmx_gstride()) {
+        float f[8], e[8];
+        unpack8(*(const uint4 *)(y + i * 8), f);
+        unpack8(*(const uint4 *)(x + i * 8), e);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) f[j] += e[j];
+        *(uint4 *)(y + i * 8) = pack8(f);
+    }
+}
+
+static bool al16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 __global__ void k_f2bf(const float *__restrict__ a, uint16_t *__restrict__ b, int64_t n)
 {
@@ -879,7 +1023,10 @@ int bmx_pool2_bwd(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, 
     if (rc) return rc;
     BMX_CHECK(dy && dx && D % 2 == 0 && H % 2 == 0 && W % 2 == 0, "bmx_pool2_bwd: bad arguments");
     const int64_t n = (int64_t)N * D * H * W * C;
-    k_pool2_bwd<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
+    if (C % 8 == 0 && al16(dy) && al16(dx))
+        k_pool2_bwd_v8<<<bmx_stride_blocks(n / 8, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
+    else
+        k_pool2_bwd<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -890,7 +1037,10 @@ int bmx_up2(bmx_ctx *ctx, const uint16_t *x, int N, int D, int H, int W, int C, 
     if (rc) return rc;
     BMX_CHECK(x && y, "bmx_up2: null pointer");
     const int64_t n = (int64_t)N * D * H * W * 8 * C;
-    k_up2<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
+    if (C % 8 == 0 && al16(x) && al16(y))
+        k_up2_v8<<<bmx_stride_blocks(n / 8, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
+    else
+        k_up2<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(x, Shape{N, D, H, W}, C, y);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -901,7 +1051,10 @@ int bmx_up2_bwd(bmx_ctx *ctx, const uint16_t *dy, int N, int D, int H, int W, in
     if (rc) return rc;
     BMX_CHECK(dy && dx, "bmx_up2_bwd: null pointer");
     const int64_t n = (int64_t)N * D * H * W * C;
-    k_up2_bwd<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
+    if (C % 8 == 0 && al16(dy) && al16(dx))
+        k_up2_bwd_v8<<<bmx_stride_blocks(n / 8, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
+    else
+        k_up2_bwd<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(dy, Shape{N, D, H, W}, C, dx);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -911,7 +1064,10 @@ int bmx_concat(bmx_ctx *ctx, const uint16_t *a, int Ca, const uint16_t *b, int C
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(a && b && y, "bmx_concat: null pointer");
-    k_concat<<<bmx_stride_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(a, Ca, b, Cb, nvox, y);
+    if (Ca % 8 == 0 && Cb % 8 == 0 && al16(a) && al16(b) && al16(y))
+        k_concat_v8<<<bmx_stride_blocks(nvox * (Ca + Cb) / 8, 256), 256, 0, ctx->stream>>>(a, Ca, b, Cb, nvox, y);
+    else
+        k_concat<<<bmx_stride_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(a, Ca, b, Cb, nvox, y);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -922,7 +1078,12 @@ int bmx_split(bmx_ctx *ctx, const uint16_t *y, int64_t nvox, uint16_t *a, int Ca
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(y && a && b, "bmx_split: null pointer");
-    k_split<<<bmx_stride_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(y, nvox, a, Ca, b, Cb, accumulate_a);
+    if (Ca % 8 == 0 && Cb % 8 == 0 && al16(a) && al16(b) && al16(y))
+        k_split_v8<<<bmx_stride_blocks(nvox * (Ca + Cb) / 8, 256), 256, 0, ctx->stream>>>(y, nvox, a, Ca, b, Cb,
+                                                                                          accumulate_a);
+    else
+        k_split<<<bmx_stride_blocks(nvox * (Ca + Cb), 256), 256, 0, ctx->stream>>>(y, nvox, a, Ca, b, Cb,
+                                                                                   accumulate_a);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -966,7 +1127,10 @@ int bmx_relu_mask(bmx_ctx *ctx, uint16_t *g, const uint16_t *a, int64_t n)
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(g && a, "bmx_relu_mask: null pointer");
-    k_relu_mask<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(g, a, n);
+    if (n % 8 == 0 && al16(g) && al16(a))
+        k_relu_mask_v8<<<bmx_stride_blocks(n / 8, 256), 256, 0, ctx->stream>>>(g, a, n / 8);
+    else
+        k_relu_mask<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(g, a, n);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
@@ -976,7 +1140,10 @@ int bmx_add_bf16(bmx_ctx *ctx, uint16_t *y, const uint16_t *x, int64_t n)
     int rc = bmx_set_device(ctx);
     if (rc) return rc;
     BMX_CHECK(y && x, "bmx_add_bf16: null pointer");
-    k_add_bf16<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(y, x, n);
+    if (n % 8 == 0 && al16(y) && al16(x))
+        k_add_bf16_v8<<<bmx_stride_blocks(n / 8, 256), 256, 0, ctx->stream>>>(y, x, n / 8);
+    else
+        k_add_bf16<<<bmx_stride_blocks(n, 256), 256, 0, ctx->stream>>>(y, x, n);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }

@@ -136,6 +136,32 @@ def test_pool_up_concat_split(eng):
     _hip.call("bmx_split", eng.ctx, cb.ptr, nv, a2.ptr, 16, b2.ptr, 32, 0)
     np.testing.assert_array_equal(down_bf16(a2, shape + (16,)), x)
     np.testing.assert_array_equal(down_bf16(b2, shape + (32,)), y)
+    # split accumulating into a, and the ReLU mask / bf16 add, 16 bytes at a time
+    _hip.call("bmx_split", eng.ctx, cb.ptr, nv, a2.ptr, 16, b2.ptr, 32, 1)
+    np.testing.assert_array_equal(down_bf16(a2, shape + (16,)), O.bf16_round(x + x))
+    gm = up_bf16(eng, y)
+    _hip.call("bmx_relu_mask", eng.ctx, gm.ptr, xb.ptr, nv * 16)
+    np.testing.assert_array_equal(down_bf16(gm, shape + (16,)).reshape(-1),
+                                  np.where(x.reshape(-1) > 0, y.reshape(-1)[:nv * 16], 0.0))
+    s1 = up_bf16(eng, x)
+    _hip.call("bmx_add_bf16", eng.ctx, s1.ptr, yb.ptr, nv * 16)
+    np.testing.assert_array_equal(down_bf16(s1, shape + (16,)), O.bf16_round(x + y.reshape(-1)[:nv * 16].reshape(x.shape)))
+
+
+def test_concat_split_odd_channels(eng):
+    """Channel counts that are not multiples of 8 take the element-wise kernels."""
+    rng = np.random.default_rng(19)
+    nv = 37
+    a = O.bf16_round(rng.standard_normal((nv, 12)))
+    b = O.bf16_round(rng.standard_normal((nv, 5)))
+    ab, bb = up_bf16(eng, a), up_bf16(eng, b)
+    cb = eng.alloc(nv * 17 * 2)
+    _hip.call("bmx_concat", eng.ctx, ab.ptr, 12, bb.ptr, 5, nv, cb.ptr)
+    np.testing.assert_array_equal(down_bf16(cb, (nv, 17)), np.concatenate([a, b], -1))
+    a2, b2 = up_bf16(eng, a), eng.alloc(nv * 5 * 2)
+    _hip.call("bmx_split", eng.ctx, cb.ptr, nv, a2.ptr, 12, b2.ptr, 5, 1)
+    np.testing.assert_array_equal(down_bf16(a2, (nv, 12)), O.bf16_round(a + a))
+    np.testing.assert_array_equal(down_bf16(b2, (nv, 5)), b)
 
 
 @pytest.mark.parametrize("L,index_targets", [(9, False), (30, False), (30, True)])
