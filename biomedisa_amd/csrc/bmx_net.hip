@@ -31,6 +31,21 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 
 
+__device__ __forceinline__ void unpack8(uint4 q, float *f)
+{
+    const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        f[2 * j] = bf2f((uint16_t)(w4[j] & 0xFFFFu));
+        f[2 * j + 1] = bf2f((uint16_t)(w4[j] >> 16));
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float *f)
+{
+    return make_uint4((uint32_t)f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16), (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16),
+                      (uint32_t)f2bf(f[4]) | ((uint32_t)f2bf(f[5]) << 16), (uint32_t)f2bf(f[6]) | ((uint32_t)f2bf(f[7]) << 16));
+}
+
 struct Shape {
     int N, D, H, W;
 };
@@ -599,14 +614,21 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, in
     // per wave: the logit gradients [64][GP] and inputs [64][XP] of its 64 voxels
     __shared__ float s_g[4][64 * GP], s_x[4][64 * XP];
     __shared__ float s_dw[LM * C], s_db[LM], s_loss;
+    // the head's parameters, rows L .. 31 zero, read by every lane at the same address
+    __shared__ __align__(16) float s_w[LM * C];
+    __shared__ float s_b[LM];
     const bool train = target || tidx;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (train) {
-        for (int i = threadIdx.x; i < LM * C; i += blockDim.x) s_dw[i] = 0.0f;
-        if (threadIdx.x < LM) s_db[threadIdx.x] = 0.0f;
-        if (threadIdx.x == 0) s_loss = 0.0f;
-        __syncthreads();
+    for (int i = threadIdx.x; i < LM * C; i += blockDim.x) {
+        s_w[i] = i < L * C ? W[i] : 0.0f;
+        s_dw[i] = 0.0f;
     }
+    if (threadIdx.x < LM) {
+        s_b[threadIdx.x] = (int)threadIdx.x < L ? b[threadIdx.x] : 0.0f;
+        s_db[threadIdx.x] = 0.0f;
+    }
+    if (threadIdx.x == 0) s_loss = 0.0f;
+    __syncthreads();
     // this lane's share of the weight gradient: label lane % 32, channels
     // 8 (lane / 32) .. +7, summed over every voxel the wave visits
     const int ml = lane & 31, mh = lane >> 5;
@@ -618,57 +640,72 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t *__restrict__ x, in
         const int64_t v = v0 + threadIdx.x;
         const bool act = v < nvox;
         float xin[C];
+        if (act) {
+            unpack8(*(const uint4 *)(x + v * C), xin);
+            unpack8(*(const uint4 *)(x + v * C + 8), xin + 8);
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) xin[c] = 0.0f;
+        }
         float lg[LM];
-        for (int c = 0; c < C; ++c) xin[c] = act ? bf2f(x[v * C + c]) : 0.0f;
         float mx = -1e30f;
+#pragma unroll
         for (int l = 0; l < LM; ++l) {
-            float s = 0.0f;
-            if (l < L) {
-                s = b[l];
-                for (int c = 0; c < C; ++c) s += W[l * C + c] * xin[c];
-                mx = fmaxf(mx, s);
-            }
+            float s = s_b[l];
+#pragma unroll
+            for (int c = 0; c < C; ++c) s = __builtin_fmaf(s_w[l * C + c], xin[c], s);
             lg[l] = s;
+            if (l < L) mx = fmaxf(mx, s);
         }
         float den = 0.0f;
-        for (int l = 0; l < LM; ++l)
-            if (l < L) {
-                lg[l] = __expf(lg[l] - mx);
-                den += lg[l];
-            }
+#pragma unroll
+        for (int l = 0; l < LM; ++l) {
+            lg[l] = l < L ? __expf(lg[l] - mx) : 0.0f;
+            den += lg[l];
+        }
+        const float rden = 1.0f / den;
+        const bool wgt = act && (!vmask || vmask[v]);
+        const int ti = (train && act && !target) ? (int)tidx[v] : -1;
+#pragma unroll
         for (int l = 0; l < LM; ++l) {
             float gl = 0.0f;
             if (l < L) {
-                const float p = lg[l] / den;
+                const float p = lg[l] * rden;
                 if (act && prob) prob[v * L + l] = p;
                 if (train) {
-                    const float w = (act && (!vmask || vmask[v])) ? 1.0f : 0.0f;
-                    const float t = !act ? 0.0f : (target ? target[v * L + l] : (tidx[v] == l ? 1.0f : 0.0f));
-                    gl = w * scale * (p - t);
-                    if (w > 0.0f && t > 0.0f) ls -= t * __logf(fmaxf(p, 1e-30f));
+                    const float t = !act ? 0.0f : (target ? target[v * L + l] : (ti == l ? 1.0f : 0.0f));
+                    gl = wgt ? scale * (p - t) : 0.0f;
+                    if (wgt && t > 0.0f) ls -= t * __logf(fmaxf(p, 1e-30f));
                 }
             }
             lg[l] = gl;   // from here on: the logit gradient
         }
         if (!train) continue;
         if (act) {
+            float d[C];
+#pragma unroll
             for (int c = 0; c < C; ++c) {
                 float s = 0.0f;
-                for (int l = 0; l < L; ++l) s += W[l * C + c] * lg[l];
-                dx[v * C + c] = f2bf(s);
+#pragma unroll
+                for (int l = 0; l < LM; ++l) s = __builtin_fmaf(s_w[l * C + c], lg[l], s);
+                d[c] = s;
             }
+            *(uint4 *)(dx + v * C) = pack8(d);
+            *(uint4 *)(dx + v * C + 8) = pack8(d + 8);
         }
         // dW[l][c] += sum over the wave's voxels of g[v][l] x[v][c], db[l] += g[v][l]:
         // each lane stages its voxel, then sums its entries over the 64 staged voxels
         float *sg = s_g[wv], *sx = s_x[wv];
+#pragma unroll
         for (int l = 0; l < LM; ++l) sg[lane * GP + l] = lg[l];
+#pragma unroll
         for (int c = 0; c < C; ++c) sx[lane * XP + c] = xin[c];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         for (int u = 0; u < 64; ++u) {
             const float g = sg[u * GP + ml];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) wacc[j] += g * sx[u * XP + 8 * mh + j];
+            for (int j = 0; j < 8; ++j) wacc[j] = __builtin_fmaf(g, sx[u * XP + 8 * mh + j], wacc[j]);
             bacc += g;
         }
         __builtin_amdgcn_wave_barrier();
@@ -714,21 +751,6 @@ __global__ void k_add_bf16(uint16_t *__restrict__ y, const uint16_t *__restrict_
 // whose channel counts are multiples of 8 (the network's are multiples of 16)
 // and whose pointers are 16-byte aligned; element for element the results are
 // those of the kernels above.
-__device__ __forceinline__ void unpack8(uint4 q, float *f)
-{
-    const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        f[2 * j] = bf2f((uint16_t)(w4[j] & 0xFFFFu));
-        f[2 * j + 1] = bf2f((uint16_t)(w4[j] >> 16));
-    }
-}
-__device__ __forceinline__ uint4 pack8(const float *f)
-{
-    return make_uint4((uint32_t)f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16), (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16),
-                      (uint32_t)f2bf(f[4]) | ((uint32_t)f2bf(f[5]) << 16), (uint32_t)f2bf(f[6]) | ((uint32_t)f2bf(f[7]) << 16));
-}
-
 __global__ void k_up2_v8(const uint16_t *__restrict__ x, Shape s, int C, uint16_t *__restrict__ y)
 {
     const int D2 = s.D * 2, H2 = s.H * 2, W2 = s.W * 2, G = C / 8;
