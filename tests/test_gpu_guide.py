@@ -372,6 +372,24 @@ def test_trained_network_gain(eng):
     assert res["untrained"][1] < 0.99 or res["untrained"][0] > 0.9
 
 
+@pytest.mark.parametrize("seed", [7106, 7105])
+def test_guided_xy_labels_agree_with_plain(eng, seed):
+    """Labels on XY planes only (every 8th, 256^3, 8 structures; volumes not
+    used for training, calibration or the benchmark): the drop-in guided call
+    with its XY defaults agrees with the plain walker at per-label Dice >= 0.99
+    (with the all-axis defaults seed 7106 fell to 0.958)."""
+    from biomedisa_amd import smart_interpolation
+    from biomedisa_amd.helpers import per_label_dice
+    from biomedisa_amd.synth import structures_volume
+    vol = structures_volume(shape=(256, 256, 256), seed=seed, n_struct=8, radius=(5, 60), every=8, allaxis=False)
+    common = dict(nbrw=10, sorw=4000, allaxis=False, smooth=0)
+    p = smart_interpolation(vol.img, vol.labels, method="plain", **common)["regular"]
+    g = smart_interpolation(vol.img, vol.labels, method="guided", **common)["regular"]
+    d = per_label_dice(g, p)
+    print("seed %d min per-label Dice %.4f" % (seed, min(d.values())))
+    assert min(d.values()) >= 0.99
+
+
 def test_slab_inference_equals_whole_volume(eng):
     """Slab-wise prediction (z-slabs with a 32-plane halo, 8-aligned) gives the
     whole-volume prediction bit for bit: the receptive field is 29 voxels."""

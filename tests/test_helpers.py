@@ -133,3 +133,15 @@ def test_guided_label_limit_falls_back_to_plain(monkeypatch):
                                 nbrw=10, sorw=50, allaxis=True, smooth=3, key=9)
     assert calls == [(S.L_MAX + 1, 10, 50, True, 3, 9)]
     assert out["fallback"] is True and out["walked_seeds"] == 7 and out["ring_seeds"] == 0
+
+
+def test_guided_defaults_for_xy_labels():
+    """XY-only labels get the stricter guided settings, with ring walk counts
+    that divide nbrw; all-axis labels keep the pipeline's own defaults."""
+    from biomedisa_amd.interpolation import guided_defaults
+    assert guided_defaults(10, True) == {}
+    assert guided_defaults(10, False) == {"thr": 0.95, "brad": 6, "far_walks": 2, "mid_walks": 5}
+    for nbrw in range(1, 40):
+        g = guided_defaults(nbrw, False)
+        assert nbrw % g["far_walks"] == 0 and nbrw % g["mid_walks"] == 0
+        assert 1 <= g["far_walks"] <= g["mid_walks"] <= max(nbrw // 2, 1)
