@@ -12,7 +12,8 @@
 // image and hit counters, both with one more voxel of halo, in LDS and runs
 // every walker of the bucket - walkers whose current voxel lies in the core -
 // until the walker steps out of the region, finishes its walk, or has made
-// `cap` steps in this round.  Every step starts in the region, so its
+// `cap` steps in this round (or, once the bucket is handed out, its wave is
+// down to half its lanes: the drain pause).  Every step starts in the region, so its
 // neighbour reads and its hit stay in LDS.  At the end the workgroup adds
 // each non-zero LDS counter to the hit volume with one global atomic.
 // Between rounds the walkers are counting-sorted by (core tile, label) again.
@@ -670,7 +671,15 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
                                         lmax0, lmax1, lmax2, weight, acc, hg, g0, g.sz, g.sy);
             const bool inside =
                 max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
-            if (step >= sorw || !inside || step - step0 >= cap) {
+            // Drain pause: once the bucket has no walkers left to hand out, a
+            // wave with at most half its lanes busy stops them like a cap stop
+            // (they resume next round, grouped with other walkers, instead of
+            // keeping the wave's 64 lanes for a few long walks).  A walker's
+            // path is a function of its seed, walk and step only and the hit
+            // counts are sums, so where it pauses does not change the result.
+            bool pause = false;
+            if (*(volatile uint32_t *)&next >= cnt) pause = __popcll(__ballot(true)) <= 32;
+            if (step >= sorw || !inside || step - step0 >= cap || pause) {
                 const uint32_t pos =
                     (uint32_t)(bz + lz) * g.sz + (uint32_t)(by + ly) * g.sy + (uint32_t)(bx + lx);
                 uint2 d = make_uint2(kDestDead, 0u);
