@@ -377,8 +377,11 @@ __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict_
 // Buckets of fewer walkers than this are walked without the LDS tile in the
 // graph-driven rounds: a wave per bucket, a lane per walker, image reads and
 // hits in global memory.  Staging a whole tile for a few dozen walkers leaves
-// the workgroup waiting on one wave's dependent steps.
-constexpr uint32_t kTinyWalkers = 64;
+// the workgroup waiting on one wave's dependent steps; walking from global
+// memory costs a device-scope atomic per step, whose rate is the device's
+// (a wave that took several tiny buckets at once was slower, not faster), so
+// the threshold stays low.
+constexpr uint32_t kTinyWalkers = 32;
 
 // Weight exp(-(B-A)^2 * div1) of the reference (pyopencl_small.py:187-190).
 __device__ __forceinline__ float weight_nonpos(float B, float A, float div1)
