@@ -413,63 +413,87 @@ __device__ __noinline__ void hit16_wrap(uint32_t *hg, int64_t gi, bool high, uin
     if (!high) atomicAdd(&hg[gi + 1], (old >> 16) == 0xFFFFu ? 0xFFFFu : 0xFFFFFFFFu);
 }
 
+// Tile coordinates packed in one word, P = lz << 20 | ly << 10 | lx: 10-bit
+// fields, every tile coordinate below 32, so a field can be tested with a
+// carry into (or a borrow out of) its bit 5 without disturbing its neighbours.
+constexpr uint32_t kP1 = (1u << 20) | (1u << 10) | 1u;   // 1 in every field
+constexpr uint32_t kPBit5 = kP1 << 5;                    // bit 5 of every field
+__device__ __forceinline__ uint32_t ppack(int z, int y, int x)
+{
+    return ((uint32_t)z << 20) | ((uint32_t)y << 10) | (uint32_t)x;
+}
+__device__ __forceinline__ int pz(uint32_t P) { return (int)(P >> 20); }
+__device__ __forceinline__ int py(uint32_t P) { return (int)((P >> 10) & 1023u); }
+__device__ __forceinline__ int px(uint32_t P) { return (int)(P & 1023u); }
+// Every field of P is in [1, R] (fields in [0, R + 1] <= 31): a field f = R + 1
+// sets bit 5 of f + 31 - R, and f = 0 borrows in P - kP1, which sets its bit
+// 5 (the borrow may flag the next field too, which changes nothing).
+template <int R>
+__device__ __forceinline__ bool p_inside(uint32_t P)
+{
+    static_assert(R + 1 <= 31, "packed tile coordinates are below 32");
+    return (((P + (uint32_t)(31 - R) * kP1) | (P - kP1)) & kPBit5) == 0u;
+}
+// Every field of P (in [0, 31]) is in [lo, hi] of LO and HI (fields in
+// [0, 31]): a field below lo borrows in P - LO, one above hi borrows in HI - P.
+__device__ __forceinline__ bool p_within(uint32_t P, uint32_t LO, uint32_t HI)
+{
+    return (((P - LO) | (HI - P)) & kPBit5) == 0u;
+}
+
 // The steps of one Philox block (4 steps) that one walker takes in this
-// tile.  Local coordinates (lz,ly,lx) address the LDS tile, whose voxel
+// tile.  Local coordinates (lz,ly,lx), packed in P, address the LDS tile, whose voxel
 // (1,1,1) is the region origin; the region proper is [1,R]^3 and the tile adds
-// one voxel on each side for neighbour reads.  li is the walker's index in the
-// tile.  Step s uses word s & 3 of block s >> 2, so a walker whose visit began
+// one voxel on each side for neighbour reads.  lq is the walker's index in the
+// tile minus RI^2 (the index of its -z neighbour), so that its six neighbours
+// and its counter are at non-negative constant offsets from lq (LDS
+// instructions take unsigned offsets).  Step s uses word s & 3 of block s >> 2, so a walker whose visit began
 // or ended inside a block resumes at the right word.  A step is taken only
 // from a voxel of the region: its neighbours and its landing voxel are in the
 // tile, so every read and hit stays in LDS; a walker that steps out of the
 // region stops, and the next round continues it from the tile that owns its
 // voxel.  EDGE: the tile may reach the volume boundary, so moves are checked
-// against [lmin, lmax] per axis (the local range of interior voxels);
+// against the local range of interior voxels (LO, HI: packed per axis,
+// clamped to [0, 31]; the walker's other coordinates are interior already);
 // otherwise every move is interior.
 template <typename T, int S, bool EDGE>
 __device__ __forceinline__ void tile_block(const T *simg, uint32_t *shits, int sorw, const uint32_t rr[4],
-                                           float B, float div1, int &step, int &li, int &lz, int &ly, int &lx,
-                                           int lmin0, int lmin1, int lmin2, int lmax0, int lmax1, int lmax2,
-                                           uint32_t weight, uint32_t &acc, uint32_t *hg, int64_t g0,
+                                           float B, float div1, int &step, int &lq, uint32_t &P, uint32_t LO,
+                                           uint32_t HI, uint32_t weight, uint32_t &acc, uint32_t *hg, int64_t g0,
                                            uint32_t gsz, uint32_t gsy)
 {
     constexpr int R = TileShape<T, S>::C + 2 * TileShape<T, S>::H, RI = R + 2;
     static_assert(TileShape<T, S>::HB == 4 || RI % 2 == 0, "16-bit counters pair voxels of one tile row");
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const bool in = max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
-        if (in && step < sorw && (step & 3) == q) {
-            const float a0 = (float)simg[li + RI * RI];
-            const float a1 = (float)simg[li - RI * RI];
-            const float a2 = (float)simg[li + RI];
-            const float a3 = (float)simg[li - RI];
-            const float a4 = (float)simg[li + 1];
-            const float a5 = (float)simg[li - 1];
+        if (p_inside<R>(P) && step < sorw && (step & 3) == q) {
+            const float a0 = (float)simg[lq + 2 * RI * RI];
+            const float a1 = (float)simg[lq];
+            const float a2 = (float)simg[lq + RI * RI + RI];
+            const float a3 = (float)simg[lq + RI * RI - RI];
+            const float a4 = (float)simg[lq + RI * RI + 1];
+            const float a5 = (float)simg[lq + RI * RI - 1];
             const int d = choose(weight_nonpos(B, a0, div1), weight_nonpos(B, a1, div1),
                                  weight_nonpos(B, a2, div1), weight_nonpos(B, a3, div1),
                                  weight_nonpos(B, a4, div1), weight_nonpos(B, a5, div1), u01(rr[q]));
             const int axis = d >> 1;
             const int sgn = 1 - ((d & 1) << 1);
-            bool ok = true;
-            if (EDGE) {
-                const int c = (axis == 0 ? lz : (axis == 1 ? ly : lx)) + sgn;
-                const int lo = axis == 0 ? lmin0 : (axis == 1 ? lmin1 : lmin2);
-                const int hi = axis == 0 ? lmax0 : (axis == 1 ? lmax1 : lmax2);
-                ok = (c >= lo) & (c <= hi);
-            }
-            if (ok) {
-                lz += axis == 0 ? sgn : 0;
-                ly += axis == 1 ? sgn : 0;
-                lx += axis == 2 ? sgn : 0;
+            const uint32_t pmag = axis == 0 ? (1u << 20) : (axis == 1 ? (1u << 10) : 1u);
+            const uint32_t nP = P + (uint32_t)(sgn * (int)pmag);
+            if (!EDGE || p_within(nP, LO, HI)) {
+                P = nP;
                 const int lmag = axis == 0 ? RI * RI : (axis == 1 ? RI : 1);
-                li += sgn * lmag;
+                lq += sgn * lmag;
                 ++acc;
                 if (TileShape<T, S>::HB == 4) {
-                    atomicAdd(&shits[li], weight);
+                    atomicAdd(&shits[lq + RI * RI], weight);
                 } else {
-                    const uint32_t sh = ((uint32_t)li & 1u) << 4;
-                    const uint32_t old = atomicAdd(&shits[li >> 1], weight << sh);
-                    if (((old >> sh) & 0xFFFFu) + weight > 0xFFFFu)
-                        hit16_wrap(hg, g0 + (int64_t)lz * gsz + (int64_t)ly * gsy + lx, sh != 0u, old);
+                    // counter index lq + RI^2 (RI^2 is even: same parity as lq;
+                    // lq may be negative in the tile's rim, >> floors)
+                    const uint32_t sh = ((uint32_t)lq & 1u) << 4;
+                    const uint32_t old = atomicAdd(&shits[(lq >> 1) + RI * RI / 2], weight << sh);
+                    if (__builtin_amdgcn_ubfe(old, sh, 16u) > 0xFFFFu - weight)
+                        hit16_wrap(hg, g0 + (int64_t)pz(P) * gsz + (int64_t)py(P) * gsy + px(P), sh != 0u, old);
                 }
             }
             ++step;
@@ -630,6 +654,9 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
         const int lmax0 = g.Z - 2 - bz, lmax1 = g.Y - 2 - by, lmax2 = g.X - 2 - bx;
         const bool edge = (lmin0 > -3) | (lmin1 > -3) | (lmin2 > -3) | (lmax0 < R + 4) |
                           (lmax1 < R + 4) | (lmax2 < R + 4);
+        const auto c31 = [](int v) { return min(max(v, 0), 31); };
+        const uint32_t LO = ppack(c31(lmin0), c31(lmin1), c31(lmin2));
+        const uint32_t HI = ppack(c31(lmax0), c31(lmax1), c31(lmax2));
 
         uint32_t *__restrict__ hg = hits + gv.hits_base + (int64_t)label * gv.V;
         // global index of tile voxel (0,0,0) (negative at the volume's low faces)
@@ -637,7 +664,8 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
         uint32_t acc = 0;  // accepted moves of this thread in this bucket (< cnt * cap < 2^32)
         bool have = false;
         uint32_t j = 0, p0 = 0, seed = 0, wl = 0, weight = 0;
-        int step = 0, step0 = 0, lz = 0, ly = 0, lx = 0, li = 0;
+        int step = 0, step0 = 0, lq = 0;
+        uint32_t P = 0;   // packed tile coordinates
         float B = 0.0f, div1 = 0.0f;
         while (true) {
             if (!have) {
@@ -657,23 +685,21 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
                 const uint32_t r = pos - (uint32_t)z * g.sz;
                 const int y = (int)div_u32(r, gv.my, gv.shy);
                 const int x = (int)(r - (uint32_t)y * g.sy);
-                lz = z - bz;
-                ly = y - by;
-                lx = x - bx;
-                li = (lz * RI + ly) * RI + lx;
+                const int lz = z - bz, ly = y - by, lx = x - bx;
+                P = ppack(lz, ly, lx);
+                lq = (lz * RI + ly) * RI + lx - RI * RI;
                 step0 = step;
                 have = true;
             }
             const bmx_u4 r4 = bmx_philox(p0, 0u, wl & 0xFFFFu, (uint32_t)(step >> 2), k0, k1);
             const uint32_t rr[4] = {r4.x, r4.y, r4.z, r4.w};
             if (edge)
-                tile_block<T, S, true>(simg, shits, sorw, rr, B, div1, step, li, lz, ly, lx, lmin0, lmin1, lmin2,
-                                       lmax0, lmax1, lmax2, weight, acc, hg, g0, g.sz, g.sy);
+                tile_block<T, S, true>(simg, shits, sorw, rr, B, div1, step, lq, P, LO, HI, weight, acc, hg, g0,
+                                       g.sz, g.sy);
             else
-                tile_block<T, S, false>(simg, shits, sorw, rr, B, div1, step, li, lz, ly, lx, lmin0, lmin1, lmin2,
-                                        lmax0, lmax1, lmax2, weight, acc, hg, g0, g.sz, g.sy);
-            const bool inside =
-                max(max((unsigned)(lz - 1), (unsigned)(ly - 1)), (unsigned)(lx - 1)) < (unsigned)R;
+                tile_block<T, S, false>(simg, shits, sorw, rr, B, div1, step, lq, P, LO, HI, weight, acc, hg, g0,
+                                        g.sz, g.sy);
+            const bool inside = p_inside<R>(P);
             // Drain pause: once the bucket has no walkers left to hand out, a
             // wave with at most half its lanes busy stops them like a cap stop
             // (they resume next round, grouped with other walkers, instead of
@@ -683,6 +709,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
             bool pause = false;
             if (*(volatile uint32_t *)&next >= cnt) pause = __popcll(__ballot(true)) <= 32;
             if (step >= sorw || !inside || step - step0 >= cap || pause) {
+                const int lz = pz(P), ly = py(P), lx = px(P);
                 const uint32_t pos =
                     (uint32_t)(bz + lz) * g.sz + (uint32_t)(by + ly) * g.sy + (uint32_t)(bx + lx);
                 uint2 d = make_uint2(kDestDead, 0u);
