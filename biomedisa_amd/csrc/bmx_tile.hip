@@ -619,25 +619,37 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
         int tid;
         asm volatile("" : "=v"(tid) : "0"((int)threadIdx.x));
         {
-            constexpr int BATCH = 8;
+            // a thread's loads in two batches (two memory latencies per tile,
+            // and few enough registers); tile element k = tid + q * NTH at
+            // (lz, ly, lx), advanced by NTH elements per entry; image indices
+            // fit 32 bits (a volume has fewer than 2^32 voxels) and are
+            // clamped only for a tile that reaches past the volume
+            constexpr int BATCH = (NT + 2 * NTH - 1) / (2 * NTH);
+            constexpr int DZ = NTH / (RI * RI), DY = (NTH / RI) % RI, DX = NTH % RI;
+            int lz = tid / (RI * RI), ly = (tid / RI) % RI, lx = tid % RI;
+            const bool inner = bz >= 0 && by >= 0 && bx >= 0 && bz + RI <= g.Z && by + RI <= g.Y && bx + RI <= g.X;
             for (int base = 0; base < NT; base += BATCH * NTH) {
                 T v[BATCH];
 #pragma unroll
                 for (int q = 0; q < BATCH; ++q) {
-                    const int k = base + tid + q * NTH;
-                    if (k < NT) {
-                        const int lz = k / (RI * RI), ly = (k / RI) % RI, lx = k % RI;
-                        const int gz = min(max(bz + lz, 0), g.Z - 1);
-                        const int gy = min(max(by + ly, 0), g.Y - 1);
-                        const int gx = min(max(bx + lx, 0), g.X - 1);
-                        v[q] = img[(int64_t)gz * g.sz + (int64_t)gy * g.sy + gx];
+                    if (base + tid + q * NTH < NT) {
+                        int gz = bz + lz, gy = by + ly, gx = bx + lx;
+                        if (!inner) {
+                            gz = min(max(gz, 0), g.Z - 1);
+                            gy = min(max(gy, 0), g.Y - 1);
+                            gx = min(max(gx, 0), g.X - 1);
+                        }
+                        v[q] = img[(uint32_t)gz * g.sz + (uint32_t)gy * g.sy + (uint32_t)gx];
                     }
+                    lx += DX;
+                    if (lx >= RI) { lx -= RI; ++ly; }
+                    ly += DY;
+                    if (ly >= RI) { ly -= RI; ++lz; }
+                    lz += DZ;
                 }
 #pragma unroll
-                for (int q = 0; q < BATCH; ++q) {
-                    const int k = base + tid + q * NTH;
-                    if (k < NT) simg[k] = v[q];
-                }
+                for (int q = 0; q < BATCH; ++q)
+                    if (base + tid + q * NTH < NT) simg[base + tid + q * NTH] = v[q];
             }
             constexpr int NW = (int)(tile_hit_bytes<T, S>() / 4);   // 32-bit counter words
             if (tid < 27) snb[tid] = 0u;
@@ -743,12 +755,28 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
         // one counter per lane, consecutive lanes on consecutive voxels; a 16-bit
         // counter k is half k of the LDS words (little-endian: voxel 2w is the
         // low half of word w), and it is not 0 only for a voxel of the volume
-        // (hit16_wrap)
-        for (int k = tid; k < NT; k += NTH) {
-            const uint32_t c = Sh::HB == 4 ? shits[k] : (uint32_t)((const uint16_t *)shits)[k];
-            if (c) {
-                const int lz2 = k / (RI * RI), ly2 = (k / RI) % RI, lx2 = k % RI;
-                atomicAdd(&hg[g0 + (int64_t)lz2 * g.sz + (int64_t)ly2 * g.sy + lx2], c);
+        // (hit16_wrap).  Counter k = tid + q * NTH at tile (lz, ly, lx) is
+        // voxel gi of the volume, both advanced by NTH counters per q; gi is
+        // kept modulo 2^32 (g0 may be negative, a voxel index is below 2^32).
+        {
+            constexpr int DZ = NTH / (RI * RI), DY = (NTH / RI) % RI, DX = NTH % RI;
+            int fz = tid / (RI * RI), fy = (tid / RI) % RI, fx = tid % RI;
+            const uint32_t fsy = g.sy, fsz = g.sz;
+            uint32_t gi = (uint32_t)g0 + (uint32_t)fz * fsz + (uint32_t)fy * fsy + (uint32_t)fx;
+            const uint32_t dgi = (uint32_t)DZ * fsz + (uint32_t)DY * fsy + (uint32_t)DX;
+#pragma unroll 4
+            for (int q = 0; q < (NT + NTH - 1) / NTH; ++q) {
+                const int k = tid + q * NTH;
+                if (k < NT) {
+                    const uint32_t c = Sh::HB == 4 ? shits[k] : (uint32_t)((const uint16_t *)shits)[k];
+                    if (c) atomicAdd(&hg[gi], c);
+                }
+                gi += dgi;
+                fx += DX;
+                if (fx >= RI) { fx -= RI; ++fy; gi += fsy - (uint32_t)RI; }
+                fy += DY;
+                if (fy >= RI) { fy -= RI; ++fz; gi += fsz - (uint32_t)RI * fsy; }
+                fz += DZ;
             }
         }
         if (!PERSIST) break;
