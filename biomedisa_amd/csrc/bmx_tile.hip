@@ -240,9 +240,9 @@ constexpr uint32_t kDestDead = 0xFFFFFFFFu, kDestDirect = 0x80000000u;
 
 // Slot in the bucket arrays of the round's bucket i (heavy ones from the front,
 // light ones from the back).
-__device__ __forceinline__ uint32_t bucket_slot(uint32_t i, uint32_t nheavy, uint32_t nkeys)
+__device__ __forceinline__ uint32_t bucket_slot(uint32_t i, uint32_t nheavy, uint32_t nbcap)
 {
-    return i < nheavy ? i : nkeys - 1u - (i - nheavy);
+    return i < nheavy ? i : nbcap - 1u - (i - nheavy);
 }
 
 // Start of a walk: n walkers in the state array, nothing listed yet.
@@ -341,15 +341,48 @@ __global__ __launch_bounds__(256) void k_round_start(uint32_t *__restrict__ coun
     }
 }
 
+// A lane's `want` consecutive entries of the counter's range, for every lane
+// of the wave at once (one atomic per wave): the first one's index.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *ctr, uint32_t want)
+{
+    const int lane = (int)(threadIdx.x & 63u);
+    uint32_t incl = want;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, (unsigned)o, 64);
+        if (lane >= o) incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(ctr, total);
+    return __shfl(base, 63, 64) + incl - want;
+}
+
+// Keys whose walkers a round splits over several buckets: a bucket holds at
+// most max(kSplitMin, live / (2 * grid)) walkers, so no bucket holds more than
+// about half a resident workgroup's share of the round.  A few tiles can hold
+// most of a sparse walk's walkers (the repair walk of the guided method); one
+// workgroup per (tile, label) would then walk them alone while the others
+// wait.  The buckets of a key stage the same tile and add their own hits, so
+// the sums are unchanged; each has its own destination reservations.
+constexpr uint32_t kSplitMin = 1024;
+
+// Extra bucket-list entries the splits can need beyond one per key: a key of
+// c walkers takes ceil(c / S) <= c / S + 1 entries, and sum(c) / S <= 2 grid.
+__host__ __device__ constexpr int64_t split_extra(int64_t grid) { return 2 * grid + 1; }
+
 // Non-empty buckets of the exclusive offsets off[0..nkeys] -> (key, offset,
 // count): buckets of at least `heavy` walkers from the front of the arrays,
-// the others from the back (nheavy + nlight <= nkeys, so the two never meet);
-// buckets of fewer than `tiny` walkers (tiny = 0: none) into the tiny arrays
-// tkey/toff/tcnt instead.
+// the others from the back of their nbcap entries (nheavy + nlight <= nbcap,
+// so the two never meet); buckets of fewer than `tiny` walkers (tiny = 0:
+// none) into the tiny arrays tkey/toff/tcnt instead.  A key of more than S
+// walkers (see kSplitMin; `grid` resident workgroups) is listed as
+// ceil(count / S) heavy buckets of at most S walkers each.
 __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict__ off, int64_t nkeys,
-                                                     uint32_t *bkey, uint32_t *boff, uint32_t *bcnt,
-                                                     uint32_t *tkey, uint32_t *toff, uint32_t *tcnt,
-                                                     RoundCtl *ctl, uint32_t heavy, uint32_t tiny)
+                                                     uint32_t nbcap, uint32_t *bkey, uint32_t *boff,
+                                                     uint32_t *bcnt, uint32_t *tkey, uint32_t *toff,
+                                                     uint32_t *tcnt, RoundCtl *ctl, uint32_t heavy, uint32_t tiny,
+                                                     uint32_t grid)
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0, o = 0;
@@ -357,17 +390,27 @@ __global__ __launch_bounds__(256) void k_bucket_list(const uint32_t *__restrict_
         o = off[k];
         c = off[k + 1] - o;
     }
+    const uint32_t live = ctl->nnext;
+    const uint32_t share = (uint32_t)(((uint64_t)live + 2ull * grid - 1ull) / (2ull * grid));
+    const uint32_t S = max(kSplitMin, share);
     const bool is_tiny = c < tiny;
     const bool is_heavy = c >= heavy;
-    const uint32_t ih = agg_atomic_inc(&ctl->nheavy, 0u, c > 0 && is_heavy);
+    const uint32_t m = c > 0 && is_heavy ? (c + S - 1u) / S : 0u;   // heavy buckets of this key
+    const uint32_t ih = wave_reserve(&ctl->nheavy, m);
     const uint32_t il = agg_atomic_inc(&ctl->nlight, 0u, c > 0 && !is_heavy && !is_tiny);
     const uint32_t it = agg_atomic_inc(&ctl->ntiny, 0u, c > 0 && is_tiny);
     if (c > 0 && is_tiny) {
         tkey[it] = (uint32_t)k;
         toff[it] = o;
         tcnt[it] = c;
+    } else if (c > 0 && is_heavy) {
+        for (uint32_t i = 0; i < m; ++i) {
+            bkey[ih + i] = (uint32_t)k;
+            boff[ih + i] = o + i * S;
+            bcnt[ih + i] = min(S, c - i * S);
+        }
     } else if (c > 0) {
-        const uint32_t idx = is_heavy ? ih : (uint32_t)nkeys - 1u - il;
+        const uint32_t idx = nbcap - 1u - il;
         bkey[idx] = (uint32_t)k;
         boff[idx] = o;
         bcnt[idx] = c;
@@ -569,7 +612,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
     const T *__restrict__ img_all, TileGeom tg, const WalkSeed *__restrict__ wseeds,
     uint4 *__restrict__ st, const uint32_t *__restrict__ bkey, const uint32_t *__restrict__ boff,
     const uint32_t *__restrict__ bcnt, const uint32_t *__restrict__ tkey, const uint32_t *__restrict__ toff,
-    const uint32_t *__restrict__ tcnt, RoundCtl *__restrict__ ctl, uint32_t nkeys, int sorw, int cap,
+    const uint32_t *__restrict__ tcnt, RoundCtl *__restrict__ ctl, uint32_t nbcap, int sorw, int cap,
     uint32_t k0, uint32_t k1, uint32_t *__restrict__ hits, unsigned long long *accepted,
     uint2 *__restrict__ dest, uint2 *__restrict__ bres, uint32_t *__restrict__ counts)
 {
@@ -595,7 +638,7 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
         // geometry lives in scalar registers
         const uint32_t bi = PERSIST ? __builtin_amdgcn_readfirstlane(bucket) : blockIdx.x;
         if (bi >= nb) break;
-        const uint32_t slot = bucket_slot(bi, nheavy, nkeys);
+        const uint32_t slot = bucket_slot(bi, nheavy, nbcap);
         const uint32_t key = bkey[slot];
         const uint32_t off = boff[slot], cnt = bcnt[slot];
         const uint32_t label = key % tg.nlabels;
@@ -718,8 +761,12 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
             // keeping the wave's 64 lanes for a few long walks).  A walker's
             // path is a function of its seed, walk and step only and the hit
             // counts are sums, so where it pauses does not change the result.
+            // A walker pauses after 4 steps of its visit at the earliest, so a
+            // visit that does not end the walk takes at least min(4, H + 1)
+            // steps, as without pauses (a visit starts in the core, H + 1
+            // steps from the outside of the region; see min_visit_steps).
             bool pause = false;
-            if (*(volatile uint32_t *)&next >= cnt) pause = __popcll(__ballot(true)) <= 32;
+            if (*(volatile uint32_t *)&next >= cnt) pause = __popcll(__ballot(true)) <= 32 && step - step0 >= 4;
             if (step >= sorw || !inside || step - step0 >= cap || pause) {
                 const int lz = pz(P), ly = py(P), lx = px(P);
                 const uint32_t pos =
@@ -830,6 +877,16 @@ __global__ __launch_bounds__((TileShape<T, S>::NTH), (TileShape<T, S>::WPE)) voi
     }
 }
 
+// The fewest steps a bucket visit takes unless it ends the walk: a visit
+// starts in the core, H + 1 steps from the outside of the region; cap stops
+// and drain pauses come after 4 steps at the earliest (the cap is a positive
+// multiple of 4); a tiny walker walks the cap.  Bounds the rounds of a walk.
+template <typename T, int S>
+constexpr int min_visit_steps()
+{
+    return TileShape<T, S>::H + 1 < 4 ? TileShape<T, S>::H + 1 : 4;
+}
+
 // Resident workgroups of the persistent k_tile_walk<T, S> on the context's
 // device: the grid of graph-driven rounds.
 template <typename T, int S>
@@ -861,6 +918,8 @@ struct RoundArgs {
     uint32_t *counts, *bkey, *boff, *bcnt, *tkey, *toff, *tcnt;
     RoundCtl *ctl;
     int64_t nkeys;
+    uint32_t nbcap;   // bucket-list entries: nkeys + split_extra(split_grid)
+    uint32_t split_grid;
     int sorw, cap;
     uint32_t k0, k1;
     uint32_t *hits;
@@ -881,9 +940,9 @@ int round_bucket(bmx_ctx *ctx, const RoundArgs<T> &a, const uint4 *in, uint4 *ou
     if (rc) return rc;
     k_round_mid<<<1, 64, 0, ctx->stream>>>(a.counts, a.nkeys, a.ctl);
     BMX_LAUNCH_CHECK();
-    k_bucket_list<<<bmx_blocks(a.nkeys, 256), 256, 0, ctx->stream>>>(a.counts, a.nkeys, a.bkey, a.boff, a.bcnt,
-                                                                     a.tkey, a.toff, a.tcnt, a.ctl,
-                                                                     (uint32_t)TileShape<T, S>::NTH, tiny);
+    k_bucket_list<<<bmx_blocks(a.nkeys, 256), 256, 0, ctx->stream>>>(
+        a.counts, a.nkeys, a.nbcap, a.bkey, a.boff, a.bcnt, a.tkey, a.toff, a.tcnt, a.ctl,
+        (uint32_t)TileShape<T, S>::NTH, tiny, a.split_grid);
     BMX_LAUNCH_CHECK();
     k_tile_place<<<grid, 256, 0, ctx->stream>>>(in, a.dest, a.ctl, a.counts, a.bres, out);
     BMX_LAUNCH_CHECK();
@@ -896,7 +955,7 @@ template <typename T, int S, bool PERSIST>
 int round_walk(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *st, unsigned grid)
 {
     k_tile_walk<T, S, PERSIST><<<grid, TileShape<T, S>::NTH, tile_lds_bytes<T, S>(), ctx->stream>>>(
-        a.img, a.tg, a.wseeds, st, a.bkey, a.boff, a.bcnt, a.tkey, a.toff, a.tcnt, a.ctl, (uint32_t)a.nkeys,
+        a.img, a.tg, a.wseeds, st, a.bkey, a.boff, a.bcnt, a.tkey, a.toff, a.tcnt, a.ctl, a.nbcap,
         a.sorw, a.cap, a.k0, a.k1, a.hits, a.acc, a.dest, a.bres, a.counts);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
@@ -942,7 +1001,7 @@ int rounds_graph(bmx_ctx *ctx, const RoundArgs<T> &a, uint4 *A, uint4 *Bst, unsi
     hipGraphExec_t exec = nullptr;
     int rc = capture_rounds<T, S>(ctx, a, A, Bst, grid, wgrid, pairs, &exec);
     if (rc) return rc;
-    const int max_rounds = a.sorw / 4 + 2 + 2 * pairs;
+    const int max_rounds = a.sorw / min_visit_steps<T, S>() + 2 + 2 * pairs;
     RoundCtl host{};
     while (rc == BMX_OK) {
         hipError_t el = hipGraphLaunch(exec, ctx->stream);
@@ -1009,15 +1068,21 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     if (rc) return rc;
     rc = grow(ctx, &ctx->d_keys, &ctx->keys_cap, (size_t)(nkeys + 1) * sizeof(uint32_t));
     if (rc) return rc;
-    // six uint32 bucket lists, then 27 neighbour reservations per bucket
-    rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap, (size_t)nkeys * (6 * sizeof(uint32_t) + 27 * sizeof(uint2)));
+    int wgrid_max = 0;
+    rc = walk_grid<T, S>(ctx, &wgrid_max);
+    if (rc) return rc;
+    // three uint32 lists of nbcap round buckets (keys split over several
+    // buckets need up to split_extra entries more than the keys), three of
+    // nkeys tiny buckets, then 27 neighbour reservations per round bucket at
+    // an 8-byte boundary
+    const int64_t nbcap = nkeys + split_extra(wgrid_max);
+    const int64_t list_words = (3 * nbcap + 3 * nkeys + 1) & ~(int64_t)1;
+    rc = grow(ctx, &ctx->d_buckets, &ctx->buckets_cap,
+              (size_t)list_words * sizeof(uint32_t) + (size_t)nbcap * 27 * sizeof(uint2));
     if (rc) return rc;
     // the scan's block partials, allocated before any capture
     rc = grow(ctx, &ctx->d_scan, &ctx->scan_cap,
               (size_t)((nkeys + 1 + kScanTile - 1) / kScanTile) * sizeof(uint32_t));
-    if (rc) return rc;
-    int wgrid_max = 0;
-    rc = walk_grid<T, S>(ctx, &wgrid_max);
     if (rc) return rc;
 
     RoundArgs<T> a;
@@ -1026,11 +1091,13 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     a.wseeds = (const WalkSeed *)ctx->d_wseeds;
     a.counts = (uint32_t *)ctx->d_keys;
     a.bkey = (uint32_t *)ctx->d_buckets;
-    a.boff = a.bkey + nkeys;
-    a.bcnt = a.boff + nkeys;
-    a.tkey = a.bcnt + nkeys;
+    a.boff = a.bkey + nbcap;
+    a.bcnt = a.boff + nbcap;
+    a.tkey = a.bcnt + nbcap;
     a.toff = a.tkey + nkeys;
     a.tcnt = a.toff + nkeys;
+    a.nbcap = (uint32_t)nbcap;
+    a.split_grid = (uint32_t)wgrid_max;
     a.ctl = (RoundCtl *)(ctx->d_counter + 8);
     static_assert(sizeof(RoundCtl) <= 8 * sizeof(uint64_t), "RoundCtl fits the context's scratch counters");
     a.nkeys = nkeys;
@@ -1042,7 +1109,7 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
     a.acc = acc;
     uint4 *A = (uint4 *)ctx->d_state, *Bst = A + n0;
     a.dest = (uint2 *)(Bst + n0);
-    a.bres = (uint2 *)(a.tcnt + nkeys);   // 8-byte aligned: nkeys * 24 bytes in
+    a.bres = (uint2 *)(a.bkey + list_words);
     if (nseeds > 0) {
         k_walk_seeds<T><<<bmx_blocks(nseeds, 256), 256, 0, ctx->stream>>>(img, a.tg, seeds, nseeds, weight, overrides,
                                                                          (WalkSeed *)ctx->d_wseeds);
@@ -1100,7 +1167,7 @@ int walk_tiled_t(bmx_ctx *ctx, const T *img, const bmx_volume *vols, int nvol, c
         uint4 *t = A; A = Bst; Bst = t;
         n = live;
         ++round;
-        BMX_CHECK(round <= sorw / 4 + 2, "bmx_walk: tiled walker did not converge");
+        BMX_CHECK(round <= sorw / (min_visit_steps<T, S>()) + 2, "bmx_walk: tiled walker did not converge");
     }
     *launches = round < max_timed ? round : max_timed;
     return BMX_OK;

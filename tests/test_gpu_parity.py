@@ -148,6 +148,35 @@ def test_tiled_walker_bitexact(eng, cap, dtype, shape):
     assert launches >= 1 and 0 < kern <= phase
 
 
+@pytest.mark.parametrize("graph_rounds", [-1, 0])
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_tiled_walker_crowded_tiles_bitexact(eng, graph_rounds, dtype):
+    """Every walker starts on one labelled plane, so a few tiles hold them all:
+    the rounds list such a (tile, label) key as several buckets of at most
+    1024 walkers, walked by different workgroups into the same hits.  Hits
+    equal the oracle's with both round drivers."""
+    rng = np.random.default_rng(47)
+    shape = (40, 36, 38)
+    _say("tiled walker, crowded tiles, graph_rounds=%d dtype=%s" % (graph_rounds, np.dtype(dtype).name))
+    lab = np.zeros(shape, np.uint8)
+    lab[20, 6:30, 6:32] = rng.integers(1, 3, (24, 26))
+    base = rng.integers(30, 220, shape).astype(np.float32)
+    img = base.astype(np.uint8) if dtype == np.uint8 else (base + rng.random(shape)).astype(np.float32)
+    _, seeds, (lb, ib, dt, sb, n) = _gpu_seeds(eng, lab, img, False)
+    nl, nbrw, sorw, key = 3, 40, 203, 79
+    # the plane's walkers over its 3 x 3 tiles and 3 labels: keys of more than 1024 walkers
+    assert n * nbrw > 27 * 1024
+    hits = eng.alloc(nl * lab.size * 4)
+    hits.zero()
+    acc = eng.walk(ib, dt, shape, sb, n, nl, nbrw, sorw, _hip.RNG_PHILOX, key, hits,
+                   algorithm=_hip.WALK_TILED, graph_rounds=graph_rounds)
+    h = hits.download((nl,) + shape, np.uint32)
+    oh, oacc = orc.philox_walk(img, seeds["pos"].astype(np.int64), seeds["label"].astype(np.uint8),
+                               seeds["var"], nl, nbrw, sorw, key)
+    assert acc == oacc
+    np.testing.assert_array_equal(h, oh)
+
+
 def test_walk_accumulates_and_handles_empty(eng):
     shape = (8, 9, 10)
     img = eng.upload(np.full(shape, 9, np.uint8))
