@@ -247,8 +247,11 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
             return;
         }
     }
-    // box of this thread's non-zero outputs (float bits, so -0 counts as non-zero)
-    int bb[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
+    // box of this thread's non-zero outputs (float bits, so -0 counts as
+    // non-zero): bit k of `rows` for row y0 + ty*RY + k of its column, and the
+    // first and last plane (planes are computed in increasing order)
+    uint32_t rows = 0u;
+    int zlo = INT_MAX, zhi = -1;
 
     // Per-thread constants of the z-independent parts: the in-plane offset and
     // in-plane validity (inside the volume and the box) of the thread's input
@@ -338,6 +341,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
         if (r2 >= zb && r2 < ze) {
             const float (*m)[PW] = L1[slot3(r2 - 1)], (*c)[PW] = L1[slot3(r2)], (*p)[PW] = L1[slot3(r2 + 1)];
             const bool zint = r2 > 0 && r2 < Z - 1;
+            uint32_t nzrows = 0u;
 #pragma unroll
             for (int k = 0; k < RY; ++k) {
                 const int ly = 2 + ty * RY + k, lx = 2 + tx;
@@ -345,16 +349,20 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
                 if (core_in[k]) {
                     const float v = round_at(m, c, p, true, zint, true, core_int[k], ly, lx);
                     out[(int64_t)r2 * sz + (int64_t)y * sy + xc] = v;
-                    if (__float_as_uint(v) != 0u) {
-                        bb[0] = min(bb[0], r2); bb[1] = max(bb[1], r2);
-                        bb[2] = min(bb[2], y); bb[3] = max(bb[3], y);
-                        bb[4] = min(bb[4], xc); bb[5] = max(bb[5], xc);
-                    }
+                    nzrows |= __float_as_uint(v) != 0u ? 1u << k : 0u;
                 }
+            }
+            if (nzrows) {
+                zlo = min(zlo, r2);
+                zhi = r2;
+                rows |= nzrows;
             }
         }
     }
     if (rec.out) {
+        const int yr = y0 + ty * RY;
+        int bb[6] = {zlo, zhi, rows ? yr + __builtin_ctz(rows) : INT_MAX, rows ? yr + 31 - __builtin_clz(rows) : -1,
+                     rows ? xc : INT_MAX, rows ? xc : -1};
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
