@@ -45,17 +45,20 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_tile_walk_512_traffic.md")
 
 
-def measured_traffic(args):
+def measured_traffic(args, launches):
     """HBM bytes per walker launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload (None for other sizes or walk parameters)."""
+    summary of this workload (None for other sizes or walk parameters).  The
+    summary gives bytes per launch over its own launch count; the walk's total
+    does not depend on how many empty rounds close its last graph replay (40 or
+    48 launches), so it is spread over this run's `launches`."""
     if (args.size, args.nbrw, args.sorw) != (512, 10, 4000) or not os.path.exists(TRAFFIC_PROFILE):
         return None
     txt = open(TRAFFIC_PROFILE).read()
     i = txt.find("```json")
     if i < 0:
         return None
-    block = txt[i + 7:txt.index("```", i + 7)]
-    return float(json.loads(block)["traffic_bytes_per_launch"])
+    prof = json.loads(txt[i + 7:txt.index("```", i + 7)])
+    return float(prof["traffic_bytes_per_launch"]) * int(prof["launches"]) / max(int(launches), 1)
 
 
 COUNTERS_PROFILE = os.path.join(HERE, "profiles", "k_tile_walk_512_counters.md")
@@ -324,13 +327,14 @@ def main():
         walk_bytes = steps_total * 6 * 1 + st["accepted"] * 8
         achieved = walk_bytes / kernel_s / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args, launches),
                             "traffic_source": "profiles/k_tile_walk_512_traffic.md (rocprofv3 FETCH_SIZE + "
                                               "WRITE_SIZE, scaled by a copy-kernel calibration)",
                             "kernel": "k_tile_walk<uint8_t, 0, true> (persistent; rounds replayed in a HIP graph, "
                                       "kernel time = summed launch spans on the device wall clock)",
                             "walker_steps": steps_total, "steps_per_s": steps_total / kernel_s,
                             "accepted_moves": st["accepted"], "launches": launches,
+                            "kernel_ms_total": tm["walk_kernel_ms"],
                             "launch_avg_ms": tm["walk_kernel_ms"] / launches,
                             "algorithmic_bytes_per_launch": walk_bytes / launches}
         valu = measured_valu(args)

@@ -163,19 +163,35 @@ def _check(rc):
         raise BmxError(rc, msg)
 
 
+# Device allocations of at least POOL_MIN_BYTES go back to their Engine's pool
+# when freed and are handed out again for a request of the same size: the
+# pipelines allocate the same hit volumes and masks call after call, and a
+# hipMalloc of tens of GB that follows frees has taken seconds on this GPU
+# (a 73 GB hit volume: 2.2 s on its third allocation, against 13 ms to clear
+# it).  The pool holds at most POOL_MAX_BYTES (BMX_POOL_MAX_GB in the
+# environment; 0 disables the pool), room for the two hit volumes of a guided
+# 1024^3 call; an allocation that fails for lack of memory empties the pool and
+# is tried once more, and Engine.release_pool() hands the memory back to the
+# device for other users.  Every kernel of an Engine runs on its one stream, so
+# a buffer handed out again is only written after the work queued on its
+# previous owner.
+POOL_MIN_BYTES = 32 << 20
+POOL_MAX_BYTES = int(float(os.environ.get("BMX_POOL_MAX_GB", "160")) * (1 << 30))
+
+
 class DeviceBuffer:
-    """Device allocation owned by an Engine; freed on close()/garbage collection."""
+    """Device allocation owned by an Engine; freed (or pooled by the Engine)
+    on free()/garbage collection."""
 
     def __init__(self, engine, nbytes):
         self.engine = engine
         self.nbytes = int(nbytes)
-        ptr = C.c_void_p()
-        _check(load().bmx_malloc(engine.ctx, self.nbytes, C.byref(ptr)))
-        self.ptr = ptr.value
+        self.ptr = None
+        self.ptr = engine._alloc_raw(self.nbytes)
 
     def free(self):
         if self.ptr is not None and self.engine.ctx is not None:
-            _check(load().bmx_free(self.engine.ctx, self.ptr))
+            self.engine._free_raw(self.ptr, self.nbytes)
         self.ptr = None
 
     def __del__(self):
@@ -203,11 +219,45 @@ class Engine:
         _check(lib.bmx_open(int(device), C.byref(ctx)))
         self.ctx = ctx.value
         self.device = device
+        self._pool = {}      # nbytes -> device pointers free for reuse
+        self._pooled = 0     # bytes held by the pool
 
     def close(self):
         if self.ctx is not None:
+            self.release_pool()
             _check(load().bmx_close(self.ctx))
             self.ctx = None
+
+    def _alloc_raw(self, nbytes):
+        if nbytes >= POOL_MIN_BYTES:
+            free = self._pool.get(nbytes)
+            if free:
+                self._pooled -= nbytes
+                return free.pop()
+        ptr = C.c_void_p()
+        rc = load().bmx_malloc(self.ctx, nbytes, C.byref(ptr))
+        if rc == BMX_E_OOM and self._pooled:
+            self.release_pool()
+            rc = load().bmx_malloc(self.ctx, nbytes, C.byref(ptr))
+        _check(rc)
+        return ptr.value
+
+    def _free_raw(self, ptr, nbytes):
+        if nbytes >= POOL_MIN_BYTES and self._pooled + nbytes <= POOL_MAX_BYTES:
+            self._pool.setdefault(nbytes, []).append(ptr)
+            self._pooled += nbytes
+            return
+        _check(load().bmx_free(self.ctx, ptr))
+
+    def release_pool(self):
+        """Return every pooled allocation to the device."""
+        pool, self._pool, self._pooled = self._pool, {}, 0
+        for ptrs in pool.values():
+            for ptr in ptrs:
+                _check(load().bmx_free(self.ctx, ptr))
+
+    def pooled_bytes(self):
+        return self._pooled
 
     def __enter__(self):
         return self

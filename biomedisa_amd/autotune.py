@@ -12,15 +12,21 @@ walker follows its own Philox stream whatever the order of work):
                    k = k rounds per replay of one captured HIP graph with a
                    persistent walk kernel (0 = 8)
 
-tune_walk times the candidates on a sample of the workload itself - every
-seed, one walk each (1/nbrw of the work, the same seed geometry) - and keeps
-the fastest.  The search is greedy: shapes at the default cap, then caps at
+tune_walk times the candidates on a sample of the workload itself and keeps
+the fastest.  The sample is the first volumes of the batch that hold about
+1/nbrw of its seeds, walked with all nbrw walks: 1/nbrw of the work at the
+walk's own walker density per tile (which decides how full the tile buckets
+are, and so which tile shape wins).  A batch whose first volume holds more
+than that (or a single volume) is sampled with every seed walking once
+instead.  The search is greedy: shapes at the default cap, then caps at
 the best shape, then the round driver.  Results are cached in-process per
 workload class (image type, volume count, voxels and seeds to within a factor
 of 2, nbrw, sorw).
 """
 import math
 import time
+
+import numpy as np
 
 from . import _hip
 
@@ -35,14 +41,30 @@ def _class_key(batch, nbrw, sorw):
             int(math.log2(max(1, nseeds))), int(nbrw), int(sorw))
 
 
-def _time_walk(batch, hits, walks, sorw, key, opts):
+def _time_walk(batch, table, hits, walks, sorw, key, opts):
     eng = batch.eng
     hits.zero()
     eng.sync()
     t0 = time.perf_counter()
-    eng.walk_batch(batch.img, batch.img_dtype, batch.table, batch.seeds, walks, sorw, key, hits, **opts)
+    eng.walk_batch(batch.img, batch.img_dtype, table, batch.seeds, walks, sorw, key, hits, **opts)
     eng.sync()
     return time.perf_counter() - t0
+
+
+def sample_table(batch, nbrw):
+    """The fewest leading rows of batch.table whose volumes hold at least
+    1/nbrw of the seeds (their seeds are the first of the seed list, as the
+    batch walk requires), or None when those hold more than 2/nbrw of the seeds
+    or no proper subset does."""
+    counts = np.asarray(batch.nseeds, dtype=np.int64)
+    total = int(counts.sum())
+    if len(counts) < 2 or total == 0 or nbrw <= 1:
+        return None
+    cum = np.cumsum(counts)
+    m = int(np.searchsorted(cum, total / nbrw)) + 1
+    if m >= len(counts) or cum[m - 1] > 2 * total / nbrw:
+        return None
+    return batch.table[:m]
 
 
 def tune_walk(batch, nbrw, sorw, key, sample_walks=1, log=None, cache=True):
@@ -52,19 +74,23 @@ def tune_walk(batch, nbrw, sorw, key, sample_walks=1, log=None, cache=True):
     if cache and ck in _CACHE:
         return dict(_CACHE[ck])
     eng = batch.eng
-    walks = max(1, min(int(sample_walks), int(nbrw)))
+    sub = sample_table(batch, nbrw)
+    if sub is not None:
+        vols, walks = sub, int(nbrw)
+    else:
+        vols, walks = batch.table, max(1, min(int(sample_walks), int(nbrw)))
     hits = eng.alloc(max(1, batch.hits_elems) * 4)
     table = []
     try:
         def trial(opts):
-            sec = _time_walk(batch, hits, walks, sorw, key, opts)
+            sec = _time_walk(batch, vols, hits, walks, sorw, key, opts)
             table.append((dict(opts), sec))
             if log:
                 log("[autotune] %s: %.1f ms" % (opts, 1e3 * sec))
             return sec
 
         # warm-up of the default shape (first-launch costs stay out of the comparison)
-        _time_walk(batch, hits, walks, sorw, key, {})
+        _time_walk(batch, vols, hits, walks, sorw, key, {})
         best = min(({"tile_shape": s, "segment_steps": 256, "graph_rounds": GRAPH_ROUNDS}
                     for s in range(_hip.TILE_SHAPES)), key=trial)
         cands = [dict(best, segment_steps=c) for c in CAPS if c != 256]
