@@ -398,24 +398,53 @@ int bmx_seeds(bmx_ctx *ctx, const uint8_t *lab, const void *img, int img_dtype, 
 // ---------------------------------------------------------------------------
 namespace {
 
+// A block takes kSelItems seeds per thread (loads strided by the block for
+// coalescing), counts its kept seeds with a block-wide scan and reserves their
+// output range with one atomic: one atomic per 4096 seeds, where one per wave
+// made the shared counter the bottleneck.
+constexpr int kSelItems = 16;
+
 __global__ __launch_bounds__(256) void k_select_seeds(const bmx_seed *__restrict__ in, int64_t n,
                                                       const uint8_t *__restrict__ mask,
                                                       bmx_seed *__restrict__ out,
                                                       unsigned long long *count)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    bmx_seed s{};
-    bool keep = false;
-    if (i < n) {
-        s = in[i];
-        keep = mask[s.pos] != 0;
+    __shared__ uint32_t wsum[4];
+    __shared__ unsigned long long sbase;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t b0 = (int64_t)blockIdx.x * 256 * kSelItems;
+    bmx_seed s[kSelItems];
+    uint32_t keep = 0;
+#pragma unroll
+    for (int j = 0; j < kSelItems; ++j) {
+        const int64_t i = b0 + (int64_t)j * 256 + tid;
+        if (i < n) {
+            s[j] = in[i];
+            if (mask[s[j].pos] != 0) keep |= 1u << j;
+        }
     }
-    const uint64_t ballot = __ballot(keep);
-    const int lane = threadIdx.x & 63;
-    unsigned long long base = 0;
-    if (lane == 0 && ballot) base = atomicAdd(count, (unsigned long long)__popcll(ballot));
-    base = __shfl(base, 0, 64);
-    if (keep) out[base + __popcll(ballot & ((1ull << lane) - 1ull))] = s;
+    // exclusive prefix of the kept counts over the block
+    const uint32_t c = (uint32_t)__popc(keep);
+    uint32_t incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, (unsigned)o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        before += w < wave ? wsum[w] : 0u;
+        total += wsum[w];
+    }
+    if (tid == 0) sbase = total ? atomicAdd(count, (unsigned long long)total) : 0ull;
+    __syncthreads();
+    unsigned long long at = sbase + before + incl - c;
+#pragma unroll
+    for (int j = 0; j < kSelItems; ++j)
+        if ((keep >> j) & 1u) out[at++] = s[j];
 }
 
 // Adaptive walks of the large-volume path (pyopencl_large.py:570-590): the
@@ -542,7 +571,7 @@ extern "C" int bmx_select_seeds(bmx_ctx *ctx, const bmx_seed *seeds, int64_t nse
     BMX_CHECK(nseeds >= 0 && nseeds < ((int64_t)1 << 32) - 256, "bmx_select_seeds: nseeds out of range");
     BMX_HIP(hipMemsetAsync(ctx->d_counter + 2, 0, sizeof(uint64_t), ctx->stream));
     if (nseeds > 0) {
-        k_select_seeds<<<bmx_blocks(nseeds, 256), 256, 0, ctx->stream>>>(
+        k_select_seeds<<<bmx_blocks(nseeds, 256 * kSelItems), 256, 0, ctx->stream>>>(
             seeds, nseeds, mask, out, (unsigned long long *)(ctx->d_counter + 2));
         BMX_LAUNCH_CHECK();
     }
