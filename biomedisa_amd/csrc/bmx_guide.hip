@@ -12,6 +12,8 @@
 // of the contributions of the seeds beyond srad.
 #include "bmx_internal.h"
 
+#include <algorithm>
+
 namespace {
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -101,48 +103,84 @@ __global__ void k_boundary(const uint8_t *__restrict__ pred, int Z, int Y, int X
     out[v] = b ? 1 : 0;
 }
 
-// out = max over |k| <= r along one axis (0: z, 1: y, 2: x) of in.
-__global__ void k_dilate_axis(const uint8_t *__restrict__ in, int Z, int Y, int X, int axis, int r,
-                              uint8_t *__restrict__ out)
-{
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= (int64_t)Z * Y * X) return;
-    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
-    const int c = axis == 0 ? z : (axis == 1 ? y : x);
-    const int n = axis == 0 ? Z : (axis == 1 ? Y : X);
-    const int64_t st = axis == 0 ? (int64_t)X * Y : (axis == 1 ? (int64_t)X : 1);
-    const int lo = max(c - r, 0), hi = min(c + r, n - 1);
-    uint8_t m = 0;
-    for (int k = lo; k <= hi && !m; ++k) m |= in[v + (int64_t)(k - c) * st];
-    out[v] = m ? 1 : 0;
-}
-
-// The same dilation by sweeps: one thread per line along z (axis 0), y (1) or
-// x (2), a forward pass marking voxels within r after a set voxel and a
-// backward pass marking those within r before one.  Along z and y adjacent
-// threads own adjacent x, so every step is a coalesced row access; along x
-// each thread reads its own row, which pays for large r only.  out != in.
+// Dilation by r along one axis: out[j] = 1 where some in[k] != 0 with
+// |k - j| <= r on the same line.  One pass per line with a read-ahead of r:
+// after in[0..i] have been read, `last` is the last set index so far, and
+// out[i - r] = (last >= i - 2r), since every index up to (i - r) + r has been
+// read.  Each voxel is read once and written once.
+//
+// Along z (axis 0) and y (axis 1): one thread per line, adjacent threads on
+// adjacent x, so every step is a coalesced row access.  out != in.
 __global__ void k_dilate_lines(const uint8_t *__restrict__ in, int Z, int Y, int X, int axis, int r,
                                uint8_t *__restrict__ out)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nlines = axis == 0 ? (int64_t)Y * X : (axis == 1 ? (int64_t)Z * X : (int64_t)Z * Y);
+    const int64_t nlines = axis == 0 ? (int64_t)Y * X : (int64_t)Z * X;
     if (t >= nlines) return;
-    const int n = axis == 0 ? Z : (axis == 1 ? Y : X);
-    const int64_t st = axis == 0 ? (int64_t)X * Y : (axis == 1 ? (int64_t)X : 1);
-    // first voxel of the line: (0, y, x) for z lines, (z, 0, x) for y lines, (z, y, 0) for x lines
-    const int64_t base = axis == 0 ? t : (axis == 1 ? (t / X) * (int64_t)X * Y + t % X : t * X);
-    int last = -r - 1;
+    const int n = axis == 0 ? Z : Y;
+    const int64_t st = axis == 0 ? (int64_t)X * Y : (int64_t)X;
+    // first voxel of the line: (0, y, x) for z lines, (z, 0, x) for y lines
+    const int64_t base = axis == 0 ? t : (t / X) * (int64_t)X * Y + t % X;
+    int last = -2 * r - 1;
 #pragma unroll 8
-    for (int i = 0; i < n; ++i) {
-        if (in[base + i * st]) last = i;
-        out[base + i * st] = (i - last <= r) ? 1 : 0;
+    for (int i = 0; i < n + r; ++i) {
+        if (i < n && in[base + (int64_t)i * st]) last = i;
+        const int j = i - r;
+        if (j >= 0) out[base + (int64_t)j * st] = last >= j - r ? 1 : 0;
     }
-    int next = n + r;
+}
+
+// Along x (rows contiguous in memory): a block stages `rpb` consecutive rows
+// (at most kDilRows, rpb * X <= kDilLds bytes) in LDS with coalesced 4-byte
+// copies, one thread per row runs the one-pass dilation on its row in place
+// (out[j] is written after in[j] has been read, and nothing reads in[j]
+// again), and the block copies the rows back.  In place on `buf`.
+constexpr int kDilRows = 64, kDilThreads = 256, kDilLds = 64 * 516;
+__global__ __launch_bounds__(kDilThreads) void k_dilate_rows(uint8_t *__restrict__ buf, int64_t nrows, int X, int r,
+                                                             int rpb)
+{
+    __shared__ uint32_t s4[kDilLds / 4];
+    uint8_t *s = (uint8_t *)s4;
+    const int tid = (int)threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * rpb;
+    const int rows = (int)min((int64_t)rpb, nrows - r0);
+    const int bytes = rows * X;
+    uint8_t *g = buf + r0 * X;
+    // 4-byte copies where the block's range is 4-aligned (all blocks when
+    // rpb * X % 4 == 0, the buffer being 256-aligned), bytes otherwise
+    const bool al = ((uintptr_t)g & 3u) == 0;
+    const int words = al ? bytes / 4 : 0;
+    for (int k = tid; k < words; k += kDilThreads) s4[k] = ((const uint32_t *)g)[k];
+    for (int k = 4 * words + tid; k < bytes; k += kDilThreads) s[k] = g[k];
+    __syncthreads();
+    if (tid < rows) {
+        uint8_t *row = s + tid * X;
+        int last = -2 * r - 1;
 #pragma unroll 8
-    for (int i = n - 1; i >= 0; --i) {
-        if (in[base + i * st]) next = i;
-        if (next - i <= r) out[base + i * st] = 1;
+        for (int i = 0; i < X + r; ++i) {
+            if (i < X && row[i]) last = i;
+            const int j = i - r;
+            if (j >= 0) row[j] = last >= j - r ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < words; k += kDilThreads) ((uint32_t *)g)[k] = s4[k];
+    for (int k = 4 * words + tid; k < bytes; k += kDilThreads) g[k] = s[k];
+}
+
+// Rows longer than kDilLds bytes: one thread per row, the same one pass, into out.
+__global__ void k_dilate_rows_long(const uint8_t *__restrict__ in, int64_t nrows, int X, int r,
+                                   uint8_t *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const uint8_t *row = in + t * X;
+    uint8_t *orow = out + t * X;
+    int last = -2 * r - 1;
+    for (int i = 0; i < X + r; ++i) {
+        if (i < X && row[i]) last = i;
+        const int j = i - r;
+        if (j >= 0) orow[j] = last >= j - r ? 1 : 0;
     }
 }
 
@@ -287,15 +325,18 @@ __global__ void k_hits_index(const uint32_t *__restrict__ hits, int nlabels, int
 int dilate(bmx_ctx *ctx, uint8_t *buf, uint8_t *tmp, int Z, int Y, int X, int r)
 {
     if (r <= 0) return BMX_OK;
-    const int64_t V = (int64_t)Z * Y * X;
-    if (r >= 8)
-        k_dilate_lines<<<bmx_blocks((int64_t)Z * Y, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 2, r, tmp);
-    else
-        k_dilate_axis<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 2, r, tmp);
-    k_dilate_lines<<<bmx_blocks((int64_t)Z * X, 256), 256, 0, ctx->stream>>>(tmp, Z, Y, X, 1, r, buf);
-    k_dilate_lines<<<bmx_blocks((int64_t)Y * X, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 0, r, tmp);
+    const int64_t V = (int64_t)Z * Y * X, nrows = (int64_t)Z * Y;
+    // x in place (or via tmp for long rows), then y into tmp, z back into buf
+    if (X <= kDilLds) {
+        const int rpb = std::min(kDilRows, kDilLds / X);
+        k_dilate_rows<<<bmx_blocks(nrows, rpb), kDilThreads, 0, ctx->stream>>>(buf, nrows, X, r, rpb);
+    } else {
+        k_dilate_rows_long<<<bmx_blocks(nrows, 256), 256, 0, ctx->stream>>>(buf, nrows, X, r, tmp);
+        BMX_HIP(hipMemcpyAsync(buf, tmp, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    k_dilate_lines<<<bmx_blocks((int64_t)Z * X, 256), 256, 0, ctx->stream>>>(buf, Z, Y, X, 1, r, tmp);
+    k_dilate_lines<<<bmx_blocks((int64_t)Y * X, 256), 256, 0, ctx->stream>>>(tmp, Z, Y, X, 0, r, buf);
     BMX_LAUNCH_CHECK();
-    BMX_HIP(hipMemcpyAsync(buf, tmp, (size_t)V, hipMemcpyDeviceToDevice, ctx->stream));
     return BMX_OK;
 }
 

@@ -171,6 +171,23 @@ def test_ring_mask_bitexact(eng, r):
     np.testing.assert_array_equal(ring.download(dims, np.uint8), G.ring_mask(need, r))
 
 
+@pytest.mark.parametrize("dims,r", [((5, 7, 600), 9), ((3, 4, 1030), 40), ((2, 3, 40000), 5), ((40, 9, 21), 60),
+                                    ((70, 3, 517), 12)])
+def test_ring_mask_long_rows_bitexact(eng, dims, r):
+    """Rows longer than one staged block of rows (fewer rows per block, or one
+    thread per row beyond the LDS stage), a row count that is not a multiple
+    of the rows per block, blocks that start off a 4-byte boundary (63 rows of
+    517 bytes), and a radius beyond every axis."""
+    rng = np.random.default_rng(sum(dims) + r)
+    need = rng.random(dims) < 0.001
+    need[0, 0, 0] = need[-1, -1, -1] = True
+    need = need.astype(np.uint8)
+    V = int(np.prod(dims))
+    nb, ring, tmp = eng.upload(need), eng.alloc(V), eng.alloc(V)
+    _hip.call("bmx_ring_mask", eng.ctx, nb.ptr, *dims, r, ring.ptr, tmp.ptr)
+    np.testing.assert_array_equal(ring.download(dims, np.uint8), G.ring_mask(need, r))
+
+
 def test_mask_andnot(eng):
     rng = np.random.default_rng(21)
     n = 70001
