@@ -219,7 +219,15 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
     // +0: the stencil of +0 inputs is +0, as are copied boundary voxels.  They
     // are written only where the buffer may hold something else (rec.prev2:
     // this buffer's boxes as the pass before last left them).
+    // Planes [zc0, zc1) of the chunk are computed; the others are +0.
+    int zc0 = zb, zc1 = ze;
     if (rec.prev) {
+        __shared__ int szr[2];
+        if (tid == 0) {
+            szr[0] = INT_MAX;
+            szr[1] = INT_MIN;
+        }
+        __syncthreads();
         bool near = false;
         if (tid < 27) {
             const int nx = (int)blockIdx.x + tid % 3 - 1, ny = (int)blockIdx.y + (tid / 3) % 3 - 1,
@@ -229,17 +237,39 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
                 const int xe = min(x0 + TX, bx.x1) - 1, ye = min(y0 + TY, bx.y1) - 1;
                 near = q[1] >= 0 && q[0] <= ze - 1 + 2 && q[1] >= zb - 2 && q[2] <= ye + 2 && q[3] >= y0 - 2 &&
                        q[4] <= xe + 2 && q[5] >= x0 - 2;
+                if (near) {
+                    atomicMin(&szr[0], q[0]);
+                    atomicMax(&szr[1], q[1]);
+                }
             }
         }
-        if (!__syncthreads_or(near)) {
-            if (!rec.prev2 || rec.prev2[6 * blk + 1] >= 0) {
-                for (int z = zb; z < ze; ++z)
+        const bool any = __syncthreads_or(near);
+        // An output plane z reads input planes z-2..z+2 only; where the near
+        // boxes hold no non-zero input plane in that reach, its outputs are +0.
+        if (any) {
+            zc0 = max(zb, szr[0] - 2);
+            zc1 = min(ze, szr[1] + 3);
+        } else {
+            zc0 = zc1 = ze;
+        }
+        // +0 is written where the buffer may hold something else: all of the
+        // block on the pass after the first, else the planes of this buffer's
+        // box from the pass before last.
+        int w0 = zb, w1 = ze;
+        if (rec.prev2) {
+            const int *q2 = rec.prev2 + 6 * blk;
+            w0 = q2[1] >= 0 ? max(zb, q2[0]) : ze;
+            w1 = q2[1] >= 0 ? min(ze, q2[1] + 1) : zb;
+        }
+        for (int z = w0; z < w1; ++z) {
+            if (z >= zc0 && z < zc1) continue;
 #pragma unroll
-                    for (int k = 0; k < RY; ++k) {
-                        const int y = y0 + ty * RY + k, x = x0 + tx;
-                        if (x < bx.x1 && y < bx.y1) out[(int64_t)z * sz + (int64_t)y * sy + x] = 0.0f;
-                    }
+            for (int k = 0; k < RY; ++k) {
+                const int y = y0 + ty * RY + k, x = x0 + tx;
+                if (x < bx.x1 && y < bx.y1) out[(int64_t)z * sz + (int64_t)y * sy + x] = 0.0f;
             }
+        }
+        if (!any) {
             if (tid == 0) {
                 int *o = rec.out + 6 * blk;
                 o[0] = INT_MAX; o[1] = -1; o[2] = INT_MAX; o[3] = -1; o[4] = INT_MAX; o[5] = -1;
@@ -311,8 +341,8 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
         for (int k = 0; k < kPairLoads; ++k)
             regs[k] = (zv && ((okmask >> k) & 1u)) ? ldf(plane, (int64_t)off[k]) : 0.0f;
     };
-    fetch(zb - 2);
-    for (int q = zb - 2; q <= ze + 1; ++q) {
+    fetch(zc0 - 2);
+    for (int q = zc0 - 2; q <= zc1 + 1; ++q) {
         float *l0 = &L0[slot3(q)][0][0];
 #pragma unroll
         for (int k = 0; k < kPairLoads; ++k) {
@@ -320,10 +350,10 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
             if (i < PW * PH) l0[i] = regs[k];
         }
         __syncthreads();
-        if (q + 1 <= ze + 1) fetch(q + 1);
+        if (q + 1 <= zc1 + 1) fetch(q + 1);
         // round t+1 at plane q-1 over frame rows 1..PH-2, columns 1..PW-2
         const int r1 = q - 1;
-        if (r1 >= zb - 1) {
+        if (r1 >= zc0 - 1) {
             const float (*m)[PW] = L0[slot3(r1 - 1)], (*c)[PW] = L0[slot3(r1)], (*p)[PW] = L0[slot3(r1 + 1)];
             float (*o)[PW] = L1[slot3(r1)];
             const bool zin = z_ok(r1), zint = r1 > 0 && r1 < Z - 1;
@@ -338,7 +368,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
         __syncthreads();
         // round t+2 at plane q-2 over the 64x16 core, written out inside the box
         const int r2 = q - 2;
-        if (r2 >= zb && r2 < ze) {
+        if (r2 >= zc0 && r2 < zc1) {
             const float (*m)[PW] = L1[slot3(r2 - 1)], (*c)[PW] = L1[slot3(r2)], (*p)[PW] = L1[slot3(r2 + 1)];
             const bool zint = r2 > 0 && r2 < Z - 1;
             uint32_t nzrows = 0u;
