@@ -40,7 +40,10 @@ __device__ __forceinline__ uint32_t step_delta(int d, uint32_t sz, uint32_t sy)
 
 // Direction from the six weights (first d with rand < prefix_d, rand == 0 -> 0).
 // The prefix sums are non-decreasing, so the first such d is the number of
-// prefixes <= rand; u < 1 keeps rand <= W5.
+// prefixes <= rand; u < 1 keeps rand <= W5.  The count is taken in two
+// halves: if rand >= W2, the first three prefixes count and W3, W4 decide
+// the rest, else only W0 and W1 can count.  A NaN prefix makes every later
+// one and rand NaN, so both forms then count the same leading prefixes.
 __device__ __forceinline__ int choose(float w0, float w1, float w2, float w3, float w4, float w5,
                                       float u)
 {
@@ -51,7 +54,9 @@ __device__ __forceinline__ int choose(float w0, float w1, float w2, float w3, fl
     const float W4 = w4 + W3;
     const float W5 = w5 + W4;
     const float rand = W5 * u;
-    const int cnt = (rand >= W0) + (rand >= W1) + (rand >= W2) + (rand >= W3) + (rand >= W4);
+    const bool hi = rand >= W2;
+    const float a = hi ? W3 : W0, b = hi ? W4 : W1;
+    const int cnt = (hi ? 3 : 0) + (rand >= a) + (rand >= b);
     return (rand == 0.0f) ? 0 : cnt;
 }
 
