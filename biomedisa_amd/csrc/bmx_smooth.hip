@@ -17,6 +17,7 @@
 #include "bmx_internal.h"
 
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -341,9 +342,13 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
         for (int k = 0; k < kPairLoads; ++k)
             regs[k] = (zv && ((okmask >> k) & 1u)) ? ldf(plane, (int64_t)off[k]) : 0.0f;
     };
-    fetch(zc0 - 2);
-    for (int q = zc0 - 2; q <= zc1 + 1; ++q) {
-        float *l0 = &L0[slot3(q)][0][0];
+    // One input plane q: stage it, round t+1 at plane q-1, round t+2 at plane
+    // q-2.  The three LDS slots rotate with q (slot3(q) = q mod 3); S0 =
+    // slot3(q) is a template constant, so every LDS address is a per-thread
+    // base plus an immediate offset (the loop below dispatches on it).
+    auto plane = [&](int q, auto phase) {
+        constexpr int S0 = decltype(phase)::value, S1 = (S0 + 2) % 3, S2 = (S0 + 1) % 3;   // q, q-1, q-2
+        float *l0 = &L0[S0][0][0];
 #pragma unroll
         for (int k = 0; k < kPairLoads; ++k) {
             const int i = tid + k * kSmoothThreads;
@@ -354,8 +359,8 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
         // round t+1 at plane q-1 over frame rows 1..PH-2, columns 1..PW-2
         const int r1 = q - 1;
         if (r1 >= zc0 - 1) {
-            const float (*m)[PW] = L0[slot3(r1 - 1)], (*c)[PW] = L0[slot3(r1)], (*p)[PW] = L0[slot3(r1 + 1)];
-            float (*o)[PW] = L1[slot3(r1)];
+            const float (*m)[PW] = L0[S2], (*c)[PW] = L0[S1], (*p)[PW] = L0[S0];
+            float (*o)[PW] = L1[S1];
             const bool zin = z_ok(r1), zint = r1 > 0 && r1 < Z - 1;
 #pragma unroll
             for (int k = 0; k < RY; ++k) {
@@ -369,7 +374,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
         // round t+2 at plane q-2 over the 64x16 core, written out inside the box
         const int r2 = q - 2;
         if (r2 >= zc0 && r2 < zc1) {
-            const float (*m)[PW] = L1[slot3(r2 - 1)], (*c)[PW] = L1[slot3(r2)], (*p)[PW] = L1[slot3(r2 + 1)];
+            const float (*m)[PW] = L1[S0], (*c)[PW] = L1[S2], (*p)[PW] = L1[S1];   // q-3, q-2, q-1
             const bool zint = r2 > 0 && r2 < Z - 1;
             uint32_t nzrows = 0u;
 #pragma unroll
@@ -388,6 +393,17 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
                 rows |= nzrows;
             }
         }
+    };
+    fetch(zc0 - 2);
+    int ph = slot3(zc0 - 2);
+    for (int q = zc0 - 2; q <= zc1 + 1; ++q) {
+        if (ph == 0)
+            plane(q, std::integral_constant<int, 0>{});
+        else if (ph == 1)
+            plane(q, std::integral_constant<int, 1>{});
+        else
+            plane(q, std::integral_constant<int, 2>{});
+        ph = ph == 2 ? 0 : ph + 1;
     }
     if (rec.out) {
         const int yr = y0 + ty * RY;
