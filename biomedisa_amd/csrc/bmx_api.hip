@@ -58,6 +58,24 @@ int bmx_set_device(bmx_ctx *ctx)
     return BMX_OK;
 }
 
+int bmx_aux_streams(bmx_ctx *ctx)
+{
+    if (ctx->aux[0]) return BMX_OK;
+    hipStream_t st[2] = {};
+    hipEvent_t ev[3] = {};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        for (int k = 0; k < 2; ++k) if (st[k]) (void)hipStreamDestroy(st[k]);
+        for (int k = 0; k < 3; ++k) if (ev[k]) (void)hipEventDestroy(ev[k]);
+        return bmx_set_error(BMX_E_DEVICE, "bmx: stream creation failed: %s", hipGetErrorString(e));
+    }
+    for (int k = 0; k < 2; ++k) ctx->aux[k] = st[k];
+    for (int k = 0; k < 3; ++k) ctx->aux_ev[k] = ev[k];
+    return BMX_OK;
+}
+
 extern "C" {
 
 const char *bmx_last_error(void) { return g_err; }
@@ -95,6 +113,13 @@ int bmx_close(bmx_ctx *ctx)
     if (!ctx) return BMX_OK;
     BMX_HIP(hipSetDevice(ctx->device));
     BMX_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->aux[0]) {
+        for (int k = 0; k < 2; ++k) {
+            BMX_HIP(hipStreamSynchronize(ctx->aux[k]));
+            BMX_HIP(hipStreamDestroy(ctx->aux[k]));
+        }
+        for (int k = 0; k < 3; ++k) BMX_HIP(hipEventDestroy(ctx->aux_ev[k]));
+    }
     if (ctx->d_block_counts) BMX_HIP(hipFree(ctx->d_block_counts));
     if (ctx->d_rec) BMX_HIP(hipFree(ctx->d_rec));
     if (ctx->d_state) BMX_HIP(hipFree(ctx->d_state));

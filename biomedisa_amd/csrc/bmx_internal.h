@@ -42,7 +42,12 @@ struct bmx_ctx {
     hipEvent_t *seg_ev;      // sorted walker: event pairs around each walk launch
     float last_walk_kernel_ms;
     int last_walk_launches;
+    hipStream_t aux[2];      // smoothing: two more streams (default priority), made on first use
+    hipEvent_t aux_ev[3];    // smoothing: ordering events between the streams
 };
+
+// Creates ctx->aux and ctx->aux_ev on first use.
+int bmx_aux_streams(bmx_ctx *ctx);
 
 // Layout of bmx_ctx::d_small (bmx_ingest.hip): counts and boxes of the label
 // classes from offset 0 (BMX_LABEL_CLASSES x 8 + x 24 bytes), two 256-byte

@@ -16,6 +16,7 @@
 // k_smooth_step pass.
 #include "bmx_internal.h"
 
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 #include <vector>
@@ -428,21 +429,48 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
     }
 }
 
+// Value of label `label` at voxel v for the argmax over labels: phi inside
+// the label's box, 0 outside; the large-volume path sets the first label's
+// negative values to 0 before the running maximum (pyopencl_large.py:464-469).
+// A NaN becomes +inf for label 0 and -inf for the others (see below).
+__device__ __forceinline__ float argmax_value(const float *__restrict__ phi, int64_t v, bool inb, int label,
+                                              int clamp_first)
+{
+    float p = inb ? phi[v] : 0.0f;
+    if (clamp_first && label == 0 && p < 0.0f) p = 0.0f;
+    if (p != p) p = label == 0 ? INFINITY : -INFINITY;
+    return p;
+}
+
+// The winner of a voxel is the label with the largest value, the lowest label
+// on ties (the np.argmax of rw_small.py:137, and the "replace when strictly
+// greater" pass over the labels in order of pyopencl_large.py:233-241), with a
+// NaN of label 0 beating every value and a NaN of any other label losing to
+// every value, as in that in-order pass.  This compares (value, -label) pairs,
+// so labels may be applied in any order and partial results merged.
+__device__ __forceinline__ bool argmax_beats(float p, int l, float q, int m)
+{
+    return p > q || (p == q && l < m);
+}
+
+__device__ __forceinline__ bool in_box(int64_t v, int Y, int X, const Box &bx)
+{
+    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
+    return z >= bx.z0 && z < bx.z1 && y >= bx.y0 && y < bx.y1 && x >= bx.x0 && x < bx.x1;
+}
+
+// One label's share of a running argmax (best, bidx); `first` marks the first
+// label applied to it.
 __global__ __launch_bounds__(256) void k_argmax_update(const float *__restrict__ phi,
                                                        float *__restrict__ best,
                                                        uint8_t *__restrict__ bidx, int label, int Z, int Y,
-                                                       int X, Box bx, int clamp_first)
+                                                       int X, Box bx, int clamp_first, int first)
 {
     const int64_t V = (int64_t)Z * Y * X;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
-    const bool inb = z >= bx.z0 && z < bx.z1 && y >= bx.y0 && y < bx.y1 && x >= bx.x0 && x < bx.x1;
-    float p = inb ? phi[v] : 0.0f;
-    // large-volume path: the first label's smoothed map has its negative values
-    // set to 0 before the running maximum (pyopencl_large.py:464-469)
-    if (clamp_first && label == 0 && p < 0.0f) p = 0.0f;
-    if (label == 0 || p > best[v]) {
+    const float p = argmax_value(phi, v, in_box(v, Y, X, bx), label, clamp_first);
+    if (first || argmax_beats(p, label, best[v], bidx[v])) {
         best[v] = p;
         bidx[v] = (uint8_t)label;
     }
@@ -452,19 +480,50 @@ struct LabelValues {
     uint8_t v[256];
 };
 
-// Bounding box of the non-zero hits of one label: box[0..5] = min z, max z,
-// min y, max y, min x, max x (min fields start at INT_MAX, max at -1).
+// Bounding box of the non-zero hits of each label (label = blockIdx.y, hits
+// of label l at hits + l * V): box[6l..6l+5] = min z, max z, min y, max y,
+// min x, max x (min fields start at INT_MAX, max at -1).  16-byte loads when
+// X is a multiple of 4 and `hits` is 16-byte aligned, else one wave per x row.
 __global__ __launch_bounds__(256) void k_hits_box(const uint32_t *__restrict__ hits, int Z, int Y, int X,
                                                   int *__restrict__ box)
 {
     const int64_t V = (int64_t)Z * Y * X;
+    const uint32_t *h = hits + (int64_t)blockIdx.y * V;
+    const int lane = threadIdx.x & 63;
     int b[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
-        if (hits[v]) {
-            const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((int64_t)X * Y));
-            b[0] = min(b[0], z); b[1] = max(b[1], z);
-            b[2] = min(b[2], y); b[3] = max(b[3], y);
-            b[4] = min(b[4], x); b[5] = max(b[5], x);
+    auto note = [&](int64_t e, int lo, int hi) {   // non-zeros at x offsets lo..hi from element e
+        const int64_t row = e / X;
+        const int x = (int)(e - row * X), z = (int)(row / Y), y = (int)(row - (int64_t)z * Y);
+        b[0] = min(b[0], z); b[1] = max(b[1], z);
+        b[2] = min(b[2], y); b[3] = max(b[3], y);
+        b[4] = min(b[4], x + lo); b[5] = max(b[5], x + hi);
+    };
+    if ((X & 3) == 0 && (reinterpret_cast<uintptr_t>(hits) & 15) == 0) {
+        // 16-byte groups never cross a row; four loads in flight per thread
+        const uint4 *h4 = reinterpret_cast<const uint4 *>(h);
+        const int64_t n4 = V / 4, stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 4 * stride) {
+            uint4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = i + u * stride < n4 ? h4[i + u * stride] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (q[u].x | q[u].y | q[u].z | q[u].w)
+                    note(4 * (i + u * stride), q[u].x ? 0 : q[u].y ? 1 : q[u].z ? 2 : 3,
+                         q[u].w ? 3 : q[u].z ? 2 : q[u].y ? 1 : 0);
+        }
+    } else {
+        // one wave per x row
+        const int64_t nrows = (int64_t)Z * Y;
+        for (int64_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < nrows; r += gridDim.x * 4) {
+            const uint32_t *row = h + r * X;
+            int lo = INT_MAX, hi = -1;
+            for (int x = lane; x < X; x += 64)
+                if (row[x]) {
+                    lo = min(lo, x);
+                    hi = x;
+                }
+            if (hi >= 0) note(r * X, lo, hi);
         }
     }
 #pragma unroll
@@ -475,12 +534,35 @@ __global__ __launch_bounds__(256) void k_hits_box(const uint32_t *__restrict__ h
             b[k + 1] = max(b[k + 1], __shfl_xor(b[k + 1], off, 64));
         }
     }
-    if ((threadIdx.x & 63) == 0 && b[1] >= 0) {
+    if (lane == 0 && b[1] >= 0) {
+        int *bl = box + 6 * blockIdx.y;
         for (int k = 0; k < 6; k += 2) {
-            atomicMin(&box[k], b[k]);
-            atomicMax(&box[k + 1], b[k + 1]);
+            atomicMin(&bl[k], b[k]);
+            atomicMax(&bl[k + 1], b[k + 1]);
         }
     }
+}
+
+// The last label and up to two partial argmaxes (null when unused) merged,
+// and the winner's label value written to out.
+__global__ __launch_bounds__(256) void k_argmax_final(const float *__restrict__ phi, int label, Box bx,
+                                                      int clamp_first, const float *__restrict__ best1,
+                                                      const uint8_t *__restrict__ bidx1,
+                                                      const float *__restrict__ best2,
+                                                      const uint8_t *__restrict__ bidx2, LabelValues lv,
+                                                      uint8_t *__restrict__ out, int Z, int Y, int X)
+{
+    const int64_t V = (int64_t)Z * Y * X;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    float p = argmax_value(phi, v, in_box(v, Y, X, bx), label, clamp_first);
+    int l = label;
+    if (best1 && argmax_beats(best1[v], bidx1[v], p, l)) {
+        p = best1[v];
+        l = bidx1[v];
+    }
+    if (best2 && argmax_beats(best2[v], bidx2[v], p, l)) l = bidx2[v];
+    out[v] = lv.v[l];
 }
 
 __global__ void k_box_init(int *box, int n)
@@ -498,24 +580,27 @@ __global__ __launch_bounds__(256) void k_map_values(const uint8_t *__restrict__ 
 }
 
 template <typename Tin>
-int launch_step(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, const Box &bx)
+int launch_step(hipStream_t st, const Tin *in, float *out, int Z, int Y, int X, const Box &bx)
 {
     dim3 grid((bx.x1 - bx.x0 + TX - 1) / TX, (bx.y1 - bx.y0 + TY - 1) / TY, (bx.z1 - bx.z0 + kZChunk - 1) / kZChunk);
-    k_smooth_step<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X, bx);
+    k_smooth_step<Tin><<<grid, kSmoothThreads, 0, st>>>(in, out, Z, Y, X, bx);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
 
 template <typename Tin>
-int launch_pair(bmx_ctx *ctx, const Tin *in, float *out, int Z, int Y, int X, const Box &bx,
+int launch_pair(hipStream_t st, const Tin *in, float *out, int Z, int Y, int X, const Box &bx,
                 PassRecords rec = PassRecords{nullptr, nullptr, nullptr})
 {
     dim3 grid((bx.x1 - bx.x0 + TX - 1) / TX, (bx.y1 - bx.y0 + TY - 1) / TY,
               (bx.z1 - bx.z0 + kPairZChunk - 1) / kPairZChunk);
-    k_smooth_pair<Tin><<<grid, kSmoothThreads, 0, ctx->stream>>>(in, out, Z, Y, X, bx, rec);
+    k_smooth_pair<Tin><<<grid, kSmoothThreads, 0, st>>>(in, out, Z, Y, X, bx, rec);
     BMX_LAUNCH_CHECK();
     return BMX_OK;
 }
+
+// Buffer sets of smooth_argmax: labels smoothed at the same time.
+constexpr int kSmoothSets = 3;
 
 // Ints of the pass records of one pair pass over the whole volume (an upper
 // bound for any box): 6 per block of the launch grid.
@@ -529,7 +614,7 @@ static int64_t pass_record_ints(int64_t Z, int64_t Y, int64_t X)
 // arrays of pass_record_ints each), pair passes after the first skip the
 // blocks that only see +0 (PassRecords).
 template <typename Tin>
-int smooth_rounds(bmx_ctx *ctx, const Tin *in, float *a, float *b, int Z, int Y, int X, int iters,
+int smooth_rounds(hipStream_t st, const Tin *in, float *a, float *b, int Z, int Y, int X, int iters,
                   const Box &bx, float **result, int *recs = nullptr, int64_t rec_ints = 0)
 {
     auto rec_of = [&](int pass) -> PassRecords {
@@ -538,14 +623,14 @@ int smooth_rounds(bmx_ctx *ctx, const Tin *in, float *a, float *b, int Z, int Y,
         return PassRecords{pass >= 1 ? r[(pass - 1) % 3] : nullptr, pass >= 2 ? r[(pass - 2) % 3] : nullptr,
                            r[pass % 3]};
     };
-    int rc = iters >= 2 ? launch_pair<Tin>(ctx, in, a, Z, Y, X, bx, rec_of(0))
-                        : launch_step<Tin>(ctx, in, a, Z, Y, X, bx);
+    int rc = iters >= 2 ? launch_pair<Tin>(st, in, a, Z, Y, X, bx, rec_of(0))
+                        : launch_step<Tin>(st, in, a, Z, Y, X, bx);
     if (rc) return rc;
     float *cur = a, *nxt = b;
     int pass = 1;
     for (int it = iters >= 2 ? 2 : 1; it < iters; it += 2, ++pass) {
-        rc = iters - it >= 2 ? launch_pair<float>(ctx, cur, nxt, Z, Y, X, bx, rec_of(pass))
-                             : launch_step<float>(ctx, cur, nxt, Z, Y, X, bx);
+        rc = iters - it >= 2 ? launch_pair<float>(st, cur, nxt, Z, Y, X, bx, rec_of(pass))
+                             : launch_step<float>(st, cur, nxt, Z, Y, X, bx);
         if (rc) return rc;
         float *t = cur; cur = nxt; nxt = t;
     }
@@ -567,8 +652,8 @@ int bmx_smooth(bmx_ctx *ctx, float *phi, float *tmp, int64_t Z, int64_t Y, int64
     float *cur = phi, *nxt = tmp;
     const Box whole{0, (int)Z, 0, (int)Y, 0, (int)X};
     for (int it = 0; it < iters; it += 2) {
-        rc = iters - it >= 2 ? launch_pair<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X, whole)
-                             : launch_step<float>(ctx, cur, nxt, (int)Z, (int)Y, (int)X, whole);
+        rc = iters - it >= 2 ? launch_pair<float>(ctx->stream, cur, nxt, (int)Z, (int)Y, (int)X, whole)
+                             : launch_step<float>(ctx->stream, cur, nxt, (int)Z, (int)Y, (int)X, whole);
         if (rc) return rc;
         float *t = cur; cur = nxt; nxt = t;
     }
@@ -594,69 +679,165 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
     BMX_CHECK(Z >= 1 && Y >= 1 && X >= 1 && Z * Y * X < (int64_t)1 << 32, "bmx_smooth_argmax: bad dims");
     BMX_CHECK(iters >= 1, "bmx_smooth_argmax: iters must be >= 1");
     const int64_t V = Z * Y * X;
-    float *a = nullptr, *b = nullptr, *best = nullptr;
-    uint8_t *bidx = nullptr;
-    int *dbox = nullptr;
-    hipError_t e = hipMalloc((void **)&a, (size_t)V * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void **)&b, (size_t)V * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void **)&best, (size_t)V * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void **)&bidx, (size_t)V);
-    if (e == hipSuccess) e = hipMalloc((void **)&dbox, (size_t)nlabels * 6 * sizeof(int));
     const int64_t rec_ints = pass_record_ints(Z, Y, X);
-    int *recs = nullptr;
-    if (e == hipSuccess) e = hipMalloc((void **)&recs, (size_t)(3 * rec_ints) * sizeof(int));
+    // Buffer sets of the labels being smoothed at the same time: two phi
+    // volumes and the pass records, and for sets 1.. a partial argmax.
+    float *a[kSmoothSets] = {}, *b[kSmoothSets] = {}, *pbest[kSmoothSets] = {};
+    uint8_t *pidx[kSmoothSets] = {};
+    int *recs[kSmoothSets] = {};
+    auto alloc_set = [&](int k) {
+        hipError_t e = hipMalloc((void **)&a[k], (size_t)V * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void **)&b[k], (size_t)V * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void **)&recs[k], (size_t)(3 * rec_ints) * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void **)&pbest[k], (size_t)V * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void **)&pidx[k], (size_t)V);
+        return e;
+    };
+    auto free_set = [&](int k) {
+        if (a[k]) (void)hipFree(a[k]);
+        if (b[k]) (void)hipFree(b[k]);
+        if (recs[k]) (void)hipFree(recs[k]);
+        if (pbest[k]) (void)hipFree(pbest[k]);
+        if (pidx[k]) (void)hipFree(pidx[k]);
+        a[k] = b[k] = pbest[k] = nullptr;
+        recs[k] = nullptr;
+        pidx[k] = nullptr;
+    };
+    int *dbox = nullptr;
+    hipError_t e = hipMalloc((void **)&dbox, (size_t)nlabels * 6 * sizeof(int));
+    if (e == hipSuccess) e = alloc_set(0);
+    int nset = 1;
+    std::vector<hipStream_t> used;
     if (e == hipSuccess) {
         // Each label is smoothed only inside the box of its non-zero hits grown by
         // `iters`: where all 19 stencil inputs are 0 the update gives exactly 0,
         // so phi's support grows by at most one voxel per round and stays inside.
         std::vector<int> box((size_t)nlabels * 6);
         k_box_init<<<bmx_blocks(nlabels * 6, 256), 256, 0, ctx->stream>>>(dbox, nlabels * 6);
-        for (int l = 0; l < nlabels; ++l)
-            k_hits_box<<<2048, 256, 0, ctx->stream>>>(hits + (int64_t)l * V, (int)Z, (int)Y, (int)X,
-                                                     dbox + 6 * l);
+        k_hits_box<<<dim3((unsigned)std::min<int64_t>((Z * Y + 15) / 16, 1024), (unsigned)nlabels), 256, 0,
+                     ctx->stream>>>(hits, (int)Z, (int)Y, (int)X, dbox);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(box.data(), dbox, box.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) !=
                 hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: hit boxes failed");
-        for (int l = 0; l < nlabels && rc == BMX_OK; ++l) {
+        std::vector<Box> bxs((size_t)nlabels, Box{0, 0, 0, 0, 0, 0});
+        std::vector<int64_t> cost((size_t)nlabels, 0);
+        int big = 0;
+        for (int l = 0; l < nlabels; ++l) {
             const int *h = &box[(size_t)6 * l];
-            Box bx{0, 0, 0, 0, 0, 0};
-            float *res = a;
-            if (h[1] >= 0) {
-                bx = Box{max(h[0] - iters, 0), min(h[1] + 1 + iters, (int)Z), max(h[2] - iters, 0),
-                         min(h[3] + 1 + iters, (int)Y), max(h[4] - iters, 0), min(h[5] + 1 + iters, (int)X)};
-                rc = smooth_rounds<uint32_t>(ctx, hits + (int64_t)l * V, a, b, (int)Z, (int)Y, (int)X, iters,
-                                             bx, &res, recs, rec_ints);
+            if (h[1] < 0) continue;
+            Box &bx = bxs[(size_t)l];
+            bx = Box{max(h[0] - iters, 0), min(h[1] + 1 + iters, (int)Z), max(h[2] - iters, 0),
+                     min(h[3] + 1 + iters, (int)Y), max(h[4] - iters, 0), min(h[5] + 1 + iters, (int)X)};
+            cost[(size_t)l] = (int64_t)(bx.z1 - bx.z0) * (bx.y1 - bx.y0) * (bx.x1 - bx.x0);
+            if (cost[(size_t)l] > cost[(size_t)big]) big = l;
+        }
+        // Labels are smoothed side by side (a small label's passes leave much
+        // of the GPU idle): the label with the largest box on the context
+        // stream in set 0, the others split over sets 1.. on streams of their
+        // own, largest boxes first, each set keeping a partial argmax of its
+        // labels.  A last kernel merges the partial argmaxes with the largest
+        // box's label.  Without the memory or the streams for a second set,
+        // the labels run one after another into set 0's argmax.
+        if (rc == BMX_OK && nlabels >= 2 && bmx_aux_streams(ctx) == BMX_OK) {
+            while (nset < std::min(kSmoothSets, nlabels)) {
+                if (alloc_set(nset) != hipSuccess) {
+                    (void)hipGetLastError();
+                    free_set(nset);
+                    break;
+                }
+                ++nset;
+            }
+        }
+        (void)hipGetLastError();
+        auto smooth_label = [&](int l, int k, hipStream_t st, float **res) -> int {
+            *res = a[k];
+            if (cost[(size_t)l] == 0) return BMX_OK;
+            return smooth_rounds<uint32_t>(st, hits + (int64_t)l * V, a[k], b[k], (int)Z, (int)Y, (int)X, iters,
+                                           bxs[(size_t)l], res, recs[k], rec_ints);
+        };
+        auto update = [&](int l, const float *res, float *best, uint8_t *bidx, int first, hipStream_t st) -> int {
+            k_argmax_update<<<bmx_blocks(V, 256), 256, 0, st>>>(res, best, bidx, l, (int)Z, (int)Y, (int)X,
+                                                                bxs[(size_t)l], clamp_first, first);
+            return hipGetLastError() == hipSuccess ? BMX_OK
+                                                   : bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
+        };
+        LabelValues lv;
+        for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
+        if (nset == 1) {
+            for (int l = 0; l < nlabels && rc == BMX_OK; ++l) {
+                float *res = nullptr;
+                rc = smooth_label(l, 0, ctx->stream, &res);
+                if (rc == BMX_OK) rc = update(l, res, pbest[0], pidx[0], l == 0, ctx->stream);
             }
             if (rc == BMX_OK) {
-                k_argmax_update<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(res, best, bidx, l, (int)Z, (int)Y,
-                                                                            (int)X, bx, clamp_first);
+                k_map_values<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(pidx[0], lv, out, V);
+                if (hipGetLastError() != hipSuccess)
+                    rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
+            }
+        } else if (rc == BMX_OK) {
+            // rolling labels, largest box first, each to the set with the least work so far
+            std::vector<int> rest;
+            for (int l = 0; l < nlabels; ++l)
+                if (l != big) rest.push_back(l);
+            std::stable_sort(rest.begin(), rest.end(),
+                             [&](int p, int q) { return cost[(size_t)p] > cost[(size_t)q]; });
+            std::vector<int64_t> load((size_t)nset, 0);
+            std::vector<int> nused((size_t)nset, 0);
+            hipEvent_t start = ctx->aux_ev[0];
+            if (hipEventRecord(start, ctx->stream) != hipSuccess)
+                rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: event failed");
+            for (int k = 1; k < nset && rc == BMX_OK; ++k) {
+                used.push_back(ctx->aux[k - 1]);
+                if (hipStreamWaitEvent(ctx->aux[k - 1], start, 0) != hipSuccess)
+                    rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: event failed");
+            }
+            float *res_big = nullptr;
+            if (rc == BMX_OK) rc = smooth_label(big, 0, ctx->stream, &res_big);
+            for (size_t i = 0; i < rest.size() && rc == BMX_OK; ++i) {
+                int k = 1;
+                for (int j = 2; j < nset; ++j)
+                    if (load[(size_t)j] < load[(size_t)k]) k = j;
+                const int l = rest[i];
+                load[(size_t)k] += cost[(size_t)l] + V / 64;   // V / 64: the update's full-volume pass
+                float *res = nullptr;
+                rc = smooth_label(l, k, ctx->aux[k - 1], &res);
+                if (rc == BMX_OK)
+                    rc = update(l, res, pbest[k], pidx[k], nused[(size_t)k] == 0, ctx->aux[k - 1]);
+                ++nused[(size_t)k];
+            }
+            for (int k = 1; k < nset && rc == BMX_OK; ++k) {
+                if (hipEventRecord(ctx->aux_ev[k], ctx->aux[k - 1]) != hipSuccess ||
+                    hipStreamWaitEvent(ctx->stream, ctx->aux_ev[k], 0) != hipSuccess)
+                    rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: event failed");
+            }
+            if (rc == BMX_OK) {
+                auto part = [&](int k, bool idx) -> const void * {
+                    if (k >= nset || nused[(size_t)k] == 0) return nullptr;
+                    return idx ? (const void *)pidx[k] : (const void *)pbest[k];
+                };
+                k_argmax_final<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(
+                    res_big, big, bxs[(size_t)big], clamp_first, (const float *)part(1, false),
+                    (const uint8_t *)part(1, true), (const float *)part(2, false), (const uint8_t *)part(2, true),
+                    lv, out, (int)Z, (int)Y, (int)X);
                 if (hipGetLastError() != hipSuccess)
                     rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
             }
         }
-        if (rc == BMX_OK) {
-            LabelValues lv;
-            for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
-            k_map_values<<<bmx_blocks(V, 256), 256, 0, ctx->stream>>>(bidx, lv, out, V);
-            if (hipGetLastError() != hipSuccess)
-                rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: launch failed");
+        used.push_back(ctx->stream);
+        for (hipStream_t st : used) {
+            hipError_t s = hipStreamSynchronize(st);
+            if (s != hipSuccess && rc == BMX_OK)
+                rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: %s", hipGetErrorString(s));
         }
-        hipError_t s = hipStreamSynchronize(ctx->stream);
-        if (s != hipSuccess && rc == BMX_OK)
-            rc = bmx_set_error(BMX_E_DEVICE, "bmx_smooth_argmax: %s", hipGetErrorString(s));
     } else {
         (void)hipGetLastError();
         rc = bmx_set_error(e == hipErrorOutOfMemory ? BMX_E_OOM : BMX_E_DEVICE,
                            "bmx_smooth_argmax: allocation failed: %s", hipGetErrorString(e));
     }
-    if (a) (void)hipFree(a);
-    if (b) (void)hipFree(b);
-    if (best) (void)hipFree(best);
-    if (bidx) (void)hipFree(bidx);
+    for (int k = 0; k < kSmoothSets; ++k) free_set(k);
     if (dbox) (void)hipFree(dbox);
-    if (recs) (void)hipFree(recs);
     return rc;
 }
 

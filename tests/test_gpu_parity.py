@@ -277,6 +277,33 @@ def test_smooth_argmax_skipped_blocks_bitexact(eng, iters):
     np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
 
 
+@pytest.mark.parametrize("clamp", [False, True])
+def test_smooth_argmax_label_sets_bitexact(eng, clamp):
+    """Seven labels smoothed side by side: the largest box belongs to label 3
+    (its update runs last), labels 1 and 5 hold the same hits (the tie goes to
+    label 1), label 4 has no hits, and the other labels take turns on the
+    reused buffer sets.  With clamp, label 0's negative values become 0 before
+    the maximum, as the large-volume driver does."""
+    rng = np.random.default_rng(71)
+    shape = (30, 40, 90)
+    nl = 7
+    h = np.zeros((nl,) + shape, np.uint32)
+    h[3] = rng.integers(0, 9, shape)
+    for l, (z, y, x) in [(0, (15, 20, 45)), (1, (3, 5, 10)), (2, (27, 35, 80)), (6, (10, 30, 60))]:
+        sl = tuple(slice(max(0, c - 3), c + 4) for c in (z, y, x))
+        h[l][sl] = rng.integers(5, 300, h[l][sl].shape)
+    h[5] = h[1]
+    values = np.array([1, 2, 3, 4, 5, 6, 7], np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(h[0].size)
+    for iters in (5, 4):
+        eng.smooth_argmax(hb, nl, shape, iters, values, out, clamp_first=clamp)
+        phi = np.stack([orc.smooth(x.astype(np.float32), iters) for x in h])
+        if clamp:
+            phi[0][phi[0] < 0] = 0
+        np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
+
+
 @pytest.mark.parametrize("iters", [2, 3, 4])
 def test_smooth_pairs_bitexact_wide(eng, iters):
     """A volume wider than one tile in x and y and taller than one z chunk."""
