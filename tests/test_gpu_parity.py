@@ -304,6 +304,32 @@ def test_smooth_argmax_label_sets_bitexact(eng, clamp):
         np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
 
 
+@pytest.mark.parametrize("iters", [1, 2, 7])
+def test_smooth_argmax_label_batches_bitexact(eng, iters):
+    """Twelve labels: the eleven beside the largest box (label 6) share the two
+    side streams, each stream reusing its buffers label after label, with an
+    empty label (2) and boxes of different sizes; even, odd and single-round
+    counts."""
+    rng = np.random.default_rng(90 + iters)
+    shape = (26, 37, 150)
+    nl = 12
+    h = np.zeros((nl,) + shape, np.uint32)
+    h[6] = rng.integers(0, 5, shape)
+    centres = rng.integers((0, 0, 0), shape, (nl, 3))
+    for l in range(nl):
+        if l in (6, 2):
+            continue
+        r = 2 + l % 4
+        sl = tuple(slice(max(0, c - r), c + r + 1) for c in centres[l])
+        h[l][sl] = rng.integers(1, 200, h[l][sl].shape)
+    values = np.arange(10, 10 + nl, dtype=np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(h[0].size)
+    eng.smooth_argmax(hb, nl, shape, iters, values, out)
+    phi = np.stack([orc.smooth(x.astype(np.float32), iters) for x in h])
+    np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
+
+
 @pytest.mark.parametrize("iters", [2, 3, 4])
 def test_smooth_pairs_bitexact_wide(eng, iters):
     """A volume wider than one tile in x and y and taller than one z chunk."""
