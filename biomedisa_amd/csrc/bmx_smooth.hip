@@ -681,7 +681,8 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
     const int64_t V = Z * Y * X;
     const int64_t rec_ints = pass_record_ints(Z, Y, X);
     // Buffer sets of the labels being smoothed at the same time: two phi
-    // volumes and the pass records, and for sets 1.. a partial argmax.
+    // volumes and the pass records, and for sets 1.. a partial argmax (set 0
+    // has one only when the labels run one after another).
     float *a[kSmoothSets] = {}, *b[kSmoothSets] = {}, *pbest[kSmoothSets] = {};
     uint8_t *pidx[kSmoothSets] = {};
     int *recs[kSmoothSets] = {};
@@ -689,8 +690,8 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
         hipError_t e = hipMalloc((void **)&a[k], (size_t)V * sizeof(float));
         if (e == hipSuccess) e = hipMalloc((void **)&b[k], (size_t)V * sizeof(float));
         if (e == hipSuccess) e = hipMalloc((void **)&recs[k], (size_t)(3 * rec_ints) * sizeof(int));
-        if (e == hipSuccess) e = hipMalloc((void **)&pbest[k], (size_t)V * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc((void **)&pidx[k], (size_t)V);
+        if (e == hipSuccess && k >= 1) e = hipMalloc((void **)&pbest[k], (size_t)V * sizeof(float));
+        if (e == hipSuccess && k >= 1) e = hipMalloc((void **)&pidx[k], (size_t)V);
         return e;
     };
     auto free_set = [&](int k) {
@@ -765,7 +766,17 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
         };
         LabelValues lv;
         for (int i = 0; i < 256; ++i) lv.v[i] = i < nlabels ? values[i] : 0;
-        if (nset == 1) {
+        if (rc == BMX_OK && nset == 1) {
+            // one after another: set 0 keeps the argmax
+            hipError_t e0 = hipMalloc((void **)&pbest[0], (size_t)V * sizeof(float));
+            if (e0 == hipSuccess) e0 = hipMalloc((void **)&pidx[0], (size_t)V);
+            if (e0 != hipSuccess) {
+                (void)hipGetLastError();
+                rc = bmx_set_error(e0 == hipErrorOutOfMemory ? BMX_E_OOM : BMX_E_DEVICE,
+                                   "bmx_smooth_argmax: allocation failed: %s", hipGetErrorString(e0));
+            }
+        }
+        if (rc == BMX_OK && nset == 1) {
             for (int l = 0; l < nlabels && rc == BMX_OK; ++l) {
                 float *res = nullptr;
                 rc = smooth_label(l, 0, ctx->stream, &res);

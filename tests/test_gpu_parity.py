@@ -330,6 +330,21 @@ def test_smooth_argmax_label_batches_bitexact(eng, iters):
     np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
 
 
+@pytest.mark.parametrize("iters", [3, 4])
+def test_smooth_argmax_single_label_bitexact(eng, iters):
+    """One label runs on the context stream alone, into its own argmax."""
+    rng = np.random.default_rng(120 + iters)
+    shape = (20, 33, 70)
+    h = np.zeros((1,) + shape, np.uint32)
+    h[0][5:12, 8:20, 30:50] = rng.integers(0, 50, (7, 12, 20))
+    values = np.array([4], np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(h[0].size)
+    eng.smooth_argmax(hb, 1, shape, iters, values, out)
+    phi = np.stack([orc.smooth(x.astype(np.float32), iters) for x in h])
+    np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
+
+
 @pytest.mark.parametrize("iters", [2, 3, 4])
 def test_smooth_pairs_bitexact_wide(eng, iters):
     """A volume wider than one tile in x and y and taller than one z chunk."""
