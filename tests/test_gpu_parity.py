@@ -345,6 +345,26 @@ def test_smooth_argmax_single_label_bitexact(eng, iters):
     np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
 
 
+@pytest.mark.parametrize("big", [0, 1])
+def test_smooth_argmax_two_labels_bitexact(eng, big):
+    """Two labels: one on the context stream, the other alone on a side
+    stream; the largest box on either label, and equal hits on a patch where
+    the two boxes overlap (the tie goes to label 0)."""
+    rng = np.random.default_rng(140 + big)
+    shape = (24, 30, 80)
+    h = np.zeros((2,) + shape, np.uint32)
+    h[big] = rng.integers(0, 12, shape)
+    h[1 - big][8:14, 10:16, 30:40] = rng.integers(1, 90, (6, 6, 10))
+    h[0][20:22, 2:4, 70:74] = 7
+    h[1][20:22, 2:4, 70:74] = 7
+    values = np.array([3, 9], np.uint8)
+    hb = eng.upload(h)
+    out = eng.alloc(h[0].size)
+    eng.smooth_argmax(hb, 2, shape, 5, values, out)
+    phi = np.stack([orc.smooth(x.astype(np.float32), 5) for x in h])
+    np.testing.assert_array_equal(out.download(shape, np.uint8), orc.argmax(phi, values))
+
+
 @pytest.mark.parametrize("iters", [2, 3, 4])
 def test_smooth_pairs_bitexact_wide(eng, iters):
     """A volume wider than one tile in x and y and taller than one z chunk."""
