@@ -190,9 +190,12 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_step(const Tin *__res
 // pass before (the input's), prev2 by the pass before that (the output
 // buffer's current contents).  prev = null: no block is skipped; prev2 = null:
 // the output buffer's contents are unknown; out = null: no boxes are kept.
+// src (first pass, prev = null): the box of the input's non-zero voxels in
+// the same format; blocks out of its reach are skipped as for prev.
 struct PassRecords {
     const int *prev, *prev2;
     int *out;
+    const int *src = nullptr;
 };
 
 constexpr int PW = TX + 4, PH = TY + 4;        // input plane with a 2-voxel halo
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
     // this buffer's boxes as the pass before last left them).
     // Planes [zc0, zc1) of the chunk are computed; the others are +0.
     int zc0 = zb, zc1 = ze;
-    if (rec.prev) {
+    if (rec.prev || rec.src) {
         __shared__ int szr[2];
         if (tid == 0) {
             szr[0] = INT_MAX;
@@ -231,18 +234,25 @@ __global__ __launch_bounds__(kSmoothThreads) void k_smooth_pair(const Tin *__res
         }
         __syncthreads();
         bool near = false;
-        if (tid < 27) {
-            const int nx = (int)blockIdx.x + tid % 3 - 1, ny = (int)blockIdx.y + (tid / 3) % 3 - 1,
-                      nz = (int)blockIdx.z + tid / 9 - 1;
-            if (nx >= 0 && nx < (int)gridDim.x && ny >= 0 && ny < (int)gridDim.y && nz >= 0 && nz < (int)gridDim.z) {
-                const int *q = rec.prev + 6 * ((nz * (int)gridDim.y + ny) * (int)gridDim.x + nx);
-                const int xe = min(x0 + TX, bx.x1) - 1, ye = min(y0 + TY, bx.y1) - 1;
-                near = q[1] >= 0 && q[0] <= ze - 1 + 2 && q[1] >= zb - 2 && q[2] <= ye + 2 && q[3] >= y0 - 2 &&
-                       q[4] <= xe + 2 && q[5] >= x0 - 2;
-                if (near) {
-                    atomicMin(&szr[0], q[0]);
-                    atomicMax(&szr[1], q[1]);
-                }
+        const int *q = nullptr;
+        if (rec.prev) {
+            if (tid < 27) {
+                const int nx = (int)blockIdx.x + tid % 3 - 1, ny = (int)blockIdx.y + (tid / 3) % 3 - 1,
+                          nz = (int)blockIdx.z + tid / 9 - 1;
+                if (nx >= 0 && nx < (int)gridDim.x && ny >= 0 && ny < (int)gridDim.y && nz >= 0 &&
+                    nz < (int)gridDim.z)
+                    q = rec.prev + 6 * ((nz * (int)gridDim.y + ny) * (int)gridDim.x + nx);
+            }
+        } else if (tid == 0) {
+            q = rec.src;
+        }
+        if (q) {
+            const int xe = min(x0 + TX, bx.x1) - 1, ye = min(y0 + TY, bx.y1) - 1;
+            near = q[1] >= 0 && q[0] <= ze - 1 + 2 && q[1] >= zb - 2 && q[2] <= ye + 2 && q[3] >= y0 - 2 &&
+                   q[4] <= xe + 2 && q[5] >= x0 - 2;
+            if (near) {
+                atomicMin(&szr[0], q[0]);
+                atomicMax(&szr[1], q[1]);
             }
         }
         const bool any = __syncthreads_or(near);
@@ -615,13 +625,14 @@ static int64_t pass_record_ints(int64_t Z, int64_t Y, int64_t X)
 // blocks that only see +0 (PassRecords).
 template <typename Tin>
 int smooth_rounds(hipStream_t st, const Tin *in, float *a, float *b, int Z, int Y, int X, int iters,
-                  const Box &bx, float **result, int *recs = nullptr, int64_t rec_ints = 0)
+                  const Box &bx, float **result, int *recs = nullptr, int64_t rec_ints = 0,
+                  const int *src = nullptr)
 {
     auto rec_of = [&](int pass) -> PassRecords {
         if (!recs) return PassRecords{nullptr, nullptr, nullptr};
         int *r[3] = {recs, recs + rec_ints, recs + 2 * rec_ints};
         return PassRecords{pass >= 1 ? r[(pass - 1) % 3] : nullptr, pass >= 2 ? r[(pass - 2) % 3] : nullptr,
-                           r[pass % 3]};
+                           r[pass % 3], pass == 0 ? src : nullptr};
     };
     int rc = iters >= 2 ? launch_pair<Tin>(st, in, a, Z, Y, X, bx, rec_of(0))
                         : launch_step<Tin>(st, in, a, Z, Y, X, bx);
@@ -756,7 +767,7 @@ int bmx_smooth_argmax_ex(bmx_ctx *ctx, const uint32_t *hits, int nlabels, int64_
             *res = a[k];
             if (cost[(size_t)l] == 0) return BMX_OK;
             return smooth_rounds<uint32_t>(st, hits + (int64_t)l * V, a[k], b[k], (int)Z, (int)Y, (int)X, iters,
-                                           bxs[(size_t)l], res, recs[k], rec_ints);
+                                           bxs[(size_t)l], res, recs[k], rec_ints, dbox + 6 * l);
         };
         auto update = [&](int l, const float *res, float *best, uint8_t *bidx, int first, hipStream_t st) -> int {
             k_argmax_update<<<bmx_blocks(V, 256), 256, 0, st>>>(res, best, bidx, l, (int)Z, (int)Y, (int)X,
