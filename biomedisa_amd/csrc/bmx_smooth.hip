@@ -501,25 +501,27 @@ __global__ __launch_bounds__(256) void k_hits_box(const uint32_t *__restrict__ h
     const uint32_t *h = hits + (int64_t)blockIdx.y * V;
     const int lane = threadIdx.x & 63;
     int b[6] = {INT_MAX, -1, INT_MAX, -1, INT_MAX, -1};
-    auto note = [&](int64_t e, int lo, int hi) {   // non-zeros at x offsets lo..hi from element e
-        const int64_t row = e / X;
-        const int x = (int)(e - row * X), z = (int)(row / Y), y = (int)(row - (int64_t)z * Y);
+    // non-zeros at x offsets lo..hi from element e of the label (e < V < 2^32)
+    auto note = [&](uint32_t e, int lo, int hi) {
+        const uint32_t row = e / (uint32_t)X, zq = row / (uint32_t)Y;
+        const int x = (int)(e - row * (uint32_t)X), z = (int)zq, y = (int)(row - zq * (uint32_t)Y);
         b[0] = min(b[0], z); b[1] = max(b[1], z);
         b[2] = min(b[2], y); b[3] = max(b[3], y);
         b[4] = min(b[4], x + lo); b[5] = max(b[5], x + hi);
     };
     if ((X & 3) == 0 && (reinterpret_cast<uintptr_t>(hits) & 15) == 0) {
         // 16-byte groups never cross a row; four loads in flight per thread
+        // n4 < 2^30 and stride <= 2^18, so the indexes stay below 2^31
         const uint4 *h4 = reinterpret_cast<const uint4 *>(h);
-        const int64_t n4 = V / 4, stride = (int64_t)gridDim.x * blockDim.x;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 4 * stride) {
+        const uint32_t n4 = (uint32_t)(V / 4), stride = gridDim.x * blockDim.x;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 4 * stride) {
             uint4 q[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) q[u] = i + u * stride < n4 ? h4[i + u * stride] : make_uint4(0, 0, 0, 0);
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (q[u].x | q[u].y | q[u].z | q[u].w)
-                    note(4 * (i + u * stride), q[u].x ? 0 : q[u].y ? 1 : q[u].z ? 2 : 3,
+                    note(4u * (i + u * stride), q[u].x ? 0 : q[u].y ? 1 : q[u].z ? 2 : 3,
                          q[u].w ? 3 : q[u].z ? 2 : q[u].y ? 1 : 0);
         }
     } else {
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(256) void k_hits_box(const uint32_t *__restrict__ h
                     lo = min(lo, x);
                     hi = x;
                 }
-            if (hi >= 0) note(r * X, lo, hi);
+            if (hi >= 0) note((uint32_t)(r * X), lo, hi);
         }
     }
 #pragma unroll
